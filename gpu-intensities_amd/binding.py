@@ -1,0 +1,271 @@
+"""ctypes binding of libgi_hip.so (the C ABI in include/gpu_intensities.h).
+
+The library is the product: there is no CPU fallback. Loading fails loudly
+when libgi_hip.so has not been built, and every call raises GIError on a
+non-zero status with the library's own message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgi_hip.so")
+_LIB = None
+
+ci, cd, cll = C.c_int, C.c_double, C.c_longlong
+P_i, P_d = C.POINTER(C.c_int), C.POINTER(C.c_double)
+
+GI_MODE_FP64 = 0
+STATUS = {0: "GI_OK", 1: "GI_EINVAL", 2: "GI_EHIP", 3: "GI_ENOMEM", 4: "GI_ESTATE",
+          5: "GI_ECAPACITY", 6: "GI_EBUDGET"}
+
+# Every symbol declared in include/gpu_intensities.h.
+EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_params", "gi_upload_bset",
+           "gi_upload_dipole", "gi_set_levels", "gi_upload_vectors", "gi_prepare", "gi_compute",
+           "gi_count_candidates", "gi_half_ls", "gi_correlate", "gi_threej_table",
+           "gi_kernel_stats", "gi_work_stats"]
+
+
+class GIError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class GiBset(C.Structure):
+    _fields_ = [("jval", ci), ("Maxsymcoeffs", ci), ("Maxcontracts", ci), ("mat_size", ci),
+                ("icontr2icase", P_i), ("iroot_correlat_j0", P_i), ("k", P_i), ("ktau", P_i),
+                ("N", P_i), ("Ntotal", P_i), ("ijterms", P_i), ("repres", C.POINTER(P_d)),
+                ("nsize", P_i)]
+
+
+class GiParams(C.Structure):
+    _fields_ = [("nrepres", ci), ("degen", P_i), ("sym_type", ci), ("nmodes", ci),
+                ("zpe", cd), ("temperature", cd), ("q_stat", cd), ("freq_window", cd * 2),
+                ("erange_lower", cd * 2), ("erange_upper", cd * 2), ("jvals", ci * 2),
+                ("gns", P_d), ("isym_pairs", P_i), ("reduced", ci), ("quanta_lower", P_i),
+                ("quanta_upper", P_i), ("thresh_linestrength", cd), ("thresh_intensity", cd)]
+
+
+class GiLines(C.Structure):
+    _fields_ = [("capacity", cll), ("n", cll), ("ilevelI", P_i), ("ilevelF", P_i),
+                ("nu", P_d), ("S", P_d), ("A_einst", P_d), ("I", P_d), ("amp", P_d)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.gi_create.argtypes = [ci, C.POINTER(vp)]
+        L.gi_destroy.argtypes = [vp]
+        L.gi_destroy.restype = None
+        L.gi_last_error.argtypes = [vp]
+        L.gi_last_error.restype = C.c_char_p
+        L.gi_set_params.argtypes = [vp, C.POINTER(GiParams)]
+        L.gi_upload_bset.argtypes = [vp, ci, C.POINTER(GiBset)]
+        L.gi_upload_dipole.argtypes = [vp, ci, vp, ci]
+        L.gi_set_levels.argtypes = [vp, ci, P_d, P_i, P_i, P_i, P_i, P_i]
+        L.gi_upload_vectors.argtypes = [vp, ci, ci, ci, ci, vp, ci]
+        L.gi_prepare.argtypes = [vp]
+        L.gi_compute.argtypes = [vp, ci, ci, P_i, C.POINTER(GiLines)]
+        L.gi_count_candidates.argtypes = [vp, ci, P_i, C.POINTER(cll)]
+        L.gi_half_ls.argtypes = [vp, ci, ci, ci, ci, P_d, P_d]
+        L.gi_correlate.argtypes = [vp, ci, ci, ci, P_d, P_d]
+        L.gi_threej_table.argtypes = [ci, P_d]
+        L.gi_kernel_stats.argtypes = [vp, ci, C.POINTER(C.c_char_p), P_d, P_i, P_i]
+        L.gi_work_stats.argtypes = [vp, P_d, ci]
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def threej_table(jmax):
+    t = np.zeros((jmax + 1) ** 2 * 9)
+    rc = lib().gi_threej_table(jmax, _p(t, P_d))
+    if rc:
+        raise GIError(rc, "gi_threej_table")
+    return t
+
+
+class Engine:
+    """One gi_ctx on a device, loaded with a synth.Job."""
+
+    def __init__(self, device=0):
+        self._keep = []
+        h = C.c_void_p()
+        rc = lib().gi_create(device, C.byref(h))
+        if rc:
+            raise GIError(rc, "gi_create failed (no HIP device?)")
+        self.h = h
+        self.job = None
+
+    def close(self):
+        if self.h:
+            lib().gi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc:
+            msg = lib().gi_last_error(self.h)
+            raise GIError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def _i32(self, a):
+        a = np.ascontiguousarray(a, np.int32)
+        self._keep.append(a)
+        return _p(a, P_i)
+
+    def _f64(self, a):
+        a = np.ascontiguousarray(a, np.float64)
+        self._keep.append(a)
+        return _p(a, P_d)
+
+    def load(self, job, device_vectors=None, device_dipole=None):
+        """Set parameters, bases, dipole, levels and eigenvector blocks.
+        device_vectors: optional {(jind, g): torch tensor on the device}."""
+        L = lib()
+        self.job = job
+        nrep = job.sym["nrep"]
+        p = GiParams(nrepres=nrep, degen=self._i32(job.sym["degen"]),
+                     sym_type=job.sym["sym_type"], nmodes=job.nmodes, zpe=job.zpe,
+                     temperature=job.temperature, q_stat=job.q_stat,
+                     freq_window=(cd * 2)(*job.freq_window),
+                     erange_lower=(cd * 2)(*job.erange_lower),
+                     erange_upper=(cd * 2)(*job.erange_upper), jvals=(ci * 2)(*job.jvals),
+                     gns=self._f64(job.gns), isym_pairs=self._i32(job.isym_pairs),
+                     reduced=job.reduced, quanta_lower=self._i32(job.quanta_lower),
+                     quanta_upper=self._i32(job.quanta_upper),
+                     thresh_linestrength=job.thresh_linestrength,
+                     thresh_intensity=job.thresh_intensity)
+        self._chk(L.gi_set_params(self.h, C.byref(p)), "gi_set_params")
+        for jind, b in enumerate(job.bsets):
+            reps = (P_d * nrep)(*[self._f64(r) for r in b.repres])
+            self._keep.append(reps)
+            s = GiBset(jval=b.jval, Maxsymcoeffs=b.Maxsymcoeffs, Maxcontracts=b.Maxcontracts,
+                       mat_size=b.mat_size, icontr2icase=self._i32(b.icontr2icase),
+                       iroot_correlat_j0=self._i32(b.iroot_correlat_j0), k=self._i32(b.k),
+                       ktau=self._i32(b.ktau), N=self._i32(b.N), Ntotal=self._i32(b.Ntotal),
+                       ijterms=self._i32(b.ijterms), repres=C.cast(reps, C.POINTER(P_d)),
+                       nsize=self._i32(b.nsize))
+            self._chk(L.gi_upload_bset(self.h, jind, C.byref(s)), "gi_upload_bset")
+        if device_dipole is not None:
+            self._chk(L.gi_upload_dipole(self.h, job.nvib, C.c_void_p(device_dipole.data_ptr()), 1),
+                      "gi_upload_dipole")
+        else:
+            mu = np.ascontiguousarray(job.dipole_flipped, np.float64)
+            self._chk(L.gi_upload_dipole(self.h, job.nvib, C.c_void_p(mu.ctypes.data), 0),
+                      "gi_upload_dipole")
+        lv = job.levels
+        self._chk(L.gi_set_levels(self.h, job.nlevels, self._f64(lv["energy"]),
+                                  self._i32(lv["jind"]), self._i32(lv["igamma"]),
+                                  self._i32(lv["ndeg"]), self._i32(lv["irec0"]),
+                                  self._i32(lv["quanta"].reshape(-1))), "gi_set_levels")
+        for jind in range(2):
+            for g in range(nrep):
+                nsize = int(job.bsets[jind].nsize[g])
+                if device_vectors is not None and (jind, g) in device_vectors:
+                    t = device_vectors[(jind, g)]
+                    nrec = int(t.shape[0]) if nsize else 0
+                    self._keep.append(t)
+                    self._chk(L.gi_upload_vectors(self.h, jind, g, nsize, nrec,
+                                                  C.c_void_p(t.data_ptr()), 1),
+                              "gi_upload_vectors")
+                else:
+                    v = job.vectors.get((jind, g))
+                    if v is None:
+                        v = np.zeros((0, nsize))
+                    v = np.ascontiguousarray(v, np.float64)
+                    nrec = int(v.shape[0])
+                    self._chk(L.gi_upload_vectors(self.h, jind, g, nsize, nrec,
+                                                  C.c_void_p(v.ctypes.data if v.size else 0), 0),
+                              "gi_upload_vectors")
+        self._chk(L.gi_prepare(self.h), "gi_prepare")
+        return self
+
+    def count_candidates(self, lower_ids=None):
+        n = cll()
+        if lower_ids is None:
+            self._chk(lib().gi_count_candidates(self.h, 0, None, C.byref(n)), "count")
+        else:
+            ids = np.ascontiguousarray(lower_ids, np.int32)
+            self._chk(lib().gi_count_candidates(self.h, len(ids), _p(ids, P_i), C.byref(n)),
+                      "count")
+        return n.value
+
+    def alloc_lines(self, cap, pinned=False):
+        cap = max(int(cap), 1)
+        if pinned:
+            import torch
+            mk = lambda n, dt: torch.empty(n, dtype=dt, pin_memory=True).numpy()
+            import torch as T
+            out = dict(ilevelI=mk(cap, T.int32), ilevelF=mk(cap, T.int32), nu=mk(cap, T.float64),
+                       S=mk(cap, T.float64), A_einst=mk(cap, T.float64), I=mk(cap, T.float64),
+                       amp=mk(cap * 4, T.float64))
+        else:
+            out = dict(ilevelI=np.empty(cap, np.int32), ilevelF=np.empty(cap, np.int32),
+                       nu=np.empty(cap), S=np.empty(cap), A_einst=np.empty(cap), I=np.empty(cap),
+                       amp=np.empty(cap * 4))
+        return out
+
+    def compute(self, lower_ids=None, mode=GI_MODE_FP64, bufs=None):
+        cap = self.count_candidates(lower_ids)
+        if bufs is None or len(bufs["S"]) < max(cap, 1):
+            bufs = self.alloc_lines(cap)
+        s = GiLines(capacity=len(bufs["S"]), n=0, ilevelI=_p(bufs["ilevelI"], P_i),
+                    ilevelF=_p(bufs["ilevelF"], P_i), nu=_p(bufs["nu"], P_d),
+                    S=_p(bufs["S"], P_d), A_einst=_p(bufs["A_einst"], P_d), I=_p(bufs["I"], P_d),
+                    amp=_p(bufs["amp"], P_d))
+        if lower_ids is None:
+            rc = lib().gi_compute(self.h, mode, 0, None, C.byref(s))
+        else:
+            ids = np.ascontiguousarray(lower_ids, np.int32)
+            rc = lib().gi_compute(self.h, mode, len(ids), _p(ids, P_i), C.byref(s))
+        self._chk(rc, "gi_compute")
+        n = s.n
+        res = {k: bufs[k][:n] for k in ("ilevelI", "ilevelF", "nu", "S", "A_einst", "I")}
+        res["amp"] = bufs["amp"][:4 * n].reshape(n, 4)
+        return res
+
+    def half_ls(self, jindI, jindF, g, ideg, vec):
+        v = np.ascontiguousarray(vec, np.float64)
+        h = np.zeros(self.job.bsets[jindF].Maxcontracts)
+        self._chk(lib().gi_half_ls(self.h, jindI, jindF, g, ideg, _p(v, P_d), _p(h, P_d)),
+                  "gi_half_ls")
+        return h
+
+    def correlate(self, jind, g, ideg, vec):
+        v = np.ascontiguousarray(vec, np.float64)
+        o = np.zeros(self.job.bsets[jind].Maxcontracts)
+        self._chk(lib().gi_correlate(self.h, jind, g, ideg, _p(v, P_d), _p(o, P_d)),
+                  "gi_correlate")
+        return o
+
+    def kernel_stats(self):
+        n = ci()
+        names = (C.c_char_p * 32)()
+        ms = np.zeros(32)
+        la = np.zeros(32, np.int32)
+        self._chk(lib().gi_kernel_stats(self.h, 32, names, _p(ms, P_d), _p(la, P_i), C.byref(n)),
+                  "gi_kernel_stats")
+        return {names[i].decode(): dict(total_ms=float(ms[i]), launches=int(la[i]))
+                for i in range(min(n.value, 32))}
+
+    def work_stats(self):
+        w = np.zeros(8)
+        self._chk(lib().gi_work_stats(self.h, _p(w, P_d), 8), "gi_work_stats")
+        keys = ["units", "candidates", "lines", "half_ls_useful_flops", "half_ls_issued_flops",
+                "corr_bytes", "mu_bytes"]
+        return dict(zip(keys, w[:7].tolist()))

@@ -1,0 +1,56 @@
+// Kernel argument structs and launchers of the gfx950 line-intensity engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#ifndef GI_MAXDEG
+#define GI_MAXDEG 2
+#endif
+
+namespace gi {
+
+// Device view of one symmetrisation map (R and R^T in CSR form).
+struct SymDev {
+    const int *rptr, *rcol;
+    const double *rval;
+    const int *tptr, *tcol;
+    const double *tval;
+};
+
+// Structure-of-arrays line records.
+struct LineSoA {
+    int *iI, *iF;
+    double *nu, *S, *A, *I, *amp;
+};
+
+struct CorrArgs {
+    int ncand, nitems, nsizeF, ndegI, ndegF, cb_mask, reduced, jI, jF;
+    const int *item_cand_start, *item_lo, *item_level;
+    const long long *item_out_off;
+    const double *item_energy;
+    const int *blk_col, *blk_lev;
+    const double *blk_en;
+    const unsigned char *blk_upok;
+    const double *vec, *Ht;
+    double gnsI, zpe, beta, q_stat, A_coef_s_1, intens_cm_mol, thresh_S, thresh_I;
+    int *flag, *out_iI, *out_iF;
+    double *out_nu, *out_S, *out_A, *out_I, *out_amp;
+};
+
+void launch_correlate_prim(hipStream_t st, int dimen, const int *r_grid, SymDev R,
+                           const double *u, double *out);
+void launch_grid_to_prim(hipStream_t st, int dimen, const int *r_grid, const double *grid,
+                         double *out);
+void launch_build_g(hipStream_t st, int nunits, int N, int SI, int SF, int SMAX,
+                    const double *const *uptr, const int *unit_ideg, SymDev r0, SymDev r1,
+                    const int *cptr, const int *csI, const double *cval, double *G);
+int launch_half_ls_gemv(hipStream_t st, int nunits, int N, int SF, int SMAX, const double *Mu,
+                        const double *G, double *H);
+void launch_fold(hipStream_t st, int nitems, int nsizeF, int ndegI, int ndegF, int SFN,
+                 const int *item_unit, SymDev t0, SymDev t1, const double *H, double *Ht);
+void launch_corr_intensity(hipStream_t st, const CorrArgs &a);
+int compact_blocks(long long n);
+void launch_compact(hipStream_t st, long long n, const int *flag, int *counts, long long *offs,
+                    LineSoA in, LineSoA out);
+
+}  // namespace gi

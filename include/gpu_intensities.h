@@ -1,0 +1,178 @@
+/*
+ * gpu_intensities.h — C ABI of the MI355X (gfx950) line-intensity engine.
+ *
+ * Plain C: POD structs, pointers and sizes only. Every entry point returns a
+ * gi_status; gi_last_error() gives the message. Nothing calls exit().
+ *
+ * The reference (ahmed-f-alrefaie/gpu-intensities) has no FFI layer: its
+ * boundary is the host-callable CUDA API of cuda_host.cuh:10-24 and
+ * dipole_kernals.cuh:4-13, driven by dipole_do_intensities_async_omp
+ * (cuda_host.cu:995-1506). Each declaration below names the reference
+ * interface it replaces.
+ *
+ * Threading: a context is used by one host thread at a time. All device
+ * memory is owned by the context and freed by gi_destroy().
+ */
+#ifndef GPU_INTENSITIES_H
+#define GPU_INTENSITIES_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gi_ctx gi_ctx;
+
+typedef enum gi_status {
+    GI_OK = 0,
+    GI_EINVAL = 1,     /* bad argument or inconsistent input                 */
+    GI_EHIP = 2,       /* a HIP runtime call failed                          */
+    GI_ENOMEM = 3,     /* device or host allocation failed                   */
+    GI_ESTATE = 4,     /* call out of order (e.g. compute before prepare)    */
+    GI_ECAPACITY = 5,  /* output arrays too small; required size is returned */
+    GI_EBUDGET = 6     /* request does not fit the context's memory budget   */
+} gi_status;
+
+/* Arithmetic of the half-linestrength and correlation products. Energies,
+ * Boltzmann factors and the intensity step are FP64 in every mode. */
+typedef enum gi_mode {
+    GI_MODE_FP64 = 0   /* plain FP64: one GEMV-form product per lower level  */
+} gi_mode;
+
+/* Basis of one J: the TO_bset_contrT fields (trove_objects.h:49-72) read by
+ * the path, after correlate_index (trove_functions.cpp:680-858) and
+ * compute_ijterms (trove_functions.cpp:927-966).
+ * Replaces create_and_copy_bset_contr_to_gpu (cuda_host.cu:368-376) and the
+ * device struct cuda_bset_contrT (cuda_objects.cuh:5-21). */
+typedef struct gi_bset {
+    int jval;
+    int Maxsymcoeffs;             /* cases                                   */
+    int Maxcontracts;             /* dimen: primitive functions              */
+    int mat_size;                 /* third extent of repres                  */
+    const int *icontr2icase;      /* [2*dimen]: icase at r, ilambda at r+dimen */
+    const int *iroot_correlat_j0; /* [dimen]: J=0 (vibrational) index        */
+    const int *k;                 /* [dimen]                                 */
+    const int *ktau;              /* [dimen]: 2k + tau                       */
+    const int *N;                 /* [nrep*Maxsymcoeffs]: irr[g].N[c] at g*Maxsymcoeffs + c */
+    const int *Ntotal;            /* [nrep]                                  */
+    const int *ijterms;           /* [nrep*Maxsymcoeffs]: c + g*Maxsymcoeffs */
+    const double *const *repres;  /* [nrep] -> Ntotal[g]*degen[g]*mat_size   */
+    const int *nsize;             /* [nrep]: eigenvector length per symmetry */
+} gi_bset;
+
+/* Job parameters: FintensityJob (fields.h:48-82) as read by the deck parser
+ * (fields.cpp:145-282). quanta_* hold lo at [i] and hi at [nmodes+i]. */
+typedef struct gi_params {
+    int nrepres;
+    const int *degen;             /* [nrepres]                               */
+    int sym_type;                 /* 1 C2V(M), 2 C3V(M), 3 D3H(M) (fields.h:9-11) */
+    int nmodes;
+    double zpe, temperature, q_stat;
+    double freq_window[2];
+    double erange_lower[2];
+    double erange_upper[2];
+    int jvals[2];
+    const double *gns;            /* [nrepres]                               */
+    const int *isym_pairs;        /* [nrepres] (SELECTION)                   */
+    int reduced;                  /* SYMMETRY REDUCED                        */
+    const int *quanta_lower;      /* [2*nmodes]                              */
+    const int *quanta_upper;      /* [2*nmodes]                              */
+    double thresh_linestrength;   /* emit only S >= this (0: all)            */
+    double thresh_intensity;      /* emit only I >= this (0: all)            */
+} gi_params;
+
+/* Emitted transitions, structure of arrays with room for `capacity` rows.
+ * amp holds 4 doubles per row: A[idegI + 2*idegF] (cuda_host.cu:1467-1471). */
+typedef struct gi_lines {
+    long long capacity;
+    long long n;                  /* out: rows written (or required)         */
+    int *ilevelI, *ilevelF;       /* indices into the level table            */
+    double *nu, *S, *A_einst, *I;
+    double *amp;
+} gi_lines;
+
+/* Context on one device. Replaces cudaSetDevice + dipole_initialise_gpu's
+ * device set-up (cuda_host.cu:995-1008, 900-909). */
+int gi_create(int device, gi_ctx **out);
+/* Frees every device allocation (the reference leaks them, cuda_host.cu:1501-1503). */
+void gi_destroy(gi_ctx *ctx);
+const char *gi_last_error(const gi_ctx *ctx);
+
+/* Replaces read_fields' job fields + copy_intensity_info
+ * (fields.cpp:236-282; dipole_kernals.cu:8-14) and find_igamma_pair
+ * (trove_functions.cpp:1041-1072; an ambiguous pairing is GI_EINVAL). */
+int gi_set_params(gi_ctx *ctx, const gi_params *p);
+
+/* Basis of jvals[jind] (jind = 0, 1). Replaces
+ * create_and_copy_bset_contr_to_gpu (cuda_host.cu:368-376). */
+int gi_upload_bset(gi_ctx *ctx, int jind, const gi_bset *b);
+
+/* Flipped dipole mu_a(vF, vI) at vF + vI*nvib + a*nvib^2, a = x, y, z.
+ * Replaces read_dipole_flipped + copy_array_to_gpu(dipole_me)
+ * (trove_functions.cpp:113-205; cuda_host.cu:983). on_device != 0 means
+ * `dipole` is a device pointer on the context's device. */
+int gi_upload_dipole(gi_ctx *ctx, int nvib, const double *dipole, int on_device);
+
+/* Level table, energy sorted, structure of arrays (TO_PTeigen,
+ * trove_objects.h:74-88, as filled by read_eigenvalues,
+ * trove_functions.cpp:1074-1429). irec0 is the 0-based eigenvector record.
+ * quanta is [n*(nmodes+1)]. */
+int gi_set_levels(gi_ctx *ctx, int n, const double *energy, const int *jind,
+                  const int *igamma, const int *ndeg, const int *irec0, const int *quanta);
+
+/* Eigenvector block of (jvals[jind], igamma): nrec records of nsize doubles,
+ * record irec0 at irec0*nsize. Kept resident in HBM. Replaces the per-level
+ * fseek/fread + cublasSetVector / cudaMemcpyAsync of cuda_host.cu:1196-1199,
+ * 1262-1273. on_device != 0: `vectors` is a device pointer that the context
+ * adopts by reference (the caller keeps it alive and unchanged). */
+int gi_upload_vectors(gi_ctx *ctx, int jind, int igamma, int nsize, int nrec,
+                      const double *vectors, int on_device);
+
+/* Builds the device plan (3j table, slot grids, symmetrisation maps,
+ * coefficient tables, per-block energy order). Call after the uploads. */
+int gi_prepare(gi_ctx *ctx);
+
+/* The hot path for the given lower levels (indices into the level table;
+ * NULL = all): half linestrengths, correlation with every upper level that
+ * passes the window and selection filters, degeneracy sum, Einstein A and
+ * intensity, thresholds, compaction and copy to `out` (host memory).
+ * Replaces the body of the lower-level loop, cuda_host.cu:1174-1488
+ * (do_1st_half_ls :767-780, correlate + cublasDdot :1284-1322, intensity
+ * :1376-1407). Rows come grouped by lower level; within a lower level, by
+ * upper level energy. If out->capacity is too small, returns GI_ECAPACITY
+ * with the required count in out->n and writes nothing. */
+int gi_compute(gi_ctx *ctx, int mode, int nlower, const int *lower_ids, gi_lines *out);
+
+/* Upper bound of the rows gi_compute can emit for these lower levels. */
+int gi_count_candidates(gi_ctx *ctx, int nlower, const int *lower_ids, long long *count);
+
+/* do_1st_half_ls (cuda_host.cu:767-780): correlate the symmetrised lower
+ * vector vec (nsize of (jvals[jindI], igammaI), host memory) for component
+ * ideg, then the half linestrength towards jvals[jindF]; h (host memory)
+ * receives dimen(jvals[jindF]) values in the primitive order of the basis. */
+int gi_half_ls(gi_ctx *ctx, int jindI, int jindF, int igammaI, int ideg,
+               const double *vec, double *h);
+
+/* device_correlate_vectors (dipole_kernals.cu:38-71): out (host, dimen of
+ * jvals[jind]) = symmetrised -> primitive transform of vec (host, nsize). */
+int gi_correlate(gi_ctx *ctx, int jind, int igamma, int ideg, const double *vec, double *out);
+
+/* precompute_threej (trove_functions.cpp:899-925): out has (jmax+1)^2*9
+ * doubles; entries outside the valid (k, k') range are zero. */
+int gi_threej_table(int jmax, double *out);
+
+/* Per-kernel timing of the last gi_compute, from HIP events recorded on the
+ * stream the kernels run on. names/avg_ms/launches receive up to `max`
+ * entries; *n is the number of kernels. */
+int gi_kernel_stats(gi_ctx *ctx, int max, const char **names, double *total_ms,
+                    int *launches, int *n);
+
+/* Work counters of the last gi_compute: [0] half-ls units (lower x J'),
+ * [1] candidates, [2] emitted lines, [3] half-ls useful flops (2*C*Nvib^2
+ * summed), [4] half-ls issued flops, [5] correlation bytes (8*nsize_F per
+ * candidate), [6] mu bytes counted once per unit (24*Nvib^2). */
+int gi_work_stats(gi_ctx *ctx, double *out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

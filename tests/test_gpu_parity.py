@@ -1,0 +1,166 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
+
+Tolerance (FP64 path): every transition amplitude A satisfies
+|A_gpu - A_ref| <= 1e-10 * max(|A_ref|, sum_r |c_F[r] h[r]|) — the second
+term is the condition floor for cancelling sums (the oracle reports it) — and
+the emitted transition sets are identical.
+"""
+import numpy as np
+import pytest
+
+from gpu_intensities_amd import binding, synth
+import oracle_py
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _sorted(r):
+    k = np.lexsort((r["ilevelF"], r["ilevelI"]))
+    return {key: (v[k] if v.ndim == 1 else v[k, :]) for key, v in r.items()}
+
+
+def compare(got, ref, tol=TOL):
+    g, r = _sorted(got), _sorted(ref)
+    assert len(g["S"]) == len(r["S"]), (len(g["S"]), len(r["S"]))
+    assert np.array_equal(g["ilevelI"], r["ilevelI"])
+    assert np.array_equal(g["ilevelF"], r["ilevelF"])
+    np.testing.assert_allclose(g["nu"], r["nu"], rtol=0, atol=0)
+    floor = np.maximum(np.abs(r["amp"]), r["amp_abs"])
+    err = np.abs(g["amp"] - r["amp"])
+    bad = err > tol * floor + 1e-300
+    assert not bad.any(), f"{bad.sum()} amplitudes out of tolerance, worst {err.max()}"
+    rel = np.abs(g["S"] - r["S"]) / np.maximum(r["S"], 1e-300)
+    via_floor = int((rel > tol).sum())
+    # S is a sum of squares: a relative S error beyond tol can only come from a
+    # cancelling amplitude, which the amplitude check above already bounds.
+    for f in ("A_einst", "I"):
+        relf = np.abs(g[f] - r[f]) / np.maximum(np.abs(r[f]), 1e-300)
+        assert (relf[rel <= tol] <= 4 * tol).all(), f
+    return dict(n=len(g["S"]), via_floor=via_floor, max_rel_S=float(rel.max()) if len(rel) else 0)
+
+
+@pytest.fixture(scope="module")
+def c1_job():
+    return synth.make_job("C2V", nvib=200, jvals=(0, 1), seed=7)
+
+
+def test_threej_table_matches_oracle():
+    for jmax in (0, 1, 5, 11, 20):
+        a = binding.threej_table(jmax)
+        b = oracle_py.threej_table(jmax)
+        np.testing.assert_allclose(a, b, rtol=4e-16, atol=0)
+
+
+def test_correlate_parity(engine, c1_job):
+    engine.load(c1_job)
+    oj = oracle_py.OracleJob(c1_job)
+    rng = np.random.default_rng(3)
+    for jind in (0, 1):
+        for g in range(4):
+            n = int(c1_job.bsets[jind].nsize[g])
+            u = rng.standard_normal(n)
+            got = engine.correlate(jind, g, 0, u)
+            ref = oj.correlate(jind, 0, g, u)
+            np.testing.assert_array_equal(got, ref)
+
+
+def test_half_ls_parity(engine, c1_job):
+    engine.load(c1_job)
+    oj = oracle_py.OracleJob(c1_job)
+    rng = np.random.default_rng(4)
+    for jindI, jindF in ((0, 1), (1, 0), (1, 1)):
+        for g in range(4):
+            n = int(c1_job.bsets[jindI].nsize[g])
+            u = rng.standard_normal(n)
+            c = oj.correlate(jindI, 0, g, u)
+            ref = oj.half_ls(jindI, jindF, c)
+            got = engine.half_ls(jindI, jindF, g, 0, u)
+            scale = np.abs(ref).max()
+            assert np.abs(got - ref).max() <= 1e-12 * scale, (jindI, jindF, g)
+
+
+def test_c1_full_parity(engine, c1_job):
+    engine.load(c1_job)
+    got = engine.compute()
+    ref = oracle_py.OracleJob(c1_job).run()
+    st = compare(got, ref)
+    assert st["n"] == len(ref["S"]) > 1000
+    print("C1", st)
+
+
+def test_lower_subset_and_order(engine, c1_job):
+    engine.load(c1_job)
+    ids = np.arange(0, c1_job.nlevels, 7)[::-1].copy()
+    got = engine.compute(ids)
+    ref = oracle_py.OracleJob(c1_job).run(ids)
+    compare(got, ref)
+    # rows grouped by lower level in request order
+    first = {}
+    for i, L in enumerate(got["ilevelI"]):
+        first.setdefault(int(L), i)
+    order = [L for L in ids if L in first]
+    assert sorted(first, key=first.get) == order
+
+
+def test_nonsymmetric_mu_shuffled_cases(engine):
+    job = synth.make_job("C2V", nvib=48, jvals=(1, 2), seed=21, symmetric_mu=False,
+                         shuffle_cases=True)
+    engine.load(job)
+    compare(engine.compute(), oracle_py.OracleJob(job).run())
+
+
+@pytest.mark.parametrize("reduced", [0, 1])
+def test_c3v_degenerate_parity(engine, reduced):
+    job = synth.make_job("C3V", nvib=30, jvals=(2, 3), seed=5, reduced=reduced)
+    engine.load(job)
+    got = engine.compute()
+    ref = oracle_py.OracleJob(job).run()
+    st = compare(got, ref)
+    assert st["n"] > 100
+
+
+def test_windows_quanta_thresholds(engine):
+    job = synth.make_job("C2V", nvib=40, jvals=(3, 4), seed=9, e_span=20000.0,
+                         freq_window=(-3000.0, 5000.0), erange_lower=(1000.0, 12000.0),
+                         erange_upper=(500.0, 15000.0), thresh_intensity=1e-3,
+                         thresh_linestrength=1e-6)
+    job.quanta_lower[:] = [0, 1, 0, 5, 4, 5]
+    job.quanta_upper[:] = [1, 0, 0, 5, 5, 3]
+    engine.load(job)
+    got = engine.compute()
+    ref = oracle_py.OracleJob(job).run()
+    compare(got, ref)
+    assert len(ref["S"]) > 0
+
+
+def test_empty_inputs(engine, c1_job):
+    engine.load(c1_job)
+    got = engine.compute(np.zeros(0, np.int32))
+    assert len(got["S"]) == 0
+    # window that admits nothing
+    job = synth.make_job("C2V", nvib=8, jvals=(0, 1), seed=2, freq_window=(1e6, 2e6))
+    engine.load(job)
+    assert len(engine.compute()["S"]) == 0
+
+
+def test_capacity_error(engine, c1_job):
+    engine.load(c1_job)
+    with pytest.raises(binding.GIError) as e:
+        _compute_with_capacity(engine, 3)
+    assert e.value.status == 5
+
+
+def _compute_with_capacity(engine, cap):
+    import ctypes as C
+    L = binding.lib()
+    b = engine.alloc_lines(cap)
+    s = binding.GiLines(capacity=cap, n=0,
+                        ilevelI=b["ilevelI"].ctypes.data_as(binding.P_i),
+                        ilevelF=b["ilevelF"].ctypes.data_as(binding.P_i),
+                        nu=b["nu"].ctypes.data_as(binding.P_d), S=b["S"].ctypes.data_as(binding.P_d),
+                        A_einst=b["A_einst"].ctypes.data_as(binding.P_d),
+                        I=b["I"].ctypes.data_as(binding.P_d), amp=b["amp"].ctypes.data_as(binding.P_d))
+    rc = L.gi_compute(engine.h, 0, 0, None, C.byref(s))
+    engine._chk(rc, "gi_compute")
