@@ -18,6 +18,8 @@ ci, cd, cll = C.c_int, C.c_double, C.c_longlong
 P_i, P_d = C.POINTER(C.c_int), C.POINTER(C.c_double)
 
 GI_MODE_FP64 = 0
+GI_MODE_FP64_BATCHED = 1
+MODES = {"fp64": GI_MODE_FP64, "fp64_batched": GI_MODE_FP64_BATCHED}
 STATUS = {0: "GI_OK", 1: "GI_EINVAL", 2: "GI_EHIP", 3: "GI_ENOMEM", 4: "GI_ESTATE",
           5: "GI_ECAPACITY", 6: "GI_EBUDGET"}
 
@@ -264,8 +266,8 @@ class Engine:
                 for i in range(min(n.value, 32))}
 
     def work_stats(self):
-        w = np.zeros(8)
-        self._chk(lib().gi_work_stats(self.h, _p(w, P_d), 8), "gi_work_stats")
-        keys = ["units", "candidates", "lines", "half_ls_useful_flops", "half_ls_issued_flops",
-                "corr_bytes", "mu_bytes"]
-        return dict(zip(keys, w[:7].tolist()))
+        w = np.zeros(10)
+        self._chk(lib().gi_work_stats(self.h, _p(w, P_d), 10), "gi_work_stats")
+        keys = ["units", "candidates", "lines", "half_ls_dense_flops", "half_ls_flops",
+                "corr_bytes", "mu_bytes", "mu_dense_bytes", "corr_flops"]
+        return dict(zip(keys, w[:9].tolist()))

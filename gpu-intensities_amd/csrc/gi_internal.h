@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/gpu_intensities.h"
+#include "gi_kernels.h"
 
 #define GI_MAXDEG 2
 
@@ -47,6 +48,36 @@ struct Coef {
     int *d_ptr = nullptr, *d_sI = nullptr;
     double *d_val = nullptr;
     double P = 0, C = 0;             // useful pair and component counts per Nvib^2
+};
+
+// One output class of a group plan: the vibrational indices [off, off+n) of
+// the permuted grid (one symmetry class) and the upper slots whose needed
+// cells lie in it. Its half linestrength block is H_co = sum over segments e
+// of mu_{a(e)}[co, ci(e)] * G_co[e], with G_co[e][vI][j] =
+// sum_{(sI, w) in recipe(e, j)} w * c[sI, vI].
+struct OutClass {
+    int off = 0, n = 0;              // vibrational range (permuted indices)
+    std::vector<int> cols;           // upper slots, ascending
+    int smax = 0;                    // padded column count
+    std::vector<int> seg_a, seg_off, seg_n, seg_row0;  // K segments (component, class range)
+    int K = 0;
+    std::vector<int> rptr, rsI;      // recipe CSR over (segment, column)
+    std::vector<double> rcoef;
+    int hbase = 0, gbase = 0;        // offsets (doubles) in a unit's H and G buffers
+    int *d_ints = nullptr;           // packed device copy: seg_a, seg_off, seg_n, seg_row0, rptr, rsI
+    double *d_rcoef = nullptr;
+};
+
+// Plan of the symmetry-blocked half linestrength for lower block (J, gI) and
+// upper J' (the upper symmetry is igamma_pair[gI]).
+struct GroupPlan {
+    bool valid = false;
+    int jindI = 0, gI = 0, jindF = 0, gF = 0;
+    std::vector<OutClass> oc;
+    int Htot = 0, Gtot = 0, smax_all = 0;
+    double macs = 0;                 // multiply-adds per unit
+    std::vector<int> tcol[GI_MAXDEG];    // fold map: symmetrised index -> H offset
+    int *d_tcol[GI_MAXDEG] = {nullptr, nullptr};
 };
 
 // Resident eigenvector block of one (J, symmetry) and its levels in energy order.
@@ -106,6 +137,10 @@ struct gi_ctx {
     std::vector<double> threej;
     gi::JPlan jp[2];
     gi::Coef coef[2][2];             // [jindI][jindF]
+    std::vector<int> vperm, vpos;    // vibrational permutation (new -> old, old -> new)
+    std::vector<int> vcls_off, vcls_n;   // symmetry classes: contiguous ranges of new indices
+    double *d_mu_perm = nullptr;     // mu with rows and columns permuted by vperm
+    std::vector<gi::GroupPlan> groups;   // [(jindI*nrep + gI)*2 + jindF]
 
     // scratch (grown on demand)
     std::map<std::string, std::pair<void *, size_t>> scratch;
@@ -113,17 +148,54 @@ struct gi_ctx {
     // statistics of the last gi_compute
     std::map<std::string, gi::KStat> kstats;
     std::vector<std::string> kstat_names;
-    double work[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double work[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 namespace gi {
 
 int fail(gi_ctx *c, int code, const std::string &msg);
+
+// Register-tile width (columns) of the half-ls kernels for `cols` columns.
+inline int smax_for(int cols)
+{
+    const int opts[] = {8, 16, 24, 32, 48, 64};
+    for (int o : opts)
+        if (cols <= o) return o;
+    return -1;
+}
 void *scratch(gi_ctx *c, const char *name, size_t bytes, int *rc);
 
+inline SymDev symdev(const SymMap &m)
+{
+    return SymDev{m.d_rptr, m.d_rcol, m.d_rval, m.d_tptr, m.d_tcol, m.d_tval};
+}
+
+// Device view of output class oc (its packed integer arrays: seg_a, seg_off,
+// seg_n, seg_row0[nseg+1], rptr[nseg*ncols+1], rsI).
+inline ClassDev classdev(const OutClass &oc)
+{
+    ClassDev d;
+    d.off = oc.off; d.n = oc.n; d.ncols = (int)oc.cols.size(); d.smax = oc.smax; d.K = oc.K;
+    d.nseg = (int)oc.seg_a.size(); d.hbase = oc.hbase; d.gbase = oc.gbase;
+    const int ns = d.nseg;
+    d.seg_a = oc.d_ints;
+    d.seg_off = oc.d_ints + ns;
+    d.seg_n = oc.d_ints + 2 * ns;
+    d.seg_row0 = oc.d_ints + 3 * ns;
+    d.rptr = oc.d_ints + 4 * ns + 1;
+    d.rsI = d.rptr + (ns * d.ncols + 1);
+    d.rcoef = oc.d_rcoef;
+    return d;
+}
+
+int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out);
+
 // host plan construction (gi_plan.cpp)
-int build_jplan(gi_ctx *c, int jind);
+int assign_slots(gi_ctx *c, int jind);
+int build_vib_classes(gi_ctx *c);
+int build_symmaps(gi_ctx *c, int jind);
 int build_coef(gi_ctx *c, int jindI, int jindF);
+int build_group(gi_ctx *c, int jindI, int gI, int jindF);
 double three_j(int j1, int j2, int j3, int k1, int k2, int k3);
 void threej_table(int jmax, double *out);
 bool energy_filter_lower(const gi_ctx *c, int J, double e, const int *q);

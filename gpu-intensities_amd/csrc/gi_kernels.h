@@ -23,6 +23,13 @@ struct LineSoA {
     double *nu, *S, *A, *I, *amp;
 };
 
+// Device view of one output class of a group plan (see OutClass).
+struct ClassDev {
+    int off, n, ncols, smax, K, nseg, hbase, gbase;
+    const int *seg_a, *seg_off, *seg_n, *seg_row0, *rptr, *rsI;
+    const double *rcoef;
+};
+
 struct CorrArgs {
     int ncand, nitems, nsizeF, ndegI, ndegF, cb_mask, reduced, jI, jF;
     const int *item_cand_start, *item_lo, *item_level;
@@ -37,6 +44,38 @@ struct CorrArgs {
     double *out_nu, *out_S, *out_A, *out_I, *out_amp;
 };
 
+// One chunk of the banded correlation product (batched mode).
+struct CorrMMArgs {
+    int Lo, Hi;                      // union of the items' windows (block-level indices)
+    int nitems, ncb, ndegI, nsize;
+    const int *item_lo, *item_hi;
+    const long long *item_out_off;
+    const int *blk_col;
+    const double *vec, *Ht;
+    double *amp;                     // candidate amplitudes, 4 per candidate
+};
+
+// Intensity step from stored amplitudes (batched mode).
+struct IntensArgs {
+    int ncand, nitems, ndegI, ndegF, cb_mask, reduced, jI, jF;
+    const int *item_cand_start, *item_lo, *item_level;
+    const long long *item_out_off;
+    const double *item_energy;
+    const int *blk_lev;
+    const double *blk_en;
+    const unsigned char *blk_upok;
+    double gnsI, zpe, beta, q_stat, A_coef_s_1, intens_cm_mol, thresh_S, thresh_I;
+    double *amp;
+    int *flag, *out_iI, *out_iF;
+    double *out_nu, *out_S, *out_A, *out_I;
+};
+
+void launch_halfls_mfma(hipStream_t st, int nunits, int N, int Gtot, int Htot, const ClassDev &cd,
+                        const double *mu, const double *G, double *H);
+void launch_corr_mfma(hipStream_t st, const CorrMMArgs *d_args, int nchunks, int max_rows,
+                      int max_cols);
+void launch_intensity(hipStream_t st, const IntensArgs &a);
+
 void launch_correlate_prim(hipStream_t st, int dimen, const int *r_grid, SymDev R,
                            const double *u, double *out);
 void launch_grid_to_prim(hipStream_t st, int dimen, const int *r_grid, const double *grid,
@@ -46,6 +85,12 @@ void launch_build_g(hipStream_t st, int nunits, int N, int SI, int SF, int SMAX,
                     const int *cptr, const int *csI, const double *cval, double *G);
 int launch_half_ls_gemv(hipStream_t st, int nunits, int N, int SF, int SMAX, const double *Mu,
                         const double *G, double *H);
+void launch_permute_mu(hipStream_t st, int N, const int *perm, const double *in, double *out);
+void launch_build_g_sym(hipStream_t st, int nunits, int N, int Gtot, const ClassDev &cd,
+                        const double *const *uptr, const int *unit_ideg, SymDev r0, SymDev r1,
+                        double *G);
+int launch_gemv_sym(hipStream_t st, int nunits, int N, int Gtot, int Htot, const ClassDev &cd,
+                    const double *mu, const double *G, double *H);
 void launch_fold(hipStream_t st, int nitems, int nsizeF, int ndegI, int ndegF, int SFN,
                  const int *item_unit, SymDev t0, SymDev t1, const double *H, double *Ht);
 void launch_corr_intensity(hipStream_t st, const CorrArgs &a);

@@ -128,6 +128,102 @@ __global__ __launch_bounds__(256) void k_half_ls_gemv(int N, int SF, const doubl
 }
 
 // ---------------------------------------------------------------------------
+// mu with rows and columns permuted so that vibrational symmetry classes are
+// contiguous: out[a][i + j*N] = in[a][perm[i] + perm[j]*N].
+__global__ __launch_bounds__(256) void k_permute_mu(int N, const int *__restrict__ perm,
+                                                    const double *__restrict__ in,
+                                                    double *__restrict__ out)
+{
+    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long NN = (long long)N * N;
+    if (idx >= 3 * NN) return;
+    const int a = (int)(idx / NN);
+    const long long rem = idx - a * NN;
+    const int j = (int)(rem / N), i = (int)(rem - (long long)j * N);
+    out[idx] = in[a * NN + perm[i] + (long long)perm[j] * N];
+}
+
+// ---------------------------------------------------------------------------
+// K1 (symmetry-blocked): G rows of one output class. Row (segment e, vI)
+// holds, for each upper slot column j, sum_{(sI, w) in recipe(e, j)} w * c[sI, vI]
+// with c = R u evaluated on the fly from the CSR of R (one or a few terms).
+__global__ __launch_bounds__(64) void k_build_g_sym(int N, int Gtot, ClassDev cd,
+                                                    const double *const *__restrict__ uptr,
+                                                    const int *__restrict__ unit_ideg,
+                                                    SymDev r0, SymDev r1, double *__restrict__ G)
+{
+    const int unit = blockIdx.y;
+    const int row = blockIdx.x * 64 + threadIdx.x;
+    if (row >= cd.K) return;
+    int e = 0;
+    while (e + 1 < cd.nseg && cd.seg_row0[e + 1] <= row) e++;
+    const int vI = cd.seg_off[e] + (row - cd.seg_row0[e]);
+    const double *u = uptr[unit];
+    const SymDev R = unit_ideg[unit] ? r1 : r0;
+    double *Gr = G + (size_t)unit * Gtot + cd.gbase + (size_t)row * cd.smax;
+    for (int j = 0; j < cd.smax; j++) {
+        double v = 0.0;
+        if (j < cd.ncols) {
+            const int q = e * cd.ncols + j;
+            for (int p = cd.rptr[q]; p < cd.rptr[q + 1]; p++) {
+                const int cell = cd.rsI[p] * N + vI;
+                double cv = 0.0;
+                for (int x = R.rptr[cell]; x < R.rptr[cell + 1]; x++) cv += u[R.rcol[x]] * R.rval[x];
+                v += cd.rcoef[p] * cv;
+            }
+        }
+        Gr[j] = v;
+    }
+}
+
+// K2 (symmetry-blocked, GEMV form): H_co[vF][j] = sum over segments e of
+// sum_k mu_{a(e)}[off_co + vF, off_e + k] * G_co[row0_e + k][j], one thread per
+// vF row, the class's columns in registers, G rows wave-uniform.
+template <int SMAX>
+__global__ __launch_bounds__(256) void k_gemv_sym(int N, int Gtot, int Htot, ClassDev cd,
+                                                  const double *__restrict__ mu,
+                                                  const double *__restrict__ G,
+                                                  double *__restrict__ H)
+{
+    const int unit = blockIdx.x;
+    const int row = blockIdx.y * 256 + threadIdx.x;
+    const int rr = row < cd.n ? row : cd.n - 1;
+    const double *__restrict__ Gu = G + (size_t)unit * Gtot + cd.gbase;
+    double acc[SMAX];
+#pragma unroll
+    for (int s = 0; s < SMAX; s++) acc[s] = 0.0;
+    for (int e = 0; e < cd.nseg; e++) {
+        const int a = cd.seg_a[e], off = cd.seg_off[e], cnt = cd.seg_n[e], r0 = cd.seg_row0[e];
+        const double *__restrict__ mp = mu + (size_t)a * N * N + (size_t)off * N + (cd.off + rr);
+        const double *__restrict__ gp = Gu + (size_t)r0 * SMAX;
+        int k = 0;
+        for (; k + 4 <= cnt; k += 4) {
+            double m[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) m[q] = mp[(size_t)(k + q) * N];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const double *gk = gp + (size_t)(k + q) * SMAX;
+#pragma unroll
+                for (int s = 0; s < SMAX; s++) acc[s] = __builtin_fma(m[q], gk[s], acc[s]);
+            }
+        }
+        for (; k < cnt; k++) {
+            const double m = mp[(size_t)k * N];
+            const double *gk = gp + (size_t)k * SMAX;
+#pragma unroll
+            for (int s = 0; s < SMAX; s++) acc[s] = __builtin_fma(m, gk[s], acc[s]);
+        }
+    }
+    if (row < cd.n) {
+        double *Hc = H + (size_t)unit * Htot + cd.hbase;
+#pragma unroll
+        for (int s = 0; s < SMAX; s++)
+            if (s < cd.ncols) Hc[(size_t)s * cd.n + row] = acc[s];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // K3: fold H into the symmetrised space of the upper block:
 // Ht[item][cb][t] = sum_{(g, w) in R_F^T[t]} w * H[unit][g], cb = idegF*ndegI + idegI.
 __global__ __launch_bounds__(256) void k_fold(int nsizeF, int ndegI, int ndegF, int SFN,
@@ -354,6 +450,38 @@ int launch_half_ls_gemv(hipStream_t st, int nunits, int N, int SF, int SMAX, con
     case 32: hipLaunchKernelGGL(k_half_ls_gemv<32>, grid, dim3(256), 0, st, N, SF, Mu, G, H); break;
     case 48: hipLaunchKernelGGL(k_half_ls_gemv<48>, grid, dim3(256), 0, st, N, SF, Mu, G, H); break;
     case 64: hipLaunchKernelGGL(k_half_ls_gemv<64>, grid, dim3(256), 0, st, N, SF, Mu, G, H); break;
+    default: return -1;
+    }
+    return 0;
+}
+
+void launch_permute_mu(hipStream_t st, int N, const int *perm, const double *in, double *out)
+{
+    const long long n = 3LL * N * N;
+    hipLaunchKernelGGL(k_permute_mu, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, N,
+                       perm, in, out);
+}
+
+void launch_build_g_sym(hipStream_t st, int nunits, int N, int Gtot, const ClassDev &cd,
+                        const double *const *uptr, const int *unit_ideg, SymDev r0, SymDev r1,
+                        double *G)
+{
+    dim3 grid((cd.K + 63) / 64, nunits);
+    hipLaunchKernelGGL(k_build_g_sym, grid, dim3(64), 0, st, N, Gtot, cd, uptr, unit_ideg, r0,
+                       r1, G);
+}
+
+int launch_gemv_sym(hipStream_t st, int nunits, int N, int Gtot, int Htot, const ClassDev &cd,
+                    const double *mu, const double *G, double *H)
+{
+    dim3 grid(nunits, (cd.n + 255) / 256);
+    switch (cd.smax) {
+    case 8: hipLaunchKernelGGL(k_gemv_sym<8>, grid, dim3(256), 0, st, N, Gtot, Htot, cd, mu, G, H); break;
+    case 16: hipLaunchKernelGGL(k_gemv_sym<16>, grid, dim3(256), 0, st, N, Gtot, Htot, cd, mu, G, H); break;
+    case 24: hipLaunchKernelGGL(k_gemv_sym<24>, grid, dim3(256), 0, st, N, Gtot, Htot, cd, mu, G, H); break;
+    case 32: hipLaunchKernelGGL(k_gemv_sym<32>, grid, dim3(256), 0, st, N, Gtot, Htot, cd, mu, G, H); break;
+    case 48: hipLaunchKernelGGL(k_gemv_sym<48>, grid, dim3(256), 0, st, N, Gtot, Htot, cd, mu, G, H); break;
+    case 64: hipLaunchKernelGGL(k_gemv_sym<64>, grid, dim3(256), 0, st, N, Gtot, Htot, cd, mu, G, H); break;
     default: return -1;
     }
     return 0;

@@ -90,8 +90,10 @@ bool energy_filter_upper(const gi_ctx *c, int J, double e, const int *q)
            (e - c->zpe) <= c->erange_upper[1];
 }
 
-// Slot grid and symmetrisation maps of basis jind.
-int build_jplan(gi_ctx *c, int jind)
+// Slot grid of basis jind: every primitive r becomes a (slot, v) cell. A slot
+// carries the rotational labels (K, tau); a repeated (v, K, tau) opens
+// another slot with the same labels. Slots are ordered by (K, tau).
+int assign_slots(gi_ctx *c, int jind)
 {
     const auto &b = c->hb[jind];
     JPlan &p = c->jp[jind];
@@ -99,9 +101,8 @@ int build_jplan(gi_ctx *c, int jind)
     p.J = b.jval;
     p.dimen = b.dimen;
     const int N = c->nvib;
-    // slots: (K, tau, copy); a repeated (v, K, tau) opens a new copy
     std::map<std::pair<int, int>, std::vector<int>> slots_of;  // (K,tau) -> slot ids
-    std::map<long long, int> used;                             // (slot, v) taken
+    std::vector<std::vector<char>> used;                       // [slot][v]
     p.r_slot.resize(b.dimen);
     p.r_v.resize(b.dimen);
     for (int r = 0; r < b.dimen; r++) {
@@ -111,21 +112,24 @@ int build_jplan(gi_ctx *c, int jind)
                                           ": iroot_correlat_j0 out of the dipole's range");
         if (K < 0 || K > b.jval)
             return fail(c, GI_EINVAL, "basis J=" + std::to_string(b.jval) + ": k out of range");
+        const int irow = b.icontr2icase[r], ib = b.icontr2icase[r + b.dimen];
+        if (irow < 0 || irow >= b.ncases || ib < 0 || ib >= b.mat_size)
+            return fail(c, GI_EINVAL, "icontr2icase out of range");
         auto &ids = slots_of[{K, tau}];
         int chosen = -1;
         for (int s : ids)
-            if (!used.count((long long)s * N + v)) { chosen = s; break; }
+            if (!used[s][v]) { chosen = s; break; }
         if (chosen < 0) {
             chosen = p.nslots++;
             p.slot_k.push_back(K);
             p.slot_tau.push_back(tau);
+            used.emplace_back(N, 0);
             ids.push_back(chosen);
         }
-        used[(long long)chosen * N + v] = r;
+        used[chosen][v] = 1;
         p.r_slot[r] = chosen;
         p.r_v[r] = v;
     }
-    // order slots by (K, tau, copy) so the grid is laid out by K
     std::vector<int> order(p.nslots);
     for (int s = 0; s < p.nslots; s++) order[s] = s;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
@@ -141,7 +145,77 @@ int build_jplan(gi_ctx *c, int jind)
     p.slot_k = sk;
     p.slot_tau = st;
     for (int r = 0; r < b.dimen; r++) p.r_slot[r] = newid[p.r_slot[r]];
+    return GI_OK;
+}
 
+// Symmetry classes of the vibrational index: v and v' are in one class when,
+// for every (J, symmetry, component, slot), the cells (slot, v) and
+// (slot, v') are both or neither touched by the symmetrisation map R. Classes
+// are made contiguous by a permutation applied to the grid and to mu, so the
+// half linestrength only multiplies the blocks of mu that connect needed
+// cells (for C2v: the four vibrational symmetries).
+int build_vib_classes(gi_ctx *c)
+{
+    const int N = c->nvib;
+    std::vector<std::vector<long long>> sig(N);
+    for (int jind = 0; jind < 2; jind++) {
+        const auto &b = c->hb[jind];
+        const JPlan &p = c->jp[jind];
+        for (int r = 0; r < b.dimen; r++) {
+            const int irow = b.icontr2icase[r], ib = b.icontr2icase[r + b.dimen];
+            for (int g = 0; g < c->nrep; g++) {
+                const int nelem = b.N[g * b.ncases + irow];
+                const int iterm = b.ijterms[irow + g * b.ncases];
+                const int Ntot = b.Ntotal[g], sdeg = c->degen[g];
+                for (int ideg = 0; ideg < sdeg; ideg++) {
+                    bool nz = false;
+                    for (int e = 0; e < nelem; e++) {
+                        const int is = iterm + e;
+                        if (is < 0 || is >= Ntot || is >= b.nsize[g])
+                            return fail(c, GI_EINVAL, "ijterms/N exceed Ntotal or nsize");
+                        if (b.repres[g][(size_t)is + (size_t)ideg * Ntot +
+                                        (size_t)ib * sdeg * Ntot] != 0.0)
+                            nz = true;
+                    }
+                    if (nz)
+                        sig[b.v[r]].push_back(
+                            (((long long)(jind * c->nrep + g) * GI_MAXDEG + ideg) << 20) +
+                            p.r_slot[r]);
+                }
+            }
+        }
+    }
+    std::map<std::vector<long long>, int> cls_of;
+    std::vector<int> vcls(N);
+    for (int v = 0; v < N; v++) {
+        std::sort(sig[v].begin(), sig[v].end());
+        sig[v].erase(std::unique(sig[v].begin(), sig[v].end()), sig[v].end());
+        auto it = cls_of.find(sig[v]);
+        if (it == cls_of.end()) it = cls_of.emplace(sig[v], (int)cls_of.size()).first;
+        vcls[v] = it->second;
+    }
+    const int ncls = (int)cls_of.size();
+    c->vperm.resize(N);
+    c->vpos.resize(N);
+    for (int v = 0; v < N; v++) c->vperm[v] = v;
+    std::stable_sort(c->vperm.begin(), c->vperm.end(),
+                     [&](int x, int y) { return vcls[x] < vcls[y]; });
+    for (int i = 0; i < N; i++) c->vpos[c->vperm[i]] = i;
+    c->vcls_off.assign(ncls, 0);
+    c->vcls_n.assign(ncls, 0);
+    for (int v = 0; v < N; v++) c->vcls_n[vcls[v]]++;
+    for (int k = 1; k < ncls; k++) c->vcls_off[k] = c->vcls_off[k - 1] + c->vcls_n[k - 1];
+    for (int jind = 0; jind < 2; jind++)
+        for (auto &v : c->jp[jind].r_v) v = c->vpos[v];
+    return GI_OK;
+}
+
+// Symmetrisation maps of basis jind over the (permuted) slot grid.
+int build_symmaps(gi_ctx *c, int jind)
+{
+    const auto &b = c->hb[jind];
+    JPlan &p = c->jp[jind];
+    const int N = c->nvib;
     const long long G = (long long)p.nslots * N;
     p.sym.assign((size_t)c->nrep * GI_MAXDEG, SymMap());
     for (int g = 0; g < c->nrep; g++) {
@@ -154,15 +228,11 @@ int build_jplan(gi_ctx *c, int jind)
             for (int r = 0; r < b.dimen; r++) {
                 const int irow = b.icontr2icase[r];
                 const int ib = b.icontr2icase[r + b.dimen];
-                if (irow < 0 || irow >= b.ncases || ib < 0 || ib >= b.mat_size)
-                    return fail(c, GI_EINVAL, "icontr2icase out of range");
                 const int iterm = b.ijterms[irow + g * b.ncases];
                 const int nelem = b.N[g * b.ncases + irow];
                 const long long gr = (long long)p.r_slot[r] * N + p.r_v[r];
                 for (int e = 0; e < nelem; e++) {
                     const int is = iterm + e;
-                    if (is < 0 || is >= Ntot || is >= m.nsize)
-                        return fail(c, GI_EINVAL, "ijterms/N exceed Ntotal or nsize");
                     const double val =
                         b.repres[g][(size_t)is + (size_t)ideg * Ntot + (size_t)ib * sdeg * Ntot];
                     if (val != 0.0) rows[gr].push_back({is, val});
@@ -186,6 +256,107 @@ int build_jplan(gi_ctx *c, int jind)
                     m.tcol.push_back(e.first);
                     m.tval.push_back(e.second);
                 }
+        }
+    }
+    return GI_OK;
+}
+
+static bool row_nonempty(const SymMap &m, long long row)
+{
+    return m.rptr[row + 1] > m.rptr[row];
+}
+
+// Symmetry-blocked plan of the half linestrength from (J, gI) to J'.
+int build_group(gi_ctx *c, int jindI, int gI, int jindF)
+{
+    GroupPlan &G = c->groups[((size_t)jindI * c->nrep + gI) * 2 + jindF];
+    G = GroupPlan();
+    const Coef &cf = c->coef[jindI][jindF];
+    if (!cf.valid) return GI_OK;
+    const int gF = c->igamma_pair[gI];
+    const JPlan &pI = c->jp[jindI], &pF = c->jp[jindF];
+    const int N = c->nvib, ncls = (int)c->vcls_off.size();
+    const int SI = pI.nslots, SF = pF.nslots;
+    G.valid = true;
+    G.jindI = jindI; G.gI = gI; G.jindF = jindF; G.gF = gF;
+    // supports per (class, slot), read off the first member of the class
+    std::vector<char> inok((size_t)ncls * SI, 0), outok((size_t)ncls * SF, 0);
+    for (int k = 0; k < ncls; k++) {
+        const long long v = c->vcls_off[k];
+        for (int s = 0; s < SI; s++)
+            for (int d = 0; d < c->degen[gI]; d++)
+                if (row_nonempty(pI.sym[gI * GI_MAXDEG + d], (long long)s * N + v))
+                    inok[(size_t)k * SI + s] = 1;
+        for (int s = 0; s < SF; s++)
+            for (int d = 0; d < c->degen[gF]; d++)
+                if (row_nonempty(pF.sym[gF * GI_MAXDEG + d], (long long)s * N + v))
+                    outok[(size_t)k * SF + s] = 1;
+    }
+    std::vector<int> cls_oc(ncls, -1);
+    for (int co = 0; co < ncls; co++) {
+        OutClass oc;
+        oc.off = c->vcls_off[co];
+        oc.n = c->vcls_n[co];
+        for (int s = 0; s < SF; s++)
+            if (outok[(size_t)co * SF + s]) oc.cols.push_back(s);
+        if (oc.cols.empty()) continue;
+        const int ncol = (int)oc.cols.size();
+        oc.smax = smax_for(ncol);
+        if (oc.smax < 0) return fail(c, GI_EINVAL, "too many rotational slots in one class");
+        for (int a = 0; a < 3; a++)
+            for (int ci = 0; ci < ncls; ci++) {
+                bool used = false;
+                for (int j = 0; j < ncol && !used; j++) {
+                    const int sF = oc.cols[j];
+                    for (int p = cf.ptr[sF * 3 + a]; p < cf.ptr[sF * 3 + a + 1]; p++)
+                        if (inok[(size_t)ci * SI + cf.sI[p]]) { used = true; break; }
+                }
+                if (!used) continue;
+                oc.seg_a.push_back(a);
+                oc.seg_off.push_back(c->vcls_off[ci]);
+                oc.seg_n.push_back(c->vcls_n[ci]);
+                oc.seg_row0.push_back(oc.K);
+                oc.K += c->vcls_n[ci];
+                for (int j = 0; j < ncol; j++) {
+                    const int sF = oc.cols[j];
+                    oc.rptr.push_back((int)oc.rsI.size());
+                    for (int p = cf.ptr[sF * 3 + a]; p < cf.ptr[sF * 3 + a + 1]; p++)
+                        if (inok[(size_t)ci * SI + cf.sI[p]]) {
+                            oc.rsI.push_back(cf.sI[p]);
+                            oc.rcoef.push_back(cf.val[p]);
+                        }
+                }
+            }
+        oc.seg_row0.push_back(oc.K);
+        oc.rptr.push_back((int)oc.rsI.size());
+        oc.hbase = G.Htot;
+        G.Htot += oc.n * ncol;
+        oc.gbase = G.Gtot;
+        G.Gtot += oc.K * oc.smax;
+        G.smax_all = std::max(G.smax_all, oc.smax);
+        G.macs += (double)oc.n * oc.K * ncol;
+        cls_oc[co] = (int)G.oc.size();
+        G.oc.push_back(std::move(oc));
+    }
+    // class of each permuted vibrational index
+    std::vector<int> vcls(N);
+    for (int k = 0; k < ncls; k++)
+        for (int i = 0; i < c->vcls_n[k]; i++) vcls[c->vcls_off[k] + i] = k;
+    for (int d = 0; d < c->degen[gF]; d++) {
+        const SymMap &m = pF.sym[gF * GI_MAXDEG + d];
+        auto &tc = G.tcol[d];
+        tc.resize(m.tcol.size());
+        for (size_t e = 0; e < m.tcol.size(); e++) {
+            const int cell = m.tcol[e];
+            const int sF = cell / N, v = cell % N;
+            const int o = cls_oc[vcls[v]];
+            if (o < 0) return fail(c, GI_EINVAL, "internal: fold cell outside the plan");
+            const OutClass &oc = G.oc[o];
+            const int j = (int)(std::lower_bound(oc.cols.begin(), oc.cols.end(), sF) -
+                                oc.cols.begin());
+            if (j >= (int)oc.cols.size() || oc.cols[j] != sF)
+                return fail(c, GI_EINVAL, "internal: fold column outside the plan");
+            tc[e] = oc.hbase + j * oc.n + (v - oc.off);
         }
     }
     return GI_OK;

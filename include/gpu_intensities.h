@@ -35,7 +35,8 @@ typedef enum gi_status {
 /* Arithmetic of the half-linestrength and correlation products. Energies,
  * Boltzmann factors and the intensity step are FP64 in every mode. */
 typedef enum gi_mode {
-    GI_MODE_FP64 = 0   /* plain FP64: one GEMV-form product per lower level  */
+    GI_MODE_FP64 = 0,          /* plain FP64: one GEMV-form product per lower level */
+    GI_MODE_FP64_BATCHED = 1   /* FP64 matrix-core products over batches of lower levels */
 } gi_mode;
 
 /* Basis of one J: the TO_bset_contrT fields (trove_objects.h:49-72) read by
@@ -166,10 +167,13 @@ int gi_threej_table(int jmax, double *out);
 int gi_kernel_stats(gi_ctx *ctx, int max, const char **names, double *total_ms,
                     int *launches, int *n);
 
-/* Work counters of the last gi_compute: [0] half-ls units (lower x J'),
- * [1] candidates, [2] emitted lines, [3] half-ls useful flops (2*C*Nvib^2
- * summed), [4] half-ls issued flops, [5] correlation bytes (8*nsize_F per
- * candidate), [6] mu bytes counted once per unit (24*Nvib^2). */
+/* Work counters of the last gi_compute: [0] half-ls units (lower level x J'
+ * x component), [1] candidates, [2] emitted lines, [3] dense half-ls flops
+ * (2*C*Nvib^2 per unit, C = component-minimum count), [4] half-ls flops
+ * executed by the symmetry-blocked products, [5] correlation bytes
+ * (8*nsize_F per candidate), [6] mu bytes read by the half-ls products
+ * (per unit in GI_MODE_FP64, per batch in GI_MODE_FP64_BATCHED), [7] dense mu
+ * bytes (24*Nvib^2 per unit), [8] correlation flops executed. */
 int gi_work_stats(gi_ctx *ctx, double *out, int n);
 
 #ifdef __cplusplus

@@ -81,19 +81,24 @@ def test_half_ls_parity(engine, c1_job):
             assert np.abs(got - ref).max() <= 1e-12 * scale, (jindI, jindF, g)
 
 
-def test_c1_full_parity(engine, c1_job):
+MODES = [binding.GI_MODE_FP64, binding.GI_MODE_FP64_BATCHED]
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_c1_full_parity(engine, c1_job, mode):
     engine.load(c1_job)
-    got = engine.compute()
+    got = engine.compute(mode=mode)
     ref = oracle_py.OracleJob(c1_job).run()
     st = compare(got, ref)
     assert st["n"] == len(ref["S"]) > 1000
     print("C1", st)
 
 
-def test_lower_subset_and_order(engine, c1_job):
+@pytest.mark.parametrize("mode", MODES)
+def test_lower_subset_and_order(engine, c1_job, mode):
     engine.load(c1_job)
     ids = np.arange(0, c1_job.nlevels, 7)[::-1].copy()
-    got = engine.compute(ids)
+    got = engine.compute(ids, mode=mode)
     ref = oracle_py.OracleJob(c1_job).run(ids)
     compare(got, ref)
     # rows grouped by lower level in request order
@@ -104,24 +109,27 @@ def test_lower_subset_and_order(engine, c1_job):
     assert sorted(first, key=first.get) == order
 
 
-def test_nonsymmetric_mu_shuffled_cases(engine):
+@pytest.mark.parametrize("mode", MODES)
+def test_nonsymmetric_mu_shuffled_cases(engine, mode):
     job = synth.make_job("C2V", nvib=48, jvals=(1, 2), seed=21, symmetric_mu=False,
                          shuffle_cases=True)
     engine.load(job)
-    compare(engine.compute(), oracle_py.OracleJob(job).run())
+    compare(engine.compute(mode=mode), oracle_py.OracleJob(job).run())
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("reduced", [0, 1])
-def test_c3v_degenerate_parity(engine, reduced):
+def test_c3v_degenerate_parity(engine, reduced, mode):
     job = synth.make_job("C3V", nvib=30, jvals=(2, 3), seed=5, reduced=reduced)
     engine.load(job)
-    got = engine.compute()
+    got = engine.compute(mode=mode)
     ref = oracle_py.OracleJob(job).run()
     st = compare(got, ref)
     assert st["n"] > 100
 
 
-def test_windows_quanta_thresholds(engine):
+@pytest.mark.parametrize("mode", MODES)
+def test_windows_quanta_thresholds(engine, mode):
     job = synth.make_job("C2V", nvib=40, jvals=(3, 4), seed=9, e_span=20000.0,
                          freq_window=(-3000.0, 5000.0), erange_lower=(1000.0, 12000.0),
                          erange_upper=(500.0, 15000.0), thresh_intensity=1e-3,
@@ -129,7 +137,7 @@ def test_windows_quanta_thresholds(engine):
     job.quanta_lower[:] = [0, 1, 0, 5, 4, 5]
     job.quanta_upper[:] = [1, 0, 0, 5, 5, 3]
     engine.load(job)
-    got = engine.compute()
+    got = engine.compute(mode=mode)
     ref = oracle_py.OracleJob(job).run()
     compare(got, ref)
     assert len(ref["S"]) > 0
@@ -164,3 +172,24 @@ def _compute_with_capacity(engine, cap):
                         I=b["I"].ctypes.data_as(binding.P_d), amp=b["amp"].ctypes.data_as(binding.P_d))
     rc = L.gi_compute(engine.h, 0, 0, None, C.byref(s))
     engine._chk(rc, "gi_compute")
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_larger_job_many_chunks(engine, mode):
+    """More than one 256-item chunk per group, J 4 <-> 5, wide windows."""
+    job = synth.make_job("C2V", nvib=64, jvals=(4, 5), seed=31, e_span=30000.0,
+                         freq_window=(-4000.0, 9000.0))
+    engine.load(job)
+    ids = np.arange(0, job.nlevels, 3)
+    got = engine.compute(ids, mode=mode)
+    ref = oracle_py.OracleJob(job).run(ids)
+    st = compare(got, ref)
+    assert st["n"] > 20000
+
+
+def test_modes_agree_on_transition_set(engine):
+    job = synth.make_job("C3V", nvib=24, jvals=(3, 4), seed=13, e_span=12000.0)
+    engine.load(job)
+    a = engine.compute(mode=binding.GI_MODE_FP64)
+    b = engine.compute(mode=binding.GI_MODE_FP64_BATCHED)
+    compare(b, dict(a, amp_abs=np.abs(a["amp"]) * 0 + np.abs(a["amp"]).max()))
