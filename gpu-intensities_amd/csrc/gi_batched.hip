@@ -17,8 +17,9 @@
 // Both kernels: 256 threads (4 wavefronts) per 64 x 64 output tile, K staged
 // through LDS in steps of 16, each wavefront owns a 32 x 32 quarter as 2 x 2
 // MFMA 16x16 blocks. Operand layout of the 16x16x4 f64 instruction: lane l
-// supplies A[l%16][l/16] and B[l/16][l%16]; the accumulator holds
-// D[4*(l/16)+r][l%16], r = 0..3.
+// supplies A[l%16][l/16] and B[l/16][l%16]; accumulator register r holds
+// D[4*r + l/16][l%16], r = 0..3 (unlike the f32 16x16x4 form, whose register r
+// holds row 4*(l/16) + r).
 #include <hip/hip_runtime.h>
 
 #include "gi_kernels.h"
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(256) void k_halfls_mfma(int N, int nunits, int Gtot
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++)
             for (int r = 0; r < 4; r++) {
-                const int m = m0 + wm + 16 * a + 4 * (lane >> 4) + r;
+                const int m = m0 + wm + 16 * a + 4 * r + (lane >> 4);
                 const int n = n0 + wn + 16 * b + (lane & 15);
                 if (m >= cd.n || n >= ncolsTot) continue;
                 const int u = n / cd.smax, j = n - u * cd.smax;
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict_
     for (int x = 0; x < 2; x++)
         for (int y = 0; y < 2; y++)
             for (int r = 0; r < 4; r++) {
-                const int f = f0 + wm + 16 * x + 4 * (lane >> 4) + r;
+                const int f = f0 + wm + 16 * x + 4 * r + (lane >> 4);
                 const int n = n0 + wn + 16 * y + (lane & 15);
                 if (f >= a.Hi || n >= ncols) continue;
                 const int i = n / a.ncb, cb = n - i * a.ncb;
