@@ -1,7 +1,7 @@
 // Batched FP64 form of the half linestrength and the correlation on the
 // gfx950 matrix cores (v_mfma_f64_16x16x4f64).
 //
-// K5 (k_halfls_mfma): for one output class co and a chunk of units,
+// K5 (k_halfls_jobs): for one output class co and a chunk of units,
 //   H_co[m, (u, j)] = sum_k A[m, k] * B[k, (u, j)],
 //   A[m, k] = mu_{a(e)}[off_co + m, off_e + k - row0_e]   (column-major, ld N)
 //   B[k, (u, j)] = G[u*Gtot + gbase + k*smax + j]
@@ -31,7 +31,9 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 #define BM 64
 #define BN 64
 #define BK 16
-#define LDP (BM + 2)  // padded LDS row (doubles)
+// LDS row of 80 doubles: the four K rows one MFMA operand read touches fall
+// 32 banks apart, so the two half-waves of a 64-bit LDS read are conflict-free.
+#define LDP 80
 
 __device__ inline void mfma_tile(const double (*As)[LDP], const double (*Bs)[LDP], int wm, int wn,
                                  int lane, d4 acc[2][2])
@@ -50,62 +52,122 @@ __device__ inline void mfma_tile(const double (*As)[LDP], const double (*Bs)[LDP
 }
 
 // ---------------------------------------------------------------------------
-// K5
-__global__ __launch_bounds__(256) void k_halfls_mfma(int N, int nunits, int Gtot, int Htot,
-                                                     ClassDev cd, const double *__restrict__ mu,
-                                                     const double *__restrict__ G,
-                                                     double *__restrict__ H)
+// Flattened launches over many (chunk, output class) jobs: a workgroup finds
+// its job by binary search over the jobs' tile prefixes.
+__device__ inline int find_job(const int *__restrict__ begin, int njobs, int wg)
+{
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (begin[mid] <= wg) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// K1 over jobs: G rows of every (chunk, class) job (same arithmetic as
+// k_build_g_sym). Workgroup = 64 rows of one unit.
+__global__ __launch_bounds__(64) void k_build_g_jobs(int N, const HalfJob *__restrict__ jobs,
+                                                     const int *__restrict__ gbegin, int njobs)
+{
+    const int wg = blockIdx.x;
+    const HalfJob &J = jobs[find_job(gbegin, njobs, wg)];
+    const int local = wg - J.gtile_begin;
+    const int unit = local / J.gkt, kt = local - unit * J.gkt;
+    const ClassDev &cd = J.cd;
+    const int row = kt * 64 + threadIdx.x;
+    if (row >= cd.K) return;
+    int e = 0;
+    while (e + 1 < cd.nseg && cd.seg_row0[e + 1] <= row) e++;
+    const int vI = cd.seg_off[e] + (row - cd.seg_row0[e]);
+    const double *u = J.uptr[unit];
+    const SymDev R = J.uideg[unit] ? J.r1 : J.r0;
+    double *Gr = J.G + (size_t)unit * J.Gtot + cd.gbase + (size_t)row * cd.smax;
+    for (int j = 0; j < cd.smax; j++) {
+        double v = 0.0;
+        if (j < cd.ncols) {
+            const int q = e * cd.ncols + j;
+            for (int p = cd.rptr[q]; p < cd.rptr[q + 1]; p++) {
+                const int cell = cd.rsI[p] * N + vI;
+                double cv = 0.0;
+                for (int x = R.rptr[cell]; x < R.rptr[cell + 1]; x++) cv += u[R.rcol[x]] * R.rval[x];
+                v += cd.rcoef[p] * cv;
+            }
+        }
+        Gr[j] = v;
+    }
+}
+
+// K5 over jobs: H_co[m, (u, j)] = sum_k mu-block[m, k] * G[k, (u, j)] on the
+// matrix cores. K runs segment by segment (a tile never straddles two mu
+// blocks); the next K tile is fetched into registers while the current one
+// is multiplied.
+__global__ __launch_bounds__(256) void k_halfls_jobs(int N, const HalfJob *__restrict__ jobs,
+                                                     const int *__restrict__ tbegin, int njobs,
+                                                     const double *__restrict__ mu)
 {
     __shared__ double As[BK][LDP];
     __shared__ double Bs[BK][LDP];
+    const int wg = blockIdx.x;
+    const HalfJob &J = jobs[find_job(tbegin, njobs, wg)];
+    const ClassDev &cd = J.cd;
+    const int local = wg - J.tile_begin;
+    const int m0 = (local % J.mt) * BM, n0 = (local / J.mt) * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
-    const int ncolsTot = nunits * cd.smax;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int ncolsTot = J.nunits * cd.smax;
+    // this thread's load coordinates: A row m, B column n, K rows kl0 + 4q
+    const int lm = tid & 63, kl0 = tid >> 6;
+    const bool mok = m0 + lm < cd.n;
+    const int n = n0 + lm;
+    const bool nok = n < ncolsTot;
+    const double *gcol = J.G;
+    if (nok) {
+        const int u = n / cd.smax, jj = n - u * cd.smax;
+        gcol = J.G + (size_t)u * J.Gtot + cd.gbase + jj;
+    }
     d4 acc[2][2];
-    for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < cd.K; k0 += BK) {
-        // A tile: 64 m x 16 k; thread -> m = tid % 64, k = tid / 64 + 4q
+    for (int x = 0; x < 2; x++)
+        for (int y = 0; y < 2; y++) acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
+    double ra[4], rb[4];
+    int e = 0, kk0 = 0;
+    auto fetch = [&](int se, int sk0) {
+        const int len = min(BK, cd.seg_n[se] - sk0);
+        const double *mp = mu + (size_t)cd.seg_a[se] * N * N +
+                           (size_t)(cd.seg_off[se] + sk0) * N + (cd.off + m0 + lm);
+        const double *gp = gcol + (size_t)(cd.seg_row0[se] + sk0) * cd.smax;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int m = tid & 63, kl = (tid >> 6) + 4 * q;
-            const int k = k0 + kl;
-            double v = 0.0;
-            if (m0 + m < cd.n && k < cd.K) {
-                int e = 0;
-                while (e + 1 < cd.nseg && cd.seg_row0[e + 1] <= k) e++;
-                const int kk = cd.seg_off[e] + (k - cd.seg_row0[e]);
-                v = mu[(size_t)cd.seg_a[e] * N * N + (size_t)kk * N + (cd.off + m0 + m)];
-            }
-            As[kl][m] = v;
+            const int kl = kl0 + 4 * q;
+            ra[q] = (mok && kl < len) ? mp[(size_t)kl * N] : 0.0;
+            rb[q] = (nok && kl < len) ? gp[(size_t)kl * cd.smax] : 0.0;
         }
-        // B tile: 16 k x 64 n; n = u*smax + j
+    };
+    if (cd.nseg > 0) fetch(0, 0);
+    while (e < cd.nseg) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int e = tid + 256 * q;
-            const int nl = e & 63, kl = e >> 6;
-            const int n = n0 + nl, k = k0 + kl;
-            double v = 0.0;
-            if (n < ncolsTot && k < cd.K) {
-                const int u = n / cd.smax, j = n - u * cd.smax;
-                v = G[(size_t)u * Gtot + cd.gbase + (size_t)k * cd.smax + j];
-            }
-            Bs[kl][nl] = v;
+            As[kl0 + 4 * q][lm] = ra[q];
+            Bs[kl0 + 4 * q][lm] = rb[q];
         }
         __syncthreads();
+        kk0 += BK;
+        if (kk0 >= cd.seg_n[e]) {
+            e++;
+            kk0 = 0;
+        }
+        if (e < cd.nseg) fetch(e, kk0);
         mfma_tile(As, Bs, wm, wn, lane, acc);
         __syncthreads();
     }
-    for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++)
+    for (int x = 0; x < 2; x++)
+        for (int y = 0; y < 2; y++)
             for (int r = 0; r < 4; r++) {
-                const int m = m0 + wm + 16 * a + 4 * r + (lane >> 4);
-                const int n = n0 + wn + 16 * b + (lane & 15);
-                if (m >= cd.n || n >= ncolsTot) continue;
-                const int u = n / cd.smax, j = n - u * cd.smax;
-                if (j >= cd.ncols) continue;
-                H[(size_t)u * Htot + cd.hbase + (size_t)j * cd.n + m] = acc[a][b][r];
+                const int m = m0 + wm + 16 * x + 4 * r + (lane >> 4);
+                const int nn = n0 + wn + 16 * y + (lane & 15);
+                if (m >= cd.n || nn >= ncolsTot) continue;
+                const int u = nn / cd.smax, jj = nn - u * cd.smax;
+                if (jj >= cd.ncols) continue;
+                J.H[(size_t)u * J.Htot + cd.hbase + (size_t)jj * cd.n + m] = acc[x][y][r];
             }
 }
 
@@ -221,12 +283,16 @@ __global__ __launch_bounds__(256) void k_intensity(IntensArgs a)
 
 // ---------------------------------------------------------------------------
 
-void launch_halfls_mfma(hipStream_t st, int nunits, int N, int Gtot, int Htot, const ClassDev &cd,
-                        const double *mu, const double *G, double *H)
+void launch_halfls_jobs(hipStream_t st, int N, const HalfJob *d_jobs, const int *d_gbegin,
+                        const int *d_tbegin, int njobs, int gtiles, int ttiles, const double *mu)
 {
-    dim3 grid((cd.n + BM - 1) / BM, (nunits * cd.smax + BN - 1) / BN);
-    hipLaunchKernelGGL(k_halfls_mfma, grid, dim3(256), 0, st, N, nunits, Gtot, Htot, cd, mu, G,
-                       H);
+    if (njobs <= 0) return;
+    if (gtiles > 0)
+        hipLaunchKernelGGL(k_build_g_jobs, dim3(gtiles), dim3(64), 0, st, N, d_jobs, d_gbegin,
+                           njobs);
+    if (ttiles > 0)
+        hipLaunchKernelGGL(k_halfls_jobs, dim3(ttiles), dim3(256), 0, st, N, d_jobs, d_tbegin,
+                           njobs, mu);
 }
 
 void launch_corr_mfma(hipStream_t st, const CorrMMArgs *d_args, int nchunks, int max_rows,

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -149,6 +150,198 @@ struct Chunk {
     size_t off_uptr, off_uideg, off_iunit, off_cstart, off_lo, off_hi, off_level, off_eI, off_out;
     size_t ht_off;  // doubles, batched mode
 };
+
+// Batched FP64 mode. Chunks are grouped into rounds whose G, H and folded
+// vectors fit a memory budget; per round one launch builds every G, one
+// launch runs every (chunk, class) product on the matrix cores, the folds run
+// per chunk, one launch runs every banded correlation, and the intensity step
+// runs per chunk.
+static int run_batched(gi_ctx *c, const std::vector<Chunk> &chunks, unsigned char *d_meta,
+                       Timeline &tl, double beta, double A_coef_s_1, double intens_cm_mol,
+                       int *flag, LineSoA cand, double &useful, double &issued,
+                       double &corr_bytes, double &mu_bytes, double &mu_dense, double &corr_flops)
+{
+    const int N = c->nvib;
+    int rc = GI_OK;
+    // device memory for one round's G, H and folded vectors (GI_BATCH_BYTES overrides)
+    size_t kRoundBytes = (size_t)12 << 30;
+    if (const char *env = std::getenv("GI_BATCH_BYTES")) {
+        const long long v = std::atoll(env);
+        if (v > 0) kRoundBytes = (size_t)v;
+    }
+    struct Need {
+        size_t g, h, ht;
+        int njobs;
+    };
+    std::vector<Need> need(chunks.size());
+    for (size_t i = 0; i < chunks.size(); i++) {
+        const Chunk &ch = chunks[i];
+        const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
+        const int ncb = c->degen[ch.gI] * c->degen[ch.gF];
+        const int nsizeF = c->blocks[ch.jindF * c->nrep + ch.gF].nsize;
+        need[i] = Need{(size_t)ch.nunits * GP.Gtot, (size_t)ch.nunits * GP.Htot,
+                       (size_t)ch.nitems * ncb * nsizeF, ch.nunits > 0 ? (int)GP.oc.size() : 0};
+    }
+    std::vector<std::pair<size_t, size_t>> rounds;
+    size_t start = 0, acc = 0, mg = 1, mh = 1, mht = 1, sg = 0, sh = 0, sht = 0;
+    int mjobs = 1, sjobs = 0, mchunks = 1;
+    for (size_t i = 0; i <= chunks.size(); i++) {
+        const size_t bytes = i < chunks.size() ? 8 * (need[i].g + need[i].h + need[i].ht) : 0;
+        if (i == chunks.size() || (i > start && acc + bytes > kRoundBytes)) {
+            rounds.push_back({start, i});
+            mg = std::max(mg, sg); mh = std::max(mh, sh); mht = std::max(mht, sht);
+            mjobs = std::max(mjobs, sjobs);
+            mchunks = std::max(mchunks, (int)(i - start));
+            start = i;
+            acc = sg = sh = sht = 0;
+            sjobs = 0;
+        }
+        if (i == chunks.size()) break;
+        acc += bytes;
+        sg += need[i].g; sh += need[i].h; sht += need[i].ht;
+        sjobs += need[i].njobs;
+    }
+    // every buffer is sized for the largest round before anything is launched
+    double *G = (double *)scratch(c, "G", mg * 8, &rc); if (rc) return rc;
+    double *H = (double *)scratch(c, "H", mh * 8, &rc); if (rc) return rc;
+    double *Ht = (double *)scratch(c, "Ht", mht * 8, &rc); if (rc) return rc;
+    const size_t jobs_bytes = (size_t)mjobs * sizeof(HalfJob);
+    unsigned char *d_jobs = (unsigned char *)scratch(c, "jobs", jobs_bytes + 2 * 4 * (size_t)mjobs, &rc);
+    if (rc) return rc;
+    CorrMMArgs *d_mm = (CorrMMArgs *)scratch(c, "mmargs", (size_t)mchunks * sizeof(CorrMMArgs), &rc);
+    if (rc) return rc;
+    std::vector<std::vector<unsigned char>> keep;  // host sources of the async copies
+
+    for (auto &rg : rounds) {
+        std::vector<HalfJob> jobs;
+        std::vector<int> gbegin, tbegin;
+        std::vector<CorrMMArgs> mm;
+        std::vector<double *> Hc(rg.second - rg.first), Htc(rg.second - rg.first);
+        int gt = 0, tt = 0;
+        size_t og = 0, oh = 0, oht = 0;
+        for (size_t i = rg.first; i < rg.second; i++) {
+            const Chunk &ch = chunks[i];
+            const JPlan &pI = c->jp[ch.jindI];
+            const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
+            const int ndegI = c->degen[ch.gI];
+            SymDev r0 = symdev(pI.sym[ch.gI * GI_MAXDEG + 0]);
+            SymDev r1 = ndegI > 1 ? symdev(pI.sym[ch.gI * GI_MAXDEG + 1]) : r0;
+            Hc[i - rg.first] = H + oh;
+            Htc[i - rg.first] = Ht + oht;
+            if (ch.nunits > 0)
+                for (const OutClass &oc : GP.oc) {
+                    HalfJob j;
+                    j.cd = classdev(oc);
+                    j.nunits = ch.nunits; j.Gtot = GP.Gtot; j.Htot = GP.Htot;
+                    j.tile_begin = tt;
+                    j.mt = (oc.n + 63) / 64;
+                    tt += j.mt * ((ch.nunits * oc.smax + 63) / 64);
+                    j.gtile_begin = gt;
+                    j.gkt = (oc.K + 63) / 64;
+                    gt += oc.K > 0 ? j.gkt * ch.nunits : 0;
+                    j.uptr = (const double *const *)(d_meta + ch.off_uptr);
+                    j.uideg = (const int *)(d_meta + ch.off_uideg);
+                    j.r0 = r0; j.r1 = r1;
+                    j.G = G + og; j.H = H + oh;
+                    jobs.push_back(j);
+                    gbegin.push_back(j.gtile_begin);
+                    tbegin.push_back(j.tile_begin);
+                }
+            og += need[i].g; oh += need[i].h; oht += need[i].ht;
+        }
+        const int njobs = (int)jobs.size();
+        if (njobs > 0) {
+            keep.emplace_back(jobs_bytes + 2 * 4 * (size_t)mjobs);
+            unsigned char *hb = keep.back().data();
+            std::memcpy(hb, jobs.data(), njobs * sizeof(HalfJob));
+            std::memcpy(hb + jobs_bytes, gbegin.data(), njobs * 4);
+            std::memcpy(hb + jobs_bytes + 4 * (size_t)mjobs, tbegin.data(), njobs * 4);
+            HIPCHK(hipMemcpyAsync(d_jobs, hb, keep.back().size(), hipMemcpyHostToDevice, c->stream));
+            const int *d_gb = (const int *)(d_jobs + jobs_bytes);
+            const int *d_tb = (const int *)(d_jobs + jobs_bytes + 4 * (size_t)mjobs);
+            TL_BEGIN("build_g");
+            launch_halfls_jobs(c->stream, N, (const HalfJob *)d_jobs, d_gb, d_tb, njobs, gt, 0,
+                               c->d_mu_perm);
+            TL_END();
+            TL_BEGIN("half_ls_mfma");
+            launch_halfls_jobs(c->stream, N, (const HalfJob *)d_jobs, d_gb, d_tb, njobs, 0, tt,
+                               c->d_mu_perm);
+            TL_END();
+        }
+        for (size_t i = rg.first; i < rg.second; i++) {
+            const Chunk &ch = chunks[i];
+            const JPlan &pF = c->jp[ch.jindF];
+            const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
+            const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
+            const int ndegI = c->degen[ch.gI], ndegF = c->degen[ch.gF];
+            SymDev t0 = symdev(pF.sym[ch.gF * GI_MAXDEG + 0]);
+            t0.tcol = GP.d_tcol[0];
+            SymDev t1 = t0;
+            if (ndegF > 1) {
+                t1 = symdev(pF.sym[ch.gF * GI_MAXDEG + 1]);
+                t1.tcol = GP.d_tcol[1];
+            }
+            TL_BEGIN("fold");
+            launch_fold(c->stream, ch.nitems, BF.nsize, ndegI, ndegF, GP.Htot,
+                        (const int *)(d_meta + ch.off_iunit), t0, t1, Hc[i - rg.first],
+                        Htc[i - rg.first]);
+            TL_END();
+            CorrMMArgs m;
+            m.Lo = ch.Lo; m.Hi = ch.Hi; m.nitems = ch.nitems; m.ncb = ndegI * ndegF;
+            m.ndegI = ndegI; m.nsize = BF.nsize;
+            m.item_lo = (const int *)(d_meta + ch.off_lo);
+            m.item_hi = (const int *)(d_meta + ch.off_hi);
+            m.item_out_off = (const long long *)(d_meta + ch.off_out);
+            m.blk_col = BF.d_col; m.vec = BF.d_vec; m.Ht = Htc[i - rg.first]; m.amp = cand.amp;
+            mm.push_back(m);
+            corr_flops += 2.0 * (double)(ch.Hi - ch.Lo) * ch.nitems * ndegI * ndegF * BF.nsize;
+            double blk_bytes = 0;
+            for (const OutClass &oc : GP.oc) blk_bytes += 8.0 * oc.n * oc.K;
+            useful += ch.nunits * 2.0 * c->coef[ch.jindI][ch.jindF].C * (double)N * N;
+            issued += ch.nunits * 2.0 * GP.macs;
+            mu_bytes += blk_bytes;
+            mu_dense += ch.nunits * 24.0 * (double)N * N;
+            corr_bytes += (double)ch.ncand * 8.0 * BF.nsize;
+        }
+        int max_rows = 0, max_cols = 0;
+        for (auto &m : mm) {
+            max_rows = std::max(max_rows, m.Hi - m.Lo);
+            max_cols = std::max(max_cols, m.nitems * m.ncb);
+        }
+        keep.emplace_back(mm.size() * sizeof(CorrMMArgs));
+        std::memcpy(keep.back().data(), mm.data(), keep.back().size());
+        HIPCHK(hipMemcpyAsync(d_mm, keep.back().data(), keep.back().size(), hipMemcpyHostToDevice,
+                              c->stream));
+        TL_BEGIN("corr_mfma");
+        launch_corr_mfma(c->stream, d_mm, (int)mm.size(), max_rows, max_cols);
+        TL_END();
+        for (size_t i = rg.first; i < rg.second; i++) {
+            const Chunk &ch = chunks[i];
+            const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
+            IntensArgs a;
+            a.ncand = ch.ncand; a.nitems = ch.nitems;
+            a.ndegI = c->degen[ch.gI]; a.ndegF = c->degen[ch.gF]; a.cb_mask = ch.cb_mask;
+            a.reduced = c->reduced; a.jI = c->jvals[ch.jindI]; a.jF = c->jvals[ch.jindF];
+            a.item_cand_start = (const int *)(d_meta + ch.off_cstart);
+            a.item_lo = (const int *)(d_meta + ch.off_lo);
+            a.item_level = (const int *)(d_meta + ch.off_level);
+            a.item_out_off = (const long long *)(d_meta + ch.off_out);
+            a.item_energy = (const double *)(d_meta + ch.off_eI);
+            a.blk_lev = BF.d_lev; a.blk_en = BF.d_en; a.blk_upok = BF.d_upok;
+            a.gnsI = c->gns[ch.gI]; a.zpe = c->zpe; a.beta = beta; a.q_stat = c->q_stat;
+            a.A_coef_s_1 = A_coef_s_1; a.intens_cm_mol = intens_cm_mol;
+            a.thresh_S = c->thresh_S; a.thresh_I = c->thresh_I;
+            a.amp = cand.amp; a.flag = flag; a.out_iI = cand.iI; a.out_iF = cand.iF;
+            a.out_nu = cand.nu; a.out_S = cand.S; a.out_A = cand.A; a.out_I = cand.I;
+            TL_BEGIN("intensity");
+            launch_intensity(c->stream, a);
+            TL_END();
+        }
+    }
+    // the async copies read `keep`; wait before it goes out of scope
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GI_OK;
+}
 
 int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
 {
@@ -296,15 +489,23 @@ int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
     }
     unsigned char *d_meta = (unsigned char *)scratch(c, "meta", meta.size(), &rc); if (rc) return rc;
     HIPCHK(hipMemcpyAsync(d_meta, meta.data(), meta.size(), hipMemcpyHostToDevice, c->stream));
-    double *G = (double *)scratch(c, "G", (size_t)max_units * max_G * 8, &rc); if (rc) return rc;
-    double *H = (double *)scratch(c, "H", (size_t)max_units * max_H * 8, &rc); if (rc) return rc;
-    const size_t ht_elems = batched ? ht_total : (size_t)max_items * max_ncb * max_nsizeF;
-    double *Ht = (double *)scratch(c, "Ht", ht_elems * 8, &rc); if (rc) return rc;
-
+    (void)ht_total;
     Timeline tl(c);
     double useful = 0, issued = 0, corr_bytes = 0, mu_bytes = 0, mu_dense = 0, corr_flops = 0;
-    std::vector<CorrMMArgs> mm;
+    if (batched) {
+        if ((rc = run_batched(c, chunks, d_meta, tl, beta, A_coef_s_1, intens_cm_mol, flag, cand,
+                              useful, issued, corr_bytes, mu_bytes, mu_dense, corr_flops)) != GI_OK)
+            return rc;
+    }
+    double *G = nullptr, *H = nullptr, *Ht = nullptr;
+    if (!batched) {
+        G = (double *)scratch(c, "G", (size_t)max_units * max_G * 8, &rc); if (rc) return rc;
+        H = (double *)scratch(c, "H", (size_t)max_units * max_H * 8, &rc); if (rc) return rc;
+        Ht = (double *)scratch(c, "Ht", (size_t)max_items * max_ncb * max_nsizeF * 8, &rc);
+        if (rc) return rc;
+    }
     for (auto &ch : chunks) {
+        if (batched) break;
         const JPlan &pI = c->jp[ch.jindI], &pF = c->jp[ch.jindF];
         const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
         const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
@@ -328,36 +529,19 @@ int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
                                        (const int *)(d_meta + ch.off_uideg), r0, r1, G);
                     TL_END();
                 }
-                if (batched) {
-                    TL_BEGIN("half_ls_mfma");
-                    launch_halfls_mfma(c->stream, ch.nunits, N, GP.Gtot, GP.Htot, cd,
-                                       c->d_mu_perm, G, H);
-                    TL_END();
-                } else {
-                    TL_BEGIN("half_ls_gemv");
-                    if (launch_gemv_sym(c->stream, ch.nunits, N, GP.Gtot, GP.Htot, cd,
-                                        c->d_mu_perm, G, H) != 0)
-                        return fail(c, GI_EINVAL, "no half-ls kernel for this column count");
-                    TL_END();
-                }
+                TL_BEGIN("half_ls_gemv");
+                if (launch_gemv_sym(c->stream, ch.nunits, N, GP.Gtot, GP.Htot, cd,
+                                    c->d_mu_perm, G, H) != 0)
+                    return fail(c, GI_EINVAL, "no half-ls kernel for this column count");
+                TL_END();
             }
         }
-        double *Htc = Ht + ch.ht_off;
+        double *Htc = Ht;
         TL_BEGIN("fold");
         launch_fold(c->stream, ch.nitems, BF.nsize, ndegI, ndegF, GP.Htot,
                     (const int *)(d_meta + ch.off_iunit), t0, t1, H, Htc);
         TL_END();
-        if (batched) {
-            CorrMMArgs m;
-            m.Lo = ch.Lo; m.Hi = ch.Hi; m.nitems = ch.nitems; m.ncb = ndegI * ndegF;
-            m.ndegI = ndegI; m.nsize = BF.nsize;
-            m.item_lo = (const int *)(d_meta + ch.off_lo);
-            m.item_hi = (const int *)(d_meta + ch.off_hi);
-            m.item_out_off = (const long long *)(d_meta + ch.off_out);
-            m.blk_col = BF.d_col; m.vec = BF.d_vec; m.Ht = Htc; m.amp = cand.amp;
-            mm.push_back(m);
-            corr_flops += 2.0 * (double)(ch.Hi - ch.Lo) * ch.nitems * ndegI * ndegF * BF.nsize;
-        } else {
+        {
             CorrArgs a;
             a.ncand = ch.ncand; a.nitems = ch.nitems; a.nsizeF = BF.nsize;
             a.ndegI = ndegI; a.ndegF = ndegF; a.cb_mask = ch.cb_mask; a.reduced = c->reduced;
@@ -383,44 +567,9 @@ int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
         for (const OutClass &oc : GP.oc) blk_bytes += 8.0 * oc.n * oc.K;
         useful += ch.nunits * 2.0 * c->coef[ch.jindI][ch.jindF].C * (double)N * N;
         issued += ch.nunits * 2.0 * GP.macs;
-        mu_bytes += (batched ? 1.0 : ch.nunits) * blk_bytes;
+        mu_bytes += ch.nunits * blk_bytes;
         mu_dense += ch.nunits * 24.0 * (double)N * N;
         corr_bytes += (double)ch.ncand * 8.0 * BF.nsize;
-    }
-    if (batched && !mm.empty()) {
-        CorrMMArgs *d_mm = (CorrMMArgs *)scratch(c, "mmargs", mm.size() * sizeof(CorrMMArgs), &rc);
-        if (rc) return rc;
-        HIPCHK(hipMemcpyAsync(d_mm, mm.data(), mm.size() * sizeof(CorrMMArgs),
-                              hipMemcpyHostToDevice, c->stream));
-        int max_rows = 0, max_cols = 0;
-        for (auto &m : mm) {
-            max_rows = std::max(max_rows, m.Hi - m.Lo);
-            max_cols = std::max(max_cols, m.nitems * m.ncb);
-        }
-        TL_BEGIN("corr_mfma");
-        launch_corr_mfma(c->stream, d_mm, (int)mm.size(), max_rows, max_cols);
-        TL_END();
-        for (auto &ch : chunks) {
-            const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
-            IntensArgs a;
-            a.ncand = ch.ncand; a.nitems = ch.nitems;
-            a.ndegI = c->degen[ch.gI]; a.ndegF = c->degen[ch.gF]; a.cb_mask = ch.cb_mask;
-            a.reduced = c->reduced; a.jI = c->jvals[ch.jindI]; a.jF = c->jvals[ch.jindF];
-            a.item_cand_start = (const int *)(d_meta + ch.off_cstart);
-            a.item_lo = (const int *)(d_meta + ch.off_lo);
-            a.item_level = (const int *)(d_meta + ch.off_level);
-            a.item_out_off = (const long long *)(d_meta + ch.off_out);
-            a.item_energy = (const double *)(d_meta + ch.off_eI);
-            a.blk_lev = BF.d_lev; a.blk_en = BF.d_en; a.blk_upok = BF.d_upok;
-            a.gnsI = c->gns[ch.gI]; a.zpe = c->zpe; a.beta = beta; a.q_stat = c->q_stat;
-            a.A_coef_s_1 = A_coef_s_1; a.intens_cm_mol = intens_cm_mol;
-            a.thresh_S = c->thresh_S; a.thresh_I = c->thresh_I;
-            a.amp = cand.amp; a.flag = flag; a.out_iI = cand.iI; a.out_iF = cand.iF;
-            a.out_nu = cand.nu; a.out_S = cand.S; a.out_A = cand.A; a.out_I = cand.I;
-            TL_BEGIN("intensity");
-            launch_intensity(c->stream, a);
-            TL_END();
-        }
     }
     HIPCHK(hipGetLastError());
     const int nb = compact_blocks(ncand);

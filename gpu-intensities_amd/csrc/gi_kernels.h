@@ -70,8 +70,20 @@ struct IntensArgs {
     double *out_nu, *out_S, *out_A, *out_I;
 };
 
-void launch_halfls_mfma(hipStream_t st, int nunits, int N, int Gtot, int Htot, const ClassDev &cd,
-                        const double *mu, const double *G, double *H);
+// One (chunk, output class) of the batched half linestrength.
+struct HalfJob {
+    ClassDev cd;
+    int nunits, Gtot, Htot;
+    int tile_begin, gtile_begin;     // first workgroup of the product / G build
+    int mt, gkt;                     // product tiles along rows, G-build tiles along K
+    const double *const *uptr;
+    const int *uideg;
+    SymDev r0, r1;
+    double *G, *H;                   // this chunk's G and H
+};
+
+void launch_halfls_jobs(hipStream_t st, int N, const HalfJob *d_jobs, const int *d_gbegin,
+                        const int *d_tbegin, int njobs, int gtiles, int ttiles, const double *mu);
 void launch_corr_mfma(hipStream_t st, const CorrMMArgs *d_args, int nchunks, int max_rows,
                       int max_cols);
 void launch_intensity(hipStream_t st, const IntensArgs &a);

@@ -193,3 +193,13 @@ def test_modes_agree_on_transition_set(engine):
     a = engine.compute(mode=binding.GI_MODE_FP64)
     b = engine.compute(mode=binding.GI_MODE_FP64_BATCHED)
     compare(b, dict(a, amp_abs=np.abs(a["amp"]) * 0 + np.abs(a["amp"]).max()))
+
+
+def test_batched_many_rounds(engine, monkeypatch):
+    """A tiny round budget splits the batched mode into many rounds."""
+    job = synth.make_job("C2V", nvib=32, jvals=(2, 3), seed=41, e_span=15000.0)
+    engine.load(job)
+    ref = oracle_py.OracleJob(job).run()
+    monkeypatch.setenv("GI_BATCH_BYTES", str(1 << 20))
+    got = engine.compute(mode=binding.GI_MODE_FP64_BATCHED)
+    compare(got, ref)
