@@ -21,13 +21,16 @@ GI_MODE_FP64 = 0
 GI_MODE_FP64_BATCHED = 1
 MODES = {"fp64": GI_MODE_FP64, "fp64_batched": GI_MODE_FP64_BATCHED}
 STATUS = {0: "GI_OK", 1: "GI_EINVAL", 2: "GI_EHIP", 3: "GI_ENOMEM", 4: "GI_ESTATE",
-          5: "GI_ECAPACITY", 6: "GI_EBUDGET"}
+          5: "GI_ECAPACITY", 6: "GI_EBUDGET", 7: "GI_ETIMEOUT"}
+GI_EBUDGET, GI_ETIMEOUT = 6, 7
 
 # Every symbol declared in include/gpu_intensities.h.
 EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_params", "gi_upload_bset",
            "gi_upload_dipole", "gi_set_levels", "gi_upload_vectors", "gi_prepare", "gi_compute",
            "gi_count_candidates", "gi_half_ls", "gi_correlate", "gi_threej_table",
-           "gi_kernel_stats", "gi_work_stats"]
+           "gi_kernel_stats", "gi_work_stats", "gi_partition_function", "gi_serve_start",
+           "gi_serve_submit", "gi_serve_wait", "gi_serve_fetch", "gi_serve_stats",
+           "gi_serve_stop"]
 
 
 class GIError(RuntimeError):
@@ -49,6 +52,13 @@ class GiParams(C.Structure):
                 ("erange_lower", cd * 2), ("erange_upper", cd * 2), ("jvals", ci * 2),
                 ("gns", P_d), ("isym_pairs", P_i), ("reduced", ci), ("quanta_lower", P_i),
                 ("quanta_upper", P_i), ("thresh_linestrength", cd), ("thresh_intensity", cd)]
+
+
+class GiRequest(C.Structure):
+    _fields_ = [("jindI", ci), ("igammaI", ci), ("jindF", ci), ("mode", ci),
+                ("temperature", cd), ("q_stat", cd), ("elow_lo", cd), ("elow_hi", cd),
+                ("nu_lo", cd), ("nu_hi", cd), ("thresh_linestrength", cd),
+                ("thresh_intensity", cd)]
 
 
 class GiLines(C.Structure):
@@ -81,6 +91,13 @@ def lib():
         L.gi_threej_table.argtypes = [ci, P_d]
         L.gi_kernel_stats.argtypes = [vp, ci, C.POINTER(C.c_char_p), P_d, P_i, P_i]
         L.gi_work_stats.argtypes = [vp, P_d, ci]
+        L.gi_partition_function.argtypes = [vp, cd, P_d]
+        L.gi_serve_start.argtypes = [vp, cll]
+        L.gi_serve_submit.argtypes = [vp, C.POINTER(GiRequest), C.POINTER(cll)]
+        L.gi_serve_wait.argtypes = [vp, cll, cd, C.POINTER(cll), P_d]
+        L.gi_serve_fetch.argtypes = [vp, cll, C.POINTER(GiLines)]
+        L.gi_serve_stats.argtypes = [vp, C.POINTER(cll), C.POINTER(cll), C.POINTER(cll)]
+        L.gi_serve_stop.argtypes = [vp]
         _LIB = L
     return _LIB
 
@@ -222,24 +239,86 @@ class Engine:
                        amp=np.empty(cap * 4))
         return out
 
+    @staticmethod
+    def _lines(bufs):
+        return GiLines(capacity=len(bufs["S"]), n=0, ilevelI=_p(bufs["ilevelI"], P_i),
+                       ilevelF=_p(bufs["ilevelF"], P_i), nu=_p(bufs["nu"], P_d),
+                       S=_p(bufs["S"], P_d), A_einst=_p(bufs["A_einst"], P_d),
+                       I=_p(bufs["I"], P_d), amp=_p(bufs["amp"], P_d))
+
+    @staticmethod
+    def _result(bufs, n):
+        res = {k: bufs[k][:n] for k in ("ilevelI", "ilevelF", "nu", "S", "A_einst", "I")}
+        res["amp"] = bufs["amp"][:4 * n].reshape(n, 4)
+        return res
+
     def compute(self, lower_ids=None, mode=GI_MODE_FP64, bufs=None):
         cap = self.count_candidates(lower_ids)
         if bufs is None or len(bufs["S"]) < max(cap, 1):
             bufs = self.alloc_lines(cap)
-        s = GiLines(capacity=len(bufs["S"]), n=0, ilevelI=_p(bufs["ilevelI"], P_i),
-                    ilevelF=_p(bufs["ilevelF"], P_i), nu=_p(bufs["nu"], P_d),
-                    S=_p(bufs["S"], P_d), A_einst=_p(bufs["A_einst"], P_d), I=_p(bufs["I"], P_d),
-                    amp=_p(bufs["amp"], P_d))
+        s = self._lines(bufs)
         if lower_ids is None:
             rc = lib().gi_compute(self.h, mode, 0, None, C.byref(s))
         else:
             ids = np.ascontiguousarray(lower_ids, np.int32)
             rc = lib().gi_compute(self.h, mode, len(ids), _p(ids, P_i), C.byref(s))
         self._chk(rc, "gi_compute")
-        n = s.n
-        res = {k: bufs[k][:n] for k in ("ilevelI", "ilevelF", "nu", "S", "A_einst", "I")}
-        res["amp"] = bufs["amp"][:4 * n].reshape(n, 4)
-        return res
+        return self._result(bufs, s.n)
+
+    def partition_function(self, temperature):
+        q = cd()
+        self._chk(lib().gi_partition_function(self.h, temperature, C.byref(q)),
+                  "gi_partition_function")
+        return q.value
+
+    # -- request server ----------------------------------------------------
+    def serve_start(self, budget_bytes):
+        self._chk(lib().gi_serve_start(self.h, int(budget_bytes)), "gi_serve_start")
+
+    def serve_submit(self, jindI, igammaI, jindF, mode=GI_MODE_FP64, temperature=None,
+                     q_stat=None, elow=(-1e300, 1e300), nu=None, thresh_linestrength=None,
+                     thresh_intensity=None):
+        """Queue one request; unset fields take the loaded job's values
+        (q_stat <= 0 asks the library for Q(T)). Returns the ticket."""
+        j = self.job
+        r = GiRequest(jindI=jindI, igammaI=igammaI, jindF=jindF, mode=mode,
+                      temperature=j.temperature if temperature is None else temperature,
+                      q_stat=j.q_stat if q_stat is None else q_stat,
+                      elow_lo=elow[0], elow_hi=elow[1],
+                      nu_lo=(j.freq_window if nu is None else nu)[0],
+                      nu_hi=(j.freq_window if nu is None else nu)[1],
+                      thresh_linestrength=(j.thresh_linestrength if thresh_linestrength is None
+                                           else thresh_linestrength),
+                      thresh_intensity=(j.thresh_intensity if thresh_intensity is None
+                                        else thresh_intensity))
+        t = cll()
+        self._chk(lib().gi_serve_submit(self.h, C.byref(r), C.byref(t)), "gi_serve_submit")
+        return t.value
+
+    def serve_wait(self, ticket, timeout_s=0.0):
+        """(nlines, latency_ms) of a finished request; GIError with status
+        GI_ETIMEOUT if it is still running after timeout_s."""
+        n, lat = cll(), cd()
+        self._chk(lib().gi_serve_wait(self.h, ticket, timeout_s, C.byref(n), C.byref(lat)),
+                  "gi_serve_wait")
+        return n.value, lat.value
+
+    def serve_fetch(self, ticket, bufs=None):
+        n, _ = self.serve_wait(ticket)
+        if bufs is None:
+            bufs = self.alloc_lines(n)
+        s = self._lines(bufs)
+        self._chk(lib().gi_serve_fetch(self.h, ticket, C.byref(s)), "gi_serve_fetch")
+        return self._result(bufs, s.n)
+
+    def serve_stats(self):
+        b, s, m = cll(), cll(), cll()
+        self._chk(lib().gi_serve_stats(self.h, C.byref(b), C.byref(s), C.byref(m)),
+                  "gi_serve_stats")
+        return dict(batches=b.value, served=s.value, max_batch=m.value)
+
+    def serve_stop(self):
+        self._chk(lib().gi_serve_stop(self.h), "gi_serve_stop")
 
     def half_ls(self, jindI, jindF, g, ideg, vec):
         v = np.ascontiguousarray(vec, np.float64)

@@ -15,10 +15,22 @@
 
 namespace gi {
 
+thread_local std::string *err_sink = nullptr;
+
 int fail(gi_ctx *c, int code, const std::string &msg)
 {
-    if (c) c->err = msg;
+    if (err_sink)
+        *err_sink = msg;
+    else if (c)
+        c->err = msg;
     return code;
+}
+
+// Calls that change or read the plan are refused while the request server's
+// worker uses it.
+static int serving(gi_ctx *c)
+{
+    return c->server ? fail(c, GI_ESTATE, "call not allowed while the request server runs") : 0;
 }
 
 #define HIPCHK(call)                                                                   \
@@ -29,12 +41,16 @@ int fail(gi_ctx *c, int code, const std::string &msg)
                         std::string(#call) + ": " + hipGetErrorString(e_));           \
     } while (0)
 
-void *scratch(gi_ctx *c, const char *name, size_t bytes, int *rc)
+void *slot_scratch(gi_ctx *c, Slot &s, const char *name, size_t bytes, int *rc)
 {
     *rc = GI_OK;
-    auto &e = c->scratch[name];
+    auto &e = s.buf[name];
     if (e.second >= bytes && e.first) return e.first;
-    if (e.first) (void)hipFree(e.first);
+    if (e.first) {
+        // the slot's stream may still read the old buffer
+        (void)hipStreamSynchronize(s.stream);
+        (void)hipFree(e.first);
+    }
     e.first = nullptr;
     e.second = 0;
     size_t want = std::max<size_t>(bytes, 256);
@@ -48,6 +64,42 @@ void *scratch(gi_ctx *c, const char *name, size_t bytes, int *rc)
     e.first = p;
     e.second = want;
     return p;
+}
+
+void *scratch(gi_ctx *c, const char *name, size_t bytes, int *rc)
+{
+    return slot_scratch(c, c->main_slot, name, bytes, rc);
+}
+
+void *slot_pinned(gi_ctx *c, Slot &s, size_t bytes, int *rc)
+{
+    *rc = GI_OK;
+    if (s.pinned_bytes >= bytes && s.pinned) return s.pinned;
+    if (s.pinned) {
+        (void)hipStreamSynchronize(s.stream);
+        (void)hipHostFree(s.pinned);
+    }
+    s.pinned = nullptr;
+    s.pinned_bytes = 0;
+    const size_t want = std::max<size_t>(bytes, 4096);
+    if (hipHostMalloc(&s.pinned, want, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        s.pinned = nullptr;
+        *rc = fail(c, GI_ENOMEM, "pinned host allocation failed");
+        return nullptr;
+    }
+    s.pinned_bytes = want;
+    return s.pinned;
+}
+
+void free_slot(Slot &s)
+{
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
+    for (auto &e : s.buf) (void)hipFree(e.second.first);
+    s.buf.clear();
+    if (s.pinned) (void)hipHostFree(s.pinned);
+    s.pinned = nullptr;
+    s.pinned_bytes = 0;
 }
 
 template <typename T>
@@ -102,6 +154,9 @@ static void free_plan(gi_ctx *c)
     c->groups.clear();
     (void)hipFree(c->d_mu_perm);
     c->d_mu_perm = nullptr;
+    (void)hipFree(c->d_lv_energy);
+    (void)hipFree(c->d_lv_weight);
+    c->d_lv_energy = c->d_lv_weight = nullptr;
     c->prepared = false;
 }
 
@@ -124,6 +179,7 @@ int gi_create(int device, gi_ctx **out)
         delete c;
         return GI_EHIP;
     }
+    c->main_slot.stream = c->stream;
     *out = c;
     return GI_OK;
 }
@@ -132,12 +188,13 @@ void gi_destroy(gi_ctx *c)
 {
     if (!c) return;
     (void)hipSetDevice(c->device);
+    server_shutdown(c);
     (void)hipStreamSynchronize(c->stream);
     free_plan(c);
     for (auto &b : c->blocks)
         if (b.owned) (void)hipFree((void *)b.d_vec);
     if (c->mu_owned) (void)hipFree(c->d_mu);
-    for (auto &s : c->scratch) (void)hipFree(s.second.first);
+    free_slot(c->main_slot);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -147,6 +204,7 @@ const char *gi_last_error(const gi_ctx *c) { return c ? c->err.c_str() : "null c
 int gi_set_params(gi_ctx *c, const gi_params *p)
 {
     if (!c || !p) return GI_EINVAL;
+    if (serving(c)) return GI_ESTATE;
     if (p->nrepres <= 0 || p->nrepres > 16 || !p->degen || !p->gns || !p->isym_pairs)
         return fail(c, GI_EINVAL, "gi_set_params: bad symmetry description");
     if (p->nmodes < 0 || (p->nmodes > 0 && (!p->quanta_lower || !p->quanta_upper)))
@@ -205,6 +263,7 @@ int gi_set_params(gi_ctx *c, const gi_params *p)
 int gi_upload_bset(gi_ctx *c, int jind, const gi_bset *b)
 {
     if (!c || !b || jind < 0 || jind > 1) return GI_EINVAL;
+    if (serving(c)) return GI_ESTATE;
     if (!c->have_params) return fail(c, GI_ESTATE, "gi_upload_bset before gi_set_params");
     if (b->jval != c->jvals[jind])
         return fail(c, GI_EINVAL, "gi_upload_bset: jval differs from jvals[jind]");
@@ -236,6 +295,7 @@ int gi_upload_bset(gi_ctx *c, int jind, const gi_bset *b)
 int gi_upload_dipole(gi_ctx *c, int nvib, const double *mu, int on_device)
 {
     if (!c || nvib <= 0 || !mu) return GI_EINVAL;
+    if (serving(c)) return GI_ESTATE;
     HIPCHK(hipSetDevice(c->device));
     free_plan(c);
     if (c->mu_owned) (void)hipFree(c->d_mu);
@@ -259,6 +319,7 @@ int gi_set_levels(gi_ctx *c, int n, const double *energy, const int *jind, const
     if (!c || n < 0 || (n > 0 && (!energy || !jind || !igamma || !ndeg || !irec0 || !quanta)))
         return GI_EINVAL;
     if (!c->have_params) return fail(c, GI_ESTATE, "gi_set_levels before gi_set_params");
+    if (serving(c)) return GI_ESTATE;
     free_plan(c);
     for (int i = 0; i < n; i++) {
         if (jind[i] < 0 || jind[i] > 1 || igamma[i] < 0 || igamma[i] >= c->nrep)
@@ -282,6 +343,7 @@ int gi_upload_vectors(gi_ctx *c, int jind, int igamma, int nsize, int nrec, cons
                       int on_device)
 {
     if (!c || jind < 0 || jind > 1 || nsize < 0 || nrec < 0) return GI_EINVAL;
+    if (serving(c)) return GI_ESTATE;
     if (!c->have_params) return fail(c, GI_ESTATE, "gi_upload_vectors before gi_set_params");
     if (igamma < 0 || igamma >= c->nrep) return GI_EINVAL;
     HIPCHK(hipSetDevice(c->device));
@@ -310,6 +372,7 @@ int gi_upload_vectors(gi_ctx *c, int jind, int igamma, int nsize, int nrec, cons
 int gi_prepare(gi_ctx *c)
 {
     if (!c) return GI_EINVAL;
+    if (serving(c)) return GI_ESTATE;
     if (!c->have_params || !c->have_bset[0] || !c->have_bset[1] || !c->d_mu)
         return fail(c, GI_ESTATE, "gi_prepare: parameters, both bases and the dipole are needed");
     HIPCHK(hipSetDevice(c->device));
@@ -412,6 +475,14 @@ int gi_prepare(gi_ctx *c)
         if ((rc = upload(c, &B.d_col, col)) != GI_OK) return rc;
         if ((rc = upload(c, &B.d_en, B.en)) != GI_OK) return rc;
         if ((rc = upload(c, &B.d_upok, B.upok)) != GI_OK) return rc;
+    }
+    // partition-function inputs: level energies and weights gns * ndeg * (2J + 1)
+    {
+        std::vector<double> w(c->nlevels);
+        for (int i = 0; i < c->nlevels; i++)
+            w[i] = c->gns[c->lv_igamma[i]] * c->lv_ndeg[i] * (2.0 * c->jvals[c->lv_jind[i]] + 1.0);
+        if ((rc = upload(c, &c->d_lv_energy, c->lv_energy)) != GI_OK) return rc;
+        if ((rc = upload(c, &c->d_lv_weight, w)) != GI_OK) return rc;
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     c->prepared = true;

@@ -1,28 +1,37 @@
-// The hot-path driver: the lower-level loop of cuda_host.cu:1174-1488 as
-// batched launches over every requested lower level.
+// The hot-path driver (the lower-level loop of cuda_host.cu:1174-1488 as
+// batched launches) and the request server built on it.
 //
-// Work is split into items (lower level, J'): the upper levels an item
-// correlates with are a contiguous range of the energy-sorted upper block
-// (frequency window). Items sharing (J, symmetry, J') form groups, cut into
-// chunks of at most kChunk items. Per chunk: the symmetry-blocked half
-// linestrength (G build + product per output class), the fold into the upper
-// block's symmetrised space, then the correlation and intensity.
-//   GI_MODE_FP64: one GEMV-form product per unit (k_gemv_sym) and one
-//     wavefront per candidate for the correlation (k_corr_intensity).
-//   GI_MODE_FP64_BATCHED: the products of a whole chunk on the matrix cores
-//     (k_halfls_mfma), the folded vectors of every chunk kept resident, one
-//     banded MFMA product for all correlations (k_corr_mfma), then
-//     k_intensity.
-// Both end with a stable compaction of the emitted candidates and one copy
-// of the lines to the caller's host arrays.
+// Work is described by queries. A query carries the temperature, partition
+// function and thresholds of one job (gi_compute) or one request
+// (gi_serve_*), and owns a contiguous range of candidate slots. Its work is
+// split into items (lower level, J'): the upper levels an item correlates
+// with are a contiguous range of the energy-sorted upper block (frequency
+// window). Items sharing (query, J, symmetry, J') form groups, cut into
+// chunks of at most kChunk items, so no chunk ever mixes two queries and a
+// query's results do not depend on what it is batched with.
+//   GI_MODE_FP64: per chunk, the symmetry-blocked half linestrength in GEMV
+//     form (k_gemv_sym), the fold, and one wavefront per candidate for the
+//     correlation and intensity (k_corr_intensity).
+//   GI_MODE_FP64_BATCHED: per round of chunks, one launch builds every G, one
+//     launch runs every (chunk, class) product on the matrix cores
+//     (k_halfls_jobs), folds per chunk, one banded matrix-core launch for
+//     every correlation (k_corr_mfma), then k_intensity per chunk.
+// Each query's candidates are compacted on their own, so its lines come back
+// contiguous.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -39,11 +48,12 @@ namespace gi {
                         std::string(#call) + ": " + hipGetErrorString(e_));           \
     } while (0)
 
-// Timed launches: events recorded on the context's stream around each launch.
+// Timed launches: events recorded on a stream around each launch (only for
+// gi_compute; the server runs untimed).
 struct Timeline {
-    gi_ctx *c;
+    hipStream_t st;
     std::vector<std::tuple<std::string, hipEvent_t, hipEvent_t>> recs;
-    explicit Timeline(gi_ctx *c_) : c(c_) {}
+    explicit Timeline(hipStream_t s) : st(s) {}
     int begin(const char *name)
     {
         hipEvent_t a, b;
@@ -53,13 +63,10 @@ struct Timeline {
             return GI_EHIP;
         }
         recs.emplace_back(name, a, b);
-        return hipEventRecord(a, c->stream) == hipSuccess ? GI_OK : GI_EHIP;
+        return hipEventRecord(a, st) == hipSuccess ? GI_OK : GI_EHIP;
     }
-    int end()
-    {
-        return hipEventRecord(std::get<2>(recs.back()), c->stream) == hipSuccess ? GI_OK : GI_EHIP;
-    }
-    void collect()
+    int end() { return hipEventRecord(std::get<2>(recs.back()), st) == hipSuccess ? GI_OK : GI_EHIP; }
+    void collect(gi_ctx *c)
     {
         c->kstats.clear();
         c->kstat_names.clear();
@@ -81,13 +88,13 @@ struct Timeline {
     }
 };
 
-#define TL_BEGIN(name)                                                         \
-    do {                                                                       \
-        if (tl.begin(name) != GI_OK) return fail(c, GI_EHIP, "hipEventRecord"); \
+#define TL_BEGIN(name)                                                                    \
+    do {                                                                                  \
+        if (tl && tl->begin(name) != GI_OK) return fail(c, GI_EHIP, "hipEventRecord");    \
     } while (0)
-#define TL_END()                                                               \
-    do {                                                                       \
-        if (tl.end() != GI_OK) return fail(c, GI_EHIP, "hipEventRecord");      \
+#define TL_END()                                                                          \
+    do {                                                                                  \
+        if (tl && tl->end() != GI_OK) return fail(c, GI_EHIP, "hipEventRecord");          \
     } while (0)
 
 // trove_functions.cpp:1495-1510
@@ -99,24 +106,79 @@ static bool degeneracy_filter(const gi_ctx *c, int gI, int gF, int idegI, int id
     return true;
 }
 
+// Constants of cuda_host.cu:12-18, with the reference's expression shapes.
+struct Consts {
+    double A_coef_s_1, intens_cm_mol, c2;  // c2 = planck*vellgt/boltz (beta = c2/T)
+    Consts()
+    {
+        const double pi = 4.0 * std::atan2(1.0, 1.0);
+        const double planck = 6.62606896 * std::pow(10.0, -27.0);
+        const double avogno = 6.0221415 * std::pow(10.0, 23.0);
+        const double vellgt = 2.99792458 * std::pow(10.0, 10.0);
+        const double boltz = 1.380658 * std::pow(10.0, -16.0);
+        A_coef_s_1 = 64.0 * std::pow(10.0, -36.0) * std::pow(pi, 4.0) /
+                     (3.0 * 6.62606896 * std::pow(10.0, -27.0));
+        intens_cm_mol = 8.0 * std::pow(10.0, -36.0) * std::pow(pi, 3.0) * avogno /
+                        (3.0 * planck * vellgt);
+        c2 = planck * vellgt / boltz;
+    }
+    // beta = planck * vellgt / (boltz * T), cuda_host.cu:1170
+    static double beta(double T)
+    {
+        const double planck = 6.62606896 * std::pow(10.0, -27.0);
+        const double vellgt = 2.99792458 * std::pow(10.0, 10.0);
+        const double boltz = 1.380658 * std::pow(10.0, -16.0);
+        return planck * vellgt / (boltz * T);
+    }
+};
+
+struct Query {
+    double beta, q_stat, zpe, thS, thI;
+    long long cand_begin = 0, cand_end = 0;
+};
+
 struct Item {
-    int level, jindI, gI, jindF, gF, lo, hi;
+    int level, jindI, gI, jindF, gF, lo, hi, query;
     long long out_off;
 };
 
-// One item per (lower level, J') whose window in the upper block is not
-// empty. The lower-level filters (cuda_host.cu:1194) and the J / symmetry
-// part of intensity_filter (trove_functions.cpp:1452-1464) are decided here;
-// the frequency window (nu = E' - E in [lo, hi]) is a contiguous range of the
-// energy-sorted upper block, found with the same comparisons as the filter.
-static void make_items(const gi_ctx *c, int nlower, const int *ids, std::vector<Item> &items,
-                       long long &ncand)
+struct Chunk {
+    int query, jindI, gI, jindF, gF, nitems, nunits, ncand, cb_mask, Lo, Hi;
+    size_t off_uptr, off_uideg, off_iunit, off_cstart, off_lo, off_hi, off_level, off_eI, off_out;
+};
+
+// Everything one launch sequence needs; lives until its work has finished.
+struct Plan {
+    int mode = GI_MODE_FP64;
+    std::vector<Query> q;
+    std::vector<Item> items;
+    long long ncand = 0;
+    std::vector<Chunk> chunks;
+    std::vector<unsigned char> meta;
+    std::vector<std::vector<unsigned char>> keep;  // host sources of async copies
+    LineSoA comp{};                                // compacted lines (slot buffers)
+    long long *counts = nullptr;                   // pinned, lines per query
+    double work[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+// Window [a, b) of nu = E' - E in [w0, w1] in an energy-sorted block, found
+// with the comparisons of intensity_filter (trove_functions.cpp:1453-1454).
+static void window(const std::vector<double> &en, double eI, double w0, double w1, int &a, int &b)
 {
-    items.clear();
-    ncand = 0;
+    a = (int)(std::partition_point(en.begin(), en.end(),
+                                   [&](double e) { return !(e - eI >= w0); }) -
+              en.begin());
+    b = (int)(std::partition_point(en.begin(), en.end(), [&](double e) { return e - eI <= w1; }) -
+              en.begin());
+}
+
+// Items of the deck job: the lower-level filters (cuda_host.cu:1194) and the
+// J / symmetry part of intensity_filter (trove_functions.cpp:1452-1464).
+static void items_job(const gi_ctx *c, int nlower, const int *ids, int query, Plan &P)
+{
     const int n = ids ? nlower : c->nlevels;
-    for (int q = 0; q < n; q++) {
-        const int L = ids ? ids[q] : q;
+    for (int k = 0; k < n; k++) {
+        const int L = ids ? ids[k] : k;
         if (L < 0 || L >= c->nlevels) continue;
         if (!c->lv_lowok[L]) continue;
         const int gI = c->lv_igamma[L];
@@ -130,38 +192,149 @@ static void make_items(const gi_ctx *c, int nlower, const int *ids, std::vector<
             const int jF = c->jvals[jindF];
             if (!((jF != c->jvals[0]) || (jI != c->jvals[0]))) continue;
             if (std::abs(jF - jI) > 1 || jI + jF < 1) continue;
-            const auto &en = c->blocks[jindF * c->nrep + gF].en;
-            const double w0 = c->freq_window[0], w1 = c->freq_window[1];
-            const int a = (int)(std::partition_point(en.begin(), en.end(),
-                                                     [&](double e) { return !(e - eI >= w0); }) -
-                                en.begin());
-            const int b = (int)(std::partition_point(en.begin(), en.end(),
-                                                     [&](double e) { return e - eI <= w1; }) -
-                                en.begin());
+            int a, b;
+            window(c->blocks[jindF * c->nrep + gF].en, eI, c->freq_window[0], c->freq_window[1],
+                   a, b);
             if (b <= a) continue;
-            items.push_back(Item{L, jindI, gI, jindF, gF, a, b, ncand});
-            ncand += b - a;
+            P.items.push_back(Item{L, jindI, gI, jindF, gF, a, b, query, P.ncand});
+            P.ncand += b - a;
         }
     }
 }
 
-struct Chunk {
-    int jindI, gI, jindF, gF, nitems, nunits, ncand, cb_mask, Lo, Hi;
-    size_t off_uptr, off_uideg, off_iunit, off_cstart, off_lo, off_hi, off_level, off_eI, off_out;
-    size_t ht_off;  // doubles, batched mode
-};
+// Items of one request: every lower level of (jindI, gI) that passes the
+// job's lower-level filters and has E - ZPE in [elow_lo, elow_hi], each with
+// the window nu in [nu_lo, nu_hi] of the upper block (jindF, igamma_pair[gI]).
+static void items_request(const gi_ctx *c, const gi_request &r, int query, Plan &P)
+{
+    const int gF = c->igamma_pair[r.igammaI];
+    const int jI = c->jvals[r.jindI], jF = c->jvals[r.jindF];
+    if (!(c->gns[r.igammaI] > 0.0) || c->isym_pairs[r.igammaI] != c->isym_pairs[gF]) return;
+    if (!((jF != c->jvals[0]) || (jI != c->jvals[0]))) return;
+    if (std::abs(jF - jI) > 1 || jI + jF < 1) return;
+    const Block &BI = c->blocks[r.jindI * c->nrep + r.igammaI];
+    const auto &enF = c->blocks[r.jindF * c->nrep + gF].en;
+    for (size_t i = 0; i < BI.lev.size(); i++) {
+        const double eI = BI.en[i];
+        if (!c->lv_lowok[BI.lev[i]]) continue;
+        if (!((eI - c->zpe) >= r.elow_lo && (eI - c->zpe) <= r.elow_hi)) continue;
+        int a, b;
+        window(enF, eI, r.nu_lo, r.nu_hi, a, b);
+        if (b <= a) continue;
+        P.items.push_back(Item{BI.lev[i], r.jindI, r.igammaI, r.jindF, gF, a, b, query, P.ncand});
+        P.ncand += b - a;
+    }
+}
 
-// Batched FP64 mode. Chunks are grouped into rounds whose G, H and folded
-// vectors fit a memory budget; per round one launch builds every G, one
-// launch runs every (chunk, class) product on the matrix cores, the folds run
-// per chunk, one launch runs every banded correlation, and the intensity step
-// runs per chunk.
-static int run_batched(gi_ctx *c, const std::vector<Chunk> &chunks, unsigned char *d_meta,
-                       Timeline &tl, double beta, double A_coef_s_1, double intens_cm_mol,
-                       int *flag, LineSoA cand, double &useful, double &issued,
-                       double &corr_bytes, double &mu_bytes, double &mu_dense, double &corr_flops)
+// Group items and cut chunks; unit pointers become device addresses.
+static int build_chunks(gi_ctx *c, Plan &P)
+{
+    const bool batched = P.mode == GI_MODE_FP64_BATCHED;
+    const int kChunk = 256;
+    std::map<std::tuple<int, int, int, int>, std::vector<int>> groups;
+    for (int i = 0; i < (int)P.items.size(); i++) {
+        const Item &it = P.items[i];
+        groups[std::make_tuple(it.query, it.jindI, it.gI, it.jindF)].push_back(i);
+    }
+    auto put = [&](const void *src, size_t bytes) {
+        const size_t o = (P.meta.size() + 255) & ~size_t(255);
+        P.meta.resize(o + std::max<size_t>(bytes, 8));
+        if (bytes) std::memcpy(P.meta.data() + o, src, bytes);
+        return o;
+    };
+    for (auto &kv : groups) {
+        const int query = std::get<0>(kv.first), jindI = std::get<1>(kv.first),
+                  gI = std::get<2>(kv.first), jindF = std::get<3>(kv.first);
+        const int gF = c->igamma_pair[gI];
+        const int ndegI = c->degen[gI], ndegF = c->degen[gF];
+        const Block &BI = c->blocks[jindI * c->nrep + gI];
+        std::vector<int> its = kv.second;
+        if (batched)  // energy order keeps every chunk's windows a narrow band
+            std::stable_sort(its.begin(), its.end(), [&](int x, int y) {
+                return c->lv_energy[P.items[x].level] < c->lv_energy[P.items[y].level];
+            });
+        const GroupPlan &GP = c->groups[((size_t)jindI * c->nrep + gI) * 2 + jindF];
+        if (!GP.valid) return fail(c, GI_EINVAL, "internal: no plan for a (J, J') pair");
+        for (size_t s0 = 0; s0 < its.size(); s0 += kChunk) {
+            const int n_it = (int)std::min<size_t>(kChunk, its.size() - s0);
+            std::vector<long long> uptr, outoff(n_it);
+            std::vector<int> uideg, iunit(n_it * GI_MAXDEG, -1), cstart(n_it), lo(n_it), hi(n_it),
+                lev(n_it);
+            std::vector<double> eI(n_it);
+            int cnt = 0, Lo = 1 << 30, Hi = 0;
+            for (int k = 0; k < n_it; k++) {
+                const Item &it = P.items[its[s0 + k]];
+                for (int ideg = 0; ideg < ndegI; ideg++) {
+                    if (!degeneracy_filter(c, gI, gF, ideg, 0)) continue;
+                    iunit[k * GI_MAXDEG + ideg] = (int)uideg.size();
+                    uideg.push_back(ideg);
+                    uptr.push_back((long long)(BI.d_vec + (size_t)c->lv_irec0[it.level] * BI.nsize));
+                }
+                cstart[k] = cnt;
+                lo[k] = it.lo;
+                hi[k] = it.hi;
+                Lo = std::min(Lo, it.lo);
+                Hi = std::max(Hi, it.hi);
+                lev[k] = it.level;
+                eI[k] = c->lv_energy[it.level];
+                outoff[k] = it.out_off;
+                cnt += it.hi - it.lo;
+            }
+            int mask = 0;
+            for (int idegF = 0; idegF < ndegF; idegF++)
+                for (int idegI = 0; idegI < ndegI; idegI++) {
+                    if (c->reduced && idegF != 0) continue;
+                    if (!degeneracy_filter(c, gI, gF, idegI, idegF)) continue;
+                    if (!degeneracy_filter(c, gI, gF, idegI, 0)) continue;  // unit exists
+                    mask |= 1 << (idegF * ndegI + idegI);
+                }
+            Chunk ch;
+            ch.query = query;
+            ch.jindI = jindI; ch.gI = gI; ch.jindF = jindF; ch.gF = gF;
+            ch.nitems = n_it; ch.nunits = (int)uideg.size(); ch.ncand = cnt; ch.cb_mask = mask;
+            ch.Lo = Lo; ch.Hi = Hi;
+            ch.off_uptr = put(uptr.data(), uptr.size() * 8);
+            ch.off_uideg = put(uideg.data(), uideg.size() * 4);
+            ch.off_iunit = put(iunit.data(), iunit.size() * 4);
+            ch.off_cstart = put(cstart.data(), cstart.size() * 4);
+            ch.off_lo = put(lo.data(), lo.size() * 4);
+            ch.off_hi = put(hi.data(), hi.size() * 4);
+            ch.off_level = put(lev.data(), lev.size() * 4);
+            ch.off_eI = put(eI.data(), eI.size() * 8);
+            ch.off_out = put(outoff.data(), outoff.size() * 8);
+            P.chunks.push_back(ch);
+        }
+    }
+    return GI_OK;
+}
+
+static void fill_corr_args(gi_ctx *c, const Plan &P, const Chunk &ch, unsigned char *d_meta,
+                           const Consts &K, CorrArgs &a)
+{
+    const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
+    const Query &q = P.q[ch.query];
+    a.ncand = ch.ncand; a.nitems = ch.nitems; a.nsizeF = BF.nsize;
+    a.ndegI = c->degen[ch.gI]; a.ndegF = c->degen[ch.gF]; a.cb_mask = ch.cb_mask;
+    a.reduced = c->reduced;
+    a.jI = c->jvals[ch.jindI]; a.jF = c->jvals[ch.jindF];
+    a.item_cand_start = (const int *)(d_meta + ch.off_cstart);
+    a.item_lo = (const int *)(d_meta + ch.off_lo);
+    a.item_level = (const int *)(d_meta + ch.off_level);
+    a.item_out_off = (const long long *)(d_meta + ch.off_out);
+    a.item_energy = (const double *)(d_meta + ch.off_eI);
+    a.blk_col = BF.d_col; a.blk_lev = BF.d_lev; a.blk_en = BF.d_en; a.blk_upok = BF.d_upok;
+    a.vec = BF.d_vec;
+    a.gnsI = c->gns[ch.gI]; a.zpe = q.zpe; a.beta = q.beta; a.q_stat = q.q_stat;
+    a.A_coef_s_1 = K.A_coef_s_1; a.intens_cm_mol = K.intens_cm_mol;
+    a.thresh_S = q.thS; a.thresh_I = q.thI;
+}
+
+// Batched FP64 mode over rounds of chunks (see the file comment).
+static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timeline *tl,
+                       int *flag, LineSoA cand)
 {
     const int N = c->nvib;
+    const Consts K;
     int rc = GI_OK;
     // device memory for one round's G, H and folded vectors (GI_BATCH_BYTES overrides)
     size_t kRoundBytes = (size_t)12 << 30;
@@ -169,6 +342,7 @@ static int run_batched(gi_ctx *c, const std::vector<Chunk> &chunks, unsigned cha
         const long long v = std::atoll(env);
         if (v > 0) kRoundBytes = (size_t)v;
     }
+    const auto &chunks = P.chunks;
     struct Need {
         size_t g, h, ht;
         int njobs;
@@ -188,7 +362,7 @@ static int run_batched(gi_ctx *c, const std::vector<Chunk> &chunks, unsigned cha
     for (size_t i = 0; i <= chunks.size(); i++) {
         const size_t bytes = i < chunks.size() ? 8 * (need[i].g + need[i].h + need[i].ht) : 0;
         if (i == chunks.size() || (i > start && acc + bytes > kRoundBytes)) {
-            rounds.push_back({start, i});
+            if (i > start) rounds.push_back({start, i});
             mg = std::max(mg, sg); mh = std::max(mh, sh); mht = std::max(mht, sht);
             mjobs = std::max(mjobs, sjobs);
             mchunks = std::max(mchunks, (int)(i - start));
@@ -202,15 +376,16 @@ static int run_batched(gi_ctx *c, const std::vector<Chunk> &chunks, unsigned cha
         sjobs += need[i].njobs;
     }
     // every buffer is sized for the largest round before anything is launched
-    double *G = (double *)scratch(c, "G", mg * 8, &rc); if (rc) return rc;
-    double *H = (double *)scratch(c, "H", mh * 8, &rc); if (rc) return rc;
-    double *Ht = (double *)scratch(c, "Ht", mht * 8, &rc); if (rc) return rc;
+    double *G = (double *)slot_scratch(c, S, "G", mg * 8, &rc); if (rc) return rc;
+    double *H = (double *)slot_scratch(c, S, "H", mh * 8, &rc); if (rc) return rc;
+    double *Ht = (double *)slot_scratch(c, S, "Ht", mht * 8, &rc); if (rc) return rc;
     const size_t jobs_bytes = (size_t)mjobs * sizeof(HalfJob);
-    unsigned char *d_jobs = (unsigned char *)scratch(c, "jobs", jobs_bytes + 2 * 4 * (size_t)mjobs, &rc);
+    unsigned char *d_jobs =
+        (unsigned char *)slot_scratch(c, S, "jobs", jobs_bytes + 2 * 4 * (size_t)mjobs, &rc);
     if (rc) return rc;
-    CorrMMArgs *d_mm = (CorrMMArgs *)scratch(c, "mmargs", (size_t)mchunks * sizeof(CorrMMArgs), &rc);
+    CorrMMArgs *d_mm =
+        (CorrMMArgs *)slot_scratch(c, S, "mmargs", (size_t)mchunks * sizeof(CorrMMArgs), &rc);
     if (rc) return rc;
-    std::vector<std::vector<unsigned char>> keep;  // host sources of the async copies
 
     for (auto &rg : rounds) {
         std::vector<HalfJob> jobs;
@@ -251,20 +426,20 @@ static int run_batched(gi_ctx *c, const std::vector<Chunk> &chunks, unsigned cha
         }
         const int njobs = (int)jobs.size();
         if (njobs > 0) {
-            keep.emplace_back(jobs_bytes + 2 * 4 * (size_t)mjobs);
-            unsigned char *hb = keep.back().data();
+            P.keep.emplace_back(jobs_bytes + 2 * 4 * (size_t)mjobs);
+            unsigned char *hb = P.keep.back().data();
             std::memcpy(hb, jobs.data(), njobs * sizeof(HalfJob));
             std::memcpy(hb + jobs_bytes, gbegin.data(), njobs * 4);
             std::memcpy(hb + jobs_bytes + 4 * (size_t)mjobs, tbegin.data(), njobs * 4);
-            HIPCHK(hipMemcpyAsync(d_jobs, hb, keep.back().size(), hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(d_jobs, hb, P.keep.back().size(), hipMemcpyHostToDevice, S.stream));
             const int *d_gb = (const int *)(d_jobs + jobs_bytes);
             const int *d_tb = (const int *)(d_jobs + jobs_bytes + 4 * (size_t)mjobs);
             TL_BEGIN("build_g");
-            launch_halfls_jobs(c->stream, N, (const HalfJob *)d_jobs, d_gb, d_tb, njobs, gt, 0,
+            launch_halfls_jobs(S.stream, N, (const HalfJob *)d_jobs, d_gb, d_tb, njobs, gt, 0,
                                c->d_mu_perm);
             TL_END();
             TL_BEGIN("half_ls_mfma");
-            launch_halfls_jobs(c->stream, N, (const HalfJob *)d_jobs, d_gb, d_tb, njobs, 0, tt,
+            launch_halfls_jobs(S.stream, N, (const HalfJob *)d_jobs, d_gb, d_tb, njobs, 0, tt,
                                c->d_mu_perm);
             TL_END();
         }
@@ -282,7 +457,7 @@ static int run_batched(gi_ctx *c, const std::vector<Chunk> &chunks, unsigned cha
                 t1.tcol = GP.d_tcol[1];
             }
             TL_BEGIN("fold");
-            launch_fold(c->stream, ch.nitems, BF.nsize, ndegI, ndegF, GP.Htot,
+            launch_fold(S.stream, ch.nitems, BF.nsize, ndegI, ndegF, GP.Htot,
                         (const int *)(d_meta + ch.off_iunit), t0, t1, Hc[i - rg.first],
                         Htc[i - rg.first]);
             TL_END();
@@ -294,341 +469,637 @@ static int run_batched(gi_ctx *c, const std::vector<Chunk> &chunks, unsigned cha
             m.item_out_off = (const long long *)(d_meta + ch.off_out);
             m.blk_col = BF.d_col; m.vec = BF.d_vec; m.Ht = Htc[i - rg.first]; m.amp = cand.amp;
             mm.push_back(m);
-            corr_flops += 2.0 * (double)(ch.Hi - ch.Lo) * ch.nitems * ndegI * ndegF * BF.nsize;
             double blk_bytes = 0;
             for (const OutClass &oc : GP.oc) blk_bytes += 8.0 * oc.n * oc.K;
-            useful += ch.nunits * 2.0 * c->coef[ch.jindI][ch.jindF].C * (double)N * N;
-            issued += ch.nunits * 2.0 * GP.macs;
-            mu_bytes += blk_bytes;
-            mu_dense += ch.nunits * 24.0 * (double)N * N;
-            corr_bytes += (double)ch.ncand * 8.0 * BF.nsize;
+            P.work[3] += ch.nunits * 2.0 * c->coef[ch.jindI][ch.jindF].C * (double)N * N;
+            P.work[4] += ch.nunits * 2.0 * GP.macs;
+            P.work[5] += (double)ch.ncand * 8.0 * BF.nsize;
+            P.work[6] += blk_bytes;
+            P.work[7] += ch.nunits * 24.0 * (double)N * N;
+            P.work[8] += 2.0 * (double)(ch.Hi - ch.Lo) * ch.nitems * ndegI * ndegF * BF.nsize;
         }
         int max_rows = 0, max_cols = 0;
         for (auto &m : mm) {
             max_rows = std::max(max_rows, m.Hi - m.Lo);
             max_cols = std::max(max_cols, m.nitems * m.ncb);
         }
-        keep.emplace_back(mm.size() * sizeof(CorrMMArgs));
-        std::memcpy(keep.back().data(), mm.data(), keep.back().size());
-        HIPCHK(hipMemcpyAsync(d_mm, keep.back().data(), keep.back().size(), hipMemcpyHostToDevice,
-                              c->stream));
+        P.keep.emplace_back(mm.size() * sizeof(CorrMMArgs));
+        std::memcpy(P.keep.back().data(), mm.data(), P.keep.back().size());
+        HIPCHK(hipMemcpyAsync(d_mm, P.keep.back().data(), P.keep.back().size(),
+                              hipMemcpyHostToDevice, S.stream));
         TL_BEGIN("corr_mfma");
-        launch_corr_mfma(c->stream, d_mm, (int)mm.size(), max_rows, max_cols);
+        launch_corr_mfma(S.stream, d_mm, (int)mm.size(), max_rows, max_cols);
         TL_END();
         for (size_t i = rg.first; i < rg.second; i++) {
             const Chunk &ch = chunks[i];
-            const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
+            CorrArgs ca;
+            fill_corr_args(c, P, ch, d_meta, K, ca);
             IntensArgs a;
-            a.ncand = ch.ncand; a.nitems = ch.nitems;
-            a.ndegI = c->degen[ch.gI]; a.ndegF = c->degen[ch.gF]; a.cb_mask = ch.cb_mask;
-            a.reduced = c->reduced; a.jI = c->jvals[ch.jindI]; a.jF = c->jvals[ch.jindF];
-            a.item_cand_start = (const int *)(d_meta + ch.off_cstart);
-            a.item_lo = (const int *)(d_meta + ch.off_lo);
-            a.item_level = (const int *)(d_meta + ch.off_level);
-            a.item_out_off = (const long long *)(d_meta + ch.off_out);
-            a.item_energy = (const double *)(d_meta + ch.off_eI);
-            a.blk_lev = BF.d_lev; a.blk_en = BF.d_en; a.blk_upok = BF.d_upok;
-            a.gnsI = c->gns[ch.gI]; a.zpe = c->zpe; a.beta = beta; a.q_stat = c->q_stat;
-            a.A_coef_s_1 = A_coef_s_1; a.intens_cm_mol = intens_cm_mol;
-            a.thresh_S = c->thresh_S; a.thresh_I = c->thresh_I;
+            a.ncand = ca.ncand; a.nitems = ca.nitems; a.ndegI = ca.ndegI; a.ndegF = ca.ndegF;
+            a.cb_mask = ca.cb_mask; a.reduced = ca.reduced; a.jI = ca.jI; a.jF = ca.jF;
+            a.item_cand_start = ca.item_cand_start; a.item_lo = ca.item_lo;
+            a.item_level = ca.item_level; a.item_out_off = ca.item_out_off;
+            a.item_energy = ca.item_energy;
+            a.blk_lev = ca.blk_lev; a.blk_en = ca.blk_en; a.blk_upok = ca.blk_upok;
+            a.gnsI = ca.gnsI; a.zpe = ca.zpe; a.beta = ca.beta; a.q_stat = ca.q_stat;
+            a.A_coef_s_1 = ca.A_coef_s_1; a.intens_cm_mol = ca.intens_cm_mol;
+            a.thresh_S = ca.thresh_S; a.thresh_I = ca.thresh_I;
             a.amp = cand.amp; a.flag = flag; a.out_iI = cand.iI; a.out_iF = cand.iF;
             a.out_nu = cand.nu; a.out_S = cand.S; a.out_A = cand.A; a.out_I = cand.I;
             TL_BEGIN("intensity");
-            launch_intensity(c->stream, a);
+            launch_intensity(S.stream, a);
             TL_END();
         }
     }
-    // the async copies read `keep`; wait before it goes out of scope
-    HIPCHK(hipStreamSynchronize(c->stream));
+    return GI_OK;
+}
+
+static LineSoA shift(LineSoA s, long long o)
+{
+    s.iI += o; s.iF += o; s.nu += o; s.S += o; s.A += o; s.I += o; s.amp += 4 * o;
+    return s;
+}
+
+// Enqueue the whole plan on slot S: half linestrengths, correlations,
+// intensities, per-query compaction, and the copy of the per-query line
+// counts to pinned memory. Nothing here waits for the device.
+static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
+{
+    const int N = c->nvib;
+    const Consts K;
+    int rc = GI_OK;
+    HIPCHK(hipSetDevice(c->device));
+    const bool batched = P.mode == GI_MODE_FP64_BATCHED;
+    const size_t nc = (size_t)std::max<long long>(P.ncand, 1);
+    int *flag = (int *)slot_scratch(c, S, "flag", nc * sizeof(int), &rc); if (rc) return rc;
+    LineSoA cand, comp;
+    cand.iI = (int *)slot_scratch(c, S, "c_iI", nc * 4, &rc); if (rc) return rc;
+    cand.iF = (int *)slot_scratch(c, S, "c_iF", nc * 4, &rc); if (rc) return rc;
+    cand.nu = (double *)slot_scratch(c, S, "c_nu", nc * 8, &rc); if (rc) return rc;
+    cand.S = (double *)slot_scratch(c, S, "c_S", nc * 8, &rc); if (rc) return rc;
+    cand.A = (double *)slot_scratch(c, S, "c_A", nc * 8, &rc); if (rc) return rc;
+    cand.I = (double *)slot_scratch(c, S, "c_I", nc * 8, &rc); if (rc) return rc;
+    cand.amp = (double *)slot_scratch(c, S, "c_amp", nc * 32, &rc); if (rc) return rc;
+    comp.iI = (int *)slot_scratch(c, S, "o_iI", nc * 4, &rc); if (rc) return rc;
+    comp.iF = (int *)slot_scratch(c, S, "o_iF", nc * 4, &rc); if (rc) return rc;
+    comp.nu = (double *)slot_scratch(c, S, "o_nu", nc * 8, &rc); if (rc) return rc;
+    comp.S = (double *)slot_scratch(c, S, "o_S", nc * 8, &rc); if (rc) return rc;
+    comp.A = (double *)slot_scratch(c, S, "o_A", nc * 8, &rc); if (rc) return rc;
+    comp.I = (double *)slot_scratch(c, S, "o_I", nc * 8, &rc); if (rc) return rc;
+    comp.amp = (double *)slot_scratch(c, S, "o_amp", nc * 32, &rc); if (rc) return rc;
+    P.comp = comp;
+    // compaction scratch: per query, block counts and offsets
+    std::vector<size_t> cbase(P.q.size()), obase(P.q.size());
+    size_t ncnt = 0, noff = 0;
+    for (size_t k = 0; k < P.q.size(); k++) {
+        const int nb = compact_blocks(P.q[k].cand_end - P.q[k].cand_begin);
+        cbase[k] = ncnt;
+        obase[k] = noff;
+        ncnt += nb;
+        noff += nb + 1;
+    }
+    int *counts = (int *)slot_scratch(c, S, "counts", std::max<size_t>(ncnt, 1) * 4, &rc);
+    if (rc) return rc;
+    long long *offs = (long long *)slot_scratch(c, S, "offs", std::max<size_t>(noff, 1) * 8, &rc);
+    if (rc) return rc;
+    P.counts = (long long *)slot_pinned(c, S, P.q.size() * 8 + 64, &rc);
+    if (rc) return rc;
+    unsigned char *d_meta = nullptr;
+    if (!P.meta.empty()) {
+        d_meta = (unsigned char *)slot_scratch(c, S, "meta", P.meta.size(), &rc); if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(d_meta, P.meta.data(), P.meta.size(), hipMemcpyHostToDevice,
+                              S.stream));
+    }
+
+    if (batched && !P.chunks.empty()) {
+        if ((rc = run_batched(c, S, P, d_meta, tl, flag, cand)) != GI_OK) return rc;
+    } else if (!P.chunks.empty()) {
+        int max_units = 1, max_items = 1, max_G = 1, max_H = 1, max_ncb = 1, max_nsizeF = 1;
+        for (auto &ch : P.chunks) {
+            const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
+            max_units = std::max(max_units, ch.nunits);
+            max_items = std::max(max_items, ch.nitems);
+            max_G = std::max(max_G, GP.Gtot);
+            max_H = std::max(max_H, GP.Htot);
+            max_ncb = std::max(max_ncb, c->degen[ch.gI] * c->degen[ch.gF]);
+            max_nsizeF = std::max(max_nsizeF, c->blocks[ch.jindF * c->nrep + ch.gF].nsize);
+        }
+        double *G = (double *)slot_scratch(c, S, "G", (size_t)max_units * max_G * 8, &rc);
+        if (rc) return rc;
+        double *H = (double *)slot_scratch(c, S, "H", (size_t)max_units * max_H * 8, &rc);
+        if (rc) return rc;
+        double *Ht = (double *)slot_scratch(c, S, "Ht", (size_t)max_items * max_ncb * max_nsizeF * 8,
+                                            &rc);
+        if (rc) return rc;
+        for (auto &ch : P.chunks) {
+            const JPlan &pI = c->jp[ch.jindI], &pF = c->jp[ch.jindF];
+            const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
+            const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
+            const int ndegI = c->degen[ch.gI], ndegF = c->degen[ch.gF];
+            SymDev r0 = symdev(pI.sym[ch.gI * GI_MAXDEG + 0]);
+            SymDev r1 = ndegI > 1 ? symdev(pI.sym[ch.gI * GI_MAXDEG + 1]) : r0;
+            SymDev t0 = symdev(pF.sym[ch.gF * GI_MAXDEG + 0]);
+            t0.tcol = GP.d_tcol[0];
+            SymDev t1 = t0;
+            if (ndegF > 1) {
+                t1 = symdev(pF.sym[ch.gF * GI_MAXDEG + 1]);
+                t1.tcol = GP.d_tcol[1];
+            }
+            if (ch.nunits > 0)
+                for (const OutClass &oc : GP.oc) {
+                    const ClassDev cd = classdev(oc);
+                    if (cd.K > 0) {
+                        TL_BEGIN("build_g");
+                        launch_build_g_sym(S.stream, ch.nunits, N, GP.Gtot, cd,
+                                           (const double *const *)(d_meta + ch.off_uptr),
+                                           (const int *)(d_meta + ch.off_uideg), r0, r1, G);
+                        TL_END();
+                    }
+                    TL_BEGIN("half_ls_gemv");
+                    if (launch_gemv_sym(S.stream, ch.nunits, N, GP.Gtot, GP.Htot, cd,
+                                        c->d_mu_perm, G, H) != 0)
+                        return fail(c, GI_EINVAL, "no half-ls kernel for this column count");
+                    TL_END();
+                }
+            TL_BEGIN("fold");
+            launch_fold(S.stream, ch.nitems, BF.nsize, ndegI, ndegF, GP.Htot,
+                        (const int *)(d_meta + ch.off_iunit), t0, t1, H, Ht);
+            TL_END();
+            CorrArgs a;
+            fill_corr_args(c, P, ch, d_meta, K, a);
+            a.Ht = Ht;
+            a.flag = flag; a.out_iI = cand.iI; a.out_iF = cand.iF; a.out_nu = cand.nu;
+            a.out_S = cand.S; a.out_A = cand.A; a.out_I = cand.I; a.out_amp = cand.amp;
+            TL_BEGIN("corr_intensity");
+            launch_corr_intensity(S.stream, a);
+            TL_END();
+            double blk_bytes = 0;
+            for (const OutClass &oc : GP.oc) blk_bytes += 8.0 * oc.n * oc.K;
+            P.work[3] += ch.nunits * 2.0 * c->coef[ch.jindI][ch.jindF].C * (double)N * N;
+            P.work[4] += ch.nunits * 2.0 * GP.macs;
+            P.work[5] += (double)ch.ncand * 8.0 * BF.nsize;
+            P.work[6] += ch.nunits * blk_bytes;
+            P.work[7] += ch.nunits * 24.0 * (double)N * N;
+            P.work[8] += 2.0 * ch.ncand * ndegI * ndegF * BF.nsize;
+        }
+    }
+    HIPCHK(hipGetLastError());
+    TL_BEGIN("compact");
+    for (size_t k = 0; k < P.q.size(); k++) {
+        const long long b = P.q[k].cand_begin, n = P.q[k].cand_end - b;
+        const int nb = compact_blocks(n);
+        if (nb > 0)
+            launch_compact(S.stream, n, flag + b, counts + cbase[k], offs + obase[k],
+                           shift(cand, b), shift(comp, b));
+        else
+            HIPCHK(hipMemsetAsync(offs + obase[k], 0, 8, S.stream));
+        HIPCHK(hipMemcpyAsync(P.counts + k, offs + obase[k] + nb, 8, hipMemcpyDeviceToHost,
+                              S.stream));
+    }
+    TL_END();
+    HIPCHK(hipGetLastError());
+    for (auto &ch : P.chunks) P.work[0] += ch.nunits;
+    P.work[1] = (double)P.ncand;
+    return GI_OK;
+}
+
+// Copy the lines of query k (count already known) to host arrays, async.
+static int copy_lines(gi_ctx *c, Slot &S, const Plan &P, size_t k, int *iI, int *iF, double *nu,
+                      double *Sv, double *A, double *I, double *amp)
+{
+    const long long n = P.counts[k];
+    if (n <= 0) return GI_OK;
+    const LineSoA s = shift(P.comp, P.q[k].cand_begin);
+    HIPCHK(hipMemcpyAsync(iI, s.iI, n * 4, hipMemcpyDeviceToHost, S.stream));
+    HIPCHK(hipMemcpyAsync(iF, s.iF, n * 4, hipMemcpyDeviceToHost, S.stream));
+    HIPCHK(hipMemcpyAsync(nu, s.nu, n * 8, hipMemcpyDeviceToHost, S.stream));
+    HIPCHK(hipMemcpyAsync(Sv, s.S, n * 8, hipMemcpyDeviceToHost, S.stream));
+    HIPCHK(hipMemcpyAsync(A, s.A, n * 8, hipMemcpyDeviceToHost, S.stream));
+    HIPCHK(hipMemcpyAsync(I, s.I, n * 8, hipMemcpyDeviceToHost, S.stream));
+    HIPCHK(hipMemcpyAsync(amp, s.amp, n * 32, hipMemcpyDeviceToHost, S.stream));
     return GI_OK;
 }
 
 int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
 {
     if (!c->prepared) return fail(c, GI_ESTATE, "gi_compute before gi_prepare");
+    if (c->server) return fail(c, GI_ESTATE, "gi_compute while the request server runs");
     if (mode != GI_MODE_FP64 && mode != GI_MODE_FP64_BATCHED)
         return fail(c, GI_EINVAL, "gi_compute: unknown mode");
-    const bool batched = mode == GI_MODE_FP64_BATCHED;
-    HIPCHK(hipSetDevice(c->device));
-    std::vector<Item> items;
-    long long ncand = 0;
-    make_items(c, nlower, ids, items, ncand);
+    int rc;
+    Plan P;
+    P.mode = mode;
+    Query q;
+    q.beta = Consts::beta(c->temperature);
+    q.q_stat = c->q_stat;
+    q.zpe = c->zpe;
+    q.thS = c->thresh_S;
+    q.thI = c->thresh_I;
+    items_job(c, nlower, ids, 0, P);
+    q.cand_begin = 0;
+    q.cand_end = P.ncand;
+    P.q.push_back(q);
     for (double &w : c->work) w = 0;
-    c->work[1] = (double)ncand;
     c->kstats.clear();
     c->kstat_names.clear();
-    if (ncand == 0) {
+    if (P.ncand == 0) {
         out->n = 0;
         return GI_OK;
     }
-    const int N = c->nvib;
-    int rc = GI_OK;
-
-    // constants of cuda_host.cu:12-18 and beta of :1170
-    const double pi = 4.0 * std::atan2(1.0, 1.0);
-    const double planck = 6.62606896 * std::pow(10.0, -27.0);
-    const double avogno = 6.0221415 * std::pow(10.0, 23.0);
-    const double vellgt = 2.99792458 * std::pow(10.0, 10.0);
-    const double A_coef_s_1 =
-        64.0 * std::pow(10.0, -36.0) * std::pow(pi, 4.0) / (3.0 * 6.62606896 * std::pow(10.0, -27.0));
-    const double intens_cm_mol =
-        8.0 * std::pow(10.0, -36.0) * std::pow(pi, 3.0) * avogno / (3.0 * planck * vellgt);
-    const double boltz = 1.380658 * std::pow(10.0, -16.0);
-    const double beta = planck * vellgt / (boltz * c->temperature);
-
-    // candidate records and compacted output
-    const size_t nc = (size_t)ncand;
-    int *flag = (int *)scratch(c, "flag", nc * sizeof(int), &rc); if (rc) return rc;
-    LineSoA cand, comp;
-    cand.iI = (int *)scratch(c, "c_iI", nc * sizeof(int), &rc); if (rc) return rc;
-    cand.iF = (int *)scratch(c, "c_iF", nc * sizeof(int), &rc); if (rc) return rc;
-    cand.nu = (double *)scratch(c, "c_nu", nc * 8, &rc); if (rc) return rc;
-    cand.S = (double *)scratch(c, "c_S", nc * 8, &rc); if (rc) return rc;
-    cand.A = (double *)scratch(c, "c_A", nc * 8, &rc); if (rc) return rc;
-    cand.I = (double *)scratch(c, "c_I", nc * 8, &rc); if (rc) return rc;
-    cand.amp = (double *)scratch(c, "c_amp", nc * 32, &rc); if (rc) return rc;
-    comp.iI = (int *)scratch(c, "o_iI", nc * sizeof(int), &rc); if (rc) return rc;
-    comp.iF = (int *)scratch(c, "o_iF", nc * sizeof(int), &rc); if (rc) return rc;
-    comp.nu = (double *)scratch(c, "o_nu", nc * 8, &rc); if (rc) return rc;
-    comp.S = (double *)scratch(c, "o_S", nc * 8, &rc); if (rc) return rc;
-    comp.A = (double *)scratch(c, "o_A", nc * 8, &rc); if (rc) return rc;
-    comp.I = (double *)scratch(c, "o_I", nc * 8, &rc); if (rc) return rc;
-    comp.amp = (double *)scratch(c, "o_amp", nc * 32, &rc); if (rc) return rc;
-
-    // groups and chunks
-    const int kChunk = 256;
-    std::map<std::tuple<int, int, int>, std::vector<int>> groups;
-    for (int i = 0; i < (int)items.size(); i++)
-        groups[std::make_tuple(items[i].jindI, items[i].gI, items[i].jindF)].push_back(i);
-    std::vector<Chunk> chunks;
-    std::vector<unsigned char> meta;
-    auto put = [&](const void *src, size_t bytes) {
-        const size_t o = (meta.size() + 255) & ~size_t(255);
-        meta.resize(o + std::max<size_t>(bytes, 8));
-        if (bytes) std::memcpy(meta.data() + o, src, bytes);
-        return o;
-    };
-    int max_units = 0, max_items = 0, max_G = 1, max_H = 1, max_ncb = 1, max_nsizeF = 0;
-    size_t ht_total = 0;
-    for (auto &kv : groups) {
-        const int jindI = std::get<0>(kv.first), gI = std::get<1>(kv.first),
-                  jindF = std::get<2>(kv.first);
-        const int gF = c->igamma_pair[gI];
-        const int ndegI = c->degen[gI], ndegF = c->degen[gF];
-        const Block &BI = c->blocks[jindI * c->nrep + gI];
-        const int nsizeF = c->blocks[jindF * c->nrep + gF].nsize;
-        std::vector<int> its = kv.second;
-        if (batched)  // energy order keeps every chunk's windows a narrow band
-            std::stable_sort(its.begin(), its.end(), [&](int x, int y) {
-                return c->lv_energy[items[x].level] < c->lv_energy[items[y].level];
-            });
-        const GroupPlan &GP = c->groups[((size_t)jindI * c->nrep + gI) * 2 + jindF];
-        if (!GP.valid) return fail(c, GI_EINVAL, "internal: no plan for a (J, J') pair");
-        for (size_t s0 = 0; s0 < its.size(); s0 += kChunk) {
-            const int n_it = (int)std::min<size_t>(kChunk, its.size() - s0);
-            std::vector<long long> uptr, outoff(n_it);
-            std::vector<int> uideg, iunit(n_it * GI_MAXDEG, -1), cstart(n_it), lo(n_it), hi(n_it),
-                lev(n_it);
-            std::vector<double> eI(n_it);
-            int cnt = 0, Lo = 1 << 30, Hi = 0;
-            for (int q = 0; q < n_it; q++) {
-                const Item &it = items[its[s0 + q]];
-                for (int ideg = 0; ideg < ndegI; ideg++) {
-                    if (!degeneracy_filter(c, gI, gF, ideg, 0)) continue;
-                    iunit[q * GI_MAXDEG + ideg] = (int)uideg.size();
-                    uideg.push_back(ideg);
-                    uptr.push_back((long long)c->lv_irec0[it.level] * BI.nsize);
-                }
-                cstart[q] = cnt;
-                lo[q] = it.lo;
-                hi[q] = it.hi;
-                Lo = std::min(Lo, it.lo);
-                Hi = std::max(Hi, it.hi);
-                lev[q] = it.level;
-                eI[q] = c->lv_energy[it.level];
-                outoff[q] = it.out_off;
-                cnt += it.hi - it.lo;
-            }
-            int mask = 0;
-            for (int idegF = 0; idegF < ndegF; idegF++)
-                for (int idegI = 0; idegI < ndegI; idegI++) {
-                    if (c->reduced && idegF != 0) continue;
-                    if (!degeneracy_filter(c, gI, gF, idegI, idegF)) continue;
-                    if (!degeneracy_filter(c, gI, gF, idegI, 0)) continue;  // unit exists
-                    mask |= 1 << (idegF * ndegI + idegI);
-                }
-            Chunk ch;
-            ch.jindI = jindI; ch.gI = gI; ch.jindF = jindF; ch.gF = gF;
-            ch.nitems = n_it; ch.nunits = (int)uideg.size(); ch.ncand = cnt; ch.cb_mask = mask;
-            ch.Lo = Lo; ch.Hi = Hi;
-            ch.off_uptr = put(uptr.data(), uptr.size() * 8);
-            ch.off_uideg = put(uideg.data(), uideg.size() * 4);
-            ch.off_iunit = put(iunit.data(), iunit.size() * 4);
-            ch.off_cstart = put(cstart.data(), cstart.size() * 4);
-            ch.off_lo = put(lo.data(), lo.size() * 4);
-            ch.off_hi = put(hi.data(), hi.size() * 4);
-            ch.off_level = put(lev.data(), lev.size() * 4);
-            ch.off_eI = put(eI.data(), eI.size() * 8);
-            ch.off_out = put(outoff.data(), outoff.size() * 8);
-            ch.ht_off = batched ? ht_total : 0;
-            ht_total += (size_t)n_it * ndegI * ndegF * nsizeF;
-            chunks.push_back(ch);
-            max_units = std::max(max_units, ch.nunits);
-            max_items = std::max(max_items, n_it);
-            max_G = std::max(max_G, GP.Gtot);
-            max_H = std::max(max_H, GP.Htot);
-            max_ncb = std::max(max_ncb, ndegI * ndegF);
-            max_nsizeF = std::max(max_nsizeF, nsizeF);
-        }
+    if ((rc = build_chunks(c, P)) != GI_OK) return rc;
+    Slot &S = c->main_slot;
+    {
+        Timeline tl(S.stream);
+        if ((rc = launch_plan(c, S, P, &tl)) != GI_OK) return rc;
+        HIPCHK(hipStreamSynchronize(S.stream));
+        tl.collect(c);
     }
-    // unit pointers: block offsets -> device addresses
-    for (auto &ch : chunks) {
-        long long *p = (long long *)(meta.data() + ch.off_uptr);
-        const double *base = c->blocks[ch.jindI * c->nrep + ch.gI].d_vec;
-        for (int u = 0; u < ch.nunits; u++) p[u] = (long long)(base + p[u]);
-    }
-    unsigned char *d_meta = (unsigned char *)scratch(c, "meta", meta.size(), &rc); if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(d_meta, meta.data(), meta.size(), hipMemcpyHostToDevice, c->stream));
-    (void)ht_total;
-    Timeline tl(c);
-    double useful = 0, issued = 0, corr_bytes = 0, mu_bytes = 0, mu_dense = 0, corr_flops = 0;
-    if (batched) {
-        if ((rc = run_batched(c, chunks, d_meta, tl, beta, A_coef_s_1, intens_cm_mol, flag, cand,
-                              useful, issued, corr_bytes, mu_bytes, mu_dense, corr_flops)) != GI_OK)
-            return rc;
-    }
-    double *G = nullptr, *H = nullptr, *Ht = nullptr;
-    if (!batched) {
-        G = (double *)scratch(c, "G", (size_t)max_units * max_G * 8, &rc); if (rc) return rc;
-        H = (double *)scratch(c, "H", (size_t)max_units * max_H * 8, &rc); if (rc) return rc;
-        Ht = (double *)scratch(c, "Ht", (size_t)max_items * max_ncb * max_nsizeF * 8, &rc);
-        if (rc) return rc;
-    }
-    for (auto &ch : chunks) {
-        if (batched) break;
-        const JPlan &pI = c->jp[ch.jindI], &pF = c->jp[ch.jindF];
-        const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
-        const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
-        const int ndegI = c->degen[ch.gI], ndegF = c->degen[ch.gF];
-        SymDev r0 = symdev(pI.sym[ch.gI * GI_MAXDEG + 0]);
-        SymDev r1 = ndegI > 1 ? symdev(pI.sym[ch.gI * GI_MAXDEG + 1]) : r0;
-        SymDev t0 = symdev(pF.sym[ch.gF * GI_MAXDEG + 0]);
-        t0.tcol = GP.d_tcol[0];
-        SymDev t1 = t0;
-        if (ndegF > 1) {
-            t1 = symdev(pF.sym[ch.gF * GI_MAXDEG + 1]);
-            t1.tcol = GP.d_tcol[1];
-        }
-        if (ch.nunits > 0) {
-            for (const OutClass &oc : GP.oc) {
-                const ClassDev cd = classdev(oc);
-                if (cd.K > 0) {
-                    TL_BEGIN("build_g");
-                    launch_build_g_sym(c->stream, ch.nunits, N, GP.Gtot, cd,
-                                       (const double *const *)(d_meta + ch.off_uptr),
-                                       (const int *)(d_meta + ch.off_uideg), r0, r1, G);
-                    TL_END();
-                }
-                TL_BEGIN("half_ls_gemv");
-                if (launch_gemv_sym(c->stream, ch.nunits, N, GP.Gtot, GP.Htot, cd,
-                                    c->d_mu_perm, G, H) != 0)
-                    return fail(c, GI_EINVAL, "no half-ls kernel for this column count");
-                TL_END();
-            }
-        }
-        double *Htc = Ht;
-        TL_BEGIN("fold");
-        launch_fold(c->stream, ch.nitems, BF.nsize, ndegI, ndegF, GP.Htot,
-                    (const int *)(d_meta + ch.off_iunit), t0, t1, H, Htc);
-        TL_END();
-        {
-            CorrArgs a;
-            a.ncand = ch.ncand; a.nitems = ch.nitems; a.nsizeF = BF.nsize;
-            a.ndegI = ndegI; a.ndegF = ndegF; a.cb_mask = ch.cb_mask; a.reduced = c->reduced;
-            a.jI = c->jvals[ch.jindI]; a.jF = c->jvals[ch.jindF];
-            a.item_cand_start = (const int *)(d_meta + ch.off_cstart);
-            a.item_lo = (const int *)(d_meta + ch.off_lo);
-            a.item_level = (const int *)(d_meta + ch.off_level);
-            a.item_out_off = (const long long *)(d_meta + ch.off_out);
-            a.item_energy = (const double *)(d_meta + ch.off_eI);
-            a.blk_col = BF.d_col; a.blk_lev = BF.d_lev; a.blk_en = BF.d_en; a.blk_upok = BF.d_upok;
-            a.vec = BF.d_vec; a.Ht = Htc;
-            a.gnsI = c->gns[ch.gI]; a.zpe = c->zpe; a.beta = beta; a.q_stat = c->q_stat;
-            a.A_coef_s_1 = A_coef_s_1; a.intens_cm_mol = intens_cm_mol;
-            a.thresh_S = c->thresh_S; a.thresh_I = c->thresh_I;
-            a.flag = flag; a.out_iI = cand.iI; a.out_iF = cand.iF; a.out_nu = cand.nu;
-            a.out_S = cand.S; a.out_A = cand.A; a.out_I = cand.I; a.out_amp = cand.amp;
-            TL_BEGIN("corr_intensity");
-            launch_corr_intensity(c->stream, a);
-            TL_END();
-            corr_flops += 2.0 * ch.ncand * ndegI * ndegF * BF.nsize;
-        }
-        double blk_bytes = 0;
-        for (const OutClass &oc : GP.oc) blk_bytes += 8.0 * oc.n * oc.K;
-        useful += ch.nunits * 2.0 * c->coef[ch.jindI][ch.jindF].C * (double)N * N;
-        issued += ch.nunits * 2.0 * GP.macs;
-        mu_bytes += ch.nunits * blk_bytes;
-        mu_dense += ch.nunits * 24.0 * (double)N * N;
-        corr_bytes += (double)ch.ncand * 8.0 * BF.nsize;
-    }
-    HIPCHK(hipGetLastError());
-    const int nb = compact_blocks(ncand);
-    int *counts = (int *)scratch(c, "counts", (size_t)nb * 4, &rc); if (rc) return rc;
-    long long *offs = (long long *)scratch(c, "offs", (size_t)(nb + 1) * 8, &rc); if (rc) return rc;
-    TL_BEGIN("compact");
-    launch_compact(c->stream, ncand, flag, counts, offs, cand, comp);
-    TL_END();
-    HIPCHK(hipGetLastError());
-    long long nout = 0;
-    HIPCHK(hipMemcpyAsync(&nout, offs + nb, 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    tl.collect();
-    c->work[0] = 0;
-    for (auto &ch : chunks) c->work[0] += ch.nunits;
+    for (int i = 0; i < 10; i++) c->work[i] = P.work[i];
+    const long long nout = P.counts[0];
     c->work[2] = (double)nout;
-    c->work[3] = useful;
-    c->work[4] = issued;
-    c->work[5] = corr_bytes;
-    c->work[6] = mu_bytes;
-    c->work[7] = mu_dense;
-    c->work[8] = corr_flops;
     out->n = nout;
     if (nout > out->capacity) return fail(c, GI_ECAPACITY, "gi_compute: output capacity too small");
     if (nout > 0) {
         if (!out->ilevelI || !out->ilevelF || !out->nu || !out->S || !out->A_einst || !out->I ||
             !out->amp)
             return fail(c, GI_EINVAL, "gi_compute: output arrays missing");
-        HIPCHK(hipMemcpyAsync(out->ilevelI, comp.iI, nout * 4, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(out->ilevelF, comp.iF, nout * 4, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(out->nu, comp.nu, nout * 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(out->S, comp.S, nout * 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(out->A_einst, comp.A, nout * 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(out->I, comp.I, nout * 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(out->amp, comp.amp, nout * 32, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+        if ((rc = copy_lines(c, S, P, 0, out->ilevelI, out->ilevelF, out->nu, out->S,
+                             out->A_einst, out->I, out->amp)) != GI_OK)
+            return rc;
+        HIPCHK(hipStreamSynchronize(S.stream));
     }
     return GI_OK;
 }
 
+int count_job(gi_ctx *c, int nlower, const int *ids, long long *count)
+{
+    Plan P;
+    items_job(c, nlower, ids, 0, P);
+    *count = P.ncand;
+    return GI_OK;
+}
+
+// ===========================================================================
+// Request server: continuous batching over two execution slots.
+
+struct Request {
+    gi_request r;
+    long long ticket = 0;
+    double est = 0;                         // device bytes while in flight
+    std::chrono::steady_clock::time_point t_submit, t_done;
+    int status = GI_OK;
+    std::string err;
+    bool done = false;
+    long long n = 0;
+    std::vector<int> iI, iF;
+    std::vector<double> nu, S, A, I, amp;
+};
+
+struct Server {
+    gi_ctx *c = nullptr;
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Request>> pending;
+    std::map<long long, std::shared_ptr<Request>> all;
+    long long next_ticket = 1;
+    bool stop = false;
+    double budget = 0, in_use = 0;
+    Slot slots[2];
+    long long batches = 0, served = 0, max_batch = 0;
+};
+
+struct Inflight {
+    std::unique_ptr<Plan> plan;
+    std::vector<std::shared_ptr<Request>> reqs;
+    int slot = 0;
+    int status = GI_OK;
+    std::string err;
+};
+
+// Q(T) = sum over resident levels of gns * ndeg * (2J+1) * exp(-(E - ZPE) * beta)
+__global__ __launch_bounds__(256) void k_partition(int n, const double *__restrict__ e,
+                                                   const double *__restrict__ w, double zpe,
+                                                   double beta, double *__restrict__ part)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += 256 * gridDim.x)
+        s += w[i] * exp(-(e[i] - zpe) * beta);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+int partition_function(gi_ctx *c, Slot &S, double T, double *q)
+{
+    if (!c->prepared) return fail(c, GI_ESTATE, "partition function before gi_prepare");
+    if (!(T > 0.0)) return fail(c, GI_EINVAL, "temperature <= 0");
+    int rc;
+    const int nb = 64;
+    double *part = (double *)slot_scratch(c, S, "qpart", nb * 8, &rc); if (rc) return rc;
+    double *hp = (double *)slot_pinned(c, S, nb * 8, &rc); if (rc) return rc;
+    hipLaunchKernelGGL(k_partition, dim3(nb), dim3(256), 0, S.stream, c->nlevels, c->d_lv_energy,
+                       c->d_lv_weight, c->zpe, Consts::beta(T), part);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(hp, part, nb * 8, hipMemcpyDeviceToHost, S.stream));
+    HIPCHK(hipStreamSynchronize(S.stream));
+    double s = 0.0;
+    for (int i = 0; i < nb; i++) s += hp[i];  // fixed order: deterministic
+    *q = s;
+    return GI_OK;
+}
+
+static void finish_batch(Server &sv, Inflight &b)
+{
+    gi_ctx *c = sv.c;
+    Slot &S = sv.slots[b.slot];
+    int rc = b.status;
+    std::string err = b.err;
+    if (rc == GI_OK && hipStreamSynchronize(S.stream) != hipSuccess) {
+        rc = GI_EHIP;
+        err = "request batch failed on the device";
+    }
+    if (rc == GI_OK) {
+        for (size_t k = 0; k < b.reqs.size(); k++) {
+            Request &R = *b.reqs[k];
+            const long long n = b.plan->counts[k];
+            R.n = n;
+            R.iI.resize(n); R.iF.resize(n); R.nu.resize(n); R.S.resize(n); R.A.resize(n);
+            R.I.resize(n); R.amp.resize(4 * n);
+            if (n > 0 && copy_lines(c, S, *b.plan, k, R.iI.data(), R.iF.data(), R.nu.data(),
+                                    R.S.data(), R.A.data(), R.I.data(), R.amp.data()) != GI_OK)
+                rc = GI_EHIP;
+        }
+        if (rc == GI_OK && hipStreamSynchronize(S.stream) != hipSuccess) rc = GI_EHIP;
+        if (rc != GI_OK) err = "copy of request lines failed";
+    }
+    const auto now = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> lk(sv.mu);
+    for (auto &r : b.reqs) {
+        r->status = rc;
+        if (rc != GI_OK) r->err = err;
+        r->t_done = now;
+        r->done = true;
+        sv.in_use -= r->est;
+    }
+    sv.served += (long long)b.reqs.size();
+    sv.cv_done.notify_all();
+}
+
+static void serve_loop(Server &sv)
+{
+    gi_ctx *c = sv.c;
+    std::string werr;
+    err_sink = &werr;  // this thread's errors stay with its batches
+    (void)hipSetDevice(c->device);
+    std::deque<Inflight> inflight;
+    int next_slot = 0;
+    for (;;) {
+        std::vector<std::shared_ptr<Request>> adm;
+        {
+            std::unique_lock<std::mutex> lk(sv.mu);
+            if (inflight.empty())
+                sv.cv_work.wait(lk, [&] { return sv.stop || !sv.pending.empty(); });
+            if (sv.stop && sv.pending.empty() && inflight.empty()) break;
+            if (inflight.size() < 2) {
+                int mode = -1;
+                while (!sv.pending.empty()) {
+                    auto &r = sv.pending.front();
+                    if (mode >= 0 && r->r.mode != mode) break;  // one mode per batch
+                    if (sv.in_use + r->est > sv.budget) break;  // admission by memory
+                    mode = r->r.mode;
+                    sv.in_use += r->est;
+                    adm.push_back(r);
+                    sv.pending.pop_front();
+                }
+            }
+        }
+        if (!adm.empty()) {
+            Inflight b;
+            b.plan.reset(new Plan());
+            b.reqs = adm;
+            b.slot = next_slot;
+            next_slot ^= 1;
+            Plan &P = *b.plan;
+            P.mode = adm[0]->r.mode;
+            int rc = GI_OK;
+            for (size_t k = 0; k < adm.size() && rc == GI_OK; k++) {
+                const gi_request &r = adm[k]->r;
+                Query q;
+                q.beta = Consts::beta(r.temperature);
+                q.q_stat = r.q_stat;
+                if (!(q.q_stat > 0.0))
+                    rc = partition_function(c, sv.slots[b.slot], r.temperature, &q.q_stat);
+                q.zpe = c->zpe;
+                q.thS = r.thresh_linestrength;
+                q.thI = r.thresh_intensity;
+                q.cand_begin = P.ncand;
+                items_request(c, r, (int)k, P);
+                q.cand_end = P.ncand;
+                P.q.push_back(q);
+            }
+            if (rc == GI_OK) rc = build_chunks(c, P);
+            if (rc == GI_OK) rc = launch_plan(c, sv.slots[b.slot], P, nullptr);
+            b.status = rc;
+            if (rc != GI_OK) b.err = werr;
+            sv.batches++;
+            sv.max_batch = std::max<long long>(sv.max_batch, (long long)adm.size());
+            inflight.push_back(std::move(b));
+        }
+        // drain the oldest batch when both slots are busy or nothing new came in
+        if (!inflight.empty() && (adm.empty() || inflight.size() == 2)) {
+            finish_batch(sv, inflight.front());
+            inflight.pop_front();
+        }
+    }
+}
+
+void server_shutdown(gi_ctx *c)
+{
+    Server *sv = c->server;
+    if (!sv) return;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+    }
+    sv->cv_work.notify_all();
+    if (sv->th.joinable()) sv->th.join();
+    for (auto &s : sv->slots) {
+        free_slot(s);
+        if (s.stream) (void)hipStreamDestroy(s.stream);
+    }
+    delete sv;
+    c->server = nullptr;
+}
+
+// Device bytes a request holds while in flight: candidate and compacted
+// records, flags, and the G, H and folded vectors of its units.
+static double request_bytes(gi_ctx *c, const gi_request &r, long long *ncand)
+{
+    Plan P;
+    items_request(c, r, 0, P);
+    *ncand = P.ncand;
+    const int gF = c->igamma_pair[r.igammaI];
+    const GroupPlan &GP = c->groups[((size_t)r.jindI * c->nrep + r.igammaI) * 2 + r.jindF];
+    const double units = (double)P.items.size() * c->degen[r.igammaI];
+    const double ncb = (double)c->degen[r.igammaI] * c->degen[gF];
+    const double nsizeF = c->blocks[r.jindF * c->nrep + gF].nsize;
+    return (double)P.ncand * (2 * 80.0 + 4.0) + units * 8.0 * (GP.Gtot + GP.Htot) +
+           (double)P.items.size() * ncb * nsizeF * 8.0 + 4096.0;
+}
+
 }  // namespace gi
+
+using namespace gi;
 
 extern "C" {
 
 int gi_count_candidates(gi_ctx *c, int nlower, const int *ids, long long *count)
 {
     if (!c || !count) return GI_EINVAL;
-    if (!c->prepared) return gi::fail(c, GI_ESTATE, "gi_count_candidates before gi_prepare");
-    std::vector<gi::Item> items;
-    long long n = 0;
-    gi::make_items(c, nlower, ids, items, n);
-    *count = n;
-    return GI_OK;
+    if (!c->prepared) return fail(c, GI_ESTATE, "gi_count_candidates before gi_prepare");
+    return count_job(c, nlower, ids, count);
 }
 
 int gi_compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
 {
     if (!c || !out) return GI_EINVAL;
-    return gi::compute(c, mode, nlower, ids, out);
+    return compute(c, mode, nlower, ids, out);
+}
+
+int gi_partition_function(gi_ctx *c, double temperature, double *q)
+{
+    if (!c || !q) return GI_EINVAL;
+    return partition_function(c, c->main_slot, temperature, q);
+}
+
+int gi_serve_start(gi_ctx *c, long long budget_bytes)
+{
+    if (!c || budget_bytes <= 0) return GI_EINVAL;
+    if (!c->prepared) return fail(c, GI_ESTATE, "gi_serve_start before gi_prepare");
+    if (c->server) return fail(c, GI_ESTATE, "request server already running");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, GI_EHIP, "hipSetDevice");
+    Server *sv = new Server();
+    sv->c = c;
+    sv->budget = (double)budget_bytes;
+    for (auto &s : sv->slots)
+        if (hipStreamCreate(&s.stream) != hipSuccess) {
+            for (auto &t : sv->slots)
+                if (t.stream) (void)hipStreamDestroy(t.stream);
+            delete sv;
+            return fail(c, GI_EHIP, "hipStreamCreate");
+        }
+    c->server = sv;
+    sv->th = std::thread(serve_loop, std::ref(*sv));
+    return GI_OK;
+}
+
+int gi_serve_submit(gi_ctx *c, const gi_request *r, long long *ticket)
+{
+    if (!c || !r || !ticket) return GI_EINVAL;
+    Server *sv = c->server;
+    if (!sv) return fail(c, GI_ESTATE, "gi_serve_submit without gi_serve_start");
+    if (r->jindI < 0 || r->jindI > 1 || r->jindF < 0 || r->jindF > 1 || r->igammaI < 0 ||
+        r->igammaI >= c->nrep)
+        return GI_EINVAL;
+    if (!c->groups[((size_t)r->jindI * c->nrep + r->igammaI) * 2 + r->jindF].valid)
+        return GI_EINVAL;  // |J' - J| > 1
+    if (r->mode != GI_MODE_FP64 && r->mode != GI_MODE_FP64_BATCHED) return GI_EINVAL;
+    if (!(r->temperature > 0.0)) return GI_EINVAL;
+    auto R = std::make_shared<Request>();
+    R->r = *r;
+    long long nc = 0;
+    R->est = request_bytes(c, *r, &nc);
+    if (R->est > sv->budget) return GI_EBUDGET;
+    R->t_submit = std::chrono::steady_clock::now();
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        R->ticket = sv->next_ticket++;
+        sv->all[R->ticket] = R;
+        sv->pending.push_back(R);
+    }
+    sv->cv_work.notify_one();
+    *ticket = R->ticket;
+    return GI_OK;
+}
+
+int gi_serve_wait(gi_ctx *c, long long ticket, double timeout_s, long long *nlines,
+                  double *latency_ms)
+{
+    if (!c) return GI_EINVAL;
+    Server *sv = c->server;
+    if (!sv) return GI_ESTATE;
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->all.find(ticket);
+    if (it == sv->all.end()) return GI_EINVAL;
+    std::shared_ptr<Request> R = it->second;
+    const auto until = std::chrono::steady_clock::now() +
+                       std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                           std::chrono::duration<double>(timeout_s > 0 ? timeout_s : 1e9));
+    if (!sv->cv_done.wait_until(lk, until, [&] { return R->done; })) return GI_ETIMEOUT;
+    if (R->status != GI_OK) {
+        c->err = R->err;
+        return R->status;
+    }
+    if (nlines) *nlines = R->n;
+    if (latency_ms)
+        *latency_ms = std::chrono::duration<double, std::milli>(R->t_done - R->t_submit).count();
+    return GI_OK;
+}
+
+int gi_serve_fetch(gi_ctx *c, long long ticket, gi_lines *out)
+{
+    if (!c || !out) return GI_EINVAL;
+    Server *sv = c->server;
+    if (!sv) return GI_ESTATE;
+    std::shared_ptr<Request> R;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        auto it = sv->all.find(ticket);
+        if (it == sv->all.end()) return GI_EINVAL;
+        R = it->second;
+        if (!R->done) return fail(c, GI_ESTATE, "gi_serve_fetch before the request finished");
+        if (R->status != GI_OK) return fail(c, R->status, R->err);
+    }
+    out->n = R->n;
+    if (R->n > out->capacity) return fail(c, GI_ECAPACITY, "gi_serve_fetch: capacity too small");
+    if (R->n > 0) {
+        std::memcpy(out->ilevelI, R->iI.data(), R->n * 4);
+        std::memcpy(out->ilevelF, R->iF.data(), R->n * 4);
+        std::memcpy(out->nu, R->nu.data(), R->n * 8);
+        std::memcpy(out->S, R->S.data(), R->n * 8);
+        std::memcpy(out->A_einst, R->A.data(), R->n * 8);
+        std::memcpy(out->I, R->I.data(), R->n * 8);
+        std::memcpy(out->amp, R->amp.data(), R->n * 32);
+    }
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->all.erase(ticket);
+    return GI_OK;
+}
+
+int gi_serve_stats(gi_ctx *c, long long *batches, long long *served, long long *max_batch)
+{
+    if (!c) return GI_EINVAL;
+    Server *sv = c->server;
+    if (!sv) return GI_ESTATE;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (batches) *batches = sv->batches;
+    if (served) *served = sv->served;
+    if (max_batch) *max_batch = sv->max_batch;
+    return GI_OK;
+}
+
+int gi_serve_stop(gi_ctx *c)
+{
+    if (!c) return GI_EINVAL;
+    if (!c->server) return GI_OK;
+    server_shutdown(c);
+    return GI_OK;
 }
 
 }  // extern "C"

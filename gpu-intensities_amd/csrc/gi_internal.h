@@ -98,6 +98,18 @@ struct KStat {
     int launches = 0;
 };
 
+// An execution slot: a stream with its own device scratch buffers and pinned
+// host staging. The context's main slot runs gi_compute; the request server
+// alternates between two more so that one batch drains while the next runs.
+struct Slot {
+    hipStream_t stream = nullptr;
+    std::map<std::string, std::pair<void *, size_t>> buf;
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
+};
+
+struct Server;
+
 }  // namespace gi
 
 struct gi_ctx {
@@ -142,8 +154,11 @@ struct gi_ctx {
     double *d_mu_perm = nullptr;     // mu with rows and columns permuted by vperm
     std::vector<gi::GroupPlan> groups;   // [(jindI*nrep + gI)*2 + jindF]
 
-    // scratch (grown on demand)
-    std::map<std::string, std::pair<void *, size_t>> scratch;
+    // main execution slot (stream = the context's stream), grown on demand
+    gi::Slot main_slot;
+    gi::Server *server = nullptr;
+    // partition-function inputs on the device: energies and gns*ndeg*(2J+1)
+    double *d_lv_energy = nullptr, *d_lv_weight = nullptr;
 
     // statistics of the last gi_compute
     std::map<std::string, gi::KStat> kstats;
@@ -153,7 +168,11 @@ struct gi_ctx {
 
 namespace gi {
 
+// Records msg as the context's last error and returns code. On a thread that
+// set err_sink (the request server's worker) the message goes there instead,
+// so the worker never writes the context's error string.
 int fail(gi_ctx *c, int code, const std::string &msg);
+extern thread_local std::string *err_sink;
 
 // Register-tile width (columns) of the half-ls kernels for `cols` columns.
 inline int smax_for(int cols)
@@ -164,6 +183,10 @@ inline int smax_for(int cols)
     return -1;
 }
 void *scratch(gi_ctx *c, const char *name, size_t bytes, int *rc);
+void *slot_scratch(gi_ctx *c, Slot &s, const char *name, size_t bytes, int *rc);
+void *slot_pinned(gi_ctx *c, Slot &s, size_t bytes, int *rc);
+void free_slot(Slot &s);
+void server_shutdown(gi_ctx *c);
 
 inline SymDev symdev(const SymMap &m)
 {

@@ -29,7 +29,8 @@ typedef enum gi_status {
     GI_ENOMEM = 3,     /* device or host allocation failed                   */
     GI_ESTATE = 4,     /* call out of order (e.g. compute before prepare)    */
     GI_ECAPACITY = 5,  /* output arrays too small; required size is returned */
-    GI_EBUDGET = 6     /* request does not fit the context's memory budget   */
+    GI_EBUDGET = 6,    /* request does not fit the context's memory budget   */
+    GI_ETIMEOUT = 7    /* gi_serve_wait: the request has not finished yet     */
 } gi_status;
 
 /* Arithmetic of the half-linestrength and correlation products. Energies,
@@ -175,6 +176,56 @@ int gi_kernel_stats(gi_ctx *ctx, int max, const char **names, double *total_ms,
  * (per unit in GI_MODE_FP64, per batch in GI_MODE_FP64_BATCHED), [7] dense mu
  * bytes (24*Nvib^2 per unit), [8] correlation flops executed. */
 int gi_work_stats(gi_ctx *ctx, double *out, int n);
+
+/* Partition function over the resident level table:
+ * Q(T) = sum_i gns(g_i) * ndeg_i * (2 J_i + 1) * exp(-(E_i - ZPE) * c2 / T),
+ * c2 = planck * vellgt / boltz with the constants of cuda_host.cu:12-18.
+ * The reference takes Q from the deck (fields.cpp:236-282, q_stat); this
+ * gives the caller a value for other temperatures. Reduced on the device in
+ * a fixed order (deterministic). */
+int gi_partition_function(gi_ctx *ctx, double temperature, double *q);
+
+/* ------------------------------------------------------------------------
+ * Request server. The reference computes one deck job per process
+ * (cuda_host.cu:995-1506); it has no request interface. These entry points
+ * keep the prepared basis, dipole and eigenvectors resident and serve
+ * requests against them: a worker thread admits queued requests while their
+ * device memory fits the budget, batches the admitted ones (same mode) into
+ * one launch sequence, and alternates two streams so that one batch's lines
+ * are copied out while the next computes. A request's lines are the same
+ * whatever it is batched with.
+ * While the server runs, calls that change the context (gi_set_params,
+ * uploads, gi_prepare) and gi_compute return GI_ESTATE. */
+typedef struct gi_request {
+    int jindI, igammaI, jindF;    /* lower block (J, symmetry) and upper J; upper
+                                     symmetry = SELECTION partner of igammaI  */
+    int mode;                     /* gi_mode                                  */
+    double temperature;           /* K                                        */
+    double q_stat;                /* partition function; <= 0: gi_partition_function(T) */
+    double elow_lo, elow_hi;      /* lower levels with E - ZPE in [lo, hi], on top of
+                                     the job's lower-level filters              */
+    double nu_lo, nu_hi;          /* frequency window (replaces freq_window)  */
+    double thresh_linestrength, thresh_intensity;
+} gi_request;
+
+/* Start the worker. budget_bytes bounds the device memory of the requests
+ * in flight (candidate records and half-linestrength buffers). */
+int gi_serve_start(gi_ctx *ctx, long long budget_bytes);
+/* Queue a request; *ticket identifies it. GI_EBUDGET if it alone exceeds
+ * the budget, GI_EINVAL if |J' - J| > 1 or a field is out of range. */
+int gi_serve_submit(gi_ctx *ctx, const gi_request *req, long long *ticket);
+/* Wait up to timeout_s (<= 0: no limit) for a request. Returns GI_OK with
+ * the line count and submit-to-done latency, GI_ETIMEOUT, or the
+ * request's own error. */
+int gi_serve_wait(gi_ctx *ctx, long long ticket, double timeout_s, long long *nlines,
+                  double *latency_ms);
+/* Copy a finished request's lines to `out` (rows in the order of
+ * gi_compute) and release the request. GI_ECAPACITY leaves it in place. */
+int gi_serve_fetch(gi_ctx *ctx, long long ticket, gi_lines *out);
+/* Batches launched, requests finished and the largest batch so far. */
+int gi_serve_stats(gi_ctx *ctx, long long *batches, long long *served, long long *max_batch);
+/* Finish every queued request, then stop the worker (also done by gi_destroy). */
+int gi_serve_stop(gi_ctx *ctx);
 
 #ifdef __cplusplus
 }
