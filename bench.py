@@ -75,9 +75,38 @@ def build_c2_job(torch, nvib=2000, jvals=(10, 11), seed=20240, lower_max=10000.0
     return job, dev
 
 
+# HIP-event timer name -> kernel symbol in the rocprofv3 summaries
+KERNEL_SYMBOL = {"half_ls_gemv": "k_gemv_sym", "half_ls_mfma": "k_halfls_jobs",
+                 "corr_mfma": "k_corr_mfma", "corr_intensity": "k_corr_intensity",
+                 "build_g": "k_build_g_jobs"}
+PMC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "c2_pmc_traffic.json")
+
+
+def measured_traffic(name):
+    """HBM bytes per dispatch of the kernel behind timer `name`, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this workload,
+    each scaled by the factor the same counter gives for a plain 1 GiB copy.
+    None when no such profile is committed."""
+    try:
+        with open(PMC_FILE) as f:
+            d = json.load(f)
+        return d["kernels"][KERNEL_SYMBOL[name]]["hbm_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def roofline_entry(name, kstats, work, nvib):
     """Roofline of one kernel: algorithmic work of all its launches in the last
     step divided by their HIP-event time (average per launch)."""
+    r = _roofline(name, kstats, work, nvib)
+    r["traffic"] = measured_traffic(name)
+    if r["traffic"] is not None:
+        r["traffic_unit"] = "bytes per launch (rocprofv3 PMC, copy-calibrated)"
+    return r
+
+
+def _roofline(name, kstats, work, nvib):
     ks = kstats[name]
     t = ks["total_ms"] / 1e3
     avg_ms = t / ks["launches"] * 1e3
@@ -106,6 +135,61 @@ def roofline_entry(name, kstats, work, nvib):
                 traffic=None, kernel=name, avg_launch_ms=avg_ms)
 
 
+def serve_sweep(eng, job, binding, lambdas, seconds, mode, seed=7, budget=8 << 30):
+    """Poisson request stream against the resident blocks: each request is a
+    random (J, J') pair that the selection rules allow, a random lower
+    symmetry, T ~ U[200, 2000] K with Q(T) from the device, a 1000 cm^-1
+    window at a random start in [0, 9000] cm^-1 and an intensity threshold
+    10^U(-32,-26). Latency = submit to lines ready in host memory (measured by
+    the library); throughput = lines / (last completion - first submit)."""
+    import threading
+    rng = np.random.default_rng(seed)
+    j0 = job.jvals[0]
+    pairs = [(a, b) for a in range(2) for b in range(2)
+             if abs(job.jvals[a] - job.jvals[b]) <= 1 and not (job.jvals[a] == job.jvals[b] == j0)]
+    nrep = job.sym["nrep"]
+    res = {}
+    for lam in lambdas:
+        n = max(int(lam * seconds), 4)
+        gaps = rng.exponential(1.0 / lam, size=n)
+        reqs = []
+        for _ in range(n):
+            a, b = pairs[int(rng.integers(len(pairs)))]
+            nu0 = float(rng.uniform(0.0, 9000.0))
+            reqs.append(dict(jindI=a, igammaI=int(rng.integers(nrep)), jindF=b, mode=mode,
+                             temperature=float(rng.uniform(200.0, 2000.0)), q_stat=0.0,
+                             nu=(nu0, nu0 + 1000.0), thresh_linestrength=0.0,
+                             thresh_intensity=float(10.0 ** rng.uniform(-32, -26))))
+        eng.serve_start(budget)
+        tickets = []
+        t0 = time.perf_counter()
+        t_next = t0
+        for gap, r in zip(gaps, reqs):
+            t_next += gap
+            d = t_next - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets.append(eng.serve_submit(**r))
+        lat, lines = [], 0
+        for t in tickets:
+            nl, ms = eng.serve_wait(t)
+            lat.append(ms)
+            lines += nl
+            eng.serve_fetch(t)
+        t1 = time.perf_counter()
+        st = eng.serve_stats()
+        eng.serve_stop()
+        lat = np.asarray(lat)
+        res[str(lam)] = dict(requests=n, offered_s=round(float(gaps.sum()), 3),
+                             p50_ms=float(np.percentile(lat, 50)),
+                             p99_ms=float(np.percentile(lat, 99)),
+                             lines=int(lines), lines_per_s=lines / (t1 - t0),
+                             requests_per_s=n / (t1 - t0), batches=st["batches"],
+                             max_batch=st["max_batch"])
+        log(f"serve mode {mode} lambda {lam}: {res[str(lam)]}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +202,10 @@ def main():
                     help="comma-separated FP64 modes to time; the fastest is reported as value")
     ap.add_argument("--cache-dir", default=None,
                     help="reuse the QR eigenvector blocks across processes (profiler runs)")
+    ap.add_argument("--serve-lambdas", default="10,50,100,200,500",
+                    help="request rates (req/s) of the serving sweep; empty to skip")
+    ap.add_argument("--serve-seconds", type=float, default=2.0,
+                    help="offered duration of each serving rate")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device copy (PMC calibration)")
     args = ap.parse_args()
@@ -178,6 +266,13 @@ def main():
     kern = modes[best]["kernels"]
     for m in modes.values():
         m.pop("_kstats", None)
+    serving = None
+    if args.serve_lambdas:
+        lams = [float(x) for x in args.serve_lambdas.split(",") if x]
+        serving = {}
+        for mname in args.modes.split(","):
+            serving[mname] = serve_sweep(eng, job, binding, lams, args.serve_seconds,
+                                         binding.MODES[mname])
     lower_ok = np.nonzero((job.levels["energy"] - job.zpe >= job.erange_lower[0]) &
                           (job.levels["energy"] - job.zpe <= job.erange_lower[1]))[0]
 
@@ -234,6 +329,7 @@ def main():
         "work": work,
         "modes": modes,
         "parity": parity,
+        "serving": serving,
     }
     print(json.dumps(out))
     eng.close()
