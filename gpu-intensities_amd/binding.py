@@ -30,7 +30,7 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_params", "gi_uplo
            "gi_count_candidates", "gi_half_ls", "gi_correlate", "gi_threej_table",
            "gi_kernel_stats", "gi_work_stats", "gi_partition_function", "gi_serve_start",
            "gi_serve_submit", "gi_serve_wait", "gi_serve_fetch", "gi_serve_stats",
-           "gi_serve_stop"]
+           "gi_serve_stop", "gi_set_tiles", "gi_get_tiles"]
 
 
 class GIError(RuntimeError):
@@ -98,6 +98,8 @@ def lib():
         L.gi_serve_fetch.argtypes = [vp, cll, C.POINTER(GiLines)]
         L.gi_serve_stats.argtypes = [vp, C.POINTER(cll), C.POINTER(cll), C.POINTER(cll)]
         L.gi_serve_stop.argtypes = [vp]
+        L.gi_set_tiles.argtypes = [vp, ci, ci]
+        L.gi_get_tiles.argtypes = [vp, P_i, P_i]
         _LIB = L
     return _LIB
 
@@ -264,6 +266,15 @@ class Engine:
             rc = lib().gi_compute(self.h, mode, len(ids), _p(ids, P_i), C.byref(s))
         self._chk(rc, "gi_compute")
         return self._result(bufs, s.n)
+
+    def set_tiles(self, halfls=-1, corr=-1):
+        """Tile shapes of the batched kernels: 0..3 = 64x64, 64x128, 128x64, 128x128."""
+        self._chk(lib().gi_set_tiles(self.h, halfls, corr), "gi_set_tiles")
+
+    def get_tiles(self):
+        a, b = ci(), ci()
+        self._chk(lib().gi_get_tiles(self.h, C.byref(a), C.byref(b)), "gi_get_tiles")
+        return a.value, b.value
 
     def partition_function(self, temperature):
         q = cd()

@@ -150,6 +150,11 @@ static void free_plan(gi_ctx *c)
             (void)hipFree(oc.d_rcoef);
         }
         for (int d = 0; d < GI_MAXDEG; d++) (void)hipFree(G.d_tcol[d]);
+        (void)hipFree(G.d_cd);
+        for (int d = 0; d < GI_MAXDEG; d++) {
+            (void)hipFree(G.d_gidx[d]);
+            (void)hipFree(G.d_gwt[d]);
+        }
     }
     c->groups.clear();
     (void)hipFree(c->d_mu_perm);
@@ -443,6 +448,19 @@ int gi_prepare(gi_ctx *c)
                 }
                 for (int d = 0; d < c->degen[G.gF]; d++)
                     if ((rc = upload(c, &G.d_tcol[d], G.tcol[d])) != GI_OK) return rc;
+                std::vector<ClassDev> cds;
+                for (auto &oc : G.oc) cds.push_back(classdev(oc));
+                if ((rc = upload(c, &G.d_cd, cds)) != GI_OK) return rc;
+                if ((rc = build_gmap(c, G)) != GI_OK) return rc;
+                if (G.gW <= kGMapMaxW)
+                    for (int d = 0; d < c->degen[G.gI]; d++) {
+                        if ((rc = upload(c, &G.d_gidx[d], G.gidx[d])) != GI_OK) return rc;
+                        if ((rc = upload(c, &G.d_gwt[d], G.gwt[d])) != GI_OK) return rc;
+                    }
+                for (int d = 0; d < GI_MAXDEG; d++) {  // host copies are not needed again
+                    std::vector<int>().swap(G.gidx[d]);
+                    std::vector<double>().swap(G.gwt[d]);
+                }
             }
     // level filters and per-block energy order
     const int nq = c->nmodes + 1;
@@ -561,6 +579,23 @@ int gi_kernel_stats(gi_ctx *c, int max, const char **names, double *total_ms, in
         if (total_ms) total_ms[i] = s.ms;
         if (launches) launches[i] = s.launches;
     }
+    return GI_OK;
+}
+
+int gi_set_tiles(gi_ctx *c, int hl, int cr)
+{
+    if (!c || hl < -1 || hl > 3 || cr < -1 || cr > 3) return GI_EINVAL;
+    if (serving(c)) return GI_ESTATE;
+    if (hl >= 0) c->tile_halfls = hl;
+    if (cr >= 0) c->tile_corr = cr;
+    return GI_OK;
+}
+
+int gi_get_tiles(gi_ctx *c, int *hl, int *cr)
+{
+    if (!c) return GI_EINVAL;
+    if (hl) *hl = c->tile_halfls;
+    if (cr) *cr = c->tile_corr;
     return GI_OK;
 }
 

@@ -14,9 +14,7 @@
 // outside every window of their items are skipped, entries outside an item's
 // own window are not stored.
 //
-// Both kernels: 256 threads (4 wavefronts) per 64 x 64 output tile, K staged
-// through LDS in steps of 16, each wavefront owns a 32 x 32 quarter as 2 x 2
-// MFMA 16x16 blocks. Operand layout of the 16x16x4 f64 instruction: lane l
+// Operand layout of the 16x16x4 f64 instruction: lane l
 // supplies A[l%16][l/16] and B[l/16][l%16]; accumulator register r holds
 // D[4*r + l/16][l%16], r = 0..3 (unlike the f32 16x16x4 form, whose register r
 // holds row 4*(l/16) + r).
@@ -28,28 +26,7 @@ namespace gi {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-#define BM 64
-#define BN 64
 #define BK 16
-// LDS row of 80 doubles: the four K rows one MFMA operand read touches fall
-// 32 banks apart, so the two half-waves of a 64-bit LDS read are conflict-free.
-#define LDP 80
-
-__device__ inline void mfma_tile(const double (*As)[LDP], const double (*Bs)[LDP], int wm, int wn,
-                                 int lane, d4 acc[2][2])
-{
-#pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-        const int k = kk + (lane >> 4);
-        const int i = lane & 15;
-        double a0 = As[k][wm + i], a1 = As[k][wm + 16 + i];
-        double b0 = Bs[k][wn + i], b1 = Bs[k][wn + 16 + i];
-        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Flattened launches over many (chunk, output class) jobs: a workgroup finds
@@ -64,91 +41,188 @@ __device__ inline int find_job(const int *__restrict__ begin, int njobs, int wg)
     return lo;
 }
 
-// K1 over jobs: G rows of every (chunk, class) job (same arithmetic as
-// k_build_g_sym). Workgroup = 64 rows of one unit.
-__global__ __launch_bounds__(64) void k_build_g_jobs(int N, const HalfJob *__restrict__ jobs,
-                                                     const int *__restrict__ gbegin, int njobs)
+// Tile shapes. BM x BN output tile per 256-thread workgroup (4 wavefronts in
+// a 2 x 2 layout, each owning a (BM/2) x (BN/2) block of 16 x 16 MFMA
+// tiles); K staged through LDS in steps of BK with a register prefetch of
+// the next step. LDS rows are padded to BM + 16 doubles: the four K rows one
+// MFMA operand read touches then fall 32 banks apart.
+template <int BM_, int BN_>
+struct Tile {
+    static constexpr int PA = BM_ + 16, PB = BN_ + 16;
+    static constexpr int MI = BM_ / 32, NI = BN_ / 32;  // 16x16 blocks per wave
+    static constexpr int LA = BM_ * BK / 256, LB = BN_ * BK / 256;
+};
+
+template <int BM_, int BN_>
+__device__ inline void mfma_step(const double *__restrict__ As, const double *__restrict__ Bs,
+                                 int wm, int wn, int lane,
+                                 d4 (&acc)[Tile<BM_, BN_>::MI][Tile<BM_, BN_>::NI])
 {
-    const int wg = blockIdx.x;
-    const HalfJob &J = jobs[find_job(gbegin, njobs, wg)];
-    const int local = wg - J.gtile_begin;
-    const int unit = local / J.gkt, kt = local - unit * J.gkt;
-    const ClassDev &cd = J.cd;
-    const int row = kt * 64 + threadIdx.x;
-    if (row >= cd.K) return;
-    int e = 0;
-    while (e + 1 < cd.nseg && cd.seg_row0[e + 1] <= row) e++;
-    const int vI = cd.seg_off[e] + (row - cd.seg_row0[e]);
+    using T = Tile<BM_, BN_>;
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+        const int k = kk + (lane >> 4);
+        const int i = lane & 15;
+        double a[T::MI], b[T::NI];
+#pragma unroll
+        for (int x = 0; x < T::MI; x++) a[x] = As[k * T::PA + wm + 16 * x + i];
+#pragma unroll
+        for (int y = 0; y < T::NI; y++) b[y] = Bs[k * T::PB + wn + 16 * y + i];
+#pragma unroll
+        for (int x = 0; x < T::MI; x++)
+#pragma unroll
+            for (int y = 0; y < T::NI; y++)
+                acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
+    }
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup w runs on
+// XCD w % 8). Tile t = (w % 8) * (grid / 8) + w / 8 gives each XCD one
+// contiguous range of tiles, so tiles that share an operand slab run at the
+// same time on the same L2. The grid is a multiple of 8; tiles >= ntiles exit.
+__device__ inline int xcd_tile(int w, int grid) { return (w & 7) * (grid >> 3) + (w >> 3); }
+static inline int xcd_grid(int ntiles) { return (ntiles + 7) & ~7; }
+
+// K1 over chunks: G of every unit, element by element in G order, so each
+// wavefront writes whole cache lines and the tiles of one unit (which read
+// the same eigenvector record) run back to back on one XCD.
+__global__ __launch_bounds__(256) void k_build_g_units(int N, const GBuildJob *__restrict__ jobs,
+                                                       const int *__restrict__ begin, int njobs,
+                                                       int ntiles)
+{
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
+    const GBuildJob &J = jobs[find_job(begin, njobs, tile)];
+    const int local = tile - J.tile_begin;
+    const int unit = local / J.tiles_per_unit, t = local - unit * J.tiles_per_unit;
     const double *u = J.uptr[unit];
     const SymDev R = J.uideg[unit] ? J.r1 : J.r0;
-    double *Gr = J.G + (size_t)unit * J.Gtot + cd.gbase + (size_t)row * cd.smax;
-    for (int j = 0; j < cd.smax; j++) {
+    double *Gu = J.G + (size_t)unit * J.Gtot;
+    for (int q = 0; q < kGTile / 256; q++) {
+        const int g = t * kGTile + q * 256 + (int)threadIdx.x;
+        if (g >= J.Gtot) break;
+        int co = 0;
+        while (co + 1 < J.nclasses && J.classes[co + 1].gbase <= g) co++;
+        const ClassDev &cd = J.classes[co];
+        const int off = g - cd.gbase;
+        const int row = off / cd.smax, j = off - row * cd.smax;
         double v = 0.0;
         if (j < cd.ncols) {
-            const int q = e * cd.ncols + j;
-            for (int p = cd.rptr[q]; p < cd.rptr[q + 1]; p++) {
+            int e = 0;
+            while (e + 1 < cd.nseg && cd.seg_row0[e + 1] <= row) e++;
+            const int vI = cd.seg_off[e] + (row - cd.seg_row0[e]);
+            const int qq = e * cd.ncols + j;
+            for (int p = cd.rptr[qq]; p < cd.rptr[qq + 1]; p++) {
                 const int cell = cd.rsI[p] * N + vI;
                 double cv = 0.0;
                 for (int x = R.rptr[cell]; x < R.rptr[cell + 1]; x++) cv += u[R.rcol[x]] * R.rval[x];
                 v += cd.rcoef[p] * cv;
             }
         }
-        Gr[j] = v;
+        Gu[g] = v;
+    }
+}
+
+// K1 from the sparse record map: tile = (unit group, 256 G elements); each
+// thread loads its element's map entries once and applies them to the
+// records of up to kGUnits units (writes are whole cache lines per unit).
+__global__ __launch_bounds__(256) void k_build_g_map(const GMapJob *__restrict__ jobs,
+                                                     const int *__restrict__ begin, int njobs,
+                                                     int ntiles)
+{
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
+    const GMapJob &J = jobs[find_job(begin, njobs, tile)];
+    const int local = tile - J.tile_begin;
+    const int ug = local / J.gtiles, gt = local - ug * J.gtiles;
+    const int g = gt * 256 + (int)threadIdx.x;
+    if (g >= J.Gtot) return;
+    int ix[kGMapMaxW];
+    double wv[kGMapMaxW];
+    int cur = -1;
+    const int u0 = ug * kGUnits, u1 = min(u0 + kGUnits, J.nunits);
+    for (int u = u0; u < u1; u++) {
+        const int d = J.uideg[u];
+        if (d != cur) {
+            const int *ip = J.idx[d] + g;
+            const double *wp = J.wt[d] + g;
+#pragma unroll
+            for (int w = 0; w < kGMapMaxW; w++)
+                if (w < J.W) {
+                    ix[w] = ip[(size_t)w * J.Gtot];
+                    wv[w] = wp[(size_t)w * J.Gtot];
+                }
+            cur = d;
+        }
+        const double *rec = J.uptr[u];
+        double v = 0.0;
+#pragma unroll
+        for (int w = 0; w < kGMapMaxW; w++)
+            if (w < J.W) v += wv[w] * rec[ix[w]];
+        J.G[(size_t)u * J.Gtot + g] = v;
     }
 }
 
 // K5 over jobs: H_co[m, (u, j)] = sum_k mu-block[m, k] * G[k, (u, j)] on the
 // matrix cores. K runs segment by segment (a tile never straddles two mu
-// blocks); the next K tile is fetched into registers while the current one
-// is multiplied.
+// blocks).
+template <int BM_, int BN_>
 __global__ __launch_bounds__(256) void k_halfls_jobs(int N, const HalfJob *__restrict__ jobs,
                                                      const int *__restrict__ tbegin, int njobs,
-                                                     const double *__restrict__ mu)
+                                                     int ntiles, const double *__restrict__ mu)
 {
-    __shared__ double As[BK][LDP];
-    __shared__ double Bs[BK][LDP];
-    const int wg = blockIdx.x;
-    const HalfJob &J = jobs[find_job(tbegin, njobs, wg)];
+    using T = Tile<BM_, BN_>;
+    __shared__ double As[BK * T::PA];
+    __shared__ double Bs[BK * T::PB];
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
+    const HalfJob &J = jobs[find_job(tbegin, njobs, tile)];
     const ClassDev &cd = J.cd;
-    const int local = wg - J.tile_begin;
-    const int m0 = (local % J.mt) * BM, n0 = (local / J.mt) * BN;
+    const int local = tile - J.tile_begin;
+    const int m0 = (local % J.mt) * BM_, n0 = (local / J.mt) * BN_;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int wm = (wave & 1) * (BM_ / 2), wn = (wave >> 1) * (BN_ / 2);
     const int ncolsTot = J.nunits * cd.smax;
-    // this thread's load coordinates: A row m, B column n, K rows kl0 + 4q
-    const int lm = tid & 63, kl0 = tid >> 6;
-    const bool mok = m0 + lm < cd.n;
-    const int n = n0 + lm;
+    // loads: A row am, K rows ak0 + q*(256/BM); B column bn, K rows bk0 + q*(256/BN)
+    const int am = tid % BM_, ak0 = tid / BM_;
+    const int bn = tid % BN_, bk0 = tid / BN_;
+    const bool mok = m0 + am < cd.n;
+    const int n = n0 + bn;
     const bool nok = n < ncolsTot;
     const double *gcol = J.G;
     if (nok) {
         const int u = n / cd.smax, jj = n - u * cd.smax;
         gcol = J.G + (size_t)u * J.Gtot + cd.gbase + jj;
     }
-    d4 acc[2][2];
-    for (int x = 0; x < 2; x++)
-        for (int y = 0; y < 2; y++) acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
-    double ra[4], rb[4];
+    d4 acc[T::MI][T::NI];
+#pragma unroll
+    for (int x = 0; x < T::MI; x++)
+#pragma unroll
+        for (int y = 0; y < T::NI; y++) acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
+    double ra[T::LA], rb[T::LB];
     int e = 0, kk0 = 0;
     auto fetch = [&](int se, int sk0) {
         const int len = min(BK, cd.seg_n[se] - sk0);
         const double *mp = mu + (size_t)cd.seg_a[se] * N * N +
-                           (size_t)(cd.seg_off[se] + sk0) * N + (cd.off + m0 + lm);
+                           (size_t)(cd.seg_off[se] + sk0) * N + (cd.off + m0 + am);
         const double *gp = gcol + (size_t)(cd.seg_row0[se] + sk0) * cd.smax;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int kl = kl0 + 4 * q;
+        for (int q = 0; q < T::LA; q++) {
+            const int kl = ak0 + q * (256 / BM_);
             ra[q] = (mok && kl < len) ? mp[(size_t)kl * N] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < T::LB; q++) {
+            const int kl = bk0 + q * (256 / BN_);
             rb[q] = (nok && kl < len) ? gp[(size_t)kl * cd.smax] : 0.0;
         }
     };
     if (cd.nseg > 0) fetch(0, 0);
     while (e < cd.nseg) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            As[kl0 + 4 * q][lm] = ra[q];
-            Bs[kl0 + 4 * q][lm] = rb[q];
-        }
+        for (int q = 0; q < T::LA; q++) As[(ak0 + q * (256 / BM_)) * T::PA + am] = ra[q];
+#pragma unroll
+        for (int q = 0; q < T::LB; q++) Bs[(bk0 + q * (256 / BN_)) * T::PB + bn] = rb[q];
         __syncthreads();
         kk0 += BK;
         if (kk0 >= cd.seg_n[e]) {
@@ -156,11 +230,14 @@ __global__ __launch_bounds__(256) void k_halfls_jobs(int N, const HalfJob *__res
             kk0 = 0;
         }
         if (e < cd.nseg) fetch(e, kk0);
-        mfma_tile(As, Bs, wm, wn, lane, acc);
+        mfma_step<BM_, BN_>(As, Bs, wm, wn, lane, acc);
         __syncthreads();
     }
-    for (int x = 0; x < 2; x++)
-        for (int y = 0; y < 2; y++)
+#pragma unroll
+    for (int x = 0; x < T::MI; x++)
+#pragma unroll
+        for (int y = 0; y < T::NI; y++)
+#pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int m = m0 + wm + 16 * x + 4 * r + (lane >> 4);
                 const int nn = n0 + wn + 16 * y + (lane & 15);
@@ -172,52 +249,87 @@ __global__ __launch_bounds__(256) void k_halfls_jobs(int N, const HalfJob *__res
 }
 
 // ---------------------------------------------------------------------------
-// K6
-__global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict__ args)
+// K6 over chunks: tile t of chunk c covers rows f0 = Lo + (local % rt) * BM
+// and columns n0 = (local / rt) * BN.
+template <int BM_, int BN_>
+__global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict__ args,
+                                                   const int *__restrict__ cbegin, int nchunks,
+                                                   int ntiles)
 {
-    __shared__ double As[BK][LDP];
-    __shared__ double Bs[BK][LDP];
+    using T = Tile<BM_, BN_>;
+    __shared__ double As[BK * T::PA];
+    __shared__ double Bs[BK * T::PB];
     __shared__ int s_need;
-    const CorrMMArgs a = args[blockIdx.z];
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
+    const int ci = find_job(cbegin, nchunks, tile);
+    const CorrMMArgs &a = args[ci];
+    const int local = tile - cbegin[ci];
+    const int rt = (a.Hi - a.Lo + BM_ - 1) / BM_;
+    const int f0 = a.Lo + (local % rt) * BM_;  // block-level index of the first row
+    const int n0 = (local / rt) * BN_;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int f0 = a.Lo + blockIdx.x * BM;  // block-level index of the first row
-    const int n0 = blockIdx.y * BN;
     const int ncols = a.nitems * a.ncb;
-    if (f0 >= a.Hi || n0 >= ncols) return;  // this chunk has fewer tiles
     // skip tiles outside every window of the tile's items
     if (tid == 0) {
         int need = 0;
-        const int i0 = n0 / a.ncb, i1 = min(a.nitems, (n0 + BN + a.ncb - 1) / a.ncb);
+        const int i0 = n0 / a.ncb, i1 = min(a.nitems, (n0 + BN_ + a.ncb - 1) / a.ncb);
         for (int i = i0; i < i1 && !need; i++)
-            if (a.item_lo[i] < f0 + BM && a.item_hi[i] > f0) need = 1;
+            if (a.item_lo[i] < f0 + BM_ && a.item_hi[i] > f0) need = 1;
         s_need = need;
     }
     __syncthreads();
     if (!s_need) return;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-    d4 acc[2][2];
-    for (int x = 0; x < 2; x++)
-        for (int y = 0; y < 2; y++) acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
-    // per-thread row pointers for the loads: row = tid / 4, 4 consecutive t
+    const int wm = (wave & 1) * (BM_ / 2), wn = (wave >> 1) * (BN_ / 2);
+    d4 acc[T::MI][T::NI];
+#pragma unroll
+    for (int x = 0; x < T::MI; x++)
+#pragma unroll
+        for (int y = 0; y < T::NI; y++) acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
+    // loads: 4 consecutive K of row lr + 64p (p < BM/64 for A, BN/64 for B)
     const int lr = tid >> 2, lt = (tid & 3) * 4;
-    const int fr = f0 + lr;
-    const double *Urow = nullptr;
-    if (fr < a.Hi) Urow = a.vec + (size_t)a.blk_col[fr] * a.nsize;
-    const int nr = n0 + lr;
-    const double *Hrow = nr < ncols ? a.Ht + (size_t)nr * a.nsize : nullptr;
-    for (int k0 = 0; k0 < a.nsize; k0 += BK) {
+    const double *Urow[BM_ / 64];
+    const double *Hrow[BN_ / 64];
+#pragma unroll
+    for (int p = 0; p < BM_ / 64; p++) {
+        const int fr = f0 + lr + 64 * p;
+        Urow[p] = fr < a.Hi ? a.vec + (size_t)a.blk_col[fr] * a.nsize : nullptr;
+    }
+#pragma unroll
+    for (int p = 0; p < BN_ / 64; p++) {
+        const int nr = n0 + lr + 64 * p;
+        Hrow[p] = nr < ncols ? a.Ht + (size_t)nr * a.nsize : nullptr;
+    }
+    double ra[BM_ / 64][4], rb[BN_ / 64][4];
+    auto fetch = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int t = k0 + lt + q;
-            As[lt + q][lr] = (Urow && t < a.nsize) ? Urow[t] : 0.0;
-            Bs[lt + q][lr] = (Hrow && t < a.nsize) ? Hrow[t] : 0.0;
+#pragma unroll
+            for (int p = 0; p < BM_ / 64; p++) ra[p][q] = (Urow[p] && t < a.nsize) ? Urow[p][t] : 0.0;
+#pragma unroll
+            for (int p = 0; p < BN_ / 64; p++) rb[p][q] = (Hrow[p] && t < a.nsize) ? Hrow[p][t] : 0.0;
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < a.nsize; k0 += BK) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+#pragma unroll
+            for (int p = 0; p < BM_ / 64; p++) As[(lt + q) * T::PA + lr + 64 * p] = ra[p][q];
+#pragma unroll
+            for (int p = 0; p < BN_ / 64; p++) Bs[(lt + q) * T::PB + lr + 64 * p] = rb[p][q];
         }
         __syncthreads();
-        mfma_tile(As, Bs, wm, wn, lane, acc);
+        if (k0 + BK < a.nsize) fetch(k0 + BK);
+        mfma_step<BM_, BN_>(As, Bs, wm, wn, lane, acc);
         __syncthreads();
     }
-    for (int x = 0; x < 2; x++)
-        for (int y = 0; y < 2; y++)
+#pragma unroll
+    for (int x = 0; x < T::MI; x++)
+#pragma unroll
+        for (int y = 0; y < T::NI; y++)
+#pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int f = f0 + wm + 16 * x + 4 * r + (lane >> 4);
                 const int n = n0 + wn + 16 * y + (lane & 15);
@@ -283,24 +395,49 @@ __global__ __launch_bounds__(256) void k_intensity(IntensArgs a)
 
 // ---------------------------------------------------------------------------
 
-void launch_halfls_jobs(hipStream_t st, int N, const HalfJob *d_jobs, const int *d_gbegin,
-                        const int *d_tbegin, int njobs, int gtiles, int ttiles, const double *mu)
+int tile_rows(int cfg) { return (cfg & 2) ? 128 : 64; }
+int tile_cols(int cfg) { return (cfg & 1) ? 128 : 64; }
+
+void launch_build_g_units(hipStream_t st, int N, const GBuildJob *d_jobs, const int *d_begin,
+                          int njobs, int ntiles)
 {
-    if (njobs <= 0) return;
-    if (gtiles > 0)
-        hipLaunchKernelGGL(k_build_g_jobs, dim3(gtiles), dim3(64), 0, st, N, d_jobs, d_gbegin,
-                           njobs);
-    if (ttiles > 0)
-        hipLaunchKernelGGL(k_halfls_jobs, dim3(ttiles), dim3(256), 0, st, N, d_jobs, d_tbegin,
-                           njobs, mu);
+    if (njobs <= 0 || ntiles <= 0) return;
+    hipLaunchKernelGGL(k_build_g_units, dim3(xcd_grid(ntiles)), dim3(256), 0, st, N, d_jobs,
+                       d_begin, njobs, ntiles);
 }
 
-void launch_corr_mfma(hipStream_t st, const CorrMMArgs *d_args, int nchunks, int max_rows,
-                      int max_cols)
+void launch_build_g_map(hipStream_t st, const GMapJob *d_jobs, const int *d_begin, int njobs,
+                        int ntiles)
 {
-    if (nchunks <= 0 || max_rows <= 0 || max_cols <= 0) return;
-    dim3 grid((max_rows + BM - 1) / BM, (max_cols + BN - 1) / BN, nchunks);
-    hipLaunchKernelGGL(k_corr_mfma, grid, dim3(256), 0, st, d_args);
+    if (njobs <= 0 || ntiles <= 0) return;
+    hipLaunchKernelGGL(k_build_g_map, dim3(xcd_grid(ntiles)), dim3(256), 0, st, d_jobs, d_begin,
+                       njobs, ntiles);
+}
+
+void launch_halfls_jobs(hipStream_t st, int cfg, int N, const HalfJob *d_jobs,
+                        const int *d_tbegin, int njobs, int ttiles, const double *mu)
+{
+    if (njobs <= 0 || ttiles <= 0) return;
+    const dim3 g(xcd_grid(ttiles)), b(256);
+    switch (cfg & 3) {
+    case 0: hipLaunchKernelGGL((k_halfls_jobs<64, 64>), g, b, 0, st, N, d_jobs, d_tbegin, njobs, ttiles, mu); break;
+    case 1: hipLaunchKernelGGL((k_halfls_jobs<64, 128>), g, b, 0, st, N, d_jobs, d_tbegin, njobs, ttiles, mu); break;
+    case 2: hipLaunchKernelGGL((k_halfls_jobs<128, 64>), g, b, 0, st, N, d_jobs, d_tbegin, njobs, ttiles, mu); break;
+    default: hipLaunchKernelGGL((k_halfls_jobs<128, 128>), g, b, 0, st, N, d_jobs, d_tbegin, njobs, ttiles, mu); break;
+    }
+}
+
+void launch_corr_mfma(hipStream_t st, int cfg, const CorrMMArgs *d_args, const int *d_cbegin,
+                      int nchunks, int ntiles)
+{
+    if (nchunks <= 0 || ntiles <= 0) return;
+    const dim3 g(xcd_grid(ntiles)), b(256);
+    switch (cfg & 3) {
+    case 0: hipLaunchKernelGGL((k_corr_mfma<64, 64>), g, b, 0, st, d_args, d_cbegin, nchunks, ntiles); break;
+    case 1: hipLaunchKernelGGL((k_corr_mfma<64, 128>), g, b, 0, st, d_args, d_cbegin, nchunks, ntiles); break;
+    case 2: hipLaunchKernelGGL((k_corr_mfma<128, 64>), g, b, 0, st, d_args, d_cbegin, nchunks, ntiles); break;
+    default: hipLaunchKernelGGL((k_corr_mfma<128, 128>), g, b, 0, st, d_args, d_cbegin, nchunks, ntiles); break;
+    }
 }
 
 void launch_intensity(hipStream_t st, const IntensArgs &a)

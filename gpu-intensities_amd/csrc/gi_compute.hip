@@ -383,13 +383,24 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
     unsigned char *d_jobs =
         (unsigned char *)slot_scratch(c, S, "jobs", jobs_bytes + 2 * 4 * (size_t)mjobs, &rc);
     if (rc) return rc;
-    CorrMMArgs *d_mm =
-        (CorrMMArgs *)slot_scratch(c, S, "mmargs", (size_t)mchunks * sizeof(CorrMMArgs), &rc);
+    unsigned char *d_mm = (unsigned char *)slot_scratch(
+        c, S, "mmargs", (size_t)mchunks * (sizeof(CorrMMArgs) + 4), &rc);
+    if (rc) return rc;
+    const int TM = tile_rows(c->tile_halfls), TN = tile_cols(c->tile_halfls);
+
+    GBuildJob *d_gjobs = (GBuildJob *)slot_scratch(
+        c, S, "gjobs", (size_t)mchunks * (sizeof(GBuildJob) + 4), &rc);
+    if (rc) return rc;
+    GMapJob *d_mjobs = (GMapJob *)slot_scratch(
+        c, S, "mjobs", (size_t)mchunks * (sizeof(GMapJob) + 4), &rc);
     if (rc) return rc;
 
     for (auto &rg : rounds) {
         std::vector<HalfJob> jobs;
-        std::vector<int> gbegin, tbegin;
+        std::vector<GBuildJob> gjobs;
+        std::vector<GMapJob> mjobs_v;
+        std::vector<int> gbegin, tbegin, mbegin;
+        int mt_tiles = 0;
         std::vector<CorrMMArgs> mm;
         std::vector<double *> Hc(rg.second - rg.first), Htc(rg.second - rg.first);
         int gt = 0, tt = 0;
@@ -403,44 +414,94 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             SymDev r1 = ndegI > 1 ? symdev(pI.sym[ch.gI * GI_MAXDEG + 1]) : r0;
             Hc[i - rg.first] = H + oh;
             Htc[i - rg.first] = Ht + oht;
+            if (ch.nunits > 0 && GP.Gtot > 0 && GP.gW <= kGMapMaxW && GP.d_gidx[0]) {
+                GMapJob mj;
+                for (int d = 0; d < 2; d++) {
+                    const int dd = d < ndegI ? d : 0;
+                    mj.idx[d] = GP.d_gidx[dd];
+                    mj.wt[d] = GP.d_gwt[dd];
+                }
+                mj.W = GP.gW;
+                mj.Gtot = GP.Gtot;
+                mj.nunits = ch.nunits;
+                mj.gtiles = (GP.Gtot + 255) / 256;
+                mj.tile_begin = mt_tiles;
+                mt_tiles += mj.gtiles * ((ch.nunits + kGUnits - 1) / kGUnits);
+                mj.uptr = (const double *const *)(d_meta + ch.off_uptr);
+                mj.uideg = (const int *)(d_meta + ch.off_uideg);
+                mj.G = G + og;
+                mjobs_v.push_back(mj);
+                mbegin.push_back(mj.tile_begin);
+            } else if (ch.nunits > 0 && GP.Gtot > 0) {
+                GBuildJob gj;
+                gj.classes = GP.d_cd;
+                gj.nclasses = (int)GP.oc.size();
+                gj.nunits = ch.nunits;
+                gj.Gtot = GP.Gtot;
+                gj.tiles_per_unit = (GP.Gtot + kGTile - 1) / kGTile;
+                gj.tile_begin = gt;
+                gt += gj.tiles_per_unit * ch.nunits;
+                gj.uptr = (const double *const *)(d_meta + ch.off_uptr);
+                gj.uideg = (const int *)(d_meta + ch.off_uideg);
+                gj.r0 = r0; gj.r1 = r1;
+                gj.G = G + og;
+                gjobs.push_back(gj);
+                gbegin.push_back(gj.tile_begin);
+            }
             if (ch.nunits > 0)
                 for (const OutClass &oc : GP.oc) {
                     HalfJob j;
                     j.cd = classdev(oc);
                     j.nunits = ch.nunits; j.Gtot = GP.Gtot; j.Htot = GP.Htot;
                     j.tile_begin = tt;
-                    j.mt = (oc.n + 63) / 64;
-                    tt += j.mt * ((ch.nunits * oc.smax + 63) / 64);
-                    j.gtile_begin = gt;
-                    j.gkt = (oc.K + 63) / 64;
-                    gt += oc.K > 0 ? j.gkt * ch.nunits : 0;
+                    j.mt = (oc.n + TM - 1) / TM;
+                    tt += j.mt * ((ch.nunits * oc.smax + TN - 1) / TN);
                     j.uptr = (const double *const *)(d_meta + ch.off_uptr);
                     j.uideg = (const int *)(d_meta + ch.off_uideg);
                     j.r0 = r0; j.r1 = r1;
                     j.G = G + og; j.H = H + oh;
                     jobs.push_back(j);
-                    gbegin.push_back(j.gtile_begin);
                     tbegin.push_back(j.tile_begin);
                 }
             og += need[i].g; oh += need[i].h; oht += need[i].ht;
         }
-        const int njobs = (int)jobs.size();
+        const int njobs = (int)jobs.size(), ngj = (int)gjobs.size(), nmj = (int)mjobs_v.size();
+        if (nmj > 0) {
+            const size_t mb = nmj * sizeof(GMapJob);
+            P.keep.emplace_back(mb + 4 * (size_t)nmj);
+            unsigned char *hb = P.keep.back().data();
+            std::memcpy(hb, mjobs_v.data(), mb);
+            std::memcpy(hb + mb, mbegin.data(), 4 * (size_t)nmj);
+            HIPCHK(hipMemcpyAsync(d_mjobs, hb, P.keep.back().size(), hipMemcpyHostToDevice,
+                                  S.stream));
+            TL_BEGIN("build_g");
+            launch_build_g_map(S.stream, d_mjobs, (const int *)((unsigned char *)d_mjobs + mb),
+                               nmj, mt_tiles);
+            TL_END();
+        }
+        if (ngj > 0) {
+            const size_t gb = ngj * sizeof(GBuildJob);
+            P.keep.emplace_back(gb + 4 * (size_t)ngj);
+            unsigned char *hb = P.keep.back().data();
+            std::memcpy(hb, gjobs.data(), gb);
+            std::memcpy(hb + gb, gbegin.data(), 4 * (size_t)ngj);
+            HIPCHK(hipMemcpyAsync(d_gjobs, hb, P.keep.back().size(), hipMemcpyHostToDevice,
+                                  S.stream));
+            TL_BEGIN("build_g");
+            launch_build_g_units(S.stream, N, d_gjobs,
+                                 (const int *)((unsigned char *)d_gjobs + gb), ngj, gt);
+            TL_END();
+        }
         if (njobs > 0) {
-            P.keep.emplace_back(jobs_bytes + 2 * 4 * (size_t)mjobs);
+            P.keep.emplace_back(jobs_bytes + 4 * (size_t)mjobs);
             unsigned char *hb = P.keep.back().data();
             std::memcpy(hb, jobs.data(), njobs * sizeof(HalfJob));
-            std::memcpy(hb + jobs_bytes, gbegin.data(), njobs * 4);
-            std::memcpy(hb + jobs_bytes + 4 * (size_t)mjobs, tbegin.data(), njobs * 4);
+            std::memcpy(hb + jobs_bytes, tbegin.data(), njobs * 4);
             HIPCHK(hipMemcpyAsync(d_jobs, hb, P.keep.back().size(), hipMemcpyHostToDevice, S.stream));
-            const int *d_gb = (const int *)(d_jobs + jobs_bytes);
-            const int *d_tb = (const int *)(d_jobs + jobs_bytes + 4 * (size_t)mjobs);
-            TL_BEGIN("build_g");
-            launch_halfls_jobs(S.stream, N, (const HalfJob *)d_jobs, d_gb, d_tb, njobs, gt, 0,
-                               c->d_mu_perm);
-            TL_END();
+            const int *d_tb = (const int *)(d_jobs + jobs_bytes);
             TL_BEGIN("half_ls_mfma");
-            launch_halfls_jobs(S.stream, N, (const HalfJob *)d_jobs, d_gb, d_tb, njobs, 0, tt,
-                               c->d_mu_perm);
+            launch_halfls_jobs(S.stream, c->tile_halfls, N, (const HalfJob *)d_jobs, d_tb, njobs,
+                               tt, c->d_mu_perm);
             TL_END();
         }
         for (size_t i = rg.first; i < rg.second; i++) {
@@ -478,17 +539,24 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             P.work[7] += ch.nunits * 24.0 * (double)N * N;
             P.work[8] += 2.0 * (double)(ch.Hi - ch.Lo) * ch.nitems * ndegI * ndegF * BF.nsize;
         }
-        int max_rows = 0, max_cols = 0;
-        for (auto &m : mm) {
-            max_rows = std::max(max_rows, m.Hi - m.Lo);
-            max_cols = std::max(max_cols, m.nitems * m.ncb);
+        // corr tiles: per chunk ceil(rows / CM) x ceil(cols / CN), flattened
+        const int CM = tile_rows(c->tile_corr), CN = tile_cols(c->tile_corr);
+        std::vector<int> cbegin(mm.size());
+        int ctiles = 0;
+        for (size_t i = 0; i < mm.size(); i++) {
+            cbegin[i] = ctiles;
+            const int rows = mm[i].Hi - mm[i].Lo, cols = mm[i].nitems * mm[i].ncb;
+            if (rows > 0 && cols > 0) ctiles += ((rows + CM - 1) / CM) * ((cols + CN - 1) / CN);
         }
-        P.keep.emplace_back(mm.size() * sizeof(CorrMMArgs));
-        std::memcpy(P.keep.back().data(), mm.data(), P.keep.back().size());
+        const size_t mm_bytes = mm.size() * sizeof(CorrMMArgs);
+        P.keep.emplace_back(mm_bytes + 4 * mm.size());
+        std::memcpy(P.keep.back().data(), mm.data(), mm_bytes);
+        std::memcpy(P.keep.back().data() + mm_bytes, cbegin.data(), 4 * mm.size());
         HIPCHK(hipMemcpyAsync(d_mm, P.keep.back().data(), P.keep.back().size(),
                               hipMemcpyHostToDevice, S.stream));
         TL_BEGIN("corr_mfma");
-        launch_corr_mfma(S.stream, d_mm, (int)mm.size(), max_rows, max_cols);
+        launch_corr_mfma(S.stream, c->tile_corr, (const CorrMMArgs *)d_mm,
+                         (const int *)((unsigned char *)d_mm + mm_bytes), (int)mm.size(), ctiles);
         TL_END();
         for (size_t i = rg.first; i < rg.second; i++) {
             const Chunk &ch = chunks[i];

@@ -78,6 +78,15 @@ struct GroupPlan {
     double macs = 0;                 // multiply-adds per unit
     std::vector<int> tcol[GI_MAXDEG];    // fold map: symmetrised index -> H offset
     int *d_tcol[GI_MAXDEG] = {nullptr, nullptr};
+    ClassDev *d_cd = nullptr;        // device copy of classdev() of every output class
+    // G as a sparse map of the lower eigenvector record, per lower component d:
+    // G[g] = sum_{w < gW} gwt[d][w*Gtot + g] * u[gidx[d][w*Gtot + g]]
+    // (the recipe of g composed with the symmetrisation map; padding has weight 0)
+    int gW = 0;
+    std::vector<int> gidx[GI_MAXDEG];
+    std::vector<double> gwt[GI_MAXDEG];
+    int *d_gidx[GI_MAXDEG] = {nullptr, nullptr};
+    double *d_gwt[GI_MAXDEG] = {nullptr, nullptr};
 };
 
 // Resident eigenvector block of one (J, symmetry) and its levels in energy order.
@@ -160,6 +169,9 @@ struct gi_ctx {
     // partition-function inputs on the device: energies and gns*ndeg*(2J+1)
     double *d_lv_energy = nullptr, *d_lv_weight = nullptr;
 
+    // tile configurations of k_halfls_jobs and k_corr_mfma (see tile_rows)
+    int tile_halfls = 3, tile_corr = 0;
+
     // statistics of the last gi_compute
     std::map<std::string, gi::KStat> kstats;
     std::vector<std::string> kstat_names;
@@ -219,6 +231,7 @@ int build_vib_classes(gi_ctx *c);
 int build_symmaps(gi_ctx *c, int jind);
 int build_coef(gi_ctx *c, int jindI, int jindF);
 int build_group(gi_ctx *c, int jindI, int gI, int jindF);
+int build_gmap(gi_ctx *c, GroupPlan &G);
 double three_j(int j1, int j2, int j3, int k1, int k2, int k3);
 void threej_table(int jmax, double *out);
 bool energy_filter_lower(const gi_ctx *c, int J, double e, const int *q);

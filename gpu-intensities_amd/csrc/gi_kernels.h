@@ -74,18 +74,51 @@ struct IntensArgs {
 struct HalfJob {
     ClassDev cd;
     int nunits, Gtot, Htot;
-    int tile_begin, gtile_begin;     // first workgroup of the product / G build
-    int mt, gkt;                     // product tiles along rows, G-build tiles along K
+    int tile_begin;                  // first tile of the product
+    int mt;                          // product tiles along rows
     const double *const *uptr;
     const int *uideg;
     SymDev r0, r1;
     double *G, *H;                   // this chunk's G and H
 };
 
-void launch_halfls_jobs(hipStream_t st, int N, const HalfJob *d_jobs, const int *d_gbegin,
-                        const int *d_tbegin, int njobs, int gtiles, int ttiles, const double *mu);
-void launch_corr_mfma(hipStream_t st, const CorrMMArgs *d_args, int nchunks, int max_rows,
-                      int max_cols);
+// G build of one chunk: each unit's G is Gtot contiguous doubles (its output
+// classes back to back at ClassDev::gbase), built in tiles of 2048.
+struct GBuildJob {
+    const ClassDev *classes;
+    int nclasses, nunits, Gtot, tiles_per_unit, tile_begin;
+    const double *const *uptr;
+    const int *uideg;
+    SymDev r0, r1;
+    double *G;
+};
+constexpr int kGTile = 2048;
+
+// G build from the sparse record map (GroupPlan::gidx/gwt): one thread per G
+// element, kGUnits units per workgroup reuse the map entries it loaded.
+struct GMapJob {
+    const int *idx[2];
+    const double *wt[2];
+    int W, Gtot, nunits, gtiles, tile_begin;
+    const double *const *uptr;
+    const int *uideg;
+    double *G;
+};
+constexpr int kGMapMaxW = 16;
+constexpr int kGUnits = 16;
+void launch_build_g_map(hipStream_t st, const GMapJob *d_jobs, const int *d_begin, int njobs,
+                        int ntiles);
+void launch_build_g_units(hipStream_t st, int N, const GBuildJob *d_jobs, const int *d_begin,
+                          int njobs, int ntiles);
+
+// Tile configurations of the two matrix-core kernels: cfg bit 1 selects 128
+// rows (else 64), bit 0 selects 128 columns (else 64).
+int tile_rows(int cfg);
+int tile_cols(int cfg);
+void launch_halfls_jobs(hipStream_t st, int cfg, int N, const HalfJob *d_jobs,
+                        const int *d_tbegin, int njobs, int ttiles, const double *mu);
+void launch_corr_mfma(hipStream_t st, int cfg, const CorrMMArgs *d_args, const int *d_cbegin,
+                      int nchunks, int ntiles);
 void launch_intensity(hipStream_t st, const IntensArgs &a);
 
 void launch_correlate_prim(hipStream_t st, int dimen, const int *r_grid, SymDev R,

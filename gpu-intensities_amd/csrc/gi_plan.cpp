@@ -362,6 +362,68 @@ int build_group(gi_ctx *c, int jindI, int gI, int jindF)
     return GI_OK;
 }
 
+// Sparse map from a lower eigenvector record to the unit's G: for every G
+// element (class, segment row, column) the recipe entries (slot_I, w) and,
+// for each, the symmetrisation-map row of cell (slot_I, v_I), composed and
+// merged per record index. Columns at or past ncols (padding) stay empty.
+int build_gmap(gi_ctx *c, GroupPlan &G)
+{
+    const int N = c->nvib;
+    const JPlan &pI = c->jp[G.jindI];
+    std::vector<std::vector<std::pair<int, double>>> ent;
+    int W = 0;
+    for (int d = 0; d < c->degen[G.gI]; d++) {
+        const SymMap &m = pI.sym[G.gI * GI_MAXDEG + d];
+        ent.assign(G.Gtot, {});
+        for (const OutClass &oc : G.oc) {
+            const int ncol = (int)oc.cols.size();
+            for (int e = 0; e < (int)oc.seg_a.size(); e++)
+                for (int row = oc.seg_row0[e]; row < oc.seg_row0[e + 1]; row++) {
+                    const int vI = oc.seg_off[e] + (row - oc.seg_row0[e]);
+                    for (int j = 0; j < ncol; j++) {
+                        auto &L = ent[(size_t)oc.gbase + (size_t)row * oc.smax + j];
+                        const int q = e * ncol + j;
+                        for (int p = oc.rptr[q]; p < oc.rptr[q + 1]; p++) {
+                            const long long cell = (long long)oc.rsI[p] * N + vI;
+                            for (int x = m.rptr[cell]; x < m.rptr[cell + 1]; x++) {
+                                const int idx = m.rcol[x];
+                                const double w = oc.rcoef[p] * m.rval[x];
+                                auto it = std::find_if(L.begin(), L.end(),
+                                                       [&](const std::pair<int, double> &t) {
+                                                           return t.first == idx;
+                                                       });
+                                if (it == L.end()) L.push_back({idx, w});
+                                else it->second += w;
+                            }
+                        }
+                        std::sort(L.begin(), L.end());
+                        W = std::max(W, (int)L.size());
+                    }
+                }
+        }
+        G.gidx[d].clear();
+        G.gwt[d].clear();
+        G.gidx[d].resize((size_t)std::max(W, 1) * G.Gtot, 0);
+        G.gwt[d].resize((size_t)std::max(W, 1) * G.Gtot, 0.0);
+        G.gW = std::max(G.gW, W);
+        // stash this component's entries; widths are fixed below
+        for (size_t g = 0; g < ent.size(); g++)
+            for (size_t w = 0; w < ent[g].size(); w++) {
+                G.gidx[d][w * G.Gtot + g] = ent[g][w].first;
+                G.gwt[d][w * G.Gtot + g] = ent[g][w].second;
+            }
+    }
+    // every component uses the final width gW (layout [w][g])
+    for (int d = 0; d < c->degen[G.gI]; d++) {
+        const size_t have = G.gidx[d].size() / std::max<size_t>(G.Gtot, 1);
+        if ((int)have < G.gW) {
+            G.gidx[d].resize((size_t)G.gW * G.Gtot, 0);
+            G.gwt[d].resize((size_t)G.gW * G.Gtot, 0.0);
+        }
+    }
+    return GI_OK;
+}
+
 // Coefficients of the half linestrength jvals[jindI] -> jvals[jindF]: the
 // factors of dipole_kernals.cu:194-223 that depend only on the rotational
 // labels, gathered per (slot_F, component).

@@ -177,6 +177,14 @@ int gi_kernel_stats(gi_ctx *ctx, int max, const char **names, double *total_ms,
  * bytes (24*Nvib^2 per unit), [8] correlation flops executed. */
 int gi_work_stats(gi_ctx *ctx, double *out, int n);
 
+/* Tile shapes of the two matrix-core kernels of GI_MODE_FP64_BATCHED (no
+ * reference counterpart: the reference has no batched form). cfg 0..3 =
+ * rows x columns 64x64, 64x128, 128x64, 128x128; -1 leaves a kernel as it
+ * is. Results are bitwise the same for every shape (the K order of each
+ * product does not depend on it). */
+int gi_set_tiles(gi_ctx *ctx, int halfls_cfg, int corr_cfg);
+int gi_get_tiles(gi_ctx *ctx, int *halfls_cfg, int *corr_cfg);
+
 /* Partition function over the resident level table:
  * Q(T) = sum_i gns(g_i) * ndeg_i * (2 J_i + 1) * exp(-(E_i - ZPE) * c2 / T),
  * c2 = planck * vellgt / boltz with the constants of cuda_host.cu:12-18.
