@@ -4,7 +4,7 @@
 // K5 (k_halfls_jobs): for one output class co and a chunk of units,
 //   H_co[m, (u, j)] = sum_k A[m, k] * B[k, (u, j)],
 //   A[m, k] = mu_{a(e)}[off_co + m, off_e + k - row0_e]   (column-major, ld N)
-//   B[k, (u, j)] = G[u*Gtot + gbase + k*smax + j]
+//   B[k, (u, j)] = G[u*Gtot + gbase_p + k*ncols + j]  (packed G)
 // so mu blocks are read once per chunk instead of once per lower level.
 //
 // K6 (k_corr_mfma): for one chunk of items (lower levels of one
@@ -102,12 +102,12 @@ __global__ __launch_bounds__(256) void k_build_g_units(int N, const GBuildJob *_
         const int g = t * kGTile + q * 256 + (int)threadIdx.x;
         if (g >= J.Gtot) break;
         int co = 0;
-        while (co + 1 < J.nclasses && J.classes[co + 1].gbase <= g) co++;
+        while (co + 1 < J.nclasses && J.classes[co + 1].gbase_p <= g) co++;
         const ClassDev &cd = J.classes[co];
-        const int off = g - cd.gbase;
-        const int row = off / cd.smax, j = off - row * cd.smax;
+        const int off = g - cd.gbase_p;
+        const int row = off / cd.ncols, j = off - row * cd.ncols;
         double v = 0.0;
-        if (j < cd.ncols) {
+        {
             int e = 0;
             while (e + 1 < cd.nseg && cd.seg_row0[e + 1] <= row) e++;
             const int vI = cd.seg_off[e] + (row - cd.seg_row0[e]);
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void k_halfls_jobs(int N, const HalfJob *__res
     const int m0 = (local % J.mt) * BM_, n0 = (local / J.mt) * BN_;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * (BM_ / 2), wn = (wave >> 1) * (BN_ / 2);
-    const int ncolsTot = J.nunits * cd.smax;
+    const int ncolsTot = J.nunits * cd.ncols;
     // loads: A row am, K rows ak0 + q*(256/BM); B column bn, K rows bk0 + q*(256/BN)
     const int am = tid % BM_, ak0 = tid / BM_;
     const int bn = tid % BN_, bk0 = tid / BN_;
@@ -191,8 +191,8 @@ __global__ __launch_bounds__(256) void k_halfls_jobs(int N, const HalfJob *__res
     const bool nok = n < ncolsTot;
     const double *gcol = J.G;
     if (nok) {
-        const int u = n / cd.smax, jj = n - u * cd.smax;
-        gcol = J.G + (size_t)u * J.Gtot + cd.gbase + jj;
+        const int u = n / cd.ncols, jj = n - u * cd.ncols;
+        gcol = J.G + (size_t)u * J.Gtot + cd.gbase_p + jj;
     }
     d4 acc[T::MI][T::NI];
 #pragma unroll
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(256) void k_halfls_jobs(int N, const HalfJob *__res
         const int len = min(BK, cd.seg_n[se] - sk0);
         const double *mp = mu + (size_t)cd.seg_a[se] * N * N +
                            (size_t)(cd.seg_off[se] + sk0) * N + (cd.off + m0 + am);
-        const double *gp = gcol + (size_t)(cd.seg_row0[se] + sk0) * cd.smax;
+        const double *gp = gcol + (size_t)(cd.seg_row0[se] + sk0) * cd.ncols;
 #pragma unroll
         for (int q = 0; q < T::LA; q++) {
             const int kl = ak0 + q * (256 / BM_);
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(256) void k_halfls_jobs(int N, const HalfJob *__res
 #pragma unroll
         for (int q = 0; q < T::LB; q++) {
             const int kl = bk0 + q * (256 / BN_);
-            rb[q] = (nok && kl < len) ? gp[(size_t)kl * cd.smax] : 0.0;
+            rb[q] = (nok && kl < len) ? gp[(size_t)kl * cd.ncols] : 0.0;
         }
     };
     if (cd.nseg > 0) fetch(0, 0);
@@ -242,8 +242,7 @@ __global__ __launch_bounds__(256) void k_halfls_jobs(int N, const HalfJob *__res
                 const int m = m0 + wm + 16 * x + 4 * r + (lane >> 4);
                 const int nn = n0 + wn + 16 * y + (lane & 15);
                 if (m >= cd.n || nn >= ncolsTot) continue;
-                const int u = nn / cd.smax, jj = nn - u * cd.smax;
-                if (jj >= cd.ncols) continue;
+                const int u = nn / cd.ncols, jj = nn - u * cd.ncols;
                 J.H[(size_t)u * J.Htot + cd.hbase + (size_t)jj * cd.n + m] = acc[x][y][r];
             }
 }
@@ -286,8 +285,10 @@ __global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict_
     for (int x = 0; x < T::MI; x++)
 #pragma unroll
         for (int y = 0; y < T::NI; y++) acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
-    // loads: 4 consecutive K of row lr + 64p (p < BM/64 for A, BN/64 for B)
-    const int lr = tid >> 2, lt = (tid & 3) * 4;
+    // loads: K = lt + 4q (q < 4) of row lr + 64p (p < BM/64 for A, BN/64 for B);
+    // the 4 lanes of a row read 32 contiguous bytes per load and store into 4
+    // consecutive LDS rows, which fall into alternate bank halves
+    const int lr = tid >> 2, lt = tid & 3;
     const double *Urow[BM_ / 64];
     const double *Hrow[BN_ / 64];
 #pragma unroll
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict_
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int t = k0 + lt + q;
+            const int t = k0 + lt + 4 * q;
 #pragma unroll
             for (int p = 0; p < BM_ / 64; p++) ra[p][q] = (Urow[p] && t < a.nsize) ? Urow[p][t] : 0.0;
 #pragma unroll
@@ -316,9 +317,9 @@ __global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict_
 #pragma unroll
         for (int q = 0; q < 4; q++) {
 #pragma unroll
-            for (int p = 0; p < BM_ / 64; p++) As[(lt + q) * T::PA + lr + 64 * p] = ra[p][q];
+            for (int p = 0; p < BM_ / 64; p++) As[(lt + 4 * q) * T::PA + lr + 64 * p] = ra[p][q];
 #pragma unroll
-            for (int p = 0; p < BN_ / 64; p++) Bs[(lt + q) * T::PB + lr + 64 * p] = rb[p][q];
+            for (int p = 0; p < BN_ / 64; p++) Bs[(lt + 4 * q) * T::PB + lr + 64 * p] = rb[p][q];
         }
         __syncthreads();
         if (k0 + BK < a.nsize) fetch(k0 + BK);

@@ -353,7 +353,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
         const int ncb = c->degen[ch.gI] * c->degen[ch.gF];
         const int nsizeF = c->blocks[ch.jindF * c->nrep + ch.gF].nsize;
-        need[i] = Need{(size_t)ch.nunits * GP.Gtot, (size_t)ch.nunits * GP.Htot,
+        need[i] = Need{(size_t)ch.nunits * GP.Gtot_p, (size_t)ch.nunits * GP.Htot,
                        (size_t)ch.nitems * ncb * nsizeF, ch.nunits > 0 ? (int)GP.oc.size() : 0};
     }
     std::vector<std::pair<size_t, size_t>> rounds;
@@ -414,7 +414,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             SymDev r1 = ndegI > 1 ? symdev(pI.sym[ch.gI * GI_MAXDEG + 1]) : r0;
             Hc[i - rg.first] = H + oh;
             Htc[i - rg.first] = Ht + oht;
-            if (ch.nunits > 0 && GP.Gtot > 0 && GP.gW <= kGMapMaxW && GP.d_gidx[0]) {
+            if (ch.nunits > 0 && GP.Gtot_p > 0 && GP.gW <= kGMapMaxW && GP.d_gidx[0]) {
                 GMapJob mj;
                 for (int d = 0; d < 2; d++) {
                     const int dd = d < ndegI ? d : 0;
@@ -422,9 +422,9 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                     mj.wt[d] = GP.d_gwt[dd];
                 }
                 mj.W = GP.gW;
-                mj.Gtot = GP.Gtot;
+                mj.Gtot = GP.Gtot_p;
                 mj.nunits = ch.nunits;
-                mj.gtiles = (GP.Gtot + 255) / 256;
+                mj.gtiles = (GP.Gtot_p + 255) / 256;
                 mj.tile_begin = mt_tiles;
                 mt_tiles += mj.gtiles * ((ch.nunits + kGUnits - 1) / kGUnits);
                 mj.uptr = (const double *const *)(d_meta + ch.off_uptr);
@@ -432,13 +432,13 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                 mj.G = G + og;
                 mjobs_v.push_back(mj);
                 mbegin.push_back(mj.tile_begin);
-            } else if (ch.nunits > 0 && GP.Gtot > 0) {
+            } else if (ch.nunits > 0 && GP.Gtot_p > 0) {
                 GBuildJob gj;
                 gj.classes = GP.d_cd;
                 gj.nclasses = (int)GP.oc.size();
                 gj.nunits = ch.nunits;
-                gj.Gtot = GP.Gtot;
-                gj.tiles_per_unit = (GP.Gtot + kGTile - 1) / kGTile;
+                gj.Gtot = GP.Gtot_p;
+                gj.tiles_per_unit = (GP.Gtot_p + kGTile - 1) / kGTile;
                 gj.tile_begin = gt;
                 gt += gj.tiles_per_unit * ch.nunits;
                 gj.uptr = (const double *const *)(d_meta + ch.off_uptr);
@@ -452,10 +452,10 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                 for (const OutClass &oc : GP.oc) {
                     HalfJob j;
                     j.cd = classdev(oc);
-                    j.nunits = ch.nunits; j.Gtot = GP.Gtot; j.Htot = GP.Htot;
+                    j.nunits = ch.nunits; j.Gtot = GP.Gtot_p; j.Htot = GP.Htot;
                     j.tile_begin = tt;
                     j.mt = (oc.n + TM - 1) / TM;
-                    tt += j.mt * ((ch.nunits * oc.smax + TN - 1) / TN);
+                    tt += j.mt * ((ch.nunits * (int)oc.cols.size() + TN - 1) / TN);
                     j.uptr = (const double *const *)(d_meta + ch.off_uptr);
                     j.uideg = (const int *)(d_meta + ch.off_uideg);
                     j.r0 = r0; j.r1 = r1;

@@ -64,6 +64,7 @@ struct OutClass {
     std::vector<int> rptr, rsI;      // recipe CSR over (segment, column)
     std::vector<double> rcoef;
     int hbase = 0, gbase = 0;        // offsets (doubles) in a unit's H and G buffers
+    int gbase_p = 0;                 // offset in the packed G (rows of ncols, batched mode)
     int *d_ints = nullptr;           // packed device copy: seg_a, seg_off, seg_n, seg_row0, rptr, rsI
     double *d_rcoef = nullptr;
 };
@@ -75,12 +76,13 @@ struct GroupPlan {
     int jindI = 0, gI = 0, jindF = 0, gF = 0;
     std::vector<OutClass> oc;
     int Htot = 0, Gtot = 0, smax_all = 0;
+    int Gtot_p = 0;                  // packed G per unit (batched mode)
     double macs = 0;                 // multiply-adds per unit
     std::vector<int> tcol[GI_MAXDEG];    // fold map: symmetrised index -> H offset
     int *d_tcol[GI_MAXDEG] = {nullptr, nullptr};
     ClassDev *d_cd = nullptr;        // device copy of classdev() of every output class
-    // G as a sparse map of the lower eigenvector record, per lower component d:
-    // G[g] = sum_{w < gW} gwt[d][w*Gtot + g] * u[gidx[d][w*Gtot + g]]
+    // packed G as a sparse map of the lower eigenvector record, per lower
+    // component d: G[g] = sum_{w < gW} gwt[d][w*Gtot_p + g] * u[gidx[d][w*Gtot_p + g]]
     // (the recipe of g composed with the symmetrisation map; padding has weight 0)
     int gW = 0;
     std::vector<int> gidx[GI_MAXDEG];
@@ -212,6 +214,7 @@ inline ClassDev classdev(const OutClass &oc)
     ClassDev d;
     d.off = oc.off; d.n = oc.n; d.ncols = (int)oc.cols.size(); d.smax = oc.smax; d.K = oc.K;
     d.nseg = (int)oc.seg_a.size(); d.hbase = oc.hbase; d.gbase = oc.gbase;
+    d.gbase_p = oc.gbase_p;
     const int ns = d.nseg;
     d.seg_a = oc.d_ints;
     d.seg_off = oc.d_ints + ns;

@@ -25,7 +25,7 @@ struct LineSoA {
 
 // Device view of one output class of a group plan (see OutClass).
 struct ClassDev {
-    int off, n, ncols, smax, K, nseg, hbase, gbase;
+    int off, n, ncols, smax, K, nseg, hbase, gbase, gbase_p;
     const int *seg_a, *seg_off, *seg_n, *seg_row0, *rptr, *rsI;
     const double *rcoef;
 };
@@ -82,8 +82,9 @@ struct HalfJob {
     double *G, *H;                   // this chunk's G and H
 };
 
-// G build of one chunk: each unit's G is Gtot contiguous doubles (its output
-// classes back to back at ClassDev::gbase), built in tiles of 2048.
+// G build of one chunk in the packed layout: each unit's G is Gtot contiguous
+// doubles, its output classes back to back at ClassDev::gbase_p as K rows of
+// ncols, built in tiles of 2048.
 struct GBuildJob {
     const ClassDev *classes;
     int nclasses, nunits, Gtot, tiles_per_unit, tile_begin;

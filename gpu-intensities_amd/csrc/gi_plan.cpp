@@ -333,6 +333,8 @@ int build_group(gi_ctx *c, int jindI, int gI, int jindF)
         G.Htot += oc.n * ncol;
         oc.gbase = G.Gtot;
         G.Gtot += oc.K * oc.smax;
+        oc.gbase_p = G.Gtot_p;
+        G.Gtot_p += oc.K * ncol;
         G.smax_all = std::max(G.smax_all, oc.smax);
         G.macs += (double)oc.n * oc.K * ncol;
         cls_oc[co] = (int)G.oc.size();
@@ -362,10 +364,11 @@ int build_group(gi_ctx *c, int jindI, int gI, int jindF)
     return GI_OK;
 }
 
-// Sparse map from a lower eigenvector record to the unit's G: for every G
-// element (class, segment row, column) the recipe entries (slot_I, w) and,
-// for each, the symmetrisation-map row of cell (slot_I, v_I), composed and
-// merged per record index. Columns at or past ncols (padding) stay empty.
+// Sparse map from a lower eigenvector record to the unit's packed G (rows of
+// ncols columns, classes back to back at gbase_p): for every G element
+// (class, segment row, column) the recipe entries (slot_I, w) and, for each,
+// the symmetrisation-map row of cell (slot_I, v_I), composed and merged per
+// record index.
 int build_gmap(gi_ctx *c, GroupPlan &G)
 {
     const int N = c->nvib;
@@ -374,14 +377,14 @@ int build_gmap(gi_ctx *c, GroupPlan &G)
     int W = 0;
     for (int d = 0; d < c->degen[G.gI]; d++) {
         const SymMap &m = pI.sym[G.gI * GI_MAXDEG + d];
-        ent.assign(G.Gtot, {});
+        ent.assign(G.Gtot_p, {});
         for (const OutClass &oc : G.oc) {
             const int ncol = (int)oc.cols.size();
             for (int e = 0; e < (int)oc.seg_a.size(); e++)
                 for (int row = oc.seg_row0[e]; row < oc.seg_row0[e + 1]; row++) {
                     const int vI = oc.seg_off[e] + (row - oc.seg_row0[e]);
                     for (int j = 0; j < ncol; j++) {
-                        auto &L = ent[(size_t)oc.gbase + (size_t)row * oc.smax + j];
+                        auto &L = ent[(size_t)oc.gbase_p + (size_t)row * ncol + j];
                         const int q = e * ncol + j;
                         for (int p = oc.rptr[q]; p < oc.rptr[q + 1]; p++) {
                             const long long cell = (long long)oc.rsI[p] * N + vI;
@@ -403,22 +406,22 @@ int build_gmap(gi_ctx *c, GroupPlan &G)
         }
         G.gidx[d].clear();
         G.gwt[d].clear();
-        G.gidx[d].resize((size_t)std::max(W, 1) * G.Gtot, 0);
-        G.gwt[d].resize((size_t)std::max(W, 1) * G.Gtot, 0.0);
+        G.gidx[d].resize((size_t)std::max(W, 1) * G.Gtot_p, 0);
+        G.gwt[d].resize((size_t)std::max(W, 1) * G.Gtot_p, 0.0);
         G.gW = std::max(G.gW, W);
         // stash this component's entries; widths are fixed below
         for (size_t g = 0; g < ent.size(); g++)
             for (size_t w = 0; w < ent[g].size(); w++) {
-                G.gidx[d][w * G.Gtot + g] = ent[g][w].first;
-                G.gwt[d][w * G.Gtot + g] = ent[g][w].second;
+                G.gidx[d][w * G.Gtot_p + g] = ent[g][w].first;
+                G.gwt[d][w * G.Gtot_p + g] = ent[g][w].second;
             }
     }
     // every component uses the final width gW (layout [w][g])
     for (int d = 0; d < c->degen[G.gI]; d++) {
-        const size_t have = G.gidx[d].size() / std::max<size_t>(G.Gtot, 1);
+        const size_t have = G.gidx[d].size() / std::max<size_t>(G.Gtot_p, 1);
         if ((int)have < G.gW) {
-            G.gidx[d].resize((size_t)G.gW * G.Gtot, 0);
-            G.gwt[d].resize((size_t)G.gW * G.Gtot, 0.0);
+            G.gidx[d].resize((size_t)G.gW * G.Gtot_p, 0);
+            G.gwt[d].resize((size_t)G.gW * G.Gtot_p, 0.0);
         }
     }
     return GI_OK;
