@@ -185,6 +185,11 @@ int gi_create(int device, gi_ctx **out)
         return GI_EHIP;
     }
     c->main_slot.stream = c->stream;
+    if (hipStreamCreate(&c->main_slot.cstream) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return GI_EHIP;
+    }
     *out = c;
     return GI_OK;
 }
@@ -200,6 +205,7 @@ void gi_destroy(gi_ctx *c)
         if (b.owned) (void)hipFree((void *)b.d_vec);
     if (c->mu_owned) (void)hipFree(c->d_mu);
     free_slot(c->main_slot);
+    (void)hipStreamDestroy(c->main_slot.cstream);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }

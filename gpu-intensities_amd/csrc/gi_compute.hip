@@ -144,11 +144,29 @@ struct Item {
 
 struct Chunk {
     int query, jindI, gI, jindF, gF, nitems, nunits, ncand, cb_mask, Lo, Hi;
+    long long cand_begin;  // first candidate slot (slots are assigned in chunk order)
     size_t off_uptr, off_uideg, off_iunit, off_cstart, off_lo, off_hi, off_level, off_eI, off_out;
+};
+
+// Candidate range of one batched round whose compacted lines are copied to
+// the caller while later rounds compute (gi_compute, batched mode).
+struct RoundOut {
+    long long cb = 0, ce = 0;
+    int nb = 0;
+    size_t cbase = 0, obase = 0;
+    hipEvent_t ev = nullptr;
 };
 
 // Everything one launch sequence needs; lives until its work has finished.
 struct Plan {
+    bool pipe = false;                             // per-round output (see RoundOut)
+    std::vector<RoundOut> rounds;
+    long long *rcounts = nullptr;                  // pinned, lines per round
+    ~Plan()
+    {
+        for (auto &r : rounds)
+            if (r.ev) (void)hipEventDestroy(r.ev);
+    }
     int mode = GI_MODE_FP64;
     std::vector<Query> q;
     std::vector<Item> items;
@@ -236,6 +254,7 @@ static int build_chunks(gi_ctx *c, Plan &P)
         const Item &it = P.items[i];
         groups[std::make_tuple(it.query, it.jindI, it.gI, it.jindF)].push_back(i);
     }
+    long long run = 0;  // candidate slots in chunk order
     auto put = [&](const void *src, size_t bytes) {
         const size_t o = (P.meta.size() + 255) & ~size_t(255);
         P.meta.resize(o + std::max<size_t>(bytes, 8));
@@ -277,7 +296,7 @@ static int build_chunks(gi_ctx *c, Plan &P)
                 Hi = std::max(Hi, it.hi);
                 lev[k] = it.level;
                 eI[k] = c->lv_energy[it.level];
-                outoff[k] = it.out_off;
+                outoff[k] = run + cnt;
                 cnt += it.hi - it.lo;
             }
             int mask = 0;
@@ -293,6 +312,8 @@ static int build_chunks(gi_ctx *c, Plan &P)
             ch.jindI = jindI; ch.gI = gI; ch.jindF = jindF; ch.gF = gF;
             ch.nitems = n_it; ch.nunits = (int)uideg.size(); ch.ncand = cnt; ch.cb_mask = mask;
             ch.Lo = Lo; ch.Hi = Hi;
+            ch.cand_begin = run;
+            run += cnt;
             ch.off_uptr = put(uptr.data(), uptr.size() * 8);
             ch.off_uideg = put(uideg.data(), uideg.size() * 4);
             ch.off_iunit = put(iunit.data(), iunit.size() * 4);
@@ -304,6 +325,18 @@ static int build_chunks(gi_ctx *c, Plan &P)
             ch.off_out = put(outoff.data(), outoff.size() * 8);
             P.chunks.push_back(ch);
         }
+    }
+    // each query's chunks are consecutive, so its candidates are one range
+    for (auto &q : P.q) q.cand_begin = q.cand_end = -1;
+    for (const Chunk &ch : P.chunks) {
+        Query &q = P.q[ch.query];
+        if (q.cand_begin < 0) q.cand_begin = ch.cand_begin;
+        q.cand_end = ch.cand_begin + ch.ncand;
+    }
+    long long pos = 0;
+    for (auto &q : P.q) {
+        if (q.cand_begin < 0) q.cand_begin = q.cand_end = pos;
+        pos = q.cand_end;
     }
     return GI_OK;
 }
@@ -329,6 +362,8 @@ static void fill_corr_args(gi_ctx *c, const Plan &P, const Chunk &ch, unsigned c
     a.thresh_S = q.thS; a.thresh_I = q.thI;
 }
 
+static LineSoA shift(LineSoA s, long long o);
+
 // Batched FP64 mode over rounds of chunks (see the file comment).
 static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timeline *tl,
                        int *flag, LineSoA cand)
@@ -343,6 +378,20 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         if (v > 0) kRoundBytes = (size_t)v;
     }
     const auto &chunks = P.chunks;
+    if (P.pipe) {
+        // at least kPipeRounds rounds (of >= 64 MB) so that the copy of one
+        // round's lines overlaps the next rounds' work (GI_PIPE_ROUNDS overrides)
+        int kPipeRounds = 2;
+        if (const char *env = std::getenv("GI_PIPE_ROUNDS")) kPipeRounds = std::max(1, std::atoi(env));
+        size_t total = 0;
+        for (const Chunk &ch : chunks) {
+            const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
+            total += 8 * ((size_t)ch.nunits * (GP.Gtot_p + GP.Htot) +
+                          (size_t)ch.nitems * c->degen[ch.gI] * c->degen[ch.gF] *
+                              c->blocks[ch.jindF * c->nrep + ch.gF].nsize);
+        }
+        kRoundBytes = std::min(kRoundBytes, std::max(total / kPipeRounds + 1, (size_t)64 << 20));
+    }
     struct Need {
         size_t g, h, ht;
         int njobs;
@@ -394,6 +443,33 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
     GMapJob *d_mjobs = (GMapJob *)slot_scratch(
         c, S, "mjobs", (size_t)mchunks * (sizeof(GMapJob) + 4), &rc);
     if (rc) return rc;
+    int *pc_counts = nullptr;
+    long long *pc_offs = nullptr;
+    if (P.pipe) {
+        size_t ncnt = 0, noff = 0;
+        P.rounds.resize(rounds.size());
+        for (size_t r = 0; r < rounds.size(); r++) {
+            RoundOut &o = P.rounds[r];
+            const Chunk &a = chunks[rounds[r].first], &b = chunks[rounds[r].second - 1];
+            o.cb = a.cand_begin;
+            o.ce = b.cand_begin + b.ncand;
+            o.nb = compact_blocks(o.ce - o.cb);
+            o.cbase = ncnt;
+            o.obase = noff;
+            ncnt += o.nb;
+            noff += o.nb + 1;
+            if (hipEventCreateWithFlags(&o.ev, hipEventDisableTiming) != hipSuccess)
+                return fail(c, GI_EHIP, "hipEventCreate");
+        }
+        pc_counts = (int *)slot_scratch(c, S, "rcounts", std::max<size_t>(ncnt, 1) * 4, &rc);
+        if (rc) return rc;
+        pc_offs = (long long *)slot_scratch(c, S, "roffs", std::max<size_t>(noff, 1) * 8, &rc);
+        if (rc) return rc;
+        P.rcounts = (long long *)slot_pinned(c, S, (P.q.size() + rounds.size()) * 8 + 64, &rc);
+        if (rc) return rc;
+        P.counts = P.rcounts + rounds.size();
+    }
+    size_t iround = 0;
 
     for (auto &rg : rounds) {
         std::vector<HalfJob> jobs;
@@ -578,6 +654,19 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             launch_intensity(S.stream, a);
             TL_END();
         }
+        if (P.pipe) {
+            RoundOut &o = P.rounds[iround];
+            if (o.nb > 0) {
+                TL_BEGIN("compact");
+                launch_compact(S.stream, o.ce - o.cb, flag + o.cb, pc_counts + o.cbase,
+                               pc_offs + o.obase, shift(cand, o.cb), shift(P.comp, o.cb));
+                TL_END();
+                HIPCHK(hipMemcpyAsync(P.rcounts + iround, pc_offs + o.obase + o.nb, 8,
+                                      hipMemcpyDeviceToHost, S.stream));
+            }
+            HIPCHK(hipEventRecord(o.ev, S.stream));
+        }
+        iround++;
     }
     return GI_OK;
 }
@@ -630,8 +719,10 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
     if (rc) return rc;
     long long *offs = (long long *)slot_scratch(c, S, "offs", std::max<size_t>(noff, 1) * 8, &rc);
     if (rc) return rc;
-    P.counts = (long long *)slot_pinned(c, S, P.q.size() * 8 + 64, &rc);
-    if (rc) return rc;
+    if (!P.pipe) {
+        P.counts = (long long *)slot_pinned(c, S, P.q.size() * 8 + 64, &rc);
+        if (rc) return rc;
+    }
     unsigned char *d_meta = nullptr;
     if (!P.meta.empty()) {
         d_meta = (unsigned char *)slot_scratch(c, S, "meta", P.meta.size(), &rc); if (rc) return rc;
@@ -712,6 +803,11 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
         }
     }
     HIPCHK(hipGetLastError());
+    if (P.pipe) {  // the rounds compacted their own ranges
+        for (auto &ch : P.chunks) P.work[0] += ch.nunits;
+        P.work[1] = (double)P.ncand;
+        return GI_OK;
+    }
     TL_BEGIN("compact");
     for (size_t k = 0; k < P.q.size(); k++) {
         const long long b = P.q[k].cand_begin, n = P.q[k].cand_end - b;
@@ -731,21 +827,30 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
     return GI_OK;
 }
 
+// Async copy of n compacted lines starting at candidate slot `from` to host
+// arrays at row `to`, on stream st.
+static int copy_range(gi_ctx *c, hipStream_t st, const LineSoA &comp, long long from, long long n,
+                      int *iI, int *iF, double *nu, double *Sv, double *A, double *I,
+                      double *amp, long long to)
+{
+    if (n <= 0) return GI_OK;
+    const LineSoA s = shift(comp, from);
+    HIPCHK(hipMemcpyAsync(iI + to, s.iI, n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(iF + to, s.iF, n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(nu + to, s.nu, n * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(Sv + to, s.S, n * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(A + to, s.A, n * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(I + to, s.I, n * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(amp + 4 * to, s.amp, n * 32, hipMemcpyDeviceToHost, st));
+    return GI_OK;
+}
+
 // Copy the lines of query k (count already known) to host arrays, async.
 static int copy_lines(gi_ctx *c, Slot &S, const Plan &P, size_t k, int *iI, int *iF, double *nu,
                       double *Sv, double *A, double *I, double *amp)
 {
-    const long long n = P.counts[k];
-    if (n <= 0) return GI_OK;
-    const LineSoA s = shift(P.comp, P.q[k].cand_begin);
-    HIPCHK(hipMemcpyAsync(iI, s.iI, n * 4, hipMemcpyDeviceToHost, S.stream));
-    HIPCHK(hipMemcpyAsync(iF, s.iF, n * 4, hipMemcpyDeviceToHost, S.stream));
-    HIPCHK(hipMemcpyAsync(nu, s.nu, n * 8, hipMemcpyDeviceToHost, S.stream));
-    HIPCHK(hipMemcpyAsync(Sv, s.S, n * 8, hipMemcpyDeviceToHost, S.stream));
-    HIPCHK(hipMemcpyAsync(A, s.A, n * 8, hipMemcpyDeviceToHost, S.stream));
-    HIPCHK(hipMemcpyAsync(I, s.I, n * 8, hipMemcpyDeviceToHost, S.stream));
-    HIPCHK(hipMemcpyAsync(amp, s.amp, n * 32, hipMemcpyDeviceToHost, S.stream));
-    return GI_OK;
+    return copy_range(c, S.stream, P.comp, P.q[k].cand_begin, P.counts[k], iI, iF, nu, Sv, A, I,
+                      amp, 0);
 }
 
 int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
@@ -774,23 +879,44 @@ int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
         out->n = 0;
         return GI_OK;
     }
+    const bool arrays = out->ilevelI && out->ilevelF && out->nu && out->S && out->A_einst &&
+                        out->I && out->amp;
+    // batched mode with room for every candidate: copy each round's lines
+    // while the next rounds compute
+    P.pipe = mode == GI_MODE_FP64_BATCHED && out->capacity >= P.ncand && arrays;
     if ((rc = build_chunks(c, P)) != GI_OK) return rc;
     Slot &S = c->main_slot;
+    long long nout = 0;
     {
         Timeline tl(S.stream);
         if ((rc = launch_plan(c, S, P, &tl)) != GI_OK) return rc;
+        if (P.pipe) {
+            for (size_t r = 0; r < P.rounds.size(); r++) {
+                const RoundOut &o = P.rounds[r];
+                HIPCHK(hipEventSynchronize(o.ev));
+                const long long n = o.nb > 0 ? P.rcounts[r] : 0;
+                if (n > 0) {
+                    HIPCHK(hipStreamWaitEvent(S.cstream, o.ev, 0));
+                    if ((rc = copy_range(c, S.cstream, P.comp, o.cb, n, out->ilevelI,
+                                         out->ilevelF, out->nu, out->S, out->A_einst, out->I,
+                                         out->amp, nout)) != GI_OK)
+                        return rc;
+                }
+                nout += n;
+            }
+            HIPCHK(hipStreamSynchronize(S.cstream));
+        }
         HIPCHK(hipStreamSynchronize(S.stream));
         tl.collect(c);
     }
     for (int i = 0; i < 10; i++) c->work[i] = P.work[i];
-    const long long nout = P.counts[0];
+    if (!P.pipe) nout = P.counts[0];
     c->work[2] = (double)nout;
     out->n = nout;
+    if (P.pipe) return GI_OK;
     if (nout > out->capacity) return fail(c, GI_ECAPACITY, "gi_compute: output capacity too small");
     if (nout > 0) {
-        if (!out->ilevelI || !out->ilevelF || !out->nu || !out->S || !out->A_einst || !out->I ||
-            !out->amp)
-            return fail(c, GI_EINVAL, "gi_compute: output arrays missing");
+        if (!arrays) return fail(c, GI_EINVAL, "gi_compute: output arrays missing");
         if ((rc = copy_lines(c, S, P, 0, out->ilevelI, out->ilevelF, out->nu, out->S,
                              out->A_einst, out->I, out->amp)) != GI_OK)
             return rc;

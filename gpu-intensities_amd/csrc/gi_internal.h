@@ -114,6 +114,7 @@ struct KStat {
 // alternates between two more so that one batch drains while the next runs.
 struct Slot {
     hipStream_t stream = nullptr;
+    hipStream_t cstream = nullptr;   // device-to-host copies that overlap `stream`
     std::map<std::string, std::pair<void *, size_t>> buf;
     void *pinned = nullptr;
     size_t pinned_bytes = 0;

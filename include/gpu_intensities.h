@@ -139,9 +139,13 @@ int gi_prepare(gi_ctx *ctx);
  * intensity, thresholds, compaction and copy to `out` (host memory).
  * Replaces the body of the lower-level loop, cuda_host.cu:1174-1488
  * (do_1st_half_ls :767-780, correlate + cublasDdot :1284-1322, intensity
- * :1376-1407). Rows come grouped by lower level; within a lower level, by
- * upper level energy. If out->capacity is too small, returns GI_ECAPACITY
- * with the required count in out->n and writes nothing. */
+ * :1376-1407). Row order: by lower block (J, symmetry) and upper J; within
+ * that, by lower level (in lower_ids order for GI_MODE_FP64, in energy order
+ * for GI_MODE_FP64_BATCHED); within a lower level, by upper level energy.
+ * If out->capacity is too small, returns GI_ECAPACITY with the required
+ * count in out->n and writes nothing. With capacity for every candidate
+ * (gi_count_candidates), the batched mode copies lines to `out` while it
+ * still computes; `out` should then be pinned host memory. */
 int gi_compute(gi_ctx *ctx, int mode, int nlower, const int *lower_ids, gi_lines *out);
 
 /* Upper bound of the rows gi_compute can emit for these lower levels. */

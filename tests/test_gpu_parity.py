@@ -101,12 +101,23 @@ def test_lower_subset_and_order(engine, c1_job, mode):
     got = engine.compute(ids, mode=mode)
     ref = oracle_py.OracleJob(c1_job).run(ids)
     compare(got, ref)
-    # rows grouped by lower level in request order
-    first = {}
-    for i, L in enumerate(got["ilevelI"]):
-        first.setdefault(int(L), i)
-    order = [L for L in ids if L in first]
-    assert sorted(first, key=first.get) == order
+    # rows of one (lower level, upper J) are contiguous, by upper energy
+    jF = c1_job.levels["jind"][got["ilevelF"]]
+    key = got["ilevelI"].astype(np.int64) * 2 + jF
+    change = np.nonzero(np.diff(key))[0] + 1
+    starts = np.concatenate([[0], change])
+    assert len(set(key[starts].tolist())) == len(starts)
+    eF = c1_job.levels["energy"][got["ilevelF"]]
+    same = key[1:] == key[:-1]
+    assert (eF[1:][same] >= eF[:-1][same]).all()
+    if mode == binding.GI_MODE_FP64:
+        # lower levels of one (lower block, upper J) in request order
+        pos = {int(L): i for i, L in enumerate(ids)}
+        blk = (c1_job.levels["jind"][got["ilevelI"]] * 8 + c1_job.levels["igamma"][got["ilevelI"]]) * 2 + jF
+        for b_ in np.unique(blk):
+            Ls = got["ilevelI"][blk == b_]
+            firsts = [int(L) for i, L in enumerate(Ls) if i == 0 or Ls[i - 1] != L]
+            assert [pos[L] for L in firsts] == sorted(pos[L] for L in firsts)
 
 
 @pytest.mark.parametrize("mode", MODES)
