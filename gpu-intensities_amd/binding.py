@@ -19,7 +19,8 @@ P_i, P_d = C.POINTER(C.c_int), C.POINTER(C.c_double)
 
 GI_MODE_FP64 = 0
 GI_MODE_FP64_BATCHED = 1
-MODES = {"fp64": GI_MODE_FP64, "fp64_batched": GI_MODE_FP64_BATCHED}
+GI_MODE_SPLIT = 2
+MODES = {"fp64": GI_MODE_FP64, "fp64_batched": GI_MODE_FP64_BATCHED, "split": GI_MODE_SPLIT}
 STATUS = {0: "GI_OK", 1: "GI_EINVAL", 2: "GI_EHIP", 3: "GI_ENOMEM", 4: "GI_ESTATE",
           5: "GI_ECAPACITY", 6: "GI_EBUDGET", 7: "GI_ETIMEOUT"}
 GI_EBUDGET, GI_ETIMEOUT = 6, 7
@@ -30,7 +31,8 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_params", "gi_uplo
            "gi_count_candidates", "gi_half_ls", "gi_correlate", "gi_threej_table",
            "gi_kernel_stats", "gi_work_stats", "gi_partition_function", "gi_serve_start",
            "gi_serve_submit", "gi_serve_wait", "gi_serve_fetch", "gi_serve_stats",
-           "gi_serve_stop", "gi_set_tiles", "gi_get_tiles"]
+           "gi_serve_stop", "gi_set_tiles", "gi_get_tiles", "gi_set_split_digits",
+           "gi_prepare_split"]
 
 
 class GIError(RuntimeError):
@@ -99,6 +101,8 @@ def lib():
         L.gi_serve_stats.argtypes = [vp, C.POINTER(cll), C.POINTER(cll), C.POINTER(cll)]
         L.gi_serve_stop.argtypes = [vp]
         L.gi_set_tiles.argtypes = [vp, ci, ci]
+        L.gi_set_split_digits.argtypes = [vp, ci]
+        L.gi_prepare_split.argtypes = [vp]
         L.gi_get_tiles.argtypes = [vp, P_i, P_i]
         _LIB = L
     return _LIB
@@ -270,6 +274,12 @@ class Engine:
     def set_tiles(self, halfls=-1, corr=-1):
         """Tile shapes of the batched kernels: 0..3 = 64x64, 64x128, 128x64, 128x128."""
         self._chk(lib().gi_set_tiles(self.h, halfls, corr), "gi_set_tiles")
+
+    def set_split_digits(self, digits):
+        self._chk(lib().gi_set_split_digits(self.h, digits), "gi_set_split_digits")
+
+    def prepare_split(self):
+        self._chk(lib().gi_prepare_split(self.h), "gi_prepare_split")
 
     def get_tiles(self):
         a, b = ci(), ci()

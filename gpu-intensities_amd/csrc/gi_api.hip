@@ -125,8 +125,68 @@ static void free_sym(SymMap &m)
     m.d_rval = m.d_tval = nullptr;
 }
 
+void free_split(gi_ctx *c)
+{
+    (void)hipFree(c->d_mud);
+    (void)hipFree(c->d_mue);
+    c->d_mud = nullptr;
+    c->d_mue = nullptr;
+    for (auto &b : c->blocks) {
+        (void)hipFree(b.d_ud); (void)hipFree(b.d_ue); (void)hipFree(b.d_umax);
+        b.d_ud = nullptr;
+        b.d_ue = nullptr;
+        b.d_umax = nullptr;
+        b.Kp = 0;
+        b.uplane = 0;
+    }
+    c->split_ready = 0;
+}
+
+// Digit planes of mu and of every resident eigenvector block for the split
+// mode (built once per digit count; freed with the plan).
+int prepare_split(gi_ctx *c)
+{
+    if (!c->prepared) return fail(c, GI_ESTATE, "split mode before gi_prepare");
+    if (c->split_ready == c->split_S) return GI_OK;
+    HIPCHK(hipSetDevice(c->device));
+    free_split(c);
+    const int SD = c->split_S, N = c->nvib;
+    // int32 sums of digit products stay exact while 4096 * S * K < 2^31
+    const long long kmax = (1LL << 31) / (4096LL * SD) - 64;
+    if (c->kpad_all > kmax) return fail(c, GI_EINVAL, "split mode: vibrational basis too large");
+    int rc;
+    std::vector<int> vinv(c->kpad_all, -1);
+    for (size_t k = 0; k < c->vcls_off.size(); k++)
+        for (int i = 0; i < c->vcls_n[k]; i++) vinv[c->vpad_off[k] + i] = c->vcls_off[k] + i;
+    int *d_vinv = nullptr;
+    if ((rc = upload(c, &d_vinv, vinv)) != GI_OK) return rc;
+    c->muplane = 3LL * N * c->kpad_all;
+    HIPCHK(hipMalloc((void **)&c->d_mud, (size_t)SD * c->muplane));
+    HIPCHK(hipMalloc((void **)&c->d_mue, sizeof(int) * N));
+    launch_split_mu(c->stream, SD, N, c->kpad_all, c->d_mu_perm, d_vinv, c->d_mud, c->muplane,
+                    c->d_mue);
+    HIPCHK(hipGetLastError());
+    for (auto &b : c->blocks) {
+        if (b.nrec <= 0 || b.nsize <= 0 || !b.d_vec) continue;
+        b.Kp = (b.nsize + 63) / 64 * 64;
+        if (b.Kp > kmax) return fail(c, GI_EINVAL, "split mode: eigenvector block too long");
+        b.uplane = (long long)b.nrec * b.Kp;
+        HIPCHK(hipMalloc((void **)&b.d_ud, (size_t)SD * b.uplane));
+        HIPCHK(hipMalloc((void **)&b.d_ue, sizeof(int) * b.nrec));
+        HIPCHK(hipMalloc((void **)&b.d_umax, sizeof(double) * b.nrec));
+        launch_split_rows(c->stream, SD, b.d_vec, b.nsize, b.nrec, b.nsize, b.Kp, b.d_ud, b.uplane,
+                          b.d_ue, b.d_umax);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    (void)hipFree(d_vinv);
+    c->split_ready = SD;
+    return GI_OK;
+}
+
 static void free_plan(gi_ctx *c)
 {
+    free_split(c);
     for (int j = 0; j < 2; j++) {
         for (auto &m : c->jp[j].sym) free_sym(m);
         (void)hipFree(c->jp[j].d_r_grid);
@@ -148,12 +208,16 @@ static void free_plan(gi_ctx *c)
         for (auto &oc : G.oc) {
             (void)hipFree(oc.d_ints);
             (void)hipFree(oc.d_rcoef);
+            (void)hipFree(oc.d_segpad);
         }
         for (int d = 0; d < GI_MAXDEG; d++) (void)hipFree(G.d_tcol[d]);
         (void)hipFree(G.d_cd);
         for (int d = 0; d < GI_MAXDEG; d++) {
             (void)hipFree(G.d_gidx[d]);
             (void)hipFree(G.d_gwt[d]);
+            (void)hipFree(G.d_gidxB[d]);
+            (void)hipFree(G.d_gwtB[d]);
+            (void)hipFree(G.d_gbnd[d]);
         }
     }
     c->groups.clear();
@@ -454,18 +518,25 @@ int gi_prepare(gi_ctx *c)
                 }
                 for (int d = 0; d < c->degen[G.gF]; d++)
                     if ((rc = upload(c, &G.d_tcol[d], G.tcol[d])) != GI_OK) return rc;
+                if ((rc = build_gmap(c, G)) != GI_OK) return rc;
+                for (auto &oc : G.oc)
+                    if ((rc = upload(c, &oc.d_segpad, oc.seg_pad)) != GI_OK) return rc;
                 std::vector<ClassDev> cds;
                 for (auto &oc : G.oc) cds.push_back(classdev(oc));
                 if ((rc = upload(c, &G.d_cd, cds)) != GI_OK) return rc;
-                if ((rc = build_gmap(c, G)) != GI_OK) return rc;
                 if (G.gW <= kGMapMaxW)
                     for (int d = 0; d < c->degen[G.gI]; d++) {
                         if ((rc = upload(c, &G.d_gidx[d], G.gidx[d])) != GI_OK) return rc;
                         if ((rc = upload(c, &G.d_gwt[d], G.gwt[d])) != GI_OK) return rc;
+                        if ((rc = upload(c, &G.d_gidxB[d], G.gidxB[d])) != GI_OK) return rc;
+                        if ((rc = upload(c, &G.d_gwtB[d], G.gwtB[d])) != GI_OK) return rc;
+                        if ((rc = upload(c, &G.d_gbnd[d], G.gbnd[d])) != GI_OK) return rc;
                     }
                 for (int d = 0; d < GI_MAXDEG; d++) {  // host copies are not needed again
                     std::vector<int>().swap(G.gidx[d]);
                     std::vector<double>().swap(G.gwt[d]);
+                    std::vector<int>().swap(G.gidxB[d]);
+                    std::vector<double>().swap(G.gwtB[d]);
                 }
             }
     // level filters and per-block energy order
@@ -586,6 +657,25 @@ int gi_kernel_stats(gi_ctx *c, int max, const char **names, double *total_ms, in
         if (launches) launches[i] = s.launches;
     }
     return GI_OK;
+}
+
+int gi_set_split_digits(gi_ctx *c, int digits)
+{
+    if (!c || digits < 3 || digits > 6) return GI_EINVAL;
+    if (serving(c)) return GI_ESTATE;
+    if (digits != c->split_S) {
+        HIPCHK(hipSetDevice(c->device));
+        free_split(c);
+        c->split_S = digits;
+    }
+    return GI_OK;
+}
+
+int gi_prepare_split(gi_ctx *c)
+{
+    if (!c) return GI_EINVAL;
+    if (serving(c)) return GI_ESTATE;
+    return prepare_split(c);
 }
 
 int gi_set_tiles(gi_ctx *c, int hl, int cr)

@@ -144,6 +144,7 @@ struct Item {
 
 struct Chunk {
     int query, jindI, gI, jindF, gF, nitems, nunits, ncand, cb_mask, Lo, Hi;
+    size_t off_urec;       // per unit: record of the lower level in its block
     long long cand_begin;  // first candidate slot (slots are assigned in chunk order)
     size_t off_uptr, off_uideg, off_iunit, off_cstart, off_lo, off_hi, off_level, off_eI, off_out;
 };
@@ -247,7 +248,7 @@ static void items_request(const gi_ctx *c, const gi_request &r, int query, Plan 
 // Group items and cut chunks; unit pointers become device addresses.
 static int build_chunks(gi_ctx *c, Plan &P)
 {
-    const bool batched = P.mode == GI_MODE_FP64_BATCHED;
+    const bool batched = P.mode != GI_MODE_FP64;  // both matrix-core modes
     const int kChunk = 256;
     std::map<std::tuple<int, int, int, int>, std::vector<int>> groups;
     for (int i = 0; i < (int)P.items.size(); i++) {
@@ -277,8 +278,8 @@ static int build_chunks(gi_ctx *c, Plan &P)
         for (size_t s0 = 0; s0 < its.size(); s0 += kChunk) {
             const int n_it = (int)std::min<size_t>(kChunk, its.size() - s0);
             std::vector<long long> uptr, outoff(n_it);
-            std::vector<int> uideg, iunit(n_it * GI_MAXDEG, -1), cstart(n_it), lo(n_it), hi(n_it),
-                lev(n_it);
+            std::vector<int> uideg, urec, iunit(n_it * GI_MAXDEG, -1), cstart(n_it), lo(n_it),
+                hi(n_it), lev(n_it);
             std::vector<double> eI(n_it);
             int cnt = 0, Lo = 1 << 30, Hi = 0;
             for (int k = 0; k < n_it; k++) {
@@ -287,6 +288,7 @@ static int build_chunks(gi_ctx *c, Plan &P)
                     if (!degeneracy_filter(c, gI, gF, ideg, 0)) continue;
                     iunit[k * GI_MAXDEG + ideg] = (int)uideg.size();
                     uideg.push_back(ideg);
+                    urec.push_back(c->lv_irec0[it.level]);
                     uptr.push_back((long long)(BI.d_vec + (size_t)c->lv_irec0[it.level] * BI.nsize));
                 }
                 cstart[k] = cnt;
@@ -316,6 +318,7 @@ static int build_chunks(gi_ctx *c, Plan &P)
             run += cnt;
             ch.off_uptr = put(uptr.data(), uptr.size() * 8);
             ch.off_uideg = put(uideg.data(), uideg.size() * 4);
+            ch.off_urec = put(urec.data(), urec.size() * 4);
             ch.off_iunit = put(iunit.data(), iunit.size() * 4);
             ch.off_cstart = put(cstart.data(), cstart.size() * 4);
             ch.off_lo = put(lo.data(), lo.size() * 4);
@@ -364,6 +367,116 @@ static void fill_corr_args(gi_ctx *c, const Plan &P, const Chunk &ch, unsigned c
 
 static LineSoA shift(LineSoA s, long long o);
 
+// Consecutive chunks grouped into rounds of at most kRoundBytes of device
+// scratch (GI_BATCH_BYTES overrides); with per-round output at least
+// kPipeRounds rounds of >= 64 MB (GI_PIPE_ROUNDS overrides).
+static std::vector<std::pair<size_t, size_t>> plan_rounds(const Plan &P,
+                                                          const std::vector<size_t> &bytes)
+{
+    size_t kRoundBytes = (size_t)12 << 30;
+    if (const char *env = std::getenv("GI_BATCH_BYTES")) {
+        const long long v = std::atoll(env);
+        if (v > 0) kRoundBytes = (size_t)v;
+    }
+    if (P.pipe) {
+        int kPipeRounds = 2;
+        if (const char *env = std::getenv("GI_PIPE_ROUNDS")) kPipeRounds = std::max(1, std::atoi(env));
+        size_t total = 0;
+        for (size_t b : bytes) total += b;
+        kRoundBytes = std::min(kRoundBytes, std::max(total / kPipeRounds + 1, (size_t)64 << 20));
+    }
+    std::vector<std::pair<size_t, size_t>> rounds;
+    size_t start = 0, acc = 0;
+    for (size_t i = 0; i < bytes.size(); i++) {
+        if (i > start && acc + bytes[i] > kRoundBytes) {
+            rounds.push_back({start, i});
+            start = i;
+            acc = 0;
+        }
+        acc += bytes[i];
+    }
+    if (start < bytes.size()) rounds.push_back({start, bytes.size()});
+    return rounds;
+}
+
+// Events and compaction scratch of the per-round output.
+static int setup_pipe(gi_ctx *c, Slot &S, Plan &P,
+                      const std::vector<std::pair<size_t, size_t>> &rounds, int **pc_counts,
+                      long long **pc_offs)
+{
+    int rc = GI_OK;
+    *pc_counts = nullptr;
+    *pc_offs = nullptr;
+    if (!P.pipe) return GI_OK;
+    size_t ncnt = 0, noff = 0;
+    P.rounds.resize(rounds.size());
+    for (size_t r = 0; r < rounds.size(); r++) {
+        RoundOut &o = P.rounds[r];
+        const Chunk &a = P.chunks[rounds[r].first], &b = P.chunks[rounds[r].second - 1];
+        o.cb = a.cand_begin;
+        o.ce = b.cand_begin + b.ncand;
+        o.nb = compact_blocks(o.ce - o.cb);
+        o.cbase = ncnt;
+        o.obase = noff;
+        ncnt += o.nb;
+        noff += o.nb + 1;
+        if (hipEventCreateWithFlags(&o.ev, hipEventDisableTiming) != hipSuccess)
+            return fail(c, GI_EHIP, "hipEventCreate");
+    }
+    *pc_counts = (int *)slot_scratch(c, S, "rcounts", std::max<size_t>(ncnt, 1) * 4, &rc);
+    if (rc) return rc;
+    *pc_offs = (long long *)slot_scratch(c, S, "roffs", std::max<size_t>(noff, 1) * 8, &rc);
+    if (rc) return rc;
+    P.rcounts = (long long *)slot_pinned(c, S, (P.q.size() + rounds.size()) * 8 + 64, &rc);
+    if (rc) return rc;
+    P.counts = P.rcounts + rounds.size();
+    return GI_OK;
+}
+
+// Compaction of round r's candidates, its line count to pinned memory and
+// the event the host waits on before copying the round's lines.
+static int round_compact(gi_ctx *c, Slot &S, Plan &P, size_t r, int *flag, LineSoA cand,
+                         int *pc_counts, long long *pc_offs, Timeline *tl)
+{
+    if (!P.pipe) return GI_OK;
+    RoundOut &o = P.rounds[r];
+    if (o.nb > 0) {
+        TL_BEGIN("compact");
+        launch_compact(S.stream, o.ce - o.cb, flag + o.cb, pc_counts + o.cbase, pc_offs + o.obase,
+                       shift(cand, o.cb), shift(P.comp, o.cb));
+        TL_END();
+        HIPCHK(hipMemcpyAsync(P.rcounts + r, pc_offs + o.obase + o.nb, 8, hipMemcpyDeviceToHost,
+                              S.stream));
+    }
+    HIPCHK(hipEventRecord(o.ev, S.stream));
+    return GI_OK;
+}
+
+// Intensity step of one chunk from its stored amplitudes.
+static int chunk_intensity(gi_ctx *c, Slot &S, const Plan &P, const Chunk &ch,
+                           unsigned char *d_meta, const Consts &K, int *flag, LineSoA cand,
+                           Timeline *tl)
+{
+    CorrArgs ca;
+    fill_corr_args(c, P, ch, d_meta, K, ca);
+    IntensArgs a;
+    a.ncand = ca.ncand; a.nitems = ca.nitems; a.ndegI = ca.ndegI; a.ndegF = ca.ndegF;
+    a.cb_mask = ca.cb_mask; a.reduced = ca.reduced; a.jI = ca.jI; a.jF = ca.jF;
+    a.item_cand_start = ca.item_cand_start; a.item_lo = ca.item_lo;
+    a.item_level = ca.item_level; a.item_out_off = ca.item_out_off;
+    a.item_energy = ca.item_energy;
+    a.blk_lev = ca.blk_lev; a.blk_en = ca.blk_en; a.blk_upok = ca.blk_upok;
+    a.gnsI = ca.gnsI; a.zpe = ca.zpe; a.beta = ca.beta; a.q_stat = ca.q_stat;
+    a.A_coef_s_1 = ca.A_coef_s_1; a.intens_cm_mol = ca.intens_cm_mol;
+    a.thresh_S = ca.thresh_S; a.thresh_I = ca.thresh_I;
+    a.amp = cand.amp; a.flag = flag; a.out_iI = cand.iI; a.out_iF = cand.iF;
+    a.out_nu = cand.nu; a.out_S = cand.S; a.out_A = cand.A; a.out_I = cand.I;
+    TL_BEGIN("intensity");
+    launch_intensity(S.stream, a);
+    TL_END();
+    return GI_OK;
+}
+
 // Batched FP64 mode over rounds of chunks (see the file comment).
 static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timeline *tl,
                        int *flag, LineSoA cand)
@@ -371,32 +484,13 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
     const int N = c->nvib;
     const Consts K;
     int rc = GI_OK;
-    // device memory for one round's G, H and folded vectors (GI_BATCH_BYTES overrides)
-    size_t kRoundBytes = (size_t)12 << 30;
-    if (const char *env = std::getenv("GI_BATCH_BYTES")) {
-        const long long v = std::atoll(env);
-        if (v > 0) kRoundBytes = (size_t)v;
-    }
     const auto &chunks = P.chunks;
-    if (P.pipe) {
-        // at least kPipeRounds rounds (of >= 64 MB) so that the copy of one
-        // round's lines overlaps the next rounds' work (GI_PIPE_ROUNDS overrides)
-        int kPipeRounds = 2;
-        if (const char *env = std::getenv("GI_PIPE_ROUNDS")) kPipeRounds = std::max(1, std::atoi(env));
-        size_t total = 0;
-        for (const Chunk &ch : chunks) {
-            const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
-            total += 8 * ((size_t)ch.nunits * (GP.Gtot_p + GP.Htot) +
-                          (size_t)ch.nitems * c->degen[ch.gI] * c->degen[ch.gF] *
-                              c->blocks[ch.jindF * c->nrep + ch.gF].nsize);
-        }
-        kRoundBytes = std::min(kRoundBytes, std::max(total / kPipeRounds + 1, (size_t)64 << 20));
-    }
     struct Need {
         size_t g, h, ht;
         int njobs;
     };
     std::vector<Need> need(chunks.size());
+    std::vector<size_t> bytes(chunks.size());
     for (size_t i = 0; i < chunks.size(); i++) {
         const Chunk &ch = chunks[i];
         const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
@@ -404,25 +498,21 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         const int nsizeF = c->blocks[ch.jindF * c->nrep + ch.gF].nsize;
         need[i] = Need{(size_t)ch.nunits * GP.Gtot_p, (size_t)ch.nunits * GP.Htot,
                        (size_t)ch.nitems * ncb * nsizeF, ch.nunits > 0 ? (int)GP.oc.size() : 0};
+        bytes[i] = 8 * (need[i].g + need[i].h + need[i].ht);
     }
-    std::vector<std::pair<size_t, size_t>> rounds;
-    size_t start = 0, acc = 0, mg = 1, mh = 1, mht = 1, sg = 0, sh = 0, sht = 0;
-    int mjobs = 1, sjobs = 0, mchunks = 1;
-    for (size_t i = 0; i <= chunks.size(); i++) {
-        const size_t bytes = i < chunks.size() ? 8 * (need[i].g + need[i].h + need[i].ht) : 0;
-        if (i == chunks.size() || (i > start && acc + bytes > kRoundBytes)) {
-            if (i > start) rounds.push_back({start, i});
-            mg = std::max(mg, sg); mh = std::max(mh, sh); mht = std::max(mht, sht);
-            mjobs = std::max(mjobs, sjobs);
-            mchunks = std::max(mchunks, (int)(i - start));
-            start = i;
-            acc = sg = sh = sht = 0;
-            sjobs = 0;
+    const auto rounds = plan_rounds(P, bytes);
+    size_t mg = 1, mh = 1, mht = 1;
+    int mjobs = 1, mchunks = 1;
+    for (auto &rg : rounds) {
+        size_t sg = 0, sh = 0, sht = 0;
+        int sjobs = 0;
+        for (size_t i = rg.first; i < rg.second; i++) {
+            sg += need[i].g; sh += need[i].h; sht += need[i].ht;
+            sjobs += need[i].njobs;
         }
-        if (i == chunks.size()) break;
-        acc += bytes;
-        sg += need[i].g; sh += need[i].h; sht += need[i].ht;
-        sjobs += need[i].njobs;
+        mg = std::max(mg, sg); mh = std::max(mh, sh); mht = std::max(mht, sht);
+        mjobs = std::max(mjobs, sjobs);
+        mchunks = std::max(mchunks, (int)(rg.second - rg.first));
     }
     // every buffer is sized for the largest round before anything is launched
     double *G = (double *)slot_scratch(c, S, "G", mg * 8, &rc); if (rc) return rc;
@@ -445,30 +535,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
     if (rc) return rc;
     int *pc_counts = nullptr;
     long long *pc_offs = nullptr;
-    if (P.pipe) {
-        size_t ncnt = 0, noff = 0;
-        P.rounds.resize(rounds.size());
-        for (size_t r = 0; r < rounds.size(); r++) {
-            RoundOut &o = P.rounds[r];
-            const Chunk &a = chunks[rounds[r].first], &b = chunks[rounds[r].second - 1];
-            o.cb = a.cand_begin;
-            o.ce = b.cand_begin + b.ncand;
-            o.nb = compact_blocks(o.ce - o.cb);
-            o.cbase = ncnt;
-            o.obase = noff;
-            ncnt += o.nb;
-            noff += o.nb + 1;
-            if (hipEventCreateWithFlags(&o.ev, hipEventDisableTiming) != hipSuccess)
-                return fail(c, GI_EHIP, "hipEventCreate");
-        }
-        pc_counts = (int *)slot_scratch(c, S, "rcounts", std::max<size_t>(ncnt, 1) * 4, &rc);
-        if (rc) return rc;
-        pc_offs = (long long *)slot_scratch(c, S, "roffs", std::max<size_t>(noff, 1) * 8, &rc);
-        if (rc) return rc;
-        P.rcounts = (long long *)slot_pinned(c, S, (P.q.size() + rounds.size()) * 8 + 64, &rc);
-        if (rc) return rc;
-        P.counts = P.rcounts + rounds.size();
-    }
+    if ((rc = setup_pipe(c, S, P, rounds, &pc_counts, &pc_offs)) != GI_OK) return rc;
     size_t iround = 0;
 
     for (auto &rg : rounds) {
@@ -634,38 +701,226 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         launch_corr_mfma(S.stream, c->tile_corr, (const CorrMMArgs *)d_mm,
                          (const int *)((unsigned char *)d_mm + mm_bytes), (int)mm.size(), ctiles);
         TL_END();
+        for (size_t i = rg.first; i < rg.second; i++)
+            if ((rc = chunk_intensity(c, S, P, chunks[i], d_meta, K, flag, cand, tl)) != GI_OK)
+                return rc;
+        if ((rc = round_compact(c, S, P, iround, flag, cand, pc_counts, pc_offs, tl)) != GI_OK)
+            return rc;
+        iround++;
+    }
+    return GI_OK;
+}
+
+// Split-precision mode: the batched rounds with both products on the int8
+// matrix cores (gi_split.hip). Per round: G digits of every unit, the
+// half-ls products per (chunk, class), FP64 folds, digits of the folded
+// vectors, the banded correlation, then the FP64 intensity step.
+static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timeline *tl,
+                     int *flag, LineSoA cand)
+{
+    const int N = c->nvib, SD = c->split_S;
+    const Consts K;
+    int rc = GI_OK;
+    const auto &chunks = P.chunks;
+    struct Need {
+        size_t gd, ge, h, ht, hd, hc;
+        int njobs;
+    };
+    std::vector<Need> need(chunks.size());
+    std::vector<size_t> bytes(chunks.size());
+    for (size_t i = 0; i < chunks.size(); i++) {
+        const Chunk &ch = chunks[i];
+        const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
+        const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
+        const size_t cols = (size_t)ch.nitems * c->degen[ch.gI] * c->degen[ch.gF];
+        if (GP.gW > kGMapMaxW || !GP.d_gidxB[0])
+            return fail(c, GI_EINVAL, "split mode: record map wider than the kernel supports");
+        need[i] = Need{(size_t)ch.nunits * GP.GtotB, (size_t)ch.nunits * GP.EGtot,
+                       (size_t)ch.nunits * GP.Htot, cols * BF.nsize, cols * BF.Kp, cols,
+                       ch.nunits > 0 ? (int)GP.oc.size() : 0};
+        bytes[i] = SD * need[i].gd + 4 * need[i].ge + 8 * need[i].h + 8 * need[i].ht +
+                   SD * need[i].hd + 4 * need[i].hc;
+    }
+    const auto rounds = plan_rounds(P, bytes);
+    size_t mgd = 64, mge = 1, mh = 1, mht = 1, mhd = 64, mhc = 1;
+    int mjobs = 1, mchunks = 1;
+    for (auto &rg : rounds) {
+        Need t{0, 0, 0, 0, 0, 0, 0};
+        for (size_t i = rg.first; i < rg.second; i++) {
+            t.gd += need[i].gd; t.ge += need[i].ge; t.h += need[i].h; t.ht += need[i].ht;
+            t.hd += need[i].hd; t.hc += need[i].hc; t.njobs += need[i].njobs;
+        }
+        mgd = std::max(mgd, t.gd); mge = std::max(mge, t.ge); mh = std::max(mh, t.h);
+        mht = std::max(mht, t.ht); mhd = std::max(mhd, t.hd); mhc = std::max(mhc, t.hc);
+        mjobs = std::max(mjobs, t.njobs);
+        mchunks = std::max(mchunks, (int)(rg.second - rg.first));
+    }
+    int8_t *Gd = (int8_t *)slot_scratch(c, S, "Gd", SD * mgd, &rc); if (rc) return rc;
+    int *Ge = (int *)slot_scratch(c, S, "Ge", 4 * mge, &rc); if (rc) return rc;
+    double *H = (double *)slot_scratch(c, S, "H", 8 * mh, &rc); if (rc) return rc;
+    double *Ht = (double *)slot_scratch(c, S, "Ht", 8 * mht, &rc); if (rc) return rc;
+    int8_t *Hd = (int8_t *)slot_scratch(c, S, "Hd", SD * mhd, &rc); if (rc) return rc;
+    int *He = (int *)slot_scratch(c, S, "He", 4 * mhc, &rc); if (rc) return rc;
+    const size_t hj_bytes = (size_t)mjobs * sizeof(SplitHalfJob);
+    unsigned char *d_hj = (unsigned char *)slot_scratch(c, S, "sjobs", hj_bytes + 4 * mjobs, &rc);
+    if (rc) return rc;
+    const size_t gj_bytes = (size_t)mchunks * sizeof(GSplitJob);
+    unsigned char *d_gj = (unsigned char *)slot_scratch(c, S, "sgjobs", gj_bytes + 4 * mchunks, &rc);
+    if (rc) return rc;
+    const size_t cj_bytes = (size_t)mchunks * sizeof(CorrSplitArgs);
+    unsigned char *d_cj = (unsigned char *)slot_scratch(c, S, "scorr", cj_bytes + 4 * mchunks, &rc);
+    if (rc) return rc;
+    int *pc_counts = nullptr;
+    long long *pc_offs = nullptr;
+    if ((rc = setup_pipe(c, S, P, rounds, &pc_counts, &pc_offs)) != GI_OK) return rc;
+
+    size_t iround = 0;
+    for (auto &rg : rounds) {
+        std::vector<GSplitJob> gj;
+        std::vector<SplitHalfJob> hj;
+        std::vector<CorrSplitArgs> cj;
+        std::vector<int> gbegin, tbegin, cbegin;
+        int gt = 0, tt = 0, ct = 0;
+        size_t ogd = 0, oge = 0, oh = 0, oht = 0, ohd = 0, ohc = 0;
+        std::vector<size_t> Hoff, Htoff, Hdoff, Hcoff;
         for (size_t i = rg.first; i < rg.second; i++) {
             const Chunk &ch = chunks[i];
-            CorrArgs ca;
-            fill_corr_args(c, P, ch, d_meta, K, ca);
-            IntensArgs a;
-            a.ncand = ca.ncand; a.nitems = ca.nitems; a.ndegI = ca.ndegI; a.ndegF = ca.ndegF;
-            a.cb_mask = ca.cb_mask; a.reduced = ca.reduced; a.jI = ca.jI; a.jF = ca.jF;
-            a.item_cand_start = ca.item_cand_start; a.item_lo = ca.item_lo;
-            a.item_level = ca.item_level; a.item_out_off = ca.item_out_off;
-            a.item_energy = ca.item_energy;
-            a.blk_lev = ca.blk_lev; a.blk_en = ca.blk_en; a.blk_upok = ca.blk_upok;
-            a.gnsI = ca.gnsI; a.zpe = ca.zpe; a.beta = ca.beta; a.q_stat = ca.q_stat;
-            a.A_coef_s_1 = ca.A_coef_s_1; a.intens_cm_mol = ca.intens_cm_mol;
-            a.thresh_S = ca.thresh_S; a.thresh_I = ca.thresh_I;
-            a.amp = cand.amp; a.flag = flag; a.out_iI = cand.iI; a.out_iF = cand.iF;
-            a.out_nu = cand.nu; a.out_S = cand.S; a.out_A = cand.A; a.out_I = cand.I;
-            TL_BEGIN("intensity");
-            launch_intensity(S.stream, a);
+            const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
+            const Block &BI = c->blocks[ch.jindI * c->nrep + ch.gI];
+            const int ndegI = c->degen[ch.gI];
+            Hoff.push_back(oh); Htoff.push_back(oht); Hdoff.push_back(ohd); Hcoff.push_back(ohc);
+            if (ch.nunits > 0 && GP.GtotB > 0) {
+                GSplitJob g;
+                g.classes = GP.d_cd;
+                g.nclasses = (int)GP.oc.size();
+                for (int d = 0; d < 2; d++) {
+                    const int dd = d < ndegI ? d : 0;
+                    g.idx[d] = GP.d_gidxB[dd];
+                    g.wt[d] = GP.d_gwtB[dd];
+                    g.bnd[d] = GP.d_gbnd[dd];
+                }
+                g.W = GP.gW;
+                g.GtotB = GP.GtotB;
+                g.EGtot = GP.EGtot;
+                g.nunits = ch.nunits;
+                g.gtiles = (GP.GtotB + 255) / 256;
+                g.tile_begin = gt;
+                gt += g.gtiles * ((ch.nunits + kGUnits - 1) / kGUnits);
+                g.uptr = (const double *const *)(d_meta + ch.off_uptr);
+                g.uideg = (const int *)(d_meta + ch.off_uideg);
+                g.urec = (const int *)(d_meta + ch.off_urec);
+                g.umax = BI.d_umax;
+                g.D = Gd + ogd;
+                g.plane = (long long)mgd;
+                g.E = Ge + oge;
+                gj.push_back(g);
+                gbegin.push_back(g.tile_begin);
+                for (const OutClass &oc : GP.oc) {
+                    SplitHalfJob h;
+                    h.cd = classdev(oc);
+                    h.nunits = ch.nunits;
+                    h.tile_begin = tt;
+                    h.mt = (oc.n + 63) / 64;
+                    tt += h.mt * ((ch.nunits * (int)oc.cols.size() + 63) / 64);
+                    h.GtotB = GP.GtotB;
+                    h.EGtot = GP.EGtot;
+                    h.Htot = GP.Htot;
+                    h.Gd = Gd + ogd;
+                    h.gplane = (long long)mgd;
+                    h.Ge = Ge + oge;
+                    h.H = H + oh;
+                    hj.push_back(h);
+                    tbegin.push_back(h.tile_begin);
+                }
+            }
+            ogd += need[i].gd; oge += need[i].ge; oh += need[i].h;
+            oht += need[i].ht; ohd += need[i].hd; ohc += need[i].hc;
+        }
+        auto send = [&](unsigned char *dst, const void *src, size_t nb, const std::vector<int> &beg,
+                        size_t cap) -> int {
+            P.keep.emplace_back(cap + 4 * beg.size());
+            std::memcpy(P.keep.back().data(), src, nb);
+            std::memcpy(P.keep.back().data() + cap, beg.data(), 4 * beg.size());
+            HIPCHK(hipMemcpyAsync(dst, P.keep.back().data(), P.keep.back().size(),
+                                  hipMemcpyHostToDevice, S.stream));
+            return GI_OK;
+        };
+        if (!gj.empty()) {
+            if ((rc = send(d_gj, gj.data(), gj.size() * sizeof(GSplitJob), gbegin, gj_bytes)) != GI_OK)
+                return rc;
+            TL_BEGIN("build_g");
+            launch_build_g_split(S.stream, SD, (const GSplitJob *)d_gj, (const int *)(d_gj + gj_bytes),
+                                 (int)gj.size(), gt);
             TL_END();
         }
-        if (P.pipe) {
-            RoundOut &o = P.rounds[iround];
-            if (o.nb > 0) {
-                TL_BEGIN("compact");
-                launch_compact(S.stream, o.ce - o.cb, flag + o.cb, pc_counts + o.cbase,
-                               pc_offs + o.obase, shift(cand, o.cb), shift(P.comp, o.cb));
-                TL_END();
-                HIPCHK(hipMemcpyAsync(P.rcounts + iround, pc_offs + o.obase + o.nb, 8,
-                                      hipMemcpyDeviceToHost, S.stream));
-            }
-            HIPCHK(hipEventRecord(o.ev, S.stream));
+        if (!hj.empty()) {
+            if ((rc = send(d_hj, hj.data(), hj.size() * sizeof(SplitHalfJob), tbegin, hj_bytes)) != GI_OK)
+                return rc;
+            TL_BEGIN("half_ls_split");
+            launch_halfls_split(S.stream, SD, N, c->kpad_all, c->d_mud, c->muplane, c->d_mue,
+                                (const SplitHalfJob *)d_hj, (const int *)(d_hj + hj_bytes),
+                                (int)hj.size(), tt);
+            TL_END();
         }
+        for (size_t i = rg.first; i < rg.second; i++) {
+            const Chunk &ch = chunks[i];
+            const size_t k = i - rg.first;
+            const JPlan &pF = c->jp[ch.jindF];
+            const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
+            const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
+            const int ndegI = c->degen[ch.gI], ndegF = c->degen[ch.gF];
+            SymDev t0 = symdev(pF.sym[ch.gF * GI_MAXDEG + 0]);
+            t0.tcol = GP.d_tcol[0];
+            SymDev t1 = t0;
+            if (ndegF > 1) {
+                t1 = symdev(pF.sym[ch.gF * GI_MAXDEG + 1]);
+                t1.tcol = GP.d_tcol[1];
+            }
+            const int cols = ch.nitems * ndegI * ndegF;
+            TL_BEGIN("fold");
+            launch_fold(S.stream, ch.nitems, BF.nsize, ndegI, ndegF, GP.Htot,
+                        (const int *)(d_meta + ch.off_iunit), t0, t1, H + Hoff[k], Ht + Htoff[k]);
+            TL_END();
+            TL_BEGIN("split_ht");
+            launch_split_rows(S.stream, SD, Ht + Htoff[k], BF.nsize, cols, BF.nsize, BF.Kp,
+                              Hd + Hdoff[k], (long long)mhd, He + Hcoff[k], nullptr);
+            TL_END();
+            CorrSplitArgs a;
+            a.Lo = ch.Lo; a.Hi = ch.Hi; a.nitems = ch.nitems; a.ncb = ndegI * ndegF;
+            a.ndegI = ndegI; a.Kp = BF.Kp;
+            a.item_lo = (const int *)(d_meta + ch.off_lo);
+            a.item_hi = (const int *)(d_meta + ch.off_hi);
+            a.item_out_off = (const long long *)(d_meta + ch.off_out);
+            a.blk_col = BF.d_col;
+            a.Ud = BF.d_ud; a.uplane = BF.uplane; a.Ue = BF.d_ue;
+            a.Hd = Hd + Hdoff[k]; a.hplane = (long long)mhd; a.He = He + Hcoff[k];
+            a.amp = cand.amp;
+            cj.push_back(a);
+            cbegin.push_back(ct);
+            const int rows = ch.Hi - ch.Lo;
+            if (rows > 0 && cols > 0) ct += ((rows + 63) / 64) * ((cols + 63) / 64);
+            double blk_bytes = 0;
+            for (const OutClass &oc : GP.oc) blk_bytes += 8.0 * oc.n * oc.K;
+            P.work[3] += ch.nunits * 2.0 * c->coef[ch.jindI][ch.jindF].C * (double)N * N;
+            P.work[4] += ch.nunits * 2.0 * GP.macs;
+            P.work[5] += (double)ch.ncand * 8.0 * BF.nsize;
+            P.work[6] += blk_bytes;
+            P.work[7] += ch.nunits * 24.0 * (double)N * N;
+            P.work[8] += 2.0 * (double)rows * ch.nitems * ndegI * ndegF * BF.nsize;
+        }
+        if (!cj.empty()) {
+            if ((rc = send(d_cj, cj.data(), cj.size() * sizeof(CorrSplitArgs), cbegin, cj_bytes)) != GI_OK)
+                return rc;
+            TL_BEGIN("corr_split");
+            launch_corr_split(S.stream, SD, (const CorrSplitArgs *)d_cj, (const int *)(d_cj + cj_bytes),
+                              (int)cj.size(), ct);
+            TL_END();
+        }
+        for (size_t i = rg.first; i < rg.second; i++)
+            if ((rc = chunk_intensity(c, S, P, chunks[i], d_meta, K, flag, cand, tl)) != GI_OK)
+                return rc;
+        if ((rc = round_compact(c, S, P, iround, flag, cand, pc_counts, pc_offs, tl)) != GI_OK)
+            return rc;
         iround++;
     }
     return GI_OK;
@@ -730,7 +985,9 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
                               S.stream));
     }
 
-    if (batched && !P.chunks.empty()) {
+    if (P.mode == GI_MODE_SPLIT && !P.chunks.empty()) {
+        if ((rc = run_split(c, S, P, d_meta, tl, flag, cand)) != GI_OK) return rc;
+    } else if (batched && !P.chunks.empty()) {
         if ((rc = run_batched(c, S, P, d_meta, tl, flag, cand)) != GI_OK) return rc;
     } else if (!P.chunks.empty()) {
         int max_units = 1, max_items = 1, max_G = 1, max_H = 1, max_ncb = 1, max_nsizeF = 1;
@@ -857,9 +1114,10 @@ int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
 {
     if (!c->prepared) return fail(c, GI_ESTATE, "gi_compute before gi_prepare");
     if (c->server) return fail(c, GI_ESTATE, "gi_compute while the request server runs");
-    if (mode != GI_MODE_FP64 && mode != GI_MODE_FP64_BATCHED)
+    if (mode != GI_MODE_FP64 && mode != GI_MODE_FP64_BATCHED && mode != GI_MODE_SPLIT)
         return fail(c, GI_EINVAL, "gi_compute: unknown mode");
     int rc;
+    if (mode == GI_MODE_SPLIT && (rc = prepare_split(c)) != GI_OK) return rc;
     Plan P;
     P.mode = mode;
     Query q;
@@ -883,7 +1141,7 @@ int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
                         out->I && out->amp;
     // batched mode with room for every candidate: copy each round's lines
     // while the next rounds compute
-    P.pipe = mode == GI_MODE_FP64_BATCHED && out->capacity >= P.ncand && arrays;
+    P.pipe = mode != GI_MODE_FP64 && out->capacity >= P.ncand && arrays;
     if ((rc = build_chunks(c, P)) != GI_OK) return rc;
     Slot &S = c->main_slot;
     long long nout = 0;
@@ -1203,7 +1461,10 @@ int gi_serve_submit(gi_ctx *c, const gi_request *r, long long *ticket)
         return GI_EINVAL;
     if (!c->groups[((size_t)r->jindI * c->nrep + r->igammaI) * 2 + r->jindF].valid)
         return GI_EINVAL;  // |J' - J| > 1
-    if (r->mode != GI_MODE_FP64 && r->mode != GI_MODE_FP64_BATCHED) return GI_EINVAL;
+    if (r->mode != GI_MODE_FP64 && r->mode != GI_MODE_FP64_BATCHED && r->mode != GI_MODE_SPLIT)
+        return GI_EINVAL;
+    if (r->mode == GI_MODE_SPLIT && c->split_ready != c->split_S)
+        return fail(c, GI_ESTATE, "split-mode request before gi_prepare_split");
     if (!(r->temperature > 0.0)) return GI_EINVAL;
     auto R = std::make_shared<Request>();
     R->r = *r;

@@ -65,6 +65,10 @@ struct OutClass {
     std::vector<double> rcoef;
     int hbase = 0, gbase = 0;        // offsets (doubles) in a unit's H and G buffers
     int gbase_p = 0;                 // offset in the packed G (rows of ncols, batched mode)
+    // split mode: G digits column-major, each segment padded to 64 rows
+    int kpad = 0, cbaseB = 0, eoff = 0;  // padded K, offset in a unit's digits, exponent offset
+    std::vector<int> seg_pad;        // per segment: (row offset in a column, padded mu column)
+    int *d_segpad = nullptr;
     int *d_ints = nullptr;           // packed device copy: seg_a, seg_off, seg_n, seg_row0, rptr, rsI
     double *d_rcoef = nullptr;
 };
@@ -84,6 +88,13 @@ struct GroupPlan {
     // packed G as a sparse map of the lower eigenvector record, per lower
     // component d: G[g] = sum_{w < gW} gwt[d][w*Gtot_p + g] * u[gidx[d][w*Gtot_p + g]]
     // (the recipe of g composed with the symmetrisation map; padding has weight 0)
+    // the same map in the split-mode layout (columns of kpad rows), and per
+    // column the bound max_rows sum_w |w| (times max|u| it bounds |G|)
+    int GtotB = 0, EGtot = 0;
+    std::vector<int> gidxB[GI_MAXDEG];
+    std::vector<double> gwtB[GI_MAXDEG], gbnd[GI_MAXDEG];
+    int *d_gidxB[GI_MAXDEG] = {nullptr, nullptr};
+    double *d_gwtB[GI_MAXDEG] = {nullptr, nullptr}, *d_gbnd[GI_MAXDEG] = {nullptr, nullptr};
     int gW = 0;
     std::vector<int> gidx[GI_MAXDEG];
     std::vector<double> gwt[GI_MAXDEG];
@@ -102,6 +113,12 @@ struct Block {
     int *d_lev = nullptr, *d_col = nullptr;
     double *d_en = nullptr;
     unsigned char *d_upok = nullptr;
+    // split mode: digit planes of every record (rows of Kp bytes), exponents, max |u|
+    int8_t *d_ud = nullptr;
+    long long uplane = 0;
+    int Kp = 0;
+    int *d_ue = nullptr;
+    double *d_umax = nullptr;
 };
 
 struct KStat {
@@ -163,6 +180,8 @@ struct gi_ctx {
     gi::Coef coef[2][2];             // [jindI][jindF]
     std::vector<int> vperm, vpos;    // vibrational permutation (new -> old, old -> new)
     std::vector<int> vcls_off, vcls_n;   // symmetry classes: contiguous ranges of new indices
+    std::vector<int> vpad_off;       // class offsets with every class padded to 64 (split mode)
+    int kpad_all = 0;
     double *d_mu_perm = nullptr;     // mu with rows and columns permuted by vperm
     std::vector<gi::GroupPlan> groups;   // [(jindI*nrep + gI)*2 + jindF]
 
@@ -174,6 +193,12 @@ struct gi_ctx {
 
     // tile configurations of k_halfls_jobs and k_corr_mfma (see tile_rows)
     int tile_halfls = 3, tile_corr = 0;
+
+    // split mode: digits per operand (3..6) and the resident digit planes of mu
+    int split_S = 6, split_ready = 0;  // split_ready = digits the planes were built with
+    int8_t *d_mud = nullptr;
+    long long muplane = 0;
+    int *d_mue = nullptr;
 
     // statistics of the last gi_compute
     std::map<std::string, gi::KStat> kstats;
@@ -216,6 +241,8 @@ inline ClassDev classdev(const OutClass &oc)
     d.off = oc.off; d.n = oc.n; d.ncols = (int)oc.cols.size(); d.smax = oc.smax; d.K = oc.K;
     d.nseg = (int)oc.seg_a.size(); d.hbase = oc.hbase; d.gbase = oc.gbase;
     d.gbase_p = oc.gbase_p;
+    d.kpad = oc.kpad; d.cbaseB = oc.cbaseB; d.eoff = oc.eoff;
+    d.seg_pad = oc.d_segpad;
     const int ns = d.nseg;
     d.seg_a = oc.d_ints;
     d.seg_off = oc.d_ints + ns;
@@ -228,6 +255,8 @@ inline ClassDev classdev(const OutClass &oc)
 }
 
 int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out);
+int prepare_split(gi_ctx *c);
+void free_split(gi_ctx *c);
 
 // host plan construction (gi_plan.cpp)
 int assign_slots(gi_ctx *c, int jind);

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+
 #ifndef GI_MAXDEG
 #define GI_MAXDEG 2
 #endif
@@ -26,6 +28,8 @@ struct LineSoA {
 // Device view of one output class of a group plan (see OutClass).
 struct ClassDev {
     int off, n, ncols, smax, K, nseg, hbase, gbase, gbase_p;
+    int kpad, cbaseB, eoff;          // split mode (see OutClass)
+    const int *seg_pad;              // [nseg][2]: row offset in a digit column, mu column offset
     const int *seg_a, *seg_off, *seg_n, *seg_row0, *rptr, *rsI;
     const double *rcoef;
 };
@@ -111,6 +115,53 @@ void launch_build_g_map(hipStream_t st, const GMapJob *d_jobs, const int *d_begi
                         int ntiles);
 void launch_build_g_units(hipStream_t st, int N, const GBuildJob *d_jobs, const int *d_begin,
                           int njobs, int ntiles);
+
+// Split-precision mode (gi_split.hip): digit planes of int8, see there.
+struct GSplitJob {
+    const ClassDev *classes;
+    int nclasses;
+    const int *idx[2];
+    const double *wt[2], *bnd[2];
+    int W, GtotB, EGtot, nunits, gtiles, tile_begin;
+    const double *const *uptr;
+    const int *uideg, *urec;         // per unit: component, record in the lower block
+    const double *umax;              // per record of the lower block: max |u|
+    int8_t *D;                       // this chunk's G digits (plane 0)
+    long long plane;
+    int *E;                          // this chunk's column exponents [unit][EGtot]
+};
+struct SplitHalfJob {
+    ClassDev cd;
+    int nunits, tile_begin, mt, GtotB, EGtot, Htot;
+    const int8_t *Gd;
+    long long gplane;
+    const int *Ge;
+    double *H;
+};
+struct CorrSplitArgs {
+    int Lo, Hi, nitems, ncb, ndegI, Kp;
+    const int *item_lo, *item_hi;
+    const long long *item_out_off;
+    const int *blk_col;
+    const int8_t *Ud;                // upper block digits (plane 0), rows of Kp
+    long long uplane;
+    const int *Ue;                   // per record
+    const int8_t *Hd;                // folded vectors' digits (plane 0), rows of Kp
+    long long hplane;
+    const int *He;                   // per column
+    double *amp;
+};
+void launch_split_rows(hipStream_t st, int S, const double *X, long long ld, int rows, int K,
+                       int Kp, int8_t *D, long long plane, int *E, double *Mx);
+void launch_split_mu(hipStream_t st, int S, int N, int kpad_all, const double *mu,
+                     const int *vinv, int8_t *D, long long plane, int *E);
+void launch_build_g_split(hipStream_t st, int S, const GSplitJob *d_jobs, const int *d_begin,
+                          int njobs, int ntiles);
+void launch_halfls_split(hipStream_t st, int S, int N, int kpad_all, const int8_t *mud,
+                         long long muplane, const int *mue, const SplitHalfJob *d_jobs,
+                         const int *d_tbegin, int njobs, int ntiles);
+void launch_corr_split(hipStream_t st, int S, const CorrSplitArgs *d_args, const int *d_cbegin,
+                       int nchunks, int ntiles);
 
 // Tile configurations of the two matrix-core kernels: cfg bit 1 selects 128
 // rows (else 64), bit 0 selects 128 columns (else 64).

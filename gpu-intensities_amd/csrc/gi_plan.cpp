@@ -205,6 +205,12 @@ int build_vib_classes(gi_ctx *c)
     c->vcls_n.assign(ncls, 0);
     for (int v = 0; v < N; v++) c->vcls_n[vcls[v]]++;
     for (int k = 1; k < ncls; k++) c->vcls_off[k] = c->vcls_off[k - 1] + c->vcls_n[k - 1];
+    c->vpad_off.assign(ncls, 0);
+    c->kpad_all = 0;
+    for (int k = 0; k < ncls; k++) {
+        c->vpad_off[k] = c->kpad_all;
+        c->kpad_all += (c->vcls_n[k] + 63) / 64 * 64;
+    }
     for (int jind = 0; jind < 2; jind++)
         for (auto &v : c->jp[jind].r_v) v = c->vpos[v];
     return GI_OK;
@@ -375,6 +381,23 @@ int build_gmap(gi_ctx *c, GroupPlan &G)
     const JPlan &pI = c->jp[G.jindI];
     std::vector<std::vector<std::pair<int, double>>> ent;
     int W = 0;
+    // split-mode layout: per class, ncols columns of kpad rows (segments padded to 64)
+    G.GtotB = G.EGtot = 0;
+    for (OutClass &oc : G.oc) {
+        oc.seg_pad.clear();
+        oc.kpad = 0;
+        for (size_t e = 0; e < oc.seg_a.size(); e++) {
+            int ci = 0;
+            while (ci + 1 < (int)c->vcls_off.size() && c->vcls_off[ci + 1] <= oc.seg_off[e]) ci++;
+            oc.seg_pad.push_back(oc.kpad);
+            oc.seg_pad.push_back(c->vpad_off[ci] + (oc.seg_off[e] - c->vcls_off[ci]));
+            oc.kpad += (oc.seg_n[e] + 63) / 64 * 64;
+        }
+        oc.cbaseB = G.GtotB;
+        G.GtotB += (int)oc.cols.size() * oc.kpad;
+        oc.eoff = G.EGtot;
+        G.EGtot += (int)oc.cols.size();
+    }
     for (int d = 0; d < c->degen[G.gI]; d++) {
         const SymMap &m = pI.sym[G.gI * GI_MAXDEG + d];
         ent.assign(G.Gtot_p, {});
@@ -409,6 +432,28 @@ int build_gmap(gi_ctx *c, GroupPlan &G)
         G.gidx[d].resize((size_t)std::max(W, 1) * G.Gtot_p, 0);
         G.gwt[d].resize((size_t)std::max(W, 1) * G.Gtot_p, 0.0);
         G.gW = std::max(G.gW, W);
+        // split layout: element (row, j) of class co at cbaseB + j*kpad + seg_pad + (row - row0)
+        G.gidxB[d].assign((size_t)std::max(W, 1) * G.GtotB, 0);
+        G.gwtB[d].assign((size_t)std::max(W, 1) * G.GtotB, 0.0);
+        G.gbnd[d].assign(std::max(G.EGtot, 1), 0.0);
+        for (const OutClass &oc : G.oc) {
+            const int ncol = (int)oc.cols.size();
+            for (int e = 0; e < (int)oc.seg_a.size(); e++)
+                for (int row = oc.seg_row0[e]; row < oc.seg_row0[e + 1]; row++)
+                    for (int j = 0; j < ncol; j++) {
+                        const auto &L = ent[(size_t)oc.gbase_p + (size_t)row * ncol + j];
+                        const size_t gb = (size_t)oc.cbaseB + (size_t)j * oc.kpad +
+                                          oc.seg_pad[2 * e] + (row - oc.seg_row0[e]);
+                        double sa = 0.0;
+                        for (size_t w = 0; w < L.size(); w++) {
+                            G.gidxB[d][w * G.GtotB + gb] = L[w].first;
+                            G.gwtB[d][w * G.GtotB + gb] = L[w].second;
+                            sa += std::fabs(L[w].second);
+                        }
+                        double &b = G.gbnd[d][oc.eoff + j];
+                        b = std::max(b, sa);
+                    }
+        }
         // stash this component's entries; widths are fixed below
         for (size_t g = 0; g < ent.size(); g++)
             for (size_t w = 0; w < ent[g].size(); w++) {
@@ -422,6 +467,8 @@ int build_gmap(gi_ctx *c, GroupPlan &G)
         if ((int)have < G.gW) {
             G.gidx[d].resize((size_t)G.gW * G.Gtot_p, 0);
             G.gwt[d].resize((size_t)G.gW * G.Gtot_p, 0.0);
+            G.gidxB[d].resize((size_t)G.gW * G.GtotB, 0);
+            G.gwtB[d].resize((size_t)G.gW * G.GtotB, 0.0);
         }
     }
     return GI_OK;

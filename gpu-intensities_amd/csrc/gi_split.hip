@@ -1,0 +1,450 @@
+// Split-precision products (GI_MODE_SPLIT) on the gfx950 int8 matrix cores
+// (v_mfma_i32_16x16x64_i8).
+//
+// Each operand row (A) or column (B) along K is scaled by its own power of
+// two and cut into S signed digits of 7 bits (Ozaki splitting):
+//   x = 2^e * sum_{i<S} d_i * 128^-(i+1) + r,   |d_i| <= 64,
+//   |r| <= 2^(e-1) * 128^-S,   with e such that max|x| <= 2^(e-1).
+// A digit product is at most 4096 in magnitude, so a K sum of them is exact
+// in int32 while 4096 * S * K < 2^31 (K < 104857 for S = 5). The product of a
+// row and a column is then
+//   sum_k a_k b_k = 2^(ea+eb) * sum_{l<S} 128^-(l+2) * sum_{i+j=l} sum_k da_ik db_jk
+// plus the dropped terms (i + j >= S and the remainders), i.e. a relative
+// error of about 128^-S of the row and column maxima. The integer sums are
+// kept per level l and combined in FP64 in the epilogue; everything around
+// the products (G build, fold, intensity) stays FP64.
+//
+// Digit planes: plane q of an operand with R rows of Kp bytes is R*Kp bytes;
+// rows are padded with zeros to a multiple of 64 so that a K step never
+// needs a bounds check. Tiles are 64 x 64 outputs per 256-thread workgroup
+// (4 wavefronts of 32 x 32 as 2 x 2 MFMA blocks), K in steps of 64 staged
+// through LDS (rows of 80 bytes: the 16 lanes of one 16-byte LDS read then
+// hit 16 distinct bank quads). Operand layout of the 16x16x64 i8 instruction
+// (checked on the device): lane l supplies A[l%16][16*(l/16) .. +15] and
+// B[16*(l/16) .. +15][l%16]; accumulator register r holds D[4*(l/16)+r][l%16].
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "gi_kernels.h"
+
+namespace gi {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+__device__ inline int digit_exp(double m)
+{
+    if (!(m > 0.0)) return 0;
+    int ex;
+    (void)frexp(m, &ex);  // m < 2^ex
+    return ex + 1;        // m <= 2^(e-1)
+}
+
+template <int S>
+__device__ inline void to_digits(double x, int e, int (&d)[S])
+{
+    double r = ldexp(x, -e);
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+        const double t = r * 128.0;
+        const double v = rint(t);
+        d[i] = (int)v;
+        r = t - v;
+    }
+}
+
+__device__ inline double block_max(double m, double *red)
+{
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + k]);
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// Rows of a row-major FP64 matrix (row r at X + r*ld, K valid entries) to
+// digit planes D[q*plane + r*Kp + k] and exponents E[r]; optional row maxima.
+template <int S>
+__global__ __launch_bounds__(256) void k_split_rows(const double *__restrict__ X, long long ld,
+                                                    int K, int Kp, int8_t *__restrict__ D,
+                                                    long long plane, int *__restrict__ E,
+                                                    double *__restrict__ Mx)
+{
+    __shared__ double red[256];
+    const long long r = blockIdx.x;
+    const double *x = X + r * ld;
+    double m = 0.0;
+    for (int k = threadIdx.x; k < K; k += 256) m = fmax(m, fabs(x[k]));
+    m = block_max(m, red);
+    const int e = digit_exp(m);
+    if (threadIdx.x == 0) {
+        E[r] = e;
+        if (Mx) Mx[r] = m;
+    }
+    int8_t *d = D + r * Kp;
+    for (int k = threadIdx.x; k < Kp; k += 256) {
+        int dg[S];
+        to_digits<S>(k < K ? x[k] : 0.0, e, dg);
+#pragma unroll
+        for (int q = 0; q < S; q++) d[q * plane + k] = (int8_t)dg[q];
+    }
+}
+
+// mu (permuted, mu_a(vF, vI) at vF + vI*N + a*N^2) to digit planes with row
+// (a, vF) at (a*N + vF)*kpad_all and column vpad(vI) (classes padded to 64;
+// vinv[p] = vI or -1 for padding); one exponent per vF over all a and vI.
+template <int S>
+__global__ __launch_bounds__(256) void k_split_mu(int N, int kpad_all,
+                                                  const double *__restrict__ mu,
+                                                  const int *__restrict__ vinv,
+                                                  int8_t *__restrict__ D, long long plane,
+                                                  int *__restrict__ E)
+{
+    __shared__ double red[256];
+    const int vF = blockIdx.x;
+    const long long NN = (long long)N * N;
+    double m = 0.0;
+    for (int i = threadIdx.x; i < 3 * N; i += 256) {
+        const int a = i / N, vI = i - a * N;
+        m = fmax(m, fabs(mu[vF + (long long)vI * N + a * NN]));
+    }
+    m = block_max(m, red);
+    const int e = digit_exp(m);
+    if (threadIdx.x == 0) E[vF] = e;
+    for (int i = threadIdx.x; i < 3 * kpad_all; i += 256) {
+        const int a = i / kpad_all, p = i - a * kpad_all;
+        const int vI = vinv[p];
+        int dg[S];
+        to_digits<S>(vI >= 0 ? mu[vF + (long long)vI * N + a * NN] : 0.0, e, dg);
+        int8_t *d = D + ((long long)a * N + vF) * kpad_all + p;
+#pragma unroll
+        for (int q = 0; q < S; q++) d[q * plane] = (int8_t)dg[q];
+    }
+}
+
+__device__ inline int find_tile_job(const int *__restrict__ begin, int njobs, int t)
+{
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (begin[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ inline int xcd_tile_s(int w, int grid) { return (w & 7) * (grid >> 3) + (w >> 3); }
+static inline int xcd_grid_s(int ntiles) { return (ntiles + 7) & ~7; }
+
+// G of every unit as digits (split layout of the sparse record map): thread
+// = one row of one column, up to kGUnits units per workgroup. The column
+// exponent comes from the bound max_rows sum|w| * max|u| of the unit's
+// record; the thread holding row 0 of a column stores it.
+template <int S>
+__global__ __launch_bounds__(256) void k_build_g_split(const GSplitJob *__restrict__ jobs,
+                                                       const int *__restrict__ begin, int njobs,
+                                                       int ntiles)
+{
+    const int tile = xcd_tile_s(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
+    const GSplitJob &J = jobs[find_tile_job(begin, njobs, tile)];
+    const int local = tile - J.tile_begin;
+    const int ug = local / J.gtiles, gt = local - ug * J.gtiles;
+    const int g = gt * 256 + (int)threadIdx.x;
+    if (g >= J.GtotB) return;
+    int co = 0;
+    while (co + 1 < J.nclasses && J.classes[co + 1].cbaseB <= g) co++;
+    const ClassDev &cd = J.classes[co];
+    const int j = (g - cd.cbaseB) / cd.kpad;
+    const bool first = (g - cd.cbaseB) - j * cd.kpad == 0;
+    int ix[kGMapMaxW];
+    double wv[kGMapMaxW];
+    double bnd = 0.0;
+    int cur = -1;
+    const int u0 = ug * kGUnits, u1 = min(u0 + kGUnits, J.nunits);
+    for (int u = u0; u < u1; u++) {
+        const int d = J.uideg[u];
+        if (d != cur) {
+            const int *ip = J.idx[d] + g;
+            const double *wp = J.wt[d] + g;
+#pragma unroll
+            for (int w = 0; w < kGMapMaxW; w++)
+                if (w < J.W) {
+                    ix[w] = ip[(size_t)w * J.GtotB];
+                    wv[w] = wp[(size_t)w * J.GtotB];
+                }
+            bnd = J.bnd[d][cd.eoff + j];
+            cur = d;
+        }
+        const double *rec = J.uptr[u];
+        const int e = digit_exp(bnd * J.umax[J.urec[u]]);
+        double v = 0.0;
+#pragma unroll
+        for (int w = 0; w < kGMapMaxW; w++)
+            if (w < J.W) v += wv[w] * rec[ix[w]];
+        int dg[S];
+        to_digits<S>(v, e, dg);
+        int8_t *dst = J.D + (size_t)u * J.GtotB + g;
+#pragma unroll
+        for (int q = 0; q < S; q++) dst[q * J.plane] = (int8_t)dg[q];
+        if (first) J.E[(size_t)u * J.EGtot + cd.eoff + j] = e;
+    }
+}
+
+// One K step of 64 for a 64 x 64 tile: the wave's 2 x 2 blocks, digit pairs
+// i + j < S into level accumulators.
+template <int S>
+__device__ inline void oz_step(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs,
+                               int wm, int wn, int lane, v4i (&acc)[2][2][S])
+{
+    const int i16 = lane & 15, k16 = (lane >> 4) * 16;
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+        const v4i a0 = *(const v4i *)(As + (i * 64 + wm + i16) * 80 + k16);
+        const v4i a1 = *(const v4i *)(As + (i * 64 + wm + 16 + i16) * 80 + k16);
+#pragma unroll
+        for (int j = 0; j < S - i; j++) {
+            const v4i b0 = *(const v4i *)(Bs + (j * 64 + wn + i16) * 80 + k16);
+            const v4i b1 = *(const v4i *)(Bs + (j * 64 + wn + 16 + i16) * 80 + k16);
+            acc[0][0][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b0, acc[0][0][i + j], 0, 0, 0);
+            acc[0][1][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b1, acc[0][1][i + j], 0, 0, 0);
+            acc[1][0][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b0, acc[1][0][i + j], 0, 0, 0);
+            acc[1][1][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b1, acc[1][1][i + j], 0, 0, 0);
+        }
+    }
+}
+
+template <int S>
+__device__ inline double oz_value(const v4i (&acc)[S], int r)
+{
+    double v = 0.0;
+#pragma unroll
+    for (int l = S - 1; l >= 0; l--) v += ldexp((double)acc[l][r], -7 * (l + 2));
+    return v;
+}
+
+// Split half linestrength: H_co[m, (u, j)] = sum over segments of mu digits x
+// G digits, per (chunk, class) job; m-tiles fastest.
+template <int S>
+__global__ __launch_bounds__(256) void k_halfls_split(int N, int kpad_all,
+                                                      const int8_t *__restrict__ mud,
+                                                      long long muplane,
+                                                      const int *__restrict__ mue,
+                                                      const SplitHalfJob *__restrict__ jobs,
+                                                      const int *__restrict__ tbegin, int njobs,
+                                                      int ntiles)
+{
+    __shared__ __attribute__((aligned(16))) int8_t As[S * 64 * 80];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[S * 64 * 80];
+    const int tile = xcd_tile_s(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
+    const SplitHalfJob &J = jobs[find_tile_job(tbegin, njobs, tile)];
+    const ClassDev &cd = J.cd;
+    const int local = tile - J.tile_begin;
+    const int m0 = (local % J.mt) * 64, n0 = (local / J.mt) * 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int ncolsTot = J.nunits * cd.ncols;
+    // loads: row/column tid/4, 16 bytes at K offset 16*(tid%4), all S planes
+    const int lr = tid >> 2, lp = (tid & 3) * 16;
+    const bool mok = m0 + lr < cd.n;
+    const int n = n0 + lr;
+    const bool nok = n < ncolsTot;
+    long long bcol = 0;
+    if (nok) {
+        const int u = n / cd.ncols, jj = n - u * cd.ncols;
+        bcol = (long long)u * J.GtotB + cd.cbaseB + (long long)jj * cd.kpad;
+    }
+    v4i acc[2][2][S];
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int l = 0; l < S; l++) acc[x][y][l] = v4i{0, 0, 0, 0};
+    v4i ra[S], rb[S];
+    auto fetch = [&](int se, int kk) {
+        const long long arow = ((long long)cd.seg_a[se] * N + cd.off + m0 + lr) * kpad_all +
+                               cd.seg_pad[2 * se + 1] + kk + lp;
+        const long long bofs = bcol + cd.seg_pad[2 * se] + kk + lp;
+#pragma unroll
+        for (int q = 0; q < S; q++) {
+            ra[q] = mok ? *(const v4i *)(mud + q * muplane + arow) : v4i{0, 0, 0, 0};
+            rb[q] = nok ? *(const v4i *)(J.Gd + q * J.gplane + bofs) : v4i{0, 0, 0, 0};
+        }
+    };
+    int e = 0, kk = 0;
+    if (cd.nseg > 0) fetch(0, 0);
+    while (e < cd.nseg) {
+#pragma unroll
+        for (int q = 0; q < S; q++) {
+            *(v4i *)(As + (q * 64 + lr) * 80 + lp) = ra[q];
+            *(v4i *)(Bs + (q * 64 + lr) * 80 + lp) = rb[q];
+        }
+        __syncthreads();
+        kk += 64;
+        if (kk >= cd.seg_n[e]) {
+            e++;
+            kk = 0;
+        }
+        if (e < cd.nseg) fetch(e, kk);
+        oz_step<S>(As, Bs, wm, wn, lane, acc);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wm + 16 * x + 4 * (lane >> 4) + r;
+                const int nn = n0 + wn + 16 * y + (lane & 15);
+                if (m >= cd.n || nn >= ncolsTot) continue;
+                const int u = nn / cd.ncols, jj = nn - u * cd.ncols;
+                const int ex = mue[cd.off + m] + J.Ge[(size_t)u * J.EGtot + cd.eoff + jj];
+                J.H[(size_t)u * J.Htot + cd.hbase + (size_t)jj * cd.n + m] =
+                    ldexp(oz_value<S>(acc[x][y], r), ex);
+            }
+}
+
+// Split banded correlation (the digit form of k_corr_mfma): rows = upper
+// records (digits of the resident block), columns = folded vectors (digits).
+template <int S>
+__global__ __launch_bounds__(256) void k_corr_split(const CorrSplitArgs *__restrict__ args,
+                                                    const int *__restrict__ cbegin, int nchunks,
+                                                    int ntiles)
+{
+    __shared__ __attribute__((aligned(16))) int8_t As[S * 64 * 80];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[S * 64 * 80];
+    __shared__ int s_need;
+    const int tile = xcd_tile_s(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
+    const int ci = find_tile_job(cbegin, nchunks, tile);
+    const CorrSplitArgs &a = args[ci];
+    const int local = tile - cbegin[ci];
+    const int rt = (a.Hi - a.Lo + 63) / 64;
+    const int f0 = a.Lo + (local % rt) * 64;
+    const int n0 = (local / rt) * 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ncols = a.nitems * a.ncb;
+    if (tid == 0) {
+        int need = 0;
+        const int i0 = n0 / a.ncb, i1 = min(a.nitems, (n0 + 64 + a.ncb - 1) / a.ncb);
+        for (int i = i0; i < i1 && !need; i++)
+            if (a.item_lo[i] < f0 + 64 && a.item_hi[i] > f0) need = 1;
+        s_need = need;
+    }
+    __syncthreads();
+    if (!s_need) return;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int lr = tid >> 2, lp = (tid & 3) * 16;
+    const int fr = f0 + lr;
+    const bool fok = fr < a.Hi;
+    const long long arow = fok ? (long long)a.blk_col[fr] * a.Kp : 0;
+    const int nr = n0 + lr;
+    const bool nok = nr < ncols;
+    const long long brow = (long long)nr * a.Kp;
+    v4i acc[2][2][S];
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int l = 0; l < S; l++) acc[x][y][l] = v4i{0, 0, 0, 0};
+    v4i ra[S], rb[S];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < S; q++) {
+            ra[q] = fok ? *(const v4i *)(a.Ud + q * a.uplane + arow + k0 + lp) : v4i{0, 0, 0, 0};
+            rb[q] = nok ? *(const v4i *)(a.Hd + q * a.hplane + brow + k0 + lp) : v4i{0, 0, 0, 0};
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < a.Kp; k0 += 64) {
+#pragma unroll
+        for (int q = 0; q < S; q++) {
+            *(v4i *)(As + (q * 64 + lr) * 80 + lp) = ra[q];
+            *(v4i *)(Bs + (q * 64 + lr) * 80 + lp) = rb[q];
+        }
+        __syncthreads();
+        if (k0 + 64 < a.Kp) fetch(k0 + 64);
+        oz_step<S>(As, Bs, wm, wn, lane, acc);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int f = f0 + wm + 16 * x + 4 * (lane >> 4) + r;
+                const int nn = n0 + wn + 16 * y + (lane & 15);
+                if (f >= a.Hi || nn >= ncols) continue;
+                const int i = nn / a.ncb, cb = nn - i * a.ncb;
+                if (f < a.item_lo[i] || f >= a.item_hi[i]) continue;
+                const long long o = a.item_out_off[i] + (f - a.item_lo[i]);
+                const int idegF = cb / a.ndegI, idegI = cb - idegF * a.ndegI;
+                const int ex = a.Ue[a.blk_col[f]] + a.He[nn];
+                a.amp[4 * o + idegI + 2 * idegF] = ldexp(oz_value<S>(acc[x][y], r), ex);
+            }
+}
+
+// ---------------------------------------------------------------------------
+
+#define GI_SPLIT_DISPATCH(S_, CALL)                                                       \
+    switch (S_) {                                                                         \
+    case 3: CALL(3); break;                                                               \
+    case 4: CALL(4); break;                                                               \
+    case 5: CALL(5); break;                                                               \
+    default: CALL(6); break;                                                              \
+    }
+
+void launch_split_rows(hipStream_t st, int S, const double *X, long long ld, int rows, int K,
+                       int Kp, int8_t *D, long long plane, int *E, double *Mx)
+{
+    if (rows <= 0) return;
+#define L_(s) hipLaunchKernelGGL(k_split_rows<s>, dim3(rows), dim3(256), 0, st, X, ld, K, Kp, D, plane, E, Mx)
+    GI_SPLIT_DISPATCH(S, L_)
+#undef L_
+}
+
+void launch_split_mu(hipStream_t st, int S, int N, int kpad_all, const double *mu,
+                     const int *vinv, int8_t *D, long long plane, int *E)
+{
+#define L_(s) hipLaunchKernelGGL(k_split_mu<s>, dim3(N), dim3(256), 0, st, N, kpad_all, mu, vinv, D, plane, E)
+    GI_SPLIT_DISPATCH(S, L_)
+#undef L_
+}
+
+void launch_build_g_split(hipStream_t st, int S, const GSplitJob *d_jobs, const int *d_begin,
+                          int njobs, int ntiles)
+{
+    if (njobs <= 0 || ntiles <= 0) return;
+#define L_(s) hipLaunchKernelGGL(k_build_g_split<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, d_jobs, d_begin, njobs, ntiles)
+    GI_SPLIT_DISPATCH(S, L_)
+#undef L_
+}
+
+void launch_halfls_split(hipStream_t st, int S, int N, int kpad_all, const int8_t *mud,
+                         long long muplane, const int *mue, const SplitHalfJob *d_jobs,
+                         const int *d_tbegin, int njobs, int ntiles)
+{
+    if (njobs <= 0 || ntiles <= 0) return;
+#define L_(s) hipLaunchKernelGGL(k_halfls_split<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, N, kpad_all, mud, muplane, mue, d_jobs, d_tbegin, njobs, ntiles)
+    GI_SPLIT_DISPATCH(S, L_)
+#undef L_
+}
+
+void launch_corr_split(hipStream_t st, int S, const CorrSplitArgs *d_args, const int *d_cbegin,
+                       int nchunks, int ntiles)
+{
+    if (nchunks <= 0 || ntiles <= 0) return;
+#define L_(s) hipLaunchKernelGGL(k_corr_split<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, d_args, d_cbegin, nchunks, ntiles)
+    GI_SPLIT_DISPATCH(S, L_)
+#undef L_
+}
+
+}  // namespace gi

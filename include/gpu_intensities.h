@@ -37,7 +37,9 @@ typedef enum gi_status {
  * Boltzmann factors and the intensity step are FP64 in every mode. */
 typedef enum gi_mode {
     GI_MODE_FP64 = 0,          /* plain FP64: one GEMV-form product per lower level */
-    GI_MODE_FP64_BATCHED = 1   /* FP64 matrix-core products over batches of lower levels */
+    GI_MODE_FP64_BATCHED = 1,  /* FP64 matrix-core products over batches of lower levels */
+    GI_MODE_SPLIT = 2          /* batched, both products as split int8 digits on the
+                                  matrix cores (see gi_set_split_digits)          */
 } gi_mode;
 
 /* Basis of one J: the TO_bset_contrT fields (trove_objects.h:49-72) read by
@@ -180,6 +182,19 @@ int gi_kernel_stats(gi_ctx *ctx, int max, const char **names, double *total_ms,
  * (per unit in GI_MODE_FP64, per batch in GI_MODE_FP64_BATCHED), [7] dense mu
  * bytes (24*Nvib^2 per unit), [8] correlation flops executed. */
 int gi_work_stats(gi_ctx *ctx, double *out, int n);
+
+/* GI_MODE_SPLIT (no reference counterpart): every operand row or column of
+ * the two products is scaled by a power of two and cut into `digits` signed
+ * 7-bit digits; digit products are summed exactly in int32 on the int8 matrix
+ * cores and combined in FP64, so the relative error of a product is about
+ * 128^-digits of its operands' row/column maxima (6 digits: ~2^-41).
+ * Energies, Boltzmann factors, Q, the fold and the intensity step stay FP64.
+ * digits: 3..6 (default 6, which keeps S within 1e-6 relative of FP64 for
+ * lines down to 1e-6 of the strongest). gi_prepare_split builds the resident digit
+ * planes of mu and of every eigenvector block (digits bytes per element);
+ * gi_compute does it on first use, the request server needs it beforehand. */
+int gi_set_split_digits(gi_ctx *ctx, int digits);
+int gi_prepare_split(gi_ctx *ctx);
 
 /* Tile shapes of the two matrix-core kernels of GI_MODE_FP64_BATCHED (no
  * reference counterpart: the reference has no batched form). cfg 0..3 =

@@ -1,0 +1,135 @@
+"""Split-precision mode (GI_MODE_SPLIT) against the FP64 oracle.
+
+Acceptance: the transition set equals the oracle's (thresholds 0, so every
+candidate in the window is emitted), and over the lines with
+S >= 1e-6 * S_max the relative error of S is at most 1e-6 with the default 6
+digits. Energies, nu and the intensity step are FP64, so nu is bit-identical
+and I / A_einst follow S to the same relative error. The error report is
+printed (pytest -s) for 296 K and 1500 K.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from gpu_intensities_amd import binding, synth
+import oracle_py
+
+pytestmark = pytest.mark.gpu
+
+TOL_S = 1e-6
+STRONG = 1e-6
+
+
+def _sorted(r):
+    k = np.lexsort((r["ilevelF"], r["ilevelI"]))
+    return {key: (v[k] if v.ndim == 1 else v[k, :]) for key, v in r.items()}
+
+
+def split_error(got, ref):
+    g, r = _sorted(got), _sorted(ref)
+    assert len(g["S"]) == len(r["S"]) > 0
+    assert np.array_equal(g["ilevelI"], r["ilevelI"])
+    assert np.array_equal(g["ilevelF"], r["ilevelF"])
+    assert np.array_equal(g["nu"], r["nu"])
+    strong = r["S"] >= STRONG * r["S"].max()
+    rel = np.abs(g["S"] - r["S"])[strong] / r["S"][strong]
+    for f in ("A_einst", "I"):
+        gf, rf = g[f][strong], r[f][strong]
+        zero = rf == 0.0  # nu = 0: both are exactly zero
+        assert (gf[zero] == 0.0).all(), f
+        relf = np.abs(gf - rf)[~zero] / np.abs(rf[~zero])
+        assert (relf <= 2 * rel[~zero] + 1e-13).all(), f
+    return dict(lines=int(len(r["S"])), strong=int(strong.sum()), max=float(rel.max()),
+                median=float(np.median(rel)))
+
+
+@pytest.mark.parametrize("T", [296.0, 1500.0])
+def test_split_c2v_vs_oracle(engine, T):
+    job = synth.make_job("C2V", nvib=200, jvals=(0, 1), seed=7, temperature=T)
+    engine.load(job)
+    got = engine.compute(mode=binding.GI_MODE_SPLIT)
+    ref = oracle_py.OracleJob(job).run()
+    rep = split_error(got, ref)
+    print(f"split C2v Nvib=200 J 0 1 T={T}: {rep}")
+    assert rep["max"] <= TOL_S
+
+
+@pytest.mark.parametrize("reduced", [0, 1])
+def test_split_c3v_vs_oracle(engine, reduced):
+    job = synth.make_job("C3V", nvib=30, jvals=(2, 3), seed=5, reduced=reduced)
+    engine.load(job)
+    got = engine.compute(mode=binding.GI_MODE_SPLIT)
+    ref = oracle_py.OracleJob(job).run()
+    rep = split_error(got, ref)
+    print(f"split C3v reduced={reduced}: {rep}")
+    assert rep["max"] <= TOL_S
+
+
+def test_split_larger_many_rounds(engine, monkeypatch):
+    """Several chunks per group and a tiny round budget (many rounds)."""
+    job = synth.make_job("C2V", nvib=64, jvals=(4, 5), seed=31, e_span=30000.0,
+                         freq_window=(-4000.0, 9000.0))
+    engine.load(job)
+    ids = np.arange(0, job.nlevels, 3)
+    ref = oracle_py.OracleJob(job).run(ids)
+    a = engine.compute(ids, mode=binding.GI_MODE_SPLIT)
+    a = {k: v.copy() for k, v in a.items()}
+    monkeypatch.setenv("GI_BATCH_BYTES", str(1 << 20))
+    b = engine.compute(ids, mode=binding.GI_MODE_SPLIT)
+    rep = split_error(b, ref)
+    print(f"split C2v J 4 5 (many rounds): {rep}")
+    assert rep["max"] <= TOL_S
+    # the same products, split differently into rounds: same bits
+    sa, sb = _sorted(a), _sorted(b)
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_split_digits_trade_accuracy(engine):
+    """Fewer digits, larger error: 3 digits (~2^-20) is visibly worse than 6."""
+    job = synth.make_job("C2V", nvib=80, jvals=(1, 2), seed=9)
+    engine.load(job)
+    ref = oracle_py.OracleJob(job).run()
+    errs = {}
+    try:
+        for d in (3, 4, 5, 6):
+            engine.set_split_digits(d)
+            errs[d] = split_error(engine.compute(mode=binding.GI_MODE_SPLIT), ref)["max"]
+    finally:
+        engine.set_split_digits(6)
+    print(f"split max rel S error by digits: {errs}")
+    assert errs[3] > errs[5] > errs[6] and errs[4] > errs[6]
+    assert errs[6] <= TOL_S
+
+
+def test_split_requests_alone_and_mixed(engine):
+    job = synth.make_job("C2V", nvib=120, jvals=(1, 2), seed=21, e_span=9000.0)
+    engine.load(job)
+    engine.prepare_split()
+    reqs = [dict(jindI=0, igammaI=0, jindF=1, temperature=800.0, q_stat=300.0),
+            dict(jindI=1, igammaI=2, jindF=0, temperature=1500.0, q_stat=0.0, nu=(100.0, 5000.0)),
+            dict(jindI=1, igammaI=3, jindF=1, temperature=296.0, q_stat=174.5)]
+    engine.serve_start(1 << 30)
+    try:
+        alone = [engine.serve_fetch(engine.serve_submit(mode=binding.GI_MODE_SPLIT, **r))
+                 for r in reqs]
+        ts = [engine.serve_submit(mode=binding.GI_MODE_SPLIT, **r) for r in reqs * 3]
+        mixed = [engine.serve_fetch(t) for t in ts]
+    finally:
+        engine.serve_stop()
+    for i, m in enumerate(mixed):
+        a = alone[i % len(reqs)]
+        for k in a:
+            assert np.array_equal(a[k], m[k]), k
+    lv = job.levels
+    for r, a in zip(reqs, alone):
+        Q = r["q_stat"] if r["q_stat"] > 0 else engine.partition_function(r["temperature"])
+        nu = r.get("nu", job.freq_window)
+        j2 = dataclasses.replace(job, temperature=r["temperature"], q_stat=Q, freq_window=nu)
+        ids = np.nonzero((lv["jind"] == r["jindI"]) & (lv["igamma"] == r["igammaI"]))[0]
+        ref = oracle_py.OracleJob(j2).run(ids)
+        keep = lv["jind"][ref["ilevelF"]] == r["jindF"]
+        ref = {k: v[keep] for k, v in ref.items()}
+        rep = split_error(a, ref)
+        assert rep["max"] <= TOL_S
