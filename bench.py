@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 METRIC = "transitions/s FP64 vs split-precision MFMA at ≤1e-6 rel; request p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
+INT8_PEAK_TOPS = 5000.0
 
 
 def log(*a):
@@ -77,6 +78,7 @@ def build_c2_job(torch, nvib=2000, jvals=(10, 11), seed=20240, lower_max=10000.0
 
 # HIP-event timer name -> kernel symbol in the rocprofv3 summaries
 KERNEL_SYMBOL = {"half_ls_gemv": "k_gemv_sym", "half_ls_mfma": "k_halfls_jobs",
+                 "half_ls_split": "k_halfls_split", "corr_split": "k_corr_split",
                  "corr_mfma": "k_corr_mfma", "corr_intensity": "k_corr_intensity",
                  "build_g": "k_build_g_jobs"}
 PMC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
@@ -126,6 +128,17 @@ def _roofline(name, kstats, work, nvib):
         return dict(bound="mfma", achieved=ach, peak=FP64_PEAK_TFS, unit="TFLOP/s",
                     frac=ach / FP64_PEAK_TFS, traffic=None, kernel=name, avg_launch_ms=avg_ms,
                     work="2*nsize_F FP64 flops per candidate")
+    if name in ("half_ls_split", "corr_split"):
+        # int8 digit products the split method needs: S(S+1)/2 per logical
+        # multiply-add of the FP64 product (S = 6 digits)
+        pairs = 21
+        logical = work["half_ls_flops"] if name == "half_ls_split" else work["corr_bytes"] / 4.0
+        ach = logical * pairs / t / 1e12
+        return dict(bound="mfma", achieved=ach, peak=INT8_PEAK_TOPS, unit="TOP/s",
+                    frac=ach / INT8_PEAK_TOPS, traffic=None, kernel=name, avg_launch_ms=avg_ms,
+                    work="int8 ops of the 6-digit split products (21 digit pairs per logical "
+                         "multiply-add); logical FP64-equivalent rate "
+                         f"{logical / t / 1e12:.1f} TFLOP/s")
     if name == "corr_intensity":
         ach = work["corr_bytes"] / t / 1e9
         return dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
@@ -133,6 +146,48 @@ def _roofline(name, kstats, work, nvib):
                     work="8*nsize_F bytes (one eigenvector column) per candidate")
     return dict(bound="hbm", achieved=None, peak=HBM_PEAK_GBS, unit="GB/s", frac=None,
                 traffic=None, kernel=name, avg_launch_ms=avg_ms)
+
+
+def split_report(S, Sref, strong=1e-6):
+    """Relative error of S over the lines with S_ref >= strong * max S_ref."""
+    m = Sref >= strong * Sref.max()
+    rel = np.abs(S[m] - Sref[m]) / Sref[m]
+    return {"strong_lines": int(m.sum()), "max_rel_S_strong": float(rel.max()),
+            "median_rel_S_strong": float(np.median(rel)), "within_1e-6": bool(rel.max() <= 1e-6)}
+
+
+def split_at_temperature(eng, job, binding, oracle_py, T, nthr, elow=(0.0, 1000.0)):
+    """Split-mode error at another temperature: one request (J 10 -> 11, A1
+    lower levels with E - ZPE in elow) with Q(T) from the device, against the
+    oracle run with the same T, Q and lower levels."""
+    import dataclasses
+    Q = eng.partition_function(T)
+    eng.serve_start(8 << 30)
+    try:
+        got = eng.serve_fetch(eng.serve_submit(jindI=0, igammaI=0, jindF=1,
+                                               mode=binding.GI_MODE_SPLIT, temperature=T,
+                                               q_stat=Q, elow=elow))
+    finally:
+        eng.serve_stop()
+    lv = job.levels
+    e = lv["energy"] - job.zpe
+    ids = np.nonzero((lv["jind"] == 0) & (lv["igamma"] == 0) & (e >= elow[0]) &
+                     (e <= elow[1]))[0].astype(np.int32)
+    ref = oracle_py.OracleJob(dataclasses.replace(job, temperature=T, q_stat=Q)).run(
+        ids, nthreads=nthr)
+    keep = lv["jind"][ref["ilevelF"]] == 1
+    ref = {k: v[keep] for k, v in ref.items()}
+    kg = np.lexsort((got["ilevelF"], got["ilevelI"]))
+    kr = np.lexsort((ref["ilevelF"], ref["ilevelI"]))
+    same = (len(kg) == len(kr) and np.array_equal(got["ilevelI"][kg], ref["ilevelI"][kr])
+            and np.array_equal(got["ilevelF"][kg], ref["ilevelF"][kr]))
+    out = {"T": T, "Q": Q, "lower_levels": int(len(ids)), "lines": int(len(kr)),
+           "same_transition_set": bool(same)}
+    if same and len(kr):
+        out.update(split_report(got["S"][kg], ref["S"][kr]))
+        relI = np.abs(got["I"][kg] - ref["I"][kr]) / np.maximum(np.abs(ref["I"][kr]), 1e-300)
+        out["max_rel_I_strong"] = float(relI[ref["S"][kr] >= 1e-6 * ref["S"][kr].max()].max())
+    return out
 
 
 def serve_sweep(eng, job, binding, lambdas, seconds, mode, seed=7, budget=8 << 30):
@@ -198,8 +253,8 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=50,
                     help="CPU baseline sample: every n-th lower level by energy")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--modes", default="fp64,fp64_batched",
-                    help="comma-separated FP64 modes to time; the fastest is reported as value")
+    ap.add_argument("--modes", default="fp64,fp64_batched,split",
+                    help="comma-separated modes to time; the fastest is reported as value")
     ap.add_argument("--cache-dir", default=None,
                     help="reuse the QR eigenvector blocks across processes (profiler runs)")
     ap.add_argument("--serve-lambdas", default="10,50,100,200,500",
@@ -235,6 +290,11 @@ def main():
     log(f"candidates {cap}")
 
     modes = {}
+    if "split" in args.modes.split(","):
+        t = time.time()
+        eng.prepare_split()  # resident digit planes, built once like the uploads
+        torch.cuda.synchronize()
+        log(f"split digit planes {time.time() - t:.1f}s")
     for mname in args.modes.split(","):
         mode = binding.MODES[mname]
         for _ in range(args.warmup):
@@ -311,18 +371,25 @@ def main():
                 p.update({"max_rel_S": float(rel.max()), "median_rel_S": float(np.median(rel)),
                           "amp_within_1e-10": bool((err <= 1e-10 * floor).all()),
                           "lines_via_floor": int((rel > 1e-10).sum())})
+            if same and len(kr) and mname == "split":
+                p.update(split_report(got["S"][kg], ref["S"][kr]))
             parity[mname] = p
+        if "split" in modes:
+            parity["split_1500K"] = split_at_temperature(eng, job, binding, oracle_py, 1500.0,
+                                                         nthr)
         log(f"parity {parity}")
 
     out = {
         "metric": METRIC, "value": value, "unit": "transitions/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": ("i8 digits x6 (split products), f64 elsewhere" if best == "split" else "f64"),
         "data": "synthetic (seeded C2v generator, GPU QR eigenvectors)",
         "config": {"workload": "C2v Nvib=2000 J 10<->11, lower E<=10000 cm-1, window "
                                "0-10000 cm-1, T=296 K",
                    "nvib": job.nvib, "jvals": list(job.jvals), "lower_levels": int(len(lower_ok)),
                    "transitions": nlines, "mode": best, "parallelism": "1 GPU"},
+        "speedup_vs_fp64_gemv": (value / modes["fp64"]["value"]) if "fp64" in modes else None,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kern,
