@@ -33,6 +33,7 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_params", "gi_uplo
            "gi_serve_submit", "gi_serve_wait", "gi_serve_fetch", "gi_serve_stats",
            "gi_serve_stop", "gi_set_tiles", "gi_get_tiles", "gi_set_split_digits",
            "gi_prepare_split"]
+# (TROVE_EXPORTS below: the symbols of include/gi_trove.h)
 
 
 class GIError(RuntimeError):
@@ -371,3 +372,153 @@ class Engine:
         keys = ["units", "candidates", "lines", "half_ls_dense_flops", "half_ls_flops",
                 "corr_bytes", "mu_bytes", "mu_dense_bytes", "corr_flops"]
         return dict(zip(keys, w[:9].tolist()))
+
+
+# -- TROVE files, deck and line list (include/gi_trove.h) ---------------------
+TROVE_EXPORTS = ["gi_trove_open", "gi_trove_error", "gi_trove_close", "gi_trove_load",
+                 "gi_trove_params", "gi_trove_bset", "gi_trove_dipole", "gi_trove_levels",
+                 "gi_trove_vectors", "gi_trove_header", "gi_trove_format"]
+_TROVE_READY = False
+
+
+def _trove_lib():
+    global _TROVE_READY
+    L = lib()
+    if not _TROVE_READY:
+        vp = C.c_void_p
+        PP_i, PP_d = C.POINTER(P_i), C.POINTER(P_d)
+        L.gi_trove_open.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(vp)]
+        L.gi_trove_error.argtypes = [vp]
+        L.gi_trove_error.restype = C.c_char_p
+        L.gi_trove_close.argtypes = [vp]
+        L.gi_trove_close.restype = None
+        L.gi_trove_load.argtypes = [vp, vp]
+        L.gi_trove_params.argtypes = [vp, C.POINTER(GiParams)]
+        L.gi_trove_bset.argtypes = [vp, ci, C.POINTER(GiBset)]
+        L.gi_trove_dipole.argtypes = [vp, P_i, PP_d]
+        L.gi_trove_levels.argtypes = [vp, P_i, PP_d, PP_i, PP_i, PP_i, PP_i, PP_i, PP_i, PP_i,
+                                      PP_i]
+        L.gi_trove_vectors.argtypes = [vp, ci, ci, P_i, P_i, PP_d]
+        L.gi_trove_header.argtypes = [vp, C.c_char_p, cll, C.POINTER(cll)]
+        L.gi_trove_format.argtypes = [vp, C.POINTER(GiLines), cll, cll, C.c_char_p, cll,
+                                      C.POINTER(cll)]
+        _TROVE_READY = True
+    return L
+
+
+def _arr(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype)
+    return np.ctypeslib.as_array(ptr, (n,)).astype(dtype, copy=True)
+
+
+class Trove:
+    """TROVE input files of one job, read by the library (gi_trove_open)."""
+
+    def __init__(self, directory, deck):
+        L = _trove_lib()
+        h = C.c_void_p()
+        rc = L.gi_trove_open(directory.encode(), deck.encode(), C.byref(h))
+        self.h = h
+        if rc:
+            msg = L.gi_trove_error(h)
+            L.gi_trove_close(h)
+            self.h = None
+            raise GIError(rc, f"gi_trove_open: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.h:
+            _trove_lib().gi_trove_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        p = GiParams()
+        _trove_lib().gi_trove_params(self.h, C.byref(p))
+        nrep, nm = p.nrepres, p.nmodes
+        return dict(nrepres=nrep, degen=_arr(p.degen, nrep, np.int32), sym_type=p.sym_type,
+                    nmodes=nm, zpe=p.zpe, temperature=p.temperature, q_stat=p.q_stat,
+                    freq_window=tuple(p.freq_window), erange_lower=tuple(p.erange_lower),
+                    erange_upper=tuple(p.erange_upper), jvals=tuple(p.jvals),
+                    gns=_arr(p.gns, nrep, np.float64), isym_pairs=_arr(p.isym_pairs, nrep, np.int32),
+                    reduced=p.reduced, quanta_lower=_arr(p.quanta_lower, 2 * nm, np.int64),
+                    quanta_upper=_arr(p.quanta_upper, 2 * nm, np.int64),
+                    thresh_linestrength=p.thresh_linestrength)
+
+    def bset(self, jind, degen):
+        b = GiBset()
+        _trove_lib().gi_trove_bset(self.h, jind, C.byref(b))
+        d, nc, nrep = b.Maxcontracts, b.Maxsymcoeffs, len(degen)
+        Ntot = _arr(b.Ntotal, nrep, np.int32)
+        return dict(jval=b.jval, Maxsymcoeffs=nc, Maxcontracts=d, mat_size=b.mat_size,
+                    icontr2icase=_arr(b.icontr2icase, 2 * d, np.int32),
+                    iroot_correlat_j0=_arr(b.iroot_correlat_j0, d, np.int32),
+                    k=_arr(b.k, d, np.int32), ktau=_arr(b.ktau, d, np.int32),
+                    N=_arr(b.N, nrep * nc, np.int32), Ntotal=Ntot,
+                    ijterms=_arr(b.ijterms, nrep * nc, np.int32),
+                    repres=[_arr(b.repres[g], int(Ntot[g]) * int(degen[g]) * b.mat_size,
+                                 np.float64) for g in range(nrep)],
+                    nsize=_arr(b.nsize, nrep, np.int32))
+
+    def dipole(self):
+        n = ci()
+        p = P_d()
+        _trove_lib().gi_trove_dipole(self.h, C.byref(n), C.byref(p))
+        return _arr(p, 3 * n.value * n.value, np.float64), n.value
+
+    def levels(self, nmodes):
+        n = ci()
+        ptrs = [P_d()] + [P_i() for _ in range(9)]
+        _trove_lib().gi_trove_levels(self.h, C.byref(n), *[C.byref(x) for x in ptrs])
+        m = n.value
+        keys = ["energy", "jind", "igamma", "ndeg", "irec0", "ilevel", "quanta", "krot", "taurot"]
+        out = {"energy": _arr(ptrs[0], m, np.float64)}
+        for k, p in zip(keys[1:], ptrs[1:]):
+            out[k] = _arr(p, m * (nmodes + 1) if k == "quanta" else m, np.int32)
+        out["quanta"] = out["quanta"].reshape(m, nmodes + 1)
+        return out
+
+    def vectors(self, jind, g):
+        ns, nr, p = ci(), ci(), P_d()
+        _trove_lib().gi_trove_vectors(self.h, jind, g, C.byref(ns), C.byref(nr), C.byref(p))
+        return _arr(p, ns.value * nr.value, np.float64).reshape(nr.value, ns.value)
+
+    def load(self, engine):
+        rc = _trove_lib().gi_trove_load(self.h, engine.h)
+        engine._chk(rc, "gi_trove_load")
+
+    def header(self):
+        n = cll()
+        L = _trove_lib()
+        L.gi_trove_header(self.h, None, 0, C.byref(n))
+        buf = C.create_string_buffer(n.value + 1)
+        L.gi_trove_header(self.h, buf, n.value + 1, C.byref(n))
+        return buf.value.decode()
+
+    def format(self, res):
+        """Line-list text of a compute() result (dict of arrays)."""
+        nrow = len(res["S"])
+        keep = {k: np.ascontiguousarray(res[k]) for k in ("ilevelI", "ilevelF", "nu", "S",
+                                                          "A_einst", "I")}
+        keep["amp"] = np.ascontiguousarray(res["amp"]).reshape(-1)
+        s = GiLines(capacity=nrow, n=nrow, ilevelI=_p(keep["ilevelI"].astype(np.int32), P_i),
+                    ilevelF=_p(keep["ilevelF"].astype(np.int32), P_i),
+                    nu=_p(keep["nu"], P_d), S=_p(keep["S"], P_d), A_einst=_p(keep["A_einst"], P_d),
+                    I=_p(keep["I"], P_d), amp=_p(keep["amp"], P_d))
+        iI, iF = keep["ilevelI"].astype(np.int32), keep["ilevelF"].astype(np.int32)
+        s.ilevelI, s.ilevelF = _p(iI, P_i), _p(iF, P_i)
+        n = cll()
+        L = _trove_lib()
+        rc = L.gi_trove_format(self.h, C.byref(s), 0, nrow, None, 0, C.byref(n))
+        if rc not in (0, 5):
+            raise GIError(rc, "gi_trove_format")
+        buf = C.create_string_buffer(n.value + 1)
+        rc = L.gi_trove_format(self.h, C.byref(s), 0, nrow, buf, n.value + 1, C.byref(n))
+        if rc:
+            raise GIError(rc, "gi_trove_format")
+        return buf.value.decode()

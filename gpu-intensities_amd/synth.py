@@ -370,6 +370,7 @@ def make_job(sym="C2V", nvib=200, jvals=(0, 1), seed=1234, *, e_span=8000.0,
         gvib = np.repeat(np.arange(4), nvib // 4)
         gvib = gvib[rng_b.permutation(nvib)]
         bsets = [build_bset_c2v(J, gvib, _rng(seed, 2, J), shuffle_cases) for J in (j0, j1)]
+        units = [(int(gvib[v]), [v]) for v in range(nvib)]
         gns_d = [1.0, 1.0, 3.0, 3.0]
         pairs_d = [1, 1, 2, 2]
         nclasses = 1
@@ -412,7 +413,8 @@ def make_job(sym="C2V", nvib=200, jvals=(0, 1), seed=1234, *, e_span=8000.0,
                quanta_upper=ql.copy(), thresh_linestrength=float(thresh_linestrength),
                thresh_intensity=float(thresh_intensity), bsets=bsets, nvib=nvib,
                dipole_flipped=mu, jmax=max(j0, j1), levels=levels, vectors=vecs,
-               meta=dict(seed=seed, e_span=e_span))
+               meta=dict(seed=seed, e_span=e_span, vib_units=units,
+                         shuffle_cases=bool(shuffle_cases)))
 
 
 def drop_level_filter(job: Job):

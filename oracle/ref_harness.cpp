@@ -4,9 +4,15 @@
 // trove_functions.cpp, fields.cpp and Util.cpp as they lie under
 // /root/reference, into oracle/_ref/libtrove_ref.so. Used only by tests to
 // pin the oracle's 3j and filter functions against the reference's own code.
-// Nothing here is shipped; the reference's CUDA files are not built.
+// Nothing here is shipped; the reference's CUDA files are not built. The
+// job reader below runs the reference's input stage on TROVE files the tests
+// wrote, so that the build's own readers can be compared with it.
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fcntl.h>
+#include <iostream>
+#include <unistd.h>
 
 #include "fields.h"
 #include "trove_functions.h"
@@ -112,6 +118,142 @@ void ref_find_igamma_pair(void *h, int *out)
     FintensityJob &j = ((RefFilterJob *)h)->job;
     find_igamma_pair(j);
     std::memcpy(out, j.igamma_pair, sizeof(int) * j.molec.sym_nrepres);
+}
+
+
+// The reference's input stage (dipole_initialise_cpu, cuda_host.cu:837-898,
+// without its GPU part) in directory `dir`, with the deck file as stdin.
+// Its console chatter goes to /dev/null.
+struct RefJob {
+    FintensityJob job;
+};
+
+void *ref_read_job(const char *dir, const char *deck_path)
+{
+    char cwd[4096];
+    if (!getcwd(cwd, sizeof cwd)) return nullptr;
+    if (chdir(dir) != 0) return nullptr;
+    if (!std::freopen(deck_path, "r", stdin)) {
+        if (chdir(cwd) != 0) return nullptr;
+        return nullptr;
+    }
+    std::cin.clear();
+    std::fflush(stdout);
+    const int saved = dup(1), nul = open("/dev/null", O_WRONLY);
+    dup2(nul, 1);
+    RefJob *r = new RefJob();
+    FintensityJob &j = r->job;
+    read_fields(&j);
+    bset_contr_factory(&j.bset_contr[0], 0, j.molec.sym_degen, j.molec.sym_nrepres);
+    j.molec.nclasses = j.bset_contr[0].Nclasses;
+    bset_contr_factory(&j.bset_contr[1], j.jvals[0], j.molec.sym_degen, j.molec.sym_nrepres);
+    bset_contr_factory(&j.bset_contr[2], j.jvals[1], j.molec.sym_degen, j.molec.sym_nrepres);
+    correlate_index(j.bset_contr[0], j.bset_contr[0]);
+    correlate_index(j.bset_contr[0], j.bset_contr[1]);
+    correlate_index(j.bset_contr[0], j.bset_contr[2]);
+    read_dipole_flipped(j.bset_contr[0], &j.dipole_me, j.dip_size);
+    compute_ijterms(j.bset_contr[1], &j.bset_contr[1].ijterms, j.molec.sym_nrepres);
+    compute_ijterms(j.bset_contr[2], &j.bset_contr[2].ijterms, j.molec.sym_nrepres);
+    read_eigenvalues(j);
+    find_igamma_pair(j);
+    std::fflush(stdout);
+    std::cout.flush();
+    dup2(saved, 1);
+    close(saved);
+    close(nul);
+    if (chdir(cwd) != 0) return r;
+    return r;
+}
+
+// Sizes: nrep, nmodes, nclasses, nlevels, N (J=0 contracted functions).
+void ref_job_sizes(void *h, int *out)
+{
+    FintensityJob &j = ((RefJob *)h)->job;
+    out[0] = j.molec.sym_nrepres;
+    out[1] = j.molec.nmodes;
+    out[2] = j.molec.nclasses;
+    out[3] = j.Neigenlevels;
+    out[4] = j.bset_contr[0].Maxcontracts;
+}
+
+// Deck fields as read_fields leaves them (isym_do from GNS).
+void ref_job_params(void *h, double *d, int *i)
+{
+    FintensityJob &j = ((RefJob *)h)->job;
+    const int nrep = j.molec.sym_nrepres, nm = j.molec.nmodes;
+    d[0] = j.ZPE; d[1] = j.temperature; d[2] = j.q_stat; d[3] = j.thresh_linestrength;
+    d[4] = j.freq_window[0]; d[5] = j.freq_window[1];
+    d[6] = j.erange_lower[0]; d[7] = j.erange_lower[1];
+    d[8] = j.erange_upper[0]; d[9] = j.erange_upper[1];
+    for (int g = 0; g < nrep; g++) d[10 + g] = j.gns[g];
+    i[0] = j.jvals[0]; i[1] = j.jvals[1]; i[2] = j.reduced ? 1 : 0; i[3] = j.molec.sym_type;
+    for (int g = 0; g < nrep; g++) {
+        i[4 + g] = j.isym_pairs[g];
+        i[4 + nrep + g] = j.isym_do[g] ? 1 : 0;
+        i[4 + 2 * nrep + g] = j.igamma_pair[g];
+    }
+    int *q = i + 4 + 3 * nrep;
+    for (int m = 0; m < nm; m++) {
+        q[m] = j.quanta_lower[0][m]; q[nm + m] = j.quanta_lower[1][m];
+        q[2 * nm + m] = j.quanta_upper[0][m]; q[3 * nm + m] = j.quanta_upper[1][m];
+    }
+}
+
+// Basis `which` (0: J=0, 1: jvals[0], 2: jvals[1]): sizes then arrays in the
+// layout of gi_bset (N transposed to [g][case]).
+void ref_job_bset_sizes(void *h, int which, int *out)
+{
+    TO_bset_contrT &b = ((RefJob *)h)->job.bset_contr[which];
+    out[0] = b.jval; out[1] = b.Maxsymcoeffs; out[2] = b.Maxcontracts; out[3] = b.mat_size;
+}
+
+void ref_job_ntotal(void *h, int which, int *out)
+{
+    FintensityJob &j = ((RefJob *)h)->job;
+    for (int g = 0; g < j.molec.sym_nrepres; g++) out[g] = j.bset_contr[which].Ntotal[g];
+}
+
+void ref_job_bset(void *h, int which, int *icontr2icase, int *iroot, int *k, int *ktau, int *N,
+                  int *Ntotal, int *ijterms, int *nsize, double **repres)
+{
+    FintensityJob &j = ((RefJob *)h)->job;
+    TO_bset_contrT &b = j.bset_contr[which];
+    const int nrep = j.molec.sym_nrepres, d = b.Maxcontracts, nc = b.Maxsymcoeffs;
+    std::memcpy(icontr2icase, b.icontr2icase, sizeof(int) * 2 * d);
+    std::memcpy(iroot, b.iroot_correlat_j0, sizeof(int) * d);
+    std::memcpy(k, b.k, sizeof(int) * d);
+    std::memcpy(ktau, b.ktau, sizeof(int) * d);
+    for (int g = 0; g < nrep; g++) {
+        Ntotal[g] = b.Ntotal[g];
+        for (int c = 0; c < nc; c++) N[g * nc + c] = b.irr[g].N[c];
+        if (which > 0) {
+            for (int c = 0; c < nc; c++) ijterms[c + g * nc] = b.ijterms[c + g * nc];
+            nsize[g] = j.isym_do[g] ? b.nsize[g] : -1;
+        }
+        std::memcpy(repres[g], b.irr[g].repres,
+                    sizeof(double) * b.Ntotal[g] * j.molec.sym_degen[g] * b.mat_size);
+    }
+}
+
+void ref_job_dipole(void *h, double *out)
+{
+    FintensityJob &j = ((RefJob *)h)->job;
+    const size_t n = j.bset_contr[0].Maxcontracts;
+    std::memcpy(out, j.dipole_me, sizeof(double) * 3 * n * n);
+}
+
+// Level table in the reference's order (its qsort).
+void ref_job_levels(void *h, double *energy, int *jind, int *igamma, int *ndeg, int *irec0,
+                    int *ilevel, int *quanta, int *krot, int *taurot)
+{
+    FintensityJob &j = ((RefJob *)h)->job;
+    const int nm = j.molec.nmodes;
+    for (int i = 0; i < j.Neigenlevels; i++) {
+        const TO_PTeigen &e = j.eigen[i];
+        energy[i] = e.energy; jind[i] = e.jind; igamma[i] = e.igamma; ndeg[i] = e.ndeg;
+        irec0[i] = e.irec[0] - 1; ilevel[i] = e.ilevel; krot[i] = e.krot; taurot[i] = e.taurot;
+        for (int q = 0; q <= nm; q++) quanta[i * (nm + 1) + q] = e.quanta[q];
+    }
 }
 
 }  // extern "C"

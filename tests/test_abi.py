@@ -9,8 +9,8 @@ from gpu_intensities_amd import binding
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def header_symbols():
-    txt = open(os.path.join(ROOT, "include", "gpu_intensities.h")).read()
+def header_symbols(name="gpu_intensities.h"):
+    txt = open(os.path.join(ROOT, "include", name)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(gi_[a-z_0-9]+)\s*\(", txt)))
 
@@ -32,3 +32,16 @@ def test_every_declared_symbol_is_exported():
 def test_library_is_gfx950_code_object():
     data = open(binding.LIB_PATH, "rb").read()
     assert b"gfx950" in data
+
+
+def test_trove_header_symbols_exported():
+    lib = ctypes.CDLL(binding.LIB_PATH)
+    syms = header_symbols("gi_trove.h")
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    assert sorted(binding.TROVE_EXPORTS) == syms
+
+
+def test_driver_built():
+    exe = os.path.join(ROOT, "gpu-intensities_amd", "gi_intens")
+    assert os.path.exists(exe) and os.access(exe, os.X_OK), "run __graft_entry__.build() first"

@@ -1,0 +1,99 @@
+"""End to end on the GPU: TROVE files + deck in, the reference's line list out.
+
+A synthetic job is written as TROVE files; `gi_intens` (the command-line
+driver) reads the deck on stdin and the files, runs the engine and prints the
+line list. Every printed transition is compared with the FP64 oracle run on
+the in-memory job: the same transitions (matched by J, symmetry and the
+level's row in its descr file), S / A / I within the printed precision
+(%16.8e) plus the mode's tolerance, and the amplitudes likewise.
+"""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from gpu_intensities_amd import binding, synth, trove_io
+import oracle_py
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "gpu-intensities_amd", "gi_intens")
+NUM = r"[-+0-9.eE]+"
+
+
+def parse(text, labels):
+    lines = text.splitlines()
+    assert lines[0].startswith("Linestrength S(f<-i)") and lines[1:3] == ["", ""]
+    out = []
+    for ln in lines[3:]:
+        jF, sF, jI, sI = int(ln[0:4]), ln[5:9].strip(), int(ln[14:18]), ln[19:23].strip()
+        m = re.search(r"\)\s+(" + NUM + r")\s+(" + NUM + r")\s+(" + NUM + r")\s+(\d+) <-\s+(\d+)",
+                      ln)
+        S, A, I = float(m.group(1)), float(m.group(2)), float(m.group(3))
+        lF, lI = int(m.group(4)) - 1, int(m.group(5)) - 1
+        amps = [float(x) for x in ln[ln.rindex(") ") + 2:].split()]
+        out.append(((jF, labels.index(sF), lF), (jI, labels.index(sI), lI), S, A, I, amps))
+    return out
+
+
+def run_driver(d, deck, mode):
+    p = subprocess.run([EXE, "--dir", d, "--mode", mode], input=deck.encode(),
+                       capture_output=True, timeout=300)
+    assert p.returncode == 0, p.stderr.decode()
+    return p.stdout.decode()
+
+
+def oracle_lines(job):
+    ref = oracle_py.OracleJob(job).run()
+    lv = job.levels
+    key = lambda i: (int(lv["jval"][i]), int(lv["igamma"][i]), int(lv["ilevel"][i]))
+    d = {}
+    for r in range(len(ref["S"])):
+        iI, iF = int(ref["ilevelI"][r]), int(ref["ilevelF"][r])
+        ndI, ndF = int(lv["ndeg"][iI]), int(lv["ndeg"][iF])
+        amps = [ref["amp"][r][dI + 2 * dF] for dF in range(ndF) for dI in range(ndI)]
+        d[(key(iF), key(iI))] = (ref["S"][r], ref["A_einst"][r], ref["I"][r], amps)
+    return d
+
+
+@pytest.mark.parametrize("mode,tol", [("fp64", 1e-10), ("batched", 1e-10), ("split", 1e-6)])
+@pytest.mark.parametrize("sym", ["C2V", "C3V"])
+def test_driver_line_list_vs_oracle(tmp_path, sym, mode, tol):
+    kw = dict(nvib=40, jvals=(0, 1), seed=13) if sym == "C2V" else \
+        dict(nvib=24, jvals=(2, 3), seed=14, reduced=1)
+    job = synth.make_job(sym, **kw)
+    d = str(tmp_path)
+    deck = trove_io.write_trove(job, d)
+    got = parse(run_driver(d, deck, mode), job.sym["labels"])
+    want = oracle_lines(job)
+    assert len(got) == len(want) > 0
+    Smax = max(v[0] for v in want.values())
+    for F, I, S, A, Iv, amps in got:
+        rS, rA, rI, ramps = want[(F, I)]
+        if rS < 1e-6 * Smax and mode == "split":
+            continue  # the split-mode bar covers lines above 1e-6 of the strongest
+        for x, y in ((S, rS), (A, rA), (Iv, rI)):
+            assert abs(x - y) <= (tol + 6e-9) * abs(y) + 1e-300, (F, I, x, y)
+        assert len(amps) == len(ramps)
+        amax = max(abs(a) for a in ramps)
+        for x, y in zip(amps, ramps):
+            assert abs(x - y) <= (tol ** 0.5 if mode == "split" else tol) * amax + 6e-10 * abs(y)
+
+
+def test_trove_load_equals_direct_load(tmp_path, engine):
+    """gi_trove_load feeds the engine exactly what the in-memory job does."""
+    job = synth.make_job("C3V", nvib=24, jvals=(2, 3), seed=15)
+    d = str(tmp_path)
+    deck = trove_io.write_trove(job, d)
+    engine.load(job)
+    a = {k: v.copy() for k, v in engine.compute(mode=binding.GI_MODE_FP64).items()}
+    t = binding.Trove(d, deck)
+    t.load(engine)
+    b = engine.compute(mode=binding.GI_MODE_FP64)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    text = t.format(b)
+    assert text.count("\n") == len(b["S"])
