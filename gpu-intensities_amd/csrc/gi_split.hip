@@ -201,18 +201,24 @@ __device__ inline void oz_step(const int8_t *__restrict__ As, const int8_t *__re
                                int wm, int wn, int lane, v4i (&acc)[2][2][S])
 {
     const int i16 = lane & 15, k16 = (lane >> 4) * 16;
+    // every B digit plane of the wave's two column blocks stays in registers
+    // for the step, so LDS delivers each operand once per wavefront
+    v4i b[S][2];
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+        b[j][0] = *(const v4i *)(Bs + (j * 64 + wn + i16) * 80 + k16);
+        b[j][1] = *(const v4i *)(Bs + (j * 64 + wn + 16 + i16) * 80 + k16);
+    }
 #pragma unroll
     for (int i = 0; i < S; i++) {
         const v4i a0 = *(const v4i *)(As + (i * 64 + wm + i16) * 80 + k16);
         const v4i a1 = *(const v4i *)(As + (i * 64 + wm + 16 + i16) * 80 + k16);
 #pragma unroll
         for (int j = 0; j < S - i; j++) {
-            const v4i b0 = *(const v4i *)(Bs + (j * 64 + wn + i16) * 80 + k16);
-            const v4i b1 = *(const v4i *)(Bs + (j * 64 + wn + 16 + i16) * 80 + k16);
-            acc[0][0][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b0, acc[0][0][i + j], 0, 0, 0);
-            acc[0][1][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b1, acc[0][1][i + j], 0, 0, 0);
-            acc[1][0][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b0, acc[1][0][i + j], 0, 0, 0);
-            acc[1][1][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b1, acc[1][1][i + j], 0, 0, 0);
+            acc[0][0][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b[j][0], acc[0][0][i + j], 0, 0, 0);
+            acc[0][1][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b[j][1], acc[0][1][i + j], 0, 0, 0);
+            acc[1][0][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b[j][0], acc[1][0][i + j], 0, 0, 0);
+            acc[1][1][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b[j][1], acc[1][1][i + j], 0, 0, 0);
         }
     }
 }
