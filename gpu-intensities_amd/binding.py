@@ -32,7 +32,7 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_params", "gi_uplo
            "gi_kernel_stats", "gi_work_stats", "gi_partition_function", "gi_serve_start",
            "gi_serve_submit", "gi_serve_wait", "gi_serve_fetch", "gi_serve_stats",
            "gi_serve_stop", "gi_set_tiles", "gi_get_tiles", "gi_set_split_digits",
-           "gi_prepare_split"]
+           "gi_prepare_split", "gi_set_jvals"]
 # (TROVE_EXPORTS below: the symbols of include/gi_trove.h)
 
 
@@ -82,6 +82,7 @@ def lib():
         L.gi_last_error.argtypes = [vp]
         L.gi_last_error.restype = C.c_char_p
         L.gi_set_params.argtypes = [vp, C.POINTER(GiParams)]
+        L.gi_set_jvals.argtypes = [vp, ci, P_i]
         L.gi_upload_bset.argtypes = [vp, ci, C.POINTER(GiBset)]
         L.gi_upload_dipole.argtypes = [vp, ci, vp, ci]
         L.gi_set_levels.argtypes = [vp, ci, P_d, P_i, P_i, P_i, P_i, P_i]
@@ -170,13 +171,16 @@ class Engine:
                      temperature=job.temperature, q_stat=job.q_stat,
                      freq_window=(cd * 2)(*job.freq_window),
                      erange_lower=(cd * 2)(*job.erange_lower),
-                     erange_upper=(cd * 2)(*job.erange_upper), jvals=(ci * 2)(*job.jvals),
+                     erange_upper=(cd * 2)(*job.erange_upper), jvals=(ci * 2)(*job.jvals[:2]),
                      gns=self._f64(job.gns), isym_pairs=self._i32(job.isym_pairs),
                      reduced=job.reduced, quanta_lower=self._i32(job.quanta_lower),
                      quanta_upper=self._i32(job.quanta_upper),
                      thresh_linestrength=job.thresh_linestrength,
                      thresh_intensity=job.thresh_intensity)
         self._chk(L.gi_set_params(self.h, C.byref(p)), "gi_set_params")
+        if len(job.jvals) != 2:
+            jv = np.ascontiguousarray(job.jvals, np.int32)
+            self._chk(L.gi_set_jvals(self.h, len(jv), _p(jv, P_i)), "gi_set_jvals")
         for jind, b in enumerate(job.bsets):
             reps = (P_d * nrep)(*[self._f64(r) for r in b.repres])
             self._keep.append(reps)
@@ -199,7 +203,7 @@ class Engine:
                                   self._i32(lv["jind"]), self._i32(lv["igamma"]),
                                   self._i32(lv["ndeg"]), self._i32(lv["irec0"]),
                                   self._i32(lv["quanta"].reshape(-1))), "gi_set_levels")
-        for jind in range(2):
+        for jind in range(len(job.jvals)):
             for g in range(nrep):
                 nsize = int(job.bsets[jind].nsize[g])
                 if device_vectors is not None and (jind, g) in device_vectors:

@@ -187,16 +187,15 @@ int prepare_split(gi_ctx *c)
 static void free_plan(gi_ctx *c)
 {
     free_split(c);
-    for (int j = 0; j < 2; j++) {
-        for (auto &m : c->jp[j].sym) free_sym(m);
-        (void)hipFree(c->jp[j].d_r_grid);
-        c->jp[j] = JPlan();
-        for (int f = 0; f < 2; f++) {
-            Coef &cf = c->coef[j][f];
-            (void)hipFree(cf.d_ptr); (void)hipFree(cf.d_sI); (void)hipFree(cf.d_val);
-            cf = Coef();
-        }
+    for (auto &p : c->jp) {
+        for (auto &m : p.sym) free_sym(m);
+        (void)hipFree(p.d_r_grid);
     }
+    c->jp.clear();
+    for (auto &cf : c->coef) {
+        (void)hipFree(cf.d_ptr); (void)hipFree(cf.d_sI); (void)hipFree(cf.d_val);
+    }
+    c->coef.clear();
     for (auto &b : c->blocks) {
         (void)hipFree(b.d_lev); (void)hipFree(b.d_col); (void)hipFree(b.d_en);
         (void)hipFree(b.d_upok);
@@ -276,6 +275,18 @@ void gi_destroy(gi_ctx *c)
 
 const char *gi_last_error(const gi_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
+// Drops the bases and eigenvector blocks: they belong to the J list.
+static void reset_inputs(gi_ctx *c)
+{
+    for (auto &b : c->blocks)
+        if (b.owned) (void)hipFree((void *)b.d_vec);
+    free_split(c);
+    c->blocks.clear();
+    c->blocks.resize((size_t)c->nJ * c->nrep);
+    c->have_bset.assign(c->nJ, 0);
+    c->hb.assign(c->nJ, gi_ctx::HostBset());
+}
+
 int gi_set_params(gi_ctx *c, const gi_params *p)
 {
     if (!c || !p) return GI_EINVAL;
@@ -302,8 +313,9 @@ int gi_set_params(gi_ctx *c, const gi_params *p)
         c->freq_window[i] = p->freq_window[i];
         c->erange_lower[i] = p->erange_lower[i];
         c->erange_upper[i] = p->erange_upper[i];
-        c->jvals[i] = p->jvals[i];
     }
+    c->nJ = 2;
+    c->jvals.assign(p->jvals, p->jvals + 2);
     c->gns.assign(p->gns, p->gns + p->nrepres);
     c->isym_pairs.assign(p->isym_pairs, p->isym_pairs + p->nrepres);
     c->reduced = p->reduced;
@@ -326,20 +338,35 @@ int gi_set_params(gi_ctx *c, const gi_params *p)
         if (c->gns[gI] != c->gns[c->igamma_pair[gI]])
             return fail(c, GI_EINVAL, "selection rules do not agree with GNS");
     }
-    for (auto &b : c->blocks)
-        if (b.owned) (void)hipFree((void *)b.d_vec);
-    c->blocks.clear();
-    c->blocks.resize(2 * c->nrep);
-    c->have_bset[0] = c->have_bset[1] = false;
+    reset_inputs(c);
     c->have_params = true;
+    return GI_OK;
+}
+
+int gi_set_jvals(gi_ctx *c, int nj, const int *jvals)
+{
+    if (!c || !jvals || nj < 1 || nj > kMaxJ) return GI_EINVAL;
+    if (serving(c)) return GI_ESTATE;
+    if (!c->have_params) return fail(c, GI_ESTATE, "gi_set_jvals before gi_set_params");
+    for (int i = 0; i < nj; i++)
+        if (jvals[i] < 0 || (i > 0 && jvals[i] <= jvals[i - 1]))
+            return fail(c, GI_EINVAL, "gi_set_jvals: J values must be >= 0 and ascending");
+    free_plan(c);
+    c->nJ = nj;
+    c->jvals.assign(jvals, jvals + nj);
+    reset_inputs(c);
+    c->nlevels = 0;
+    c->lv_energy.clear(); c->lv_jind.clear(); c->lv_igamma.clear();
+    c->lv_ndeg.clear(); c->lv_irec0.clear(); c->lv_quanta.clear();
     return GI_OK;
 }
 
 int gi_upload_bset(gi_ctx *c, int jind, const gi_bset *b)
 {
-    if (!c || !b || jind < 0 || jind > 1) return GI_EINVAL;
+    if (!c || !b) return GI_EINVAL;
     if (serving(c)) return GI_ESTATE;
     if (!c->have_params) return fail(c, GI_ESTATE, "gi_upload_bset before gi_set_params");
+    if (jind < 0 || jind >= c->nJ) return fail(c, GI_EINVAL, "gi_upload_bset: jind outside the J list");
     if (b->jval != c->jvals[jind])
         return fail(c, GI_EINVAL, "gi_upload_bset: jval differs from jvals[jind]");
     if (b->Maxcontracts < 0 || b->Maxsymcoeffs < 0 || b->mat_size < 1)
@@ -397,7 +424,7 @@ int gi_set_levels(gi_ctx *c, int n, const double *energy, const int *jind, const
     if (serving(c)) return GI_ESTATE;
     free_plan(c);
     for (int i = 0; i < n; i++) {
-        if (jind[i] < 0 || jind[i] > 1 || igamma[i] < 0 || igamma[i] >= c->nrep)
+        if (jind[i] < 0 || jind[i] >= c->nJ || igamma[i] < 0 || igamma[i] >= c->nrep)
             return fail(c, GI_EINVAL, "gi_set_levels: jind/igamma out of range at " + std::to_string(i));
         if (ndeg[i] != c->degen[igamma[i]])
             return fail(c, GI_EINVAL, "gi_set_levels: ndeg differs from the symmetry's degeneracy");
@@ -417,9 +444,10 @@ int gi_set_levels(gi_ctx *c, int n, const double *energy, const int *jind, const
 int gi_upload_vectors(gi_ctx *c, int jind, int igamma, int nsize, int nrec, const double *v,
                       int on_device)
 {
-    if (!c || jind < 0 || jind > 1 || nsize < 0 || nrec < 0) return GI_EINVAL;
+    if (!c || nsize < 0 || nrec < 0) return GI_EINVAL;
     if (serving(c)) return GI_ESTATE;
     if (!c->have_params) return fail(c, GI_ESTATE, "gi_upload_vectors before gi_set_params");
+    if (jind < 0 || jind >= c->nJ) return fail(c, GI_EINVAL, "gi_upload_vectors: jind outside the J list");
     if (igamma < 0 || igamma >= c->nrep) return GI_EINVAL;
     HIPCHK(hipSetDevice(c->device));
     free_plan(c);
@@ -448,15 +476,21 @@ int gi_prepare(gi_ctx *c)
 {
     if (!c) return GI_EINVAL;
     if (serving(c)) return GI_ESTATE;
-    if (!c->have_params || !c->have_bset[0] || !c->have_bset[1] || !c->d_mu)
-        return fail(c, GI_ESTATE, "gi_prepare: parameters, both bases and the dipole are needed");
+    if (!c->have_params || !c->d_mu)
+        return fail(c, GI_ESTATE, "gi_prepare: parameters, every basis and the dipole are needed");
+    for (int j = 0; j < c->nJ; j++)
+        if (!c->have_bset[j])
+            return fail(c, GI_ESTATE, "gi_prepare: no basis for J=" + std::to_string(c->jvals[j]));
     HIPCHK(hipSetDevice(c->device));
     free_plan(c);
-    c->jmax = std::max(c->jvals[0], c->jvals[1]);
+    const int nJ = c->nJ;
+    c->jp.assign(nJ, JPlan());
+    c->coef.assign((size_t)nJ * nJ, Coef());
+    c->jmax = *std::max_element(c->jvals.begin(), c->jvals.end());
     c->threej.assign((size_t)(c->jmax + 1) * (c->jmax + 1) * 9, 0.0);
     threej_table(c->jmax, c->threej.data());
     int rc;
-    for (int j = 0; j < 2; j++) {
+    for (int j = 0; j < nJ; j++) {
         for (int g = 0; g < c->nrep; g++)
             if (c->hb[j].nsize[g] != c->hb[j].Ntotal[g])
                 return fail(c, GI_EINVAL, "nsize differs from Ntotal for J=" +
@@ -474,7 +508,7 @@ int gi_prepare(gi_ctx *c)
         HIPCHK(hipStreamSynchronize(c->stream));
         (void)hipFree(d_perm);
     }
-    for (int j = 0; j < 2; j++) {
+    for (int j = 0; j < nJ; j++) {
         if ((rc = build_symmaps(c, j)) != GI_OK) return rc;
         JPlan &p = c->jp[j];
         std::vector<int> rg(p.dimen);
@@ -489,21 +523,23 @@ int gi_prepare(gi_ctx *c)
             if ((rc = upload(c, &m.d_tval, m.tval)) != GI_OK) return rc;
         }
     }
-    for (int jI = 0; jI < 2; jI++)
-        for (int jF = 0; jF < 2; jF++) {
+    for (int jI = 0; jI < nJ; jI++)
+        for (int jF = 0; jF < nJ; jF++) {
+            if (std::abs(c->jvals[jI] - c->jvals[jF]) > 1) continue;
             if ((rc = build_coef(c, jI, jF)) != GI_OK) return rc;
-            Coef &cf = c->coef[jI][jF];
+            Coef &cf = coef_of(c, jI, jF);
             if (!cf.valid) continue;
             if ((rc = upload(c, &cf.d_ptr, cf.ptr)) != GI_OK) return rc;
             if ((rc = upload(c, &cf.d_sI, cf.sI)) != GI_OK) return rc;
             if ((rc = upload(c, &cf.d_val, cf.val)) != GI_OK) return rc;
         }
-    c->groups.assign((size_t)2 * c->nrep * 2, GroupPlan());
-    for (int jI = 0; jI < 2; jI++)
+    c->groups.assign((size_t)nJ * c->nrep * nJ, GroupPlan());
+    for (int jI = 0; jI < nJ; jI++)
         for (int g = 0; g < c->nrep; g++)
-            for (int jF = 0; jF < 2; jF++) {
+            for (int jF = 0; jF < nJ; jF++) {
+                if (!coef_of(c, jI, jF).valid) continue;
                 if ((rc = build_group(c, jI, g, jF)) != GI_OK) return rc;
-                GroupPlan &G = c->groups[((size_t)jI * c->nrep + g) * 2 + jF];
+                GroupPlan &G = group_of(c, jI, g, jF);
                 if (!G.valid) continue;
                 for (auto &oc : G.oc) {
                     std::vector<int> ints;
@@ -549,6 +585,8 @@ int gi_prepare(gi_ctx *c)
         B.upok.clear();
     }
     for (int i = 0; i < c->nlevels; i++) {
+        if (c->lv_jind[i] >= nJ)
+            return fail(c, GI_EINVAL, "level " + std::to_string(i) + ": jind outside the J list");
         const int J = c->jvals[c->lv_jind[i]];
         const int *q = &c->lv_quanta[(size_t)i * nq];
         c->lv_lowok[i] = energy_filter_lower(c, J, c->lv_energy[i], q);
@@ -586,10 +624,11 @@ int gi_prepare(gi_ctx *c)
 
 int gi_half_ls(gi_ctx *c, int jindI, int jindF, int gI, int ideg, const double *vec, double *h)
 {
-    if (!c || !vec || !h || jindI < 0 || jindI > 1 || jindF < 0 || jindF > 1) return GI_EINVAL;
+    if (!c || !vec || !h) return GI_EINVAL;
     if (!c->prepared) return fail(c, GI_ESTATE, "gi_half_ls before gi_prepare");
+    if (jindI < 0 || jindI >= c->nJ || jindF < 0 || jindF >= c->nJ) return GI_EINVAL;
     if (gI < 0 || gI >= c->nrep || ideg < 0 || ideg >= c->degen[gI]) return GI_EINVAL;
-    const Coef &cf = c->coef[jindI][jindF];
+    const Coef &cf = coef_of(c, jindI, jindF);
     if (!cf.valid) return fail(c, GI_EINVAL, "gi_half_ls: |J' - J| > 1");
     HIPCHK(hipSetDevice(c->device));
     const JPlan &pI = c->jp[jindI], &pF = c->jp[jindF];
@@ -622,8 +661,9 @@ int gi_half_ls(gi_ctx *c, int jindI, int jindF, int gI, int ideg, const double *
 
 int gi_correlate(gi_ctx *c, int jind, int g, int ideg, const double *vec, double *out)
 {
-    if (!c || !vec || !out || jind < 0 || jind > 1) return GI_EINVAL;
+    if (!c || !vec || !out) return GI_EINVAL;
     if (!c->prepared) return fail(c, GI_ESTATE, "gi_correlate before gi_prepare");
+    if (jind < 0 || jind >= c->nJ) return GI_EINVAL;
     if (g < 0 || g >= c->nrep || ideg < 0 || ideg >= c->degen[g]) return GI_EINVAL;
     HIPCHK(hipSetDevice(c->device));
     const JPlan &p = c->jp[jind];

@@ -207,7 +207,7 @@ static void items_job(const gi_ctx *c, int nlower, const int *ids, int query, Pl
         const double eI = c->lv_energy[L];
         const int gF = c->igamma_pair[gI];
         if (c->isym_pairs[gI] != c->isym_pairs[gF]) continue;
-        for (int jindF = 0; jindF < 2; jindF++) {
+        for (int jindF = 0; jindF < c->nJ; jindF++) {
             const int jF = c->jvals[jindF];
             if (!((jF != c->jvals[0]) || (jI != c->jvals[0]))) continue;
             if (std::abs(jF - jI) > 1 || jI + jF < 1) continue;
@@ -273,7 +273,7 @@ static int build_chunks(gi_ctx *c, Plan &P)
             std::stable_sort(its.begin(), its.end(), [&](int x, int y) {
                 return c->lv_energy[P.items[x].level] < c->lv_energy[P.items[y].level];
             });
-        const GroupPlan &GP = c->groups[((size_t)jindI * c->nrep + gI) * 2 + jindF];
+        const GroupPlan &GP = group_of(c, jindI, gI, jindF);
         if (!GP.valid) return fail(c, GI_EINVAL, "internal: no plan for a (J, J') pair");
         for (size_t s0 = 0; s0 < its.size(); s0 += kChunk) {
             const int n_it = (int)std::min<size_t>(kChunk, its.size() - s0);
@@ -493,7 +493,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
     std::vector<size_t> bytes(chunks.size());
     for (size_t i = 0; i < chunks.size(); i++) {
         const Chunk &ch = chunks[i];
-        const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
+        const GroupPlan &GP = group_of(c, ch.jindI, ch.gI, ch.jindF);
         const int ncb = c->degen[ch.gI] * c->degen[ch.gF];
         const int nsizeF = c->blocks[ch.jindF * c->nrep + ch.gF].nsize;
         need[i] = Need{(size_t)ch.nunits * GP.Gtot_p, (size_t)ch.nunits * GP.Htot,
@@ -551,7 +551,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         for (size_t i = rg.first; i < rg.second; i++) {
             const Chunk &ch = chunks[i];
             const JPlan &pI = c->jp[ch.jindI];
-            const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
+            const GroupPlan &GP = group_of(c, ch.jindI, ch.gI, ch.jindF);
             const int ndegI = c->degen[ch.gI];
             SymDev r0 = symdev(pI.sym[ch.gI * GI_MAXDEG + 0]);
             SymDev r1 = ndegI > 1 ? symdev(pI.sym[ch.gI * GI_MAXDEG + 1]) : r0;
@@ -650,7 +650,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         for (size_t i = rg.first; i < rg.second; i++) {
             const Chunk &ch = chunks[i];
             const JPlan &pF = c->jp[ch.jindF];
-            const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
+            const GroupPlan &GP = group_of(c, ch.jindI, ch.gI, ch.jindF);
             const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
             const int ndegI = c->degen[ch.gI], ndegF = c->degen[ch.gF];
             SymDev t0 = symdev(pF.sym[ch.gF * GI_MAXDEG + 0]);
@@ -675,7 +675,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             mm.push_back(m);
             double blk_bytes = 0;
             for (const OutClass &oc : GP.oc) blk_bytes += 8.0 * oc.n * oc.K;
-            P.work[3] += ch.nunits * 2.0 * c->coef[ch.jindI][ch.jindF].C * (double)N * N;
+            P.work[3] += ch.nunits * 2.0 * coef_of(c, ch.jindI, ch.jindF).C * (double)N * N;
             P.work[4] += ch.nunits * 2.0 * GP.macs;
             P.work[5] += (double)ch.ncand * 8.0 * BF.nsize;
             P.work[6] += blk_bytes;
@@ -730,7 +730,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
     std::vector<size_t> bytes(chunks.size());
     for (size_t i = 0; i < chunks.size(); i++) {
         const Chunk &ch = chunks[i];
-        const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
+        const GroupPlan &GP = group_of(c, ch.jindI, ch.gI, ch.jindF);
         const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
         const size_t cols = (size_t)ch.nitems * c->degen[ch.gI] * c->degen[ch.gF];
         if (GP.gW > kGMapMaxW || !GP.d_gidxB[0])
@@ -785,7 +785,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
         std::vector<size_t> Hoff, Htoff, Hdoff, Hcoff;
         for (size_t i = rg.first; i < rg.second; i++) {
             const Chunk &ch = chunks[i];
-            const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
+            const GroupPlan &GP = group_of(c, ch.jindI, ch.gI, ch.jindF);
             const Block &BI = c->blocks[ch.jindI * c->nrep + ch.gI];
             const int ndegI = c->degen[ch.gI];
             Hoff.push_back(oh); Htoff.push_back(oht); Hdoff.push_back(ohd); Hcoff.push_back(ohc);
@@ -866,7 +866,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             const Chunk &ch = chunks[i];
             const size_t k = i - rg.first;
             const JPlan &pF = c->jp[ch.jindF];
-            const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
+            const GroupPlan &GP = group_of(c, ch.jindI, ch.gI, ch.jindF);
             const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
             const int ndegI = c->degen[ch.gI], ndegF = c->degen[ch.gF];
             SymDev t0 = symdev(pF.sym[ch.gF * GI_MAXDEG + 0]);
@@ -901,7 +901,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             if (rows > 0 && cols > 0) ct += ((rows + 63) / 64) * ((cols + 63) / 64);
             double blk_bytes = 0;
             for (const OutClass &oc : GP.oc) blk_bytes += 8.0 * oc.n * oc.K;
-            P.work[3] += ch.nunits * 2.0 * c->coef[ch.jindI][ch.jindF].C * (double)N * N;
+            P.work[3] += ch.nunits * 2.0 * coef_of(c, ch.jindI, ch.jindF).C * (double)N * N;
             P.work[4] += ch.nunits * 2.0 * GP.macs;
             P.work[5] += (double)ch.ncand * 8.0 * BF.nsize;
             P.work[6] += blk_bytes;
@@ -992,7 +992,7 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
     } else if (!P.chunks.empty()) {
         int max_units = 1, max_items = 1, max_G = 1, max_H = 1, max_ncb = 1, max_nsizeF = 1;
         for (auto &ch : P.chunks) {
-            const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
+            const GroupPlan &GP = group_of(c, ch.jindI, ch.gI, ch.jindF);
             max_units = std::max(max_units, ch.nunits);
             max_items = std::max(max_items, ch.nitems);
             max_G = std::max(max_G, GP.Gtot);
@@ -1009,7 +1009,7 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
         if (rc) return rc;
         for (auto &ch : P.chunks) {
             const JPlan &pI = c->jp[ch.jindI], &pF = c->jp[ch.jindF];
-            const GroupPlan &GP = c->groups[((size_t)ch.jindI * c->nrep + ch.gI) * 2 + ch.jindF];
+            const GroupPlan &GP = group_of(c, ch.jindI, ch.gI, ch.jindF);
             const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
             const int ndegI = c->degen[ch.gI], ndegF = c->degen[ch.gF];
             SymDev r0 = symdev(pI.sym[ch.gI * GI_MAXDEG + 0]);
@@ -1051,7 +1051,7 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
             TL_END();
             double blk_bytes = 0;
             for (const OutClass &oc : GP.oc) blk_bytes += 8.0 * oc.n * oc.K;
-            P.work[3] += ch.nunits * 2.0 * c->coef[ch.jindI][ch.jindF].C * (double)N * N;
+            P.work[3] += ch.nunits * 2.0 * coef_of(c, ch.jindI, ch.jindF).C * (double)N * N;
             P.work[4] += ch.nunits * 2.0 * GP.macs;
             P.work[5] += (double)ch.ncand * 8.0 * BF.nsize;
             P.work[6] += ch.nunits * blk_bytes;
@@ -1397,7 +1397,7 @@ static double request_bytes(gi_ctx *c, const gi_request &r, long long *ncand)
     items_request(c, r, 0, P);
     *ncand = P.ncand;
     const int gF = c->igamma_pair[r.igammaI];
-    const GroupPlan &GP = c->groups[((size_t)r.jindI * c->nrep + r.igammaI) * 2 + r.jindF];
+    const GroupPlan &GP = group_of(c, r.jindI, r.igammaI, r.jindF);
     const double units = (double)P.items.size() * c->degen[r.igammaI];
     const double ncb = (double)c->degen[r.igammaI] * c->degen[gF];
     const double nsizeF = c->blocks[r.jindF * c->nrep + gF].nsize;
@@ -1456,10 +1456,10 @@ int gi_serve_submit(gi_ctx *c, const gi_request *r, long long *ticket)
     if (!c || !r || !ticket) return GI_EINVAL;
     Server *sv = c->server;
     if (!sv) return fail(c, GI_ESTATE, "gi_serve_submit without gi_serve_start");
-    if (r->jindI < 0 || r->jindI > 1 || r->jindF < 0 || r->jindF > 1 || r->igammaI < 0 ||
+    if (r->jindI < 0 || r->jindI >= c->nJ || r->jindF < 0 || r->jindF >= c->nJ || r->igammaI < 0 ||
         r->igammaI >= c->nrep)
         return GI_EINVAL;
-    if (!c->groups[((size_t)r->jindI * c->nrep + r->igammaI) * 2 + r->jindF].valid)
+    if (!group_of(c, r->jindI, r->igammaI, r->jindF).valid)
         return GI_EINVAL;  // |J' - J| > 1
     if (r->mode != GI_MODE_FP64 && r->mode != GI_MODE_FP64_BATCHED && r->mode != GI_MODE_SPLIT)
         return GI_EINVAL;

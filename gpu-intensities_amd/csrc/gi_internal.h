@@ -13,6 +13,11 @@
 #define GI_MAXDEG 2
 
 namespace gi {
+// Largest J list of a context (gi_set_jvals).
+constexpr int kMaxJ = 128;
+}  // namespace gi
+
+namespace gi {
 
 // Symmetrisation map of one (J, symmetry, component): the sparse matrix R of
 // the correlate step (dipole_kernals.cu:38-71) over the slot grid, and its
@@ -153,16 +158,20 @@ struct gi_ctx {
     std::vector<double> gns;
     double zpe = 0, temperature = 0, q_stat = 1;
     double freq_window[2] = {0, 0}, erange_lower[2] = {0, 0}, erange_upper[2] = {0, 0};
-    int jvals[2] = {0, 0};
+    // J values of the context: the deck's two (jvals[0] is the one of the Q-branch
+    // exclusion), or an ascending list set by gi_set_jvals; jind indexes it
+    std::vector<int> jvals = {0, 0};
+    int nJ = 2;
     double thresh_S = 0, thresh_I = 0;
 
     // inputs
-    bool have_bset[2] = {false, false};
+    std::vector<char> have_bset = {0, 0};
     struct HostBset {
-        int jval, ncases, dimen, mat_size;
+        int jval = 0, ncases = 0, dimen = 0, mat_size = 0;
         std::vector<int> icontr2icase, v, k, ktau, N, Ntotal, ijterms, nsize;
         std::vector<std::vector<double>> repres;
-    } hb[2];
+    };
+    std::vector<HostBset> hb = std::vector<HostBset>(2);
     int nvib = 0;
     double *d_mu = nullptr;
     bool mu_owned = false;
@@ -176,14 +185,14 @@ struct gi_ctx {
     bool prepared = false;
     int jmax = 0;
     std::vector<double> threej;
-    gi::JPlan jp[2];
-    gi::Coef coef[2][2];             // [jindI][jindF]
+    std::vector<gi::JPlan> jp;       // [jind]
+    std::vector<gi::Coef> coef;      // [jindI*nJ + jindF]
     std::vector<int> vperm, vpos;    // vibrational permutation (new -> old, old -> new)
     std::vector<int> vcls_off, vcls_n;   // symmetry classes: contiguous ranges of new indices
     std::vector<int> vpad_off;       // class offsets with every class padded to 64 (split mode)
     int kpad_all = 0;
     double *d_mu_perm = nullptr;     // mu with rows and columns permuted by vperm
-    std::vector<gi::GroupPlan> groups;   // [(jindI*nrep + gI)*2 + jindF]
+    std::vector<gi::GroupPlan> groups;   // [(jindI*nrep + gI)*nJ + jindF]
 
     // main execution slot (stream = the context's stream), grown on demand
     gi::Slot main_slot;
@@ -213,6 +222,21 @@ namespace gi {
 // so the worker never writes the context's error string.
 int fail(gi_ctx *c, int code, const std::string &msg);
 extern thread_local std::string *err_sink;
+
+// Plan tables indexed by positions in the context's J list.
+inline Coef &coef_of(gi_ctx *c, int jindI, int jindF) { return c->coef[(size_t)jindI * c->nJ + jindF]; }
+inline const Coef &coef_of(const gi_ctx *c, int jindI, int jindF)
+{
+    return c->coef[(size_t)jindI * c->nJ + jindF];
+}
+inline GroupPlan &group_of(gi_ctx *c, int jindI, int gI, int jindF)
+{
+    return c->groups[((size_t)jindI * c->nrep + gI) * c->nJ + jindF];
+}
+inline const GroupPlan &group_of(const gi_ctx *c, int jindI, int gI, int jindF)
+{
+    return c->groups[((size_t)jindI * c->nrep + gI) * c->nJ + jindF];
+}
 
 // Register-tile width (columns) of the half-ls kernels for `cols` columns.
 inline int smax_for(int cols)

@@ -75,7 +75,7 @@ bool energy_filter_lower(const gi_ctx *c, int J, double e, const int *q)
         if (q[i + 1] < c->quanta_lower[i]) return false;
         if (q[i + 1] > c->quanta_lower[c->nmodes + i]) return false;
     }
-    return J >= c->jvals[0] && J <= c->jvals[1] && (e - c->zpe) >= c->erange_lower[0] &&
+    return J >= c->jvals.front() && J <= c->jvals.back() && (e - c->zpe) >= c->erange_lower[0] &&
            (e - c->zpe) <= c->erange_lower[1];
 }
 
@@ -86,7 +86,7 @@ bool energy_filter_upper(const gi_ctx *c, int J, double e, const int *q)
         if (q[i + 1] < c->quanta_upper[i]) return false;
         if (q[i + 1] > c->quanta_upper[c->nmodes + i]) return false;
     }
-    return J >= c->jvals[0] && J <= c->jvals[1] && (e - c->zpe) >= c->erange_upper[0] &&
+    return J >= c->jvals.front() && J <= c->jvals.back() && (e - c->zpe) >= c->erange_upper[0] &&
            (e - c->zpe) <= c->erange_upper[1];
 }
 
@@ -158,7 +158,7 @@ int build_vib_classes(gi_ctx *c)
 {
     const int N = c->nvib;
     std::vector<std::vector<long long>> sig(N);
-    for (int jind = 0; jind < 2; jind++) {
+    for (int jind = 0; jind < c->nJ; jind++) {
         const auto &b = c->hb[jind];
         const JPlan &p = c->jp[jind];
         for (int r = 0; r < b.dimen; r++) {
@@ -211,7 +211,7 @@ int build_vib_classes(gi_ctx *c)
         c->vpad_off[k] = c->kpad_all;
         c->kpad_all += (c->vcls_n[k] + 63) / 64 * 64;
     }
-    for (int jind = 0; jind < 2; jind++)
+    for (int jind = 0; jind < c->nJ; jind++)
         for (auto &v : c->jp[jind].r_v) v = c->vpos[v];
     return GI_OK;
 }
@@ -275,9 +275,9 @@ static bool row_nonempty(const SymMap &m, long long row)
 // Symmetry-blocked plan of the half linestrength from (J, gI) to J'.
 int build_group(gi_ctx *c, int jindI, int gI, int jindF)
 {
-    GroupPlan &G = c->groups[((size_t)jindI * c->nrep + gI) * 2 + jindF];
+    GroupPlan &G = group_of(c, jindI, gI, jindF);
     G = GroupPlan();
-    const Coef &cf = c->coef[jindI][jindF];
+    const Coef &cf = coef_of(c, jindI, jindF);
     if (!cf.valid) return GI_OK;
     const int gF = c->igamma_pair[gI];
     const JPlan &pI = c->jp[jindI], &pF = c->jp[jindF];
@@ -479,7 +479,7 @@ int build_gmap(gi_ctx *c, GroupPlan &G)
 // labels, gathered per (slot_F, component).
 int build_coef(gi_ctx *c, int jindI, int jindF)
 {
-    Coef &cf = c->coef[jindI][jindF];
+    Coef &cf = coef_of(c, jindI, jindF);
     cf = Coef();
     const JPlan &pI = c->jp[jindI], &pF = c->jp[jindF];
     const int jI = pI.J, jF = pF.J;

@@ -362,14 +362,17 @@ def make_job(sym="C2V", nvib=200, jvals=(0, 1), seed=1234, *, e_span=8000.0,
     vectors='none' leaves them for the caller (e.g. built on the GPU)."""
     sy = dict(C2V if sym.upper().startswith("C2V") else C3V)
     nrep = sy["nrep"]
-    j0, j1 = int(jvals[0]), int(jvals[1])
+    jv = tuple(int(j) for j in jvals)
+    if len(jv) < 2 or (len(jv) > 2 and any(b <= a for a, b in zip(jv, jv[1:]))):
+        raise ValueError("jvals: the deck's two J values, or an ascending list")
+    j0, j1 = jv[0], jv[1]
     rng_b = _rng(seed, 1)
     if sy["sym_type"] == 1:
         if nvib % 4:
             raise ValueError("C2v generator wants nvib divisible by 4")
         gvib = np.repeat(np.arange(4), nvib // 4)
         gvib = gvib[rng_b.permutation(nvib)]
-        bsets = [build_bset_c2v(J, gvib, _rng(seed, 2, J), shuffle_cases) for J in (j0, j1)]
+        bsets = [build_bset_c2v(J, gvib, _rng(seed, 2, J), shuffle_cases) for J in jv]
         units = [(int(gvib[v]), [v]) for v in range(nvib)]
         gns_d = [1.0, 1.0, 3.0, 3.0]
         pairs_d = [1, 1, 2, 2]
@@ -386,7 +389,7 @@ def make_job(sym="C2V", nvib=200, jvals=(0, 1), seed=1234, *, e_span=8000.0,
             else:
                 units.append((kd if kd < 2 else 0, [v]))
                 v += 1
-        bsets = [build_bset_c3v(J, units, _rng(seed, 2, J)) for J in (j0, j1)]
+        bsets = [build_bset_c3v(J, units, _rng(seed, 2, J)) for J in jv]
         gns_d = [1.0, 1.0, 1.0]
         pairs_d = [1, 1, 2]
         nclasses = 1
@@ -394,11 +397,11 @@ def make_job(sym="C2V", nvib=200, jvals=(0, 1), seed=1234, *, e_span=8000.0,
     pairs = np.asarray(isym_pairs if isym_pairs is not None else pairs_d, np.int32)
     igp = find_igamma_pair(pairs)
     mu = make_dipole(nvib, _rng(seed, 4), symmetric=symmetric_mu)
-    levels = _levels_for_blocks(sy, (j0, j1), bsets, nmodes, nclasses, 0.0, e_span,
+    levels = _levels_for_blocks(sy, jv, bsets, nmodes, nclasses, 0.0, e_span,
                                 _rng(seed, 5))
     vecs = {}
     if vectors == "numpy":
-        for jind, J in enumerate((j0, j1)):
+        for jind, J in enumerate(jv):
             for g in range(nrep):
                 n = int(bsets[jind].nsize[g])
                 q = orthogonal_block(n, _rng(seed, 6, J, g))
@@ -408,11 +411,11 @@ def make_job(sym="C2V", nvib=200, jvals=(0, 1), seed=1234, *, e_span=8000.0,
     ql = np.array([0] * nmodes + [big] * nmodes, np.int32)
     return Job(sym=sy, nmodes=nmodes, nclasses=nclasses, zpe=0.0, temperature=temperature,
                q_stat=q_stat, freq_window=tuple(freq_window), erange_lower=tuple(erange_lower),
-               erange_upper=tuple(erange_upper), jvals=(j0, j1), gns=gns, isym_pairs=pairs,
+               erange_upper=tuple(erange_upper), jvals=jv, gns=gns, isym_pairs=pairs,
                igamma_pair=igp, reduced=int(reduced), quanta_lower=ql.copy(),
                quanta_upper=ql.copy(), thresh_linestrength=float(thresh_linestrength),
                thresh_intensity=float(thresh_intensity), bsets=bsets, nvib=nvib,
-               dipole_flipped=mu, jmax=max(j0, j1), levels=levels, vectors=vecs,
+               dipole_flipped=mu, jmax=max(jv), levels=levels, vectors=vecs,
                meta=dict(seed=seed, e_span=e_span, vib_units=units,
                          shuffle_cases=bool(shuffle_cases)))
 
@@ -429,3 +432,4 @@ def drop_level_filter(job: Job):
     keep = isym_do[job.levels["igamma"]] & (e >= lo) & (e <= hi)
     job.levels = {k: np.ascontiguousarray(v[keep]) for k, v in job.levels.items()}
     return job
+

@@ -106,8 +106,20 @@ const char *gi_last_error(const gi_ctx *ctx);
  * (trove_functions.cpp:1041-1072; an ambiguous pairing is GI_EINVAL). */
 int gi_set_params(gi_ctx *ctx, const gi_params *p);
 
-/* Basis of jvals[jind] (jind = 0, 1). Replaces
- * create_and_copy_bset_contr_to_gpu (cuda_host.cu:368-376). */
+/* J list of the context. gi_set_params sets the deck's two J values
+ * (jvals[0], jvals[1]); this replaces them with nj ascending values, so that
+ * one context holds the bases, eigenvector blocks and levels of many J and
+ * computes every pair with |J' - J| <= 1 in the same launches. The reference
+ * holds two J per run (fields.h:61 bset_contr[3], nJ = 2 at
+ * cuda_host.cu:1005) and is run once per (J, J+1); with the Q-branch rule
+ * (trove_functions.cpp:1457: no J = J' = jvals[0]) the list jvals[0..nj-1]
+ * gives exactly the union of the runs (jvals[i], jvals[i+1]), i < nj - 1.
+ * Call after gi_set_params; drops the bases, eigenvector blocks and level
+ * table (their jind index this list). nj: 1..128. */
+int gi_set_jvals(gi_ctx *ctx, int nj, const int *jvals);
+
+/* Basis of jvals[jind] (jind = 0, 1, or an index into the gi_set_jvals
+ * list). Replaces create_and_copy_bset_contr_to_gpu (cuda_host.cu:368-376). */
 int gi_upload_bset(gi_ctx *ctx, int jind, const gi_bset *b);
 
 /* Flipped dipole mu_a(vF, vI) at vF + vI*nvib + a*nvib^2, a = x, y, z.

@@ -6,6 +6,7 @@ The product code never imports this module.
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 import os
 import numpy as np
 
@@ -209,6 +210,38 @@ class OracleJob:
                          _p(self.threej, P_d), self.job.jmax, _p(v, P_d), _p(h, P_d),
                          int(noskip))
         return h
+
+
+def pair_job(job, i):
+    """The two-J job of the run (jvals[i], jvals[i+1]) cut from a job with a J
+    list: the same bases, dipole, eigenvector blocks and parameters, and the
+    level table restricted to those two J (jind renumbered 0, 1). Returns the
+    job and the indices of its levels in the full table."""
+    keep = np.nonzero((job.levels["jind"] == i) | (job.levels["jind"] == i + 1))[0]
+    lv = {k: np.ascontiguousarray(v[keep]) for k, v in job.levels.items()}
+    lv["jind"] = np.ascontiguousarray(lv["jind"] - i, np.int32)
+    vecs = {(jind - i, g): v for (jind, g), v in job.vectors.items() if jind in (i, i + 1)}
+    sub = dataclasses.replace(job, jvals=(job.jvals[i], job.jvals[i + 1]),
+                              bsets=[job.bsets[i], job.bsets[i + 1]], levels=lv,
+                              vectors=vecs, jmax=max(job.jvals[i], job.jvals[i + 1]),
+                              meta=dict(job.meta))
+    return sub, keep
+
+
+def run_chain(job, nthreads=0):
+    """Lines of a job with a J list the way the reference produces them: one
+    run per consecutive pair (jvals[i], jvals[i+1]) — a run holds two J
+    (fields.h:61, cuda_host.cu:1005) — with level ids mapped back to the full
+    table. The Q-branch rule of each run (trove_functions.cpp:1457) keeps
+    every (J, J') pair in exactly one run."""
+    parts = []
+    for i in range(len(job.jvals) - 1):
+        sub, keep = pair_job(job, i)
+        r = OracleJob(sub).run(nthreads=nthreads)
+        r["ilevelI"] = keep[r["ilevelI"]].astype(np.int32)
+        r["ilevelF"] = keep[r["ilevelF"]].astype(np.int32)
+        parts.append(r)
+    return {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
 
 
 def three_j(*a):
