@@ -3,14 +3,20 @@
 Workload (default): C2v(M), Nvib = 2000, J 10 <-> 11 (blocks of 10,500 and
 11,500 symmetrised functions, 88,000 levels), lower levels with
 E <= 10,000 cm^-1, upper levels unrestricted, frequency window
-0-10,000 cm^-1, T = 296 K. Synthetic seeded data: eigenvector blocks are QR
-factors of N(0,1) blocks built on the GPU, the dipole is
+0-10,000 cm^-1, T = 296 K. Synthetic seeded data: eigenvector blocks are the
+Q factors of N(0,1) blocks built on the GPU, the dipole is
 10^(-|v-v'|/3) * LogNormal(0,1) with random signs.
 
 One step = one pass of gi_compute over every lower level (half
 linestrengths, correlation, intensity, filter, compaction and the copy of
 the emitted lines to pinned host memory). Inputs are resident in HBM when
 the timed region starts.
+
+After the headline workload (reported as `value`), the same measurement on
+one context holding C2v J = 0..20 (Nvib = 2000: 84 blocks of 500..20,500
+functions, 99 GB of FP64 eigenvectors plus 74 GB of split digit planes, every
+|dJ| <= 1 pair in the same launches; `multi_j`), and the Poisson request
+sweep against those resident blocks (`serving`).
 
 Prints one JSON line. --gpus above 1 is refused: this build measures one GPU.
 """
@@ -49,30 +55,89 @@ def build_c2_job(torch, nvib=2000, jvals=(10, 11), seed=20240, lower_max=10000.0
                          freq_window=window, erange_lower=(0.0, lower_max),
                          erange_upper=(0.0, 1e9), vectors="none")
     log(f"host generation {time.time() - t:.1f}s; levels {job.nlevels}")
+    dev = device_blocks(torch, job, seed, f"{nvib}_{jvals[0]}_{jvals[1]}_{seed}", cache_dir)
+    return job, dev
+
+
+def _right_solve(torch, r, q, b=2048):
+    """X with X R = Q, R upper triangular: the library triangular solve, or
+    (if it fails) column blocks of GEMMs with b x b diagonal-block inverses."""
+    try:
+        return torch.linalg.solve_triangular(r, q, upper=True, left=False)
+    except RuntimeError:
+        pass
+    x = torch.empty_like(q)
+    for j in range(0, r.shape[0], b):
+        e = min(j + b, r.shape[0])
+        rhs = q[:, j:e] - x[:, :j] @ r[:j, j:e] if j else q[:, j:e]
+        x[:, j:e] = rhs @ torch.linalg.inv(r[j:e, j:e])
+    return x
+
+
+def gpu_orthogonal(torch, n, seed):
+    """Q of A = QR for a seeded N(0,1) block A, diag(R) > 0 (the unique such
+    factor), by CholeskyQR2: two passes of R = chol(Q^T Q), Q <- Q R^-1
+    (DGEMM, Cholesky, triangular solve), as orthogonal as Householder QR
+    (|Q^T Q - 1| ~ 1e-14 at n = 20480) in a third of the time. Householder QR
+    if a Cholesky factorisation fails (A too ill-conditioned)."""
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    a = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=gen)
+    q = a
+    for _ in range(2):
+        r, info = torch.linalg.cholesky_ex(q.t() @ q, upper=True)
+        if int(info) != 0:
+            break
+        q = _right_solve(torch, r, q)
+        del r
+    else:
+        return q
+    q, r = torch.linalg.qr(a)
+    q *= torch.sign(torch.diagonal(r))[None, :]
+    return q
+
+
+def device_blocks(torch, job, seed, tag, cache_dir=None):
+    """Eigenvector block of every (J, symmetry) of the job on the GPU:
+    record irec0 = column irec0 of a Haar-random orthogonal matrix."""
     t = time.time()
     dev = {}
-    gen = torch.Generator(device="cuda")
-    tag = f"{nvib}_{jvals[0]}_{jvals[1]}_{seed}"
-    for jind in range(2):
+    for jind in range(len(job.jvals)):
         for g in range(job.sym["nrep"]):
             n = int(job.bsets[jind].nsize[g])
             path = os.path.join(cache_dir, f"u_{tag}_{jind}_{g}.npy") if cache_dir else None
             if path and os.path.exists(path):
                 dev[(jind, g)] = torch.from_numpy(np.load(path)).cuda()
                 continue
-            gen.manual_seed(seed * 1000 + jind * 10 + g)
-            a = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=gen)
-            q, r = torch.linalg.qr(a)
-            q *= torch.sign(torch.diagonal(r))[None, :]
-            del a, r
-            dev[(jind, g)] = q.t().contiguous()  # record irec0 = column irec0 of q
+            q = gpu_orthogonal(torch, n, seed * 1000 + jind * 10 + g)
+            dev[(jind, g)] = q.t().contiguous()
             del q
             if path:
                 os.makedirs(cache_dir, exist_ok=True)
                 np.save(path + ".tmp.npy", dev[(jind, g)].cpu().numpy())
                 os.replace(path + ".tmp.npy", path)
     torch.cuda.synchronize()
-    log(f"eigenvector blocks ready {time.time() - t:.1f}s (QR on the GPU, or the cache)")
+    torch.cuda.empty_cache()
+    gb = sum(v.numel() for v in dev.values()) * 8 / 1e9
+    log(f"eigenvector blocks ready {time.time() - t:.1f}s, {gb:.1f} GB "
+        f"(CholeskyQR2 on the GPU, or the cache)")
+    return dev
+
+
+def build_list_job(torch, nvib=2000, jmax=20, seed=20250, lower_max=10000.0,
+                   window=(0.0, 10000.0), e_span=10000.0 * 88000 / 3000, cache_dir=None):
+    """C2v J = 0..jmax in one context: every (J, symmetry) block, level
+    density per block as in the J 10 <-> 11 workload (e_span), lower levels
+    with E <= lower_max, frequency window `window`."""
+    import __graft_entry__ as ge
+    ge.load_package()
+    from gpu_intensities_amd import synth
+    t = time.time()
+    job = synth.make_job("C2V", nvib=nvib, jvals=tuple(range(jmax + 1)), seed=seed,
+                         e_span=e_span, freq_window=window, erange_lower=(0.0, lower_max),
+                         erange_upper=(0.0, 1e9), vectors="none")
+    log(f"host generation {time.time() - t:.1f}s; levels {job.nlevels}")
+    dev = device_blocks(torch, job, seed, f"list{nvib}_{jmax}_{seed}", cache_dir)
     return job, dev
 
 
@@ -156,32 +221,35 @@ def split_report(S, Sref, strong=1e-6):
             "median_rel_S_strong": float(np.median(rel)), "within_1e-6": bool(rel.max() <= 1e-6)}
 
 
-def split_at_temperature(eng, job, binding, oracle_py, T, nthr, elow=(0.0, 1000.0)):
-    """Split-mode error at another temperature: one request (J 10 -> 11, A1
-    lower levels with E - ZPE in elow) with Q(T) from the device, against the
-    oracle run with the same T, Q and lower levels."""
+def split_at_temperature(eng, job, binding, oracle_py, T, nthr, elow=(0.0, 1000.0),
+                         jindI=0, igammaI=0, jindF=1):
+    """Split-mode error at another temperature: one request (lower block
+    (jindI, igammaI), upper J jindF, lower levels with E - ZPE in elow) with
+    Q(T) from the device, against the oracle run with the same T, Q and lower
+    levels."""
     import dataclasses
     Q = eng.partition_function(T)
     eng.serve_start(8 << 30)
     try:
-        got = eng.serve_fetch(eng.serve_submit(jindI=0, igammaI=0, jindF=1,
+        got = eng.serve_fetch(eng.serve_submit(jindI=jindI, igammaI=igammaI, jindF=jindF,
                                                mode=binding.GI_MODE_SPLIT, temperature=T,
                                                q_stat=Q, elow=elow))
     finally:
         eng.serve_stop()
     lv = job.levels
     e = lv["energy"] - job.zpe
-    ids = np.nonzero((lv["jind"] == 0) & (lv["igamma"] == 0) & (e >= elow[0]) &
+    ids = np.nonzero((lv["jind"] == jindI) & (lv["igamma"] == igammaI) & (e >= elow[0]) &
                      (e <= elow[1]))[0].astype(np.int32)
-    ref = oracle_py.OracleJob(dataclasses.replace(job, temperature=T, q_stat=Q)).run(
-        ids, nthreads=nthr)
-    keep = lv["jind"][ref["ilevelF"]] == 1
+    ref = oracle_py.run_chain(dataclasses.replace(job, temperature=T, q_stat=Q), ids,
+                              nthreads=nthr)
+    keep = lv["jind"][ref["ilevelF"]] == jindF
     ref = {k: v[keep] for k, v in ref.items()}
     kg = np.lexsort((got["ilevelF"], got["ilevelI"]))
     kr = np.lexsort((ref["ilevelF"], ref["ilevelI"]))
     same = (len(kg) == len(kr) and np.array_equal(got["ilevelI"][kg], ref["ilevelI"][kr])
             and np.array_equal(got["ilevelF"][kg], ref["ilevelF"][kr]))
-    out = {"T": T, "Q": Q, "lower_levels": int(len(ids)), "lines": int(len(kr)),
+    out = {"T": T, "Q": Q, "J": [int(job.jvals[jindI]), int(job.jvals[jindF])],
+           "lower_levels": int(len(ids)), "lines": int(len(kr)),
            "same_transition_set": bool(same)}
     if same and len(kr):
         out.update(split_report(got["S"][kg], ref["S"][kr]))
@@ -199,9 +267,10 @@ def serve_sweep(eng, job, binding, lambdas, seconds, mode, seed=7, budget=8 << 3
     the library); throughput = lines / (last completion - first submit)."""
     import threading
     rng = np.random.default_rng(seed)
-    j0 = job.jvals[0]
-    pairs = [(a, b) for a in range(2) for b in range(2)
-             if abs(job.jvals[a] - job.jvals[b]) <= 1 and not (job.jvals[a] == job.jvals[b] == j0)]
+    jv = job.jvals
+    nj = len(jv)
+    pairs = [(a, b) for a in range(nj) for b in range(nj)
+             if abs(jv[a] - jv[b]) <= 1 and jv[a] + jv[b] >= 1 and not (jv[a] == jv[b] == jv[0])]
     nrep = job.sym["nrep"]
     res = {}
     for lam in lambdas:
@@ -245,6 +314,157 @@ def serve_sweep(eng, job, binding, lambdas, seconds, mode, seed=7, budget=8 << 3
     return res
 
 
+def time_modes(eng, binding, bufs, mode_names, steps, warmup):
+    """Time `steps` passes of gi_compute over every lower level per mode."""
+    import torch
+    modes = {}
+    for mname in mode_names:
+        mode = binding.MODES[mname]
+        for _ in range(warmup):
+            eng.compute(bufs=bufs, mode=mode)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            res = eng.compute(bufs=bufs, mode=mode)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ms = (t1 - t0) / steps * 1e3
+        nl = int(len(res["S"]))
+        kstats = eng.kernel_stats()
+        work = eng.work_stats()
+        modes[mname] = dict(ms_per_step=ms, transitions=nl, value=nl / (ms / 1e3),
+                            kernels={k: dict(total_ms=round(v["total_ms"], 4),
+                                             launches=v["launches"]) for k, v in kstats.items()},
+                            work=work, _kstats=kstats)
+        log(f"{mname}: {ms:.2f} ms/step, {nl} transitions, {nl / (ms / 1e3):.4g}/s; "
+            f"kernels {modes[mname]['kernels']}")
+    return modes
+
+
+def sample_parity(eng, binding, ref, sample, bufs, mode_names):
+    """Each mode on the sample's lower levels against the oracle's lines:
+    transition set, FP64 amplitude tolerance (with the condition floor), and
+    for the split mode the error report over the strong lines."""
+    out = {}
+    kr = np.lexsort((ref["ilevelF"], ref["ilevelI"]))
+    for mname in mode_names:
+        got = eng.compute(sample, bufs=bufs, mode=binding.MODES[mname])
+        kg = np.lexsort((got["ilevelF"], got["ilevelI"]))
+        same = (len(kg) == len(kr) and np.array_equal(got["ilevelI"][kg], ref["ilevelI"][kr])
+                and np.array_equal(got["ilevelF"][kg], ref["ilevelF"][kr]))
+        p = dict(sample_lines=int(len(kr)), same_transition_set=bool(same))
+        if same and len(kr):
+            floor = np.maximum(np.abs(ref["amp"][kr]), ref["amp_abs"][kr])
+            err = np.abs(got["amp"][kg] - ref["amp"][kr])
+            rel = np.abs(got["S"][kg] - ref["S"][kr]) / np.maximum(ref["S"][kr], 1e-300)
+            p.update({"max_rel_S": float(rel.max()), "median_rel_S": float(np.median(rel)),
+                      "amp_within_1e-10": bool((err <= 1e-10 * floor).all()),
+                      "lines_via_floor": int((rel > 1e-10).sum())})
+        if same and len(kr) and mname == "split":
+            p.update(split_report(got["S"][kg], ref["S"][kr]))
+        out[mname] = p
+    return out
+
+
+def mode_rooflines(modes, nvib):
+    """Matrix-core utilisation of the two product kernels of each batched mode."""
+    out = {}
+    for mname, m in modes.items():
+        ks = m["_kstats"]
+        for k in ("half_ls_mfma", "corr_mfma", "half_ls_split", "corr_split"):
+            if k in ks and ks[k]["launches"]:
+                r = _roofline(k, ks, m["work"], nvib)
+                out.setdefault(mname, {})[k] = {x: r[x] for x in
+                                                ("achieved", "peak", "unit", "frac")}
+    return out
+
+
+def list_section(torch, binding, args, oracle_py, nthr):
+    """C2v J = 0..jmax in one context: every (J, J') pair with |J' - J| <= 1
+    in the same launches (timed per mode), a sampled oracle check at full
+    size, and the Poisson request sweep against the resident blocks."""
+    job, dev = build_list_job(torch, nvib=args.list_nvib, jmax=args.list_jmax,
+                              cache_dir=args.cache_dir)
+    eng = binding.Engine(0)
+    t = time.time()
+    eng.load(job, device_vectors=dev)
+    t_prep = time.time() - t
+    mode_names = [m for m in args.list_modes.split(",") if m]
+    t = time.time()
+    if "split" in mode_names or "split" in args.serve_modes.split(","):
+        eng.prepare_split()
+    torch.cuda.synchronize()
+    t_split = time.time() - t
+    sizes = [int(b.nsize[g]) for b in job.bsets for g in range(job.sym["nrep"])]
+    fp64_gb = sum(n * n for n in sizes) * 8 / 1e9
+    log(f"list context: prepare {t_prep:.1f}s, split planes {t_split:.1f}s, "
+        f"blocks {min(sizes)}..{max(sizes)}, FP64 blocks {fp64_gb:.1f} GB")
+    cap = eng.count_candidates()
+    bufs = eng.alloc_lines(cap, pinned=True)
+    lv = job.levels
+    lower_ok = np.nonzero((lv["energy"] - job.zpe >= job.erange_lower[0]) &
+                          (lv["energy"] - job.zpe <= job.erange_lower[1]))[0]
+    log(f"list candidates {cap}, lower levels {len(lower_ok)}")
+    modes = time_modes(eng, binding, bufs, mode_names, args.list_steps, 1)
+    roofs = mode_rooflines(modes, job.nvib)
+    for m in modes.values():
+        m.pop("_kstats", None)
+    out = dict(workload=f"C2v Nvib={job.nvib} J 0..{args.list_jmax} in one context, every "
+                        "|dJ|<=1 pair, lower E<=10000 cm-1, window 0-10000 cm-1, T=296 K",
+               nvib=job.nvib, jvals=[int(job.jvals[0]), int(job.jvals[-1])],
+               blocks=len(sizes), block_sizes=[min(sizes), max(sizes)],
+               fp64_blocks_GB=round(fp64_gb, 2), levels=int(job.nlevels),
+               lower_levels=int(len(lower_ok)), candidates=int(cap),
+               prepare_s=round(t_prep, 2), split_planes_s=round(t_split, 2),
+               modes=modes, matrix_core=roofs)
+    if "fp64" in modes:
+        for m in modes:
+            modes[m]["speedup_vs_fp64_gemv"] = modes[m]["value"] / modes["fp64"]["value"]
+    if oracle_py is not None:
+        # oracle sample: J=1 lower levels (every 2nd) and J=jmax A1 lower
+        # levels (every 25th); only the blocks those touch go to the host
+        J = lv["jval"][lower_ok]
+        g = lv["igamma"][lower_ok]
+        jm = int(job.jvals[-1])
+        sample = np.concatenate([lower_ok[J == 1][::2],
+                                 lower_ok[(J == jm) & (g == 0)][::25]]).astype(np.int32)
+        gF = int(job.igamma_pair[0])
+        need = {(j, gg) for j in (0, 1, 2) for gg in range(job.sym["nrep"])}
+        need |= {(jm, 0), (jm, gF), (jm - 1, gF)}
+        t = time.time()
+        for key in need:
+            job.vectors[key] = dev[key].cpu().numpy()
+        t = time.perf_counter()
+        ref = oracle_py.run_chain(job, sample, nthreads=nthr)
+        tc = time.perf_counter() - t
+        par = sample_parity(eng, binding, ref, sample, bufs, mode_names)
+        par["sample"] = (f"{len(sample)} lower levels (J=1 every 2nd, J={jm} A1 every 25th), "
+                         f"oracle chain of pair runs {tc:.1f} s")
+        if "split" in mode_names:
+            par["split_1500K"] = split_at_temperature(eng, job, binding, oracle_py, 1500.0, nthr,
+                                                      elow=(0.0, 300.0), jindI=jm, igammaI=0,
+                                                      jindF=jm)
+        for key in need:
+            del job.vectors[key]
+        out["parity"] = par
+        log(f"list parity {par}")
+    serving = None
+    lams = [float(x) for x in args.serve_lambdas.split(",") if x]
+    if lams:
+        serving = {"workload": f"Poisson requests against the resident J 0..{args.list_jmax} "
+                               "blocks: random (J, J') pair and lower symmetry, T ~ U[200,2000] K "
+                               "with Q(T) on the device, 1000 cm-1 window at a random start in "
+                               "[0, 9000], intensity threshold 10^U(-32,-26); lower levels of "
+                               "the block with E <= 10000 cm-1"}
+        for mname in [m for m in args.serve_modes.split(",") if m]:
+            serving[mname] = serve_sweep(eng, job, binding, lams, args.serve_seconds,
+                                         binding.MODES[mname])
+    eng.close()
+    del dev, bufs
+    torch.cuda.empty_cache()
+    return out, serving
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,9 +476,15 @@ def main():
     ap.add_argument("--modes", default="fp64,fp64_batched,split",
                     help="comma-separated modes to time; the fastest is reported as value")
     ap.add_argument("--cache-dir", default=None,
-                    help="reuse the QR eigenvector blocks across processes (profiler runs)")
+                    help="reuse the eigenvector blocks across processes (profiler runs)")
+    ap.add_argument("--list-jmax", type=int, default=20,
+                    help="J list 0..jmax in one context (the J-list workload); -1 to skip")
+    ap.add_argument("--list-nvib", type=int, default=2000)
+    ap.add_argument("--list-modes", default="fp64,fp64_batched,split")
+    ap.add_argument("--list-steps", type=int, default=3)
     ap.add_argument("--serve-lambdas", default="10,50,100,200,500",
                     help="request rates (req/s) of the serving sweep; empty to skip")
+    ap.add_argument("--serve-modes", default="fp64_batched,split")
     ap.add_argument("--serve-seconds", type=float, default=2.0,
                     help="offered duration of each serving rate")
     ap.add_argument("--copy-probe", action="store_true",
@@ -268,6 +494,9 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1"}))
         sys.exit(2)
 
+    # room for the library triangular solves of the eigenvector generator
+    os.environ.setdefault("HIPBLAS_WORKSPACE_CONFIG", ":262144:2")
+    os.environ.setdefault("CUBLAS_WORKSPACE_CONFIG", ":262144:2")
     import torch
     if args.copy_probe:
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
@@ -281,6 +510,12 @@ def main():
     import __graft_entry__ as ge
     ge.load_package()
     from gpu_intensities_amd import binding
+    oracle_py = None
+    nthr = 0
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py
+        nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or oracle_py.max_threads()
 
     job, dev = build_c2_job(torch, cache_dir=args.cache_dir)
     eng = binding.Engine(0)
@@ -289,65 +524,37 @@ def main():
     bufs = eng.alloc_lines(cap, pinned=True)
     log(f"candidates {cap}")
 
-    modes = {}
-    if "split" in args.modes.split(","):
+    mode_names = [m for m in args.modes.split(",") if m]
+    if "split" in mode_names:
         t = time.time()
         eng.prepare_split()  # resident digit planes, built once like the uploads
         torch.cuda.synchronize()
         log(f"split digit planes {time.time() - t:.1f}s")
-    for mname in args.modes.split(","):
-        mode = binding.MODES[mname]
-        for _ in range(args.warmup):
-            eng.compute(bufs=bufs, mode=mode)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            res = eng.compute(bufs=bufs, mode=mode)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        ms = (t1 - t0) / args.steps * 1e3
-        nl = int(len(res["S"]))
-        kstats = eng.kernel_stats()
-        work = eng.work_stats()
-        modes[mname] = dict(ms_per_step=ms, transitions=nl, value=nl / (ms / 1e3),
-                            kernels={k: dict(total_ms=round(v["total_ms"], 4),
-                                             launches=v["launches"]) for k, v in kstats.items()},
-                            work=work, _kstats=kstats)
-        log(f"{mname}: {ms:.2f} ms/step, {nl} transitions, {nl / (ms / 1e3):.4g}/s; "
-            f"kernels {modes[mname]['kernels']}")
+    modes = time_modes(eng, binding, bufs, mode_names, args.steps, args.warmup)
     best = max(modes, key=lambda m: modes[m]["value"])
     ms_per_step = modes[best]["ms_per_step"]
     nlines = modes[best]["transitions"]
     value = modes[best]["value"]
-    kstats = modes[best].pop("_kstats")
+    kstats = modes[best]["_kstats"]
     work = modes[best]["work"]
     dom = max(kstats, key=lambda k: kstats[k]["total_ms"])
     roof = roofline_entry(dom, kstats, work, job.nvib)
+    roofs = mode_rooflines(modes, job.nvib)
     kern = modes[best]["kernels"]
     for m in modes.values():
         m.pop("_kstats", None)
-    serving = None
-    if args.serve_lambdas:
-        lams = [float(x) for x in args.serve_lambdas.split(",") if x]
-        serving = {}
-        for mname in args.modes.split(","):
-            serving[mname] = serve_sweep(eng, job, binding, lams, args.serve_seconds,
-                                         binding.MODES[mname])
     lower_ok = np.nonzero((job.levels["energy"] - job.zpe >= job.erange_lower[0]) &
                           (job.levels["energy"] - job.zpe <= job.erange_lower[1]))[0]
 
     cpu = None
     parity = None
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle_py
+    if oracle_py is not None:
         sample = lower_ok[::args.cpu_stride].astype(np.int32)
         t = time.time()
         for key, tens in dev.items():
             job.vectors[key] = tens.cpu().numpy()
         log(f"vectors to host {time.time() - t:.1f}s")
         oj = oracle_py.OracleJob(job)
-        nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or oracle_py.max_threads()
         t = time.perf_counter()
         ref = oj.run(sample, nthreads=nthr)
         tc = time.perf_counter() - t
@@ -356,35 +563,27 @@ def main():
                           f"({len(sample)} of {len(lower_ok)}), {len(ref['S'])} transitions, "
                           f"{tc:.1f} s, FP64 oracle (K-masked loops, OpenMP over lower levels)")
         log(f"cpu baseline {cpu}")
-        parity = {}
-        kr = np.lexsort((ref["ilevelF"], ref["ilevelI"]))
-        for mname in modes:
-            got = eng.compute(sample, bufs=bufs, mode=binding.MODES[mname])
-            kg = np.lexsort((got["ilevelF"], got["ilevelI"]))
-            same = (len(kg) == len(kr) and np.array_equal(got["ilevelI"][kg], ref["ilevelI"][kr])
-                    and np.array_equal(got["ilevelF"][kg], ref["ilevelF"][kr]))
-            p = dict(sample_lines=int(len(kr)), same_transition_set=bool(same))
-            if same and len(kr):
-                floor = np.maximum(np.abs(ref["amp"][kr]), ref["amp_abs"][kr])
-                err = np.abs(got["amp"][kg] - ref["amp"][kr])
-                rel = np.abs(got["S"][kg] - ref["S"][kr]) / np.maximum(ref["S"][kr], 1e-300)
-                p.update({"max_rel_S": float(rel.max()), "median_rel_S": float(np.median(rel)),
-                          "amp_within_1e-10": bool((err <= 1e-10 * floor).all()),
-                          "lines_via_floor": int((rel > 1e-10).sum())})
-            if same and len(kr) and mname == "split":
-                p.update(split_report(got["S"][kg], ref["S"][kr]))
-            parity[mname] = p
+        parity = sample_parity(eng, binding, ref, sample, bufs, mode_names)
         if "split" in modes:
             parity["split_1500K"] = split_at_temperature(eng, job, binding, oracle_py, 1500.0,
                                                          nthr)
         log(f"parity {parity}")
+        job.vectors.clear()
+    eng.close()
+    del dev, bufs
+    torch.cuda.empty_cache()
+
+    multi_j = serving = None
+    if args.list_jmax >= 1:
+        multi_j, serving = list_section(torch, binding, args, oracle_py, nthr)
 
     out = {
         "metric": METRIC, "value": value, "unit": "transitions/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": ("i8 digits x6 (split products), f64 elsewhere" if best == "split" else "f64"),
-        "data": "synthetic (seeded C2v generator, GPU QR eigenvectors)",
+        "data": "synthetic (seeded C2v generator, eigenvectors: Haar-random orthogonal "
+                "blocks built on the GPU)",
         "config": {"workload": "C2v Nvib=2000 J 10<->11, lower E<=10000 cm-1, window "
                                "0-10000 cm-1, T=296 K",
                    "nvib": job.nvib, "jvals": list(job.jvals), "lower_levels": int(len(lower_ok)),
@@ -395,11 +594,12 @@ def main():
         "kernels": kern,
         "work": work,
         "modes": modes,
+        "matrix_core": roofs,
         "parity": parity,
+        "multi_j": multi_j,
         "serving": serving,
     }
     print(json.dumps(out))
-    eng.close()
 
 
 if __name__ == "__main__":

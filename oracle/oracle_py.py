@@ -186,12 +186,17 @@ class OracleJob:
             res["amp"] = np.ctypeslib.as_array(out.amp, (n * 4,)).reshape(n, 4).copy()
             res["amp_abs"] = np.ctypeslib.as_array(out.amp_abs, (n * 4,)).reshape(n, 4).copy()
         else:
-            res = {"ilevelI": np.zeros(0, np.int32), "ilevelF": np.zeros(0, np.int32)}
-            for f in ("nu", "S", "A_einst", "I"):
-                res[f] = np.zeros(0)
-            res["amp"] = np.zeros((0, 4))
-            res["amp_abs"] = np.zeros((0, 4))
+            res = OracleJob.empty()
         L.or_free_lines(C.byref(out))
+        return res
+
+    @staticmethod
+    def empty():
+        res = {"ilevelI": np.zeros(0, np.int32), "ilevelF": np.zeros(0, np.int32)}
+        for f in ("nu", "S", "A_einst", "I"):
+            res[f] = np.zeros(0)
+        res["amp"] = np.zeros((0, 4))
+        res["amp_abs"] = np.zeros((0, 4))
         return res
 
     def correlate(self, jind, ideg, g, vec):
@@ -228,19 +233,30 @@ def pair_job(job, i):
     return sub, keep
 
 
-def run_chain(job, nthreads=0):
+def run_chain(job, lower_ids=None, nthreads=0):
     """Lines of a job with a J list the way the reference produces them: one
     run per consecutive pair (jvals[i], jvals[i+1]) — a run holds two J
     (fields.h:61, cuda_host.cu:1005) — with level ids mapped back to the full
     table. The Q-branch rule of each run (trove_functions.cpp:1457) keeps
-    every (J, J') pair in exactly one run."""
+    every (J, J') pair in exactly one run. lower_ids (full-table ids) limits
+    every run to those of its lower levels; runs with none are skipped."""
     parts = []
     for i in range(len(job.jvals) - 1):
         sub, keep = pair_job(job, i)
-        r = OracleJob(sub).run(nthreads=nthreads)
+        if lower_ids is None:
+            r = OracleJob(sub).run(nthreads=nthreads)
+        else:
+            ids = np.asarray(lower_ids)
+            pos = np.searchsorted(keep, ids)
+            ok = (pos < len(keep)) & (keep[np.minimum(pos, len(keep) - 1)] == ids)
+            if not ok.any():
+                continue
+            r = OracleJob(sub).run(pos[ok].astype(np.int32), nthreads=nthreads)
         r["ilevelI"] = keep[r["ilevelI"]].astype(np.int32)
         r["ilevelF"] = keep[r["ilevelF"]].astype(np.int32)
         parts.append(r)
+    if not parts:
+        return OracleJob.empty()
     return {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
 
 

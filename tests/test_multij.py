@@ -89,3 +89,16 @@ def test_make_job_list_shares_blocks_with_pair_jobs():
         assert np.array_equal(sub.vectors[key], b.vectors[key])
     with pytest.raises(ValueError):
         synth.make_job("C2V", nvib=16, jvals=(1, 3, 2), seed=35)
+
+
+def test_chain_with_lower_ids_is_the_subset():
+    job = synth.make_job("C2V", nvib=16, jvals=(0, 1, 2, 3), seed=36)
+    full = oracle_py.run_chain(job)
+    ids = np.arange(3, job.nlevels, 5)
+    sub = oracle_py.run_chain(job, lower_ids=ids)
+    m = np.isin(full["ilevelI"], ids)
+    a = set(zip(full["ilevelI"][m].tolist(), full["ilevelF"][m].tolist()))
+    b = list(zip(sub["ilevelI"].tolist(), sub["ilevelF"].tolist()))
+    assert len(b) == len(set(b)) and set(b) == a and len(a) > 0
+    none = oracle_py.run_chain(job, lower_ids=np.zeros(0, np.int64))
+    assert len(none["S"]) == 0
