@@ -315,11 +315,19 @@ def serve_sweep(eng, job, binding, lambdas, seconds, mode, seed=7, budget=8 << 3
 
 
 def time_modes(eng, binding, bufs, mode_names, steps, warmup):
-    """Time `steps` passes of gi_compute over every lower level per mode."""
+    """Time `steps` passes of gi_compute over every lower level per mode; the
+    matrix-core modes are first tuned on this workload by gi_autotune."""
     import torch
     modes = {}
     for mname in mode_names:
         mode = binding.MODES[mname]
+        tuning = None
+        if mode != binding.GI_MODE_FP64:
+            t = time.time()
+            best, table = eng.autotune(mode, bufs=bufs, reps=2)
+            tuning = dict(best, seconds=round(time.time() - t, 2),
+                          table=table.strip().splitlines())
+            log(f"{mname} autotune {best} in {tuning['seconds']} s")
         for _ in range(warmup):
             eng.compute(bufs=bufs, mode=mode)
         torch.cuda.synchronize()
@@ -335,7 +343,7 @@ def time_modes(eng, binding, bufs, mode_names, steps, warmup):
         modes[mname] = dict(ms_per_step=ms, transitions=nl, value=nl / (ms / 1e3),
                             kernels={k: dict(total_ms=round(v["total_ms"], 4),
                                              launches=v["launches"]) for k, v in kstats.items()},
-                            work=work, _kstats=kstats)
+                            work=work, tuning=tuning, _kstats=kstats)
         log(f"{mname}: {ms:.2f} ms/step, {nl} transitions, {nl / (ms / 1e3):.4g}/s; "
             f"kernels {modes[mname]['kernels']}")
     return modes

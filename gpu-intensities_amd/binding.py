@@ -32,7 +32,8 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_params", "gi_uplo
            "gi_kernel_stats", "gi_work_stats", "gi_partition_function", "gi_serve_start",
            "gi_serve_submit", "gi_serve_wait", "gi_serve_fetch", "gi_serve_stats",
            "gi_serve_stop", "gi_set_tiles", "gi_get_tiles", "gi_set_split_digits",
-           "gi_prepare_split", "gi_set_jvals"]
+           "gi_prepare_split", "gi_set_jvals", "gi_set_tuning", "gi_get_tuning",
+           "gi_autotune"]
 # (TROVE_EXPORTS below: the symbols of include/gi_trove.h)
 
 
@@ -62,6 +63,10 @@ class GiRequest(C.Structure):
                 ("temperature", cd), ("q_stat", cd), ("elow_lo", cd), ("elow_hi", cd),
                 ("nu_lo", cd), ("nu_hi", cd), ("thresh_linestrength", cd),
                 ("thresh_intensity", cd)]
+
+
+class GiTuning(C.Structure):
+    _fields_ = [("halfls_cfg", ci), ("corr_cfg", ci), ("pipe_rounds", ci)]
 
 
 class GiLines(C.Structure):
@@ -106,6 +111,10 @@ def lib():
         L.gi_set_split_digits.argtypes = [vp, ci]
         L.gi_prepare_split.argtypes = [vp]
         L.gi_get_tiles.argtypes = [vp, P_i, P_i]
+        L.gi_set_tuning.argtypes = [vp, C.POINTER(GiTuning)]
+        L.gi_get_tuning.argtypes = [vp, C.POINTER(GiTuning)]
+        L.gi_autotune.argtypes = [vp, ci, ci, P_i, C.POINTER(GiLines), ci, C.POINTER(GiTuning),
+                                  C.c_char_p, cll, C.POINTER(cll)]
         _LIB = L
     return _LIB
 
@@ -290,6 +299,35 @@ class Engine:
         a, b = ci(), ci()
         self._chk(lib().gi_get_tiles(self.h, C.byref(a), C.byref(b)), "gi_get_tiles")
         return a.value, b.value
+
+    def set_tuning(self, halfls_cfg, corr_cfg, pipe_rounds):
+        t = GiTuning(halfls_cfg=halfls_cfg, corr_cfg=corr_cfg, pipe_rounds=pipe_rounds)
+        self._chk(lib().gi_set_tuning(self.h, C.byref(t)), "gi_set_tuning")
+
+    def get_tuning(self):
+        t = GiTuning()
+        self._chk(lib().gi_get_tuning(self.h, C.byref(t)), "gi_get_tuning")
+        return dict(halfls_cfg=t.halfls_cfg, corr_cfg=t.corr_cfg, pipe_rounds=t.pipe_rounds)
+
+    def autotune(self, mode, lower_ids=None, bufs=None, reps=2):
+        """gi_autotune on these lower levels; returns (chosen setting, table text)."""
+        cap = self.count_candidates(lower_ids)
+        if bufs is None or len(bufs["S"]) < max(cap, 1):
+            bufs = self.alloc_lines(cap)
+        s = self._lines(bufs)
+        best = GiTuning()
+        buf = C.create_string_buffer(1 << 16)
+        n = cll()
+        if lower_ids is None:
+            ids, nids = None, 0
+        else:
+            ids = np.ascontiguousarray(lower_ids, np.int32)
+            nids = len(ids)
+        rc = lib().gi_autotune(self.h, mode, nids, None if ids is None else _p(ids, P_i),
+                               C.byref(s), reps, C.byref(best), buf, len(buf), C.byref(n))
+        self._chk(rc, "gi_autotune")
+        return (dict(halfls_cfg=best.halfls_cfg, corr_cfg=best.corr_cfg,
+                     pipe_rounds=best.pipe_rounds), buf.value.decode())
 
     def partition_function(self, temperature):
         q = cd()

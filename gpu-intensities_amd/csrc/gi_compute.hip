@@ -369,8 +369,8 @@ static LineSoA shift(LineSoA s, long long o);
 
 // Consecutive chunks grouped into rounds of at most kRoundBytes of device
 // scratch (GI_BATCH_BYTES overrides); with per-round output at least
-// kPipeRounds rounds of >= 64 MB (GI_PIPE_ROUNDS overrides).
-static std::vector<std::pair<size_t, size_t>> plan_rounds(const Plan &P,
+// c->pipe_rounds rounds of >= 1 MB (gi_set_tuning, gi_autotune).
+static std::vector<std::pair<size_t, size_t>> plan_rounds(const gi_ctx *c, const Plan &P,
                                                           const std::vector<size_t> &bytes)
 {
     size_t kRoundBytes = (size_t)12 << 30;
@@ -379,11 +379,10 @@ static std::vector<std::pair<size_t, size_t>> plan_rounds(const Plan &P,
         if (v > 0) kRoundBytes = (size_t)v;
     }
     if (P.pipe) {
-        int kPipeRounds = 2;
-        if (const char *env = std::getenv("GI_PIPE_ROUNDS")) kPipeRounds = std::max(1, std::atoi(env));
         size_t total = 0;
         for (size_t b : bytes) total += b;
-        kRoundBytes = std::min(kRoundBytes, std::max(total / kPipeRounds + 1, (size_t)64 << 20));
+        kRoundBytes = std::min(kRoundBytes,
+                               std::max(total / c->pipe_rounds + 1, (size_t)1 << 20));
     }
     std::vector<std::pair<size_t, size_t>> rounds;
     size_t start = 0, acc = 0;
@@ -500,7 +499,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                        (size_t)ch.nitems * ncb * nsizeF, ch.nunits > 0 ? (int)GP.oc.size() : 0};
         bytes[i] = 8 * (need[i].g + need[i].h + need[i].ht);
     }
-    const auto rounds = plan_rounds(P, bytes);
+    const auto rounds = plan_rounds(c, P, bytes);
     size_t mg = 1, mh = 1, mht = 1;
     int mjobs = 1, mchunks = 1;
     for (auto &rg : rounds) {
@@ -741,7 +740,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
         bytes[i] = SD * need[i].gd + 4 * need[i].ge + 8 * need[i].h + 8 * need[i].ht +
                    SD * need[i].hd + 4 * need[i].hc;
     }
-    const auto rounds = plan_rounds(P, bytes);
+    const auto rounds = plan_rounds(c, P, bytes);
     size_t mgd = 64, mge = 1, mh = 1, mht = 1, mhd = 64, mhc = 1;
     int mjobs = 1, mchunks = 1;
     for (auto &rg : rounds) {
@@ -1183,6 +1182,104 @@ int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
     return GI_OK;
 }
 
+// Project autotuner (no reference counterpart): times the launch
+// configurations of the batched modes on the caller's workload and keeps the
+// fastest. Each setting is scanned in a fixed order from the built-in default;
+// a candidate replaces the incumbent only when it is more than 3% faster, so
+// timing noise between near-equal settings does not change the pick. Tile
+// shapes are judged by their kernel's HIP-event time, the pipeline round count
+// by the wall time of the whole call (its effect is the copy overlap). Every
+// setting gives bitwise-identical lines (the K order of each product and the
+// chunk order do not depend on them).
+int autotune(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out, int reps,
+             gi_tuning *best, char *table, long long cap, long long *len)
+{
+    if (mode != GI_MODE_FP64 && mode != GI_MODE_FP64_BATCHED && mode != GI_MODE_SPLIT)
+        return fail(c, GI_EINVAL, "gi_autotune: unknown mode");
+    if (c->server) return fail(c, GI_ESTATE, "gi_autotune while the request server runs");
+    const gi_tuning saved{c->tile_halfls, c->tile_corr, c->pipe_rounds};
+    std::string txt;
+    char line[256];
+    long long ncand = 0;
+    int rc = count_job(c, nlower, ids, &ncand);
+    if (rc) return rc;
+    std::snprintf(line, sizeof line, "# gi_autotune mode=%d lower=%d candidates=%lld reps=%d\n",
+                  mode, ids ? nlower : c->nlevels, ncand, reps);
+    txt += line;
+    // one measurement: min over reps of a kernel's total time in a call, or of the call
+    auto measure = [&](const char *kernel, double *ms) -> int {
+        *ms = 1e300;
+        for (int r = 0; r < reps; r++) {
+            const auto t0 = std::chrono::steady_clock::now();
+            const int e = compute(c, mode, nlower, ids, out);
+            const auto t1 = std::chrono::steady_clock::now();
+            if (e) return e;
+            double v = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            if (kernel) {
+                auto it = c->kstats.find(kernel);
+                v = it == c->kstats.end() ? 0.0 : it->second.ms;
+            }
+            *ms = std::min(*ms, v);
+        }
+        return GI_OK;
+    };
+    auto scan = [&](const char *what, const char *kernel, int *knob, const int *opts, int nopt,
+                    const char *const *label) -> int {
+        const int start = *knob;
+        double tbest = 0;
+        int pick = start;
+        // the incumbent (built-in default) first, then the others in a fixed order
+        std::vector<int> order{start};
+        for (int i = 0; i < nopt; i++)
+            if (opts[i] != start) order.push_back(opts[i]);
+        for (size_t i = 0; i < order.size(); i++) {
+            *knob = order[i];
+            double ms;
+            const int e = measure(kernel, &ms);
+            if (e) { *knob = start; return e; }
+            int li = 0;
+            while (li < nopt && opts[li] != order[i]) li++;
+            std::snprintf(line, sizeof line, "%s %d%s%s %.4f ms\n", what, order[i],
+                          label ? " " : "", label ? label[li] : "", ms);
+            txt += line;
+            if (i == 0 || ms < 0.97 * tbest) {
+                tbest = ms;
+                pick = order[i];
+            }
+        }
+        *knob = pick;
+        return GI_OK;
+    };
+    static const int tiles[4] = {0, 1, 2, 3};
+    static const char *const tname[4] = {"64x64", "64x128", "128x64", "128x128"};
+    static const int rounds[6] = {1, 2, 3, 4, 6, 8};
+    if (mode == GI_MODE_FP64_BATCHED) {
+        if ((rc = scan("halfls_tile", "half_ls_mfma", &c->tile_halfls, tiles, 4, tname)) ||
+            (rc = scan("corr_tile", "corr_mfma", &c->tile_corr, tiles, 4, tname))) {
+            c->tile_halfls = saved.halfls_cfg; c->tile_corr = saved.corr_cfg;
+            return rc;
+        }
+    }
+    if (mode != GI_MODE_FP64 && out->capacity >= ncand) {  // pipelined output only
+        if ((rc = scan("pipe_rounds", nullptr, &c->pipe_rounds, rounds, 6, nullptr))) {
+            c->tile_halfls = saved.halfls_cfg; c->tile_corr = saved.corr_cfg;
+            c->pipe_rounds = saved.pipe_rounds;
+            return rc;
+        }
+    }
+    std::snprintf(line, sizeof line, "best halfls_tile=%d corr_tile=%d pipe_rounds=%d\n",
+                  c->tile_halfls, c->tile_corr, c->pipe_rounds);
+    txt += line;
+    if (best) *best = gi_tuning{c->tile_halfls, c->tile_corr, c->pipe_rounds};
+    if (len) *len = (long long)txt.size();
+    if (table && cap > 0) {
+        const size_t n = std::min<size_t>((size_t)cap - 1, txt.size());
+        std::memcpy(table, txt.data(), n);
+        table[n] = 0;
+    }
+    return GI_OK;
+}
+
 int count_job(gi_ctx *c, int nlower, const int *ids, long long *count)
 {
     Plan P;
@@ -1422,6 +1519,35 @@ int gi_compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
 {
     if (!c || !out) return GI_EINVAL;
     return compute(c, mode, nlower, ids, out);
+}
+
+int gi_set_tuning(gi_ctx *c, const gi_tuning *t)
+{
+    if (!c || !t) return GI_EINVAL;
+    if (c->server) return fail(c, GI_ESTATE, "gi_set_tuning while the request server runs");
+    if (t->halfls_cfg < 0 || t->halfls_cfg > 3 || t->corr_cfg < 0 || t->corr_cfg > 3 ||
+        t->pipe_rounds < 1 || t->pipe_rounds > kMaxPipeRounds)
+        return fail(c, GI_EINVAL, "gi_set_tuning: value out of range");
+    c->tile_halfls = t->halfls_cfg;
+    c->tile_corr = t->corr_cfg;
+    c->pipe_rounds = t->pipe_rounds;
+    return GI_OK;
+}
+
+int gi_get_tuning(gi_ctx *c, gi_tuning *t)
+{
+    if (!c || !t) return GI_EINVAL;
+    t->halfls_cfg = c->tile_halfls;
+    t->corr_cfg = c->tile_corr;
+    t->pipe_rounds = c->pipe_rounds;
+    return GI_OK;
+}
+
+int gi_autotune(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out, int reps,
+                gi_tuning *best, char *table, long long cap, long long *len)
+{
+    if (!c || !out || reps < 1 || reps > 100) return GI_EINVAL;
+    return autotune(c, mode, nlower, ids, out, reps, best, table, cap, len);
 }
 
 int gi_partition_function(gi_ctx *c, double temperature, double *q)

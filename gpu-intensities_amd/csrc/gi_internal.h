@@ -15,6 +15,8 @@
 namespace gi {
 // Largest J list of a context (gi_set_jvals).
 constexpr int kMaxJ = 128;
+// Largest pipeline round count (gi_set_tuning).
+constexpr int kMaxPipeRounds = 64;
 }  // namespace gi
 
 namespace gi {
@@ -200,8 +202,10 @@ struct gi_ctx {
     // partition-function inputs on the device: energies and gns*ndeg*(2J+1)
     double *d_lv_energy = nullptr, *d_lv_weight = nullptr;
 
-    // tile configurations of k_halfls_jobs and k_corr_mfma (see tile_rows)
+    // tile configurations of k_halfls_jobs and k_corr_mfma (see tile_rows), and
+    // the rounds of a pipelined gi_compute (gi_set_tuning, gi_autotune)
     int tile_halfls = 3, tile_corr = 0;
+    int pipe_rounds = 2;
 
     // split mode: digits per operand (3..6) and the resident digit planes of mu
     int split_S = 6, split_ready = 0;  // split_ready = digits the planes were built with
@@ -279,6 +283,9 @@ inline ClassDev classdev(const OutClass &oc)
 }
 
 int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out);
+int count_job(gi_ctx *c, int nlower, const int *ids, long long *count);
+int autotune(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out, int reps,
+             gi_tuning *best, char *table, long long cap, long long *len);
 int prepare_split(gi_ctx *c);
 void free_split(gi_ctx *c);
 

@@ -216,6 +216,35 @@ int gi_prepare_split(gi_ctx *ctx);
 int gi_set_tiles(gi_ctx *ctx, int halfls_cfg, int corr_cfg);
 int gi_get_tiles(gi_ctx *ctx, int *halfls_cfg, int *corr_cfg);
 
+/* Launch configuration of the batched modes (no reference counterpart: the
+ * reference has no batched form). halfls_cfg / corr_cfg are the tile shapes
+ * of gi_set_tiles; pipe_rounds is the number of rounds a gi_compute with room
+ * for every candidate is cut into, each round's lines being copied to the
+ * host while later rounds compute (1..64; default 2). Lines are bitwise the
+ * same for every setting. */
+typedef struct gi_tuning {
+    int halfls_cfg;
+    int corr_cfg;
+    int pipe_rounds;
+} gi_tuning;
+int gi_set_tuning(gi_ctx *ctx, const gi_tuning *t);
+int gi_get_tuning(gi_ctx *ctx, gi_tuning *t);
+
+/* Autotuner (no reference counterpart): runs gi_compute in `mode` over the
+ * given lower levels (NULL = all) `reps` times per setting into `out`,
+ * and keeps the fastest setting of each knob: the two tile shapes (FP64
+ * batched mode, judged by their kernel's time) and the pipeline round count
+ * (both matrix-core modes when `out` has room for every candidate, judged by
+ * the call's wall time). Settings are scanned in a fixed order starting from
+ * the current one, and a setting replaces the incumbent only when it is more
+ * than 3% faster, so repeated runs pick the same table unless the timings
+ * really differ. The chosen setting is applied and returned in *best;
+ * `table` (cap bytes, NUL-terminated, *len = full length) receives one line
+ * per measurement and a final "best ..." line, to be kept and re-applied with
+ * gi_set_tuning. */
+int gi_autotune(gi_ctx *ctx, int mode, int nlower, const int *lower_ids, gi_lines *out,
+                int reps, gi_tuning *best, char *table, long long cap, long long *len);
+
 /* Partition function over the resident level table:
  * Q(T) = sum_i gns(g_i) * ndeg_i * (2 J_i + 1) * exp(-(E_i - ZPE) * c2 / T),
  * c2 = planck * vellgt / boltz with the constants of cuda_host.cu:12-18.

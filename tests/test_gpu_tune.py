@@ -1,0 +1,103 @@
+"""Autotuner and launch configuration (gi_autotune, gi_set_tuning).
+
+The autotuner times the tile shapes of the FP64 matrix-core kernels and the
+pipeline round count on the caller's workload and applies the fastest; every
+setting must give bitwise-identical lines, so tuning never changes results.
+"""
+import numpy as np
+import pytest
+
+from gpu_intensities_amd import binding, synth
+import oracle_py
+from test_gpu_parity import compare
+
+pytestmark = pytest.mark.gpu
+
+DEFAULT = dict(halfls_cfg=3, corr_cfg=0, pipe_rounds=2)
+
+
+@pytest.fixture
+def tuned_engine(engine):
+    yield engine
+    engine.set_tuning(**DEFAULT)
+
+
+def _copy(r):
+    return {k: v.copy() for k, v in r.items()}
+
+
+@pytest.mark.parametrize("mode", [binding.GI_MODE_FP64_BATCHED, binding.GI_MODE_SPLIT])
+def test_autotune_picks_a_setting_and_keeps_the_bits(tuned_engine, mode):
+    eng = tuned_engine
+    job = synth.make_job("C2V", nvib=48, jvals=(3, 4), seed=51, e_span=12000.0)
+    eng.load(job)
+    assert eng.get_tuning() == DEFAULT
+    before = _copy(eng.compute(mode=mode))
+    best, table = eng.autotune(mode, reps=2)
+    assert eng.get_tuning() == best
+    lines = table.strip().splitlines()
+    assert lines[0].startswith("# gi_autotune") and lines[-1].startswith("best ")
+    kinds = [ln.split()[0] for ln in lines[1:-1]]
+    want = (["halfls_tile"] * 4 + ["corr_tile"] * 4 if mode == binding.GI_MODE_FP64_BATCHED
+            else []) + ["pipe_rounds"] * 6
+    assert kinds == want
+    assert lines[-1] == ("best halfls_tile=%d corr_tile=%d pipe_rounds=%d"
+                         % (best["halfls_cfg"], best["corr_cfg"], best["pipe_rounds"]))
+    after = _copy(eng.compute(mode=mode))
+    for k in before:
+        assert np.array_equal(before[k], after[k]), k
+    if mode == binding.GI_MODE_FP64_BATCHED:
+        compare(after, oracle_py.OracleJob(job).run())
+
+
+def test_every_round_count_gives_the_same_bits(tuned_engine, monkeypatch):
+    """Round counts with the pipelined output: forced to several rounds by a
+    small round budget, and at each pipe_rounds setting, the lines are the
+    same bits as one round."""
+    eng = tuned_engine
+    job = synth.make_job("C2V", nvib=40, jvals=(2, 3), seed=52, e_span=15000.0)
+    eng.load(job)
+    for mode in (binding.GI_MODE_FP64_BATCHED, binding.GI_MODE_SPLIT):
+        eng.set_tuning(3, 0, 1)
+        ref = _copy(eng.compute(mode=mode))
+        for rounds in (2, 3, 8, 64):
+            eng.set_tuning(3, 0, rounds)
+            got = eng.compute(mode=mode)
+            for k in ref:
+                assert np.array_equal(ref[k], got[k]), (mode, rounds, k)
+        monkeypatch.setenv("GI_BATCH_BYTES", str(1 << 20))
+        got = eng.compute(mode=mode)
+        monkeypatch.delenv("GI_BATCH_BYTES")
+        for k in ref:
+            assert np.array_equal(ref[k], got[k]), (mode, "small rounds", k)
+
+
+def test_autotune_plain_mode_has_nothing_to_tune(tuned_engine):
+    eng = tuned_engine
+    job = synth.make_job("C2V", nvib=16, jvals=(0, 1), seed=53)
+    eng.load(job)
+    best, table = eng.autotune(binding.GI_MODE_FP64, reps=1)
+    assert best == DEFAULT
+    assert len(table.strip().splitlines()) == 2
+
+
+def test_set_tuning_rules(tuned_engine):
+    eng = tuned_engine
+    job = synth.make_job("C2V", nvib=16, jvals=(0, 1), seed=54)
+    eng.load(job)
+    for bad in ((4, 0, 2), (0, -1, 2), (0, 0, 0), (0, 0, 65)):
+        with pytest.raises(binding.GIError):
+            eng.set_tuning(*bad)
+    assert eng.get_tuning() == DEFAULT
+    eng.set_tuning(1, 2, 5)
+    assert eng.get_tuning() == dict(halfls_cfg=1, corr_cfg=2, pipe_rounds=5)
+    assert eng.get_tiles() == (1, 2)
+    eng.serve_start(1 << 28)
+    try:
+        with pytest.raises(binding.GIError) as e:
+            eng.set_tuning(**DEFAULT)
+        assert e.value.status == 4
+        with pytest.raises(binding.GIError):
+            eng.autotune(binding.GI_MODE_FP64_BATCHED)
+    finally:
+        eng.serve_stop()
