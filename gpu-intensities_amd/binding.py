@@ -33,7 +33,8 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_params", "gi_uplo
            "gi_serve_submit", "gi_serve_wait", "gi_serve_fetch", "gi_serve_stats",
            "gi_serve_stop", "gi_set_tiles", "gi_get_tiles", "gi_set_split_digits",
            "gi_prepare_split", "gi_set_jvals", "gi_set_tuning", "gi_get_tuning",
-           "gi_autotune"]
+           "gi_autotune", "gi_sweep_create", "gi_sweep_run", "gi_sweep_candidates",
+           "gi_sweep_destroy"]
 # (TROVE_EXPORTS below: the symbols of include/gi_trove.h)
 
 
@@ -111,6 +112,11 @@ def lib():
         L.gi_set_split_digits.argtypes = [vp, ci]
         L.gi_prepare_split.argtypes = [vp]
         L.gi_get_tiles.argtypes = [vp, P_i, P_i]
+        L.gi_sweep_create.argtypes = [vp, ci, ci, P_i, C.POINTER(vp)]
+        L.gi_sweep_run.argtypes = [vp, cd, cd, cd, cd, C.POINTER(GiLines)]
+        L.gi_sweep_candidates.argtypes = [vp]
+        L.gi_sweep_candidates.restype = cll
+        L.gi_sweep_destroy.argtypes = [vp]
         L.gi_set_tuning.argtypes = [vp, C.POINTER(GiTuning)]
         L.gi_get_tuning.argtypes = [vp, C.POINTER(GiTuning)]
         L.gi_autotune.argtypes = [vp, ci, ci, P_i, C.POINTER(GiLines), ci, C.POINTER(GiTuning),
@@ -329,6 +335,39 @@ class Engine:
         return (dict(halfls_cfg=best.halfls_cfg, corr_cfg=best.corr_cfg,
                      pipe_rounds=best.pipe_rounds), buf.value.decode())
 
+    # -- captured sweeps ---------------------------------------------------
+    def sweep_create(self, mode, lower_ids=None):
+        """gi_sweep_create; returns an opaque handle (destroy with sweep_destroy)."""
+        h = C.c_void_p()
+        if lower_ids is None:
+            rc = lib().gi_sweep_create(self.h, mode, 0, None, C.byref(h))
+        else:
+            ids = np.ascontiguousarray(lower_ids, np.int32)
+            rc = lib().gi_sweep_create(self.h, mode, len(ids), _p(ids, P_i), C.byref(h))
+        self._chk(rc, "gi_sweep_create")
+        return h
+
+    def sweep_candidates(self, sw):
+        return int(lib().gi_sweep_candidates(sw))
+
+    def sweep_run(self, sw, temperature=None, q_stat=None, thresh_linestrength=None,
+                  thresh_intensity=None, bufs=None):
+        """gi_sweep_run; unset fields take the loaded job's values (q_stat <= 0: Q(T))."""
+        j = self.job
+        if bufs is None:
+            bufs = self.alloc_lines(self.sweep_candidates(sw))
+        s = self._lines(bufs)
+        rc = lib().gi_sweep_run(
+            sw, j.temperature if temperature is None else temperature,
+            j.q_stat if q_stat is None else q_stat,
+            j.thresh_linestrength if thresh_linestrength is None else thresh_linestrength,
+            j.thresh_intensity if thresh_intensity is None else thresh_intensity, C.byref(s))
+        self._chk(rc, "gi_sweep_run")
+        return self._result(bufs, s.n)
+
+    def sweep_destroy(self, sw):
+        self._chk(lib().gi_sweep_destroy(sw), "gi_sweep_destroy")
+
     def partition_function(self, temperature):
         q = cd()
         self._chk(lib().gi_partition_function(self.h, temperature, C.byref(q)),
@@ -409,11 +448,12 @@ class Engine:
                 for i in range(min(n.value, 32))}
 
     def work_stats(self):
-        w = np.zeros(10)
-        self._chk(lib().gi_work_stats(self.h, _p(w, P_d), 10), "gi_work_stats")
+        w = np.zeros(14)
+        self._chk(lib().gi_work_stats(self.h, _p(w, P_d), 14), "gi_work_stats")
         keys = ["units", "candidates", "lines", "half_ls_dense_flops", "half_ls_flops",
-                "corr_bytes", "mu_bytes", "mu_dense_bytes", "corr_flops"]
-        return dict(zip(keys, w[:9].tolist()))
+                "corr_bytes", "mu_bytes", "mu_dense_bytes", "corr_flops", "host_plan_ms",
+                "host_enqueue_ms", "host_wait_copy_ms", "out_bytes", "call_ms"]
+        return dict(zip(keys, w.tolist()))
 
 
 # -- TROVE files, deck and line list (include/gi_trove.h) ---------------------

@@ -27,10 +27,13 @@ int fail(gi_ctx *c, int code, const std::string &msg)
 }
 
 // Calls that change or read the plan are refused while the request server's
-// worker uses it.
+// worker uses it or sweeps hold pointers into it.
 static int serving(gi_ctx *c)
 {
-    return c->server ? fail(c, GI_ESTATE, "call not allowed while the request server runs") : 0;
+    if (c->server) return fail(c, GI_ESTATE, "call not allowed while the request server runs");
+    if (!c->sweeps.empty())
+        return fail(c, GI_ESTATE, "call not allowed while sweeps exist (gi_sweep_destroy first)");
+    return 0;
 }
 
 #define HIPCHK(call)                                                                   \
@@ -95,11 +98,45 @@ void *slot_pinned(gi_ctx *c, Slot &s, size_t bytes, int *rc)
 void free_slot(Slot &s)
 {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
+    if (s.cstream) (void)hipStreamSynchronize(s.cstream);
     for (auto &e : s.buf) (void)hipFree(e.second.first);
     s.buf.clear();
     if (s.pinned) (void)hipHostFree(s.pinned);
     s.pinned = nullptr;
     s.pinned_bytes = 0;
+    s.arena.release();
+}
+
+unsigned char *HostArena::alloc(size_t n)
+{
+    n = (n + 255) & ~size_t(255);
+    while (cur < blocks.size()) {
+        if (off + n <= blocks[cur].second) {
+            unsigned char *p = blocks[cur].first + off;
+            off += n;
+            return p;
+        }
+        cur++;
+        off = 0;
+    }
+    const size_t last = blocks.empty() ? 0 : blocks.back().second;
+    const size_t want = std::max({n, (size_t)4 << 20, 2 * last});
+    void *p = nullptr;
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    blocks.push_back({(unsigned char *)p, want});
+    cur = blocks.size() - 1;
+    off = n;
+    return (unsigned char *)p;
+}
+
+void HostArena::release()
+{
+    for (auto &b : blocks) (void)hipHostFree(b.first);
+    blocks.clear();
+    reset();
 }
 
 template <typename T>
@@ -262,6 +299,7 @@ void gi_destroy(gi_ctx *c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     server_shutdown(c);
+    sweeps_shutdown(c);
     (void)hipStreamSynchronize(c->stream);
     free_plan(c);
     for (auto &b : c->blocks)
@@ -738,7 +776,7 @@ int gi_get_tiles(gi_ctx *c, int *hl, int *cr)
 int gi_work_stats(gi_ctx *c, double *out, int n)
 {
     if (!c || !out) return GI_EINVAL;
-    for (int i = 0; i < n && i < 10; i++) out[i] = c->work[i];
+    for (int i = 0; i < n && i < kWorkStats; i++) out[i] = c->work[i];
     return GI_OK;
 }
 

@@ -167,6 +167,7 @@ struct Plan {
     {
         for (auto &r : rounds)
             if (r.ev) (void)hipEventDestroy(r.ev);
+        for (auto g : graphs) (void)hipGraphExecDestroy(g);
     }
     int mode = GI_MODE_FP64;
     std::vector<Query> q;
@@ -174,10 +175,14 @@ struct Plan {
     long long ncand = 0;
     std::vector<Chunk> chunks;
     std::vector<unsigned char> meta;
-    std::vector<std::vector<unsigned char>> keep;  // host sources of async copies
     LineSoA comp{};                                // compacted lines (slot buffers)
+    // sweep capture: each round's launches become one graph instead of running;
+    // the intensity step then reads (beta, Q, thresholds) from qpar
+    bool capture = false;
+    std::vector<hipGraphExec_t> graphs;
+    const double *qpar = nullptr;
     long long *counts = nullptr;                   // pinned, lines per query
-    double work[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double work[kWorkStats] = {};
 };
 
 // Window [a, b) of nu = E' - E in [w0, w1] in an energy-sorted block, found
@@ -447,7 +452,7 @@ static int round_compact(gi_ctx *c, Slot &S, Plan &P, size_t r, int *flag, LineS
         HIPCHK(hipMemcpyAsync(P.rcounts + r, pc_offs + o.obase + o.nb, 8, hipMemcpyDeviceToHost,
                               S.stream));
     }
-    HIPCHK(hipEventRecord(o.ev, S.stream));
+    if (!P.capture) HIPCHK(hipEventRecord(o.ev, S.stream));  // a sweep records it per replay
     return GI_OK;
 }
 
@@ -468,6 +473,7 @@ static int chunk_intensity(gi_ctx *c, Slot &S, const Plan &P, const Chunk &ch,
     a.gnsI = ca.gnsI; a.zpe = ca.zpe; a.beta = ca.beta; a.q_stat = ca.q_stat;
     a.A_coef_s_1 = ca.A_coef_s_1; a.intens_cm_mol = ca.intens_cm_mol;
     a.thresh_S = ca.thresh_S; a.thresh_I = ca.thresh_I;
+    a.qpar = P.qpar;
     a.amp = cand.amp; a.flag = flag; a.out_iI = cand.iI; a.out_iF = cand.iF;
     a.out_nu = cand.nu; a.out_S = cand.S; a.out_A = cand.A; a.out_I = cand.I;
     TL_BEGIN("intensity");
@@ -475,6 +481,59 @@ static int chunk_intensity(gi_ctx *c, Slot &S, const Plan &P, const Chunk &ch,
     TL_END();
     return GI_OK;
 }
+
+// Pinned staging of an upload on slot S (valid until the slot's next launch
+// sequence resets its arena).
+static unsigned char *stage(gi_ctx *c, Slot &S, const void *src, size_t n, int *rc)
+{
+    unsigned char *p = S.arena.alloc(std::max<size_t>(n, 8));
+    if (!p) {
+        *rc = fail(c, GI_ENOMEM, "pinned host allocation (upload staging) failed");
+        return nullptr;
+    }
+    if (src && n) std::memcpy(p, src, n);
+    *rc = GI_OK;
+    return p;
+}
+
+// Stream capture of one round into a graph (sweeps). On an error path the
+// destructor ends the capture so the stream is usable again.
+struct RoundCapture {
+    hipStream_t st = nullptr;
+    bool on = false;
+    int begin(gi_ctx *c, const Plan &P, hipStream_t s)
+    {
+        if (!P.capture) return GI_OK;
+        st = s;
+        if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess)
+            return fail(c, GI_EHIP, "hipStreamBeginCapture");
+        on = true;
+        return GI_OK;
+    }
+    int end(gi_ctx *c, Plan &P)
+    {
+        if (!on) return GI_OK;
+        on = false;
+        hipGraph_t g = nullptr;
+        if (hipStreamEndCapture(st, &g) != hipSuccess || !g)
+            return fail(c, GI_EHIP, "hipStreamEndCapture");
+        hipGraphExec_t e = nullptr;
+        const hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (r != hipSuccess) return fail(c, GI_EHIP, "hipGraphInstantiate");
+        P.graphs.push_back(e);
+        return GI_OK;
+    }
+    ~RoundCapture()
+    {
+        if (on) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+        }
+    }
+};
 
 // Batched FP64 mode over rounds of chunks (see the file comment).
 static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timeline *tl,
@@ -538,6 +597,8 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
     size_t iround = 0;
 
     for (auto &rg : rounds) {
+        RoundCapture cap;
+        if ((rc = cap.begin(c, P, S.stream)) != GI_OK) return rc;
         std::vector<HalfJob> jobs;
         std::vector<GBuildJob> gjobs;
         std::vector<GMapJob> mjobs_v;
@@ -610,11 +671,10 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         const int njobs = (int)jobs.size(), ngj = (int)gjobs.size(), nmj = (int)mjobs_v.size();
         if (nmj > 0) {
             const size_t mb = nmj * sizeof(GMapJob);
-            P.keep.emplace_back(mb + 4 * (size_t)nmj);
-            unsigned char *hb = P.keep.back().data();
-            std::memcpy(hb, mjobs_v.data(), mb);
+            unsigned char *hb = stage(c, S, mjobs_v.data(), mb + 4 * (size_t)nmj, &rc);
+            if (rc) return rc;
             std::memcpy(hb + mb, mbegin.data(), 4 * (size_t)nmj);
-            HIPCHK(hipMemcpyAsync(d_mjobs, hb, P.keep.back().size(), hipMemcpyHostToDevice,
+            HIPCHK(hipMemcpyAsync(d_mjobs, hb, mb + 4 * (size_t)nmj, hipMemcpyHostToDevice,
                                   S.stream));
             TL_BEGIN("build_g");
             launch_build_g_map(S.stream, d_mjobs, (const int *)((unsigned char *)d_mjobs + mb),
@@ -623,11 +683,11 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         }
         if (ngj > 0) {
             const size_t gb = ngj * sizeof(GBuildJob);
-            P.keep.emplace_back(gb + 4 * (size_t)ngj);
-            unsigned char *hb = P.keep.back().data();
+            unsigned char *hb = stage(c, S, nullptr, gb + 4 * (size_t)ngj, &rc);
+            if (rc) return rc;
             std::memcpy(hb, gjobs.data(), gb);
             std::memcpy(hb + gb, gbegin.data(), 4 * (size_t)ngj);
-            HIPCHK(hipMemcpyAsync(d_gjobs, hb, P.keep.back().size(), hipMemcpyHostToDevice,
+            HIPCHK(hipMemcpyAsync(d_gjobs, hb, gb + 4 * (size_t)ngj, hipMemcpyHostToDevice,
                                   S.stream));
             TL_BEGIN("build_g");
             launch_build_g_units(S.stream, N, d_gjobs,
@@ -635,11 +695,12 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             TL_END();
         }
         if (njobs > 0) {
-            P.keep.emplace_back(jobs_bytes + 4 * (size_t)mjobs);
-            unsigned char *hb = P.keep.back().data();
+            unsigned char *hb = stage(c, S, nullptr, jobs_bytes + 4 * (size_t)mjobs, &rc);
+            if (rc) return rc;
             std::memcpy(hb, jobs.data(), njobs * sizeof(HalfJob));
             std::memcpy(hb + jobs_bytes, tbegin.data(), njobs * 4);
-            HIPCHK(hipMemcpyAsync(d_jobs, hb, P.keep.back().size(), hipMemcpyHostToDevice, S.stream));
+            HIPCHK(hipMemcpyAsync(d_jobs, hb, jobs_bytes + 4 * (size_t)mjobs,
+                                  hipMemcpyHostToDevice, S.stream));
             const int *d_tb = (const int *)(d_jobs + jobs_bytes);
             TL_BEGIN("half_ls_mfma");
             launch_halfls_jobs(S.stream, c->tile_halfls, N, (const HalfJob *)d_jobs, d_tb, njobs,
@@ -691,11 +752,11 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             if (rows > 0 && cols > 0) ctiles += ((rows + CM - 1) / CM) * ((cols + CN - 1) / CN);
         }
         const size_t mm_bytes = mm.size() * sizeof(CorrMMArgs);
-        P.keep.emplace_back(mm_bytes + 4 * mm.size());
-        std::memcpy(P.keep.back().data(), mm.data(), mm_bytes);
-        std::memcpy(P.keep.back().data() + mm_bytes, cbegin.data(), 4 * mm.size());
-        HIPCHK(hipMemcpyAsync(d_mm, P.keep.back().data(), P.keep.back().size(),
-                              hipMemcpyHostToDevice, S.stream));
+        unsigned char *hmm = stage(c, S, mm.data(), mm_bytes + 4 * mm.size(), &rc);
+        if (rc) return rc;
+        std::memcpy(hmm + mm_bytes, cbegin.data(), 4 * mm.size());
+        HIPCHK(hipMemcpyAsync(d_mm, hmm, mm_bytes + 4 * mm.size(), hipMemcpyHostToDevice,
+                              S.stream));
         TL_BEGIN("corr_mfma");
         launch_corr_mfma(S.stream, c->tile_corr, (const CorrMMArgs *)d_mm,
                          (const int *)((unsigned char *)d_mm + mm_bytes), (int)mm.size(), ctiles);
@@ -705,6 +766,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                 return rc;
         if ((rc = round_compact(c, S, P, iround, flag, cand, pc_counts, pc_offs, tl)) != GI_OK)
             return rc;
+        if ((rc = cap.end(c, P)) != GI_OK) return rc;
         iround++;
     }
     return GI_OK;
@@ -775,6 +837,8 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
 
     size_t iround = 0;
     for (auto &rg : rounds) {
+        RoundCapture cap;
+        if ((rc = cap.begin(c, P, S.stream)) != GI_OK) return rc;
         std::vector<GSplitJob> gj;
         std::vector<SplitHalfJob> hj;
         std::vector<CorrSplitArgs> cj;
@@ -837,11 +901,12 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
         }
         auto send = [&](unsigned char *dst, const void *src, size_t nb, const std::vector<int> &beg,
                         size_t cap) -> int {
-            P.keep.emplace_back(cap + 4 * beg.size());
-            std::memcpy(P.keep.back().data(), src, nb);
-            std::memcpy(P.keep.back().data() + cap, beg.data(), 4 * beg.size());
-            HIPCHK(hipMemcpyAsync(dst, P.keep.back().data(), P.keep.back().size(),
-                                  hipMemcpyHostToDevice, S.stream));
+            int e = GI_OK;
+            unsigned char *hb = stage(c, S, nullptr, cap + 4 * beg.size(), &e);
+            if (e) return e;
+            std::memcpy(hb, src, nb);
+            std::memcpy(hb + cap, beg.data(), 4 * beg.size());
+            HIPCHK(hipMemcpyAsync(dst, hb, cap + 4 * beg.size(), hipMemcpyHostToDevice, S.stream));
             return GI_OK;
         };
         if (!gj.empty()) {
@@ -920,6 +985,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                 return rc;
         if ((rc = round_compact(c, S, P, iround, flag, cand, pc_counts, pc_offs, tl)) != GI_OK)
             return rc;
+        if ((rc = cap.end(c, P)) != GI_OK) return rc;
         iround++;
     }
     return GI_OK;
@@ -940,6 +1006,7 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
     const Consts K;
     int rc = GI_OK;
     HIPCHK(hipSetDevice(c->device));
+    S.arena.reset();  // the slot's previous sequence has finished
     const bool batched = P.mode == GI_MODE_FP64_BATCHED;
     const size_t nc = (size_t)std::max<long long>(P.ncand, 1);
     int *flag = (int *)slot_scratch(c, S, "flag", nc * sizeof(int), &rc); if (rc) return rc;
@@ -980,8 +1047,9 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
     unsigned char *d_meta = nullptr;
     if (!P.meta.empty()) {
         d_meta = (unsigned char *)slot_scratch(c, S, "meta", P.meta.size(), &rc); if (rc) return rc;
-        HIPCHK(hipMemcpyAsync(d_meta, P.meta.data(), P.meta.size(), hipMemcpyHostToDevice,
-                              S.stream));
+        unsigned char *hm = stage(c, S, P.meta.data(), P.meta.size(), &rc);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(d_meta, hm, P.meta.size(), hipMemcpyHostToDevice, S.stream));
     }
 
     if (P.mode == GI_MODE_SPLIT && !P.chunks.empty()) {
@@ -1083,6 +1151,9 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
     return GI_OK;
 }
 
+// Host bytes of one emitted line (ilevelI, ilevelF, nu, S, A, I, amp[4]).
+constexpr double kLineBytes = 4 + 4 + 8 * 4 + 32;
+
 // Async copy of n compacted lines starting at candidate slot `from` to host
 // arrays at row `to`, on stream st.
 static int copy_range(gi_ctx *c, hipStream_t st, const LineSoA &comp, long long from, long long n,
@@ -1109,6 +1180,41 @@ static int copy_lines(gi_ctx *c, Slot &S, const Plan &P, size_t k, int *iI, int 
                       amp, 0);
 }
 
+// Pipelined output: as each round's event fires, copy its compacted lines to
+// `out` on the copy stream (overlapping the rounds still computing).
+static int drain_rounds(gi_ctx *c, Slot &S, const Plan &P, gi_lines *out, long long *nout)
+{
+    int rc;
+    *nout = 0;
+    for (size_t r = 0; r < P.rounds.size(); r++) {
+        const RoundOut &o = P.rounds[r];
+        HIPCHK(hipEventSynchronize(o.ev));
+        const long long n = o.nb > 0 ? P.rcounts[r] : 0;
+        if (n > 0) {
+            HIPCHK(hipStreamWaitEvent(S.cstream, o.ev, 0));
+            if ((rc = copy_range(c, S.cstream, P.comp, o.cb, n, out->ilevelI, out->ilevelF,
+                                 out->nu, out->S, out->A_einst, out->I, out->amp, *nout)) !=
+                GI_OK)
+                return rc;
+        }
+        *nout += n;
+    }
+    HIPCHK(hipStreamSynchronize(S.cstream));
+    return GI_OK;
+}
+
+// The deck job's query (temperature, Q, thresholds of gi_set_params).
+static Query job_query(const gi_ctx *c)
+{
+    Query q;
+    q.beta = Consts::beta(c->temperature);
+    q.q_stat = c->q_stat;
+    q.zpe = c->zpe;
+    q.thS = c->thresh_S;
+    q.thI = c->thresh_I;
+    return q;
+}
+
 int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
 {
     if (!c->prepared) return fail(c, GI_ESTATE, "gi_compute before gi_prepare");
@@ -1117,14 +1223,14 @@ int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
         return fail(c, GI_EINVAL, "gi_compute: unknown mode");
     int rc;
     if (mode == GI_MODE_SPLIT && (rc = prepare_split(c)) != GI_OK) return rc;
+    using clk = std::chrono::steady_clock;
+    const auto ms_since = [](clk::time_point a) {
+        return std::chrono::duration<double, std::milli>(clk::now() - a).count();
+    };
+    const auto t_start = clk::now();
     Plan P;
     P.mode = mode;
-    Query q;
-    q.beta = Consts::beta(c->temperature);
-    q.q_stat = c->q_stat;
-    q.zpe = c->zpe;
-    q.thS = c->thresh_S;
-    q.thI = c->thresh_I;
+    Query q = job_query(c);
     items_job(c, nlower, ids, 0, P);
     q.cand_begin = 0;
     q.cand_end = P.ncand;
@@ -1144,41 +1250,36 @@ int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
     if ((rc = build_chunks(c, P)) != GI_OK) return rc;
     Slot &S = c->main_slot;
     long long nout = 0;
+    double t_plan = ms_since(t_start), t_enqueue = 0;
     {
         Timeline tl(S.stream);
+        const auto t_launch = clk::now();
         if ((rc = launch_plan(c, S, P, &tl)) != GI_OK) return rc;
-        if (P.pipe) {
-            for (size_t r = 0; r < P.rounds.size(); r++) {
-                const RoundOut &o = P.rounds[r];
-                HIPCHK(hipEventSynchronize(o.ev));
-                const long long n = o.nb > 0 ? P.rcounts[r] : 0;
-                if (n > 0) {
-                    HIPCHK(hipStreamWaitEvent(S.cstream, o.ev, 0));
-                    if ((rc = copy_range(c, S.cstream, P.comp, o.cb, n, out->ilevelI,
-                                         out->ilevelF, out->nu, out->S, out->A_einst, out->I,
-                                         out->amp, nout)) != GI_OK)
-                        return rc;
-                }
-                nout += n;
-            }
-            HIPCHK(hipStreamSynchronize(S.cstream));
-        }
+        t_enqueue = ms_since(t_launch);
+        if (P.pipe && (rc = drain_rounds(c, S, P, out, &nout)) != GI_OK) return rc;
         HIPCHK(hipStreamSynchronize(S.stream));
         tl.collect(c);
     }
-    for (int i = 0; i < 10; i++) c->work[i] = P.work[i];
+    for (int i = 0; i < kWorkStats; i++) c->work[i] = P.work[i];
     if (!P.pipe) nout = P.counts[0];
     c->work[2] = (double)nout;
+    c->work[9] = t_plan;
+    c->work[10] = t_enqueue;
+    c->work[12] = (double)nout * kLineBytes;
     out->n = nout;
-    if (P.pipe) return GI_OK;
-    if (nout > out->capacity) return fail(c, GI_ECAPACITY, "gi_compute: output capacity too small");
-    if (nout > 0) {
-        if (!arrays) return fail(c, GI_EINVAL, "gi_compute: output arrays missing");
-        if ((rc = copy_lines(c, S, P, 0, out->ilevelI, out->ilevelF, out->nu, out->S,
-                             out->A_einst, out->I, out->amp)) != GI_OK)
-            return rc;
-        HIPCHK(hipStreamSynchronize(S.stream));
+    if (!P.pipe) {
+        if (nout > out->capacity)
+            return fail(c, GI_ECAPACITY, "gi_compute: output capacity too small");
+        if (nout > 0) {
+            if (!arrays) return fail(c, GI_EINVAL, "gi_compute: output arrays missing");
+            if ((rc = copy_lines(c, S, P, 0, out->ilevelI, out->ilevelF, out->nu, out->S,
+                                 out->A_einst, out->I, out->amp)) != GI_OK)
+                return rc;
+            HIPCHK(hipStreamSynchronize(S.stream));
+        }
     }
+    c->work[13] = ms_since(t_start);
+    c->work[11] = c->work[13] - t_plan - t_enqueue;
     return GI_OK;
 }
 
@@ -1504,6 +1605,114 @@ static double request_bytes(gi_ctx *c, const gi_request &r, long long *ncand)
 
 }  // namespace gi
 
+// A captured sweep (gi_sweep_create): the plan of one gi_compute with its
+// launch sequence recorded as one HIP graph per round, replayed by
+// gi_sweep_run with new intensity parameters.
+struct gi_sweep {
+    gi_ctx *c = nullptr;
+    int mode = 0;
+    std::unique_ptr<gi::Plan> P;
+    gi::Slot slot;
+    double *d_qpar = nullptr;        // beta, Q, thresh_S, thresh_I (device, read by k_intensity)
+    double *h_qpar = nullptr;        // pinned source of d_qpar
+};
+
+namespace gi {
+
+static void sweep_free(gi_sweep *w)
+{
+    (void)hipSetDevice(w->c->device);
+    w->P.reset();  // graphs and events
+    free_slot(w->slot);
+    if (w->slot.cstream) (void)hipStreamDestroy(w->slot.cstream);
+    if (w->slot.stream) (void)hipStreamDestroy(w->slot.stream);
+    if (w->h_qpar) (void)hipHostFree(w->h_qpar);
+    delete w;
+}
+
+void sweeps_shutdown(gi_ctx *c)
+{
+    for (gi_sweep *w : c->sweeps) sweep_free(w);
+    c->sweeps.clear();
+}
+
+static int sweep_create(gi_ctx *c, int mode, int nlower, const int *ids, gi_sweep **out)
+{
+    if (!c->prepared) return fail(c, GI_ESTATE, "gi_sweep_create before gi_prepare");
+    if (c->server) return fail(c, GI_ESTATE, "gi_sweep_create while the request server runs");
+    if (mode != GI_MODE_FP64_BATCHED && mode != GI_MODE_SPLIT)
+        return fail(c, GI_EINVAL, "gi_sweep_create: sweeps run the matrix-core modes");
+    int rc;
+    if (mode == GI_MODE_SPLIT && (rc = prepare_split(c)) != GI_OK) return rc;
+    HIPCHK(hipSetDevice(c->device));
+    auto w = std::make_unique<gi_sweep>();
+    w->c = c;
+    w->mode = mode;
+    w->P = std::make_unique<Plan>();
+    Plan &P = *w->P;
+    P.mode = mode;
+    Query q = job_query(c);
+    items_job(c, nlower, ids, 0, P);
+    q.cand_begin = 0;
+    q.cand_end = P.ncand;
+    P.q.push_back(q);
+    P.pipe = true;
+    P.capture = true;
+    auto cleanup = [&](int code) {
+        gi_sweep *raw = w.release();
+        sweep_free(raw);
+        return code;
+    };
+    if ((rc = build_chunks(c, P)) != GI_OK) return cleanup(rc);
+    if (hipStreamCreate(&w->slot.stream) != hipSuccess ||
+        hipStreamCreate(&w->slot.cstream) != hipSuccess)
+        return cleanup(fail(c, GI_EHIP, "hipStreamCreate"));
+    if (hipHostMalloc((void **)&w->h_qpar, 64, hipHostMallocDefault) != hipSuccess)
+        return cleanup(fail(c, GI_ENOMEM, "pinned host allocation failed"));
+    w->d_qpar = (double *)slot_scratch(c, w->slot, "qpar", 64, &rc);
+    if (rc) return cleanup(rc);
+    P.qpar = w->d_qpar;
+    if (P.ncand > 0 && (rc = launch_plan(c, w->slot, P, nullptr)) != GI_OK) return cleanup(rc);
+    if (hipStreamSynchronize(w->slot.stream) != hipSuccess)
+        return cleanup(fail(c, GI_EHIP, "gi_sweep_create: uploads failed"));
+    c->sweeps.push_back(w.get());
+    *out = w.release();
+    return GI_OK;
+}
+
+static int sweep_run(gi_sweep *w, double T, double Q, double thS, double thI, gi_lines *out)
+{
+    gi_ctx *c = w->c;
+    Plan &P = *w->P;
+    if (!(T > 0.0)) return fail(c, GI_EINVAL, "gi_sweep_run: temperature <= 0");
+    if (out->capacity < P.ncand)
+        return fail(c, GI_ECAPACITY, "gi_sweep_run: out needs room for every candidate");
+    if (!(out->ilevelI && out->ilevelF && out->nu && out->S && out->A_einst && out->I && out->amp))
+        return fail(c, GI_EINVAL, "gi_sweep_run: output arrays missing");
+    int rc;
+    HIPCHK(hipSetDevice(c->device));
+    if (!(Q > 0.0) && (rc = partition_function(c, c->main_slot, T, &Q)) != GI_OK) return rc;
+    out->n = 0;
+    if (P.ncand == 0) return GI_OK;
+    Slot &S = w->slot;
+    w->h_qpar[0] = Consts::beta(T);
+    w->h_qpar[1] = Q;
+    w->h_qpar[2] = thS;
+    w->h_qpar[3] = thI;
+    HIPCHK(hipMemcpyAsync(w->d_qpar, w->h_qpar, 32, hipMemcpyHostToDevice, S.stream));
+    for (size_t r = 0; r < P.graphs.size(); r++) {
+        HIPCHK(hipGraphLaunch(P.graphs[r], S.stream));
+        HIPCHK(hipEventRecord(P.rounds[r].ev, S.stream));
+    }
+    long long nout = 0;
+    if ((rc = drain_rounds(c, S, P, out, &nout)) != GI_OK) return rc;
+    HIPCHK(hipStreamSynchronize(S.stream));
+    out->n = nout;
+    return GI_OK;
+}
+
+}  // namespace gi
+
 using namespace gi;
 
 extern "C" {
@@ -1548,6 +1757,31 @@ int gi_autotune(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out, 
 {
     if (!c || !out || reps < 1 || reps > 100) return GI_EINVAL;
     return autotune(c, mode, nlower, ids, out, reps, best, table, cap, len);
+}
+
+int gi_sweep_create(gi_ctx *c, int mode, int nlower, const int *ids, gi_sweep **out)
+{
+    if (!c || !out) return GI_EINVAL;
+    *out = nullptr;
+    return sweep_create(c, mode, nlower, ids, out);
+}
+
+int gi_sweep_run(gi_sweep *w, double temperature, double q_stat, double thresh_linestrength,
+                 double thresh_intensity, gi_lines *out)
+{
+    if (!w || !out) return GI_EINVAL;
+    return sweep_run(w, temperature, q_stat, thresh_linestrength, thresh_intensity, out);
+}
+
+long long gi_sweep_candidates(const gi_sweep *w) { return w ? w->P->ncand : -1; }
+
+int gi_sweep_destroy(gi_sweep *w)
+{
+    if (!w) return GI_EINVAL;
+    auto &v = w->c->sweeps;
+    v.erase(std::remove(v.begin(), v.end(), w), v.end());
+    sweep_free(w);
+    return GI_OK;
 }
 
 int gi_partition_function(gi_ctx *c, double temperature, double *q)

@@ -17,6 +17,8 @@ namespace gi {
 constexpr int kMaxJ = 128;
 // Largest pipeline round count (gi_set_tuning).
 constexpr int kMaxPipeRounds = 64;
+// Counters of gi_work_stats.
+constexpr int kWorkStats = 14;
 }  // namespace gi
 
 namespace gi {
@@ -133,15 +135,29 @@ struct KStat {
     int launches = 0;
 };
 
+// Pinned host memory handed out in pieces for the uploads of one launch
+// sequence (job descriptors, chunk metadata): the copies are asynchronous and
+// can be recorded in a HIP graph. Pieces stay valid until reset(), which the
+// owner calls once the sequence has run.
+struct HostArena {
+    std::vector<std::pair<unsigned char *, size_t>> blocks;
+    size_t cur = 0, off = 0;
+    unsigned char *alloc(size_t n);  // nullptr if pinned memory runs out
+    void reset() { cur = 0; off = 0; }
+    void release();
+};
+
 // An execution slot: a stream with its own device scratch buffers and pinned
 // host staging. The context's main slot runs gi_compute; the request server
-// alternates between two more so that one batch drains while the next runs.
+// alternates between two more so that one batch drains while the next runs;
+// each sweep (gi_sweep_create) owns one.
 struct Slot {
     hipStream_t stream = nullptr;
     hipStream_t cstream = nullptr;   // device-to-host copies that overlap `stream`
     std::map<std::string, std::pair<void *, size_t>> buf;
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
+    HostArena arena;
 };
 
 struct Server;
@@ -199,6 +215,7 @@ struct gi_ctx {
     // main execution slot (stream = the context's stream), grown on demand
     gi::Slot main_slot;
     gi::Server *server = nullptr;
+    std::vector<gi_sweep *> sweeps;  // live sweeps (they hold pointers into the plan)
     // partition-function inputs on the device: energies and gns*ndeg*(2J+1)
     double *d_lv_energy = nullptr, *d_lv_weight = nullptr;
 
@@ -216,7 +233,7 @@ struct gi_ctx {
     // statistics of the last gi_compute
     std::map<std::string, gi::KStat> kstats;
     std::vector<std::string> kstat_names;
-    double work[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double work[gi::kWorkStats] = {};
 };
 
 namespace gi {
@@ -255,6 +272,8 @@ void *slot_scratch(gi_ctx *c, Slot &s, const char *name, size_t bytes, int *rc);
 void *slot_pinned(gi_ctx *c, Slot &s, size_t bytes, int *rc);
 void free_slot(Slot &s);
 void server_shutdown(gi_ctx *c);
+void sweeps_shutdown(gi_ctx *c);
+int partition_function(gi_ctx *c, Slot &S, double T, double *q);
 
 inline SymDev symdev(const SymMap &m)
 {

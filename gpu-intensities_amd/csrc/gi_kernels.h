@@ -69,6 +69,7 @@ struct IntensArgs {
     const double *blk_en;
     const unsigned char *blk_upok;
     double gnsI, zpe, beta, q_stat, A_coef_s_1, intens_cm_mol, thresh_S, thresh_I;
+    const double *qpar;              // non-null: beta, q_stat, thresh_S, thresh_I read here
     double *amp;
     int *flag, *out_iI, *out_iF;
     double *out_nu, *out_S, *out_A, *out_I;

@@ -192,7 +192,10 @@ int gi_kernel_stats(gi_ctx *ctx, int max, const char **names, double *total_ms,
  * executed by the symmetry-blocked products, [5] correlation bytes
  * (8*nsize_F per candidate), [6] mu bytes read by the half-ls products
  * (per unit in GI_MODE_FP64, per batch in GI_MODE_FP64_BATCHED), [7] dense mu
- * bytes (24*Nvib^2 per unit), [8] correlation flops executed. */
+ * bytes (24*Nvib^2 per unit), [8] correlation flops executed; host side of
+ * the call in ms: [9] planning (items, chunks, metadata), [10] enqueueing the
+ * launches, [11] waiting for the device and the copies after that, [13] the
+ * whole call; [12] bytes of the lines copied to `out`. n: up to 14. */
 int gi_work_stats(gi_ctx *ctx, double *out, int n);
 
 /* GI_MODE_SPLIT (no reference counterpart): every operand row or column of
@@ -244,6 +247,27 @@ int gi_get_tuning(gi_ctx *ctx, gi_tuning *t);
  * gi_set_tuning. */
 int gi_autotune(gi_ctx *ctx, int mode, int nlower, const int *lower_ids, gi_lines *out,
                 int reps, gi_tuning *best, char *table, long long cap, long long *len);
+
+/* Captured sweeps (no reference counterpart). gi_sweep_create plans one
+ * gi_compute in a matrix-core mode over the given lower levels (NULL = all)
+ * and records its launch sequence as one HIP graph per pipeline round,
+ * without running it; the sweep owns a stream and scratch of its own.
+ * gi_sweep_run replays the graphs with a new temperature, partition function
+ * (<= 0: Q(T) from gi_partition_function) and thresholds: every product is
+ * recomputed on the device, the host only launches the graphs and copies the
+ * lines, so a series of runs (line lists at several temperatures, or the same
+ * list again) skips the host planning and per-kernel launches of gi_compute.
+ * `out` needs room for every candidate (gi_sweep_candidates); rows come in
+ * the order of gi_compute. The other job parameters are those in force at
+ * creation. While sweeps exist, calls that change the context (gi_set_params,
+ * uploads, gi_prepare, gi_set_split_digits) return GI_ESTATE; gi_destroy
+ * frees the context's sweeps, whose handles are then invalid. */
+typedef struct gi_sweep gi_sweep;
+int gi_sweep_create(gi_ctx *ctx, int mode, int nlower, const int *lower_ids, gi_sweep **out);
+int gi_sweep_run(gi_sweep *sweep, double temperature, double q_stat,
+                 double thresh_linestrength, double thresh_intensity, gi_lines *out);
+long long gi_sweep_candidates(const gi_sweep *sweep);
+int gi_sweep_destroy(gi_sweep *sweep);
 
 /* Partition function over the resident level table:
  * Q(T) = sum_i gns(g_i) * ndeg_i * (2 J_i + 1) * exp(-(E_i - ZPE) * c2 / T),
