@@ -346,7 +346,36 @@ def time_modes(eng, binding, bufs, mode_names, steps, warmup):
                             work=work, tuning=tuning, _kstats=kstats)
         log(f"{mname}: {ms:.2f} ms/step, {nl} transitions, {nl / (ms / 1e3):.4g}/s; "
             f"kernels {modes[mname]['kernels']}")
+        if mode != binding.GI_MODE_FP64:
+            eager = {k: res[k].copy() for k in ("ilevelI", "ilevelF", "S", "I")}
+            modes[mname]["graph_sweep"] = time_sweep(eng, binding, bufs, mode, steps, eager)
     return modes
+
+
+def time_sweep(eng, binding, bufs, mode, steps, eager):
+    """The same pass as a captured sweep: gi_sweep_create once, then `steps`
+    replays of its graphs (every product recomputed on the device; the host
+    only launches the round graphs and copies the lines). The first replay is
+    checked bit for bit against the eager pass."""
+    import torch
+    t = time.perf_counter()
+    sw = eng.sweep_create(mode)
+    t_create = (time.perf_counter() - t) * 1e3
+    try:
+        first = eng.sweep_run(sw, bufs=bufs)
+        same = all(np.array_equal(first[k], eager[k]) for k in ("ilevelI", "ilevelF", "S", "I"))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r = eng.sweep_run(sw, bufs=bufs)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+    finally:
+        eng.sweep_destroy(sw)
+    out = dict(create_ms=round(t_create, 2), ms_per_step=ms, value=len(r["S"]) / (ms / 1e3),
+               first_replay_bitwise_equal=bool(same))
+    log(f"  sweep replay: {out}")
+    return out
 
 
 def sample_parity(eng, binding, ref, sample, bufs, mode_names):
