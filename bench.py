@@ -15,8 +15,11 @@ the timed region starts.
 After the headline workload (reported as `value`), the same measurement on
 one context holding C2v J = 0..20 (Nvib = 2000: 84 blocks of 500..20,500
 functions, 99 GB of FP64 eigenvectors plus 74 GB of split digit planes, every
-|dJ| <= 1 pair in the same launches; `multi_j`), and the Poisson request
-sweep against those resident blocks (`serving`).
+|dJ| <= 1 pair in the same launches; `multi_j`), the Poisson request sweep
+against those resident blocks (`serving`), and one context holding C3v(M)
+J = 0..15 (Nvib = 3000, A1/A2/E, blocks up to 30,742 functions, 65 GB of FP64
+eigenvectors) in the two matrix-core modes with the split-mode error report
+at 296 K and 1500 K (`c3v_multi_j`).
 
 Prints one JSON line. --gpus above 1 is refused: this build measures one GPU.
 """
@@ -124,20 +127,20 @@ def device_blocks(torch, job, seed, tag, cache_dir=None):
     return dev
 
 
-def build_list_job(torch, nvib=2000, jmax=20, seed=20250, lower_max=10000.0,
+def build_list_job(torch, sym="C2V", nvib=2000, jmax=20, seed=20250, lower_max=10000.0,
                    window=(0.0, 10000.0), e_span=10000.0 * 88000 / 3000, cache_dir=None):
-    """C2v J = 0..jmax in one context: every (J, symmetry) block, level
-    density per block as in the J 10 <-> 11 workload (e_span), lower levels
-    with E <= lower_max, frequency window `window`."""
+    """J = 0..jmax in one context: every (J, symmetry) block, level density
+    per block as in the J 10 <-> 11 workload (e_span), lower levels with
+    E <= lower_max, frequency window `window`."""
     import __graft_entry__ as ge
     ge.load_package()
     from gpu_intensities_amd import synth
     t = time.time()
-    job = synth.make_job("C2V", nvib=nvib, jvals=tuple(range(jmax + 1)), seed=seed,
+    job = synth.make_job(sym, nvib=nvib, jvals=tuple(range(jmax + 1)), seed=seed,
                          e_span=e_span, freq_window=window, erange_lower=(0.0, lower_max),
                          erange_upper=(0.0, 1e9), vectors="none")
     log(f"host generation {time.time() - t:.1f}s; levels {job.nlevels}")
-    dev = device_blocks(torch, job, seed, f"list{nvib}_{jmax}_{seed}", cache_dir)
+    dev = device_blocks(torch, job, seed, f"list{sym}{nvib}_{jmax}_{seed}", cache_dir)
     return job, dev
 
 
@@ -314,15 +317,16 @@ def serve_sweep(eng, job, binding, lambdas, seconds, mode, seed=7, budget=8 << 3
     return res
 
 
-def time_modes(eng, binding, bufs, mode_names, steps, warmup):
-    """Time `steps` passes of gi_compute over every lower level per mode; the
-    matrix-core modes are first tuned on this workload by gi_autotune."""
+def time_modes(eng, binding, bufs, mode_names, steps, warmup, tune=True, sweep=True):
+    """Time `steps` passes of gi_compute over every lower level per mode; with
+    `tune` the matrix-core modes are first tuned on this workload by
+    gi_autotune, with `sweep` the captured sweep is timed beside them."""
     import torch
     modes = {}
     for mname in mode_names:
         mode = binding.MODES[mname]
         tuning = None
-        if mode != binding.GI_MODE_FP64:
+        if mode != binding.GI_MODE_FP64 and tune:
             t = time.time()
             best, table = eng.autotune(mode, bufs=bufs, reps=2)
             tuning = dict(best, seconds=round(time.time() - t, 2),
@@ -346,7 +350,7 @@ def time_modes(eng, binding, bufs, mode_names, steps, warmup):
                             work=work, tuning=tuning, _kstats=kstats)
         log(f"{mname}: {ms:.2f} ms/step, {nl} transitions, {nl / (ms / 1e3):.4g}/s; "
             f"kernels {modes[mname]['kernels']}")
-        if mode != binding.GI_MODE_FP64:
+        if mode != binding.GI_MODE_FP64 and sweep:
             eager = {k: res[k].copy() for k in ("ilevelI", "ilevelF", "S", "I")}
             modes[mname]["graph_sweep"] = time_sweep(eng, binding, bufs, mode, steps, eager)
     return modes
@@ -416,25 +420,28 @@ def mode_rooflines(modes, nvib):
     return out
 
 
-def list_section(torch, binding, args, oracle_py, nthr):
-    """C2v J = 0..jmax in one context: every (J, J') pair with |J' - J| <= 1
-    in the same launches (timed per mode), a sampled oracle check at full
-    size, and the Poisson request sweep against the resident blocks."""
-    job, dev = build_list_job(torch, nvib=args.list_nvib, jmax=args.list_jmax,
-                              cache_dir=args.cache_dir)
+def list_section(torch, binding, args, oracle_py, nthr, sym, nvib, jmax, mode_names,
+                 steps, sample_g, serve=False, ntop=20, full=True):
+    """J = 0..jmax in one context: every (J, J') pair with |J' - J| <= 1 in
+    the same launches (timed per mode; with `full` also tuned first and timed
+    as a captured sweep), a sampled oracle check at full size (J = 1 lower
+    levels, every 2nd, and `ntop` J = jmax lower levels of symmetry sample_g,
+    at 296 K and 1500 K) and, with serve, the Poisson request sweep against
+    the resident blocks."""
+    job, dev = build_list_job(torch, sym=sym, nvib=nvib, jmax=jmax, cache_dir=args.cache_dir)
     eng = binding.Engine(0)
     t = time.time()
     eng.load(job, device_vectors=dev)
     t_prep = time.time() - t
-    mode_names = [m for m in args.list_modes.split(",") if m]
+    serve_modes = [m for m in args.serve_modes.split(",") if m] if serve else []
     t = time.time()
-    if "split" in mode_names or "split" in args.serve_modes.split(","):
+    if "split" in mode_names or "split" in serve_modes:
         eng.prepare_split()
     torch.cuda.synchronize()
     t_split = time.time() - t
     sizes = [int(b.nsize[g]) for b in job.bsets for g in range(job.sym["nrep"])]
     fp64_gb = sum(n * n for n in sizes) * 8 / 1e9
-    log(f"list context: prepare {t_prep:.1f}s, split planes {t_split:.1f}s, "
+    log(f"{sym} list context: prepare {t_prep:.1f}s, split planes {t_split:.1f}s, "
         f"blocks {min(sizes)}..{max(sizes)}, FP64 blocks {fp64_gb:.1f} GB")
     cap = eng.count_candidates()
     bufs = eng.alloc_lines(cap, pinned=True)
@@ -442,11 +449,11 @@ def list_section(torch, binding, args, oracle_py, nthr):
     lower_ok = np.nonzero((lv["energy"] - job.zpe >= job.erange_lower[0]) &
                           (lv["energy"] - job.zpe <= job.erange_lower[1]))[0]
     log(f"list candidates {cap}, lower levels {len(lower_ok)}")
-    modes = time_modes(eng, binding, bufs, mode_names, args.list_steps, 1)
+    modes = time_modes(eng, binding, bufs, mode_names, steps, 1, tune=full, sweep=full)
     roofs = mode_rooflines(modes, job.nvib)
     for m in modes.values():
         m.pop("_kstats", None)
-    out = dict(workload=f"C2v Nvib={job.nvib} J 0..{args.list_jmax} in one context, every "
+    out = dict(workload=f"{job.sym['name']} Nvib={job.nvib} J 0..{jmax} in one context, every "
                         "|dJ|<=1 pair, lower E<=10000 cm-1, window 0-10000 cm-1, T=296 K",
                nvib=job.nvib, jvals=[int(job.jvals[0]), int(job.jvals[-1])],
                blocks=len(sizes), block_sizes=[min(sizes), max(sizes)],
@@ -458,42 +465,41 @@ def list_section(torch, binding, args, oracle_py, nthr):
         for m in modes:
             modes[m]["speedup_vs_fp64_gemv"] = modes[m]["value"] / modes["fp64"]["value"]
     if oracle_py is not None:
-        # oracle sample: J=1 lower levels (every 2nd) and J=jmax A1 lower
-        # levels (every 25th); only the blocks those touch go to the host
+        # only the blocks the sample touches go to the host
         J = lv["jval"][lower_ok]
         g = lv["igamma"][lower_ok]
-        jm = int(job.jvals[-1])
-        sample = np.concatenate([lower_ok[J == 1][::2],
-                                 lower_ok[(J == jm) & (g == 0)][::25]]).astype(np.int32)
-        gF = int(job.igamma_pair[0])
+        top = lower_ok[(J == jmax) & (g == sample_g)]
+        stride = max(1, len(top) // ntop)
+        sample = np.concatenate([lower_ok[J == 1][::2], top[::stride][:ntop]]).astype(np.int32)
+        gF = int(job.igamma_pair[sample_g])
         need = {(j, gg) for j in (0, 1, 2) for gg in range(job.sym["nrep"])}
-        need |= {(jm, 0), (jm, gF), (jm - 1, gF)}
-        t = time.time()
+        need |= {(jmax, sample_g), (jmax, gF), (jmax - 1, gF)}
         for key in need:
             job.vectors[key] = dev[key].cpu().numpy()
         t = time.perf_counter()
         ref = oracle_py.run_chain(job, sample, nthreads=nthr)
         tc = time.perf_counter() - t
         par = sample_parity(eng, binding, ref, sample, bufs, mode_names)
-        par["sample"] = (f"{len(sample)} lower levels (J=1 every 2nd, J={jm} A1 every 25th), "
+        par["sample"] = (f"{len(sample)} lower levels (J=1 every 2nd, J={jmax} "
+                         f"{job.sym['labels'][sample_g]} 1 in {stride}), "
                          f"oracle chain of pair runs {tc:.1f} s")
         if "split" in mode_names:
             par["split_1500K"] = split_at_temperature(eng, job, binding, oracle_py, 1500.0, nthr,
-                                                      elow=(0.0, 300.0), jindI=jm, igammaI=0,
-                                                      jindF=jm)
+                                                      elow=(0.0, 300.0), jindI=jmax,
+                                                      igammaI=sample_g, jindF=jmax)
         for key in need:
             del job.vectors[key]
         out["parity"] = par
         log(f"list parity {par}")
     serving = None
     lams = [float(x) for x in args.serve_lambdas.split(",") if x]
-    if lams:
-        serving = {"workload": f"Poisson requests against the resident J 0..{args.list_jmax} "
+    if serve_modes and lams:
+        serving = {"workload": f"Poisson requests against the resident J 0..{jmax} "
                                "blocks: random (J, J') pair and lower symmetry, T ~ U[200,2000] K "
                                "with Q(T) on the device, 1000 cm-1 window at a random start in "
                                "[0, 9000], intensity threshold 10^U(-32,-26); lower levels of "
                                "the block with E <= 10000 cm-1"}
-        for mname in [m for m in args.serve_modes.split(",") if m]:
+        for mname in serve_modes:
             serving[mname] = serve_sweep(eng, job, binding, lams, args.serve_seconds,
                                          binding.MODES[mname])
     eng.close()
@@ -515,10 +521,14 @@ def main():
     ap.add_argument("--cache-dir", default=None,
                     help="reuse the eigenvector blocks across processes (profiler runs)")
     ap.add_argument("--list-jmax", type=int, default=20,
-                    help="J list 0..jmax in one context (the J-list workload); -1 to skip")
+                    help="C2v J list 0..jmax in one context (the J-list workload); -1 to skip")
     ap.add_argument("--list-nvib", type=int, default=2000)
     ap.add_argument("--list-modes", default="fp64,fp64_batched,split")
     ap.add_argument("--list-steps", type=int, default=3)
+    ap.add_argument("--c3v-jmax", type=int, default=15,
+                    help="C3v J list 0..jmax (the mixed-precision C3v workload); -1 to skip")
+    ap.add_argument("--c3v-nvib", type=int, default=3000)
+    ap.add_argument("--c3v-modes", default="fp64_batched,split")
     ap.add_argument("--serve-lambdas", default="10,50,100,200,500",
                     help="request rates (req/s) of the serving sweep; empty to skip")
     ap.add_argument("--serve-modes", default="fp64_batched,split")
@@ -610,9 +620,17 @@ def main():
     del dev, bufs
     torch.cuda.empty_cache()
 
-    multi_j = serving = None
+    multi_j = serving = c3v = None
     if args.list_jmax >= 1:
-        multi_j, serving = list_section(torch, binding, args, oracle_py, nthr)
+        multi_j, serving = list_section(torch, binding, args, oracle_py, nthr, "C2V",
+                                        args.list_nvib, args.list_jmax,
+                                        [m for m in args.list_modes.split(",") if m],
+                                        args.list_steps, 0, serve=True)
+    if args.c3v_jmax >= 1:
+        # the heavy C3v passes (~1.3 s each) run untuned, 1 + 2 times, without a sweep
+        c3v, _ = list_section(torch, binding, args, oracle_py, nthr, "C3V", args.c3v_nvib,
+                              args.c3v_jmax, [m for m in args.c3v_modes.split(",") if m], 2, 2,
+                              ntop=8, full=False)
 
     out = {
         "metric": METRIC, "value": value, "unit": "transitions/s", "n_gpus": 1,
@@ -635,6 +653,7 @@ def main():
         "parity": parity,
         "multi_j": multi_j,
         "serving": serving,
+        "c3v_multi_j": c3v,
     }
     print(json.dumps(out))
 

@@ -703,6 +703,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                                   hipMemcpyHostToDevice, S.stream));
             const int *d_tb = (const int *)(d_jobs + jobs_bytes);
             TL_BEGIN("half_ls_mfma");
+            P.work[14] += tt;
             launch_halfls_jobs(S.stream, c->tile_halfls, N, (const HalfJob *)d_jobs, d_tb, njobs,
                                tt, c->d_mu_perm);
             TL_END();
@@ -758,6 +759,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         HIPCHK(hipMemcpyAsync(d_mm, hmm, mm_bytes + 4 * mm.size(), hipMemcpyHostToDevice,
                               S.stream));
         TL_BEGIN("corr_mfma");
+        P.work[15] += ctiles;
         launch_corr_mfma(S.stream, c->tile_corr, (const CorrMMArgs *)d_mm,
                          (const int *)((unsigned char *)d_mm + mm_bytes), (int)mm.size(), ctiles);
         TL_END();
@@ -921,6 +923,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             if ((rc = send(d_hj, hj.data(), hj.size() * sizeof(SplitHalfJob), tbegin, hj_bytes)) != GI_OK)
                 return rc;
             TL_BEGIN("half_ls_split");
+            P.work[14] += tt;
             launch_halfls_split(S.stream, SD, N, c->kpad_all, c->d_mud, c->muplane, c->d_mue,
                                 (const SplitHalfJob *)d_hj, (const int *)(d_hj + hj_bytes),
                                 (int)hj.size(), tt);
@@ -976,6 +979,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             if ((rc = send(d_cj, cj.data(), cj.size() * sizeof(CorrSplitArgs), cbegin, cj_bytes)) != GI_OK)
                 return rc;
             TL_BEGIN("corr_split");
+            P.work[15] += ct;
             launch_corr_split(S.stream, SD, (const CorrSplitArgs *)d_cj, (const int *)(d_cj + cj_bytes),
                               (int)cj.size(), ct);
             TL_END();

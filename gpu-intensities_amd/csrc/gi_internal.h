@@ -18,7 +18,7 @@ constexpr int kMaxJ = 128;
 // Largest pipeline round count (gi_set_tuning).
 constexpr int kMaxPipeRounds = 64;
 // Counters of gi_work_stats.
-constexpr int kWorkStats = 14;
+constexpr int kWorkStats = 16;
 }  // namespace gi
 
 namespace gi {

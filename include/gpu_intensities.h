@@ -195,7 +195,9 @@ int gi_kernel_stats(gi_ctx *ctx, int max, const char **names, double *total_ms,
  * bytes (24*Nvib^2 per unit), [8] correlation flops executed; host side of
  * the call in ms: [9] planning (items, chunks, metadata), [10] enqueueing the
  * launches, [11] waiting for the device and the copies after that, [13] the
- * whole call; [12] bytes of the lines copied to `out`. n: up to 14. */
+ * whole call; [12] bytes of the lines copied to `out`; [14] tiles of the
+ * half-ls products and [15] of the correlation products (matrix-core modes,
+ * summed over launches). n: up to 16. */
 int gi_work_stats(gi_ctx *ctx, double *out, int n);
 
 /* GI_MODE_SPLIT (no reference counterpart): every operand row or column of
