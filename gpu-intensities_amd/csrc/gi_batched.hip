@@ -263,12 +263,15 @@ __global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict_
     if (tile >= ntiles) return;
     const int ci = find_job(cbegin, nchunks, tile);
     const CorrMMArgs &a = args[ci];
-    const int local = tile - cbegin[ci];
-    const int rt = (a.Hi - a.Lo + BM_ - 1) / BM_;
-    const int f0 = a.Lo + (local % rt) * BM_;  // block-level index of the first row
-    const int n0 = (local / rt) * BN_;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int ncols = a.nitems * a.ncb;
+    const int rt = (a.Hi - a.Lo + BM_ - 1) / BM_, per = rt * ((ncols + BN_ - 1) / BN_);
+    const int local = tile - cbegin[ci];
+    const int kp = local / per, l2 = local - kp * per;
+    const int f0 = a.Lo + (l2 % rt) * BM_;  // block-level index of the first row
+    const int n0 = (l2 / rt) * BN_;
+    const int kb = kp * a.kchunk, ke = min(a.nsize, kb + a.kchunk);
+    double *dst = kp == 0 ? a.amp : a.ampx + (long long)(kp - 1) * a.ampx_stride;
     // skip tiles outside every window of the tile's items
     if (tid == 0) {
         int need = 0;
@@ -307,13 +310,13 @@ __global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict_
         for (int q = 0; q < 4; q++) {
             const int t = k0 + lt + 4 * q;
 #pragma unroll
-            for (int p = 0; p < BM_ / 64; p++) ra[p][q] = (Urow[p] && t < a.nsize) ? Urow[p][t] : 0.0;
+            for (int p = 0; p < BM_ / 64; p++) ra[p][q] = (Urow[p] && t < ke) ? Urow[p][t] : 0.0;
 #pragma unroll
-            for (int p = 0; p < BN_ / 64; p++) rb[p][q] = (Hrow[p] && t < a.nsize) ? Hrow[p][t] : 0.0;
+            for (int p = 0; p < BN_ / 64; p++) rb[p][q] = (Hrow[p] && t < ke) ? Hrow[p][t] : 0.0;
         }
     };
-    fetch(0);
-    for (int k0 = 0; k0 < a.nsize; k0 += BK) {
+    fetch(kb);
+    for (int k0 = kb; k0 < ke; k0 += BK) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
 #pragma unroll
@@ -322,7 +325,7 @@ __global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict_
             for (int p = 0; p < BN_ / 64; p++) Bs[(lt + 4 * q) * T::PB + lr + 64 * p] = rb[p][q];
         }
         __syncthreads();
-        if (k0 + BK < a.nsize) fetch(k0 + BK);
+        if (k0 + BK < ke) fetch(k0 + BK);
         mfma_step<BM_, BN_>(As, Bs, wm, wn, lane, acc);
         __syncthreads();
     }
@@ -339,16 +342,15 @@ __global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict_
                 if (f < a.item_lo[i] || f >= a.item_hi[i]) continue;
                 const long long o = a.item_out_off[i] + (f - a.item_lo[i]);
                 const int idegF = cb / a.ndegI, idegI = cb - idegF * a.ndegI;
-                a.amp[4 * o + idegI + 2 * idegF] = acc[x][y][r];
+                dst[4 * o + idegI + 2 * idegF] = acc[x][y][r];
             }
 }
 
 // ---------------------------------------------------------------------------
 // Degeneracy sum, Einstein A, intensity and emit test per candidate, from the
 // amplitudes K6 stored (cuda_host.cu:1388-1407).
-__global__ __launch_bounds__(256) void k_intensity(IntensArgs a)
+__device__ inline void intensity_one(const IntensArgs &a, int c)
 {
-    const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= a.ncand) return;
     int lo = 0, hi = a.nitems - 1;
     while (lo < hi) {
@@ -365,7 +367,11 @@ __global__ __launch_bounds__(256) void k_intensity(IntensArgs a)
         for (int idegI = 0; idegI < a.ndegI; idegI++) {
             const int cb = idegF * a.ndegI + idegI;
             double A = 0.0;
-            if ((a.cb_mask >> cb) & 1) A = am[idegI + 2 * idegF];
+            if ((a.cb_mask >> cb) & 1) {
+                A = am[idegI + 2 * idegF];
+                for (int p = 1; p < a.kparts; p++)
+                    A += a.ampx[(long long)(p - 1) * a.ampx_stride + 4 * o + idegI + 2 * idegF];
+            }
             am[idegI + 2 * idegF] = A;
             ls += A * A;
         }
@@ -398,6 +404,22 @@ __global__ __launch_bounds__(256) void k_intensity(IntensArgs a)
     a.out_S[o] = S;
     a.out_A[o] = A_einst;
     a.out_I[o] = I;
+}
+
+__global__ __launch_bounds__(256) void k_intensity(IntensArgs a)
+{
+    intensity_one(a, blockIdx.x * 256 + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_intensity_jobs(const IntensArgs *__restrict__ jobs,
+                                                        const int *__restrict__ begin, int njobs)
+{
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (begin[mid] <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
+    }
+    intensity_one(jobs[lo], (blockIdx.x - begin[lo]) * 256 + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -445,6 +467,13 @@ void launch_corr_mfma(hipStream_t st, int cfg, const CorrMMArgs *d_args, const i
     case 2: hipLaunchKernelGGL((k_corr_mfma<128, 64>), g, b, 0, st, d_args, d_cbegin, nchunks, ntiles); break;
     default: hipLaunchKernelGGL((k_corr_mfma<128, 128>), g, b, 0, st, d_args, d_cbegin, nchunks, ntiles); break;
     }
+}
+
+void launch_intensity_jobs(hipStream_t st, const IntensArgs *d_jobs, const int *d_begin,
+                           int njobs, int nblocks)
+{
+    if (njobs <= 0 || nblocks <= 0) return;
+    hipLaunchKernelGGL(k_intensity_jobs, dim3(nblocks), dim3(256), 0, st, d_jobs, d_begin, njobs);
 }
 
 void launch_intensity(hipStream_t st, const IntensArgs &a)

@@ -176,6 +176,10 @@ struct Plan {
     std::vector<Chunk> chunks;
     std::vector<unsigned char> meta;
     LineSoA comp{};                                // compacted lines (slot buffers)
+    LineSoA cand{};                                // candidate lines (their own compaction
+                                                   // when every candidate passes)
+    double *ampx = nullptr;                        // correlation parts beyond the first
+    long long ampx_stride = 0;
     // sweep capture: each round's launches become one graph instead of running;
     // the intensity step then reads (beta, Q, thresholds) from qpar
     bool capture = false;
@@ -456,10 +460,20 @@ static int round_compact(gi_ctx *c, Slot &S, Plan &P, size_t r, int *flag, LineS
     return GI_OK;
 }
 
-// Intensity step of one chunk from its stored amplitudes.
-static int chunk_intensity(gi_ctx *c, Slot &S, const Plan &P, const Chunk &ch,
-                           unsigned char *d_meta, const Consts &K, int *flag, LineSoA cand,
-                           Timeline *tl)
+// Split-K parts of a chunk's split-mode correlation: a function of the upper
+// block only (never of the batch), so a candidate's sums do not depend on what
+// it is batched with. Long products are cut so that a round fills the GPU.
+static int corr_kparts(const gi_ctx *c, const Plan &P, const Chunk &ch)
+{
+    if (P.mode == GI_MODE_FP64) return 1;
+    const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
+    const int K = P.mode == GI_MODE_SPLIT ? BF.Kp : BF.nsize;
+    return std::min(kMaxKParts, std::max(1, K / 2048));
+}
+
+// Arguments of one chunk's intensity step from its stored amplitudes.
+static IntensArgs intens_args(gi_ctx *c, const Plan &P, const Chunk &ch, unsigned char *d_meta,
+                              const Consts &K, int *flag, LineSoA cand)
 {
     CorrArgs ca;
     fill_corr_args(c, P, ch, d_meta, K, ca);
@@ -474,10 +488,52 @@ static int chunk_intensity(gi_ctx *c, Slot &S, const Plan &P, const Chunk &ch,
     a.A_coef_s_1 = ca.A_coef_s_1; a.intens_cm_mol = ca.intens_cm_mol;
     a.thresh_S = ca.thresh_S; a.thresh_I = ca.thresh_I;
     a.qpar = P.qpar;
+    a.kparts = corr_kparts(c, P, ch);
+    a.ampx = P.ampx;
+    a.ampx_stride = P.ampx_stride;
     a.amp = cand.amp; a.flag = flag; a.out_iI = cand.iI; a.out_iF = cand.iF;
     a.out_nu = cand.nu; a.out_S = cand.S; a.out_A = cand.A; a.out_I = cand.I;
+    return a;
+}
+
+static unsigned char *stage(gi_ctx *c, Slot &S, const void *src, size_t n, int *rc);
+
+// Upload a job table (jobs, then the int first-block of each) through the
+// slot's pinned arena into dev.
+template <typename T>
+static int send_jobs(gi_ctx *c, Slot &S, unsigned char *dev, const std::vector<T> &jobs,
+                     const std::vector<int> &begin, size_t cap)
+{
+    int rc = GI_OK;
+    unsigned char *hb = stage(c, S, nullptr, cap + 4 * begin.size(), &rc);
+    if (rc) return rc;
+    std::memcpy(hb, jobs.data(), jobs.size() * sizeof(T));
+    std::memcpy(hb + cap, begin.data(), 4 * begin.size());
+    HIPCHK(hipMemcpyAsync(dev, hb, cap + 4 * begin.size(), hipMemcpyHostToDevice, S.stream));
+    return GI_OK;
+}
+
+// Intensity steps of chunks [c0, c1) in one launch (table in dev, room for cap bytes).
+static int round_intensity(gi_ctx *c, Slot &S, const Plan &P, size_t c0, size_t c1,
+                           unsigned char *d_meta, const Consts &K, int *flag, LineSoA cand,
+                           unsigned char *dev, size_t cap, Timeline *tl)
+{
+    std::vector<IntensArgs> jobs;
+    std::vector<int> begin;
+    int nb = 0;
+    for (size_t i = c0; i < c1; i++) {
+        const Chunk &ch = P.chunks[i];
+        if (ch.ncand <= 0) continue;
+        jobs.push_back(intens_args(c, P, ch, d_meta, K, flag, cand));
+        begin.push_back(nb);
+        nb += (int)((ch.ncand + 255) / 256);
+    }
+    if (jobs.empty()) return GI_OK;
+    int rc;
+    if ((rc = send_jobs(c, S, dev, jobs, begin, cap)) != GI_OK) return rc;
     TL_BEGIN("intensity");
-    launch_intensity(S.stream, a);
+    launch_intensity_jobs(S.stream, (const IntensArgs *)dev, (const int *)(dev + cap),
+                          (int)jobs.size(), nb);
     TL_END();
     return GI_OK;
 }
@@ -590,6 +646,18 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
     if (rc) return rc;
     GMapJob *d_mjobs = (GMapJob *)slot_scratch(
         c, S, "mjobs", (size_t)mchunks * (sizeof(GMapJob) + 4), &rc);
+    if (rc) return rc;
+    const size_t fcap = (size_t)mchunks * sizeof(FoldJob), icap = (size_t)mchunks * sizeof(IntensArgs);
+    unsigned char *d_fold = (unsigned char *)slot_scratch(c, S, "foldjobs", fcap + 4 * mchunks, &rc);
+    if (rc) return rc;
+    unsigned char *d_int = (unsigned char *)slot_scratch(c, S, "intjobs", icap + 4 * mchunks, &rc);
+    if (rc) return rc;
+    int maxparts = 1;
+    for (const Chunk &ch : chunks) maxparts = std::max(maxparts, corr_kparts(c, P, ch));
+    P.ampx_stride = 4 * std::max<long long>(P.ncand, 1);
+    P.ampx = maxparts > 1 ? (double *)slot_scratch(c, S, "c_ampx",
+                                                   8 * (size_t)(maxparts - 1) * P.ampx_stride, &rc)
+                          : nullptr;
     if (rc) return rc;
     int *pc_counts = nullptr;
     long long *pc_offs = nullptr;
@@ -708,6 +776,9 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                                tt, c->d_mu_perm);
             TL_END();
         }
+        std::vector<FoldJob> fjobs;
+        std::vector<int> fbegin;
+        int fblocks = 0;
         for (size_t i = rg.first; i < rg.second; i++) {
             const Chunk &ch = chunks[i];
             const JPlan &pF = c->jp[ch.jindF];
@@ -721,11 +792,14 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                 t1 = symdev(pF.sym[ch.gF * GI_MAXDEG + 1]);
                 t1.tcol = GP.d_tcol[1];
             }
-            TL_BEGIN("fold");
-            launch_fold(S.stream, ch.nitems, BF.nsize, ndegI, ndegF, GP.Htot,
-                        (const int *)(d_meta + ch.off_iunit), t0, t1, Hc[i - rg.first],
-                        Htc[i - rg.first]);
-            TL_END();
+            if (ch.nitems > 0 && BF.nsize > 0) {
+                FoldJob fj{ch.nitems, BF.nsize, ndegI, ndegF, GP.Htot, (BF.nsize + 255) / 256,
+                           (const int *)(d_meta + ch.off_iunit), t0, t1, Hc[i - rg.first],
+                           Htc[i - rg.first]};
+                fjobs.push_back(fj);
+                fbegin.push_back(fblocks);
+                fblocks += fj.xt * ch.nitems * ndegI * ndegF;
+            }
             CorrMMArgs m;
             m.Lo = ch.Lo; m.Hi = ch.Hi; m.nitems = ch.nitems; m.ncb = ndegI * ndegF;
             m.ndegI = ndegI; m.nsize = BF.nsize;
@@ -733,6 +807,10 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             m.item_hi = (const int *)(d_meta + ch.off_hi);
             m.item_out_off = (const long long *)(d_meta + ch.off_out);
             m.blk_col = BF.d_col; m.vec = BF.d_vec; m.Ht = Htc[i - rg.first]; m.amp = cand.amp;
+            m.kparts = corr_kparts(c, P, ch);
+            m.kchunk = (BF.nsize + 16 * m.kparts - 1) / (16 * m.kparts) * 16;
+            m.ampx = P.ampx;
+            m.ampx_stride = P.ampx_stride;
             mm.push_back(m);
             double blk_bytes = 0;
             for (const OutClass &oc : GP.oc) blk_bytes += 8.0 * oc.n * oc.K;
@@ -743,6 +821,13 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             P.work[7] += ch.nunits * 24.0 * (double)N * N;
             P.work[8] += 2.0 * (double)(ch.Hi - ch.Lo) * ch.nitems * ndegI * ndegF * BF.nsize;
         }
+        if (!fjobs.empty()) {
+            if ((rc = send_jobs(c, S, d_fold, fjobs, fbegin, fcap)) != GI_OK) return rc;
+            TL_BEGIN("fold");
+            launch_fold_jobs(S.stream, (const FoldJob *)d_fold, (const int *)(d_fold + fcap),
+                             (int)fjobs.size(), fblocks);
+            TL_END();
+        }
         // corr tiles: per chunk ceil(rows / CM) x ceil(cols / CN), flattened
         const int CM = tile_rows(c->tile_corr), CN = tile_cols(c->tile_corr);
         std::vector<int> cbegin(mm.size());
@@ -750,7 +835,8 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         for (size_t i = 0; i < mm.size(); i++) {
             cbegin[i] = ctiles;
             const int rows = mm[i].Hi - mm[i].Lo, cols = mm[i].nitems * mm[i].ncb;
-            if (rows > 0 && cols > 0) ctiles += ((rows + CM - 1) / CM) * ((cols + CN - 1) / CN);
+            if (rows > 0 && cols > 0)
+                ctiles += ((rows + CM - 1) / CM) * ((cols + CN - 1) / CN) * mm[i].kparts;
         }
         const size_t mm_bytes = mm.size() * sizeof(CorrMMArgs);
         unsigned char *hmm = stage(c, S, mm.data(), mm_bytes + 4 * mm.size(), &rc);
@@ -763,9 +849,9 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         launch_corr_mfma(S.stream, c->tile_corr, (const CorrMMArgs *)d_mm,
                          (const int *)((unsigned char *)d_mm + mm_bytes), (int)mm.size(), ctiles);
         TL_END();
-        for (size_t i = rg.first; i < rg.second; i++)
-            if ((rc = chunk_intensity(c, S, P, chunks[i], d_meta, K, flag, cand, tl)) != GI_OK)
-                return rc;
+        if ((rc = round_intensity(c, S, P, rg.first, rg.second, d_meta, K, flag, cand, d_int,
+                                  icap, tl)) != GI_OK)
+            return rc;
         if ((rc = round_compact(c, S, P, iround, flag, cand, pc_counts, pc_offs, tl)) != GI_OK)
             return rc;
         if ((rc = cap.end(c, P)) != GI_OK) return rc;
@@ -832,6 +918,21 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
     if (rc) return rc;
     const size_t cj_bytes = (size_t)mchunks * sizeof(CorrSplitArgs);
     unsigned char *d_cj = (unsigned char *)slot_scratch(c, S, "scorr", cj_bytes + 4 * mchunks, &rc);
+    if (rc) return rc;
+    const size_t fcap = (size_t)mchunks * sizeof(FoldJob), icap = (size_t)mchunks * sizeof(IntensArgs),
+                 scap = (size_t)mchunks * sizeof(SplitRowsJob);
+    unsigned char *d_fold = (unsigned char *)slot_scratch(c, S, "foldjobs", fcap + 4 * mchunks, &rc);
+    if (rc) return rc;
+    unsigned char *d_int = (unsigned char *)slot_scratch(c, S, "intjobs", icap + 4 * mchunks, &rc);
+    if (rc) return rc;
+    unsigned char *d_srj = (unsigned char *)slot_scratch(c, S, "srjobs", scap + 4 * mchunks, &rc);
+    if (rc) return rc;
+    int maxparts = 1;
+    for (const Chunk &ch : chunks) maxparts = std::max(maxparts, corr_kparts(c, P, ch));
+    P.ampx_stride = 4 * std::max<long long>(P.ncand, 1);
+    P.ampx = maxparts > 1 ? (double *)slot_scratch(c, S, "c_ampx",
+                                                   8 * (size_t)(maxparts - 1) * P.ampx_stride, &rc)
+                          : nullptr;
     if (rc) return rc;
     int *pc_counts = nullptr;
     long long *pc_offs = nullptr;
@@ -929,6 +1030,10 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                                 (int)hj.size(), tt);
             TL_END();
         }
+        std::vector<FoldJob> fjobs;
+        std::vector<SplitRowsJob> sjobs;
+        std::vector<int> fbegin, sbegin;
+        int fblocks = 0, srows = 0;
         for (size_t i = rg.first; i < rg.second; i++) {
             const Chunk &ch = chunks[i];
             const size_t k = i - rg.first;
@@ -944,14 +1049,18 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                 t1.tcol = GP.d_tcol[1];
             }
             const int cols = ch.nitems * ndegI * ndegF;
-            TL_BEGIN("fold");
-            launch_fold(S.stream, ch.nitems, BF.nsize, ndegI, ndegF, GP.Htot,
-                        (const int *)(d_meta + ch.off_iunit), t0, t1, H + Hoff[k], Ht + Htoff[k]);
-            TL_END();
-            TL_BEGIN("split_ht");
-            launch_split_rows(S.stream, SD, Ht + Htoff[k], BF.nsize, cols, BF.nsize, BF.Kp,
-                              Hd + Hdoff[k], (long long)mhd, He + Hcoff[k], nullptr);
-            TL_END();
+            if (cols > 0 && BF.nsize > 0) {
+                FoldJob fj{ch.nitems, BF.nsize, ndegI, ndegF, GP.Htot, (BF.nsize + 255) / 256,
+                           (const int *)(d_meta + ch.off_iunit), t0, t1, H + Hoff[k],
+                           Ht + Htoff[k]};
+                fjobs.push_back(fj);
+                fbegin.push_back(fblocks);
+                fblocks += fj.xt * cols;
+                sjobs.push_back(SplitRowsJob{Ht + Htoff[k], (long long)BF.nsize, BF.nsize, BF.Kp,
+                                             Hd + Hdoff[k], (long long)mhd, He + Hcoff[k]});
+                sbegin.push_back(srows);
+                srows += cols;
+            }
             CorrSplitArgs a;
             a.Lo = ch.Lo; a.Hi = ch.Hi; a.nitems = ch.nitems; a.ncb = ndegI * ndegF;
             a.ndegI = ndegI; a.Kp = BF.Kp;
@@ -962,10 +1071,14 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             a.Ud = BF.d_ud; a.uplane = BF.uplane; a.Ue = BF.d_ue;
             a.Hd = Hd + Hdoff[k]; a.hplane = (long long)mhd; a.He = He + Hcoff[k];
             a.amp = cand.amp;
+            a.kparts = corr_kparts(c, P, ch);
+            a.kchunk = (BF.Kp + 64 * a.kparts - 1) / (64 * a.kparts) * 64;
+            a.ampx = P.ampx;
+            a.ampx_stride = P.ampx_stride;
             cj.push_back(a);
             cbegin.push_back(ct);
             const int rows = ch.Hi - ch.Lo;
-            if (rows > 0 && cols > 0) ct += ((rows + 63) / 64) * ((cols + 63) / 64);
+            if (rows > 0 && cols > 0) ct += ((rows + 63) / 64) * ((cols + 63) / 64) * a.kparts;
             double blk_bytes = 0;
             for (const OutClass &oc : GP.oc) blk_bytes += 8.0 * oc.n * oc.K;
             P.work[3] += ch.nunits * 2.0 * coef_of(c, ch.jindI, ch.jindF).C * (double)N * N;
@@ -974,6 +1087,18 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             P.work[6] += blk_bytes;
             P.work[7] += ch.nunits * 24.0 * (double)N * N;
             P.work[8] += 2.0 * (double)rows * ch.nitems * ndegI * ndegF * BF.nsize;
+        }
+        if (!fjobs.empty()) {
+            if ((rc = send_jobs(c, S, d_fold, fjobs, fbegin, fcap)) != GI_OK) return rc;
+            TL_BEGIN("fold");
+            launch_fold_jobs(S.stream, (const FoldJob *)d_fold, (const int *)(d_fold + fcap),
+                             (int)fjobs.size(), fblocks);
+            TL_END();
+            if ((rc = send_jobs(c, S, d_srj, sjobs, sbegin, scap)) != GI_OK) return rc;
+            TL_BEGIN("split_ht");
+            launch_split_rows_jobs(S.stream, SD, (const SplitRowsJob *)d_srj,
+                                   (const int *)(d_srj + scap), (int)sjobs.size(), srows);
+            TL_END();
         }
         if (!cj.empty()) {
             if ((rc = send(d_cj, cj.data(), cj.size() * sizeof(CorrSplitArgs), cbegin, cj_bytes)) != GI_OK)
@@ -984,9 +1109,9 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                               (int)cj.size(), ct);
             TL_END();
         }
-        for (size_t i = rg.first; i < rg.second; i++)
-            if ((rc = chunk_intensity(c, S, P, chunks[i], d_meta, K, flag, cand, tl)) != GI_OK)
-                return rc;
+        if ((rc = round_intensity(c, S, P, rg.first, rg.second, d_meta, K, flag, cand, d_int,
+                                  icap, tl)) != GI_OK)
+            return rc;
         if ((rc = round_compact(c, S, P, iround, flag, cand, pc_counts, pc_offs, tl)) != GI_OK)
             return rc;
         if ((rc = cap.end(c, P)) != GI_OK) return rc;
@@ -1030,6 +1155,7 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
     comp.I = (double *)slot_scratch(c, S, "o_I", nc * 8, &rc); if (rc) return rc;
     comp.amp = (double *)slot_scratch(c, S, "o_amp", nc * 32, &rc); if (rc) return rc;
     P.comp = comp;
+    P.cand = cand;
     // compaction scratch: per query, block counts and offsets
     std::vector<size_t> cbase(P.q.size()), obase(P.q.size());
     size_t ncnt = 0, noff = 0;
@@ -1180,7 +1306,9 @@ static int copy_range(gi_ctx *c, hipStream_t st, const LineSoA &comp, long long 
 static int copy_lines(gi_ctx *c, Slot &S, const Plan &P, size_t k, int *iI, int *iF, double *nu,
                       double *Sv, double *A, double *I, double *amp)
 {
-    return copy_range(c, S.stream, P.comp, P.q[k].cand_begin, P.counts[k], iI, iF, nu, Sv, A, I,
+    const long long range = P.q[k].cand_end - P.q[k].cand_begin;
+    const LineSoA &src = P.counts[k] == range ? P.cand : P.comp;  // all passed: no scatter
+    return copy_range(c, S.stream, src, P.q[k].cand_begin, P.counts[k], iI, iF, nu, Sv, A, I,
                       amp, 0);
 }
 
@@ -1196,7 +1324,8 @@ static int drain_rounds(gi_ctx *c, Slot &S, const Plan &P, gi_lines *out, long l
         const long long n = o.nb > 0 ? P.rcounts[r] : 0;
         if (n > 0) {
             HIPCHK(hipStreamWaitEvent(S.cstream, o.ev, 0));
-            if ((rc = copy_range(c, S.cstream, P.comp, o.cb, n, out->ilevelI, out->ilevelF,
+            const LineSoA &src = n == o.ce - o.cb ? P.cand : P.comp;  // all passed: no scatter
+            if ((rc = copy_range(c, S.cstream, src, o.cb, n, out->ilevelI, out->ilevelF,
                                  out->nu, out->S, out->A_einst, out->I, out->amp, *nout)) !=
                 GI_OK)
                 return rc;

@@ -17,6 +17,8 @@ namespace gi {
 constexpr int kMaxJ = 128;
 // Largest pipeline round count (gi_set_tuning).
 constexpr int kMaxPipeRounds = 64;
+// Most split-K parts of a correlation product.
+constexpr int kMaxKParts = 4;
 // Counters of gi_work_stats.
 constexpr int kWorkStats = 16;
 }  // namespace gi

@@ -57,6 +57,9 @@ struct CorrMMArgs {
     const int *blk_col;
     const double *vec, *Ht;
     double *amp;                     // candidate amplitudes, 4 per candidate
+    int kparts, kchunk;              // split K as in CorrSplitArgs
+    double *ampx;
+    long long ampx_stride;
 };
 
 // Intensity step from stored amplitudes (batched mode).
@@ -71,6 +74,9 @@ struct IntensArgs {
     double gnsI, zpe, beta, q_stat, A_coef_s_1, intens_cm_mol, thresh_S, thresh_I;
     const double *qpar;              // non-null: beta, q_stat, thresh_S, thresh_I read here
     double *amp;
+    int kparts;                      // correlation parts: amp + ampx[0..kparts-2], in order
+    const double *ampx;
+    long long ampx_stride;
     int *flag, *out_iI, *out_iF;
     double *out_nu, *out_S, *out_A, *out_I;
 };
@@ -151,6 +157,11 @@ struct CorrSplitArgs {
     long long hplane;
     const int *He;                   // per column
     double *amp;
+    // split K: part p of kparts sums K in [p*kchunk, (p+1)*kchunk); part 0 writes
+    // amp, part p > 0 writes ampx + (p-1)*ampx_stride (same candidate layout)
+    int kparts, kchunk;
+    double *ampx;
+    long long ampx_stride;
 };
 void launch_split_rows(hipStream_t st, int S, const double *X, long long ld, int rows, int K,
                        int Kp, int8_t *D, long long plane, int *E, double *Mx);
@@ -191,6 +202,32 @@ int launch_gemv_sym(hipStream_t st, int nunits, int N, int Gtot, int Htot, const
                     const double *mu, const double *G, double *H);
 void launch_fold(hipStream_t st, int nitems, int nsizeF, int ndegI, int ndegF, int SFN,
                  const int *item_unit, SymDev t0, SymDev t1, const double *H, double *Ht);
+
+// The folds of every chunk of a round in one launch: job = one chunk's
+// launch_fold arguments; blocks of a job = xt * nitems * ndegI * ndegF.
+struct FoldJob {
+    int nitems, nsizeF, ndegI, ndegF, SFN, xt;
+    const int *item_unit;
+    SymDev t0, t1;
+    const double *H;
+    double *Ht;
+};
+void launch_fold_jobs(hipStream_t st, const FoldJob *d_jobs, const int *d_begin, int njobs,
+                      int nblocks);
+// The digit splits of every chunk of a round: one workgroup per row.
+struct SplitRowsJob {
+    const double *X;
+    long long ld;
+    int K, Kp;
+    int8_t *D;
+    long long plane;
+    int *E;
+};
+void launch_split_rows_jobs(hipStream_t st, int S, const SplitRowsJob *d_jobs,
+                            const int *d_begin, int njobs, int nrows);
+// The intensity steps of every chunk of a round: blocks of a job = ceil(ncand / 256).
+void launch_intensity_jobs(hipStream_t st, const IntensArgs *d_jobs, const int *d_begin,
+                           int njobs, int nblocks);
 void launch_corr_intensity(hipStream_t st, const CorrArgs &a);
 int compact_blocks(long long n);
 void launch_compact(hipStream_t st, long long n, const int *flag, int *counts, long long *offs,

@@ -226,15 +226,11 @@ __global__ __launch_bounds__(256) void k_gemv_sym(int N, int Gtot, int Htot, Cla
 // ---------------------------------------------------------------------------
 // K3: fold H into the symmetrised space of the upper block:
 // Ht[item][cb][t] = sum_{(g, w) in R_F^T[t]} w * H[unit][g], cb = idegF*ndegI + idegI.
-__global__ __launch_bounds__(256) void k_fold(int nsizeF, int ndegI, int ndegF, int SFN,
-                                              const int *__restrict__ item_unit,
-                                              SymDev t0, SymDev t1,
-                                              const double *__restrict__ H,
-                                              double *__restrict__ Ht)
+__device__ inline void fold_one(int t, int item, int cb, int nsizeF, int ndegI, int ndegF,
+                                int SFN, const int *__restrict__ item_unit, const SymDev &t0,
+                                const SymDev &t1, const double *__restrict__ H,
+                                double *__restrict__ Ht)
 {
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    const int item = blockIdx.y;
-    const int cb = blockIdx.z;
     if (t >= nsizeF) return;
     const int idegF = cb / ndegI, idegI = cb % ndegI;
     const int unit = item_unit[item * GI_MAXDEG + idegI];
@@ -245,6 +241,39 @@ __global__ __launch_bounds__(256) void k_fold(int nsizeF, int ndegI, int ndegF, 
         for (int e = T.tptr[t]; e < T.tptr[t + 1]; e++) acc += T.tval[e] * Hu[T.tcol[e]];
     }
     Ht[((size_t)item * ndegI * ndegF + cb) * nsizeF + t] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_fold(int nsizeF, int ndegI, int ndegF, int SFN,
+                                              const int *__restrict__ item_unit,
+                                              SymDev t0, SymDev t1,
+                                              const double *__restrict__ H,
+                                              double *__restrict__ Ht)
+{
+    fold_one(blockIdx.x * 256 + threadIdx.x, blockIdx.y, blockIdx.z, nsizeF, ndegI, ndegF, SFN,
+             item_unit, t0, t1, H, Ht);
+}
+
+// Job owning block b: the last job whose first block is <= b.
+__device__ inline int job_of(const int *__restrict__ begin, int njobs, int b)
+{
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (begin[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_fold_jobs(const FoldJob *__restrict__ jobs,
+                                                   const int *__restrict__ begin, int njobs)
+{
+    const int j = job_of(begin, njobs, blockIdx.x);
+    const FoldJob &J = jobs[j];
+    const int local = blockIdx.x - begin[j];
+    const int xb = local % J.xt, rest = local / J.xt;
+    const int item = rest % J.nitems, cb = rest / J.nitems;
+    fold_one(xb * 256 + threadIdx.x, item, cb, J.nsizeF, J.ndegI, J.ndegF, J.SFN, J.item_unit,
+             J.t0, J.t1, J.H, J.Ht);
 }
 
 __device__ inline double wave_sum(double v)
@@ -382,6 +411,7 @@ __global__ __launch_bounds__(GI_CT) void k_scatter(long long n, const int *__res
                                                    LineSoA in, LineSoA out)
 {
     __shared__ int pre[GI_CT];
+    if (offs[gridDim.x] == n) return;  // all pass: the input is its own compaction
     const long long base = (long long)blockIdx.x * GI_CPB + (long long)threadIdx.x * GI_CPT;
     int c = 0;
     for (int i = 0; i < GI_CPT; i++)
@@ -495,6 +525,13 @@ void launch_fold(hipStream_t st, int nitems, int nsizeF, int ndegI, int ndegF, i
                        t1, H, Ht);
 }
 
+void launch_fold_jobs(hipStream_t st, const FoldJob *d_jobs, const int *d_begin, int njobs,
+                      int nblocks)
+{
+    if (njobs <= 0 || nblocks <= 0) return;
+    hipLaunchKernelGGL(k_fold_jobs, dim3(nblocks), dim3(256), 0, st, d_jobs, d_begin, njobs);
+}
+
 void launch_corr_intensity(hipStream_t st, const CorrArgs &a)
 {
     const long long threads = (long long)a.ncand * 64;
@@ -509,6 +546,8 @@ void launch_compact(hipStream_t st, long long n, const int *flag, int *counts, l
 {
     const int nb = compact_blocks(n);
     if (nb == 0) return;
+    // when every candidate passes, k_scatter leaves `out` alone and the
+    // caller copies from `in` (same rows, same order)
     hipLaunchKernelGGL(k_block_count, dim3(nb), dim3(GI_CT), 0, st, n, flag, counts);
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(GI_CT), 0, st, nb, counts, offs);
     hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(GI_CT), 0, st, n, flag, offs, in, out);

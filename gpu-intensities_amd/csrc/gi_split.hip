@@ -69,13 +69,10 @@ __device__ inline double block_max(double m, double *red)
 // Rows of a row-major FP64 matrix (row r at X + r*ld, K valid entries) to
 // digit planes D[q*plane + r*Kp + k] and exponents E[r]; optional row maxima.
 template <int S>
-__global__ __launch_bounds__(256) void k_split_rows(const double *__restrict__ X, long long ld,
-                                                    int K, int Kp, int8_t *__restrict__ D,
-                                                    long long plane, int *__restrict__ E,
-                                                    double *__restrict__ Mx)
+__device__ inline void split_row(const double *__restrict__ X, long long ld, int K, int Kp,
+                                 int8_t *__restrict__ D, long long plane, int *__restrict__ E,
+                                 double *__restrict__ Mx, long long r, double *red)
 {
-    __shared__ double red[256];
-    const long long r = blockIdx.x;
     const double *x = X + r * ld;
     double m = 0.0;
     for (int k = threadIdx.x; k < K; k += 256) m = fmax(m, fabs(x[k]));
@@ -92,6 +89,31 @@ __global__ __launch_bounds__(256) void k_split_rows(const double *__restrict__ X
 #pragma unroll
         for (int q = 0; q < S; q++) d[q * plane + k] = (int8_t)dg[q];
     }
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_split_rows(const double *__restrict__ X, long long ld,
+                                                    int K, int Kp, int8_t *__restrict__ D,
+                                                    long long plane, int *__restrict__ E,
+                                                    double *__restrict__ Mx)
+{
+    __shared__ double red[256];
+    split_row<S>(X, ld, K, Kp, D, plane, E, Mx, blockIdx.x, red);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_split_rows_jobs(const SplitRowsJob *__restrict__ jobs,
+                                                         const int *__restrict__ begin,
+                                                         int njobs)
+{
+    __shared__ double red[256];
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (begin[mid] <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
+    }
+    const SplitRowsJob &J = jobs[lo];
+    split_row<S>(J.X, J.ld, J.K, J.Kp, J.D, J.plane, J.E, nullptr, blockIdx.x - begin[lo], red);
 }
 
 // mu (permuted, mu_a(vF, vI) at vF + vI*N + a*N^2) to digit planes with row
@@ -330,12 +352,15 @@ __global__ __launch_bounds__(256) void k_corr_split(const CorrSplitArgs *__restr
     if (tile >= ntiles) return;
     const int ci = find_tile_job(cbegin, nchunks, tile);
     const CorrSplitArgs &a = args[ci];
-    const int local = tile - cbegin[ci];
-    const int rt = (a.Hi - a.Lo + 63) / 64;
-    const int f0 = a.Lo + (local % rt) * 64;
-    const int n0 = (local / rt) * 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int ncols = a.nitems * a.ncb;
+    const int rt = (a.Hi - a.Lo + 63) / 64, per = rt * ((ncols + 63) / 64);
+    const int local = tile - cbegin[ci];
+    const int kp = local / per, l2 = local - kp * per;
+    const int f0 = a.Lo + (l2 % rt) * 64;
+    const int n0 = (l2 / rt) * 64;
+    const int kb = kp * a.kchunk, ke = min(a.Kp, kb + a.kchunk);
+    double *dst = kp == 0 ? a.amp : a.ampx + (long long)(kp - 1) * a.ampx_stride;
     if (tid == 0) {
         int need = 0;
         const int i0 = n0 / a.ncb, i1 = min(a.nitems, (n0 + 64 + a.ncb - 1) / a.ncb);
@@ -368,15 +393,15 @@ __global__ __launch_bounds__(256) void k_corr_split(const CorrSplitArgs *__restr
             rb[q] = nok ? *(const v4i *)(a.Hd + q * a.hplane + brow + k0 + lp) : v4i{0, 0, 0, 0};
         }
     };
-    fetch(0);
-    for (int k0 = 0; k0 < a.Kp; k0 += 64) {
+    fetch(kb);
+    for (int k0 = kb; k0 < ke; k0 += 64) {
 #pragma unroll
         for (int q = 0; q < S; q++) {
             *(v4i *)(As + (q * 64 + lr) * 80 + lp) = ra[q];
             *(v4i *)(Bs + (q * 64 + lr) * 80 + lp) = rb[q];
         }
         __syncthreads();
-        if (k0 + 64 < a.Kp) fetch(k0 + 64);
+        if (k0 + 64 < ke) fetch(k0 + 64);
         oz_step<S>(As, Bs, wm, wn, lane, acc);
         __syncthreads();
     }
@@ -394,7 +419,7 @@ __global__ __launch_bounds__(256) void k_corr_split(const CorrSplitArgs *__restr
                 const long long o = a.item_out_off[i] + (f - a.item_lo[i]);
                 const int idegF = cb / a.ndegI, idegI = cb - idegF * a.ndegI;
                 const int ex = a.Ue[a.blk_col[f]] + a.He[nn];
-                a.amp[4 * o + idegI + 2 * idegF] = ldexp(oz_value<S>(acc[x][y], r), ex);
+                dst[4 * o + idegI + 2 * idegF] = ldexp(oz_value<S>(acc[x][y], r), ex);
             }
 }
 
@@ -413,6 +438,15 @@ void launch_split_rows(hipStream_t st, int S, const double *X, long long ld, int
 {
     if (rows <= 0) return;
 #define L_(s) hipLaunchKernelGGL(k_split_rows<s>, dim3(rows), dim3(256), 0, st, X, ld, K, Kp, D, plane, E, Mx)
+    GI_SPLIT_DISPATCH(S, L_)
+#undef L_
+}
+
+void launch_split_rows_jobs(hipStream_t st, int S, const SplitRowsJob *d_jobs,
+                            const int *d_begin, int njobs, int nrows)
+{
+    if (njobs <= 0 || nrows <= 0) return;
+#define L_(s) hipLaunchKernelGGL(k_split_rows_jobs<s>, dim3(nrows), dim3(256), 0, st, d_jobs, d_begin, njobs)
     GI_SPLIT_DISPATCH(S, L_)
 #undef L_
 }
