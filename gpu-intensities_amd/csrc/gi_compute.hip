@@ -377,8 +377,12 @@ static void fill_corr_args(gi_ctx *c, const Plan &P, const Chunk &ch, unsigned c
 static LineSoA shift(LineSoA s, long long o);
 
 // Consecutive chunks grouped into rounds of at most kRoundBytes of device
-// scratch (GI_BATCH_BYTES overrides); with per-round output at least
-// c->pipe_rounds rounds of >= 1 MB (gi_set_tuning, gi_autotune).
+// scratch (GI_BATCH_BYTES overrides). With per-round output the first
+// c->pipe_rounds - 1 rounds each take half of the work still left (at least
+// 1 MB of scratch) and the last takes the rest: each round's line copy then
+// overlaps the next round's compute (copies outrun the compute that produces
+// the lines), and the copy left after the last round is 1/2^(pipe_rounds-1)
+// of the output instead of 1/pipe_rounds.
 static std::vector<std::pair<size_t, size_t>> plan_rounds(const gi_ctx *c, const Plan &P,
                                                           const std::vector<size_t> &bytes)
 {
@@ -387,19 +391,21 @@ static std::vector<std::pair<size_t, size_t>> plan_rounds(const gi_ctx *c, const
         const long long v = std::atoll(env);
         if (v > 0) kRoundBytes = (size_t)v;
     }
-    if (P.pipe) {
-        size_t total = 0;
-        for (size_t b : bytes) total += b;
-        kRoundBytes = std::min(kRoundBytes,
-                               std::max(total / c->pipe_rounds + 1, (size_t)1 << 20));
-    }
+    size_t left = 0;
+    for (size_t b : bytes) left += b;
+    auto budget = [&](size_t r) {
+        if (!P.pipe || (int)r + 1 >= c->pipe_rounds) return kRoundBytes;
+        return std::min(kRoundBytes, std::max(left / 2, (size_t)1 << 20));
+    };
     std::vector<std::pair<size_t, size_t>> rounds;
-    size_t start = 0, acc = 0;
+    size_t start = 0, acc = 0, cap = budget(0);
     for (size_t i = 0; i < bytes.size(); i++) {
-        if (i > start && acc + bytes[i] > kRoundBytes) {
+        if (i > start && acc + bytes[i] > cap) {
             rounds.push_back({start, i});
+            left -= acc;
             start = i;
             acc = 0;
+            cap = budget(rounds.size());
         }
         acc += bytes[i];
     }

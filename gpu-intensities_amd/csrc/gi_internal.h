@@ -224,7 +224,7 @@ struct gi_ctx {
     // tile configurations of k_halfls_jobs and k_corr_mfma (see tile_rows), and
     // the rounds of a pipelined gi_compute (gi_set_tuning, gi_autotune)
     int tile_halfls = 3, tile_corr = 0;
-    int pipe_rounds = 2;
+    int pipe_rounds = 4;
 
     // split mode: digits per operand (3..6) and the resident digit planes of mu
     int split_S = 6, split_ready = 0;  // split_ready = digits the planes were built with

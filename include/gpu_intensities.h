@@ -225,8 +225,9 @@ int gi_get_tiles(gi_ctx *ctx, int *halfls_cfg, int *corr_cfg);
  * reference has no batched form). halfls_cfg / corr_cfg are the tile shapes
  * of gi_set_tiles; pipe_rounds is the number of rounds a gi_compute with room
  * for every candidate is cut into, each round's lines being copied to the
- * host while later rounds compute (1..64; default 2). Lines are bitwise the
- * same for every setting. */
+ * host while later rounds compute (1..64; default 4): every round but the
+ * last takes half of the work left, so the copy after the last round is
+ * small. Lines are bitwise the same for every setting. */
 typedef struct gi_tuning {
     int halfls_cfg;
     int corr_cfg;

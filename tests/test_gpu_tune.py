@@ -13,7 +13,7 @@ from test_gpu_parity import compare
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = dict(halfls_cfg=3, corr_cfg=0, pipe_rounds=2)
+DEFAULT = dict(halfls_cfg=3, corr_cfg=0, pipe_rounds=4)
 
 
 @pytest.fixture
