@@ -33,7 +33,7 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_params", "gi_uplo
            "gi_serve_submit", "gi_serve_wait", "gi_serve_fetch", "gi_serve_stats",
            "gi_serve_stop", "gi_set_tiles", "gi_get_tiles", "gi_set_split_digits",
            "gi_prepare_split", "gi_set_jvals", "gi_set_tuning", "gi_get_tuning",
-           "gi_autotune", "gi_sweep_create", "gi_sweep_run", "gi_sweep_candidates",
+           "gi_autotune", "gi_set_split_digits_each", "gi_sweep_create", "gi_sweep_run", "gi_sweep_candidates",
            "gi_sweep_destroy"]
 # (TROVE_EXPORTS below: the symbols of include/gi_trove.h)
 
@@ -110,6 +110,7 @@ def lib():
         L.gi_serve_stop.argtypes = [vp]
         L.gi_set_tiles.argtypes = [vp, ci, ci]
         L.gi_set_split_digits.argtypes = [vp, ci]
+        L.gi_set_split_digits_each.argtypes = [vp, ci, ci]
         L.gi_prepare_split.argtypes = [vp]
         L.gi_get_tiles.argtypes = [vp, P_i, P_i]
         L.gi_sweep_create.argtypes = [vp, ci, ci, P_i, C.POINTER(vp)]
@@ -297,6 +298,10 @@ class Engine:
 
     def set_split_digits(self, digits):
         self._chk(lib().gi_set_split_digits(self.h, digits), "gi_set_split_digits")
+
+    def set_split_digits_each(self, halfls_digits, corr_digits):
+        self._chk(lib().gi_set_split_digits_each(self.h, halfls_digits, corr_digits),
+                  "gi_set_split_digits_each")
 
     def prepare_split(self):
         self._chk(lib().gi_prepare_split(self.h), "gi_prepare_split")

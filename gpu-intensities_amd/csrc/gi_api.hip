@@ -184,13 +184,14 @@ void free_split(gi_ctx *c)
 int prepare_split(gi_ctx *c)
 {
     if (!c->prepared) return fail(c, GI_ESTATE, "split mode before gi_prepare");
-    if (c->split_ready == c->split_S) return GI_OK;
+    if (c->split_ready == c->split_key()) return GI_OK;
     HIPCHK(hipSetDevice(c->device));
     free_split(c);
-    const int SD = c->split_S, N = c->nvib;
+    const int SD = c->split_S, SH = c->split_Sh, N = c->nvib;
     // int32 sums of digit products stay exact while 4096 * S * K < 2^31
     const long long kmax = (1LL << 31) / (4096LL * SD) - 64;
-    if (c->kpad_all > kmax) return fail(c, GI_EINVAL, "split mode: vibrational basis too large");
+    if (c->kpad_all > (1LL << 31) / (4096LL * SH) - 64)
+        return fail(c, GI_EINVAL, "split mode: vibrational basis too large");
     int rc;
     std::vector<int> vinv(c->kpad_all, -1);
     for (size_t k = 0; k < c->vcls_off.size(); k++)
@@ -198,9 +199,9 @@ int prepare_split(gi_ctx *c)
     int *d_vinv = nullptr;
     if ((rc = upload(c, &d_vinv, vinv)) != GI_OK) return rc;
     c->muplane = 3LL * N * c->kpad_all;
-    HIPCHK(hipMalloc((void **)&c->d_mud, (size_t)SD * c->muplane));
+    HIPCHK(hipMalloc((void **)&c->d_mud, (size_t)SH * c->muplane));
     HIPCHK(hipMalloc((void **)&c->d_mue, sizeof(int) * N));
-    launch_split_mu(c->stream, SD, N, c->kpad_all, c->d_mu_perm, d_vinv, c->d_mud, c->muplane,
+    launch_split_mu(c->stream, SH, N, c->kpad_all, c->d_mu_perm, d_vinv, c->d_mud, c->muplane,
                     c->d_mue);
     HIPCHK(hipGetLastError());
     for (auto &b : c->blocks) {
@@ -217,7 +218,7 @@ int prepare_split(gi_ctx *c)
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     (void)hipFree(d_vinv);
-    c->split_ready = SD;
+    c->split_ready = c->split_key();
     return GI_OK;
 }
 
@@ -737,17 +738,21 @@ int gi_kernel_stats(gi_ctx *c, int max, const char **names, double *total_ms, in
     return GI_OK;
 }
 
-int gi_set_split_digits(gi_ctx *c, int digits)
+int gi_set_split_digits_each(gi_ctx *c, int halfls_digits, int corr_digits)
 {
-    if (!c || digits < 3 || digits > 6) return GI_EINVAL;
+    if (!c || halfls_digits < 3 || halfls_digits > 6 || corr_digits < 3 || corr_digits > 6)
+        return GI_EINVAL;
     if (serving(c)) return GI_ESTATE;
-    if (digits != c->split_S) {
+    if (halfls_digits != c->split_Sh || corr_digits != c->split_S) {
         HIPCHK(hipSetDevice(c->device));
         free_split(c);
-        c->split_S = digits;
+        c->split_Sh = halfls_digits;
+        c->split_S = corr_digits;
     }
     return GI_OK;
 }
+
+int gi_set_split_digits(gi_ctx *c, int digits) { return gi_set_split_digits_each(c, digits, digits); }
 
 int gi_prepare_split(gi_ctx *c)
 {

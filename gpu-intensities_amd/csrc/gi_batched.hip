@@ -435,6 +435,40 @@ void launch_build_g_units(hipStream_t st, int N, const GBuildJob *d_jobs, const 
                        d_begin, njobs, ntiles);
 }
 
+// G of one unit per workgroup: the unit's record read once into LDS, the map's
+// gathers served from there (in global memory they land on scattered lines).
+__global__ __launch_bounds__(1024) void k_build_g_map_lds(const GMapJob *__restrict__ jobs,
+                                                          const int *__restrict__ ubegin,
+                                                          int njobs)
+{
+    __shared__ double rec[kRecLds];
+    const int ji = find_job(ubegin, njobs, blockIdx.x);
+    const GMapJob &J = jobs[ji];
+    const int u = blockIdx.x - ubegin[ji];
+    const double *src = J.uptr[u];
+    for (int i = threadIdx.x; i < J.nsize; i += 1024) rec[i] = src[i];
+    __syncthreads();
+    const int d = J.uideg[u];
+    const int *ip = J.idx[d];
+    const double *wp = J.wt[d];
+    double *G = J.G + (size_t)u * J.Gtot;
+    for (int g = threadIdx.x; g < J.Gtot; g += 1024) {
+        double v = 0.0;
+#pragma unroll
+        for (int w = 0; w < 4; w++)
+            if (w < J.W) v += wp[(size_t)w * J.Gtot + g] * rec[ip[(size_t)w * J.Gtot + g]];
+        G[g] = v;
+    }
+}
+
+void launch_build_g_map_lds(hipStream_t st, const GMapJob *d_jobs, const int *d_begin,
+                            int njobs, int nunits)
+{
+    if (njobs <= 0 || nunits <= 0) return;
+    hipLaunchKernelGGL(k_build_g_map_lds, dim3(nunits), dim3(1024), 0, st, d_jobs, d_begin,
+                       njobs);
+}
+
 void launch_build_g_map(hipStream_t st, const GMapJob *d_jobs, const int *d_begin, int njobs,
                         int ntiles)
 {

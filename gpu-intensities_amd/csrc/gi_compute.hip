@@ -676,7 +676,9 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         std::vector<HalfJob> jobs;
         std::vector<GBuildJob> gjobs;
         std::vector<GMapJob> mjobs_v;
-        std::vector<int> gbegin, tbegin, mbegin;
+        std::vector<int> gbegin, tbegin, mbegin, mubeg;
+        bool mlds = true;  // every map job fits the one-unit-per-workgroup build
+        int munits = 0;
         int mt_tiles = 0;
         std::vector<CorrMMArgs> mm;
         std::vector<double *> Hc(rg.second - rg.first), Htc(rg.second - rg.first);
@@ -707,8 +709,12 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                 mj.uptr = (const double *const *)(d_meta + ch.off_uptr);
                 mj.uideg = (const int *)(d_meta + ch.off_uideg);
                 mj.G = G + og;
+                mj.nsize = c->blocks[ch.jindI * c->nrep + ch.gI].nsize;
+                mlds = mlds && GP.gW <= 4 && mj.nsize <= kRecLds;
                 mjobs_v.push_back(mj);
                 mbegin.push_back(mj.tile_begin);
+                mubeg.push_back(munits);
+                munits += ch.nunits;
             } else if (ch.nunits > 0 && GP.Gtot_p > 0) {
                 GBuildJob gj;
                 gj.classes = GP.d_cd;
@@ -747,12 +753,16 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             const size_t mb = nmj * sizeof(GMapJob);
             unsigned char *hb = stage(c, S, mjobs_v.data(), mb + 4 * (size_t)nmj, &rc);
             if (rc) return rc;
-            std::memcpy(hb + mb, mbegin.data(), 4 * (size_t)nmj);
+            std::memcpy(hb + mb, (mlds ? mubeg : mbegin).data(), 4 * (size_t)nmj);
             HIPCHK(hipMemcpyAsync(d_mjobs, hb, mb + 4 * (size_t)nmj, hipMemcpyHostToDevice,
                                   S.stream));
             TL_BEGIN("build_g");
-            launch_build_g_map(S.stream, d_mjobs, (const int *)((unsigned char *)d_mjobs + mb),
-                               nmj, mt_tiles);
+            if (mlds)
+                launch_build_g_map_lds(S.stream, d_mjobs,
+                                       (const int *)((unsigned char *)d_mjobs + mb), nmj, munits);
+            else
+                launch_build_g_map(S.stream, d_mjobs, (const int *)((unsigned char *)d_mjobs + mb),
+                                   nmj, mt_tiles);
             TL_END();
         }
         if (ngj > 0) {
@@ -873,7 +883,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
 static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timeline *tl,
                      int *flag, LineSoA cand)
 {
-    const int N = c->nvib, SD = c->split_S;
+    const int N = c->nvib, SD = c->split_S, SH = c->split_Sh;
     const Consts K;
     int rc = GI_OK;
     const auto &chunks = P.chunks;
@@ -893,7 +903,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
         need[i] = Need{(size_t)ch.nunits * GP.GtotB, (size_t)ch.nunits * GP.EGtot,
                        (size_t)ch.nunits * GP.Htot, cols * BF.nsize, cols * BF.Kp, cols,
                        ch.nunits > 0 ? (int)GP.oc.size() : 0};
-        bytes[i] = SD * need[i].gd + 4 * need[i].ge + 8 * need[i].h + 8 * need[i].ht +
+        bytes[i] = SH * need[i].gd + 4 * need[i].ge + 8 * need[i].h + 8 * need[i].ht +
                    SD * need[i].hd + 4 * need[i].hc;
     }
     const auto rounds = plan_rounds(c, P, bytes);
@@ -910,7 +920,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
         mjobs = std::max(mjobs, t.njobs);
         mchunks = std::max(mchunks, (int)(rg.second - rg.first));
     }
-    int8_t *Gd = (int8_t *)slot_scratch(c, S, "Gd", SD * mgd, &rc); if (rc) return rc;
+    int8_t *Gd = (int8_t *)slot_scratch(c, S, "Gd", SH * mgd, &rc); if (rc) return rc;
     int *Ge = (int *)slot_scratch(c, S, "Ge", 4 * mge, &rc); if (rc) return rc;
     double *H = (double *)slot_scratch(c, S, "H", 8 * mh, &rc); if (rc) return rc;
     double *Ht = (double *)slot_scratch(c, S, "Ht", 8 * mht, &rc); if (rc) return rc;
@@ -944,6 +954,15 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
     long long *pc_offs = nullptr;
     if ((rc = setup_pipe(c, S, P, rounds, &pc_counts, &pc_offs)) != GI_OK) return rc;
 
+    // G digits four rows per thread when every record map is at most 4 wide, and
+    // one unit per workgroup with its record in LDS when every record also fits
+    bool g4 = true, glds = true;
+    for (const Chunk &ch : chunks) {
+        g4 = g4 && group_of(c, ch.jindI, ch.gI, ch.jindF).gW <= 4;
+        glds = glds && c->blocks[ch.jindI * c->nrep + ch.gI].nsize <= kRecLds;
+    }
+    glds = glds && g4;
+    const int grows = g4 ? 1024 : 256;
     size_t iround = 0;
     for (auto &rg : rounds) {
         RoundCapture cap;
@@ -952,7 +971,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
         std::vector<SplitHalfJob> hj;
         std::vector<CorrSplitArgs> cj;
         std::vector<int> gbegin, tbegin, cbegin;
-        int gt = 0, tt = 0, ct = 0;
+        int gt = 0, tt = 0, ct = 0, ubeg = 0;
         size_t ogd = 0, oge = 0, oh = 0, oht = 0, ohd = 0, ohc = 0;
         std::vector<size_t> Hoff, Htoff, Hdoff, Hcoff;
         for (size_t i = rg.first; i < rg.second; i++) {
@@ -975,7 +994,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                 g.GtotB = GP.GtotB;
                 g.EGtot = GP.EGtot;
                 g.nunits = ch.nunits;
-                g.gtiles = (GP.GtotB + 255) / 256;
+                g.gtiles = (GP.GtotB + grows - 1) / grows;
                 g.tile_begin = gt;
                 gt += g.gtiles * ((ch.nunits + kGUnits - 1) / kGUnits);
                 g.uptr = (const double *const *)(d_meta + ch.off_uptr);
@@ -985,8 +1004,10 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                 g.D = Gd + ogd;
                 g.plane = (long long)mgd;
                 g.E = Ge + oge;
+                g.nsize = BI.nsize;
                 gj.push_back(g);
-                gbegin.push_back(g.tile_begin);
+                gbegin.push_back(glds ? ubeg : g.tile_begin);
+                ubeg += ch.nunits;
                 for (const OutClass &oc : GP.oc) {
                     SplitHalfJob h;
                     h.cd = classdev(oc);
@@ -1022,8 +1043,12 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             if ((rc = send(d_gj, gj.data(), gj.size() * sizeof(GSplitJob), gbegin, gj_bytes)) != GI_OK)
                 return rc;
             TL_BEGIN("build_g");
-            launch_build_g_split(S.stream, SD, (const GSplitJob *)d_gj, (const int *)(d_gj + gj_bytes),
-                                 (int)gj.size(), gt);
+            if (glds)
+                launch_build_g_split_lds(S.stream, SH, (const GSplitJob *)d_gj,
+                                         (const int *)(d_gj + gj_bytes), (int)gj.size(), ubeg);
+            else
+                launch_build_g_split(S.stream, SH, (const GSplitJob *)d_gj,
+                                     (const int *)(d_gj + gj_bytes), (int)gj.size(), gt, g4);
             TL_END();
         }
         if (!hj.empty()) {
@@ -1031,7 +1056,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                 return rc;
             TL_BEGIN("half_ls_split");
             P.work[14] += tt;
-            launch_halfls_split(S.stream, SD, N, c->kpad_all, c->d_mud, c->muplane, c->d_mue,
+            launch_halfls_split(S.stream, SH, N, c->kpad_all, c->d_mud, c->muplane, c->d_mue,
                                 (const SplitHalfJob *)d_hj, (const int *)(d_hj + hj_bytes),
                                 (int)hj.size(), tt);
             TL_END();
@@ -1962,7 +1987,7 @@ int gi_serve_submit(gi_ctx *c, const gi_request *r, long long *ticket)
         return GI_EINVAL;  // |J' - J| > 1
     if (r->mode != GI_MODE_FP64 && r->mode != GI_MODE_FP64_BATCHED && r->mode != GI_MODE_SPLIT)
         return GI_EINVAL;
-    if (r->mode == GI_MODE_SPLIT && c->split_ready != c->split_S)
+    if (r->mode == GI_MODE_SPLIT && c->split_ready != c->split_key())
         return fail(c, GI_ESTATE, "split-mode request before gi_prepare_split");
     if (!(r->temperature > 0.0)) return GI_EINVAL;
     auto R = std::make_shared<Request>();

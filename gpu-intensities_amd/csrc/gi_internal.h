@@ -227,7 +227,10 @@ struct gi_ctx {
     int pipe_rounds = 4;
 
     // split mode: digits per operand (3..6) and the resident digit planes of mu
-    int split_S = 6, split_ready = 0;  // split_ready = digits the planes were built with
+    // digits of the correlation operands (eigenvector blocks, folded vectors) and
+    // of the half-ls operands (mu, G); split_ready = split_key() the planes were built with
+    int split_S = 6, split_Sh = 6, split_ready = 0;
+    int split_key() const { return split_S * 16 + split_Sh; }
     int8_t *d_mud = nullptr;
     long long muplane = 0;
     int *d_mue = nullptr;

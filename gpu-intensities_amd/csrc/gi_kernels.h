@@ -115,11 +115,19 @@ struct GMapJob {
     const double *const *uptr;
     const int *uideg;
     double *G;
+    int nsize;                       // record length of the lower block
 };
 constexpr int kGMapMaxW = 16;
 constexpr int kGUnits = 16;
+// Records up to this many doubles are staged in LDS by the one-unit-per-workgroup
+// G builds (launch_build_g_map_lds, launch_build_g_split_lds).
+constexpr int kRecLds = 16384;
 void launch_build_g_map(hipStream_t st, const GMapJob *d_jobs, const int *d_begin, int njobs,
                         int ntiles);
+// One workgroup per unit, its record in LDS (maps of width <= 4, records of <=
+// kRecLds doubles); d_begin = first unit of each job.
+void launch_build_g_map_lds(hipStream_t st, const GMapJob *d_jobs, const int *d_begin,
+                            int njobs, int nunits);
 void launch_build_g_units(hipStream_t st, int N, const GBuildJob *d_jobs, const int *d_begin,
                           int njobs, int ntiles);
 
@@ -136,6 +144,7 @@ struct GSplitJob {
     int8_t *D;                       // this chunk's G digits (plane 0)
     long long plane;
     int *E;                          // this chunk's column exponents [unit][EGtot]
+    int nsize;                       // record length of the lower block
 };
 struct SplitHalfJob {
     ClassDev cd;
@@ -167,8 +176,14 @@ void launch_split_rows(hipStream_t st, int S, const double *X, long long ld, int
                        int Kp, int8_t *D, long long plane, int *E, double *Mx);
 void launch_split_mu(hipStream_t st, int S, int N, int kpad_all, const double *mu,
                      const int *vinv, int8_t *D, long long plane, int *E);
+// four: every job's record map has width <= 4 and gtiles counts 1024 rows
+// (k_build_g_split4); otherwise 256 rows per tile.
 void launch_build_g_split(hipStream_t st, int S, const GSplitJob *d_jobs, const int *d_begin,
-                          int njobs, int ntiles);
+                          int njobs, int ntiles, bool four);
+// One workgroup per unit: its record in LDS, then every G row of the unit
+// (maps of width <= 4, records of <= kRecLds doubles); d_begin = first unit of each job.
+void launch_build_g_split_lds(hipStream_t st, int S, const GSplitJob *d_jobs,
+                              const int *d_begin, int njobs, int nunits);
 void launch_halfls_split(hipStream_t st, int S, int N, int kpad_all, const int8_t *mud,
                          long long muplane, const int *mue, const SplitHalfJob *d_jobs,
                          const int *d_tbegin, int njobs, int ntiles);

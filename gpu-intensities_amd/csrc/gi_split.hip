@@ -216,6 +216,119 @@ __global__ __launch_bounds__(256) void k_build_g_split(const GSplitJob *__restri
     }
 }
 
+// The same for record maps of width <= 4, four consecutive rows of one column
+// per thread (columns are padded to 64 rows): vector loads of the map and one
+// 4-byte store per digit plane instead of four byte stores.
+template <int S>
+__global__ __launch_bounds__(256) void k_build_g_split4(const GSplitJob *__restrict__ jobs,
+                                                        const int *__restrict__ begin, int njobs,
+                                                        int ntiles)
+{
+    const int tile = xcd_tile_s(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
+    const GSplitJob &J = jobs[find_tile_job(begin, njobs, tile)];
+    const int local = tile - J.tile_begin;
+    const int ug = local / J.gtiles, gt = local - ug * J.gtiles;
+    const int g = (gt * 256 + (int)threadIdx.x) * 4;
+    if (g >= J.GtotB) return;
+    int co = 0;
+    while (co + 1 < J.nclasses && J.classes[co + 1].cbaseB <= g) co++;
+    const ClassDev &cd = J.classes[co];
+    const int j = (g - cd.cbaseB) / cd.kpad;
+    const bool first = (g - cd.cbaseB) - j * cd.kpad == 0;
+    int4 ix[4];
+    double wv[4][4];
+    double bnd = 0.0;
+    int cur = -1;
+    const int u0 = ug * kGUnits, u1 = min(u0 + kGUnits, J.nunits);
+    for (int u = u0; u < u1; u++) {
+        const int d = J.uideg[u];
+        if (d != cur) {
+#pragma unroll
+            for (int w = 0; w < 4; w++)
+                if (w < J.W) {
+                    const size_t o = (size_t)w * J.GtotB + g;
+                    ix[w] = *(const int4 *)(J.idx[d] + o);
+                    const double2 a = *(const double2 *)(J.wt[d] + o);
+                    const double2 b = *(const double2 *)(J.wt[d] + o + 2);
+                    wv[w][0] = a.x; wv[w][1] = a.y; wv[w][2] = b.x; wv[w][3] = b.y;
+                }
+            bnd = J.bnd[d][cd.eoff + j];
+            cur = d;
+        }
+        const double *rec = J.uptr[u];
+        const int e = digit_exp(bnd * J.umax[J.urec[u]]);
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int w = 0; w < 4; w++)
+            if (w < J.W) {
+                v[0] += wv[w][0] * rec[ix[w].x];
+                v[1] += wv[w][1] * rec[ix[w].y];
+                v[2] += wv[w][2] * rec[ix[w].z];
+                v[3] += wv[w][3] * rec[ix[w].w];
+            }
+        int dg[4][S];
+#pragma unroll
+        for (int r = 0; r < 4; r++) to_digits<S>(v[r], e, dg[r]);
+        int8_t *dst = J.D + (size_t)u * J.GtotB + g;
+#pragma unroll
+        for (int q = 0; q < S; q++)
+            *(int *)(dst + q * J.plane) = (dg[0][q] & 255) | ((dg[1][q] & 255) << 8) |
+                                          ((dg[2][q] & 255) << 16) | ((dg[3][q] & 255) << 24);
+        if (first) J.E[(size_t)u * J.EGtot + cd.eoff + j] = e;
+    }
+}
+
+// G digits of one unit per workgroup: the unit's record is read once, with
+// coalesced loads, into LDS, and the gathers of the record map read it there
+// (they land on scattered cache lines when they read global memory); four
+// consecutive rows of one column per thread.
+template <int S>
+__global__ __launch_bounds__(1024) void k_build_g_split_lds(const GSplitJob *__restrict__ jobs,
+                                                            const int *__restrict__ ubegin,
+                                                            int njobs)
+{
+    __shared__ double rec[kRecLds];
+    const int ji = find_tile_job(ubegin, njobs, blockIdx.x);
+    const GSplitJob &J = jobs[ji];
+    const int u = blockIdx.x - ubegin[ji];
+    const double *src = J.uptr[u];
+    for (int i = threadIdx.x; i < J.nsize; i += 1024) rec[i] = src[i];
+    __syncthreads();
+    const int d = J.uideg[u];
+    const double um = J.umax[J.urec[u]];
+    int8_t *D = J.D + (size_t)u * J.GtotB;
+    int co = 0;
+    for (int g = threadIdx.x * 4; g < J.GtotB; g += 4096) {
+        while (co + 1 < J.nclasses && J.classes[co + 1].cbaseB <= g) co++;
+        const ClassDev &cd = J.classes[co];
+        const int j = (g - cd.cbaseB) / cd.kpad;
+        const bool first = (g - cd.cbaseB) - j * cd.kpad == 0;
+        const int e = digit_exp(J.bnd[d][cd.eoff + j] * um);
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int w = 0; w < 4; w++)
+            if (w < J.W) {
+                const size_t o = (size_t)w * J.GtotB + g;
+                const int4 ix = *(const int4 *)(J.idx[d] + o);
+                const double2 a = *(const double2 *)(J.wt[d] + o);
+                const double2 b = *(const double2 *)(J.wt[d] + o + 2);
+                v[0] += a.x * rec[ix.x];
+                v[1] += a.y * rec[ix.y];
+                v[2] += b.x * rec[ix.z];
+                v[3] += b.y * rec[ix.w];
+            }
+        int dg[4][S];
+#pragma unroll
+        for (int r = 0; r < 4; r++) to_digits<S>(v[r], e, dg[r]);
+#pragma unroll
+        for (int q = 0; q < S; q++)
+            *(int *)(D + q * J.plane + g) = (dg[0][q] & 255) | ((dg[1][q] & 255) << 8) |
+                                            ((dg[2][q] & 255) << 16) | ((dg[3][q] & 255) << 24);
+        if (first) J.E[(size_t)u * J.EGtot + cd.eoff + j] = e;
+    }
+}
+
 // One K step of 64 for a 64 x 64 tile: the wave's 2 x 2 blocks, digit pairs
 // i + j < S into level accumulators.
 template <int S>
@@ -460,10 +573,25 @@ void launch_split_mu(hipStream_t st, int S, int N, int kpad_all, const double *m
 }
 
 void launch_build_g_split(hipStream_t st, int S, const GSplitJob *d_jobs, const int *d_begin,
-                          int njobs, int ntiles)
+                          int njobs, int ntiles, bool four)
 {
     if (njobs <= 0 || ntiles <= 0) return;
+    if (four) {
+#define L_(s) hipLaunchKernelGGL(k_build_g_split4<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, d_jobs, d_begin, njobs, ntiles)
+        GI_SPLIT_DISPATCH(S, L_)
+#undef L_
+        return;
+    }
 #define L_(s) hipLaunchKernelGGL(k_build_g_split<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, d_jobs, d_begin, njobs, ntiles)
+    GI_SPLIT_DISPATCH(S, L_)
+#undef L_
+}
+
+void launch_build_g_split_lds(hipStream_t st, int S, const GSplitJob *d_jobs,
+                              const int *d_begin, int njobs, int nunits)
+{
+    if (njobs <= 0 || nunits <= 0) return;
+#define L_(s) hipLaunchKernelGGL(k_build_g_split_lds<s>, dim3(nunits), dim3(1024), 0, st, d_jobs, d_begin, njobs)
     GI_SPLIT_DISPATCH(S, L_)
 #undef L_
 }

@@ -211,6 +211,9 @@ int gi_work_stats(gi_ctx *ctx, double *out, int n);
  * planes of mu and of every eigenvector block (digits bytes per element);
  * gi_compute does it on first use, the request server needs it beforehand. */
 int gi_set_split_digits(gi_ctx *ctx, int digits);
+/* The same with a digit count per product: the half linestrength (mu and G)
+ * and the correlation (eigenvector blocks and folded vectors), each 3..6. */
+int gi_set_split_digits_each(gi_ctx *ctx, int halfls_digits, int corr_digits);
 int gi_prepare_split(gi_ctx *ctx);
 
 /* Tile shapes of the two matrix-core kernels of GI_MODE_FP64_BATCHED (no
