@@ -317,16 +317,36 @@ def serve_sweep(eng, job, binding, lambdas, seconds, mode, seed=7, budget=8 << 3
     return res
 
 
-def time_modes(eng, binding, bufs, mode_names, steps, warmup, tune=True, sweep=True):
+def read_tuning(path):
+    """Per-mode settings from a saved gi_autotune table: the "best ..." line
+    after each "## mode <name>" header."""
+    out, mode = {}, None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("## mode "):
+                mode = line.split()[2]
+            elif line.startswith("best ") and mode:
+                kv = dict(x.split("=") for x in line.split()[1:])
+                out[mode] = (int(kv["halfls_tile"]), int(kv["corr_tile"]), int(kv["pipe_rounds"]))
+    return out
+
+
+def time_modes(eng, binding, bufs, mode_names, steps, warmup, tune=True, sweep=True,
+               saved=None):
     """Time `steps` passes of gi_compute over every lower level per mode; with
     `tune` the matrix-core modes are first tuned on this workload by
-    gi_autotune, with `sweep` the captured sweep is timed beside them."""
+    gi_autotune (or, with `saved`, set from a saved table), with `sweep` the
+    captured sweep is timed beside them."""
     import torch
     modes = {}
     for mname in mode_names:
         mode = binding.MODES[mname]
         tuning = None
-        if mode != binding.GI_MODE_FP64 and tune:
+        if mode != binding.GI_MODE_FP64 and saved and mname in saved:
+            eng.set_tuning(*saved[mname])
+            tuning = dict(eng.get_tuning(), source="saved table")
+            log(f"{mname} tuning from the saved table: {tuning}")
+        elif mode != binding.GI_MODE_FP64 and tune:
             t = time.time()
             best, table = eng.autotune(mode, bufs=bufs, reps=2)
             tuning = dict(best, seconds=round(time.time() - t, 2),
@@ -536,6 +556,8 @@ def main():
                     help="offered duration of each serving rate")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device copy (PMC calibration)")
+    ap.add_argument("--tuning", default=None,
+                    help="saved gi_autotune table for the headline workload (instead of tuning)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1"}))
@@ -577,7 +599,8 @@ def main():
         eng.prepare_split()  # resident digit planes, built once like the uploads
         torch.cuda.synchronize()
         log(f"split digit planes {time.time() - t:.1f}s")
-    modes = time_modes(eng, binding, bufs, mode_names, args.steps, args.warmup)
+    saved = read_tuning(args.tuning) if args.tuning else None
+    modes = time_modes(eng, binding, bufs, mode_names, args.steps, args.warmup, saved=saved)
     best = max(modes, key=lambda m: modes[m]["value"])
     ms_per_step = modes[best]["ms_per_step"]
     nlines = modes[best]["transitions"]
