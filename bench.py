@@ -327,7 +327,8 @@ def read_tuning(path):
                 mode = line.split()[2]
             elif line.startswith("best ") and mode:
                 kv = dict(x.split("=") for x in line.split()[1:])
-                out[mode] = (int(kv["halfls_tile"]), int(kv["corr_tile"]), int(kv["pipe_rounds"]))
+                out[mode] = (int(kv["halfls_tile"]), int(kv["corr_tile"]), int(kv["pipe_rounds"]),
+                             int(kv.get("split_tile", 0)))
     return out
 
 

@@ -67,7 +67,7 @@ class GiRequest(C.Structure):
 
 
 class GiTuning(C.Structure):
-    _fields_ = [("halfls_cfg", ci), ("corr_cfg", ci), ("pipe_rounds", ci)]
+    _fields_ = [("halfls_cfg", ci), ("corr_cfg", ci), ("pipe_rounds", ci), ("split_cfg", ci)]
 
 
 class GiLines(C.Structure):
@@ -311,14 +311,16 @@ class Engine:
         self._chk(lib().gi_get_tiles(self.h, C.byref(a), C.byref(b)), "gi_get_tiles")
         return a.value, b.value
 
-    def set_tuning(self, halfls_cfg, corr_cfg, pipe_rounds):
-        t = GiTuning(halfls_cfg=halfls_cfg, corr_cfg=corr_cfg, pipe_rounds=pipe_rounds)
+    def set_tuning(self, halfls_cfg, corr_cfg, pipe_rounds, split_cfg=0):
+        t = GiTuning(halfls_cfg=halfls_cfg, corr_cfg=corr_cfg, pipe_rounds=pipe_rounds,
+                     split_cfg=split_cfg)
         self._chk(lib().gi_set_tuning(self.h, C.byref(t)), "gi_set_tuning")
 
     def get_tuning(self):
         t = GiTuning()
         self._chk(lib().gi_get_tuning(self.h, C.byref(t)), "gi_get_tuning")
-        return dict(halfls_cfg=t.halfls_cfg, corr_cfg=t.corr_cfg, pipe_rounds=t.pipe_rounds)
+        return dict(halfls_cfg=t.halfls_cfg, corr_cfg=t.corr_cfg, pipe_rounds=t.pipe_rounds,
+                    split_cfg=t.split_cfg)
 
     def autotune(self, mode, lower_ids=None, bufs=None, reps=2):
         """gi_autotune on these lower levels; returns (chosen setting, table text)."""
@@ -338,7 +340,8 @@ class Engine:
                                C.byref(s), reps, C.byref(best), buf, len(buf), C.byref(n))
         self._chk(rc, "gi_autotune")
         return (dict(halfls_cfg=best.halfls_cfg, corr_cfg=best.corr_cfg,
-                     pipe_rounds=best.pipe_rounds), buf.value.decode())
+                     pipe_rounds=best.pipe_rounds, split_cfg=best.split_cfg),
+                buf.value.decode())
 
     # -- captured sweeps ---------------------------------------------------
     def sweep_create(self, mode, lower_ids=None):

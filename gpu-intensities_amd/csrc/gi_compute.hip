@@ -963,6 +963,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
     }
     glds = glds && g4;
     const int grows = g4 ? 1024 : 256;
+    const int srows_tile = c->tile_split ? 128 : 64;  // k_halfls_split(_wide) row tile
     size_t iround = 0;
     for (auto &rg : rounds) {
         RoundCapture cap;
@@ -1013,7 +1014,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                     h.cd = classdev(oc);
                     h.nunits = ch.nunits;
                     h.tile_begin = tt;
-                    h.mt = (oc.n + 63) / 64;
+                    h.mt = (oc.n + srows_tile - 1) / srows_tile;
                     tt += h.mt * ((ch.nunits * (int)oc.cols.size() + 63) / 64);
                     h.GtotB = GP.GtotB;
                     h.EGtot = GP.EGtot;
@@ -1058,7 +1059,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             P.work[14] += tt;
             launch_halfls_split(S.stream, SH, N, c->kpad_all, c->d_mud, c->muplane, c->d_mue,
                                 (const SplitHalfJob *)d_hj, (const int *)(d_hj + hj_bytes),
-                                (int)hj.size(), tt);
+                                (int)hj.size(), tt, c->tile_split != 0);
             TL_END();
         }
         std::vector<FoldJob> fjobs;
@@ -1462,7 +1463,7 @@ int autotune(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out, int
     if (mode != GI_MODE_FP64 && mode != GI_MODE_FP64_BATCHED && mode != GI_MODE_SPLIT)
         return fail(c, GI_EINVAL, "gi_autotune: unknown mode");
     if (c->server) return fail(c, GI_ESTATE, "gi_autotune while the request server runs");
-    const gi_tuning saved{c->tile_halfls, c->tile_corr, c->pipe_rounds};
+    const gi_tuning saved{c->tile_halfls, c->tile_corr, c->pipe_rounds, c->tile_split};
     std::string txt;
     char line[256];
     long long ncand = 0;
@@ -1518,6 +1519,13 @@ int autotune(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out, int
     static const int tiles[4] = {0, 1, 2, 3};
     static const char *const tname[4] = {"64x64", "64x128", "128x64", "128x128"};
     static const int rounds[6] = {1, 2, 3, 4, 6, 8};
+    static const int stiles[2] = {0, 1};
+    static const char *const sname[2] = {"64x64", "128x64"};
+    if (mode == GI_MODE_SPLIT &&
+        (rc = scan("split_halfls_tile", "half_ls_split", &c->tile_split, stiles, 2, sname))) {
+        c->tile_split = saved.split_cfg;
+        return rc;
+    }
     if (mode == GI_MODE_FP64_BATCHED) {
         if ((rc = scan("halfls_tile", "half_ls_mfma", &c->tile_halfls, tiles, 4, tname)) ||
             (rc = scan("corr_tile", "corr_mfma", &c->tile_corr, tiles, 4, tname))) {
@@ -1528,14 +1536,15 @@ int autotune(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out, int
     if (mode != GI_MODE_FP64 && out->capacity >= ncand) {  // pipelined output only
         if ((rc = scan("pipe_rounds", nullptr, &c->pipe_rounds, rounds, 6, nullptr))) {
             c->tile_halfls = saved.halfls_cfg; c->tile_corr = saved.corr_cfg;
-            c->pipe_rounds = saved.pipe_rounds;
+            c->pipe_rounds = saved.pipe_rounds; c->tile_split = saved.split_cfg;
             return rc;
         }
     }
-    std::snprintf(line, sizeof line, "best halfls_tile=%d corr_tile=%d pipe_rounds=%d\n",
-                  c->tile_halfls, c->tile_corr, c->pipe_rounds);
+    std::snprintf(line, sizeof line,
+                  "best halfls_tile=%d corr_tile=%d pipe_rounds=%d split_tile=%d\n",
+                  c->tile_halfls, c->tile_corr, c->pipe_rounds, c->tile_split);
     txt += line;
-    if (best) *best = gi_tuning{c->tile_halfls, c->tile_corr, c->pipe_rounds};
+    if (best) *best = gi_tuning{c->tile_halfls, c->tile_corr, c->pipe_rounds, c->tile_split};
     if (len) *len = (long long)txt.size();
     if (table && cap > 0) {
         const size_t n = std::min<size_t>((size_t)cap - 1, txt.size());
@@ -1899,11 +1908,13 @@ int gi_set_tuning(gi_ctx *c, const gi_tuning *t)
     if (!c || !t) return GI_EINVAL;
     if (c->server) return fail(c, GI_ESTATE, "gi_set_tuning while the request server runs");
     if (t->halfls_cfg < 0 || t->halfls_cfg > 3 || t->corr_cfg < 0 || t->corr_cfg > 3 ||
-        t->pipe_rounds < 1 || t->pipe_rounds > kMaxPipeRounds)
+        t->pipe_rounds < 1 || t->pipe_rounds > kMaxPipeRounds || t->split_cfg < 0 ||
+        t->split_cfg > 1)
         return fail(c, GI_EINVAL, "gi_set_tuning: value out of range");
     c->tile_halfls = t->halfls_cfg;
     c->tile_corr = t->corr_cfg;
     c->pipe_rounds = t->pipe_rounds;
+    c->tile_split = t->split_cfg;
     return GI_OK;
 }
 
@@ -1913,6 +1924,7 @@ int gi_get_tuning(gi_ctx *c, gi_tuning *t)
     t->halfls_cfg = c->tile_halfls;
     t->corr_cfg = c->tile_corr;
     t->pipe_rounds = c->pipe_rounds;
+    t->split_cfg = c->tile_split;
     return GI_OK;
 }
 

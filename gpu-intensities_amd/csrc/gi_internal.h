@@ -225,6 +225,7 @@ struct gi_ctx {
     // the rounds of a pipelined gi_compute (gi_set_tuning, gi_autotune)
     int tile_halfls = 3, tile_corr = 0;
     int pipe_rounds = 4;
+    int tile_split = 0;              // k_halfls_split rows: 0 = 64, 1 = 128
 
     // split mode: digits per operand (3..6) and the resident digit planes of mu
     // digits of the correlation operands (eigenvector blocks, folded vectors) and

@@ -184,9 +184,10 @@ void launch_build_g_split(hipStream_t st, int S, const GSplitJob *d_jobs, const 
 // (maps of width <= 4, records of <= kRecLds doubles); d_begin = first unit of each job.
 void launch_build_g_split_lds(hipStream_t st, int S, const GSplitJob *d_jobs,
                               const int *d_begin, int njobs, int nunits);
+// wide: 128-row tiles (SplitHalfJob.mt counts 128-row tiles), else 64-row.
 void launch_halfls_split(hipStream_t st, int S, int N, int kpad_all, const int8_t *mud,
                          long long muplane, const int *mue, const SplitHalfJob *d_jobs,
-                         const int *d_tbegin, int njobs, int ntiles);
+                         const int *d_tbegin, int njobs, int ntiles, bool wide);
 void launch_corr_split(hipStream_t st, int S, const CorrSplitArgs *d_args, const int *d_cbegin,
                        int nchunks, int ntiles);
 

@@ -451,6 +451,121 @@ __global__ __launch_bounds__(256) void k_halfls_split(int N, int kpad_all,
             }
 }
 
+// One K step of 64 for a 128 x 64 tile: the wave's 4 x 2 blocks (64 rows x
+// 32 columns), so each LDS operand read feeds twice as many products as in
+// oz_step; A rows of a plane are 128 apart in LDS.
+template <int S>
+__device__ inline void oz_step_wide(const int8_t *__restrict__ As,
+                                    const int8_t *__restrict__ Bs, int wm, int wn, int lane,
+                                    v4i (&acc)[4][2][S])
+{
+    const int i16 = lane & 15, k16 = (lane >> 4) * 16;
+    v4i b[S][2];
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+        b[j][0] = *(const v4i *)(Bs + (j * 64 + wn + i16) * 80 + k16);
+        b[j][1] = *(const v4i *)(Bs + (j * 64 + wn + 16 + i16) * 80 + k16);
+    }
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+        v4i a[4];
+#pragma unroll
+        for (int x = 0; x < 4; x++)
+            a[x] = *(const v4i *)(As + (i * 128 + wm + 16 * x + i16) * 80 + k16);
+#pragma unroll
+        for (int j = 0; j < S - i; j++)
+#pragma unroll
+            for (int x = 0; x < 4; x++) {
+                acc[x][0][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[x], b[j][0], acc[x][0][i + j], 0, 0, 0);
+                acc[x][1][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[x], b[j][1], acc[x][1][i + j], 0, 0, 0);
+            }
+    }
+}
+
+// k_halfls_split with 128-row tiles (4 wavefronts of 64 x 32): each G digit
+// column is read by half as many tiles. The level sums are exact integers,
+// so the result is bitwise that of the 64-row kernel.
+template <int S>
+__global__ __launch_bounds__(256, 1) void k_halfls_split_wide(
+    int N, int kpad_all, const int8_t *__restrict__ mud, long long muplane,
+    const int *__restrict__ mue, const SplitHalfJob *__restrict__ jobs,
+    const int *__restrict__ tbegin, int njobs, int ntiles)
+{
+    __shared__ __attribute__((aligned(16))) int8_t As[S * 128 * 80];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[S * 64 * 80];
+    const int tile = xcd_tile_s(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
+    const SplitHalfJob &J = jobs[find_tile_job(tbegin, njobs, tile)];
+    const ClassDev &cd = J.cd;
+    const int local = tile - J.tile_begin;
+    const int m0 = (local % J.mt) * 128, n0 = (local / J.mt) * 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 32;
+    const int ncolsTot = J.nunits * cd.ncols;
+    const int lr = tid >> 2, lp = (tid & 3) * 16;
+    const bool mok0 = m0 + lr < cd.n, mok1 = m0 + 64 + lr < cd.n;
+    const int n = n0 + lr;
+    const bool nok = n < ncolsTot;
+    long long bcol = 0;
+    if (nok) {
+        const int u = n / cd.ncols, jj = n - u * cd.ncols;
+        bcol = (long long)u * J.GtotB + cd.cbaseB + (long long)jj * cd.kpad;
+    }
+    v4i acc[4][2][S];
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int l = 0; l < S; l++) acc[x][y][l] = v4i{0, 0, 0, 0};
+    v4i ra0[S], ra1[S], rb[S];
+    auto fetch = [&](int se, int kk) {
+        const long long arow = ((long long)cd.seg_a[se] * N + cd.off + m0 + lr) * kpad_all +
+                               cd.seg_pad[2 * se + 1] + kk + lp;
+        const long long bofs = bcol + cd.seg_pad[2 * se] + kk + lp;
+#pragma unroll
+        for (int q = 0; q < S; q++) {
+            ra0[q] = mok0 ? *(const v4i *)(mud + q * muplane + arow) : v4i{0, 0, 0, 0};
+            ra1[q] = mok1 ? *(const v4i *)(mud + q * muplane + arow + 64LL * kpad_all)
+                          : v4i{0, 0, 0, 0};
+            rb[q] = nok ? *(const v4i *)(J.Gd + q * J.gplane + bofs) : v4i{0, 0, 0, 0};
+        }
+    };
+    int e = 0, kk = 0;
+    if (cd.nseg > 0) fetch(0, 0);
+    while (e < cd.nseg) {
+#pragma unroll
+        for (int q = 0; q < S; q++) {
+            *(v4i *)(As + (q * 128 + lr) * 80 + lp) = ra0[q];
+            *(v4i *)(As + (q * 128 + 64 + lr) * 80 + lp) = ra1[q];
+            *(v4i *)(Bs + (q * 64 + lr) * 80 + lp) = rb[q];
+        }
+        __syncthreads();
+        kk += 64;
+        if (kk >= cd.seg_n[e]) {
+            e++;
+            kk = 0;
+        }
+        if (e < cd.nseg) fetch(e, kk);
+        oz_step_wide<S>(As, Bs, wm, wn, lane, acc);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wm + 16 * x + 4 * (lane >> 4) + r;
+                const int nn = n0 + wn + 16 * y + (lane & 15);
+                if (m >= cd.n || nn >= ncolsTot) continue;
+                const int u = nn / cd.ncols, jj = nn - u * cd.ncols;
+                const int ex = mue[cd.off + m] + J.Ge[(size_t)u * J.EGtot + cd.eoff + jj];
+                J.H[(size_t)u * J.Htot + cd.hbase + (size_t)jj * cd.n + m] =
+                    ldexp(oz_value<S>(acc[x][y], r), ex);
+            }
+}
+
 // Split banded correlation (the digit form of k_corr_mfma): rows = upper
 // records (digits of the resident block), columns = folded vectors (digits).
 template <int S>
@@ -598,9 +713,15 @@ void launch_build_g_split_lds(hipStream_t st, int S, const GSplitJob *d_jobs,
 
 void launch_halfls_split(hipStream_t st, int S, int N, int kpad_all, const int8_t *mud,
                          long long muplane, const int *mue, const SplitHalfJob *d_jobs,
-                         const int *d_tbegin, int njobs, int ntiles)
+                         const int *d_tbegin, int njobs, int ntiles, bool wide)
 {
     if (njobs <= 0 || ntiles <= 0) return;
+    if (wide) {
+#define L_(s) hipLaunchKernelGGL(k_halfls_split_wide<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, N, kpad_all, mud, muplane, mue, d_jobs, d_tbegin, njobs, ntiles)
+        GI_SPLIT_DISPATCH(S, L_)
+#undef L_
+        return;
+    }
 #define L_(s) hipLaunchKernelGGL(k_halfls_split<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, N, kpad_all, mud, muplane, mue, d_jobs, d_tbegin, njobs, ntiles)
     GI_SPLIT_DISPATCH(S, L_)
 #undef L_

@@ -235,6 +235,7 @@ typedef struct gi_tuning {
     int halfls_cfg;
     int corr_cfg;
     int pipe_rounds;
+    int split_cfg;    /* GI_MODE_SPLIT half-ls tiles: 0 = 64 x 64, 1 = 128 x 64 */
 } gi_tuning;
 int gi_set_tuning(gi_ctx *ctx, const gi_tuning *t);
 int gi_get_tuning(gi_ctx *ctx, gi_tuning *t);
@@ -242,7 +243,8 @@ int gi_get_tuning(gi_ctx *ctx, gi_tuning *t);
 /* Autotuner (no reference counterpart): runs gi_compute in `mode` over the
  * given lower levels (NULL = all) `reps` times per setting into `out`,
  * and keeps the fastest setting of each knob: the two tile shapes (FP64
- * batched mode, judged by their kernel's time) and the pipeline round count
+ * batched mode) or the split half-ls tile shape (split mode), each judged by
+ * its kernel's time, and the pipeline round count
  * (both matrix-core modes when `out` has room for every candidate, judged by
  * the call's wall time). Settings are scanned in a fixed order starting from
  * the current one, and a setting replaces the incumbent only when it is more

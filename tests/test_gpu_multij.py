@@ -124,7 +124,8 @@ def test_requests_across_the_list(engine, c2v_list):
     lv = c2v_list.levels
     cases = [(0, 0, 1), (2, 1, 1), (2, 3, 3), (3, 2, 2), (4, 0, 3), (4, 2, 4)]
     want = {}
-    for mode in (binding.GI_MODE_FP64, binding.GI_MODE_FP64_BATCHED):
+    engine.prepare_split()
+    for mode in ALL_MODES:
         for jI, g, jF in cases:
             ids = np.nonzero((lv["jind"] == jI) & (lv["igamma"] == g))[0]
             r = engine.compute(ids, mode=mode)

@@ -103,6 +103,34 @@ def test_split_digits_trade_accuracy(engine):
     assert errs[6] <= TOL_S
 
 
+def test_split_digits_per_product(engine):
+    """gi_set_split_digits_each: (6, 6) is the default bit for bit; 5 digits in
+    one product lands between the 5-digit and 6-digit errors of both."""
+    job = synth.make_job("C2V", nvib=80, jvals=(1, 2), seed=9)
+    engine.load(job)
+    ref = oracle_py.OracleJob(job).run()
+    try:
+        base = {k: v.copy() for k, v in engine.compute(mode=binding.GI_MODE_SPLIT).items()}
+        engine.set_split_digits_each(6, 6)
+        same = engine.compute(mode=binding.GI_MODE_SPLIT)
+        for k in base:
+            assert np.array_equal(base[k], same[k]), k
+        e66 = split_error(base, ref)["max"]
+        engine.set_split_digits_each(5, 6)
+        e56 = split_error(engine.compute(mode=binding.GI_MODE_SPLIT), ref)["max"]
+        engine.set_split_digits_each(6, 5)
+        e65 = split_error(engine.compute(mode=binding.GI_MODE_SPLIT), ref)["max"]
+        engine.set_split_digits(5)
+        e55 = split_error(engine.compute(mode=binding.GI_MODE_SPLIT), ref)["max"]
+    finally:
+        engine.set_split_digits(6)
+    print(f"split max rel S error (halfls, corr digits): 66 {e66:.2e} 56 {e56:.2e} "
+          f"65 {e65:.2e} 55 {e55:.2e}")
+    assert e66 < min(e56, e65) and max(e56, e65) <= 4 * e55
+    with pytest.raises(binding.GIError):
+        engine.set_split_digits_each(7, 6)
+
+
 def test_split_requests_alone_and_mixed(engine):
     job = synth.make_job("C2V", nvib=120, jvals=(1, 2), seed=21, e_span=9000.0)
     engine.load(job)

@@ -13,7 +13,7 @@ from test_gpu_parity import compare
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = dict(halfls_cfg=3, corr_cfg=0, pipe_rounds=4)
+DEFAULT = dict(halfls_cfg=3, corr_cfg=0, pipe_rounds=4, split_cfg=0)
 
 
 @pytest.fixture
@@ -39,10 +39,11 @@ def test_autotune_picks_a_setting_and_keeps_the_bits(tuned_engine, mode):
     assert lines[0].startswith("# gi_autotune") and lines[-1].startswith("best ")
     kinds = [ln.split()[0] for ln in lines[1:-1]]
     want = (["halfls_tile"] * 4 + ["corr_tile"] * 4 if mode == binding.GI_MODE_FP64_BATCHED
-            else []) + ["pipe_rounds"] * 6
+            else ["split_halfls_tile"] * 2) + ["pipe_rounds"] * 6
     assert kinds == want
-    assert lines[-1] == ("best halfls_tile=%d corr_tile=%d pipe_rounds=%d"
-                         % (best["halfls_cfg"], best["corr_cfg"], best["pipe_rounds"]))
+    assert lines[-1] == ("best halfls_tile=%d corr_tile=%d pipe_rounds=%d split_tile=%d"
+                         % (best["halfls_cfg"], best["corr_cfg"], best["pipe_rounds"],
+                            best["split_cfg"]))
     after = _copy(eng.compute(mode=mode))
     for k in before:
         assert np.array_equal(before[k], after[k]), k
@@ -70,6 +71,13 @@ def test_every_round_count_gives_the_same_bits(tuned_engine, monkeypatch):
         monkeypatch.delenv("GI_BATCH_BYTES")
         for k in ref:
             assert np.array_equal(ref[k], got[k]), (mode, "small rounds", k)
+    # the split half-ls tile shapes: exact integer level sums, so the same bits
+    eng.set_tuning(3, 0, 2, 0)
+    ref = _copy(eng.compute(mode=binding.GI_MODE_SPLIT))
+    eng.set_tuning(3, 0, 2, 1)
+    got = eng.compute(mode=binding.GI_MODE_SPLIT)
+    for k in ref:
+        assert np.array_equal(ref[k], got[k]), ("split tile", k)
 
 
 def test_autotune_plain_mode_has_nothing_to_tune(tuned_engine):
@@ -85,12 +93,12 @@ def test_set_tuning_rules(tuned_engine):
     eng = tuned_engine
     job = synth.make_job("C2V", nvib=16, jvals=(0, 1), seed=54)
     eng.load(job)
-    for bad in ((4, 0, 2), (0, -1, 2), (0, 0, 0), (0, 0, 65)):
+    for bad in ((4, 0, 2, 0), (0, -1, 2, 0), (0, 0, 0, 0), (0, 0, 65, 0), (0, 0, 2, 2)):
         with pytest.raises(binding.GIError):
             eng.set_tuning(*bad)
     assert eng.get_tuning() == DEFAULT
-    eng.set_tuning(1, 2, 5)
-    assert eng.get_tuning() == dict(halfls_cfg=1, corr_cfg=2, pipe_rounds=5)
+    eng.set_tuning(1, 2, 5, 1)
+    assert eng.get_tuning() == dict(halfls_cfg=1, corr_cfg=2, pipe_rounds=5, split_cfg=1)
     assert eng.get_tiles() == (1, 2)
     eng.serve_start(1 << 28)
     try:
