@@ -317,6 +317,35 @@ def serve_sweep(eng, job, binding, lambdas, seconds, mode, seed=7, budget=8 << 3
     return res
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def copy_peak(torch):
+    """Measured device copy bandwidth: 1 GiB read + 1 GiB written per copy,
+    timed with HIP events over 5 copies after one warm-up."""
+    a = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    gbps = 5 * 2 * a.numel() * 8 / (e0.elapsed_time(e1) / 1e3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return gbps
+
+
 def read_tuning(path):
     """Per-mode settings from a saved gi_autotune table: the "best ..." line
     after each "## mode <name>" header."""
@@ -610,6 +639,7 @@ def main():
     work = modes[best]["work"]
     dom = max(kstats, key=lambda k: kstats[k]["total_ms"])
     roof = roofline_entry(dom, kstats, work, job.nvib)
+    roof["measured_copy_GBps"] = round(copy_peak(torch), 1)
     roofs = mode_rooflines(modes, job.nvib)
     kern = modes[best]["kernels"]
     for m in modes.values():
@@ -632,7 +662,24 @@ def main():
         cpu = dict(value=len(ref["S"]) / tc, unit="transitions/s", cores=nthr, kind="port",
                    sample=f"every {args.cpu_stride}th lower level by energy "
                           f"({len(sample)} of {len(lower_ok)}), {len(ref['S'])} transitions, "
-                          f"{tc:.1f} s, FP64 oracle (K-masked loops, OpenMP over lower levels)")
+                          f"{tc:.1f} s, FP64 oracle (K-masked loops, OpenMP over lower levels)",
+                   cpu_model=cpu_model(), nproc=os.cpu_count())
+        # one thread on every 20th level of the sample
+        one = sample[::20]
+        t = time.perf_counter()
+        r1 = oj.run(one, nthreads=1)
+        t1 = time.perf_counter() - t
+        cpu["one_thread"] = dict(value=len(r1["S"]) / t1, lower_levels=int(len(one)),
+                                 transitions=int(len(r1["S"])), seconds=round(t1, 2))
+        # the GPU on the same sample (inputs resident, lines to host memory)
+        cpu["gpu_same_sample"] = {}
+        for mname in mode_names:
+            eng.compute(sample, bufs=bufs, mode=binding.MODES[mname])
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            g = eng.compute(sample, bufs=bufs, mode=binding.MODES[mname])
+            tg = time.perf_counter() - t
+            cpu["gpu_same_sample"][mname] = len(g["S"]) / tg
         log(f"cpu baseline {cpu}")
         parity = sample_parity(eng, binding, ref, sample, bufs, mode_names)
         if "split" in modes:

@@ -280,15 +280,20 @@ class Engine:
         return res
 
     def compute(self, lower_ids=None, mode=GI_MODE_FP64, bufs=None):
-        cap = self.count_candidates(lower_ids)
-        if bufs is None or len(bufs["S"]) < max(cap, 1):
-            bufs = self.alloc_lines(cap)
-        s = self._lines(bufs)
-        if lower_ids is None:
-            rc = lib().gi_compute(self.h, mode, 0, None, C.byref(s))
-        else:
-            ids = np.ascontiguousarray(lower_ids, np.int32)
-            rc = lib().gi_compute(self.h, mode, len(ids), _p(ids, P_i), C.byref(s))
+        """gi_compute into bufs (allocated for every candidate when None; a
+        buffer that turns out too small is replaced and the call repeated)."""
+        if bufs is None:
+            bufs = self.alloc_lines(self.count_candidates(lower_ids))
+        ids = None if lower_ids is None else np.ascontiguousarray(lower_ids, np.int32)
+        for _ in range(2):
+            s = self._lines(bufs)
+            if ids is None:
+                rc = lib().gi_compute(self.h, mode, 0, None, C.byref(s))
+            else:
+                rc = lib().gi_compute(self.h, mode, len(ids), _p(ids, P_i), C.byref(s))
+            if rc != 5:  # GI_ECAPACITY: s.n holds the rows needed
+                break
+            bufs = self.alloc_lines(max(s.n, self.count_candidates(lower_ids)))
         self._chk(rc, "gi_compute")
         return self._result(bufs, s.n)
 
