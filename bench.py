@@ -192,15 +192,15 @@ def _roofline(name, kstats, work, nvib):
                     frac=ach / FP64_PEAK_TFS, traffic=None, kernel=name, avg_launch_ms=avg_ms,
                     work="FP64 flops of the symmetry-blocked products")
     if name == "corr_mfma":
-        ach = work["corr_bytes"] / 4.0 / t / 1e12  # 2*nsize_F flops per candidate
+        ach = work["corr_useful_flops"] / t / 1e12
         return dict(bound="mfma", achieved=ach, peak=FP64_PEAK_TFS, unit="TFLOP/s",
                     frac=ach / FP64_PEAK_TFS, traffic=None, kernel=name, avg_launch_ms=avg_ms,
-                    work="2*nsize_F FP64 flops per candidate")
+                    work="2*nsize_F FP64 flops per computed component pair of a candidate")
     if name in ("half_ls_split", "corr_split"):
         # int8 digit products the split method needs: S(S+1)/2 per logical
         # multiply-add of the FP64 product (S = 6 digits)
         pairs = 21
-        logical = work["half_ls_flops"] if name == "half_ls_split" else work["corr_bytes"] / 4.0
+        logical = work["half_ls_flops"] if name == "half_ls_split" else work["corr_useful_flops"]
         ach = logical * pairs / t / 1e12
         return dict(bound="mfma", achieved=ach, peak=INT8_PEAK_TOPS, unit="TOP/s",
                     frac=ach / INT8_PEAK_TOPS, traffic=None, kernel=name, avg_launch_ms=avg_ms,

@@ -461,12 +461,12 @@ class Engine:
                 for i in range(min(n.value, 32))}
 
     def work_stats(self):
-        w = np.zeros(16)
-        self._chk(lib().gi_work_stats(self.h, _p(w, P_d), 16), "gi_work_stats")
+        w = np.zeros(17)
+        self._chk(lib().gi_work_stats(self.h, _p(w, P_d), 17), "gi_work_stats")
         keys = ["units", "candidates", "lines", "half_ls_dense_flops", "half_ls_flops",
                 "corr_bytes", "mu_bytes", "mu_dense_bytes", "corr_flops", "host_plan_ms",
                 "host_enqueue_ms", "host_wait_copy_ms", "out_bytes", "call_ms",
-                "half_ls_tiles", "corr_tiles"]
+                "half_ls_tiles", "corr_tiles", "corr_useful_flops"]
         return dict(zip(keys, w.tolist()))
 
 

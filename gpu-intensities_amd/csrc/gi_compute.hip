@@ -833,6 +833,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             P.work[3] += ch.nunits * 2.0 * coef_of(c, ch.jindI, ch.jindF).C * (double)N * N;
             P.work[4] += ch.nunits * 2.0 * GP.macs;
             P.work[5] += (double)ch.ncand * 8.0 * BF.nsize;
+            P.work[16] += (double)ch.ncand * 2.0 * BF.nsize * __builtin_popcount(ch.cb_mask);
             P.work[6] += blk_bytes;
             P.work[7] += ch.nunits * 24.0 * (double)N * N;
             P.work[8] += 2.0 * (double)(ch.Hi - ch.Lo) * ch.nitems * ndegI * ndegF * BF.nsize;
@@ -1116,6 +1117,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             P.work[3] += ch.nunits * 2.0 * coef_of(c, ch.jindI, ch.jindF).C * (double)N * N;
             P.work[4] += ch.nunits * 2.0 * GP.macs;
             P.work[5] += (double)ch.ncand * 8.0 * BF.nsize;
+            P.work[16] += (double)ch.ncand * 2.0 * BF.nsize * __builtin_popcount(ch.cb_mask);
             P.work[6] += blk_bytes;
             P.work[7] += ch.nunits * 24.0 * (double)N * N;
             P.work[8] += 2.0 * (double)rows * ch.nitems * ndegI * ndegF * BF.nsize;
@@ -1283,6 +1285,7 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
             P.work[3] += ch.nunits * 2.0 * coef_of(c, ch.jindI, ch.jindF).C * (double)N * N;
             P.work[4] += ch.nunits * 2.0 * GP.macs;
             P.work[5] += (double)ch.ncand * 8.0 * BF.nsize;
+            P.work[16] += (double)ch.ncand * 2.0 * BF.nsize * __builtin_popcount(ch.cb_mask);
             P.work[6] += ch.nunits * blk_bytes;
             P.work[7] += ch.nunits * 24.0 * (double)N * N;
             P.work[8] += 2.0 * ch.ncand * ndegI * ndegF * BF.nsize;

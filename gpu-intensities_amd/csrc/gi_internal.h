@@ -20,7 +20,7 @@ constexpr int kMaxPipeRounds = 64;
 // Most split-K parts of a correlation product.
 constexpr int kMaxKParts = 4;
 // Counters of gi_work_stats.
-constexpr int kWorkStats = 16;
+constexpr int kWorkStats = 17;
 }  // namespace gi
 
 namespace gi {

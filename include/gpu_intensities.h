@@ -197,7 +197,8 @@ int gi_kernel_stats(gi_ctx *ctx, int max, const char **names, double *total_ms,
  * launches, [11] waiting for the device and the copies after that, [13] the
  * whole call; [12] bytes of the lines copied to `out`; [14] tiles of the
  * half-ls products and [15] of the correlation products (matrix-core modes,
- * summed over launches). n: up to 16. */
+ * summed over launches); [16] useful correlation flops (2*nsize_F per
+ * computed component pair of a candidate). n: up to 17. */
 int gi_work_stats(gi_ctx *ctx, double *out, int n);
 
 /* GI_MODE_SPLIT (no reference counterpart): every operand row or column of
