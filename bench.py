@@ -458,11 +458,13 @@ def sample_parity(eng, binding, ref, sample, bufs, mode_names):
 
 
 def mode_rooflines(modes, nvib):
-    """Matrix-core utilisation of the two product kernels of each batched mode."""
+    """Roofline of the two product kernels of each mode: HBM GB/s for the plain
+    FP64 (GEMV-form) kernels, matrix-core utilisation for the batched ones."""
     out = {}
     for mname, m in modes.items():
         ks = m["_kstats"]
-        for k in ("half_ls_mfma", "corr_mfma", "half_ls_split", "corr_split"):
+        for k in ("half_ls_gemv", "corr_intensity", "half_ls_mfma", "corr_mfma",
+                  "half_ls_split", "corr_split"):
             if k in ks and ks[k]["launches"]:
                 r = _roofline(k, ks, m["work"], nvib)
                 out.setdefault(mname, {})[k] = {x: r[x] for x in
@@ -510,7 +512,7 @@ def list_section(torch, binding, args, oracle_py, nthr, sym, nvib, jmax, mode_na
                fp64_blocks_GB=round(fp64_gb, 2), levels=int(job.nlevels),
                lower_levels=int(len(lower_ok)), candidates=int(cap),
                prepare_s=round(t_prep, 2), split_planes_s=round(t_split, 2),
-               modes=modes, matrix_core=roofs)
+               modes=modes, kernel_rooflines=roofs)
     if "fp64" in modes:
         for m in modes:
             modes[m]["speedup_vs_fp64_gemv"] = modes[m]["value"] / modes["fp64"]["value"]
@@ -720,7 +722,7 @@ def main():
         "kernels": kern,
         "work": work,
         "modes": modes,
-        "matrix_core": roofs,
+        "kernel_rooflines": roofs,
         "parity": parity,
         "multi_j": multi_j,
         "serving": serving,
