@@ -944,6 +944,18 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
     if (rc) return rc;
     unsigned char *d_srj = (unsigned char *)slot_scratch(c, S, "srjobs", scap + 4 * mchunks, &rc);
     if (rc) return rc;
+    const size_t fscap = (size_t)mchunks * sizeof(FoldSplitJob);
+    unsigned char *d_fsj = (unsigned char *)slot_scratch(c, S, "fsjobs", fscap + 4 * mchunks, &rc);
+    if (rc) return rc;
+    // fold and digit split fused when every upper block's column fits one
+    // workgroup (GI_FOLD_SPLIT=0 forces the two-kernel path; same bits)
+    const char *fenv = std::getenv("GI_FOLD_SPLIT");
+    bool fused = !(fenv && fenv[0] == '0');
+    int maxkp = 0;
+    for (const Chunk &ch : chunks) {
+        maxkp = std::max(maxkp, c->blocks[ch.jindF * c->nrep + ch.gF].Kp);
+        fused = fused && c->blocks[ch.jindF * c->nrep + ch.gF].Kp <= kFoldSplitMax;
+    }
     int maxparts = 1;
     for (const Chunk &ch : chunks) maxparts = std::max(maxparts, corr_kparts(c, P, ch));
     P.ampx_stride = 4 * std::max<long long>(P.ncand, 1);
@@ -1065,6 +1077,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
         }
         std::vector<FoldJob> fjobs;
         std::vector<SplitRowsJob> sjobs;
+        std::vector<FoldSplitJob> fsjobs;
         std::vector<int> fbegin, sbegin;
         int fblocks = 0, srows = 0;
         for (size_t i = rg.first; i < rg.second; i++) {
@@ -1091,6 +1104,8 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                 fblocks += fj.xt * cols;
                 sjobs.push_back(SplitRowsJob{Ht + Htoff[k], (long long)BF.nsize, BF.nsize, BF.Kp,
                                              Hd + Hdoff[k], (long long)mhd, He + Hcoff[k]});
+                fsjobs.push_back(FoldSplitJob{fj, BF.Kp, Hd + Hdoff[k], (long long)mhd,
+                                              He + Hcoff[k]});
                 sbegin.push_back(srows);
                 srows += cols;
             }
@@ -1122,7 +1137,14 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             P.work[7] += ch.nunits * 24.0 * (double)N * N;
             P.work[8] += 2.0 * (double)rows * ch.nitems * ndegI * ndegF * BF.nsize;
         }
-        if (!fjobs.empty()) {
+        if (!fjobs.empty() && fused) {
+            if ((rc = send_jobs(c, S, d_fsj, fsjobs, sbegin, fscap)) != GI_OK) return rc;
+            TL_BEGIN("fold_split");
+            launch_fold_split_jobs(S.stream, SD, (const FoldSplitJob *)d_fsj,
+                                   (const int *)(d_fsj + fscap), (int)fsjobs.size(), srows,
+                                   maxkp);
+            TL_END();
+        } else if (!fjobs.empty()) {
             if ((rc = send_jobs(c, S, d_fold, fjobs, fbegin, fcap)) != GI_OK) return rc;
             TL_BEGIN("fold");
             launch_fold_jobs(S.stream, (const FoldJob *)d_fold, (const int *)(d_fold + fcap),

@@ -241,6 +241,20 @@ struct SplitRowsJob {
 };
 void launch_split_rows_jobs(hipStream_t st, int S, const SplitRowsJob *d_jobs,
                             const int *d_begin, int njobs, int nrows);
+// Fold and digit split in one pass (split mode): one 1024-thread workgroup per
+// folded column, the column's values kept in registers, so the FP64 folded
+// vectors are never written; columns of up to kFoldSplitMax rows.
+constexpr int kFoldSplitMax = 32768;
+struct FoldSplitJob {
+    FoldJob f;                       // the fold of launch_fold_jobs (xt unused)
+    int Kp;
+    int8_t *D;                       // digits of column c at D + c * Kp (plane 0)
+    long long plane;
+    int *E;                          // exponent per column
+};
+// maxrows: the largest Kp of the jobs (<= kFoldSplitMax).
+void launch_fold_split_jobs(hipStream_t st, int S, const FoldSplitJob *d_jobs,
+                            const int *d_begin, int njobs, int ncols, int maxrows);
 // The intensity steps of every chunk of a round: blocks of a job = ceil(ncand / 256).
 void launch_intensity_jobs(hipStream_t st, const IntensArgs *d_jobs, const int *d_begin,
                            int njobs, int nblocks);

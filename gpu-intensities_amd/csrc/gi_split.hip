@@ -329,6 +329,56 @@ __global__ __launch_bounds__(1024) void k_build_g_split_lds(const GSplitJob *__r
     }
 }
 
+template <int S, int PER>
+__global__ __launch_bounds__(1024) void k_fold_split_jobs(const FoldSplitJob *__restrict__ jobs,
+                                                          const int *__restrict__ begin,
+                                                          int njobs)
+{
+    __shared__ double red[16];
+    const int ji = find_tile_job(begin, njobs, blockIdx.x);
+    const FoldSplitJob &J = jobs[ji];
+    const FoldJob &F = J.f;
+    const int col = blockIdx.x - begin[ji];
+    const int ncb = F.ndegI * F.ndegF;
+    const int item = col / ncb, cb = col - item * ncb;
+    const int idegF = cb / F.ndegI, idegI = cb - idegF * F.ndegI;
+    const int unit = F.item_unit[item * GI_MAXDEG + idegI];
+    const SymDev &T = idegF ? F.t1 : F.t0;
+    const double *Hu = F.H + (size_t)(unit >= 0 ? unit : 0) * F.SFN;
+    double v[PER];
+    double m = 0.0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int t = (int)threadIdx.x + k * 1024;
+        double acc = 0.0;
+        if (t < F.nsizeF && unit >= 0)
+            for (int e = T.tptr[t]; e < T.tptr[t + 1]; e++) acc += T.tval[e] * Hu[T.tcol[e]];
+        v[k] = acc;
+        m = fmax(m, fabs(acc));
+    }
+    // workgroup maximum: wavefront reduction, then the 16 wavefront results
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = red[0];
+#pragma unroll
+    for (int w = 1; w < 16; w++) m = fmax(m, red[w]);
+    const int e = digit_exp(m);
+    if (threadIdx.x == 0) J.E[col] = e;
+    int8_t *d = J.D + (size_t)col * J.Kp;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int t = (int)threadIdx.x + k * 1024;
+        if (t < J.Kp) {
+            int dg[S];
+            to_digits<S>(t < F.nsizeF ? v[k] : 0.0, e, dg);
+#pragma unroll
+            for (int q = 0; q < S; q++) d[q * J.plane + t] = (int8_t)dg[q];
+        }
+    }
+}
+
 // One K step of 64 for a 64 x 64 tile: the wave's 2 x 2 blocks, digit pairs
 // i + j < S into level accumulators.
 template <int S>
@@ -675,6 +725,21 @@ void launch_split_rows_jobs(hipStream_t st, int S, const SplitRowsJob *d_jobs,
 {
     if (njobs <= 0 || nrows <= 0) return;
 #define L_(s) hipLaunchKernelGGL(k_split_rows_jobs<s>, dim3(nrows), dim3(256), 0, st, d_jobs, d_begin, njobs)
+    GI_SPLIT_DISPATCH(S, L_)
+#undef L_
+}
+
+void launch_fold_split_jobs(hipStream_t st, int S, const FoldSplitJob *d_jobs,
+                            const int *d_begin, int njobs, int ncols, int maxrows)
+{
+    if (njobs <= 0 || ncols <= 0) return;
+    if (maxrows <= 16384) {
+#define L_(s) hipLaunchKernelGGL((k_fold_split_jobs<s, 16>), dim3(ncols), dim3(1024), 0, st, d_jobs, d_begin, njobs)
+        GI_SPLIT_DISPATCH(S, L_)
+#undef L_
+        return;
+    }
+#define L_(s) hipLaunchKernelGGL((k_fold_split_jobs<s, kFoldSplitMax / 1024>), dim3(ncols), dim3(1024), 0, st, d_jobs, d_begin, njobs)
     GI_SPLIT_DISPATCH(S, L_)
 #undef L_
 }

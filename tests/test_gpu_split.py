@@ -131,6 +131,18 @@ def test_split_digits_per_product(engine):
         engine.set_split_digits_each(7, 6)
 
 
+def test_fused_fold_split_equals_two_kernels(engine, monkeypatch):
+    """The fused fold + digit split gives the bits of the fold followed by the
+    separate digit split (same FP64 values, same column maxima)."""
+    job = synth.make_job("C3V", nvib=40, jvals=(3, 4), seed=19)
+    engine.load(job)
+    fused = {k: v.copy() for k, v in engine.compute(mode=binding.GI_MODE_SPLIT).items()}
+    monkeypatch.setenv("GI_FOLD_SPLIT", "0")
+    two = engine.compute(mode=binding.GI_MODE_SPLIT)
+    for k in fused:
+        assert np.array_equal(fused[k], two[k]), k
+
+
 def test_split_requests_alone_and_mixed(engine):
     job = synth.make_job("C2V", nvib=120, jvals=(1, 2), seed=21, e_span=9000.0)
     engine.load(job)
