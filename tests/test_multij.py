@@ -102,3 +102,4 @@ def test_chain_with_lower_ids_is_the_subset():
     assert len(b) == len(set(b)) and set(b) == a and len(a) > 0
     none = oracle_py.run_chain(job, lower_ids=np.zeros(0, np.int64))
     assert len(none["S"]) == 0
+
