@@ -34,7 +34,7 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_params", "gi_uplo
            "gi_serve_stop", "gi_set_tiles", "gi_get_tiles", "gi_set_split_digits",
            "gi_prepare_split", "gi_set_jvals", "gi_set_tuning", "gi_get_tuning",
            "gi_autotune", "gi_set_split_digits_each", "gi_sweep_create", "gi_sweep_run", "gi_sweep_candidates",
-           "gi_sweep_destroy"]
+           "gi_sweep_destroy", "gi_amp_width"]
 # (TROVE_EXPORTS below: the symbols of include/gi_trove.h)
 
 
@@ -96,6 +96,7 @@ def lib():
         L.gi_prepare.argtypes = [vp]
         L.gi_compute.argtypes = [vp, ci, ci, P_i, C.POINTER(GiLines)]
         L.gi_count_candidates.argtypes = [vp, ci, P_i, C.POINTER(cll)]
+        L.gi_amp_width.argtypes = [vp]
         L.gi_half_ls.argtypes = [vp, ci, ci, ci, ci, P_d, P_d]
         L.gi_correlate.argtypes = [vp, ci, ci, ci, P_d, P_d]
         L.gi_threej_table.argtypes = [ci, P_d]
@@ -251,32 +252,38 @@ class Engine:
                       "count")
         return n.value
 
+    def amp_width(self):
+        """Amplitudes per line (gi_amp_width): m*m, m the largest degeneracy."""
+        return lib().gi_amp_width(self.h)
+
     def alloc_lines(self, cap, pinned=False):
         cap = max(int(cap), 1)
+        w = max(self.amp_width(), 1)
         if pinned:
             import torch
             mk = lambda n, dt: torch.empty(n, dtype=dt, pin_memory=True).numpy()
             import torch as T
             out = dict(ilevelI=mk(cap, T.int32), ilevelF=mk(cap, T.int32), nu=mk(cap, T.float64),
                        S=mk(cap, T.float64), A_einst=mk(cap, T.float64), I=mk(cap, T.float64),
-                       amp=mk(cap * 4, T.float64))
+                       amp=mk(cap * w, T.float64))
         else:
             out = dict(ilevelI=np.empty(cap, np.int32), ilevelF=np.empty(cap, np.int32),
                        nu=np.empty(cap), S=np.empty(cap), A_einst=np.empty(cap), I=np.empty(cap),
-                       amp=np.empty(cap * 4))
+                       amp=np.empty(cap * w))
         return out
 
-    @staticmethod
-    def _lines(bufs):
-        return GiLines(capacity=len(bufs["S"]), n=0, ilevelI=_p(bufs["ilevelI"], P_i),
+    def _lines(self, bufs):
+        cap = min(len(bufs["S"]), len(bufs["amp"]) // max(self.amp_width(), 1))
+        return GiLines(capacity=cap, n=0, ilevelI=_p(bufs["ilevelI"], P_i),
                        ilevelF=_p(bufs["ilevelF"], P_i), nu=_p(bufs["nu"], P_d),
                        S=_p(bufs["S"], P_d), A_einst=_p(bufs["A_einst"], P_d),
                        I=_p(bufs["I"], P_d), amp=_p(bufs["amp"], P_d))
 
-    @staticmethod
-    def _result(bufs, n):
+    def _result(self, bufs, n):
+        """Rows [0, n) of bufs; amp as an (n, m*m) array, column idegI + m*idegF."""
+        w = self.amp_width()
         res = {k: bufs[k][:n] for k in ("ilevelI", "ilevelF", "nu", "S", "A_einst", "I")}
-        res["amp"] = bufs["amp"][:4 * n].reshape(n, 4)
+        res["amp"] = bufs["amp"][:w * n].reshape(n, w)
         return res
 
     def compute(self, lower_ids=None, mode=GI_MODE_FP64, bufs=None):

@@ -343,6 +343,7 @@ int gi_set_params(gi_ctx *c, const gi_params *p)
     for (int d : c->degen)
         if (d < 1 || d > GI_MAXDEG)
             return fail(c, GI_EINVAL, "gi_set_params: degeneracy must be 1 or 2");
+    c->amp_m = *std::max_element(c->degen.begin(), c->degen.end());
     c->sym_type = p->sym_type;
     c->nmodes = p->nmodes;
     c->zpe = p->zpe;
@@ -618,6 +619,7 @@ int gi_prepare(gi_ctx *c)
     const int nq = c->nmodes + 1;
     c->lv_lowok.assign(c->nlevels, 0);
     c->lv_upok.assign(c->nlevels, 0);
+    c->lv_low.clear();
     for (auto &B : c->blocks) {
         B.lev.clear();
         B.en.clear();
@@ -629,6 +631,7 @@ int gi_prepare(gi_ctx *c)
         const int J = c->jvals[c->lv_jind[i]];
         const int *q = &c->lv_quanta[(size_t)i * nq];
         c->lv_lowok[i] = energy_filter_lower(c, J, c->lv_energy[i], q);
+        if (c->lv_lowok[i]) c->lv_low.push_back(i);
         c->lv_upok[i] = energy_filter_upper(c, J, c->lv_energy[i], q);
         Block &B = c->blocks[c->lv_jind[i] * c->nrep + c->lv_igamma[i]];
         if (c->lv_irec0[i] < 0 || c->lv_irec0[i] >= B.nrec)

@@ -342,7 +342,7 @@ __global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict_
                 if (f < a.item_lo[i] || f >= a.item_hi[i]) continue;
                 const long long o = a.item_out_off[i] + (f - a.item_lo[i]);
                 const int idegF = cb / a.ndegI, idegI = cb - idegF * a.ndegI;
-                dst[4 * o + idegI + 2 * idegF] = acc[x][y][r];
+                dst[(long long)(a.amp_m * a.amp_m) * o + idegI + a.amp_m * idegF] = acc[x][y][r];
             }
 }
 
@@ -361,22 +361,24 @@ __device__ inline void intensity_one(const IntensArgs &a, int c)
     const int j = c - a.item_cand_start[item];
     const int f = a.item_lo[item] + j;
     const long long o = a.item_out_off[item] + j;
-    double *am = a.amp + 4 * o;
+    const int m = a.amp_m;
+    double *am = a.amp + (long long)(m * m) * o;
     double ls = 0.0;
     for (int idegF = 0; idegF < a.ndegF; idegF++)
         for (int idegI = 0; idegI < a.ndegI; idegI++) {
             const int cb = idegF * a.ndegI + idegI;
             double A = 0.0;
             if ((a.cb_mask >> cb) & 1) {
-                A = am[idegI + 2 * idegF];
+                A = am[idegI + m * idegF];
                 for (int p = 1; p < a.kparts; p++)
-                    A += a.ampx[(long long)(p - 1) * a.ampx_stride + 4 * o + idegI + 2 * idegF];
+                    A += a.ampx[(long long)(p - 1) * a.ampx_stride + (long long)(m * m) * o +
+                                idegI + m * idegF];
             }
-            am[idegI + 2 * idegF] = A;
+            am[idegI + m * idegF] = A;
             ls += A * A;
         }
-    for (int q = 0; q < 4; q++) {
-        const int idegI = q & 1, idegF = q >> 1;
+    for (int q = 0; q < m * m; q++) {
+        const int idegI = q % m, idegF = q / m;
         if (idegI >= a.ndegI || idegF >= a.ndegF) am[q] = 0.0;
     }
     ls /= (double)a.ndegI;

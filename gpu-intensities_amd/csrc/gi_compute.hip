@@ -200,13 +200,53 @@ static void window(const std::vector<double> &en, double eI, double w0, double w
               en.begin());
 }
 
+// First index i >= s of a partitioned range with pred(en[i]) false, given
+// pred true on [0, s): exponential steps from s, then a binary search in the
+// last step. Equals std::partition_point over the whole range.
+template <class Pred>
+static int gallop(const std::vector<double> &en, int s, Pred pred)
+{
+    const int n = (int)en.size();
+    int lo = s, hi = s, step = 1;
+    while (hi < n && pred(en[hi])) {
+        lo = hi + 1;
+        hi = lo + step;
+        step <<= 1;
+    }
+    hi = std::min(hi, n);
+    return (int)(std::partition_point(en.begin() + lo, en.begin() + hi, pred) - en.begin());
+}
+
+// window() for a run of calls on one block: while eI does not decrease, both
+// window ends only move up, so each search starts from the previous ends.
+struct WindowCursor {
+    double eI = 0.0;
+    int a = 0, b = 0;
+    bool valid = false;
+    void find(const std::vector<double> &en, double e, double w0, double w1, int &ra, int &rb)
+    {
+        if (!(valid && e >= eI)) {
+            window(en, e, w0, w1, ra, rb);
+        } else {
+            ra = gallop(en, a, [&](double x) { return !(x - e >= w0); });
+            rb = gallop(en, b, [&](double x) { return x - e <= w1; });
+        }
+        valid = true;
+        eI = e;
+        a = ra;
+        b = rb;
+    }
+};
+
 // Items of the deck job: the lower-level filters (cuda_host.cu:1194) and the
 // J / symmetry part of intensity_filter (trove_functions.cpp:1452-1464).
 static void items_job(const gi_ctx *c, int nlower, const int *ids, int query, Plan &P)
 {
-    const int n = ids ? nlower : c->nlevels;
+    // every level: only those that pass the lower-state filters, in id order
+    const int n = ids ? nlower : (int)c->lv_low.size();
+    std::vector<WindowCursor> cur(c->blocks.size());
     for (int k = 0; k < n; k++) {
-        const int L = ids ? ids[k] : k;
+        const int L = ids ? ids[k] : c->lv_low[k];
         if (L < 0 || L >= c->nlevels) continue;
         if (!c->lv_lowok[L]) continue;
         const int gI = c->lv_igamma[L];
@@ -221,8 +261,8 @@ static void items_job(const gi_ctx *c, int nlower, const int *ids, int query, Pl
             if (!((jF != c->jvals[0]) || (jI != c->jvals[0]))) continue;
             if (std::abs(jF - jI) > 1 || jI + jF < 1) continue;
             int a, b;
-            window(c->blocks[jindF * c->nrep + gF].en, eI, c->freq_window[0], c->freq_window[1],
-                   a, b);
+            const int bF = jindF * c->nrep + gF;
+            cur[bF].find(c->blocks[bF].en, eI, c->freq_window[0], c->freq_window[1], a, b);
             if (b <= a) continue;
             P.items.push_back(Item{L, jindI, gI, jindF, gF, a, b, query, P.ncand});
             P.ncand += b - a;
@@ -278,10 +318,13 @@ static int build_chunks(gi_ctx *c, Plan &P)
         const int ndegI = c->degen[gI], ndegF = c->degen[gF];
         const Block &BI = c->blocks[jindI * c->nrep + gI];
         std::vector<int> its = kv.second;
-        if (batched)  // energy order keeps every chunk's windows a narrow band
-            std::stable_sort(its.begin(), its.end(), [&](int x, int y) {
-                return c->lv_energy[P.items[x].level] < c->lv_energy[P.items[y].level];
-            });
+        // energy order keeps every chunk's windows a narrow band (level tables
+        // usually list a block's levels in energy order already)
+        const auto by_energy = [&](int x, int y) {
+            return c->lv_energy[P.items[x].level] < c->lv_energy[P.items[y].level];
+        };
+        if (batched && !std::is_sorted(its.begin(), its.end(), by_energy))
+            std::stable_sort(its.begin(), its.end(), by_energy);
         const GroupPlan &GP = group_of(c, jindI, gI, jindF);
         if (!GP.valid) return fail(c, GI_EINVAL, "internal: no plan for a (J, J') pair");
         for (size_t s0 = 0; s0 < its.size(); s0 += kChunk) {
@@ -497,7 +540,8 @@ static IntensArgs intens_args(gi_ctx *c, const Plan &P, const Chunk &ch, unsigne
     a.kparts = corr_kparts(c, P, ch);
     a.ampx = P.ampx;
     a.ampx_stride = P.ampx_stride;
-    a.amp = cand.amp; a.flag = flag; a.out_iI = cand.iI; a.out_iF = cand.iF;
+    a.amp = cand.amp; a.amp_m = c->amp_m;
+    a.flag = flag; a.out_iI = cand.iI; a.out_iF = cand.iF;
     a.out_nu = cand.nu; a.out_S = cand.S; a.out_A = cand.A; a.out_I = cand.I;
     return a;
 }
@@ -660,7 +704,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
     if (rc) return rc;
     int maxparts = 1;
     for (const Chunk &ch : chunks) maxparts = std::max(maxparts, corr_kparts(c, P, ch));
-    P.ampx_stride = 4 * std::max<long long>(P.ncand, 1);
+    P.ampx_stride = (long long)c->amp_m * c->amp_m * std::max<long long>(P.ncand, 1);
     P.ampx = maxparts > 1 ? (double *)slot_scratch(c, S, "c_ampx",
                                                    8 * (size_t)(maxparts - 1) * P.ampx_stride, &rc)
                           : nullptr;
@@ -823,6 +867,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             m.item_hi = (const int *)(d_meta + ch.off_hi);
             m.item_out_off = (const long long *)(d_meta + ch.off_out);
             m.blk_col = BF.d_col; m.vec = BF.d_vec; m.Ht = Htc[i - rg.first]; m.amp = cand.amp;
+            m.amp_m = c->amp_m;
             m.kparts = corr_kparts(c, P, ch);
             m.kchunk = (BF.nsize + 16 * m.kparts - 1) / (16 * m.kparts) * 16;
             m.ampx = P.ampx;
@@ -958,7 +1003,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
     }
     int maxparts = 1;
     for (const Chunk &ch : chunks) maxparts = std::max(maxparts, corr_kparts(c, P, ch));
-    P.ampx_stride = 4 * std::max<long long>(P.ncand, 1);
+    P.ampx_stride = (long long)c->amp_m * c->amp_m * std::max<long long>(P.ncand, 1);
     P.ampx = maxparts > 1 ? (double *)slot_scratch(c, S, "c_ampx",
                                                    8 * (size_t)(maxparts - 1) * P.ampx_stride, &rc)
                           : nullptr;
@@ -1119,6 +1164,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             a.Ud = BF.d_ud; a.uplane = BF.uplane; a.Ue = BF.d_ue;
             a.Hd = Hd + Hdoff[k]; a.hplane = (long long)mhd; a.He = He + Hcoff[k];
             a.amp = cand.amp;
+            a.amp_m = c->amp_m;
             a.kparts = corr_kparts(c, P, ch);
             a.kchunk = (BF.Kp + 64 * a.kparts - 1) / (64 * a.kparts) * 64;
             a.ampx = P.ampx;
@@ -1178,7 +1224,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
 
 static LineSoA shift(LineSoA s, long long o)
 {
-    s.iI += o; s.iF += o; s.nu += o; s.S += o; s.A += o; s.I += o; s.amp += 4 * o;
+    s.iI += o; s.iF += o; s.nu += o; s.S += o; s.A += o; s.I += o; s.amp += s.amp_w * o;
     return s;
 }
 
@@ -1202,14 +1248,17 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
     cand.S = (double *)slot_scratch(c, S, "c_S", nc * 8, &rc); if (rc) return rc;
     cand.A = (double *)slot_scratch(c, S, "c_A", nc * 8, &rc); if (rc) return rc;
     cand.I = (double *)slot_scratch(c, S, "c_I", nc * 8, &rc); if (rc) return rc;
-    cand.amp = (double *)slot_scratch(c, S, "c_amp", nc * 32, &rc); if (rc) return rc;
+    const int W = c->amp_m * c->amp_m;
+    cand.amp = (double *)slot_scratch(c, S, "c_amp", nc * 8 * W, &rc); if (rc) return rc;
+    cand.amp_w = W;
     comp.iI = (int *)slot_scratch(c, S, "o_iI", nc * 4, &rc); if (rc) return rc;
     comp.iF = (int *)slot_scratch(c, S, "o_iF", nc * 4, &rc); if (rc) return rc;
     comp.nu = (double *)slot_scratch(c, S, "o_nu", nc * 8, &rc); if (rc) return rc;
     comp.S = (double *)slot_scratch(c, S, "o_S", nc * 8, &rc); if (rc) return rc;
     comp.A = (double *)slot_scratch(c, S, "o_A", nc * 8, &rc); if (rc) return rc;
     comp.I = (double *)slot_scratch(c, S, "o_I", nc * 8, &rc); if (rc) return rc;
-    comp.amp = (double *)slot_scratch(c, S, "o_amp", nc * 32, &rc); if (rc) return rc;
+    comp.amp = (double *)slot_scratch(c, S, "o_amp", nc * 8 * W, &rc); if (rc) return rc;
+    comp.amp_w = W;
     P.comp = comp;
     P.cand = cand;
     // compaction scratch: per query, block counts and offsets
@@ -1299,6 +1348,7 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
             a.Ht = Ht;
             a.flag = flag; a.out_iI = cand.iI; a.out_iF = cand.iF; a.out_nu = cand.nu;
             a.out_S = cand.S; a.out_A = cand.A; a.out_I = cand.I; a.out_amp = cand.amp;
+            a.amp_m = c->amp_m;
             TL_BEGIN("corr_intensity");
             launch_corr_intensity(S.stream, a);
             TL_END();
@@ -1338,8 +1388,8 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
     return GI_OK;
 }
 
-// Host bytes of one emitted line (ilevelI, ilevelF, nu, S, A, I, amp[4]).
-constexpr double kLineBytes = 4 + 4 + 8 * 4 + 32;
+// Host bytes of one emitted line without its amplitudes (ilevelI, ilevelF, nu, S, A, I).
+constexpr double kLineBytes = 4 + 4 + 8 * 4;
 
 // Async copy of n compacted lines starting at candidate slot `from` to host
 // arrays at row `to`, on stream st.
@@ -1355,7 +1405,8 @@ static int copy_range(gi_ctx *c, hipStream_t st, const LineSoA &comp, long long 
     HIPCHK(hipMemcpyAsync(Sv + to, s.S, n * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(A + to, s.A, n * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(I + to, s.I, n * 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(amp + 4 * to, s.amp, n * 32, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(amp + s.amp_w * to, s.amp, n * 8 * s.amp_w, hipMemcpyDeviceToHost,
+                          st));
     return GI_OK;
 }
 
@@ -1455,7 +1506,7 @@ int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
     c->work[2] = (double)nout;
     c->work[9] = t_plan;
     c->work[10] = t_enqueue;
-    c->work[12] = (double)nout * kLineBytes;
+    c->work[12] = (double)nout * (kLineBytes + 8.0 * c->amp_m * c->amp_m);
     out->n = nout;
     if (!P.pipe) {
         if (nout > out->capacity)
@@ -1678,7 +1729,7 @@ static void finish_batch(Server &sv, Inflight &b)
             const long long n = b.plan->counts[k];
             R.n = n;
             R.iI.resize(n); R.iF.resize(n); R.nu.resize(n); R.S.resize(n); R.A.resize(n);
-            R.I.resize(n); R.amp.resize(4 * n);
+            R.I.resize(n); R.amp.resize((size_t)c->amp_m * c->amp_m * n);
             if (n > 0 && copy_lines(c, S, *b.plan, k, R.iI.data(), R.iF.data(), R.nu.data(),
                                     R.S.data(), R.A.data(), R.I.data(), R.amp.data()) != GI_OK)
                 rc = GI_EHIP;
@@ -1922,6 +1973,8 @@ int gi_count_candidates(gi_ctx *c, int nlower, const int *ids, long long *count)
     return count_job(c, nlower, ids, count);
 }
 
+int gi_amp_width(const gi_ctx *c) { return c ? c->amp_m * c->amp_m : 0; }
+
 int gi_compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
 {
     if (!c || !out) return GI_EINVAL;
@@ -2091,7 +2144,7 @@ int gi_serve_fetch(gi_ctx *c, long long ticket, gi_lines *out)
         std::memcpy(out->S, R->S.data(), R->n * 8);
         std::memcpy(out->A_einst, R->A.data(), R->n * 8);
         std::memcpy(out->I, R->I.data(), R->n * 8);
-        std::memcpy(out->amp, R->amp.data(), R->n * 32);
+        std::memcpy(out->amp, R->amp.data(), R->amp.size() * 8);
     }
     std::lock_guard<std::mutex> lk(sv->mu);
     sv->all.erase(ticket);

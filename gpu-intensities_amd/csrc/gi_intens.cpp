@@ -62,7 +62,8 @@ int main(int argc, char **argv)
     long long cap = 0;
     if (rc == GI_OK) rc = gi_count_candidates(ctx, 0, nullptr, &cap);
     std::vector<int> iI(cap + 1), iF(cap + 1);
-    std::vector<double> nu(cap + 1), S(cap + 1), A(cap + 1), I(cap + 1), amp(4 * (cap + 1));
+    std::vector<double> nu(cap + 1), S(cap + 1), A(cap + 1), I(cap + 1),
+        amp((size_t)gi_amp_width(ctx) * (cap + 1));
     gi_lines out = {cap, 0, iI.data(), iF.data(), nu.data(), S.data(), A.data(), I.data(),
                     amp.data()};
     if (rc == GI_OK) rc = gi_compute(ctx, mode, 0, nullptr, &out);

@@ -175,6 +175,8 @@ struct gi_ctx {
     bool have_params = false;
     int nrep = 0, sym_type = 0, nmodes = 0, reduced = 0;
     std::vector<int> degen, isym_pairs, igamma_pair, quanta_lower, quanta_upper;
+    // largest degeneracy m: lines carry m*m amplitudes, entry idegI + m*idegF
+    int amp_m = 0;
     std::vector<double> gns;
     double zpe = 0, temperature = 0, q_stat = 1;
     double freq_window[2] = {0, 0}, erange_lower[2] = {0, 0}, erange_upper[2] = {0, 0};
@@ -199,6 +201,7 @@ struct gi_ctx {
     std::vector<double> lv_energy;
     std::vector<int> lv_jind, lv_igamma, lv_ndeg, lv_irec0, lv_quanta;
     std::vector<unsigned char> lv_lowok, lv_upok;
+    std::vector<int> lv_low;          // ascending ids of the levels with lv_lowok
     std::vector<gi::Block> blocks;   // [jind*nrep + g]
 
     // plan

@@ -19,10 +19,11 @@ struct SymDev {
     const double *tval;
 };
 
-// Structure-of-arrays line records.
+// Structure-of-arrays line records; amp_w amplitudes per line.
 struct LineSoA {
     int *iI, *iF;
     double *nu, *S, *A, *I, *amp;
+    int amp_w = 4;
 };
 
 // Device view of one output class of a group plan (see OutClass).
@@ -46,6 +47,7 @@ struct CorrArgs {
     double gnsI, zpe, beta, q_stat, A_coef_s_1, intens_cm_mol, thresh_S, thresh_I;
     int *flag, *out_iI, *out_iF;
     double *out_nu, *out_S, *out_A, *out_I, *out_amp;
+    int amp_m;                       // amplitudes per line amp_m^2, entry idegI + amp_m*idegF
 };
 
 // One chunk of the banded correlation product (batched mode).
@@ -56,7 +58,8 @@ struct CorrMMArgs {
     const long long *item_out_off;
     const int *blk_col;
     const double *vec, *Ht;
-    double *amp;                     // candidate amplitudes, 4 per candidate
+    double *amp;                     // candidate amplitudes, amp_m^2 per candidate
+    int amp_m;                       // entry idegI + amp_m*idegF
     int kparts, kchunk;              // split K as in CorrSplitArgs
     double *ampx;
     long long ampx_stride;
@@ -74,6 +77,7 @@ struct IntensArgs {
     double gnsI, zpe, beta, q_stat, A_coef_s_1, intens_cm_mol, thresh_S, thresh_I;
     const double *qpar;              // non-null: beta, q_stat, thresh_S, thresh_I read here
     double *amp;
+    int amp_m;                       // amp_m^2 amplitudes per candidate, idegI + amp_m*idegF
     int kparts;                      // correlation parts: amp + ampx[0..kparts-2], in order
     const double *ampx;
     long long ampx_stride;
@@ -166,6 +170,7 @@ struct CorrSplitArgs {
     long long hplane;
     const int *He;                   // per column
     double *amp;
+    int amp_m;                       // amp_m^2 amplitudes per candidate, idegI + amp_m*idegF
     // split K: part p of kparts sums K in [p*kchunk, (p+1)*kchunk); part 0 writes
     // amp, part p > 0 writes ampx + (p-1)*ampx_stride (same candidate layout)
     int kparts, kchunk;

@@ -346,11 +346,10 @@ __global__ __launch_bounds__(256) void k_corr_intensity(CorrArgs a)
     a.out_S[o] = S;
     a.out_A[o] = A_einst;
     a.out_I[o] = I;
-    double *am = a.out_amp + 4 * o;
-    am[0] = amp[0];
-    am[1] = amp[1];
-    am[2] = amp[2];
-    am[3] = amp[3];
+    const int m = a.amp_m;
+    double *am = a.out_amp + (long long)(m * m) * o;
+    for (int idegF = 0; idegF < m; idegF++)
+        for (int idegI = 0; idegI < m; idegI++) am[idegI + m * idegF] = amp[idegI + 2 * idegF];
 }
 
 // ---------------------------------------------------------------------------
@@ -437,7 +436,8 @@ __global__ __launch_bounds__(GI_CT) void k_scatter(long long n, const int *__res
             out.S[o] = in.S[s];
             out.A[o] = in.A[s];
             out.I[o] = in.I[s];
-            for (int q = 0; q < 4; q++) out.amp[4 * o + q] = in.amp[4 * s + q];
+            for (int q = 0; q < in.amp_w; q++)
+                out.amp[(long long)in.amp_w * o + q] = in.amp[(long long)in.amp_w * s + q];
             o++;
         }
     }

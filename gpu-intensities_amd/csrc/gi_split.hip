@@ -697,7 +697,8 @@ __global__ __launch_bounds__(256) void k_corr_split(const CorrSplitArgs *__restr
                 const long long o = a.item_out_off[i] + (f - a.item_lo[i]);
                 const int idegF = cb / a.ndegI, idegI = cb - idegF * a.ndegI;
                 const int ex = a.Ue[a.blk_col[f]] + a.He[nn];
-                dst[4 * o + idegI + 2 * idegF] = ldexp(oz_value<S>(acc[x][y], r), ex);
+                dst[(long long)(a.amp_m * a.amp_m) * o + idegI + a.amp_m * idegF] =
+                    ldexp(oz_value<S>(acc[x][y], r), ex);
             }
 }
 

@@ -872,6 +872,8 @@ int gi_trove_format(const gi_trove *t, const gi_lines *L, long long first, long 
     if (!t || !L || first < 0 || count < 0 || first + count > L->n) return GI_EINVAL;
     const Deck &d = t->deck;
     const int nm = d.nmodes, ncl = t->b0.nclasses;
+    // amplitudes per row: m*m, m the largest degeneracy (gi_amp_width)
+    const int m = *std::max_element(d.degen.begin(), d.degen.end());
     std::string s;
     s.reserve((size_t)count * 320);
     for (long long r = first; r < first + count; r++) {
@@ -897,7 +899,7 @@ int gi_trove_format(const gi_trove *t, const gi_lines *L, long long first, long 
         for (int i = 1; i <= nm; i++) appendf(s, " %3i", I.normal[i]);
         s += " ) ";
         for (int dF = 0; dF < F.ndeg; dF++)
-            for (int dI = 0; dI < I.ndeg; dI++) appendf(s, " %16.9e", L->amp[4 * r + dI + 2 * dF]);
+            for (int dI = 0; dI < I.ndeg; dI++) appendf(s, " %16.9e", L->amp[m * m * r + dI + m * dF]);
         s += "\n";
     }
     return put(s, buf, cap, len);

@@ -85,7 +85,13 @@ typedef struct gi_params {
 } gi_params;
 
 /* Emitted transitions, structure of arrays with room for `capacity` rows.
- * amp holds 4 doubles per row: A[idegI + 2*idegF] (cuda_host.cu:1467-1471). */
+ * amp holds W = m*m doubles per row, m the largest degeneracy of the
+ * context's symmetry (gi_params.degen; the reference's sym_maxdegen): entry
+ * idegI + m*idegF is the amplitude A(idegI, idegF), zero for components a
+ * level does not have. This is the layout of the reference's line_str per
+ * final state (cuda_host.cu:1467-1471: idegI + idegF*sym_maxdegen), so a
+ * symmetry without degenerate irreps (C2v) carries one amplitude per line.
+ * gi_amp_width gives W; amp needs room for capacity*W doubles. */
 typedef struct gi_lines {
     long long capacity;
     long long n;                  /* out: rows written (or required)         */
@@ -164,6 +170,11 @@ int gi_compute(gi_ctx *ctx, int mode, int nlower, const int *lower_ids, gi_lines
 
 /* Upper bound of the rows gi_compute can emit for these lower levels. */
 int gi_count_candidates(gi_ctx *ctx, int nlower, const int *lower_ids, long long *count);
+
+/* Doubles per row of gi_lines.amp for this context (1 or 4; see gi_lines),
+ * fixed by gi_set_params; 0 for a null context or before gi_set_params. Replaces the reference's sym_maxdegen**2 stride of
+ * line_str (cuda_host.cu:1467-1471). */
+int gi_amp_width(const gi_ctx *ctx);
 
 /* do_1st_half_ls (cuda_host.cu:767-780): correlate the symmetrised lower
  * vector vec (nsize of (jvals[jindI], igammaI), host memory) for component

@@ -19,6 +19,14 @@ P_i = C.POINTER(C.c_int)
 P_d = C.POINTER(C.c_double)
 
 
+def amp_columns(m):
+    """Columns of the oracle's 4-wide amplitude rows (entry idegI + 2*idegF)
+    in the order of a line's amplitudes for largest degeneracy m: entry
+    idegI + m*idegF, the reference's line_str layout per final state with
+    sym_maxdegen = m (cuda_host.cu:1467-1471)."""
+    return [dI + 2 * dF for dF in range(m) for dI in range(m)]
+
+
 class OrBset(C.Structure):
     _fields_ = [("jval", ci), ("Maxsymcoeffs", ci), ("Maxcontracts", ci), ("mat_size", ci),
                 ("icontr2icase", P_i), ("iroot_correlat_j0", P_i), ("k", P_i), ("ktau", P_i),
@@ -183,20 +191,22 @@ class OracleJob:
             res["ilevelF"] = np.ctypeslib.as_array(out.ilevelF, (n,)).copy()
             for f in ("nu", "S", "A_einst", "I"):
                 res[f] = np.ctypeslib.as_array(getattr(out, f), (n,)).copy()
-            res["amp"] = np.ctypeslib.as_array(out.amp, (n * 4,)).reshape(n, 4).copy()
-            res["amp_abs"] = np.ctypeslib.as_array(out.amp_abs, (n * 4,)).reshape(n, 4).copy()
+            cols = amp_columns(max(self.job.sym["degen"]))
+            for f in ("amp", "amp_abs"):
+                a = np.ctypeslib.as_array(getattr(out, f), (n * 4,)).reshape(n, 4)
+                res[f] = a[:, cols].copy()
         else:
-            res = OracleJob.empty()
+            res = OracleJob.empty(len(amp_columns(max(self.job.sym["degen"]))))
         L.or_free_lines(C.byref(out))
         return res
 
     @staticmethod
-    def empty():
+    def empty(width=4):
         res = {"ilevelI": np.zeros(0, np.int32), "ilevelF": np.zeros(0, np.int32)}
         for f in ("nu", "S", "A_einst", "I"):
             res[f] = np.zeros(0)
-        res["amp"] = np.zeros((0, 4))
-        res["amp_abs"] = np.zeros((0, 4))
+        res["amp"] = np.zeros((0, width))
+        res["amp_abs"] = np.zeros((0, width))
         return res
 
     def correlate(self, jind, ideg, g, vec):
@@ -256,7 +266,7 @@ def run_chain(job, lower_ids=None, nthreads=0):
         r["ilevelF"] = keep[r["ilevelF"]].astype(np.int32)
         parts.append(r)
     if not parts:
-        return OracleJob.empty()
+        return OracleJob.empty(len(amp_columns(max(job.sym["degen"]))))
     return {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
 
 

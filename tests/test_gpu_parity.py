@@ -121,6 +121,24 @@ def test_lower_subset_and_order(engine, c1_job, mode):
 
 
 @pytest.mark.parametrize("mode", MODES)
+def test_lower_ids_in_random_order(engine, mode):
+    """Lower levels in random order, with repeats and out-of-range ids: the
+    upper-level windows are searched afresh whenever the lower energy drops,
+    so every window equals that of a lone search (the oracle's)."""
+    job = synth.make_job("C2V", nvib=40, jvals=(3, 4), seed=23, e_span=15000.0,
+                         freq_window=(-2000.0, 6000.0))
+    engine.load(job)
+    rng = np.random.default_rng(5)
+    ids = rng.permutation(job.nlevels)[: job.nlevels // 2]
+    ids = np.concatenate([ids, ids[:40], [-1, job.nlevels]]).astype(np.int32)
+    got = engine.compute(ids, mode=mode)
+    keep = ids[(ids >= 0) & (ids < job.nlevels)]
+    ref = oracle_py.OracleJob(job).run(keep)
+    compare(got, ref)
+    assert len(ref["S"]) > 1000
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_nonsymmetric_mu_shuffled_cases(engine, mode):
     job = synth.make_job("C2V", nvib=48, jvals=(1, 2), seed=21, symmetric_mu=False,
                          shuffle_cases=True)

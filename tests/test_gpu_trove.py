@@ -50,11 +50,12 @@ def oracle_lines(job):
     ref = oracle_py.OracleJob(job).run()
     lv = job.levels
     key = lambda i: (int(lv["jval"][i]), int(lv["igamma"][i]), int(lv["ilevel"][i]))
+    m = max(job.sym["degen"])
     d = {}
     for r in range(len(ref["S"])):
         iI, iF = int(ref["ilevelI"][r]), int(ref["ilevelF"][r])
         ndI, ndF = int(lv["ndeg"][iI]), int(lv["ndeg"][iF])
-        amps = [ref["amp"][r][dI + 2 * dF] for dF in range(ndF) for dI in range(ndI)]
+        amps = [ref["amp"][r][dI + m * dF] for dF in range(ndF) for dI in range(ndI)]
         d[(key(iF), key(iI))] = (ref["S"][r], ref["A_einst"][r], ref["I"][r], amps)
     return d
 

@@ -280,9 +280,10 @@ def _printf_line(t_lv, labels, nm, ncl, zpe, res, r, cg, normal):
         t_lv["ilevel"][iI] + 1, 0, 0)
     s += "".join(" %3i" % normal[iF][i] for i in range(1, nm + 1)) + " ) <-  ( "
     s += "".join(" %3i" % normal[iI][i] for i in range(1, nm + 1)) + " ) "
+    m = int(round(np.sqrt(res["amp"].shape[1])))  # amplitudes idegI + m*idegF
     for dF in range(t_lv["ndeg"][iF]):
         for dI in range(t_lv["ndeg"][iI]):
-            s += " %16.9e" % res["amp"][r][dI + 2 * dF]
+            s += " %16.9e" % res["amp"][r][dI + m * dF]
     return s + "\n"
 
 
@@ -300,7 +301,7 @@ def test_line_layout(tmp_path, sym):
                ilevelF=rng.integers(0, len(lv["energy"]), n).astype(np.int32),
                nu=rng.normal(0, 3000, n), S=10 ** rng.uniform(-40, 2, n),
                A_einst=10 ** rng.uniform(-20, 3, n), I=10 ** rng.uniform(-45, -18, n),
-               amp=rng.normal(0, 1e-3, (n, 4)))
+               amp=rng.normal(0, 1e-3, (n, max(job.sym["degen"]) ** 2)))
     # ΔJ within ±1 so the branch letter exists
     jI, jF = lv["jval"][res["ilevelI"]], lv["jval"][res["ilevelF"]]
     assert (np.abs(jF - jI) <= 1).all()
