@@ -18,8 +18,10 @@
 // rows are padded with zeros to a multiple of 64 so that a K step never
 // needs a bounds check. Tiles are 64 x 64 outputs per 256-thread workgroup
 // (4 wavefronts of 32 x 32 as 2 x 2 MFMA blocks), K in steps of 64 staged
-// through LDS (rows of 80 bytes: the 16 lanes of one 16-byte LDS read then
-// hit 16 distinct bank quads). Operand layout of the 16x16x64 i8 instruction
+// through LDS in unpadded rows of kRowB = 64 bytes (measured on gfx950: both
+// the staging stores, 4 lanes per 64-byte row, and the operand reads, 16 rows
+// of one 16-byte chunk, then run at the LDS rate of 128 B/clk; an 80-byte
+// pitch halves the rate of the stores). Operand layout of the 16x16x64 i8 instruction
 // (checked on the device): lane l supplies A[l%16][16*(l/16) .. +15] and
 // B[16*(l/16) .. +15][l%16]; accumulator register r holds D[4*(l/16)+r][l%16].
 #include <hip/hip_runtime.h>
@@ -29,6 +31,8 @@
 #include "gi_kernels.h"
 
 namespace gi {
+
+constexpr int kRowB = 64;  // LDS row pitch of the digit tiles, bytes
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
@@ -391,13 +395,13 @@ __device__ inline void oz_step(const int8_t *__restrict__ As, const int8_t *__re
     v4i b[S][2];
 #pragma unroll
     for (int j = 0; j < S; j++) {
-        b[j][0] = *(const v4i *)(Bs + (j * 64 + wn + i16) * 80 + k16);
-        b[j][1] = *(const v4i *)(Bs + (j * 64 + wn + 16 + i16) * 80 + k16);
+        b[j][0] = *(const v4i *)(Bs + (j * 64 + wn + i16) * kRowB + k16);
+        b[j][1] = *(const v4i *)(Bs + (j * 64 + wn + 16 + i16) * kRowB + k16);
     }
 #pragma unroll
     for (int i = 0; i < S; i++) {
-        const v4i a0 = *(const v4i *)(As + (i * 64 + wm + i16) * 80 + k16);
-        const v4i a1 = *(const v4i *)(As + (i * 64 + wm + 16 + i16) * 80 + k16);
+        const v4i a0 = *(const v4i *)(As + (i * 64 + wm + i16) * kRowB + k16);
+        const v4i a1 = *(const v4i *)(As + (i * 64 + wm + 16 + i16) * kRowB + k16);
 #pragma unroll
         for (int j = 0; j < S - i; j++) {
             acc[0][0][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b[j][0], acc[0][0][i + j], 0, 0, 0);
@@ -428,8 +432,8 @@ __global__ __launch_bounds__(256) void k_halfls_split(int N, int kpad_all,
                                                       const int *__restrict__ tbegin, int njobs,
                                                       int ntiles)
 {
-    __shared__ __attribute__((aligned(16))) int8_t As[S * 64 * 80];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[S * 64 * 80];
+    __shared__ __attribute__((aligned(16))) int8_t As[S * 64 * kRowB];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[S * 64 * kRowB];
     const int tile = xcd_tile_s(blockIdx.x, gridDim.x);
     if (tile >= ntiles) return;
     const SplitHalfJob &J = jobs[find_tile_job(tbegin, njobs, tile)];
@@ -472,8 +476,8 @@ __global__ __launch_bounds__(256) void k_halfls_split(int N, int kpad_all,
     while (e < cd.nseg) {
 #pragma unroll
         for (int q = 0; q < S; q++) {
-            *(v4i *)(As + (q * 64 + lr) * 80 + lp) = ra[q];
-            *(v4i *)(Bs + (q * 64 + lr) * 80 + lp) = rb[q];
+            *(v4i *)(As + (q * 64 + lr) * kRowB + lp) = ra[q];
+            *(v4i *)(Bs + (q * 64 + lr) * kRowB + lp) = rb[q];
         }
         __syncthreads();
         kk += 64;
@@ -513,15 +517,15 @@ __device__ inline void oz_step_wide(const int8_t *__restrict__ As,
     v4i b[S][2];
 #pragma unroll
     for (int j = 0; j < S; j++) {
-        b[j][0] = *(const v4i *)(Bs + (j * 64 + wn + i16) * 80 + k16);
-        b[j][1] = *(const v4i *)(Bs + (j * 64 + wn + 16 + i16) * 80 + k16);
+        b[j][0] = *(const v4i *)(Bs + (j * 64 + wn + i16) * kRowB + k16);
+        b[j][1] = *(const v4i *)(Bs + (j * 64 + wn + 16 + i16) * kRowB + k16);
     }
 #pragma unroll
     for (int i = 0; i < S; i++) {
         v4i a[4];
 #pragma unroll
         for (int x = 0; x < 4; x++)
-            a[x] = *(const v4i *)(As + (i * 128 + wm + 16 * x + i16) * 80 + k16);
+            a[x] = *(const v4i *)(As + (i * 128 + wm + 16 * x + i16) * kRowB + k16);
 #pragma unroll
         for (int j = 0; j < S - i; j++)
 #pragma unroll
@@ -541,8 +545,8 @@ __global__ __launch_bounds__(256, 1) void k_halfls_split_wide(
     const int *__restrict__ mue, const SplitHalfJob *__restrict__ jobs,
     const int *__restrict__ tbegin, int njobs, int ntiles)
 {
-    __shared__ __attribute__((aligned(16))) int8_t As[S * 128 * 80];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[S * 64 * 80];
+    __shared__ __attribute__((aligned(16))) int8_t As[S * 128 * kRowB];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[S * 64 * kRowB];
     const int tile = xcd_tile_s(blockIdx.x, gridDim.x);
     if (tile >= ntiles) return;
     const SplitHalfJob &J = jobs[find_tile_job(tbegin, njobs, tile)];
@@ -586,9 +590,9 @@ __global__ __launch_bounds__(256, 1) void k_halfls_split_wide(
     while (e < cd.nseg) {
 #pragma unroll
         for (int q = 0; q < S; q++) {
-            *(v4i *)(As + (q * 128 + lr) * 80 + lp) = ra0[q];
-            *(v4i *)(As + (q * 128 + 64 + lr) * 80 + lp) = ra1[q];
-            *(v4i *)(Bs + (q * 64 + lr) * 80 + lp) = rb[q];
+            *(v4i *)(As + (q * 128 + lr) * kRowB + lp) = ra0[q];
+            *(v4i *)(As + (q * 128 + 64 + lr) * kRowB + lp) = ra1[q];
+            *(v4i *)(Bs + (q * 64 + lr) * kRowB + lp) = rb[q];
         }
         __syncthreads();
         kk += 64;
@@ -623,8 +627,8 @@ __global__ __launch_bounds__(256) void k_corr_split(const CorrSplitArgs *__restr
                                                     const int *__restrict__ cbegin, int nchunks,
                                                     int ntiles)
 {
-    __shared__ __attribute__((aligned(16))) int8_t As[S * 64 * 80];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[S * 64 * 80];
+    __shared__ __attribute__((aligned(16))) int8_t As[S * 64 * kRowB];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[S * 64 * kRowB];
     __shared__ int s_need;
     const int tile = xcd_tile_s(blockIdx.x, gridDim.x);
     if (tile >= ntiles) return;
@@ -675,8 +679,8 @@ __global__ __launch_bounds__(256) void k_corr_split(const CorrSplitArgs *__restr
     for (int k0 = kb; k0 < ke; k0 += 64) {
 #pragma unroll
         for (int q = 0; q < S; q++) {
-            *(v4i *)(As + (q * 64 + lr) * 80 + lp) = ra[q];
-            *(v4i *)(Bs + (q * 64 + lr) * 80 + lp) = rb[q];
+            *(v4i *)(As + (q * 64 + lr) * kRowB + lp) = ra[q];
+            *(v4i *)(Bs + (q * 64 + lr) * kRowB + lp) = rb[q];
         }
         __syncthreads();
         if (k0 + 64 < ke) fetch(k0 + 64);
