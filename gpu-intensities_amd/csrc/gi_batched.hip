@@ -44,11 +44,13 @@ __device__ inline int find_job(const int *__restrict__ begin, int njobs, int wg)
 // Tile shapes. BM x BN output tile per 256-thread workgroup (4 wavefronts in
 // a 2 x 2 layout, each owning a (BM/2) x (BN/2) block of 16 x 16 MFMA
 // tiles); K staged through LDS in steps of BK with a register prefetch of
-// the next step. LDS rows are padded to BM + 16 doubles: the four K rows one
-// MFMA operand read touches then fall 32 banks apart.
+// the next step. LDS rows are padded to BM + 4 doubles (measured on gfx950: the
+// correlation's staging stores, 4 lanes on 4 K rows of one record, take 16
+// cycles per wave instruction at a pitch of BM + 16 and 6.3 at BM + 4; the
+// operand reads and the half-ls stores run at the same rate with either).
 template <int BM_, int BN_>
 struct Tile {
-    static constexpr int PA = BM_ + 16, PB = BN_ + 16;
+    static constexpr int PA = BM_ + 4, PB = BN_ + 4;
     static constexpr int MI = BM_ / 32, NI = BN_ / 32;  // 16x16 blocks per wave
     static constexpr int LA = BM_ * BK / 256, LB = BN_ * BK / 256;
 };
@@ -290,7 +292,7 @@ __global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict_
         for (int y = 0; y < T::NI; y++) acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
     // loads: K = lt + 4q (q < 4) of row lr + 64p (p < BM/64 for A, BN/64 for B);
     // the 4 lanes of a row read 32 contiguous bytes per load and store into 4
-    // consecutive LDS rows, which fall into alternate bank halves
+    // consecutive LDS rows, BM + 4 doubles apart, so on distinct banks
     const int lr = tid >> 2, lt = tid & 3;
     const double *Urow[BM_ / 64];
     const double *Hrow[BN_ / 64];
