@@ -455,13 +455,33 @@ __global__ __launch_bounds__(1024) void k_build_g_map_lds(const GMapJob *__restr
     const int d = J.uideg[u];
     const int *ip = J.idx[d];
     const double *wp = J.wt[d];
-    double *G = J.G + (size_t)u * J.Gtot;
-    for (int g = threadIdx.x; g < J.Gtot; g += 1024) {
-        double v = 0.0;
+    const int Gt = J.Gtot, W = J.W;
+    double *G = J.G + (size_t)u * Gt;
+    // four elements per thread per pass, 1024 apart: every map load of the pass
+    // is in flight before the first gather (one element at a time left the
+    // waves waiting on L2 latency most of the time)
+    for (int g0 = threadIdx.x; g0 < Gt; g0 += 4096) {
+        int ix[4][4];
+        double wv[4][4];
 #pragma unroll
-        for (int w = 0; w < 4; w++)
-            if (w < J.W) v += wp[(size_t)w * J.Gtot + g] * rec[ip[(size_t)w * J.Gtot + g]];
-        G[g] = v;
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int w = 0; w < 4; w++) {
+                const int g = g0 + 1024 * r;
+                const bool ok = w < W && g < Gt;
+                ix[r][w] = ok ? ip[(size_t)w * Gt + g] : 0;
+                wv[r][w] = ok ? wp[(size_t)w * Gt + g] : 0.0;
+            }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int g = g0 + 1024 * r;
+            if (g >= Gt) break;
+            double v = 0.0;
+#pragma unroll
+            for (int w = 0; w < 4; w++)
+                if (w < W) v += wv[r][w] * rec[ix[r][w]];
+            G[g] = v;
+        }
     }
 }
 

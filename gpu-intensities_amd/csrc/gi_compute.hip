@@ -1017,7 +1017,8 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
     bool g4 = true, glds = true;
     for (const Chunk &ch : chunks) {
         g4 = g4 && group_of(c, ch.jindI, ch.gI, ch.jindF).gW <= 4;
-        glds = glds && c->blocks[ch.jindI * c->nrep + ch.gI].nsize <= kRecLds;
+        glds = glds && c->blocks[ch.jindI * c->nrep + ch.gI].nsize <= kRecLds &&
+               group_of(c, ch.jindI, ch.gI, ch.jindF).oc.size() <= (size_t)kClsLds;
     }
     glds = glds && g4;
     const int grows = g4 ? 1024 : 256;

@@ -122,6 +122,7 @@ struct GMapJob {
     int nsize;                       // record length of the lower block
 };
 constexpr int kGMapMaxW = 16;
+constexpr int kClsLds = 32;  // output classes of a group that k_build_g_split_lds keeps in LDS
 constexpr int kGUnits = 16;
 // Records up to this many doubles are staged in LDS by the one-unit-per-workgroup
 // G builds (launch_build_g_map_lds, launch_build_g_split_lds).

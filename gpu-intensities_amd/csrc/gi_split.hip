@@ -283,6 +283,67 @@ __global__ __launch_bounds__(256) void k_build_g_split4(const GSplitJob *__restr
     }
 }
 
+// The G rows of one unit for k_build_g_split_lds: NC chunks of four consecutive
+// rows of one column per thread and pass (chunks 4096 rows apart), every map
+// load of the pass in flight before the first gather from the record in LDS
+// (maps of width <= WMAX).
+template <int S, int NC, int WMAX>
+__device__ inline void g_split_passes(const GSplitJob &J, int d, int u, double um,
+                                      const double *rec, const int (*cls)[kClsLds])
+{
+    int8_t *D = J.D + (size_t)u * J.GtotB;
+    const int W = J.W, Gt = J.GtotB;
+    const double *bnd = J.bnd[d];
+    int co = 0;
+    for (int g0 = threadIdx.x * 4; g0 < Gt; g0 += 4096 * NC) {
+        int4 ix[NC][WMAX];
+        double2 wa[NC][WMAX], wb[NC][WMAX];
+        double bv[NC];
+        int col[NC];  // column index within the unit's exponents; -1: not its first row
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const int g = g0 + 4096 * c;
+            if (g >= Gt) continue;
+            // class of row g from the table in LDS (cbaseB, kpad, eoff)
+            while (co + 1 < J.nclasses && cls[0][co + 1] <= g) co++;
+            const int j = (g - cls[0][co]) / cls[1][co];
+            bv[c] = bnd[cls[2][co] + j];
+            col[c] = (g - cls[0][co]) - j * cls[1][co] == 0 ? cls[2][co] + j : -1;
+#pragma unroll
+            for (int w = 0; w < WMAX; w++)
+                if (w < W) {
+                    const size_t o = (size_t)w * Gt + g;
+                    ix[c][w] = *(const int4 *)(J.idx[d] + o);
+                    wa[c][w] = *(const double2 *)(J.wt[d] + o);
+                    wb[c][w] = *(const double2 *)(J.wt[d] + o + 2);
+                }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const int g = g0 + 4096 * c;
+            if (g >= Gt) break;
+            const int e = digit_exp(bv[c] * um);
+            double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int w = 0; w < WMAX; w++)
+                if (w < W) {
+                    v[0] += wa[c][w].x * rec[ix[c][w].x];
+                    v[1] += wa[c][w].y * rec[ix[c][w].y];
+                    v[2] += wb[c][w].x * rec[ix[c][w].z];
+                    v[3] += wb[c][w].y * rec[ix[c][w].w];
+                }
+            int dg[4][S];
+#pragma unroll
+            for (int r = 0; r < 4; r++) to_digits<S>(v[r], e, dg[r]);
+#pragma unroll
+            for (int q = 0; q < S; q++)
+                *(int *)(D + q * J.plane + g) = (dg[0][q] & 255) | ((dg[1][q] & 255) << 8) |
+                                                ((dg[2][q] & 255) << 16) | ((dg[3][q] & 255) << 24);
+            if (col[c] >= 0) J.E[(size_t)u * J.EGtot + col[c]] = e;
+        }
+    }
+}
+
 // G digits of one unit per workgroup: the unit's record is read once, with
 // coalesced loads, into LDS, and the gathers of the record map read it there
 // (they land on scattered cache lines when they read global memory); four
@@ -293,44 +354,23 @@ __global__ __launch_bounds__(1024) void k_build_g_split_lds(const GSplitJob *__r
                                                             int njobs)
 {
     __shared__ double rec[kRecLds];
+    __shared__ int cls[3][kClsLds];  // per output class: cbaseB, kpad, eoff
     const int ji = find_tile_job(ubegin, njobs, blockIdx.x);
     const GSplitJob &J = jobs[ji];
     const int u = blockIdx.x - ubegin[ji];
     const double *src = J.uptr[u];
     for (int i = threadIdx.x; i < J.nsize; i += 1024) rec[i] = src[i];
+    if (threadIdx.x < min(J.nclasses, kClsLds)) {
+        const ClassDev &cd = J.classes[threadIdx.x];
+        cls[0][threadIdx.x] = cd.cbaseB;
+        cls[1][threadIdx.x] = cd.kpad;
+        cls[2][threadIdx.x] = cd.eoff;
+    }
     __syncthreads();
     const int d = J.uideg[u];
     const double um = J.umax[J.urec[u]];
-    int8_t *D = J.D + (size_t)u * J.GtotB;
-    int co = 0;
-    for (int g = threadIdx.x * 4; g < J.GtotB; g += 4096) {
-        while (co + 1 < J.nclasses && J.classes[co + 1].cbaseB <= g) co++;
-        const ClassDev &cd = J.classes[co];
-        const int j = (g - cd.cbaseB) / cd.kpad;
-        const bool first = (g - cd.cbaseB) - j * cd.kpad == 0;
-        const int e = digit_exp(J.bnd[d][cd.eoff + j] * um);
-        double v[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int w = 0; w < 4; w++)
-            if (w < J.W) {
-                const size_t o = (size_t)w * J.GtotB + g;
-                const int4 ix = *(const int4 *)(J.idx[d] + o);
-                const double2 a = *(const double2 *)(J.wt[d] + o);
-                const double2 b = *(const double2 *)(J.wt[d] + o + 2);
-                v[0] += a.x * rec[ix.x];
-                v[1] += a.y * rec[ix.y];
-                v[2] += b.x * rec[ix.z];
-                v[3] += b.y * rec[ix.w];
-            }
-        int dg[4][S];
-#pragma unroll
-        for (int r = 0; r < 4; r++) to_digits<S>(v[r], e, dg[r]);
-#pragma unroll
-        for (int q = 0; q < S; q++)
-            *(int *)(D + q * J.plane + g) = (dg[0][q] & 255) | ((dg[1][q] & 255) << 8) |
-                                            ((dg[2][q] & 255) << 16) | ((dg[3][q] & 255) << 24);
-        if (first) J.E[(size_t)u * J.EGtot + cd.eoff + j] = e;
-    }
+    if (J.W <= 2) g_split_passes<S, 2, 2>(J, d, u, um, rec, cls);
+    else g_split_passes<S, 1, 4>(J, d, u, um, rec, cls);
 }
 
 template <int S, int PER>
