@@ -853,7 +853,8 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                 t1.tcol = GP.d_tcol[1];
             }
             if (ch.nitems > 0 && BF.nsize > 0) {
-                FoldJob fj{ch.nitems, BF.nsize, ndegI, ndegF, GP.Htot, (BF.nsize + 255) / 256,
+                FoldJob fj{ch.nitems, BF.nsize, ndegI, ndegF, GP.Htot,
+                           (BF.nsize + kFoldRows - 1) / kFoldRows,
                            (const int *)(d_meta + ch.off_iunit), t0, t1, Hc[i - rg.first],
                            Htc[i - rg.first]};
                 fjobs.push_back(fj);
@@ -1142,7 +1143,8 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             }
             const int cols = ch.nitems * ndegI * ndegF;
             if (cols > 0 && BF.nsize > 0) {
-                FoldJob fj{ch.nitems, BF.nsize, ndegI, ndegF, GP.Htot, (BF.nsize + 255) / 256,
+                FoldJob fj{ch.nitems, BF.nsize, ndegI, ndegF, GP.Htot,
+                           (BF.nsize + kFoldRows - 1) / kFoldRows,
                            (const int *)(d_meta + ch.off_iunit), t0, t1, H + Hoff[k],
                            Ht + Htoff[k]};
                 fjobs.push_back(fj);

@@ -228,12 +228,48 @@ void launch_fold(hipStream_t st, int nitems, int nsizeF, int ndegI, int ndegF, i
 // The folds of every chunk of a round in one launch: job = one chunk's
 // launch_fold arguments; blocks of a job = xt * nitems * ndegI * ndegF.
 struct FoldJob {
-    int nitems, nsizeF, ndegI, ndegF, SFN, xt;
+    int nitems, nsizeF, ndegI, ndegF, SFN, xt;  // xt: blocks of kFoldRows rows per column
     const int *item_unit;
     SymDev t0, t1;
     const double *H;
     double *Ht;
 };
+constexpr int kFoldRows = 1024;  // rows of one k_fold_jobs block (4 per thread)
+
+// R rows of one folded column, acc[r] = sum over the CSR row t[r] of R_F^T of
+// w * Hu[g] (rows with ok[r] false give 0). The bounds of every row, then the
+// first two entries of each row with their H values, are loaded before any is
+// used, so R rows' loads are in flight together; longer rows finish in a loop.
+// The sum runs in entry order, as one row at a time would.
+template <int R>
+__device__ inline void fold_rows(const SymDev &T, const double *__restrict__ Hu,
+                                 const int (&t)[R], const bool (&ok)[R], double (&acc)[R])
+{
+    int p0[R], p1[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        p0[r] = ok[r] ? T.tptr[t[r]] : 0;
+        p1[r] = ok[r] ? T.tptr[t[r] + 1] : 0;
+    }
+    double w0[R], w1[R], h0[R], h1[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const bool a = p0[r] < p1[r], b = p0[r] + 1 < p1[r];
+        const int c0 = a ? T.tcol[p0[r]] : 0, c1 = b ? T.tcol[p0[r] + 1] : 0;
+        w0[r] = a ? T.tval[p0[r]] : 0.0;
+        w1[r] = b ? T.tval[p0[r] + 1] : 0.0;
+        h0[r] = a ? Hu[c0] : 0.0;
+        h1[r] = b ? Hu[c1] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        double v = 0.0;
+        if (p0[r] < p1[r]) v += w0[r] * h0[r];
+        if (p0[r] + 1 < p1[r]) v += w1[r] * h1[r];
+        for (int e = p0[r] + 2; e < p1[r]; e++) v += T.tval[e] * Hu[T.tcol[e]];
+        acc[r] = v;
+    }
+}
 void launch_fold_jobs(hipStream_t st, const FoldJob *d_jobs, const int *d_begin, int njobs,
                       int nblocks);
 // The digit splits of every chunk of a round: one workgroup per row.

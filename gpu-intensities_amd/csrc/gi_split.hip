@@ -391,14 +391,37 @@ __global__ __launch_bounds__(1024) void k_fold_split_jobs(const FoldSplitJob *__
     const double *Hu = F.H + (size_t)(unit >= 0 ? unit : 0) * F.SFN;
     double v[PER];
     double m = 0.0;
+    if constexpr (PER <= 16) {
+        // four rows at a time with their loads in flight together (fold_rows)
 #pragma unroll
-    for (int k = 0; k < PER; k++) {
-        const int t = (int)threadIdx.x + k * 1024;
-        double acc = 0.0;
-        if (t < F.nsizeF && unit >= 0)
-            for (int e = T.tptr[t]; e < T.tptr[t + 1]; e++) acc += T.tval[e] * Hu[T.tcol[e]];
-        v[k] = acc;
-        m = fmax(m, fabs(acc));
+        for (int k0 = 0; k0 < PER; k0 += 4) {
+            int t[4];
+            bool ok[4];
+            double acc[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                t[r] = (int)threadIdx.x + (k0 + r) * 1024;
+                ok[r] = t[r] < F.nsizeF && unit >= 0;
+            }
+            fold_rows<4>(T, Hu, t, ok, acc);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                v[k0 + r] = acc[r];
+                m = fmax(m, fabs(acc[r]));
+            }
+        }
+    } else {
+        // one row at a time: the 32 values per thread leave no registers to
+        // keep several rows' loads in flight (it spills more that way)
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int t = (int)threadIdx.x + k * 1024;
+            double acc = 0.0;
+            if (t < F.nsizeF && unit >= 0)
+                for (int e = T.tptr[t]; e < T.tptr[t + 1]; e++) acc += T.tval[e] * Hu[T.tcol[e]];
+            v[k] = acc;
+            m = fmax(m, fabs(acc));
+        }
     }
     // workgroup maximum: wavefront reduction, then the 16 wavefront results
 #pragma unroll
