@@ -357,7 +357,7 @@ def read_tuning(path):
             elif line.startswith("best ") and mode:
                 kv = dict(x.split("=") for x in line.split()[1:])
                 out[mode] = (int(kv["halfls_tile"]), int(kv["corr_tile"]), int(kv["pipe_rounds"]),
-                             int(kv.get("split_tile", 0)))
+                             int(kv.get("split_tile", 0)), int(kv.get("pipe_share", 50)))
     return out
 
 

@@ -67,7 +67,8 @@ class GiRequest(C.Structure):
 
 
 class GiTuning(C.Structure):
-    _fields_ = [("halfls_cfg", ci), ("corr_cfg", ci), ("pipe_rounds", ci), ("split_cfg", ci)]
+    _fields_ = [("halfls_cfg", ci), ("corr_cfg", ci), ("pipe_rounds", ci), ("split_cfg", ci),
+                ("pipe_share", ci)]
 
 
 class GiLines(C.Structure):
@@ -323,16 +324,16 @@ class Engine:
         self._chk(lib().gi_get_tiles(self.h, C.byref(a), C.byref(b)), "gi_get_tiles")
         return a.value, b.value
 
-    def set_tuning(self, halfls_cfg, corr_cfg, pipe_rounds, split_cfg=0):
+    def set_tuning(self, halfls_cfg, corr_cfg, pipe_rounds, split_cfg=0, pipe_share=50):
         t = GiTuning(halfls_cfg=halfls_cfg, corr_cfg=corr_cfg, pipe_rounds=pipe_rounds,
-                     split_cfg=split_cfg)
+                     split_cfg=split_cfg, pipe_share=pipe_share)
         self._chk(lib().gi_set_tuning(self.h, C.byref(t)), "gi_set_tuning")
 
     def get_tuning(self):
         t = GiTuning()
         self._chk(lib().gi_get_tuning(self.h, C.byref(t)), "gi_get_tuning")
         return dict(halfls_cfg=t.halfls_cfg, corr_cfg=t.corr_cfg, pipe_rounds=t.pipe_rounds,
-                    split_cfg=t.split_cfg)
+                    split_cfg=t.split_cfg, pipe_share=t.pipe_share)
 
     def autotune(self, mode, lower_ids=None, bufs=None, reps=2):
         """gi_autotune on these lower levels; returns (chosen setting, table text)."""
@@ -352,7 +353,8 @@ class Engine:
                                C.byref(s), reps, C.byref(best), buf, len(buf), C.byref(n))
         self._chk(rc, "gi_autotune")
         return (dict(halfls_cfg=best.halfls_cfg, corr_cfg=best.corr_cfg,
-                     pipe_rounds=best.pipe_rounds, split_cfg=best.split_cfg),
+                     pipe_rounds=best.pipe_rounds, split_cfg=best.split_cfg,
+                     pipe_share=best.pipe_share),
                 buf.value.decode())
 
     # -- captured sweeps ---------------------------------------------------

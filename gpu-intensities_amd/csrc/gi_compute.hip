@@ -421,11 +421,11 @@ static LineSoA shift(LineSoA s, long long o);
 
 // Consecutive chunks grouped into rounds of at most kRoundBytes of device
 // scratch (GI_BATCH_BYTES overrides). With per-round output the first
-// c->pipe_rounds - 1 rounds each take half of the work still left (at least
-// 1 MB of scratch) and the last takes the rest: each round's line copy then
-// overlaps the next round's compute (copies outrun the compute that produces
-// the lines), and the copy left after the last round is 1/2^(pipe_rounds-1)
-// of the output instead of 1/pipe_rounds.
+// c->pipe_rounds - 1 rounds each take c->pipe_share percent of the work still
+// left (at least 1 MB of scratch) and the last takes the rest: each round's
+// line copy overlaps the next round's compute while the copy of a round takes
+// less than the next round's compute, and the copy left after the last round
+// is (1 - share)^(pipe_rounds-1) of the output instead of 1/pipe_rounds.
 static std::vector<std::pair<size_t, size_t>> plan_rounds(const gi_ctx *c, const Plan &P,
                                                           const std::vector<size_t> &bytes)
 {
@@ -438,7 +438,7 @@ static std::vector<std::pair<size_t, size_t>> plan_rounds(const gi_ctx *c, const
     for (size_t b : bytes) left += b;
     auto budget = [&](size_t r) {
         if (!P.pipe || (int)r + 1 >= c->pipe_rounds) return kRoundBytes;
-        return std::min(kRoundBytes, std::max(left / 2, (size_t)1 << 20));
+        return std::min(kRoundBytes, std::max(left / 100 * c->pipe_share, (size_t)1 << 20));
     };
     std::vector<std::pair<size_t, size_t>> rounds;
     size_t start = 0, acc = 0, cap = budget(0);
@@ -1542,7 +1542,8 @@ int autotune(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out, int
     if (mode != GI_MODE_FP64 && mode != GI_MODE_FP64_BATCHED && mode != GI_MODE_SPLIT)
         return fail(c, GI_EINVAL, "gi_autotune: unknown mode");
     if (c->server) return fail(c, GI_ESTATE, "gi_autotune while the request server runs");
-    const gi_tuning saved{c->tile_halfls, c->tile_corr, c->pipe_rounds, c->tile_split};
+    const gi_tuning saved{c->tile_halfls, c->tile_corr, c->pipe_rounds, c->tile_split,
+                          c->pipe_share};
     std::string txt;
     char line[256];
     long long ncand = 0;
@@ -1598,6 +1599,7 @@ int autotune(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out, int
     static const int tiles[4] = {0, 1, 2, 3};
     static const char *const tname[4] = {"64x64", "64x128", "128x64", "128x128"};
     static const int rounds[6] = {1, 2, 3, 4, 6, 8};
+    static const int shares[4] = {50, 60, 70, 80};
     static const int stiles[2] = {0, 1};
     static const char *const sname[2] = {"64x64", "128x64"};
     if (mode == GI_MODE_SPLIT &&
@@ -1613,17 +1615,21 @@ int autotune(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out, int
         }
     }
     if (mode != GI_MODE_FP64 && out->capacity >= ncand) {  // pipelined output only
-        if ((rc = scan("pipe_rounds", nullptr, &c->pipe_rounds, rounds, 6, nullptr))) {
+        if ((rc = scan("pipe_rounds", nullptr, &c->pipe_rounds, rounds, 6, nullptr)) ||
+            (rc = scan("pipe_share", nullptr, &c->pipe_share, shares, 4, nullptr))) {
             c->tile_halfls = saved.halfls_cfg; c->tile_corr = saved.corr_cfg;
             c->pipe_rounds = saved.pipe_rounds; c->tile_split = saved.split_cfg;
+            c->pipe_share = saved.pipe_share;
             return rc;
         }
     }
     std::snprintf(line, sizeof line,
-                  "best halfls_tile=%d corr_tile=%d pipe_rounds=%d split_tile=%d\n",
-                  c->tile_halfls, c->tile_corr, c->pipe_rounds, c->tile_split);
+                  "best halfls_tile=%d corr_tile=%d pipe_rounds=%d split_tile=%d pipe_share=%d\n",
+                  c->tile_halfls, c->tile_corr, c->pipe_rounds, c->tile_split, c->pipe_share);
     txt += line;
-    if (best) *best = gi_tuning{c->tile_halfls, c->tile_corr, c->pipe_rounds, c->tile_split};
+    if (best)
+        *best = gi_tuning{c->tile_halfls, c->tile_corr, c->pipe_rounds, c->tile_split,
+                          c->pipe_share};
     if (len) *len = (long long)txt.size();
     if (table && cap > 0) {
         const size_t n = std::min<size_t>((size_t)cap - 1, txt.size());
@@ -1990,12 +1996,13 @@ int gi_set_tuning(gi_ctx *c, const gi_tuning *t)
     if (c->server) return fail(c, GI_ESTATE, "gi_set_tuning while the request server runs");
     if (t->halfls_cfg < 0 || t->halfls_cfg > 3 || t->corr_cfg < 0 || t->corr_cfg > 3 ||
         t->pipe_rounds < 1 || t->pipe_rounds > kMaxPipeRounds || t->split_cfg < 0 ||
-        t->split_cfg > 1)
+        t->split_cfg > 1 || t->pipe_share < 10 || t->pipe_share > 90)
         return fail(c, GI_EINVAL, "gi_set_tuning: value out of range");
     c->tile_halfls = t->halfls_cfg;
     c->tile_corr = t->corr_cfg;
     c->pipe_rounds = t->pipe_rounds;
     c->tile_split = t->split_cfg;
+    c->pipe_share = t->pipe_share;
     return GI_OK;
 }
 
@@ -2006,6 +2013,7 @@ int gi_get_tuning(gi_ctx *c, gi_tuning *t)
     t->corr_cfg = c->tile_corr;
     t->pipe_rounds = c->pipe_rounds;
     t->split_cfg = c->tile_split;
+    t->pipe_share = c->pipe_share;
     return GI_OK;
 }
 

@@ -228,6 +228,7 @@ struct gi_ctx {
     // the rounds of a pipelined gi_compute (gi_set_tuning, gi_autotune)
     int tile_halfls = 3, tile_corr = 0;
     int pipe_rounds = 4;
+    int pipe_share = 50;             // percent of the remaining work per round but the last
     int tile_split = 0;              // k_halfls_split rows: 0 = 64, 1 = 128
 
     // split mode: digits per operand (3..6) and the resident digit planes of mu

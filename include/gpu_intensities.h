@@ -241,13 +241,16 @@ int gi_get_tiles(gi_ctx *ctx, int *halfls_cfg, int *corr_cfg);
  * of gi_set_tiles; pipe_rounds is the number of rounds a gi_compute with room
  * for every candidate is cut into, each round's lines being copied to the
  * host while later rounds compute (1..64; default 4): every round but the
- * last takes half of the work left, so the copy after the last round is
- * small. Lines are bitwise the same for every setting. */
+ * last takes pipe_share percent of the work left (10..90; default 50), so the
+ * copy after the last round is small. A share above 50 shrinks the last
+ * round further and suits workloads whose line copy takes well under half of
+ * the compute time. Lines are bitwise the same for every setting. */
 typedef struct gi_tuning {
     int halfls_cfg;
     int corr_cfg;
     int pipe_rounds;
     int split_cfg;    /* GI_MODE_SPLIT half-ls tiles: 0 = 64 x 64, 1 = 128 x 64 */
+    int pipe_share;   /* percent of the remaining work per round but the last */
 } gi_tuning;
 int gi_set_tuning(gi_ctx *ctx, const gi_tuning *t);
 int gi_get_tuning(gi_ctx *ctx, gi_tuning *t);

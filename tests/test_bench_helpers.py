@@ -15,7 +15,8 @@ def test_bench_reads_saved_tuning_table(tmp_path):
     p.write_text("# gi_autotune tables\n\n## mode fp64_batched (tuned in 1 s)\n"
                  "halfls_tile 3 128x128 4.0 ms\nbest halfls_tile=3 corr_tile=1 pipe_rounds=4\n"
                  "\n## mode split (tuned in 1 s)\nsplit_halfls_tile 0 64x64 2.0 ms\n"
-                 "best halfls_tile=3 corr_tile=0 pipe_rounds=3 split_tile=1\n")
-    assert bench.read_tuning(str(p)) == {"fp64_batched": (3, 1, 4, 0), "split": (3, 0, 3, 1)}
+                 "best halfls_tile=3 corr_tile=0 pipe_rounds=3 split_tile=1 pipe_share=60\n")
+    assert bench.read_tuning(str(p)) == {"fp64_batched": (3, 1, 4, 0, 50),
+                                         "split": (3, 0, 3, 1, 60)}
     committed = bench.read_tuning(os.path.join(ROOT, "profiles", "autotune_c2.txt"))
     assert set(committed) == {"fp64_batched", "split"}

@@ -13,7 +13,7 @@ from test_gpu_parity import compare
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = dict(halfls_cfg=3, corr_cfg=0, pipe_rounds=4, split_cfg=0)
+DEFAULT = dict(halfls_cfg=3, corr_cfg=0, pipe_rounds=4, split_cfg=0, pipe_share=50)
 
 
 @pytest.fixture
@@ -39,11 +39,12 @@ def test_autotune_picks_a_setting_and_keeps_the_bits(tuned_engine, mode):
     assert lines[0].startswith("# gi_autotune") and lines[-1].startswith("best ")
     kinds = [ln.split()[0] for ln in lines[1:-1]]
     want = (["halfls_tile"] * 4 + ["corr_tile"] * 4 if mode == binding.GI_MODE_FP64_BATCHED
-            else ["split_halfls_tile"] * 2) + ["pipe_rounds"] * 6
+            else ["split_halfls_tile"] * 2) + ["pipe_rounds"] * 6 + ["pipe_share"] * 4
     assert kinds == want
-    assert lines[-1] == ("best halfls_tile=%d corr_tile=%d pipe_rounds=%d split_tile=%d"
-                         % (best["halfls_cfg"], best["corr_cfg"], best["pipe_rounds"],
-                            best["split_cfg"]))
+    assert lines[-1] == ("best halfls_tile=%d corr_tile=%d pipe_rounds=%d split_tile=%d "
+                         "pipe_share=%d" % (best["halfls_cfg"], best["corr_cfg"],
+                                            best["pipe_rounds"], best["split_cfg"],
+                                            best["pipe_share"]))
     after = _copy(eng.compute(mode=mode))
     for k in before:
         assert np.array_equal(before[k], after[k]), k
@@ -62,10 +63,12 @@ def test_every_round_count_gives_the_same_bits(tuned_engine, monkeypatch):
         eng.set_tuning(3, 0, 1)
         ref = _copy(eng.compute(mode=mode))
         for rounds in (2, 3, 8, 64):
-            eng.set_tuning(3, 0, rounds)
-            got = eng.compute(mode=mode)
-            for k in ref:
-                assert np.array_equal(ref[k], got[k]), (mode, rounds, k)
+            for share in (50, 70, 10, 90):
+                eng.set_tuning(3, 0, rounds, 0, share)
+                got = eng.compute(mode=mode)
+                for k in ref:
+                    assert np.array_equal(ref[k], got[k]), (mode, rounds, share, k)
+        eng.set_tuning(3, 0, 1)
         monkeypatch.setenv("GI_BATCH_BYTES", str(1 << 20))
         got = eng.compute(mode=mode)
         monkeypatch.delenv("GI_BATCH_BYTES")
@@ -93,12 +96,14 @@ def test_set_tuning_rules(tuned_engine):
     eng = tuned_engine
     job = synth.make_job("C2V", nvib=16, jvals=(0, 1), seed=54)
     eng.load(job)
-    for bad in ((4, 0, 2, 0), (0, -1, 2, 0), (0, 0, 0, 0), (0, 0, 65, 0), (0, 0, 2, 2)):
+    for bad in ((4, 0, 2, 0), (0, -1, 2, 0), (0, 0, 0, 0), (0, 0, 65, 0), (0, 0, 2, 2),
+                (0, 0, 2, 0, 9), (0, 0, 2, 0, 91)):
         with pytest.raises(binding.GIError):
             eng.set_tuning(*bad)
     assert eng.get_tuning() == DEFAULT
-    eng.set_tuning(1, 2, 5, 1)
-    assert eng.get_tuning() == dict(halfls_cfg=1, corr_cfg=2, pipe_rounds=5, split_cfg=1)
+    eng.set_tuning(1, 2, 5, 1, 65)
+    assert eng.get_tuning() == dict(halfls_cfg=1, corr_cfg=2, pipe_rounds=5, split_cfg=1,
+                                    pipe_share=65)
     assert eng.get_tiles() == (1, 2)
     eng.serve_start(1 << 28)
     try:
