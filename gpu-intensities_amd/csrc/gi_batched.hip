@@ -328,7 +328,12 @@ __global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict_
         }
         __syncthreads();
         if (k0 + BK < ke) fetch(k0 + BK);
+        // raised issue priority while this wavefront multiplies, so the other
+        // workgroup's loads and LDS stores fill the gaps (k_corr_mfma 1.47 -> 1.38 ms
+        // at C2; the half-ls kernels gain nothing or lose)
+        __builtin_amdgcn_s_setprio(1);
         mfma_step<BM_, BN_>(As, Bs, wm, wn, lane, acc);
+        __builtin_amdgcn_s_setprio(0);
         __syncthreads();
     }
 #pragma unroll
