@@ -447,14 +447,24 @@ __global__ __launch_bounds__(1024) void k_fold_split_jobs(const FoldSplitJob *__
 }
 
 // One K step of 64 for a 64 x 64 tile: the wave's 2 x 2 blocks, digit pairs
-// i + j < S into level accumulators.
+// i + j < S into level accumulators. Every B digit plane of the wave's two
+// column blocks stays in registers for the step, so LDS delivers each operand
+// once per wavefront; the A fragments of digit i + 1 are read before the
+// products of digit i (sched_barrier keeps them there), so their LDS latency
+// hides behind those products instead of stalling the next group. Each
+// accumulator takes its products in the same order as one digit at a time.
 template <int S>
 __device__ inline void oz_step(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs,
                                int wm, int wn, int lane, v4i (&acc)[2][2][S])
 {
+    // nothing of the caller's next-step fetch moves past this point: its scalar
+    // loads are then complete, and the LDS waits below can count exactly
+    __builtin_amdgcn_sched_barrier(0);
     const int i16 = lane & 15, k16 = (lane >> 4) * 16;
-    // every B digit plane of the wave's two column blocks stays in registers
-    // for the step, so LDS delivers each operand once per wavefront
+    auto ldA = [&](int i, int x) {
+        return *(const v4i *)(As + (i * 64 + wm + 16 * x + i16) * kRowB + k16);
+    };
+    v4i a0 = ldA(0, 0), a1 = ldA(0, 1);
     v4i b[S][2];
 #pragma unroll
     for (int j = 0; j < S; j++) {
@@ -463,8 +473,12 @@ __device__ inline void oz_step(const int8_t *__restrict__ As, const int8_t *__re
     }
 #pragma unroll
     for (int i = 0; i < S; i++) {
-        const v4i a0 = *(const v4i *)(As + (i * 64 + wm + i16) * kRowB + k16);
-        const v4i a1 = *(const v4i *)(As + (i * 64 + wm + 16 + i16) * kRowB + k16);
+        v4i n0 = a0, n1 = a1;
+        if (i + 1 < S) {
+            n0 = ldA(i + 1, 0);
+            n1 = ldA(i + 1, 1);
+        }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int j = 0; j < S - i; j++) {
             acc[0][0][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b[j][0], acc[0][0][i + j], 0, 0, 0);
@@ -472,6 +486,9 @@ __device__ inline void oz_step(const int8_t *__restrict__ As, const int8_t *__re
             acc[1][0][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b[j][0], acc[1][0][i + j], 0, 0, 0);
             acc[1][1][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b[j][1], acc[1][1][i + j], 0, 0, 0);
         }
+        __builtin_amdgcn_sched_barrier(0);
+        a0 = n0;
+        a1 = n1;
     }
 }
 
