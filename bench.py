@@ -198,14 +198,17 @@ def _roofline(name, kstats, work, nvib):
                     work="2*nsize_F FP64 flops per computed component pair of a candidate")
     if name in ("half_ls_split", "corr_split"):
         # int8 digit products the split method needs: S(S+1)/2 per logical
-        # multiply-add of the FP64 product (S = 6 digits)
-        pairs = 21
+        # multiply-add of the FP64 product, S the product's digit count
+        S = int(work.get("split_digits_halfls" if name == "half_ls_split"
+                         else "split_digits_corr", 6))
+        R = int(work.get("split_radix", 128))
+        pairs = S * (S + 1) // 2
         logical = work["half_ls_flops"] if name == "half_ls_split" else work["corr_useful_flops"]
         ach = logical * pairs / t / 1e12
         return dict(bound="mfma", achieved=ach, peak=INT8_PEAK_TOPS, unit="TOP/s",
                     frac=ach / INT8_PEAK_TOPS, traffic=None, kernel=name, avg_launch_ms=avg_ms,
-                    work="int8 ops of the 6-digit split products (21 digit pairs per logical "
-                         "multiply-add); logical FP64-equivalent rate "
+                    work=f"int8 ops of the {S}-digit radix-{R} split products ({pairs} digit "
+                         "pairs per logical multiply-add); logical FP64-equivalent rate "
                          f"{logical / t / 1e12:.1f} TFLOP/s")
     if name == "corr_intensity":
         ach = work["corr_bytes"] / t / 1e9
@@ -394,6 +397,9 @@ def time_modes(eng, binding, bufs, mode_names, steps, warmup, tune=True, sweep=T
         nl = int(len(res["S"]))
         kstats = eng.kernel_stats()
         work = eng.work_stats()
+        if mname == "split":  # the digit format the products ran in
+            work["split_radix"], work["split_digits_halfls"], work["split_digits_corr"] = \
+                eng.split_format()
         modes[mname] = dict(ms_per_step=ms, transitions=nl, value=nl / (ms / 1e3),
                             kernels={k: dict(total_ms=round(v["total_ms"], 4),
                                              launches=v["launches"]) for k, v in kstats.items()},
@@ -709,7 +715,9 @@ def main():
         "metric": METRIC, "value": value, "unit": "transitions/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": ("i8 digits x6 (split products), f64 elsewhere" if best == "split" else "f64"),
+        "dtype": ((f"i8 digits x{work['split_digits_halfls']}/x{work['split_digits_corr']} "
+                   f"radix {work['split_radix']} (split products), f64 elsewhere")
+                  if best == "split" else "f64"),
         "data": "synthetic (seeded C2v generator, eigenvectors: Haar-random orthogonal "
                 "blocks built on the GPU)",
         "config": {"workload": "C2v Nvib=2000 J 10<->11, lower E<=10000 cm-1, window "

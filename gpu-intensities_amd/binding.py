@@ -34,7 +34,7 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_params", "gi_uplo
            "gi_serve_stop", "gi_set_tiles", "gi_get_tiles", "gi_set_split_digits",
            "gi_prepare_split", "gi_set_jvals", "gi_set_tuning", "gi_get_tuning",
            "gi_autotune", "gi_set_split_digits_each", "gi_sweep_create", "gi_sweep_run", "gi_sweep_candidates",
-           "gi_sweep_destroy", "gi_amp_width"]
+           "gi_sweep_destroy", "gi_amp_width", "gi_set_split_format", "gi_get_split_format"]
 # (TROVE_EXPORTS below: the symbols of include/gi_trove.h)
 
 
@@ -113,6 +113,8 @@ def lib():
         L.gi_set_tiles.argtypes = [vp, ci, ci]
         L.gi_set_split_digits.argtypes = [vp, ci]
         L.gi_set_split_digits_each.argtypes = [vp, ci, ci]
+        L.gi_set_split_format.argtypes = [vp, ci, ci, ci]
+        L.gi_get_split_format.argtypes = [vp, P_i, P_i, P_i]
         L.gi_prepare_split.argtypes = [vp]
         L.gi_get_tiles.argtypes = [vp, P_i, P_i]
         L.gi_sweep_create.argtypes = [vp, ci, ci, P_i, C.POINTER(vp)]
@@ -311,6 +313,19 @@ class Engine:
 
     def set_split_digits(self, digits):
         self._chk(lib().gi_set_split_digits(self.h, digits), "gi_set_split_digits")
+
+    def set_split_format(self, radix=0, halfls_digits=0, corr_digits=0):
+        """gi_set_split_format: radix 0 (automatic), 128 or 254; digits 0 (the
+        radix's default) or 3..6 per product."""
+        self._chk(lib().gi_set_split_format(self.h, radix, halfls_digits, corr_digits),
+                  "gi_set_split_format")
+
+    def split_format(self):
+        """(radix, half-ls digits, correlation digits) of the resident planes."""
+        r, h, c = ci(), ci(), ci()
+        self._chk(lib().gi_get_split_format(self.h, C.byref(r), C.byref(h), C.byref(c)),
+                  "gi_get_split_format")
+        return r.value, h.value, c.value
 
     def set_split_digits_each(self, halfls_digits, corr_digits):
         self._chk(lib().gi_set_split_digits_each(self.h, halfls_digits, corr_digits),

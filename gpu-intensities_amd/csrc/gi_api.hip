@@ -187,11 +187,41 @@ int prepare_split(gi_ctx *c)
     if (c->split_ready == c->split_key()) return GI_OK;
     HIPCHK(hipSetDevice(c->device));
     free_split(c);
+    // The format: an int32 sum of digit products is exact while
+    // S * (R/2)^2 * K < 2^31 for its length K: a class of the half linestrength
+    // (the sum of its K segments), a split-K part of the correlation. Radix 254
+    // (8-bit digits, 5 by default) when both products allow it, else 128
+    // (7-bit digits, 6 by default), unless the caller asked for one.
+    long long Kh = 0, Kc = 0;
+    for (const GroupPlan &G : c->groups)
+        for (const OutClass &oc : G.oc) Kh = std::max<long long>(Kh, oc.K);
+    for (const Block &b : c->blocks)
+        if (b.nrec > 0 && b.nsize > 0) {
+            const int Kp = (b.nsize + 63) / 64 * 64;
+            const int parts = std::min(kMaxKParts, std::max(1, Kp / 2048));
+            Kc = std::max<long long>(Kc, (long long)(Kp + 64 * parts - 1) / (64 * parts) * 64);
+        }
+    const auto fits = [](double R, int S, long long K) {
+        const long long dmax = (long long)(R / 2);
+        return (long long)S * dmax * dmax * K < (1LL << 31);
+    };
+    bool chosen = false;
+    for (const double R : {254.0, 128.0}) {
+        if (c->split_req_radix && c->split_req_radix != (int)R) continue;
+        const int def = R > 200 ? 5 : 6;
+        const int sh = c->split_req_Sh ? c->split_req_Sh : def;
+        const int sd = c->split_req_S ? c->split_req_S : def;
+        if (fits(R, sh, Kh) && fits(R, sd, Kc)) {
+            c->split_radix = R;
+            c->split_Sh = sh;
+            c->split_S = sd;
+            chosen = true;
+            break;
+        }
+    }
+    if (!chosen)
+        return fail(c, GI_EINVAL, "split mode: products too long for exact int32 digit sums");
     const int SD = c->split_S, SH = c->split_Sh, N = c->nvib;
-    // int32 sums of digit products stay exact while 4096 * S * K < 2^31
-    const long long kmax = (1LL << 31) / (4096LL * SD) - 64;
-    if (c->kpad_all > (1LL << 31) / (4096LL * SH) - 64)
-        return fail(c, GI_EINVAL, "split mode: vibrational basis too large");
     int rc;
     std::vector<int> vinv(c->kpad_all, -1);
     for (size_t k = 0; k < c->vcls_off.size(); k++)
@@ -201,18 +231,17 @@ int prepare_split(gi_ctx *c)
     c->muplane = 3LL * N * c->kpad_all;
     HIPCHK(hipMalloc((void **)&c->d_mud, (size_t)SH * c->muplane));
     HIPCHK(hipMalloc((void **)&c->d_mue, sizeof(int) * N));
-    launch_split_mu(c->stream, SH, N, c->kpad_all, c->d_mu_perm, d_vinv, c->d_mud, c->muplane,
+    launch_split_mu(c->stream, SH, c->split_radix, N, c->kpad_all, c->d_mu_perm, d_vinv, c->d_mud, c->muplane,
                     c->d_mue);
     HIPCHK(hipGetLastError());
     for (auto &b : c->blocks) {
         if (b.nrec <= 0 || b.nsize <= 0 || !b.d_vec) continue;
         b.Kp = (b.nsize + 63) / 64 * 64;
-        if (b.Kp > kmax) return fail(c, GI_EINVAL, "split mode: eigenvector block too long");
         b.uplane = (long long)b.nrec * b.Kp;
         HIPCHK(hipMalloc((void **)&b.d_ud, (size_t)SD * b.uplane));
         HIPCHK(hipMalloc((void **)&b.d_ue, sizeof(int) * b.nrec));
         HIPCHK(hipMalloc((void **)&b.d_umax, sizeof(double) * b.nrec));
-        launch_split_rows(c->stream, SD, b.d_vec, b.nsize, b.nrec, b.nsize, b.Kp, b.d_ud, b.uplane,
+        launch_split_rows(c->stream, SD, c->split_radix, b.d_vec, b.nsize, b.nrec, b.nsize, b.Kp, b.d_ud, b.uplane,
                           b.d_ue, b.d_umax);
         HIPCHK(hipGetLastError());
     }
@@ -741,18 +770,38 @@ int gi_kernel_stats(gi_ctx *c, int max, const char **names, double *total_ms, in
     return GI_OK;
 }
 
+int gi_set_split_format(gi_ctx *c, int radix, int halfls_digits, int corr_digits)
+{
+    const auto ok_digits = [](int d) { return d == 0 || (d >= 3 && d <= 6); };
+    if (!c || (radix != 0 && radix != 128 && radix != 254) || !ok_digits(halfls_digits) ||
+        !ok_digits(corr_digits))
+        return GI_EINVAL;
+    if (serving(c)) return GI_ESTATE;
+    if (radix != c->split_req_radix || halfls_digits != c->split_req_Sh ||
+        corr_digits != c->split_req_S) {
+        HIPCHK(hipSetDevice(c->device));
+        free_split(c);
+        c->split_req_radix = radix;
+        c->split_req_Sh = halfls_digits;
+        c->split_req_S = corr_digits;
+    }
+    return GI_OK;
+}
+
+int gi_get_split_format(gi_ctx *c, int *radix, int *halfls_digits, int *corr_digits)
+{
+    if (!c || !radix || !halfls_digits || !corr_digits) return GI_EINVAL;
+    *radix = (int)c->split_radix;
+    *halfls_digits = c->split_Sh;
+    *corr_digits = c->split_S;
+    return GI_OK;
+}
+
 int gi_set_split_digits_each(gi_ctx *c, int halfls_digits, int corr_digits)
 {
     if (!c || halfls_digits < 3 || halfls_digits > 6 || corr_digits < 3 || corr_digits > 6)
         return GI_EINVAL;
-    if (serving(c)) return GI_ESTATE;
-    if (halfls_digits != c->split_Sh || corr_digits != c->split_S) {
-        HIPCHK(hipSetDevice(c->device));
-        free_split(c);
-        c->split_Sh = halfls_digits;
-        c->split_S = corr_digits;
-    }
-    return GI_OK;
+    return gi_set_split_format(c, c->split_req_radix, halfls_digits, corr_digits);
 }
 
 int gi_set_split_digits(gi_ctx *c, int digits) { return gi_set_split_digits_each(c, digits, digits); }

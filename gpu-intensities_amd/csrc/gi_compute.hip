@@ -1105,10 +1105,10 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                 return rc;
             TL_BEGIN("build_g");
             if (glds)
-                launch_build_g_split_lds(S.stream, SH, (const GSplitJob *)d_gj,
+                launch_build_g_split_lds(S.stream, SH, c->split_radix, (const GSplitJob *)d_gj,
                                          (const int *)(d_gj + gj_bytes), (int)gj.size(), ubeg);
             else
-                launch_build_g_split(S.stream, SH, (const GSplitJob *)d_gj,
+                launch_build_g_split(S.stream, SH, c->split_radix, (const GSplitJob *)d_gj,
                                      (const int *)(d_gj + gj_bytes), (int)gj.size(), gt, g4);
             TL_END();
         }
@@ -1117,7 +1117,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                 return rc;
             TL_BEGIN("half_ls_split");
             P.work[14] += tt;
-            launch_halfls_split(S.stream, SH, N, c->kpad_all, c->d_mud, c->muplane, c->d_mue,
+            launch_halfls_split(S.stream, SH, c->split_radix, N, c->kpad_all, c->d_mud, c->muplane, c->d_mue,
                                 (const SplitHalfJob *)d_hj, (const int *)(d_hj + hj_bytes),
                                 (int)hj.size(), tt, c->tile_split != 0);
             TL_END();
@@ -1189,7 +1189,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
         if (!fjobs.empty() && fused) {
             if ((rc = send_jobs(c, S, d_fsj, fsjobs, sbegin, fscap)) != GI_OK) return rc;
             TL_BEGIN("fold_split");
-            launch_fold_split_jobs(S.stream, SD, (const FoldSplitJob *)d_fsj,
+            launch_fold_split_jobs(S.stream, SD, c->split_radix, (const FoldSplitJob *)d_fsj,
                                    (const int *)(d_fsj + fscap), (int)fsjobs.size(), srows,
                                    maxkp);
             TL_END();
@@ -1201,7 +1201,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             TL_END();
             if ((rc = send_jobs(c, S, d_srj, sjobs, sbegin, scap)) != GI_OK) return rc;
             TL_BEGIN("split_ht");
-            launch_split_rows_jobs(S.stream, SD, (const SplitRowsJob *)d_srj,
+            launch_split_rows_jobs(S.stream, SD, c->split_radix, (const SplitRowsJob *)d_srj,
                                    (const int *)(d_srj + scap), (int)sjobs.size(), srows);
             TL_END();
         }
@@ -1210,7 +1210,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                 return rc;
             TL_BEGIN("corr_split");
             P.work[15] += ct;
-            launch_corr_split(S.stream, SD, (const CorrSplitArgs *)d_cj, (const int *)(d_cj + cj_bytes),
+            launch_corr_split(S.stream, SD, c->split_radix, (const CorrSplitArgs *)d_cj, (const int *)(d_cj + cj_bytes),
                               (int)cj.size(), ct);
             TL_END();
         }

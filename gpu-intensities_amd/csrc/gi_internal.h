@@ -231,11 +231,15 @@ struct gi_ctx {
     int pipe_share = 50;             // percent of the remaining work per round but the last
     int tile_split = 0;              // k_halfls_split rows: 0 = 64, 1 = 128
 
-    // split mode: digits per operand (3..6) and the resident digit planes of mu
-    // digits of the correlation operands (eigenvector blocks, folded vectors) and
-    // of the half-ls operands (mu, G); split_ready = split_key() the planes were built with
-    int split_S = 6, split_Sh = 6, split_ready = 0;
-    int split_key() const { return split_S * 16 + split_Sh; }
+    // split mode. Requested format (0 = automatic): radix 128 or 254, digits of
+    // the correlation (eigenvector blocks, folded vectors) and of the half-ls
+    // (mu, G) products, 3..6. Format of the resident digit planes, chosen by
+    // prepare_split: split_radix, split_S, split_Sh. split_ready = split_key()
+    // of the request the planes were built for (0: none).
+    int split_req_radix = 0, split_req_S = 0, split_req_Sh = 0;
+    double split_radix = 254.0;
+    int split_S = 5, split_Sh = 5, split_ready = 0;
+    int split_key() const { return 1 + split_req_radix * 64 + split_req_S * 8 + split_req_Sh; }
     int8_t *d_mud = nullptr;
     long long muplane = 0;
     int *d_mue = nullptr;

@@ -178,23 +178,23 @@ struct CorrSplitArgs {
     double *ampx;
     long long ampx_stride;
 };
-void launch_split_rows(hipStream_t st, int S, const double *X, long long ld, int rows, int K,
+void launch_split_rows(hipStream_t st, int S, double R, const double *X, long long ld, int rows, int K,
                        int Kp, int8_t *D, long long plane, int *E, double *Mx);
-void launch_split_mu(hipStream_t st, int S, int N, int kpad_all, const double *mu,
+void launch_split_mu(hipStream_t st, int S, double R, int N, int kpad_all, const double *mu,
                      const int *vinv, int8_t *D, long long plane, int *E);
 // four: every job's record map has width <= 4 and gtiles counts 1024 rows
 // (k_build_g_split4); otherwise 256 rows per tile.
-void launch_build_g_split(hipStream_t st, int S, const GSplitJob *d_jobs, const int *d_begin,
+void launch_build_g_split(hipStream_t st, int S, double R, const GSplitJob *d_jobs, const int *d_begin,
                           int njobs, int ntiles, bool four);
 // One workgroup per unit: its record in LDS, then every G row of the unit
 // (maps of width <= 4, records of <= kRecLds doubles); d_begin = first unit of each job.
-void launch_build_g_split_lds(hipStream_t st, int S, const GSplitJob *d_jobs,
+void launch_build_g_split_lds(hipStream_t st, int S, double R, const GSplitJob *d_jobs,
                               const int *d_begin, int njobs, int nunits);
 // wide: 128-row tiles (SplitHalfJob.mt counts 128-row tiles), else 64-row.
-void launch_halfls_split(hipStream_t st, int S, int N, int kpad_all, const int8_t *mud,
+void launch_halfls_split(hipStream_t st, int S, double R, int N, int kpad_all, const int8_t *mud,
                          long long muplane, const int *mue, const SplitHalfJob *d_jobs,
                          const int *d_tbegin, int njobs, int ntiles, bool wide);
-void launch_corr_split(hipStream_t st, int S, const CorrSplitArgs *d_args, const int *d_cbegin,
+void launch_corr_split(hipStream_t st, int S, double R, const CorrSplitArgs *d_args, const int *d_cbegin,
                        int nchunks, int ntiles);
 
 // Tile configurations of the two matrix-core kernels: cfg bit 1 selects 128
@@ -281,7 +281,7 @@ struct SplitRowsJob {
     long long plane;
     int *E;
 };
-void launch_split_rows_jobs(hipStream_t st, int S, const SplitRowsJob *d_jobs,
+void launch_split_rows_jobs(hipStream_t st, int S, double R, const SplitRowsJob *d_jobs,
                             const int *d_begin, int njobs, int nrows);
 // Fold and digit split in one pass (split mode): one 1024-thread workgroup per
 // folded column, the column's values kept in registers, so the FP64 folded
@@ -295,7 +295,7 @@ struct FoldSplitJob {
     int *E;                          // exponent per column
 };
 // maxrows: the largest Kp of the jobs (<= kFoldSplitMax).
-void launch_fold_split_jobs(hipStream_t st, int S, const FoldSplitJob *d_jobs,
+void launch_fold_split_jobs(hipStream_t st, int S, double R, const FoldSplitJob *d_jobs,
                             const int *d_begin, int njobs, int ncols, int maxrows);
 // The intensity steps of every chunk of a round: blocks of a job = ceil(ncand / 256).
 void launch_intensity_jobs(hipStream_t st, const IntensArgs *d_jobs, const int *d_begin,

@@ -2,17 +2,22 @@
 // (v_mfma_i32_16x16x64_i8).
 //
 // Each operand row (A) or column (B) along K is scaled by its own power of
-// two and cut into S signed digits of 7 bits (Ozaki splitting):
-//   x = 2^e * sum_{i<S} d_i * 128^-(i+1) + r,   |d_i| <= 64,
-//   |r| <= 2^(e-1) * 128^-S,   with e such that max|x| <= 2^(e-1).
-// A digit product is at most 4096 in magnitude, so a K sum of them is exact
-// in int32 while 4096 * S * K < 2^31 (K < 104857 for S = 5). The product of a
-// row and a column is then
-//   sum_k a_k b_k = 2^(ea+eb) * sum_{l<S} 128^-(l+2) * sum_{i+j=l} sum_k da_ik db_jk
+// two and cut into S signed int8 digits of radix R (Ozaki splitting):
+//   x = 2^e * sum_{i<S} d_i * R^-(i+1) + r,   |d_i| <= R/2,
+//   |r| <= 2^(e-1) * R^-S,   with e such that max|x| <= 2^(e-1).
+// R = 254 gives digits in [-127, 127] (8 bits each, the whole signed int8
+// range but -128); R = 128 gives 7-bit digits in [-64, 64]. A digit product
+// is at most (R/2)^2 in magnitude, so a K sum of them is exact in int32 while
+// S * (R/2)^2 * K < 2^31 (R = 254, S = 5: K < 26,600; R = 128, S = 6:
+// K < 87,300); the context takes R = 254 when every product allows it. The
+// product of a row and a column is then
+//   sum_k a_k b_k = 2^(ea+eb) * sum_{l<S} R^-(l+2) * sum_{i+j=l} sum_k da_ik db_jk
 // plus the dropped terms (i + j >= S and the remainders), i.e. a relative
-// error of about 128^-S of the row and column maxima. The integer sums are
-// kept per level l and combined in FP64 in the epilogue; everything around
-// the products (G build, fold, intensity) stays FP64.
+// error of about R^-S of the row and column maxima, with S(S+1)/2 digit-pair
+// products (15 for R = 254, S = 5 against 21 for R = 128, S = 6 at about the
+// same accuracy). The integer sums are kept per level l and combined in FP64
+// in the epilogue; everything around the products (G build, fold, intensity)
+// stays FP64.
 //
 // Digit planes: plane q of an operand with R rows of Kp bytes is R*Kp bytes;
 // rows are padded with zeros to a multiple of 64 so that a K step never
@@ -44,13 +49,17 @@ __device__ inline int digit_exp(double m)
     return ex + 1;        // m <= 2^(e-1)
 }
 
+// S digits of radix R (128 or 254) of x / 2^e, |x| <= 2^(e-1): each step
+// scales the remainder (|r| <= 1/2) by R and rounds, so |d| <= R/2 (64 or 127,
+// inside int8) and the next remainder is again within 1/2. With R = 128 every
+// step is exact; with R = 254 the product r * R rounds by at most 2^-53 of it.
 template <int S>
-__device__ inline void to_digits(double x, int e, int (&d)[S])
+__device__ inline void to_digits(double x, int e, int (&d)[S], double R)
 {
     double r = ldexp(x, -e);
 #pragma unroll
     for (int i = 0; i < S; i++) {
-        const double t = r * 128.0;
+        const double t = r * R;
         const double v = rint(t);
         d[i] = (int)v;
         r = t - v;
@@ -75,7 +84,7 @@ __device__ inline double block_max(double m, double *red)
 template <int S>
 __device__ inline void split_row(const double *__restrict__ X, long long ld, int K, int Kp,
                                  int8_t *__restrict__ D, long long plane, int *__restrict__ E,
-                                 double *__restrict__ Mx, long long r, double *red)
+                                 double *__restrict__ Mx, long long r, double *red, double R)
 {
     const double *x = X + r * ld;
     double m = 0.0;
@@ -89,7 +98,7 @@ __device__ inline void split_row(const double *__restrict__ X, long long ld, int
     int8_t *d = D + r * Kp;
     for (int k = threadIdx.x; k < Kp; k += 256) {
         int dg[S];
-        to_digits<S>(k < K ? x[k] : 0.0, e, dg);
+        to_digits<S>(k < K ? x[k] : 0.0, e, dg, R);
 #pragma unroll
         for (int q = 0; q < S; q++) d[q * plane + k] = (int8_t)dg[q];
     }
@@ -99,16 +108,16 @@ template <int S>
 __global__ __launch_bounds__(256) void k_split_rows(const double *__restrict__ X, long long ld,
                                                     int K, int Kp, int8_t *__restrict__ D,
                                                     long long plane, int *__restrict__ E,
-                                                    double *__restrict__ Mx)
+                                                    double *__restrict__ Mx, double R)
 {
     __shared__ double red[256];
-    split_row<S>(X, ld, K, Kp, D, plane, E, Mx, blockIdx.x, red);
+    split_row<S>(X, ld, K, Kp, D, plane, E, Mx, blockIdx.x, red, R);
 }
 
 template <int S>
 __global__ __launch_bounds__(256) void k_split_rows_jobs(const SplitRowsJob *__restrict__ jobs,
                                                          const int *__restrict__ begin,
-                                                         int njobs)
+                                                         int njobs, double R)
 {
     __shared__ double red[256];
     int lo = 0, hi = njobs - 1;
@@ -117,7 +126,8 @@ __global__ __launch_bounds__(256) void k_split_rows_jobs(const SplitRowsJob *__r
         if (begin[mid] <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
     }
     const SplitRowsJob &J = jobs[lo];
-    split_row<S>(J.X, J.ld, J.K, J.Kp, J.D, J.plane, J.E, nullptr, blockIdx.x - begin[lo], red);
+    split_row<S>(J.X, J.ld, J.K, J.Kp, J.D, J.plane, J.E, nullptr, blockIdx.x - begin[lo], red,
+                 R);
 }
 
 // mu (permuted, mu_a(vF, vI) at vF + vI*N + a*N^2) to digit planes with row
@@ -128,7 +138,7 @@ __global__ __launch_bounds__(256) void k_split_mu(int N, int kpad_all,
                                                   const double *__restrict__ mu,
                                                   const int *__restrict__ vinv,
                                                   int8_t *__restrict__ D, long long plane,
-                                                  int *__restrict__ E)
+                                                  int *__restrict__ E, double R)
 {
     __shared__ double red[256];
     const int vF = blockIdx.x;
@@ -145,7 +155,7 @@ __global__ __launch_bounds__(256) void k_split_mu(int N, int kpad_all,
         const int a = i / kpad_all, p = i - a * kpad_all;
         const int vI = vinv[p];
         int dg[S];
-        to_digits<S>(vI >= 0 ? mu[vF + (long long)vI * N + a * NN] : 0.0, e, dg);
+        to_digits<S>(vI >= 0 ? mu[vF + (long long)vI * N + a * NN] : 0.0, e, dg, R);
         int8_t *d = D + ((long long)a * N + vF) * kpad_all + p;
 #pragma unroll
         for (int q = 0; q < S; q++) d[q * plane] = (int8_t)dg[q];
@@ -172,7 +182,7 @@ static inline int xcd_grid_s(int ntiles) { return (ntiles + 7) & ~7; }
 template <int S>
 __global__ __launch_bounds__(256) void k_build_g_split(const GSplitJob *__restrict__ jobs,
                                                        const int *__restrict__ begin, int njobs,
-                                                       int ntiles)
+                                                       int ntiles, double R)
 {
     const int tile = xcd_tile_s(blockIdx.x, gridDim.x);
     if (tile >= ntiles) return;
@@ -212,7 +222,7 @@ __global__ __launch_bounds__(256) void k_build_g_split(const GSplitJob *__restri
         for (int w = 0; w < kGMapMaxW; w++)
             if (w < J.W) v += wv[w] * rec[ix[w]];
         int dg[S];
-        to_digits<S>(v, e, dg);
+        to_digits<S>(v, e, dg, R);
         int8_t *dst = J.D + (size_t)u * J.GtotB + g;
 #pragma unroll
         for (int q = 0; q < S; q++) dst[q * J.plane] = (int8_t)dg[q];
@@ -226,7 +236,7 @@ __global__ __launch_bounds__(256) void k_build_g_split(const GSplitJob *__restri
 template <int S>
 __global__ __launch_bounds__(256) void k_build_g_split4(const GSplitJob *__restrict__ jobs,
                                                         const int *__restrict__ begin, int njobs,
-                                                        int ntiles)
+                                                        int ntiles, double R)
 {
     const int tile = xcd_tile_s(blockIdx.x, gridDim.x);
     if (tile >= ntiles) return;
@@ -273,7 +283,7 @@ __global__ __launch_bounds__(256) void k_build_g_split4(const GSplitJob *__restr
             }
         int dg[4][S];
 #pragma unroll
-        for (int r = 0; r < 4; r++) to_digits<S>(v[r], e, dg[r]);
+        for (int r = 0; r < 4; r++) to_digits<S>(v[r], e, dg[r], R);
         int8_t *dst = J.D + (size_t)u * J.GtotB + g;
 #pragma unroll
         for (int q = 0; q < S; q++)
@@ -289,7 +299,7 @@ __global__ __launch_bounds__(256) void k_build_g_split4(const GSplitJob *__restr
 // (maps of width <= WMAX).
 template <int S, int NC, int WMAX>
 __device__ inline void g_split_passes(const GSplitJob &J, int d, int u, double um,
-                                      const double *rec, const int (*cls)[kClsLds])
+                                      const double *rec, const int (*cls)[kClsLds], double R)
 {
     int8_t *D = J.D + (size_t)u * J.GtotB;
     const int W = J.W, Gt = J.GtotB;
@@ -334,7 +344,7 @@ __device__ inline void g_split_passes(const GSplitJob &J, int d, int u, double u
                 }
             int dg[4][S];
 #pragma unroll
-            for (int r = 0; r < 4; r++) to_digits<S>(v[r], e, dg[r]);
+            for (int r = 0; r < 4; r++) to_digits<S>(v[r], e, dg[r], R);
 #pragma unroll
             for (int q = 0; q < S; q++)
                 *(int *)(D + q * J.plane + g) = (dg[0][q] & 255) | ((dg[1][q] & 255) << 8) |
@@ -351,7 +361,7 @@ __device__ inline void g_split_passes(const GSplitJob &J, int d, int u, double u
 template <int S>
 __global__ __launch_bounds__(1024) void k_build_g_split_lds(const GSplitJob *__restrict__ jobs,
                                                             const int *__restrict__ ubegin,
-                                                            int njobs)
+                                                            int njobs, double R)
 {
     __shared__ double rec[kRecLds];
     __shared__ int cls[3][kClsLds];  // per output class: cbaseB, kpad, eoff
@@ -369,14 +379,14 @@ __global__ __launch_bounds__(1024) void k_build_g_split_lds(const GSplitJob *__r
     __syncthreads();
     const int d = J.uideg[u];
     const double um = J.umax[J.urec[u]];
-    if (J.W <= 2) g_split_passes<S, 2, 2>(J, d, u, um, rec, cls);
-    else g_split_passes<S, 1, 4>(J, d, u, um, rec, cls);
+    if (J.W <= 2) g_split_passes<S, 2, 2>(J, d, u, um, rec, cls, R);
+    else g_split_passes<S, 1, 4>(J, d, u, um, rec, cls, R);
 }
 
 template <int S, int PER>
 __global__ __launch_bounds__(1024) void k_fold_split_jobs(const FoldSplitJob *__restrict__ jobs,
                                                           const int *__restrict__ begin,
-                                                          int njobs)
+                                                          int njobs, double R)
 {
     __shared__ double red[16];
     const int ji = find_tile_job(begin, njobs, blockIdx.x);
@@ -439,7 +449,7 @@ __global__ __launch_bounds__(1024) void k_fold_split_jobs(const FoldSplitJob *__
         const int t = (int)threadIdx.x + k * 1024;
         if (t < J.Kp) {
             int dg[S];
-            to_digits<S>(t < F.nsizeF ? v[k] : 0.0, e, dg);
+            to_digits<S>(t < F.nsizeF ? v[k] : 0.0, e, dg, R);
 #pragma unroll
             for (int q = 0; q < S; q++) d[q * J.plane + t] = (int8_t)dg[q];
         }
@@ -492,12 +502,18 @@ __device__ inline void oz_step(const int8_t *__restrict__ As, const int8_t *__re
     }
 }
 
+// sum over levels l of L_l * R^-(l+2), smallest first (rinv = 1/R; exact
+// powers of two for R = 128)
 template <int S>
-__device__ inline double oz_value(const v4i (&acc)[S], int r)
+__device__ inline double oz_value(const v4i (&acc)[S], int r, double rinv)
 {
+    double sc[S];
+    sc[0] = rinv * rinv;
+#pragma unroll
+    for (int l = 1; l < S; l++) sc[l] = sc[l - 1] * rinv;
     double v = 0.0;
 #pragma unroll
-    for (int l = S - 1; l >= 0; l--) v += ldexp((double)acc[l][r], -7 * (l + 2));
+    for (int l = S - 1; l >= 0; l--) v += (double)acc[l][r] * sc[l];
     return v;
 }
 
@@ -510,7 +526,7 @@ __global__ __launch_bounds__(256) void k_halfls_split(int N, int kpad_all,
                                                       const int *__restrict__ mue,
                                                       const SplitHalfJob *__restrict__ jobs,
                                                       const int *__restrict__ tbegin, int njobs,
-                                                      int ntiles)
+                                                      int ntiles, double R)
 {
     __shared__ __attribute__((aligned(16))) int8_t As[S * 64 * kRowB];
     __shared__ __attribute__((aligned(16))) int8_t Bs[S * 64 * kRowB];
@@ -581,7 +597,7 @@ __global__ __launch_bounds__(256) void k_halfls_split(int N, int kpad_all,
                 const int u = nn / cd.ncols, jj = nn - u * cd.ncols;
                 const int ex = mue[cd.off + m] + J.Ge[(size_t)u * J.EGtot + cd.eoff + jj];
                 J.H[(size_t)u * J.Htot + cd.hbase + (size_t)jj * cd.n + m] =
-                    ldexp(oz_value<S>(acc[x][y], r), ex);
+                    ldexp(oz_value<S>(acc[x][y], r, 1.0 / R), ex);
             }
 }
 
@@ -623,7 +639,7 @@ template <int S>
 __global__ __launch_bounds__(256, 1) void k_halfls_split_wide(
     int N, int kpad_all, const int8_t *__restrict__ mud, long long muplane,
     const int *__restrict__ mue, const SplitHalfJob *__restrict__ jobs,
-    const int *__restrict__ tbegin, int njobs, int ntiles)
+    const int *__restrict__ tbegin, int njobs, int ntiles, double R)
 {
     __shared__ __attribute__((aligned(16))) int8_t As[S * 128 * kRowB];
     __shared__ __attribute__((aligned(16))) int8_t Bs[S * 64 * kRowB];
@@ -696,7 +712,7 @@ __global__ __launch_bounds__(256, 1) void k_halfls_split_wide(
                 const int u = nn / cd.ncols, jj = nn - u * cd.ncols;
                 const int ex = mue[cd.off + m] + J.Ge[(size_t)u * J.EGtot + cd.eoff + jj];
                 J.H[(size_t)u * J.Htot + cd.hbase + (size_t)jj * cd.n + m] =
-                    ldexp(oz_value<S>(acc[x][y], r), ex);
+                    ldexp(oz_value<S>(acc[x][y], r, 1.0 / R), ex);
             }
 }
 
@@ -705,7 +721,7 @@ __global__ __launch_bounds__(256, 1) void k_halfls_split_wide(
 template <int S>
 __global__ __launch_bounds__(256) void k_corr_split(const CorrSplitArgs *__restrict__ args,
                                                     const int *__restrict__ cbegin, int nchunks,
-                                                    int ntiles)
+                                                    int ntiles, double R)
 {
     __shared__ __attribute__((aligned(16))) int8_t As[S * 64 * kRowB];
     __shared__ __attribute__((aligned(16))) int8_t Bs[S * 64 * kRowB];
@@ -786,7 +802,7 @@ __global__ __launch_bounds__(256) void k_corr_split(const CorrSplitArgs *__restr
                 const int idegF = cb / a.ndegI, idegI = cb - idegF * a.ndegI;
                 const int ex = a.Ue[a.blk_col[f]] + a.He[nn];
                 dst[(long long)(a.amp_m * a.amp_m) * o + idegI + a.amp_m * idegF] =
-                    ldexp(oz_value<S>(acc[x][y], r), ex);
+                    ldexp(oz_value<S>(acc[x][y], r, 1.0 / R), ex);
             }
 }
 
@@ -800,92 +816,92 @@ __global__ __launch_bounds__(256) void k_corr_split(const CorrSplitArgs *__restr
     default: CALL(6); break;                                                              \
     }
 
-void launch_split_rows(hipStream_t st, int S, const double *X, long long ld, int rows, int K,
+void launch_split_rows(hipStream_t st, int S, double R, const double *X, long long ld, int rows, int K,
                        int Kp, int8_t *D, long long plane, int *E, double *Mx)
 {
     if (rows <= 0) return;
-#define L_(s) hipLaunchKernelGGL(k_split_rows<s>, dim3(rows), dim3(256), 0, st, X, ld, K, Kp, D, plane, E, Mx)
+#define L_(s) hipLaunchKernelGGL(k_split_rows<s>, dim3(rows), dim3(256), 0, st, X, ld, K, Kp, D, plane, E, Mx, R)
     GI_SPLIT_DISPATCH(S, L_)
 #undef L_
 }
 
-void launch_split_rows_jobs(hipStream_t st, int S, const SplitRowsJob *d_jobs,
+void launch_split_rows_jobs(hipStream_t st, int S, double R, const SplitRowsJob *d_jobs,
                             const int *d_begin, int njobs, int nrows)
 {
     if (njobs <= 0 || nrows <= 0) return;
-#define L_(s) hipLaunchKernelGGL(k_split_rows_jobs<s>, dim3(nrows), dim3(256), 0, st, d_jobs, d_begin, njobs)
+#define L_(s) hipLaunchKernelGGL(k_split_rows_jobs<s>, dim3(nrows), dim3(256), 0, st, d_jobs, d_begin, njobs, R)
     GI_SPLIT_DISPATCH(S, L_)
 #undef L_
 }
 
-void launch_fold_split_jobs(hipStream_t st, int S, const FoldSplitJob *d_jobs,
+void launch_fold_split_jobs(hipStream_t st, int S, double R, const FoldSplitJob *d_jobs,
                             const int *d_begin, int njobs, int ncols, int maxrows)
 {
     if (njobs <= 0 || ncols <= 0) return;
     if (maxrows <= 16384) {
-#define L_(s) hipLaunchKernelGGL((k_fold_split_jobs<s, 16>), dim3(ncols), dim3(1024), 0, st, d_jobs, d_begin, njobs)
+#define L_(s) hipLaunchKernelGGL((k_fold_split_jobs<s, 16>), dim3(ncols), dim3(1024), 0, st, d_jobs, d_begin, njobs, R)
         GI_SPLIT_DISPATCH(S, L_)
 #undef L_
         return;
     }
-#define L_(s) hipLaunchKernelGGL((k_fold_split_jobs<s, kFoldSplitMax / 1024>), dim3(ncols), dim3(1024), 0, st, d_jobs, d_begin, njobs)
+#define L_(s) hipLaunchKernelGGL((k_fold_split_jobs<s, kFoldSplitMax / 1024>), dim3(ncols), dim3(1024), 0, st, d_jobs, d_begin, njobs, R)
     GI_SPLIT_DISPATCH(S, L_)
 #undef L_
 }
 
-void launch_split_mu(hipStream_t st, int S, int N, int kpad_all, const double *mu,
+void launch_split_mu(hipStream_t st, int S, double R, int N, int kpad_all, const double *mu,
                      const int *vinv, int8_t *D, long long plane, int *E)
 {
-#define L_(s) hipLaunchKernelGGL(k_split_mu<s>, dim3(N), dim3(256), 0, st, N, kpad_all, mu, vinv, D, plane, E)
+#define L_(s) hipLaunchKernelGGL(k_split_mu<s>, dim3(N), dim3(256), 0, st, N, kpad_all, mu, vinv, D, plane, E, R)
     GI_SPLIT_DISPATCH(S, L_)
 #undef L_
 }
 
-void launch_build_g_split(hipStream_t st, int S, const GSplitJob *d_jobs, const int *d_begin,
+void launch_build_g_split(hipStream_t st, int S, double R, const GSplitJob *d_jobs, const int *d_begin,
                           int njobs, int ntiles, bool four)
 {
     if (njobs <= 0 || ntiles <= 0) return;
     if (four) {
-#define L_(s) hipLaunchKernelGGL(k_build_g_split4<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, d_jobs, d_begin, njobs, ntiles)
+#define L_(s) hipLaunchKernelGGL(k_build_g_split4<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, d_jobs, d_begin, njobs, ntiles, R)
         GI_SPLIT_DISPATCH(S, L_)
 #undef L_
         return;
     }
-#define L_(s) hipLaunchKernelGGL(k_build_g_split<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, d_jobs, d_begin, njobs, ntiles)
+#define L_(s) hipLaunchKernelGGL(k_build_g_split<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, d_jobs, d_begin, njobs, ntiles, R)
     GI_SPLIT_DISPATCH(S, L_)
 #undef L_
 }
 
-void launch_build_g_split_lds(hipStream_t st, int S, const GSplitJob *d_jobs,
+void launch_build_g_split_lds(hipStream_t st, int S, double R, const GSplitJob *d_jobs,
                               const int *d_begin, int njobs, int nunits)
 {
     if (njobs <= 0 || nunits <= 0) return;
-#define L_(s) hipLaunchKernelGGL(k_build_g_split_lds<s>, dim3(nunits), dim3(1024), 0, st, d_jobs, d_begin, njobs)
+#define L_(s) hipLaunchKernelGGL(k_build_g_split_lds<s>, dim3(nunits), dim3(1024), 0, st, d_jobs, d_begin, njobs, R)
     GI_SPLIT_DISPATCH(S, L_)
 #undef L_
 }
 
-void launch_halfls_split(hipStream_t st, int S, int N, int kpad_all, const int8_t *mud,
+void launch_halfls_split(hipStream_t st, int S, double R, int N, int kpad_all, const int8_t *mud,
                          long long muplane, const int *mue, const SplitHalfJob *d_jobs,
                          const int *d_tbegin, int njobs, int ntiles, bool wide)
 {
     if (njobs <= 0 || ntiles <= 0) return;
     if (wide) {
-#define L_(s) hipLaunchKernelGGL(k_halfls_split_wide<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, N, kpad_all, mud, muplane, mue, d_jobs, d_tbegin, njobs, ntiles)
+#define L_(s) hipLaunchKernelGGL(k_halfls_split_wide<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, N, kpad_all, mud, muplane, mue, d_jobs, d_tbegin, njobs, ntiles, R)
         GI_SPLIT_DISPATCH(S, L_)
 #undef L_
         return;
     }
-#define L_(s) hipLaunchKernelGGL(k_halfls_split<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, N, kpad_all, mud, muplane, mue, d_jobs, d_tbegin, njobs, ntiles)
+#define L_(s) hipLaunchKernelGGL(k_halfls_split<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, N, kpad_all, mud, muplane, mue, d_jobs, d_tbegin, njobs, ntiles, R)
     GI_SPLIT_DISPATCH(S, L_)
 #undef L_
 }
 
-void launch_corr_split(hipStream_t st, int S, const CorrSplitArgs *d_args, const int *d_cbegin,
+void launch_corr_split(hipStream_t st, int S, double R, const CorrSplitArgs *d_args, const int *d_cbegin,
                        int nchunks, int ntiles)
 {
     if (nchunks <= 0 || ntiles <= 0) return;
-#define L_(s) hipLaunchKernelGGL(k_corr_split<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, d_args, d_cbegin, nchunks, ntiles)
+#define L_(s) hipLaunchKernelGGL(k_corr_split<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, d_args, d_cbegin, nchunks, ntiles, R)
     GI_SPLIT_DISPATCH(S, L_)
 #undef L_
 }

@@ -39,7 +39,7 @@ typedef enum gi_mode {
     GI_MODE_FP64 = 0,          /* plain FP64: one GEMV-form product per lower level */
     GI_MODE_FP64_BATCHED = 1,  /* FP64 matrix-core products over batches of lower levels */
     GI_MODE_SPLIT = 2          /* batched, both products as split int8 digits on the
-                                  matrix cores (see gi_set_split_digits)          */
+                                  matrix cores (see gi_set_split_format)          */
 } gi_mode;
 
 /* Basis of one J: the TO_bset_contrT fields (trove_objects.h:49-72) read by
@@ -214,17 +214,30 @@ int gi_work_stats(gi_ctx *ctx, double *out, int n);
 
 /* GI_MODE_SPLIT (no reference counterpart): every operand row or column of
  * the two products is scaled by a power of two and cut into `digits` signed
- * 7-bit digits; digit products are summed exactly in int32 on the int8 matrix
- * cores and combined in FP64, so the relative error of a product is about
- * 128^-digits of its operands' row/column maxima (6 digits: ~2^-41).
- * Energies, Boltzmann factors, Q, the fold and the intensity step stay FP64.
- * digits: 3..6 (default 6, which keeps S within 1e-6 relative of FP64 for
- * lines down to 1e-6 of the strongest). gi_prepare_split builds the resident digit
- * planes of mu and of every eigenvector block (digits bytes per element);
- * gi_compute does it on first use, the request server needs it beforehand. */
+ * int8 digits of radix R; the digit-pair products (digits*(digits+1)/2 per
+ * multiply-add) are summed exactly in int32 on the int8 matrix cores and
+ * combined in FP64, so the relative error of a product is about R^-digits of
+ * its operands' row/column maxima. R = 254 (digits in [-127, 127], default 5
+ * digits: ~2^-40) or R = 128 (digits in [-64, 64], default 6: ~2^-42); a sum is
+ * exact while digits * (R/2)^2 * K < 2^31 for the product length K (a class of
+ * the half linestrength, a split-K part of the correlation), and the format is
+ * chosen at gi_prepare_split: R = 254 when both products allow it, else 128.
+ * Either default keeps S within 1e-6 relative of FP64 for lines down to 1e-6 of
+ * the strongest. Energies, Boltzmann factors, Q, the fold and the intensity
+ * step stay FP64. gi_prepare_split builds the resident digit planes of mu and
+ * of every eigenvector block (digits bytes per element); gi_compute does it on
+ * first use, the request server needs it beforehand. */
+/* radix: 0 (automatic), 128 or 254; digits per product: 0 (the default of the
+ * radix) or 3..6. GI_EINVAL from gi_prepare_split when no allowed format keeps
+ * the sums exact. */
+int gi_set_split_format(gi_ctx *ctx, int radix, int halfls_digits, int corr_digits);
+/* The format of the resident digit planes (that of the last gi_prepare_split;
+ * before one, radix 254 with 5 digits). */
+int gi_get_split_format(gi_ctx *ctx, int *radix, int *halfls_digits, int *corr_digits);
+/* gi_set_split_format with the radix request kept: `digits` (3..6) for both
+ * products, or one count per product, the half linestrength (mu and G) and the
+ * correlation (eigenvector blocks and folded vectors). */
 int gi_set_split_digits(gi_ctx *ctx, int digits);
-/* The same with a digit count per product: the half linestrength (mu and G)
- * and the correlation (eigenvector blocks and folded vectors), each 3..6. */
 int gi_set_split_digits_each(gi_ctx *ctx, int halfls_digits, int corr_digits);
 int gi_prepare_split(gi_ctx *ctx);
 

@@ -2,8 +2,9 @@
 
 Acceptance: the transition set equals the oracle's (thresholds 0, so every
 candidate in the window is emitted), and over the lines with
-S >= 1e-6 * S_max the relative error of S is at most 1e-6 with the default 6
-digits. Energies, nu and the intensity step are FP64, so nu is bit-identical
+S >= 1e-6 * S_max the relative error of S is at most 1e-6 with the default
+format (5 digits of radix 254, or 6 of radix 128 where products are too long
+for the former). Energies, nu and the intensity step are FP64, so nu is bit-identical
 and I / A_einst follow S to the same relative error. The error report is
 printed (pytest -s) for 296 K and 1500 K.
 """
@@ -87,7 +88,8 @@ def test_split_larger_many_rounds(engine, monkeypatch):
 
 
 def test_split_digits_trade_accuracy(engine):
-    """Fewer digits, larger error: 3 digits (~2^-20) is visibly worse than 6."""
+    """Fewer digits, larger error (radix 254: about 8 bits a digit); the
+    default 5 digits are within the bar."""
     job = synth.make_job("C2V", nvib=80, jvals=(1, 2), seed=9)
     engine.load(job)
     ref = oracle_py.OracleJob(job).run()
@@ -96,39 +98,70 @@ def test_split_digits_trade_accuracy(engine):
         for d in (3, 4, 5, 6):
             engine.set_split_digits(d)
             errs[d] = split_error(engine.compute(mode=binding.GI_MODE_SPLIT), ref)["max"]
+            assert engine.split_format() == (254, d, d)
     finally:
-        engine.set_split_digits(6)
-    print(f"split max rel S error by digits: {errs}")
-    assert errs[3] > errs[5] > errs[6] and errs[4] > errs[6]
-    assert errs[6] <= TOL_S
+        engine.set_split_format(0, 0, 0)
+    print(f"split max rel S error by digits (radix 254): {errs}")
+    assert errs[3] > errs[4] > errs[5] > errs[6]
+    assert errs[5] <= TOL_S
 
 
 def test_split_digits_per_product(engine):
-    """gi_set_split_digits_each: (6, 6) is the default bit for bit; 5 digits in
-    one product lands between the 5-digit and 6-digit errors of both."""
+    """gi_set_split_digits_each: (5, 5) is the default bit for bit; 4 digits in
+    one product lands between the 4-digit and 5-digit errors of both."""
     job = synth.make_job("C2V", nvib=80, jvals=(1, 2), seed=9)
     engine.load(job)
     ref = oracle_py.OracleJob(job).run()
     try:
         base = {k: v.copy() for k, v in engine.compute(mode=binding.GI_MODE_SPLIT).items()}
-        engine.set_split_digits_each(6, 6)
+        assert engine.split_format() == (254, 5, 5)
+        engine.set_split_digits_each(5, 5)
         same = engine.compute(mode=binding.GI_MODE_SPLIT)
         for k in base:
             assert np.array_equal(base[k], same[k]), k
-        e66 = split_error(base, ref)["max"]
-        engine.set_split_digits_each(5, 6)
-        e56 = split_error(engine.compute(mode=binding.GI_MODE_SPLIT), ref)["max"]
-        engine.set_split_digits_each(6, 5)
-        e65 = split_error(engine.compute(mode=binding.GI_MODE_SPLIT), ref)["max"]
-        engine.set_split_digits(5)
-        e55 = split_error(engine.compute(mode=binding.GI_MODE_SPLIT), ref)["max"]
+        e55 = split_error(base, ref)["max"]
+        engine.set_split_digits_each(4, 5)
+        e45 = split_error(engine.compute(mode=binding.GI_MODE_SPLIT), ref)["max"]
+        engine.set_split_digits_each(5, 4)
+        e54 = split_error(engine.compute(mode=binding.GI_MODE_SPLIT), ref)["max"]
+        engine.set_split_digits(4)
+        e44 = split_error(engine.compute(mode=binding.GI_MODE_SPLIT), ref)["max"]
     finally:
-        engine.set_split_digits(6)
-    print(f"split max rel S error (halfls, corr digits): 66 {e66:.2e} 56 {e56:.2e} "
-          f"65 {e65:.2e} 55 {e55:.2e}")
-    assert e66 < min(e56, e65) and max(e56, e65) <= 4 * e55
+        engine.set_split_format(0, 0, 0)
+    print(f"split max rel S error (halfls, corr digits): 55 {e55:.2e} 45 {e45:.2e} "
+          f"54 {e54:.2e} 44 {e44:.2e}")
+    assert e55 < min(e45, e54) and max(e45, e54) <= 4 * e44
     with pytest.raises(binding.GIError):
         engine.set_split_digits_each(7, 6)
+
+
+def test_split_format(engine):
+    """gi_set_split_format: radix 128 with its default 6 digits (the format
+    taken when a product is too long for radix 254) is within the bar, equal
+    bit for bit to asking for its digits explicitly; bad requests are refused
+    and leave the format alone."""
+    job = synth.make_job("C3V", nvib=30, jvals=(2, 3), seed=5, reduced=1)
+    engine.load(job)
+    ref = oracle_py.OracleJob(job).run()
+    try:
+        e254 = split_error(engine.compute(mode=binding.GI_MODE_SPLIT), ref)["max"]
+        assert engine.split_format() == (254, 5, 5)
+        engine.set_split_format(128)
+        r128 = {k: v.copy() for k, v in engine.compute(mode=binding.GI_MODE_SPLIT).items()}
+        assert engine.split_format() == (128, 6, 6)
+        e128 = split_error(r128, ref)["max"]
+        engine.set_split_format(128, 6, 6)
+        again = engine.compute(mode=binding.GI_MODE_SPLIT)
+        for k in r128:
+            assert np.array_equal(r128[k], again[k]), k
+        for bad in ((100, 0, 0), (254, 7, 0), (0, 0, 2)):
+            with pytest.raises(binding.GIError):
+                engine.set_split_format(*bad)
+        assert engine.split_format() == (128, 6, 6)
+    finally:
+        engine.set_split_format(0, 0, 0)
+    print(f"split max rel S error: radix 254 x5 {e254:.2e}, radix 128 x6 {e128:.2e}")
+    assert e254 <= TOL_S and e128 <= TOL_S
 
 
 def test_fused_fold_split_equals_two_kernels(engine, monkeypatch):
