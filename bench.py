@@ -12,9 +12,11 @@ linestrengths, correlation, intensity, filter, compaction and the copy of
 the emitted lines to pinned host memory). Inputs are resident in HBM when
 the timed region starts.
 
-After the headline workload (reported as `value`), the same measurement on
+After the headline workload (reported as `value`), GI_MODE_FP64 on it again
+with mu held in host memory and streamed in parts (`mu_streamed`), then the
+same measurement on
 one context holding C2v J = 0..20 (Nvib = 2000: 84 blocks of 500..20,500
-functions, 99 GB of FP64 eigenvectors plus 74 GB of split digit planes, every
+functions, 99 GB of FP64 eigenvectors plus 62 GB of split digit planes, every
 |dJ| <= 1 pair in the same launches; `multi_j`), the Poisson request sweep
 against those resident blocks (`serving`), and one context holding C3v(M)
 J = 0..15 (Nvib = 3000, A1/A2/E, blocks up to 30,742 functions, 65 GB of FP64
@@ -463,6 +465,43 @@ def sample_parity(eng, binding, ref, sample, bufs, mode_names):
     return out
 
 
+def mu_stream_section(eng, binding, job, dev, bufs, parts, steps):
+    """GI_MODE_FP64 with mu kept in host memory and copied to the device in
+    `parts` parts of vF rows for every batch of lower levels (gi_set_mu_budget,
+    the reference's split dipole); the lines must be the bits of resident mu."""
+    mode = binding.GI_MODE_FP64
+    ref = {k: v.copy() for k, v in eng.compute(bufs=bufs, mode=mode).items()}
+    row = 24 * job.nvib
+    budget = -(-job.nvib // parts) * row
+    eng.load(job, device_vectors=dev, mu_budget=budget)
+    try:
+        got = eng.compute(bufs=bufs, mode=mode)
+        same = all(np.array_equal(ref[k], got[k]) for k in ref)
+        ts = []
+        for _ in range(steps):
+            t = time.perf_counter()
+            eng.compute(bufs=bufs, mode=mode)
+            ts.append(time.perf_counter() - t)
+        ks = eng.kernel_stats()
+        ms = 1e3 * sum(ts) / len(ts)
+        cp = ks.get("mu_copy", {"total_ms": 0.0, "launches": 0})
+        np_rows = -(-job.nvib // parts)
+        part_bytes = 3 * job.nvib * np_rows * 8
+        out = dict(parts=eng.mu_parts(), budget_bytes=budget, part_bytes=part_bytes,
+                   ms_per_step=ms, value=len(ref["S"]) / (ms / 1e3), transitions=len(ref["S"]),
+                   bitwise_equal_resident=bool(same), mu_copies=int(cp["launches"]),
+                   mu_copy_ms=cp["total_ms"],
+                   mu_copy_GBps=(cp["launches"] * part_bytes / (cp["total_ms"] / 1e3) / 1e9
+                                 if cp["total_ms"] > 0 else None),
+                   kernels={k: {"total_ms": round(v["total_ms"], 4), "launches": v["launches"]}
+                            for k, v in ks.items()})
+        log(f"mu streamed in {parts} parts: {ms:.2f} ms/step, bitwise equal {same}, "
+            f"{out['mu_copies']} part copies, {out['mu_copy_GBps']} GB/s")
+        return out
+    finally:
+        eng.load(job, device_vectors=dev)
+
+
 def mode_rooflines(modes, nvib):
     """Roofline of the two product kernels of each mode: HBM GB/s for the plain
     FP64 (GEMV-form) kernels, matrix-core utilisation for the batched ones."""
@@ -594,6 +633,9 @@ def main():
                     help="offered duration of each serving rate")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device copy (PMC calibration)")
+    ap.add_argument("--mu-parts", type=int, default=4,
+                    help="also time GI_MODE_FP64 with mu streamed from host memory in this "
+                         "many parts (0 to skip)")
     ap.add_argument("--tuning", default=None,
                     help="saved gi_autotune table for the headline workload (instead of tuning)")
     args = ap.parse_args()
@@ -695,6 +737,9 @@ def main():
                                                          nthr)
         log(f"parity {parity}")
         job.vectors.clear()
+    mu_streamed = None
+    if "fp64" in mode_names and args.mu_parts > 1:
+        mu_streamed = mu_stream_section(eng, binding, job, dev, bufs, args.mu_parts, 2)
     eng.close()
     del dev, bufs
     torch.cuda.empty_cache()
@@ -732,6 +777,7 @@ def main():
         "modes": modes,
         "kernel_rooflines": roofs,
         "parity": parity,
+        "mu_streamed": mu_streamed,
         "multi_j": multi_j,
         "serving": serving,
         "c3v_multi_j": c3v,

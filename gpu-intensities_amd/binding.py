@@ -34,7 +34,8 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_params", "gi_uplo
            "gi_serve_stop", "gi_set_tiles", "gi_get_tiles", "gi_set_split_digits",
            "gi_prepare_split", "gi_set_jvals", "gi_set_tuning", "gi_get_tuning",
            "gi_autotune", "gi_set_split_digits_each", "gi_sweep_create", "gi_sweep_run", "gi_sweep_candidates",
-           "gi_sweep_destroy", "gi_amp_width", "gi_set_split_format", "gi_get_split_format"]
+           "gi_sweep_destroy", "gi_amp_width", "gi_set_split_format", "gi_get_split_format",
+           "gi_set_mu_budget", "gi_mu_parts"]
 # (TROVE_EXPORTS below: the symbols of include/gi_trove.h)
 
 
@@ -92,6 +93,8 @@ def lib():
         L.gi_set_jvals.argtypes = [vp, ci, P_i]
         L.gi_upload_bset.argtypes = [vp, ci, C.POINTER(GiBset)]
         L.gi_upload_dipole.argtypes = [vp, ci, vp, ci]
+        L.gi_set_mu_budget.argtypes = [vp, cll]
+        L.gi_mu_parts.argtypes = [vp]
         L.gi_set_levels.argtypes = [vp, ci, P_d, P_i, P_i, P_i, P_i, P_i]
         L.gi_upload_vectors.argtypes = [vp, ci, ci, ci, ci, vp, ci]
         L.gi_prepare.argtypes = [vp]
@@ -180,9 +183,10 @@ class Engine:
         self._keep.append(a)
         return _p(a, P_d)
 
-    def load(self, job, device_vectors=None, device_dipole=None):
+    def load(self, job, device_vectors=None, device_dipole=None, mu_budget=0):
         """Set parameters, bases, dipole, levels and eigenvector blocks.
-        device_vectors: optional {(jind, g): torch tensor on the device}."""
+        device_vectors: optional {(jind, g): torch tensor on the device};
+        mu_budget: device bytes for mu (gi_set_mu_budget; 0: no limit)."""
         L = lib()
         self.job = job
         nrep = job.sym["nrep"]
@@ -211,6 +215,7 @@ class Engine:
                        ijterms=self._i32(b.ijterms), repres=C.cast(reps, C.POINTER(P_d)),
                        nsize=self._i32(b.nsize))
             self._chk(L.gi_upload_bset(self.h, jind, C.byref(s)), "gi_upload_bset")
+        self._chk(L.gi_set_mu_budget(self.h, int(mu_budget)), "gi_set_mu_budget")
         if device_dipole is not None:
             self._chk(L.gi_upload_dipole(self.h, job.nvib, C.c_void_p(device_dipole.data_ptr()), 1),
                       "gi_upload_dipole")
@@ -254,6 +259,10 @@ class Engine:
             self._chk(lib().gi_count_candidates(self.h, len(ids), _p(ids, P_i), C.byref(n)),
                       "count")
         return n.value
+
+    def mu_parts(self):
+        """Parts mu is streamed in (gi_mu_parts; 1: resident)."""
+        return lib().gi_mu_parts(self.h)
 
     def amp_width(self):
         """Amplitudes per line (gi_amp_width): m*m, m the largest degeneracy."""

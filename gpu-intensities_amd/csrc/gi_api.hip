@@ -99,6 +99,10 @@ void free_slot(Slot &s)
 {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     if (s.cstream) (void)hipStreamSynchronize(s.cstream);
+    for (auto &e : s.mu_ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
     for (auto &e : s.buf) (void)hipFree(e.second.first);
     s.buf.clear();
     if (s.pinned) (void)hipHostFree(s.pinned);
@@ -184,6 +188,7 @@ void free_split(gi_ctx *c)
 int prepare_split(gi_ctx *c)
 {
     if (!c->prepared) return fail(c, GI_ESTATE, "split mode before gi_prepare");
+    if (c->mu_streamed) return fail(c, GI_ESTATE, "split mode needs mu resident");
     if (c->split_ready == c->split_key()) return GI_OK;
     HIPCHK(hipSetDevice(c->device));
     free_split(c);
@@ -289,6 +294,8 @@ static void free_plan(gi_ctx *c)
     c->groups.clear();
     (void)hipFree(c->d_mu_perm);
     c->d_mu_perm = nullptr;
+    (void)hipHostFree(c->h_mu_perm);
+    c->h_mu_perm = nullptr;
     (void)hipFree(c->d_lv_energy);
     (void)hipFree(c->d_lv_weight);
     c->d_lv_energy = c->d_lv_weight = nullptr;
@@ -463,6 +470,21 @@ int gi_upload_bset(gi_ctx *c, int jind, const gi_bset *b)
     return GI_OK;
 }
 
+int gi_set_mu_budget(gi_ctx *c, long long bytes)
+{
+    if (!c || bytes < 0) return GI_EINVAL;
+    if (serving(c)) return GI_ESTATE;
+    c->mu_budget = bytes;
+    return GI_OK;
+}
+
+int gi_mu_parts(const gi_ctx *c)
+{
+    if (!c || c->nvib <= 0) return 0;
+    if (!c->mu_streamed) return 1;
+    return (c->nvib + c->mu_np - 1) / c->mu_np;
+}
+
 int gi_upload_dipole(gi_ctx *c, int nvib, const double *mu, int on_device)
 {
     if (!c || nvib <= 0 || !mu) return GI_EINVAL;
@@ -472,12 +494,30 @@ int gi_upload_dipole(gi_ctx *c, int nvib, const double *mu, int on_device)
     if (c->mu_owned) (void)hipFree(c->d_mu);
     c->d_mu = nullptr;
     c->mu_owned = false;
+    c->mu_streamed = false;
+    c->mu_np = 0;
+    std::vector<double>().swap(c->h_mu);
     c->nvib = nvib;
+    const size_t bytes = sizeof(double) * 3 * (size_t)nvib * nvib;
+    if (c->mu_budget > 0 && (long long)bytes > c->mu_budget) {
+        // rows of one part: 3 * nvib doubles each
+        const long long row = 3LL * nvib * (long long)sizeof(double);
+        if (c->mu_budget < row)
+            return fail(c, GI_EINVAL, "gi_upload_dipole: mu budget below one vF row of mu (" +
+                                          std::to_string(row) + " bytes)");
+        c->mu_np = (int)std::min<long long>(nvib, c->mu_budget / row);
+        c->h_mu.resize((size_t)3 * nvib * nvib);
+        if (on_device)
+            HIPCHK(hipMemcpy(c->h_mu.data(), mu, bytes, hipMemcpyDeviceToHost));
+        else
+            std::memcpy(c->h_mu.data(), mu, bytes);
+        c->mu_streamed = true;
+        return GI_OK;
+    }
     if (on_device) {
         c->d_mu = const_cast<double *>(mu);
         return GI_OK;
     }
-    const size_t bytes = sizeof(double) * 3 * (size_t)nvib * nvib;
     HIPCHK(hipMalloc((void **)&c->d_mu, bytes));
     c->mu_owned = true;
     HIPCHK(hipMemcpy(c->d_mu, mu, bytes, hipMemcpyHostToDevice));
@@ -545,7 +585,7 @@ int gi_prepare(gi_ctx *c)
 {
     if (!c) return GI_EINVAL;
     if (serving(c)) return GI_ESTATE;
-    if (!c->have_params || !c->d_mu)
+    if (!c->have_params || (!c->d_mu && !c->mu_streamed))
         return fail(c, GI_ESTATE, "gi_prepare: parameters, every basis and the dipole are needed");
     for (int j = 0; j < c->nJ; j++)
         if (!c->have_bset[j])
@@ -567,7 +607,19 @@ int gi_prepare(gi_ctx *c)
         if ((rc = assign_slots(c, j)) != GI_OK) return rc;
     }
     if ((rc = build_vib_classes(c)) != GI_OK) return rc;
-    {
+    if (c->mu_streamed) {
+        // permuted mu in pinned host memory, element (a, vI, vF) at
+        // a*N*N + vI*N + vF as on the device (k_permute_mu)
+        const int N = c->nvib;
+        const size_t NN = (size_t)N * N;
+        HIPCHK(hipHostMalloc((void **)&c->h_mu_perm, sizeof(double) * 3 * NN, hipHostMallocDefault));
+        for (int a = 0; a < 3; a++)
+            for (int j = 0; j < N; j++) {
+                const double *src = c->h_mu.data() + a * NN + (size_t)c->vperm[j] * N;
+                double *dst = c->h_mu_perm + a * NN + (size_t)j * N;
+                for (int i = 0; i < N; i++) dst[i] = src[c->vperm[i]];
+            }
+    } else {
         const int N = c->nvib;
         int *d_perm = nullptr;
         if ((rc = upload(c, &d_perm, c->vperm)) != GI_OK) return rc;
@@ -701,6 +753,7 @@ int gi_half_ls(gi_ctx *c, int jindI, int jindF, int gI, int ideg, const double *
     if (gI < 0 || gI >= c->nrep || ideg < 0 || ideg >= c->degen[gI]) return GI_EINVAL;
     const Coef &cf = coef_of(c, jindI, jindF);
     if (!cf.valid) return fail(c, GI_EINVAL, "gi_half_ls: |J' - J| > 1");
+    if (c->mu_streamed) return fail(c, GI_ESTATE, "gi_half_ls: needs mu resident");
     HIPCHK(hipSetDevice(c->device));
     const JPlan &pI = c->jp[jindI], &pF = c->jp[jindF];
     const int N = c->nvib, SI = pI.nslots, SF = pF.nslots, SMAX = smax_for(SF);

@@ -54,7 +54,7 @@ struct Timeline {
     hipStream_t st;
     std::vector<std::tuple<std::string, hipEvent_t, hipEvent_t>> recs;
     explicit Timeline(hipStream_t s) : st(s) {}
-    int begin(const char *name)
+    int begin(const char *name, hipStream_t on = nullptr)
     {
         hipEvent_t a, b;
         if (hipEventCreate(&a) != hipSuccess) return GI_EHIP;
@@ -63,9 +63,13 @@ struct Timeline {
             return GI_EHIP;
         }
         recs.emplace_back(name, a, b);
-        return hipEventRecord(a, st) == hipSuccess ? GI_OK : GI_EHIP;
+        return hipEventRecord(a, on ? on : st) == hipSuccess ? GI_OK : GI_EHIP;
     }
-    int end() { return hipEventRecord(std::get<2>(recs.back()), st) == hipSuccess ? GI_OK : GI_EHIP; }
+    int end(hipStream_t on = nullptr)
+    {
+        return hipEventRecord(std::get<2>(recs.back()), on ? on : st) == hipSuccess ? GI_OK
+                                                                                      : GI_EHIP;
+    }
     void collect(gi_ctx *c)
     {
         c->kstats.clear();
@@ -95,6 +99,15 @@ struct Timeline {
 #define TL_END()                                                                          \
     do {                                                                                  \
         if (tl && tl->end() != GI_OK) return fail(c, GI_EHIP, "hipEventRecord");          \
+    } while (0)
+#define TL_BEGIN_ON(name, stream)                                                         \
+    do {                                                                                  \
+        if (tl && tl->begin(name, stream) != GI_OK)                                       \
+            return fail(c, GI_EHIP, "hipEventRecord");                                    \
+    } while (0)
+#define TL_END_ON(stream)                                                                 \
+    do {                                                                                  \
+        if (tl && tl->end(stream) != GI_OK) return fail(c, GI_EHIP, "hipEventRecord");    \
     } while (0)
 
 // trove_functions.cpp:1495-1510
@@ -1312,6 +1325,21 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
         double *Ht = (double *)slot_scratch(c, S, "Ht", (size_t)max_items * max_ncb * max_nsizeF * 8,
                                             &rc);
         if (rc) return rc;
+        // streamed mu: two device parts of mu_np vF rows (every a and vI); the
+        // copy stream fills one while the half linestrength reads the other
+        double *mupart[2] = {nullptr, nullptr};
+        hipStream_t cs = S.cstream ? S.cstream : S.stream;
+        if (c->mu_streamed) {
+            for (int b = 0; b < 2; b++) {
+                mupart[b] = (double *)slot_scratch(c, S, b ? "mupart1" : "mupart0",
+                                                   (size_t)3 * N * c->mu_np * 8, &rc);
+                if (rc) return rc;
+            }
+            for (auto &e : S.mu_ev)
+                if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+                    return fail(c, GI_EHIP, "hipEventCreate");
+        }
+        int nparts = 0;  // parts copied so far; part q goes to buffer q & 1
         for (auto &ch : P.chunks) {
             const JPlan &pI = c->jp[ch.jindI], &pF = c->jp[ch.jindF];
             const GroupPlan &GP = group_of(c, ch.jindI, ch.gI, ch.jindF);
@@ -1336,12 +1364,45 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
                                            (const int *)(d_meta + ch.off_uideg), r0, r1, G);
                         TL_END();
                     }
+                    if (c->mu_streamed) continue;  // after every class's G, by parts below
                     TL_BEGIN("half_ls_gemv");
                     if (launch_gemv_sym(S.stream, ch.nunits, N, GP.Gtot, GP.Htot, cd,
                                         c->d_mu_perm, G, H) != 0)
                         return fail(c, GI_EINVAL, "no half-ls kernel for this column count");
                     TL_END();
                 }
+            // Streamed mu (do_1st_half_ls_blocks, cuda_host.cu:783-817): each
+            // part holds vF rows [v0, v0 + np) of mu; the rows of every class
+            // inside it are computed from it whole (the same sums in the same
+            // order as from resident mu).
+            for (int v0 = 0; c->mu_streamed && ch.nunits > 0 && v0 < N; v0 += c->mu_np) {
+                const int np = std::min(c->mu_np, N - v0);
+                bool used = false;
+                for (const OutClass &oc : GP.oc)
+                    used = used || (oc.off < v0 + np && oc.off + oc.n > v0);
+                if (!used) continue;
+                const int b = nparts++ & 1;
+                // the buffer is free once the products of the part before last are done
+                if (nparts > 2) HIPCHK(hipStreamWaitEvent(cs, S.mu_ev[2 + b], 0));
+                TL_BEGIN_ON("mu_copy", cs);
+                HIPCHK(hipMemcpy2DAsync(mupart[b], (size_t)np * 8, c->h_mu_perm + v0,
+                                        (size_t)N * 8, (size_t)np * 8, (size_t)3 * N,
+                                        hipMemcpyHostToDevice, cs));
+                TL_END_ON(cs);
+                HIPCHK(hipEventRecord(S.mu_ev[b], cs));
+                HIPCHK(hipStreamWaitEvent(S.stream, S.mu_ev[b], 0));
+                for (const OutClass &oc : GP.oc) {
+                    const ClassDev cd = classdev(oc);
+                    const int rlo = std::max(0, v0 - cd.off), rhi = std::min(cd.n, v0 + np - cd.off);
+                    if (rlo >= rhi) continue;
+                    TL_BEGIN("half_ls_gemv");
+                    if (launch_gemv_sym(S.stream, ch.nunits, N, GP.Gtot, GP.Htot, cd, mupart[b],
+                                        G, H, np, v0, rlo, rhi) != 0)
+                        return fail(c, GI_EINVAL, "no half-ls kernel for this column count");
+                    TL_END();
+                }
+                HIPCHK(hipEventRecord(S.mu_ev[2 + b], S.stream));
+            }
             TL_BEGIN("fold");
             launch_fold(S.stream, ch.nitems, BF.nsize, ndegI, ndegF, GP.Htot,
                         (const int *)(d_meta + ch.off_iunit), t0, t1, H, Ht);
@@ -1465,6 +1526,9 @@ int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
     if (c->server) return fail(c, GI_ESTATE, "gi_compute while the request server runs");
     if (mode != GI_MODE_FP64 && mode != GI_MODE_FP64_BATCHED && mode != GI_MODE_SPLIT)
         return fail(c, GI_EINVAL, "gi_compute: unknown mode");
+    if (mode != GI_MODE_FP64 && c->mu_streamed)
+        return fail(c, GI_ESTATE, "gi_compute: the matrix-core modes need mu resident "
+                                  "(raise gi_set_mu_budget and upload mu again)");
     int rc;
     if (mode == GI_MODE_SPLIT && (rc = prepare_split(c)) != GI_OK) return rc;
     using clk = std::chrono::steady_clock;
@@ -1900,6 +1964,8 @@ static int sweep_create(gi_ctx *c, int mode, int nlower, const int *ids, gi_swee
     if (c->server) return fail(c, GI_ESTATE, "gi_sweep_create while the request server runs");
     if (mode != GI_MODE_FP64_BATCHED && mode != GI_MODE_SPLIT)
         return fail(c, GI_EINVAL, "gi_sweep_create: sweeps run the matrix-core modes");
+    if (c->mu_streamed)
+        return fail(c, GI_ESTATE, "gi_sweep_create: the matrix-core modes need mu resident");
     int rc;
     if (mode == GI_MODE_SPLIT && (rc = prepare_split(c)) != GI_OK) return rc;
     HIPCHK(hipSetDevice(c->device));
@@ -2090,6 +2156,8 @@ int gi_serve_submit(gi_ctx *c, const gi_request *r, long long *ticket)
         return GI_EINVAL;
     if (r->mode == GI_MODE_SPLIT && c->split_ready != c->split_key())
         return fail(c, GI_ESTATE, "split-mode request before gi_prepare_split");
+    if (r->mode != GI_MODE_FP64 && c->mu_streamed)
+        return fail(c, GI_ESTATE, "gi_serve_submit: the matrix-core modes need mu resident");
     if (!(r->temperature > 0.0)) return GI_EINVAL;
     auto R = std::make_shared<Request>();
     R->r = *r;

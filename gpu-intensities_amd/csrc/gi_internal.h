@@ -156,6 +156,7 @@ struct HostArena {
 struct Slot {
     hipStream_t stream = nullptr;
     hipStream_t cstream = nullptr;   // device-to-host copies that overlap `stream`
+    hipEvent_t mu_ev[4] = {};        // streamed mu: part ready / part free, two buffers
     std::map<std::string, std::pair<void *, size_t>> buf;
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
@@ -197,6 +198,14 @@ struct gi_ctx {
     int nvib = 0;
     double *d_mu = nullptr;
     bool mu_owned = false;
+    // streamed mu (gi_set_mu_budget): mu larger than the budget stays on the
+    // host; gi_prepare writes its permuted copy to pinned memory and the plain
+    // FP64 half linestrength copies it to the device in parts of mu_np rows
+    long long mu_budget = 0;         // device bytes for mu; 0: no limit
+    bool mu_streamed = false;        // set by gi_upload_dipole
+    std::vector<double> h_mu;        // mu as uploaded (streamed mode only)
+    double *h_mu_perm = nullptr;     // pinned, permuted (streamed mode only)
+    int mu_np = 0;                   // rows (vF) per part
     int nlevels = 0;
     std::vector<double> lv_energy;
     std::vector<int> lv_jind, lv_igamma, lv_ndeg, lv_irec0, lv_quanta;

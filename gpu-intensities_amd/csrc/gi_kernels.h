@@ -220,8 +220,11 @@ void launch_permute_mu(hipStream_t st, int N, const int *perm, const double *in,
 void launch_build_g_sym(hipStream_t st, int nunits, int N, int Gtot, const ClassDev &cd,
                         const double *const *uptr, const int *unit_ideg, SymDev r0, SymDev r1,
                         double *G);
+// rows [rlo, rhi) of the class (rhi < 0: all); mu with leading dimension ld
+// (<= 0: N) holding vF rows from vbase (a streamed part of mu)
 int launch_gemv_sym(hipStream_t st, int nunits, int N, int Gtot, int Htot, const ClassDev &cd,
-                    const double *mu, const double *G, double *H);
+                    const double *mu, const double *G, double *H, int ld = 0, int vbase = 0,
+                    int rlo = 0, int rhi = -1);
 void launch_fold(hipStream_t st, int nitems, int nsizeF, int ndegI, int ndegF, int SFN,
                  const int *item_unit, SymDev t0, SymDev t1, const double *H, double *Ht);
 

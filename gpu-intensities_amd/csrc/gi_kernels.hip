@@ -178,29 +178,33 @@ __global__ __launch_bounds__(64) void k_build_g_sym(int N, int Gtot, ClassDev cd
 
 // K2 (symmetry-blocked, GEMV form): H_co[vF][j] = sum over segments e of
 // sum_k mu_{a(e)}[off_co + vF, off_e + k] * G_co[row0_e + k][j], one thread per
-// vF row, the class's columns in registers, G rows wave-uniform.
+// vF row, the class's columns in registers, G rows wave-uniform. Rows
+// [rlo, rhi) of the class; mu element (a, vI, vF) at a*N*ld + vI*ld + vF - vbase
+// (resident mu: ld = N, vbase = 0; a streamed part: its vF rows from vbase).
 template <int SMAX>
 __global__ __launch_bounds__(256) void k_gemv_sym(int N, int Gtot, int Htot, ClassDev cd,
-                                                  const double *__restrict__ mu,
+                                                  const double *__restrict__ mu, int ld,
+                                                  int vbase, int rlo, int rhi,
                                                   const double *__restrict__ G,
                                                   double *__restrict__ H)
 {
     const int unit = blockIdx.x;
-    const int row = blockIdx.y * 256 + threadIdx.x;
-    const int rr = row < cd.n ? row : cd.n - 1;
+    const int row = rlo + blockIdx.y * 256 + threadIdx.x;
+    const int rr = row < rhi ? row : rhi - 1;
     const double *__restrict__ Gu = G + (size_t)unit * Gtot + cd.gbase;
     double acc[SMAX];
 #pragma unroll
     for (int s = 0; s < SMAX; s++) acc[s] = 0.0;
     for (int e = 0; e < cd.nseg; e++) {
         const int a = cd.seg_a[e], off = cd.seg_off[e], cnt = cd.seg_n[e], r0 = cd.seg_row0[e];
-        const double *__restrict__ mp = mu + (size_t)a * N * N + (size_t)off * N + (cd.off + rr);
+        const double *__restrict__ mp =
+            mu + (size_t)a * N * ld + (size_t)off * ld + (cd.off + rr - vbase);
         const double *__restrict__ gp = Gu + (size_t)r0 * SMAX;
         int k = 0;
         for (; k + 4 <= cnt; k += 4) {
             double m[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) m[q] = mp[(size_t)(k + q) * N];
+            for (int q = 0; q < 4; q++) m[q] = mp[(size_t)(k + q) * ld];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const double *gk = gp + (size_t)(k + q) * SMAX;
@@ -209,13 +213,13 @@ __global__ __launch_bounds__(256) void k_gemv_sym(int N, int Gtot, int Htot, Cla
             }
         }
         for (; k < cnt; k++) {
-            const double m = mp[(size_t)k * N];
+            const double m = mp[(size_t)k * ld];
             const double *gk = gp + (size_t)k * SMAX;
 #pragma unroll
             for (int s = 0; s < SMAX; s++) acc[s] = __builtin_fma(m, gk[s], acc[s]);
         }
     }
-    if (row < cd.n) {
+    if (row < rhi) {
         double *Hc = H + (size_t)unit * Htot + cd.hbase;
 #pragma unroll
         for (int s = 0; s < SMAX; s++)
@@ -517,18 +521,24 @@ void launch_build_g_sym(hipStream_t st, int nunits, int N, int Gtot, const Class
 }
 
 int launch_gemv_sym(hipStream_t st, int nunits, int N, int Gtot, int Htot, const ClassDev &cd,
-                    const double *mu, const double *G, double *H)
+                    const double *mu, const double *G, double *H, int ld, int vbase, int rlo,
+                    int rhi)
 {
-    dim3 grid(nunits, (cd.n + 255) / 256);
+    if (ld <= 0) ld = N;
+    if (rhi < 0) rhi = cd.n;
+    if (rlo >= rhi || nunits <= 0) return 0;
+    dim3 grid(nunits, (rhi - rlo + 255) / 256);
+#define L_(m) hipLaunchKernelGGL(k_gemv_sym<m>, grid, dim3(256), 0, st, N, Gtot, Htot, cd, mu, ld, vbase, rlo, rhi, G, H)
     switch (cd.smax) {
-    case 8: hipLaunchKernelGGL(k_gemv_sym<8>, grid, dim3(256), 0, st, N, Gtot, Htot, cd, mu, G, H); break;
-    case 16: hipLaunchKernelGGL(k_gemv_sym<16>, grid, dim3(256), 0, st, N, Gtot, Htot, cd, mu, G, H); break;
-    case 24: hipLaunchKernelGGL(k_gemv_sym<24>, grid, dim3(256), 0, st, N, Gtot, Htot, cd, mu, G, H); break;
-    case 32: hipLaunchKernelGGL(k_gemv_sym<32>, grid, dim3(256), 0, st, N, Gtot, Htot, cd, mu, G, H); break;
-    case 48: hipLaunchKernelGGL(k_gemv_sym<48>, grid, dim3(256), 0, st, N, Gtot, Htot, cd, mu, G, H); break;
-    case 64: hipLaunchKernelGGL(k_gemv_sym<64>, grid, dim3(256), 0, st, N, Gtot, Htot, cd, mu, G, H); break;
+    case 8: L_(8); break;
+    case 16: L_(16); break;
+    case 24: L_(24); break;
+    case 32: L_(32); break;
+    case 48: L_(48); break;
+    case 64: L_(64); break;
     default: return -1;
     }
+#undef L_
     return 0;
 }
 
