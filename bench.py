@@ -158,7 +158,7 @@ PMC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         "c2_pmc_traffic.json")
 
 
-def measured_traffic(name):
+def measured_traffic(name, sym=None):
     """HBM bytes per dispatch of the kernel behind timer `name`, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this workload,
     each scaled by the factor the same counter gives for a plain 1 GiB copy.
@@ -166,16 +166,17 @@ def measured_traffic(name):
     try:
         with open(PMC_FILE) as f:
             d = json.load(f)
-        return d["kernels"][KERNEL_SYMBOL[name]]["hbm_bytes"]
+        return d["kernels"][sym or KERNEL_SYMBOL[name]]["hbm_bytes"]
     except (OSError, KeyError, ValueError):
         return None
 
 
-def roofline_entry(name, kstats, work, nvib):
+def roofline_entry(name, kstats, work, nvib, split_cfg=0):
     """Roofline of one kernel: algorithmic work of all its launches in the last
     step divided by their HIP-event time (average per launch)."""
     r = _roofline(name, kstats, work, nvib)
-    r["traffic"] = measured_traffic(name)
+    sym = "k_halfls_split2" if name == "half_ls_split" and split_cfg == 2 else None
+    r["traffic"] = measured_traffic(name, sym)
     if r["traffic"] is not None:
         r["traffic_unit"] = "bytes per launch (rocprofv3 PMC, copy-calibrated)"
     return r
@@ -701,7 +702,8 @@ def main():
     kstats = modes[best]["_kstats"]
     work = modes[best]["work"]
     dom = max(kstats, key=lambda k: kstats[k]["total_ms"])
-    roof = roofline_entry(dom, kstats, work, job.nvib)
+    roof = roofline_entry(dom, kstats, work, job.nvib,
+                          split_cfg=(modes[best].get("tuning") or {}).get("split_cfg", 0))
     from_table = (modes[best].get("tuning") or {}).get("source") == "saved table"
     if not (from_table and os.path.abspath(tuning_file) == TUNING_FILE):
         # the committed PMC passes ran with the committed table's settings

@@ -1036,7 +1036,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
     }
     glds = glds && g4;
     const int grows = g4 ? 1024 : 256;
-    const int srows_tile = c->tile_split ? 128 : 64;  // k_halfls_split(_wide) row tile
+    const int srows_tile = c->tile_split == 1 ? 128 : 64;  // k_halfls_split(_wide, 2) row tile
     size_t iround = 0;
     for (auto &rg : rounds) {
         RoundCapture cap;
@@ -1046,6 +1046,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
         std::vector<CorrSplitArgs> cj;
         std::vector<int> gbegin, tbegin, cbegin;
         int gt = 0, tt = 0, ct = 0, ubeg = 0;
+        bool steps_fit = true;  // every class within k_halfls_split2's step table
         size_t ogd = 0, oge = 0, oh = 0, oht = 0, ohd = 0, ohc = 0;
         std::vector<size_t> Hoff, Htoff, Hdoff, Hcoff;
         for (size_t i = rg.first; i < rg.second; i++) {
@@ -1083,6 +1084,10 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                 gbegin.push_back(glds ? ubeg : g.tile_begin);
                 ubeg += ch.nunits;
                 for (const OutClass &oc : GP.oc) {
+                    int steps = 0;
+                    for (int n : oc.seg_n) steps += (n + 63) / 64;
+                    steps_fit = steps_fit && steps <= kSplitMaxSteps &&
+                                (int)oc.seg_n.size() <= kSplitMaxSegs;
                     SplitHalfJob h;
                     h.cd = classdev(oc);
                     h.nunits = ch.nunits;
@@ -1132,7 +1137,8 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             P.work[14] += tt;
             launch_halfls_split(S.stream, SH, c->split_radix, N, c->kpad_all, c->d_mud, c->muplane, c->d_mue,
                                 (const SplitHalfJob *)d_hj, (const int *)(d_hj + hj_bytes),
-                                (int)hj.size(), tt, c->tile_split != 0);
+                                (int)hj.size(), tt,
+                                c->tile_split == 2 && !steps_fit ? 0 : c->tile_split);
             TL_END();
         }
         std::vector<FoldJob> fjobs;
@@ -1664,10 +1670,10 @@ int autotune(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out, int
     static const char *const tname[4] = {"64x64", "64x128", "128x64", "128x128"};
     static const int rounds[6] = {1, 2, 3, 4, 6, 8};
     static const int shares[4] = {50, 60, 70, 80};
-    static const int stiles[2] = {0, 1};
-    static const char *const sname[2] = {"64x64", "128x64"};
+    static const int stiles[3] = {0, 1, 2};
+    static const char *const sname[3] = {"64x64", "128x64", "64x64/2-steps"};
     if (mode == GI_MODE_SPLIT &&
-        (rc = scan("split_halfls_tile", "half_ls_split", &c->tile_split, stiles, 2, sname))) {
+        (rc = scan("split_halfls_tile", "half_ls_split", &c->tile_split, stiles, 3, sname))) {
         c->tile_split = saved.split_cfg;
         return rc;
     }
@@ -2062,7 +2068,7 @@ int gi_set_tuning(gi_ctx *c, const gi_tuning *t)
     if (c->server) return fail(c, GI_ESTATE, "gi_set_tuning while the request server runs");
     if (t->halfls_cfg < 0 || t->halfls_cfg > 3 || t->corr_cfg < 0 || t->corr_cfg > 3 ||
         t->pipe_rounds < 1 || t->pipe_rounds > kMaxPipeRounds || t->split_cfg < 0 ||
-        t->split_cfg > 1 || t->pipe_share < 10 || t->pipe_share > 90)
+        t->split_cfg > 2 || t->pipe_share < 10 || t->pipe_share > 90)
         return fail(c, GI_EINVAL, "gi_set_tuning: value out of range");
     c->tile_halfls = t->halfls_cfg;
     c->tile_corr = t->corr_cfg;

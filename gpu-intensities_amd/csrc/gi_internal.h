@@ -238,7 +238,7 @@ struct gi_ctx {
     int tile_halfls = 3, tile_corr = 0;
     int pipe_rounds = 4;
     int pipe_share = 50;             // percent of the remaining work per round but the last
-    int tile_split = 0;              // k_halfls_split rows: 0 = 64, 1 = 128
+    int tile_split = 2;              // split half-ls: 0 = 64 rows, 1 = 128, 2 = 2 steps in flight
 
     // split mode. Requested format (0 = automatic): radix 128 or 254, digits of
     // the correlation (eigenvector blocks, folded vectors) and of the half-ls

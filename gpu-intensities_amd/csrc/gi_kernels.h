@@ -190,10 +190,14 @@ void launch_build_g_split(hipStream_t st, int S, double R, const GSplitJob *d_jo
 // (maps of width <= 4, records of <= kRecLds doubles); d_begin = first unit of each job.
 void launch_build_g_split_lds(hipStream_t st, int S, double R, const GSplitJob *d_jobs,
                               const int *d_begin, int njobs, int nunits);
-// wide: 128-row tiles (SplitHalfJob.mt counts 128-row tiles), else 64-row.
+// cfg 0: 64 x 64 tiles; 1: 128-row tiles (SplitHalfJob.mt counts 128-row
+// tiles); 2: 64 x 64 with two K steps of loads in flight, for classes of at
+// most kSplitMaxSteps K steps and kSplitMaxSegs segments and at most 5 digits
+// (else the 64 x 64 kernel runs).
+constexpr int kSplitMaxSteps = 512, kSplitMaxSegs = 64;
 void launch_halfls_split(hipStream_t st, int S, double R, int N, int kpad_all, const int8_t *mud,
                          long long muplane, const int *mue, const SplitHalfJob *d_jobs,
-                         const int *d_tbegin, int njobs, int ntiles, bool wide);
+                         const int *d_tbegin, int njobs, int ntiles, int cfg);
 void launch_corr_split(hipStream_t st, int S, double R, const CorrSplitArgs *d_args, const int *d_cbegin,
                        int nchunks, int ntiles);
 

@@ -601,6 +601,125 @@ __global__ __launch_bounds__(256) void k_halfls_split(int N, int kpad_all,
             }
 }
 
+
+// k_halfls_split with the operand loads of two K steps in flight: the K steps
+// of the class (64 bytes of one segment each) are first listed in LDS as mu
+// and G offsets, then a counted loop alternates two register sets, loading
+// step s + 2 while step s multiplies. Same products in the same order as
+// k_halfls_split: the same bits. Classes with more than kSplitMaxSteps steps use
+// k_halfls_split.
+template <int S>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_halfls_split2(
+    int N, int kpad_all,
+                                                       const int8_t *__restrict__ mud,
+                                                       long long muplane,
+                                                       const int *__restrict__ mue,
+                                                       const SplitHalfJob *__restrict__ jobs,
+                                                       const int *__restrict__ tbegin, int njobs,
+                                                       int ntiles, double R)
+{
+    __shared__ __attribute__((aligned(16))) int8_t As[S * 64 * kRowB];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[S * 64 * kRowB];
+    __shared__ long long s_a[kSplitMaxSteps];
+    __shared__ int s_b[kSplitMaxSteps];
+    __shared__ int s_first[kSplitMaxSegs + 1];
+    const int tile = xcd_tile_s(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
+    const SplitHalfJob &J = jobs[find_tile_job(tbegin, njobs, tile)];
+    const ClassDev &cd = J.cd;
+    const int local = tile - J.tile_begin;
+    const int m0 = (local % J.mt) * 64, n0 = (local / J.mt) * 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int ncolsTot = J.nunits * cd.ncols;
+    // step table: segment e's steps start at s_first[e]
+    if (tid == 0) {
+        int n = 0;
+        for (int e = 0; e < cd.nseg && e < kSplitMaxSegs; e++) {
+            s_first[e] = n;
+            n += (cd.seg_n[e] + 63) / 64;
+        }
+        s_first[min(cd.nseg, kSplitMaxSegs)] = n;
+    }
+    __syncthreads();
+    // the host sends only classes within both limits to this kernel
+    const int nsteps = min(s_first[min(cd.nseg, kSplitMaxSegs)], kSplitMaxSteps);
+    for (int e = 0; e < cd.nseg && e < kSplitMaxSegs; e++)
+        for (int st = s_first[e] + tid; st < s_first[e + 1] && st < kSplitMaxSteps; st += 256) {
+            const int kk = (st - s_first[e]) * 64;
+            s_a[st] = ((long long)cd.seg_a[e] * N + cd.off + m0) * kpad_all +
+                      cd.seg_pad[2 * e + 1] + kk;
+            s_b[st] = cd.seg_pad[2 * e] + kk;
+        }
+    __syncthreads();
+    const int lr = tid >> 2, lp = (tid & 3) * 16;
+    const bool mok = m0 + lr < cd.n;
+    const int n = n0 + lr;
+    const bool nok = n < ncolsTot;
+    long long bcol = 0;
+    if (nok) {
+        const int u = n / cd.ncols, jj = n - u * cd.ncols;
+        bcol = (long long)u * J.GtotB + cd.cbaseB + (long long)jj * cd.kpad;
+    }
+    const long long arow = (long long)lr * kpad_all + lp;
+    const long long bofs = bcol + lp;
+    v4i acc[2][2][S];
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int l = 0; l < S; l++) acc[x][y][l] = v4i{0, 0, 0, 0};
+    v4i ra0[S], rb0[S], ra1[S], rb1[S];
+#define GI_FETCH(RA, RB, ST)                                                                   \
+    do {                                                                                       \
+        const long long ao = s_a[ST] + arow;                                                   \
+        const long long bo = bofs + s_b[ST];                                                   \
+        _Pragma("unroll") for (int q = 0; q < S; q++) {                                        \
+            RA[q] = mok ? *(const v4i *)(mud + q * muplane + ao) : v4i{0, 0, 0, 0};           \
+            RB[q] = nok ? *(const v4i *)(J.Gd + q * J.gplane + bo) : v4i{0, 0, 0, 0};          \
+        }                                                                                      \
+    } while (0)
+#define GI_STAGE(RA, RB)                                                                       \
+    do {                                                                                       \
+        _Pragma("unroll") for (int q = 0; q < S; q++) {                                        \
+            *(v4i *)(As + (q * 64 + lr) * kRowB + lp) = RA[q];                                 \
+            *(v4i *)(Bs + (q * 64 + lr) * kRowB + lp) = RB[q];                                 \
+        }                                                                                      \
+    } while (0)
+    if (nsteps > 0) GI_FETCH(ra0, rb0, 0);
+    if (nsteps > 1) GI_FETCH(ra1, rb1, 1);
+    for (int st = 0; st < nsteps; st += 2) {
+        GI_STAGE(ra0, rb0);
+        __syncthreads();
+        if (st + 2 < nsteps) GI_FETCH(ra0, rb0, st + 2);
+        oz_step<S>(As, Bs, wm, wn, lane, acc);
+        __syncthreads();
+        if (st + 1 >= nsteps) break;
+        GI_STAGE(ra1, rb1);
+        __syncthreads();
+        if (st + 3 < nsteps) GI_FETCH(ra1, rb1, st + 3);
+        oz_step<S>(As, Bs, wm, wn, lane, acc);
+        __syncthreads();
+    }
+#undef GI_FETCH
+#undef GI_STAGE
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wm + 16 * x + 4 * (lane >> 4) + r;
+                const int nn = n0 + wn + 16 * y + (lane & 15);
+                if (m >= cd.n || nn >= ncolsTot) continue;
+                const int u = nn / cd.ncols, jj = nn - u * cd.ncols;
+                const int ex = mue[cd.off + m] + J.Ge[(size_t)u * J.EGtot + cd.eoff + jj];
+                J.H[(size_t)u * J.Htot + cd.hbase + (size_t)jj * cd.n + m] =
+                    ldexp(oz_value<S>(acc[x][y], r, 1.0 / R), ex);
+            }
+}
+
 // One K step of 64 for a 128 x 64 tile: the wave's 4 x 2 blocks (64 rows x
 // 32 columns), so each LDS operand read feeds twice as many products as in
 // oz_step; A rows of a plane are 128 apart in LDS.
@@ -719,7 +838,8 @@ __global__ __launch_bounds__(256, 1) void k_halfls_split_wide(
 // Split banded correlation (the digit form of k_corr_mfma): rows = upper
 // records (digits of the resident block), columns = folded vectors (digits).
 template <int S>
-__global__ __launch_bounds__(256) void k_corr_split(const CorrSplitArgs *__restrict__ args,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_corr_split(
+    const CorrSplitArgs *__restrict__ args,
                                                     const int *__restrict__ cbegin, int nchunks,
                                                     int ntiles, double R)
 {
@@ -763,6 +883,50 @@ __global__ __launch_bounds__(256) void k_corr_split(const CorrSplitArgs *__restr
         for (int y = 0; y < 2; y++)
 #pragma unroll
             for (int l = 0; l < S; l++) acc[x][y][l] = v4i{0, 0, 0, 0};
+    if constexpr (S <= 5) {
+    // two register sets: the loads of K step k0 + 128 are issued while step k0
+    // multiplies (as in k_halfls_split2; at 6 digits they exceed the registers)
+    v4i ra0[S], rb0[S], ra1[S], rb1[S];
+#define GI_FETCH(RA, RB, K0)                                                                   \
+    do {                                                                                       \
+        _Pragma("unroll") for (int q = 0; q < S; q++) {                                        \
+            RA[q] = fok ? *(const v4i *)(a.Ud + q * a.uplane + arow + (K0) + lp)                 \
+                        : v4i{0, 0, 0, 0};                                                     \
+            RB[q] = nok ? *(const v4i *)(a.Hd + q * a.hplane + brow + (K0) + lp)                 \
+                        : v4i{0, 0, 0, 0};                                                     \
+        }                                                                                      \
+    } while (0)
+#define GI_STAGE(RA, RB)                                                                       \
+    do {                                                                                       \
+        _Pragma("unroll") for (int q = 0; q < S; q++) {                                        \
+            *(v4i *)(As + (q * 64 + lr) * kRowB + lp) = RA[q];                                 \
+            *(v4i *)(Bs + (q * 64 + lr) * kRowB + lp) = RB[q];                                 \
+        }                                                                                      \
+    } while (0)
+    if (kb < ke) GI_FETCH(ra0, rb0, kb);
+    if (kb + 64 < ke) GI_FETCH(ra1, rb1, kb + 64);
+    for (int k0 = kb; k0 < ke; k0 += 128) {
+        GI_STAGE(ra0, rb0);
+        __syncthreads();
+        if (k0 + 128 < ke) GI_FETCH(ra0, rb0, k0 + 128);
+        // raised issue priority while this wavefront multiplies (as in k_corr_mfma;
+        // 0.83 -> 0.80 ms at C2)
+        __builtin_amdgcn_s_setprio(1);
+        oz_step<S>(As, Bs, wm, wn, lane, acc);
+        __builtin_amdgcn_s_setprio(0);
+        __syncthreads();
+        if (k0 + 64 >= ke) break;
+        GI_STAGE(ra1, rb1);
+        __syncthreads();
+        if (k0 + 192 < ke) GI_FETCH(ra1, rb1, k0 + 192);
+        __builtin_amdgcn_s_setprio(1);
+        oz_step<S>(As, Bs, wm, wn, lane, acc);
+        __builtin_amdgcn_s_setprio(0);
+        __syncthreads();
+    }
+#undef GI_FETCH
+#undef GI_STAGE
+    } else {
     v4i ra[S], rb[S];
     auto fetch = [&](int k0) {
 #pragma unroll
@@ -780,12 +944,11 @@ __global__ __launch_bounds__(256) void k_corr_split(const CorrSplitArgs *__restr
         }
         __syncthreads();
         if (k0 + 64 < ke) fetch(k0 + 64);
-        // raised issue priority while this wavefront multiplies (as in k_corr_mfma;
-        // 0.83 -> 0.80 ms at C2)
         __builtin_amdgcn_s_setprio(1);
         oz_step<S>(As, Bs, wm, wn, lane, acc);
         __builtin_amdgcn_s_setprio(0);
         __syncthreads();
+    }
     }
 #pragma unroll
     for (int x = 0; x < 2; x++)
@@ -883,10 +1046,20 @@ void launch_build_g_split_lds(hipStream_t st, int S, double R, const GSplitJob *
 
 void launch_halfls_split(hipStream_t st, int S, double R, int N, int kpad_all, const int8_t *mud,
                          long long muplane, const int *mue, const SplitHalfJob *d_jobs,
-                         const int *d_tbegin, int njobs, int ntiles, bool wide)
+                         const int *d_tbegin, int njobs, int ntiles, int cfg)
 {
     if (njobs <= 0 || ntiles <= 0) return;
-    if (wide) {
+    if (cfg == 2 && S <= 5) {  // two register sets exceed the registers at 6 digits
+#define L_(s) hipLaunchKernelGGL(k_halfls_split2<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, N, kpad_all, mud, muplane, mue, d_jobs, d_tbegin, njobs, ntiles, R)
+        switch (S) {
+        case 3: L_(3); break;
+        case 4: L_(4); break;
+        default: L_(5); break;
+        }
+#undef L_
+        return;
+    }
+    if (cfg == 1) {
 #define L_(s) hipLaunchKernelGGL(k_halfls_split_wide<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, N, kpad_all, mud, muplane, mue, d_jobs, d_tbegin, njobs, ntiles, R)
         GI_SPLIT_DISPATCH(S, L_)
 #undef L_

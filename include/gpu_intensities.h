@@ -278,7 +278,8 @@ typedef struct gi_tuning {
     int halfls_cfg;
     int corr_cfg;
     int pipe_rounds;
-    int split_cfg;    /* GI_MODE_SPLIT half-ls tiles: 0 = 64 x 64, 1 = 128 x 64 */
+    int split_cfg;    /* GI_MODE_SPLIT half-ls tiles: 0 = 64 x 64, 1 = 128 x 64,
+                         2 = 64 x 64 with two K steps of loads in flight */
     int pipe_share;   /* percent of the remaining work per round but the last */
 } gi_tuning;
 int gi_set_tuning(gi_ctx *ctx, const gi_tuning *t);

@@ -13,7 +13,7 @@ from test_gpu_parity import compare
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = dict(halfls_cfg=3, corr_cfg=0, pipe_rounds=4, split_cfg=0, pipe_share=50)
+DEFAULT = dict(halfls_cfg=3, corr_cfg=0, pipe_rounds=4, split_cfg=2, pipe_share=50)
 
 
 @pytest.fixture
@@ -39,7 +39,7 @@ def test_autotune_picks_a_setting_and_keeps_the_bits(tuned_engine, mode):
     assert lines[0].startswith("# gi_autotune") and lines[-1].startswith("best ")
     kinds = [ln.split()[0] for ln in lines[1:-1]]
     want = (["halfls_tile"] * 4 + ["corr_tile"] * 4 if mode == binding.GI_MODE_FP64_BATCHED
-            else ["split_halfls_tile"] * 2) + ["pipe_rounds"] * 6 + ["pipe_share"] * 4
+            else ["split_halfls_tile"] * 3) + ["pipe_rounds"] * 6 + ["pipe_share"] * 4
     assert kinds == want
     assert lines[-1] == ("best halfls_tile=%d corr_tile=%d pipe_rounds=%d split_tile=%d "
                          "pipe_share=%d" % (best["halfls_cfg"], best["corr_cfg"],
@@ -77,10 +77,11 @@ def test_every_round_count_gives_the_same_bits(tuned_engine, monkeypatch):
     # the split half-ls tile shapes: exact integer level sums, so the same bits
     eng.set_tuning(3, 0, 2, 0)
     ref = _copy(eng.compute(mode=binding.GI_MODE_SPLIT))
-    eng.set_tuning(3, 0, 2, 1)
-    got = eng.compute(mode=binding.GI_MODE_SPLIT)
-    for k in ref:
-        assert np.array_equal(ref[k], got[k]), ("split tile", k)
+    for cfg in (1, 2):
+        eng.set_tuning(3, 0, 2, cfg)
+        got = eng.compute(mode=binding.GI_MODE_SPLIT)
+        for k in ref:
+            assert np.array_equal(ref[k], got[k]), ("split tile", cfg, k)
 
 
 def test_autotune_plain_mode_has_nothing_to_tune(tuned_engine):
@@ -96,7 +97,7 @@ def test_set_tuning_rules(tuned_engine):
     eng = tuned_engine
     job = synth.make_job("C2V", nvib=16, jvals=(0, 1), seed=54)
     eng.load(job)
-    for bad in ((4, 0, 2, 0), (0, -1, 2, 0), (0, 0, 0, 0), (0, 0, 65, 0), (0, 0, 2, 2),
+    for bad in ((4, 0, 2, 0), (0, -1, 2, 0), (0, 0, 0, 0), (0, 0, 65, 0), (0, 0, 2, 3),
                 (0, 0, 2, 0, 9), (0, 0, 2, 0, 91)):
         with pytest.raises(binding.GIError):
             eng.set_tuning(*bad)
