@@ -41,6 +41,8 @@ import numpy as np
 C2V = dict(name="C2V(M)", nrep=4, degen=[1, 1, 1, 1], labels=["A1", "A2", "B1", "B2"],
            sym_type=1)
 C3V = dict(name="C3V(M)", nrep=3, degen=[1, 1, 2], labels=["A1", "A2", "E"], sym_type=2)
+D3H = dict(name="D3H(M)", nrep=6, degen=[1, 1, 2, 1, 1, 2],
+           labels=["A1'", "A2'", "E'", 'A1"', 'A2"', 'E"'], sym_type=3)
 
 
 @dataclass
@@ -282,6 +284,42 @@ def build_bset_c3v(J, vib_units, rng):
                 nsize=Ntotal.copy())
 
 
+def direct_sum_bsets(bA, bB, nrepA):
+    """One basis whose irreps 0..nrepA-1 are those of bA and the next nrepA
+    those of bB (cases of bA, then of bB): a structurally valid basis of a
+    group with twice the irreps, used for the D3H(M) generator (the ' and "
+    functions of two C3v-like bases). Every field keeps the layout of the
+    reference's bset_contr (N[g*ncases + c], repres[t + comp*Ntotal + ib*degen*Ntotal])."""
+    nA = bA.Maxsymcoeffs
+    nB = bB.Maxsymcoeffs
+    nc = nA + nB
+    dA, dB = bA.Maxcontracts, bB.Maxcontracts
+    icA, icB = bA.icontr2icase, bB.icontr2icase
+    ic = np.concatenate([icA[:dA], icB[:dB] + nA, icA[dA:], icB[dB:]]).astype(np.int32)
+    ms = max(bA.mat_size, bB.mat_size)
+    N = np.zeros((2 * nrepA, nc), np.int32)
+    N[:nrepA, :nA] = bA.N.reshape(nrepA, nA)
+    N[nrepA:, nA:] = bB.N.reshape(nrepA, nB)
+    ijterms = np.zeros((2 * nrepA, nc), np.int32)
+    for g in range(2 * nrepA):
+        ijterms[g, 1:] = np.cumsum(N[g, :-1])
+    Ntotal = N.sum(axis=1).astype(np.int32)
+    degen = [1, 1, 2] * 2
+    repres = []
+    for g in range(2 * nrepA):
+        src, sg = (bA, g) if g < nrepA else (bB, g - nrepA)
+        out = np.zeros(int(Ntotal[g]) * degen[g] * ms, np.float64)
+        old = src.repres[sg]
+        out[:old.size] = old  # ib * degen * Ntotal: the same offsets for ib < the old mat_size
+        repres.append(out)
+    return Bset(jval=bA.jval, Maxsymcoeffs=nc, Maxcontracts=dA + dB, mat_size=ms,
+                icontr2icase=ic,
+                iroot_correlat_j0=np.concatenate([bA.iroot_correlat_j0, bB.iroot_correlat_j0]),
+                k=np.concatenate([bA.k, bB.k]), ktau=np.concatenate([bA.ktau, bB.ktau]),
+                N=N.reshape(-1).copy(), Ntotal=Ntotal, ijterms=ijterms.reshape(-1).copy(),
+                repres=repres, nsize=Ntotal.copy())
+
+
 def make_dipole(nvib, rng, symmetric=True):
     """Flipped dipole mu_flip[vF + vI*N + a*N^2] = file_a[vI, vF]."""
     v = np.arange(nvib)
@@ -360,7 +398,8 @@ def make_job(sym="C2V", nvib=200, jvals=(0, 1), seed=1234, *, e_span=8000.0,
              symmetric_mu=True, shuffle_cases=False, vectors="numpy", nmodes=3):
     """Build a synthetic job. vectors='numpy' builds eigenvectors on the host;
     vectors='none' leaves them for the caller (e.g. built on the GPU)."""
-    sy = dict(C2V if sym.upper().startswith("C2V") else C3V)
+    su = sym.upper()
+    sy = dict(C2V if su.startswith("C2V") else D3H if su.startswith("D3H") else C3V)
     nrep = sy["nrep"]
     jv = tuple(int(j) for j in jvals)
     if len(jv) < 2 or (len(jv) > 2 and any(b <= a for a, b in zip(jv, jv[1:]))):
@@ -376,6 +415,37 @@ def make_job(sym="C2V", nvib=200, jvals=(0, 1), seed=1234, *, e_span=8000.0,
         units = [(int(gvib[v]), [v]) for v in range(nvib)]
         gns_d = [1.0, 1.0, 3.0, 3.0]
         pairs_d = [1, 1, 2, 2]
+        nclasses = 1
+    elif sy["sym_type"] == 3:
+        # D3h: C3v-like units (A1, A2 singles, E pairs), each ' or "; the basis
+        # is the direct sum of the C3v bases of the ' and of the " units, and
+        # the ' units take the first vibrational indices so that the J = 0
+        # functions of that basis come in v order
+        rk, rp = _rng(seed, 3), _rng(seed, 7)
+        specs, n = [], 0
+        while n < nvib:
+            kd, p = int(rk.integers(0, 3)), int(rp.integers(0, 2))
+            if kd == 2 and n + 2 <= nvib:
+                specs.append((2, 2, p))
+                n += 2
+            else:
+                specs.append((kd if kd < 2 else 0, 1, p))
+                n += 1
+        units, v = [], 0
+        for want in (0, 1):
+            for kd, sz, p in specs:
+                if p == want:
+                    units.append((kd, list(range(v, v + sz)), p))
+                    v += sz
+        bsets = []
+        for J in jv:
+            b1 = build_bset_c3v(J, [(g, vs) for g, vs, p in units if p == 0], _rng(seed, 2, J))
+            b2 = build_bset_c3v(J, [(g, vs) for g, vs, p in units if p == 1],
+                                _rng(seed, 8, J))
+            bsets.append(direct_sum_bsets(b1, b2, 3))
+        units = [(g + 3 * p, vs) for g, vs, p in units]
+        gns_d = [1.0, 1.0, 1.0, 1.0, 1.0, 1.0]
+        pairs_d = [1, 2, 3, 2, 1, 3]  # A1' - A2", A2' - A1", E' - E"
         nclasses = 1
     else:
         # C3v units: A1, A2 singles and E pairs; nvib counts functions

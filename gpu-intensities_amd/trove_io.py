@@ -46,6 +46,11 @@ def _j0_basis(job):
     if job.sym["sym_type"] == 1:
         gvib = np.array([g for g, _ in units])
         return synth.build_bset_c2v(0, gvib, synth._rng(0, 7))
+    if job.sym["sym_type"] == 3:  # D3h: the ' units (irreps 0-2), then the " units
+        b1 = synth.build_bset_c3v(0, [(g, vs) for g, vs in units if g < 3], synth._rng(0, 7))
+        b2 = synth.build_bset_c3v(0, [(g - 3, vs) for g, vs in units if g >= 3],
+                                  synth._rng(0, 8))
+        return synth.direct_sum_bsets(b1, b2, 3)
     return synth.build_bset_c3v(0, units, synth._rng(0, 7))
 
 

@@ -158,6 +158,21 @@ def test_c3v_degenerate_parity(engine, reduced, mode):
 
 
 @pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("reduced", [0, 1])
+def test_d3h_six_irreps_parity(engine, reduced, mode):
+    """D3H(M): six irreps, two of them degenerate, dipole partners A1'-A2",
+    A2'-A1", E'-E" (fields.cpp:70-105); the synthetic basis is the direct sum
+    of two C3v-like bases (the ' and the " vibrational functions)."""
+    job = synth.make_job("D3H", nvib=24, jvals=(2, 3), seed=61, reduced=reduced)
+    engine.load(job)
+    got = engine.compute(mode=mode)
+    ref = oracle_py.OracleJob(job).run()
+    st = compare(got, ref)
+    assert st["n"] > 100
+    assert set(np.unique(job.levels["igamma"][got["ilevelI"]])) <= set(range(6))
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_windows_quanta_thresholds(engine, mode):
     job = synth.make_job("C2V", nvib=40, jvals=(3, 4), seed=9, e_span=20000.0,
                          freq_window=(-3000.0, 5000.0), erange_lower=(1000.0, 12000.0),

@@ -67,6 +67,35 @@ def test_split_c3v_vs_oracle(engine, reduced):
     assert rep["max"] <= TOL_S
 
 
+def test_split_d3h_vs_oracle(engine):
+    job = synth.make_job("D3H", nvib=24, jvals=(2, 3), seed=61)
+    engine.load(job)
+    rep = split_error(engine.compute(mode=binding.GI_MODE_SPLIT), oracle_py.OracleJob(job).run())
+    print(f"split D3h: {rep}")
+    assert rep["max"] <= TOL_S
+
+
+def test_split_d3h_reduced_digits(engine):
+    """D3h with SYMMETRY REDUCED: at the default 5 digits one line of this
+    synthetic job (cancellation in its sums) ends 1.2e-6 from FP64, above the
+    1e-6 bar; 6 digits of radix 254 bring every line within it. The error is
+    the truncation of the digits, not the structure: it falls by about the
+    radix per digit."""
+    job = synth.make_job("D3H", nvib=24, jvals=(2, 3), seed=61, reduced=1)
+    engine.load(job)
+    ref = oracle_py.OracleJob(job).run()
+    try:
+        e5 = split_error(engine.compute(mode=binding.GI_MODE_SPLIT), ref)["max"]
+        engine.set_split_digits(6)
+        e6 = split_error(engine.compute(mode=binding.GI_MODE_SPLIT), ref)["max"]
+        assert engine.split_format() == (254, 6, 6)
+    finally:
+        engine.set_split_format(0, 0, 0)
+    print(f"split D3h reduced: 5 digits {e5:.2e}, 6 digits {e6:.2e}")
+    assert e6 <= TOL_S
+    assert e5 > 30 * e6 and e5 < 1e-5
+
+
 def test_split_larger_many_rounds(engine, monkeypatch):
     """Several chunks per group and a tiny round budget (many rounds)."""
     job = synth.make_job("C2V", nvib=64, jvals=(4, 5), seed=31, e_span=30000.0,

@@ -25,6 +25,7 @@ REF_SO = os.path.join(ROOT, "oracle", "_ref", "libtrove_ref.so")
 CASES = [
     dict(sym="C2V", nvib=16, jvals=(0, 1), seed=3),
     dict(sym="C3V", nvib=12, jvals=(2, 3), seed=4, reduced=1),
+    dict(sym="D3H", nvib=14, jvals=(1, 2), seed=6),
     dict(sym="C2V", nvib=24, jvals=(3, 4), seed=5, erange_lower=(100.0, 6000.0),
          erange_upper=(50.0, 7000.0), shuffle_cases=True),
 ]
@@ -287,9 +288,9 @@ def _printf_line(t_lv, labels, nm, ncl, zpe, res, r, cg, normal):
     return s + "\n"
 
 
-@pytest.mark.parametrize("sym", ["C2V", "C3V"])
+@pytest.mark.parametrize("sym", ["C2V", "C3V", "D3H"])
 def test_line_layout(tmp_path, sym):
-    job = synth.make_job(sym, nvib=12 if sym == "C3V" else 16, jvals=(1, 2), seed=12)
+    job = synth.make_job(sym, nvib=16 if sym == "C2V" else 12, jvals=(1, 2), seed=12)
     d = str(tmp_path)
     deck = trove_io.write_trove(job, d)
     t = binding.Trove(d, deck)
