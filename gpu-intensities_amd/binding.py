@@ -35,7 +35,8 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_params", "gi_uplo
            "gi_prepare_split", "gi_set_jvals", "gi_set_tuning", "gi_get_tuning",
            "gi_autotune", "gi_set_split_digits_each", "gi_sweep_create", "gi_sweep_run", "gi_sweep_candidates",
            "gi_sweep_destroy", "gi_amp_width", "gi_set_split_format", "gi_get_split_format",
-           "gi_set_mu_budget", "gi_mu_parts"]
+           "gi_set_mu_budget", "gi_mu_parts",
+           "gi_set_path_limits"]
 # (TROVE_EXPORTS below: the symbols of include/gi_trove.h)
 
 
@@ -127,6 +128,7 @@ def lib():
         L.gi_sweep_destroy.argtypes = [vp]
         L.gi_set_tuning.argtypes = [vp, C.POINTER(GiTuning)]
         L.gi_get_tuning.argtypes = [vp, C.POINTER(GiTuning)]
+        L.gi_set_path_limits.argtypes = [vp, ci, ci, ci, ci, ci]
         L.gi_autotune.argtypes = [vp, ci, ci, P_i, C.POINTER(GiLines), ci, C.POINTER(GiTuning),
                                   C.c_char_p, cll, C.POINTER(cll)]
         _LIB = L
@@ -352,6 +354,13 @@ class Engine:
         t = GiTuning(halfls_cfg=halfls_cfg, corr_cfg=corr_cfg, pipe_rounds=pipe_rounds,
                      split_cfg=split_cfg, pipe_share=pipe_share)
         self._chk(lib().gi_set_tuning(self.h, C.byref(t)), "gi_set_tuning")
+
+    def set_path_limits(self, kpart_len=0, rec_lds=0, split2_steps=0, split2_segs=0,
+                        fold_split_rows=0):
+        """gi_set_path_limits: lower the routing thresholds of the batched
+        kernels (0 restores a default)."""
+        self._chk(lib().gi_set_path_limits(self.h, kpart_len, rec_lds, split2_steps,
+                                           split2_segs, fold_split_rows), "gi_set_path_limits")
 
     def get_tuning(self):
         t = GiTuning()

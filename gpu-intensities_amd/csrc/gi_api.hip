@@ -203,7 +203,7 @@ int prepare_split(gi_ctx *c)
     for (const Block &b : c->blocks)
         if (b.nrec > 0 && b.nsize > 0) {
             const int Kp = (b.nsize + 63) / 64 * 64;
-            const int parts = std::min(kMaxKParts, std::max(1, Kp / 2048));
+            const int parts = corr_parts(c, Kp);
             Kc = std::max<long long>(Kc, (long long)(Kp + 64 * parts - 1) / (64 * parts) * 64);
         }
     const auto fits = [](double R, int S, long long K) {

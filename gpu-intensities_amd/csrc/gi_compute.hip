@@ -148,6 +148,7 @@ struct Consts {
 struct Query {
     double beta, q_stat, zpe, thS, thI;
     long long cand_begin = 0, cand_end = 0;
+    double T = 0;           // temperature, for Q(T) on the device when q_stat <= 0
 };
 
 struct Item {
@@ -198,6 +199,10 @@ struct Plan {
     bool capture = false;
     std::vector<hipGraphExec_t> graphs;
     const double *qpar = nullptr;
+    // requests: (beta, Q, thresholds) of every query in device memory, Q(T)
+    // reduced there for the queries with q_stat <= 0 (no host wait)
+    bool device_q = false;
+    double *qdev = nullptr;
     long long *counts = nullptr;                   // pinned, lines per query
     double work[kWorkStats] = {};
 };
@@ -428,6 +433,7 @@ static void fill_corr_args(gi_ctx *c, const Plan &P, const Chunk &ch, unsigned c
     a.gnsI = c->gns[ch.gI]; a.zpe = q.zpe; a.beta = q.beta; a.q_stat = q.q_stat;
     a.A_coef_s_1 = K.A_coef_s_1; a.intens_cm_mol = K.intens_cm_mol;
     a.thresh_S = q.thS; a.thresh_I = q.thI;
+    a.qpar = P.qpar ? P.qpar : (P.qdev ? P.qdev + 4 * ch.query : nullptr);
 }
 
 static LineSoA shift(LineSoA s, long long o);
@@ -529,8 +535,7 @@ static int corr_kparts(const gi_ctx *c, const Plan &P, const Chunk &ch)
 {
     if (P.mode == GI_MODE_FP64) return 1;
     const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
-    const int K = P.mode == GI_MODE_SPLIT ? BF.Kp : BF.nsize;
-    return std::min(kMaxKParts, std::max(1, K / 2048));
+    return corr_parts(c, P.mode == GI_MODE_SPLIT ? BF.Kp : BF.nsize);
 }
 
 // Arguments of one chunk's intensity step from its stored amplitudes.
@@ -549,7 +554,7 @@ static IntensArgs intens_args(gi_ctx *c, const Plan &P, const Chunk &ch, unsigne
     a.gnsI = ca.gnsI; a.zpe = ca.zpe; a.beta = ca.beta; a.q_stat = ca.q_stat;
     a.A_coef_s_1 = ca.A_coef_s_1; a.intens_cm_mol = ca.intens_cm_mol;
     a.thresh_S = ca.thresh_S; a.thresh_I = ca.thresh_I;
-    a.qpar = P.qpar;
+    a.qpar = ca.qpar;
     a.kparts = corr_kparts(c, P, ch);
     a.ampx = P.ampx;
     a.ampx_stride = P.ampx_stride;
@@ -767,7 +772,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                 mj.uideg = (const int *)(d_meta + ch.off_uideg);
                 mj.G = G + og;
                 mj.nsize = c->blocks[ch.jindI * c->nrep + ch.gI].nsize;
-                mlds = mlds && GP.gW <= 4 && mj.nsize <= kRecLds;
+                mlds = mlds && GP.gW <= 4 && mj.nsize <= c->lim_rec_lds;
                 mjobs_v.push_back(mj);
                 mbegin.push_back(mj.tile_begin);
                 mubeg.push_back(munits);
@@ -1013,7 +1018,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
     int maxkp = 0;
     for (const Chunk &ch : chunks) {
         maxkp = std::max(maxkp, c->blocks[ch.jindF * c->nrep + ch.gF].Kp);
-        fused = fused && c->blocks[ch.jindF * c->nrep + ch.gF].Kp <= kFoldSplitMax;
+        fused = fused && c->blocks[ch.jindF * c->nrep + ch.gF].Kp <= c->lim_fold_split;
     }
     int maxparts = 1;
     for (const Chunk &ch : chunks) maxparts = std::max(maxparts, corr_kparts(c, P, ch));
@@ -1031,7 +1036,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
     bool g4 = true, glds = true;
     for (const Chunk &ch : chunks) {
         g4 = g4 && group_of(c, ch.jindI, ch.gI, ch.jindF).gW <= 4;
-        glds = glds && c->blocks[ch.jindI * c->nrep + ch.gI].nsize <= kRecLds &&
+        glds = glds && c->blocks[ch.jindI * c->nrep + ch.gI].nsize <= c->lim_rec_lds &&
                group_of(c, ch.jindI, ch.gI, ch.jindF).oc.size() <= (size_t)kClsLds;
     }
     glds = glds && g4;
@@ -1084,10 +1089,11 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                 gbegin.push_back(glds ? ubeg : g.tile_begin);
                 ubeg += ch.nunits;
                 for (const OutClass &oc : GP.oc) {
-                    int steps = 0;
-                    for (int n : oc.seg_n) steps += (n + 63) / 64;
-                    steps_fit = steps_fit && steps <= kSplitMaxSteps &&
-                                (int)oc.seg_n.size() <= kSplitMaxSegs;
+                    steps_fit = steps_fit &&
+                                split2_class_fits(oc.seg_n.data(), (int)oc.seg_n.size()) &&
+                                split2_steps(oc.seg_n.data(), (int)oc.seg_n.size()) <=
+                                    c->lim_split2_steps &&
+                                (int)oc.seg_n.size() <= c->lim_split2_segs;
                     SplitHalfJob h;
                     h.cd = classdev(oc);
                     h.nunits = ch.nunits;
@@ -1250,6 +1256,8 @@ static LineSoA shift(LineSoA s, long long o)
     return s;
 }
 
+static int setup_query_params(gi_ctx *c, Slot &S, Plan &P);
+
 // Enqueue the whole plan on slot S: half linestrengths, correlations,
 // intensities, per-query compaction, and the copy of the per-query line
 // counts to pinned memory. Nothing here waits for the device.
@@ -1309,6 +1317,7 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
         HIPCHK(hipMemcpyAsync(d_meta, hm, P.meta.size(), hipMemcpyHostToDevice, S.stream));
     }
 
+    if (P.device_q && (rc = setup_query_params(c, S, P)) != GI_OK) return rc;
     if (P.mode == GI_MODE_SPLIT && !P.chunks.empty()) {
         if ((rc = run_split(c, S, P, d_meta, tl, flag, cand)) != GI_OK) return rc;
     } else if (batched && !P.chunks.empty()) {
@@ -1773,12 +1782,64 @@ __global__ __launch_bounds__(256) void k_partition(int n, const double *__restri
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// The host's fixed-order sum of the partial sums of k_partition, on the
+// device (the same additions in the same order: the same bits).
+__global__ void k_partition_sum(int nb, const double *__restrict__ part, double *__restrict__ q)
+{
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int i = 0; i < nb; i++) s += part[i];
+    *q = s;
+}
+
+constexpr int kPartBlocks = 64;
+
+// The (beta, Q, thresh_S, thresh_I) block of every query of a request batch
+// in device memory, read by the intensity steps; Q(T) is reduced on the
+// device, on the batch's stream, for the queries without a Q of their own.
+static int setup_query_params(gi_ctx *c, Slot &S, Plan &P)
+{
+    int rc;
+    const size_t nq = P.q.size();
+    double *qdev = (double *)slot_scratch(c, S, "qdev", 32 * std::max<size_t>(nq, 1), &rc);
+    if (rc) return rc;
+    size_t ndev = 0;
+    for (const Query &q : P.q) ndev += !(q.q_stat > 0.0);
+    double *part = nullptr;
+    if (ndev) {
+        part = (double *)slot_scratch(c, S, "qparts", 8 * kPartBlocks * ndev, &rc);
+        if (rc) return rc;
+    }
+    unsigned char *hb = stage(c, S, nullptr, 32 * std::max<size_t>(nq, 1), &rc);
+    if (rc) return rc;
+    double *h = (double *)hb;
+    for (size_t k = 0; k < nq; k++) {
+        h[4 * k] = P.q[k].beta;
+        h[4 * k + 1] = P.q[k].q_stat > 0.0 ? P.q[k].q_stat : 0.0;
+        h[4 * k + 2] = P.q[k].thS;
+        h[4 * k + 3] = P.q[k].thI;
+    }
+    HIPCHK(hipMemcpyAsync(qdev, hb, 32 * nq, hipMemcpyHostToDevice, S.stream));
+    size_t i = 0;
+    for (size_t k = 0; k < nq; k++) {
+        if (P.q[k].q_stat > 0.0) continue;
+        double *pk = part + kPartBlocks * i++;
+        hipLaunchKernelGGL(k_partition, dim3(kPartBlocks), dim3(256), 0, S.stream, c->nlevels,
+                           c->d_lv_energy, c->d_lv_weight, c->zpe, P.q[k].beta, pk);
+        hipLaunchKernelGGL(k_partition_sum, dim3(1), dim3(64), 0, S.stream, kPartBlocks, pk,
+                           qdev + 4 * k + 1);
+    }
+    HIPCHK(hipGetLastError());
+    P.qdev = qdev;
+    return GI_OK;
+}
+
 int partition_function(gi_ctx *c, Slot &S, double T, double *q)
 {
     if (!c->prepared) return fail(c, GI_ESTATE, "partition function before gi_prepare");
     if (!(T > 0.0)) return fail(c, GI_EINVAL, "temperature <= 0");
     int rc;
-    const int nb = 64;
+    const int nb = kPartBlocks;
     double *part = (double *)slot_scratch(c, S, "qpart", nb * 8, &rc); if (rc) return rc;
     double *hp = (double *)slot_pinned(c, S, nb * 8, &rc); if (rc) return rc;
     hipLaunchKernelGGL(k_partition, dim3(nb), dim3(256), 0, S.stream, c->nlevels, c->d_lv_energy,
@@ -1865,14 +1926,14 @@ static void serve_loop(Server &sv)
             next_slot ^= 1;
             Plan &P = *b.plan;
             P.mode = adm[0]->r.mode;
+            P.device_q = true;  // Q(T) of requests without a Q is reduced on the device
             int rc = GI_OK;
             for (size_t k = 0; k < adm.size() && rc == GI_OK; k++) {
                 const gi_request &r = adm[k]->r;
                 Query q;
                 q.beta = Consts::beta(r.temperature);
                 q.q_stat = r.q_stat;
-                if (!(q.q_stat > 0.0))
-                    rc = partition_function(c, sv.slots[b.slot], r.temperature, &q.q_stat);
+                q.T = r.temperature;
                 q.zpe = c->zpe;
                 q.thS = r.thresh_linestrength;
                 q.thI = r.thresh_intensity;
@@ -2075,6 +2136,31 @@ int gi_set_tuning(gi_ctx *c, const gi_tuning *t)
     c->pipe_rounds = t->pipe_rounds;
     c->tile_split = t->split_cfg;
     c->pipe_share = t->pipe_share;
+    return GI_OK;
+}
+
+int gi_set_path_limits(gi_ctx *c, int kpart_len, int rec_lds, int split2_steps,
+                       int split2_segs, int fold_split_rows)
+{
+    if (!c) return GI_EINVAL;
+    if (c->server || !c->sweeps.empty())
+        return fail(c, GI_ESTATE, "gi_set_path_limits while the server runs or sweeps exist");
+    const auto pick = [](int v, int def, int lo, int *out) {
+        if (v == 0) v = def;
+        if (v < lo || v > def) return false;
+        *out = v;
+        return true;
+    };
+    int a, b, d, e, f;
+    if (!pick(kpart_len, 2048, 64, &a) || !pick(rec_lds, kRecLds, 1, &b) ||
+        !pick(split2_steps, kSplitMaxSteps, 1, &d) || !pick(split2_segs, kSplitMaxSegs, 1, &e) ||
+        !pick(fold_split_rows, kFoldSplitMax, 1, &f))
+        return fail(c, GI_EINVAL, "gi_set_path_limits: a limit outside [minimum, default]");
+    c->lim_kpart = a;
+    c->lim_rec_lds = b;
+    c->lim_split2_steps = d;
+    c->lim_split2_segs = e;
+    c->lim_fold_split = f;
     return GI_OK;
 }
 

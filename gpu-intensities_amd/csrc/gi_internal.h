@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -239,6 +240,12 @@ struct gi_ctx {
     int pipe_rounds = 4;
     int pipe_share = 50;             // percent of the remaining work per round but the last
     int tile_split = 2;              // split half-ls: 0 = 64 rows, 1 = 128, 2 = 2 steps in flight
+    // routing thresholds of the batched kernels (gi_set_path_limits; the
+    // defaults are the kernels' own limits): split-K part length of the
+    // correlation, longest record staged in LDS by the G builds, step and
+    // segment limits of k_halfls_split2, longest column of the fused fold-split
+    int lim_kpart = 2048, lim_rec_lds = gi::kRecLds, lim_split2_steps = gi::kSplitMaxSteps,
+        lim_split2_segs = gi::kSplitMaxSegs, lim_fold_split = gi::kFoldSplitMax;
 
     // split mode. Requested format (0 = automatic): radix 128 or 254, digits of
     // the correlation (eigenvector blocks, folded vectors) and of the half-ls
@@ -280,6 +287,13 @@ inline GroupPlan &group_of(gi_ctx *c, int jindI, int gI, int jindF)
 inline const GroupPlan &group_of(const gi_ctx *c, int jindI, int gI, int jindF)
 {
     return c->groups[((size_t)jindI * c->nrep + gI) * c->nJ + jindF];
+}
+
+// Split-K parts of a correlation product of length K (a function of the
+// upper block only, so a candidate's sums never depend on its batch).
+inline int corr_parts(const gi_ctx *c, int K)
+{
+    return std::min(kMaxKParts, std::max(1, K / c->lim_kpart));
 }
 
 // Register-tile width (columns) of the half-ls kernels for `cols` columns.

@@ -45,6 +45,7 @@ struct CorrArgs {
     const unsigned char *blk_upok;
     const double *vec, *Ht;
     double gnsI, zpe, beta, q_stat, A_coef_s_1, intens_cm_mol, thresh_S, thresh_I;
+    const double *qpar;              // non-null: beta, q_stat, thresh_S, thresh_I read here
     int *flag, *out_iI, *out_iF;
     double *out_nu, *out_S, *out_A, *out_I, *out_amp;
     int amp_m;                       // amplitudes per line amp_m^2, entry idegI + amp_m*idegF
@@ -195,6 +196,20 @@ void launch_build_g_split_lds(hipStream_t st, int S, double R, const GSplitJob *
 // most kSplitMaxSteps K steps and kSplitMaxSegs segments and at most 5 digits
 // (else the 64 x 64 kernel runs).
 constexpr int kSplitMaxSteps = 512, kSplitMaxSegs = 64;
+// K steps (64 bytes of one segment each) of a class with segments of seg_n rows.
+__host__ __device__ inline int split2_steps(const int *seg_n, int nseg)
+{
+    int n = 0;
+    for (int e = 0; e < nseg; e++) n += (seg_n[e] + 63) / 64;
+    return n;
+}
+// Whether k_halfls_split2's step table holds the class. The host routes only
+// such classes to it; a class that does not fit gets NaN results there instead
+// of silently dropped K steps.
+__host__ __device__ inline bool split2_class_fits(const int *seg_n, int nseg)
+{
+    return nseg <= kSplitMaxSegs && split2_steps(seg_n, nseg) <= kSplitMaxSteps;
+}
 void launch_halfls_split(hipStream_t st, int S, double R, int N, int kpad_all, const int8_t *mud,
                          long long muplane, const int *mue, const SplitHalfJob *d_jobs,
                          const int *d_tbegin, int njobs, int ntiles, int cfg);

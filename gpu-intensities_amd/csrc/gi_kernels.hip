@@ -351,12 +351,19 @@ __global__ __launch_bounds__(256) void k_corr_intensity(CorrArgs a)
     const double eF = a.blk_en[f];
     const double nu = eF - eI;
     const double anu = fabs(nu);
+    double beta = a.beta, q_stat = a.q_stat, thS = a.thresh_S, thI = a.thresh_I;
+    if (a.qpar) {  // a request's parameters, Q(T) reduced on the device
+        beta = a.qpar[0];
+        q_stat = a.qpar[1];
+        thS = a.qpar[2];
+        thI = a.qpar[3];
+    }
     const double boltz_fc =
-        anu * exp(-(eI - a.zpe) * a.beta) * (1.0 - exp(-anu * a.beta)) / a.q_stat;
+        anu * exp(-(eI - a.zpe) * beta) * (1.0 - exp(-anu * beta)) / q_stat;
     const double A_einst = a.A_coef_s_1 * (double)((2 * a.jI) + 1) * ls * pow(anu, 3.0);
     const double S = ls * a.gnsI * (double)((2 * a.jI + 1) * (2 * a.jF + 1));
     const double I = S * a.intens_cm_mol * boltz_fc;
-    const bool pass = a.blk_upok[f] && S >= a.thresh_S && I >= a.thresh_I;
+    const bool pass = a.blk_upok[f] && S >= thS && I >= thI;
     const long long o = a.item_out_off[item] + j;
     a.flag[o] = pass ? 1 : 0;
     a.out_iI[o] = a.item_level[item];

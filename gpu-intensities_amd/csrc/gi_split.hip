@@ -633,6 +633,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
     const int ncolsTot = J.nunits * cd.ncols;
     // step table: segment e's steps start at s_first[e]
+    __shared__ int s_fits;
     if (tid == 0) {
         int n = 0;
         for (int e = 0; e < cd.nseg && e < kSplitMaxSegs; e++) {
@@ -640,12 +641,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             n += (cd.seg_n[e] + 63) / 64;
         }
         s_first[min(cd.nseg, kSplitMaxSegs)] = n;
+        s_fits = split2_class_fits(cd.seg_n, cd.nseg);
     }
     __syncthreads();
-    // the host sends only classes within both limits to this kernel
-    const int nsteps = min(s_first[min(cd.nseg, kSplitMaxSegs)], kSplitMaxSteps);
-    for (int e = 0; e < cd.nseg && e < kSplitMaxSegs; e++)
-        for (int st = s_first[e] + tid; st < s_first[e + 1] && st < kSplitMaxSteps; st += 256) {
+    // the host sends only classes within both limits to this kernel; one that
+    // is not gets NaN in its tile rather than a sum with K steps left out
+    if (!s_fits) {
+        for (int i = tid; i < 64 * 64; i += 256) {
+            const int m = m0 + (i & 63), nn = n0 + (i >> 6);
+            if (m >= cd.n || nn >= ncolsTot) continue;
+            const int u = nn / cd.ncols, jj = nn - u * cd.ncols;
+            J.H[(size_t)u * J.Htot + cd.hbase + (size_t)jj * cd.n + m] = __builtin_nan("");
+        }
+        return;
+    }
+    const int nsteps = s_first[cd.nseg];
+    for (int e = 0; e < cd.nseg; e++)
+        for (int st = s_first[e] + tid; st < s_first[e + 1]; st += 256) {
             const int kk = (st - s_first[e]) * 64;
             s_a[st] = ((long long)cd.seg_a[e] * N + cd.off + m0) * kpad_all +
                       cd.seg_pad[2 * e + 1] + kk;
@@ -935,7 +947,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             rb[q] = nok ? *(const v4i *)(a.Hd + q * a.hplane + brow + k0 + lp) : v4i{0, 0, 0, 0};
         }
     };
-    fetch(kb);
+    if (kb < ke) fetch(kb);  // a part past the end of K (kb >= ke) stays zero
     for (int k0 = kb; k0 < ke; k0 += 64) {
 #pragma unroll
         for (int q = 0; q < S; q++) {

@@ -285,6 +285,22 @@ typedef struct gi_tuning {
 int gi_set_tuning(gi_ctx *ctx, const gi_tuning *t);
 int gi_get_tuning(gi_ctx *ctx, gi_tuning *t);
 
+/* Routing thresholds of the batched modes (no reference counterpart), for
+ * tests that reach at small sizes the kernels real sizes select: kpart_len,
+ * the correlation K per split-K part (parts = min(4, K / kpart_len); default
+ * 2048, at least 64); rec_lds, the longest eigenvector record the G builds
+ * stage in LDS (longer ones use the global-gather builds; default 16384);
+ * split2_steps / split2_segs, the largest class (64-byte K steps, segments)
+ * the two-step split half-ls takes (larger ones run the one-step kernel;
+ * default 512 / 64); fold_split_rows, the longest folded column the fused
+ * fold-and-digit-split kernel takes (default 32768). 0 restores a default;
+ * values above the default are GI_EINVAL (the kernels' own limits). The
+ * global-gather builds, the one-step split kernel and the two-kernel fold
+ * give bitwise the same lines as the paths they replace; another split-K
+ * part count changes the correlation's summation order. */
+int gi_set_path_limits(gi_ctx *ctx, int kpart_len, int rec_lds, int split2_steps,
+                       int split2_segs, int fold_split_rows);
+
 /* Autotuner (no reference counterpart): runs gi_compute in `mode` over the
  * given lower levels (NULL = all) `reps` times per setting into `out`,
  * and keeps the fastest setting of each knob: the two tile shapes (FP64

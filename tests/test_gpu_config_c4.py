@@ -1,0 +1,91 @@
+"""The C3v(M) J = 0..15 workload at full size, FP64 and split precision.
+
+One context holds C3v(M) (A1, A2, E) J = 0..15 at Nvib = 3000: 48 symmetry
+blocks of 736 to 30,736 functions (65 GB of FP64 eigenvectors). The E blocks
+are degenerate (two components, two-term symmetrisation rows) and from J = 8
+longer than the 16,384 doubles the G builds stage in LDS. The sample (J = 1
+every 4th lower level and four J = 15 E-block lower levels) runs in both
+matrix-core modes against the oracle's chain of pair runs; the split mode's
+error over lines above 1e-6 of the strongest is held to 1e-6 at 296 K and, for
+a J = 15 E-block request with Q(T) from the device, at 1500 K.
+"""
+import dataclasses
+import time
+
+import numpy as np
+import pytest
+
+from gpu_intensities_amd import binding
+import oracle_py
+import config_util as cu
+from test_gpu_parity import compare
+from test_gpu_split import split_error
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
+
+E = 2  # the E irrep of C3v(M)
+
+
+@pytest.fixture(scope="module")
+def c4():
+    job, dev, eng = cu.load("c4")
+    assert int(job.sym["degen"][E]) == 2
+    low = cu.lower_ok(job)
+    lv = job.levels
+    J, g = lv["jval"][low], lv["igamma"][low]
+    top = low[(J == 15) & (g == E)]
+    sample = np.sort(np.concatenate([low[J == 1][::4],
+                                     top[:: max(1, len(top) // 4)][:4]])).astype(np.int32)
+    cu.to_host(job, dev, cu.blocks_read_by(job, sample))
+    t = time.time()
+    ref = oracle_py.run_chain(job, sample)
+    cu.say(f"c4: oracle chain on {len(sample)} lower levels, {len(ref['S'])} lines, "
+           f"{time.time() - t:.1f}s")
+    yield job, eng, sample, ref
+    cu.release(job, dev, eng)
+
+
+def test_c4_sizes(c4):
+    job, eng, sample, ref = c4
+    assert len(job.jvals) == 16 and job.nvib == 3000
+    assert int(job.bsets[15].nsize[E]) > 16384
+    assert (job.levels["jval"][sample] == 15).sum() == 4
+
+
+def test_c4_fp64_batched_vs_oracle(c4):
+    job, eng, sample, ref = c4
+    rep = compare(eng.compute(sample, mode=binding.GI_MODE_FP64_BATCHED), ref)
+    cu.say(f"c4 fp64_batched: {rep}")
+
+
+def test_c4_split_296K(c4):
+    job, eng, sample, ref = c4
+    rep = split_error(eng.compute(sample, mode=binding.GI_MODE_SPLIT), ref)
+    cu.say(f"c4 split 296 K: {rep}")
+    assert rep["max"] <= 1e-6
+
+
+def test_c4_split_1500K_request(c4):
+    """A J = 15 -> 15 E-block request of the lower levels with
+    E - ZPE <= 300 cm^-1 at 1500 K, Q(T) from the device."""
+    job, eng, sample, ref = c4
+    T = 1500.0
+    eng.serve_start(8 << 30)
+    try:
+        got = eng.serve_fetch(eng.serve_submit(jindI=15, igammaI=E, jindF=15,
+                                               mode=binding.GI_MODE_SPLIT, temperature=T,
+                                               q_stat=0.0, elow=(0.0, 300.0)))
+    finally:
+        eng.serve_stop()
+    Q = eng.partition_function(T)
+    lv = job.levels
+    e = lv["energy"] - job.zpe
+    ids = np.nonzero((lv["jind"] == 15) & (lv["igamma"] == E) & (e >= 0.0) &
+                     (e <= 300.0))[0].astype(np.int32)
+    assert len(ids) > 0
+    r = oracle_py.run_chain(dataclasses.replace(job, temperature=T, q_stat=Q), ids)
+    keep = lv["jind"][r["ilevelF"]] == 15
+    r = {k: v[keep] for k, v in r.items()}
+    rep = split_error(got, r)
+    cu.say(f"c4 split 1500 K request ({len(ids)} lower levels): {rep}")
+    assert rep["max"] <= 1e-6

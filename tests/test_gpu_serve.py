@@ -125,19 +125,33 @@ def test_request_equals_compute_subset(engine, job):
         engine.serve_stop()
 
 
-def test_auto_partition_function(engine, job):
-    """q_stat <= 0: the request uses gi_partition_function(T)."""
+@pytest.mark.parametrize("mode", [binding.GI_MODE_FP64, binding.GI_MODE_FP64_BATCHED,
+                                  binding.GI_MODE_SPLIT])
+def test_auto_partition_function(engine, job, mode):
+    """q_stat <= 0: the request uses gi_partition_function(T), reduced on the
+    device on the batch's stream (the same additions in the same order, so
+    the same bits), also for several temperatures in one batch."""
     engine.load(job)
-    T = 1200.0
-    Q = engine.partition_function(T)
+    if mode == binding.GI_MODE_SPLIT:
+        engine.prepare_split()
+    Ts = (1200.0, 296.0, 2000.0)
+    Qs = [engine.partition_function(T) for T in Ts]
     engine.serve_start(1 << 30)
     try:
-        a = _alone(engine, jindI=1, igammaI=0, jindF=1, temperature=T, q_stat=0.0)
-        b = _alone(engine, jindI=1, igammaI=0, jindF=1, temperature=T, q_stat=Q)
+        a = [_alone(engine, jindI=1, igammaI=0, jindF=1, temperature=T, q_stat=0.0, mode=mode)
+             for T in Ts]
+        b = [_alone(engine, jindI=1, igammaI=0, jindF=1, temperature=T, q_stat=Q, mode=mode)
+             for T, Q in zip(Ts, Qs)]
+        tickets = [engine.serve_submit(jindI=1, igammaI=0, jindF=1, temperature=T, q_stat=-1.0,
+                                       mode=mode) for T in Ts]
+        c = [engine.serve_fetch(t) for t in tickets]
     finally:
         engine.serve_stop()
-    assert len(a["S"]) > 0
-    _same(a, b)
+    for x, y, z in zip(a, b, c):
+        assert len(x["S"]) > 0
+        _same(x, y)
+        _same(x, z)
+    assert not np.array_equal(a[0]["I"], a[1]["I"])
 
 
 def test_mix_equals_alone(engine, job):

@@ -411,6 +411,65 @@ def test_intensity_step_closed_form():
         assert r["I"][i] == pytest.approx(Int, rel=1e-13)
 
 
+def _hitran_residual(job, r, A, I):
+    """Relative residual of the Einstein-A / line-intensity relation of
+    Simeckova et al. (JQSRT 98, 2006, the relation HITRAN uses):
+        I / N_A = g' A / (8 pi c nu^2) * exp(-c2 E''/T) * (1 - exp(-c2 nu/T)) / Q
+    with g' = gns (2J' + 1) the upper level's weight, E'' the lower energy above
+    the ZPE, c2 = h c / k, and N_A, c, h, k of cuda_host.cu:12-18 (the
+    reference's I is per mole). Lines with nu > 0 (absorption from I to F)."""
+    k = O.constants()
+    lv = job.levels
+    iI, iF = r["ilevelI"], r["ilevelF"]
+    nu = lv["energy"][iF] - lv["energy"][iI]
+    m = nu > 0
+    c2 = k["planck"] * k["vellgt"] / k["boltz"]
+    T = job.temperature
+    gup = np.asarray(job.gns)[lv["igamma"][iF]] * (2 * lv["jval"][iF] + 1)
+    rhs = (gup[m] * A[m] / (8 * math.pi * k["vellgt"] * nu[m] ** 2) *
+           np.exp(-c2 * (lv["energy"][iI][m] - job.zpe) / T) *
+           (1 - np.exp(-c2 * nu[m] / T)) / job.q_stat)
+    lhs = I[m] / k["avogno"]
+    return np.abs(lhs - rhs) / np.abs(rhs), m
+
+
+@pytest.mark.parametrize("sym,jvals", [("C2V", (1, 2)), ("C3V", (2, 3))])
+@pytest.mark.parametrize("T", [296.0, 1000.0, 2500.0])
+def test_einstein_a_against_line_intensity(sym, jvals, T):
+    """The emitted A_einst and I satisfy the HITRAN Einstein-A / intensity
+    relation line by line (an independent statement of A4-A5: it fixes the
+    (2J+1) factor of A to the lower J, the nu^3 of A against the nu of I, the
+    Boltzmann and stimulated-emission factors and the ratio of the two
+    constants). Mistakes planted in the oracle's A or I fail it."""
+    job = synth.make_job(sym, nvib=24, jvals=jvals, seed=9, temperature=T, e_span=6000.0)
+    r = O.OracleJob(job).run()
+    res, m = _hitran_residual(job, r, r["A_einst"], r["I"])
+    assert m.sum() > 50
+    assert res.max() <= 1e-12, res.max()
+    lv = job.levels
+    jI, jF = lv["jval"][r["ilevelI"]], lv["jval"][r["ilevelF"]]
+    nu = np.abs(lv["energy"][r["ilevelF"]] - lv["energy"][r["ilevelI"]])
+    dj = (jI != jF)[m]
+    assert dj.any()
+    # (2J'+1) in A instead of (2J+1): fails on every line with J' != J
+    bad, _ = _hitran_residual(job, r, r["A_einst"] * (2 * jF + 1) / (2 * jI + 1), r["I"])
+    assert (bad[dj] > 1e-3).all()
+    # nu^4 (or nu^2) in A: fails wherever nu != 1
+    bad, _ = _hitran_residual(job, r, r["A_einst"] * nu, r["I"])
+    assert (bad[np.abs(nu[m] - 1) > 1e-3] > 1e-3).all()
+    bad, _ = _hitran_residual(job, r, r["A_einst"] / nu, r["I"])
+    assert (bad[np.abs(nu[m] - 1) > 1e-3] > 1e-3).all()
+    # I without the stimulated-emission factor: visible on the lines where that
+    # factor differs from 1 (low nu against kT)
+    k = O.constants()
+    beta = k["planck"] * k["vellgt"] / (k["boltz"] * T)
+    stim = 1 - np.exp(-nu * beta)
+    bad, _ = _hitran_residual(job, r, r["A_einst"], r["I"] / stim)
+    sel = stim[m] < 1 - 1e-6
+    assert sel.sum() > 10
+    assert (bad[sel] > 1e-7).all()
+
+
 def test_b8_q_branch_of_first_j_is_excluded():
     """trove_functions.cpp:1457: no line with J = J' = jvals[0]."""
     job = synth.make_job("C2V", nvib=16, jvals=(1, 2), seed=4)
