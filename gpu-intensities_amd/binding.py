@@ -35,7 +35,7 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_params", "gi_uplo
            "gi_prepare_split", "gi_set_jvals", "gi_set_tuning", "gi_get_tuning",
            "gi_autotune", "gi_set_split_digits_each", "gi_sweep_create", "gi_sweep_run", "gi_sweep_candidates",
            "gi_sweep_destroy", "gi_amp_width", "gi_set_split_format", "gi_get_split_format",
-           "gi_set_mu_budget", "gi_mu_parts",
+           "gi_set_mu_budget", "gi_mu_parts", "gi_half_ls_noalloc",
            "gi_set_path_limits"]
 # (TROVE_EXPORTS below: the symbols of include/gi_trove.h)
 
@@ -103,6 +103,7 @@ def lib():
         L.gi_count_candidates.argtypes = [vp, ci, P_i, C.POINTER(cll)]
         L.gi_amp_width.argtypes = [vp]
         L.gi_half_ls.argtypes = [vp, ci, ci, ci, ci, P_d, P_d]
+        L.gi_half_ls_noalloc.argtypes = [vp, ci, ci, ci, ci, P_d, P_d]
         L.gi_correlate.argtypes = [vp, ci, ci, ci, P_d, P_d]
         L.gi_threej_table.argtypes = [ci, P_d]
         L.gi_kernel_stats.argtypes = [vp, ci, C.POINTER(C.c_char_p), P_d, P_i, P_i]
@@ -483,6 +484,15 @@ class Engine:
         h = np.zeros(self.job.bsets[jindF].Maxcontracts)
         self._chk(lib().gi_half_ls(self.h, jindI, jindF, g, ideg, _p(v, P_d), _p(h, P_d)),
                   "gi_half_ls")
+        return h
+
+    def half_ls_noalloc(self, jindI, jindF, g, ideg, vec):
+        """gi_half_ls_noalloc: gi_half_ls with its device buffers in the
+        library's static scratch array (no allocation)."""
+        v = np.ascontiguousarray(vec, np.float64)
+        h = np.zeros(self.job.bsets[jindF].Maxcontracts)
+        self._chk(lib().gi_half_ls_noalloc(self.h, jindI, jindF, g, ideg, _p(v, P_d),
+                                           _p(h, P_d)), "gi_half_ls_noalloc")
         return h
 
     def correlate(self, jind, g, ideg, vec):

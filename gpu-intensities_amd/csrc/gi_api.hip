@@ -322,6 +322,7 @@ int gi_create(int device, gi_ctx **out)
         return GI_EHIP;
     }
     c->main_slot.stream = c->stream;
+    noalloc_init(device);  // once per device; a failure is reported by gi_half_ls_noalloc
     if (hipStreamCreate(&c->main_slot.cstream) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;

@@ -311,6 +311,8 @@ void free_slot(Slot &s);
 void server_shutdown(gi_ctx *c);
 void sweeps_shutdown(gi_ctx *c);
 int partition_function(gi_ctx *c, Slot &S, double T, double *q);
+// Looks up the no-allocation scratch array's address on `device` (current).
+void noalloc_init(int device);
 
 inline SymDev symdev(const SymMap &m)
 {

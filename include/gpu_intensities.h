@@ -199,6 +199,18 @@ int gi_amp_width(const gi_ctx *ctx);
 int gi_half_ls(gi_ctx *ctx, int jindI, int jindF, int igammaI, int ideg,
                const double *vec, double *h);
 
+/* gi_half_ls without allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same arguments,
+ * result and errors as gi_half_ls, but every device buffer (the lower vector,
+ * G, H and the result) lives in one static 64 MiB device array per GPU, so the
+ * call makes no hipMalloc, hipFree or other allocation. GI_EINVAL when the
+ * job needs more than the array holds (about Nvib <= 25,000). Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (several host
+ * threads, or contexts used concurrently) race. */
+int gi_half_ls_noalloc(gi_ctx *ctx, int jindI, int jindF, int igammaI, int ideg,
+                       const double *vec, double *h);
+
 /* device_correlate_vectors (dipole_kernals.cu:38-71): out (host, dimen of
  * jvals[jind]) = symmetrised -> primitive transform of vec (host, nsize). */
 int gi_correlate(gi_ctx *ctx, int jind, int igamma, int ideg, const double *vec, double *out);
