@@ -524,6 +524,7 @@ class Engine:
 
 # -- TROVE files, deck and line list (include/gi_trove.h) ---------------------
 TROVE_EXPORTS = ["gi_trove_open", "gi_trove_error", "gi_trove_close", "gi_trove_load",
+                 "gi_trove_apply_thresh_line",
                  "gi_trove_params", "gi_trove_bset", "gi_trove_dipole", "gi_trove_levels",
                  "gi_trove_vectors", "gi_trove_header", "gi_trove_format"]
 _TROVE_READY = False
@@ -541,6 +542,7 @@ def _trove_lib():
         L.gi_trove_close.argtypes = [vp]
         L.gi_trove_close.restype = None
         L.gi_trove_load.argtypes = [vp, vp]
+        L.gi_trove_apply_thresh_line.argtypes = [vp, ci]
         L.gi_trove_params.argtypes = [vp, C.POINTER(GiParams)]
         L.gi_trove_bset.argtypes = [vp, ci, C.POINTER(GiBset)]
         L.gi_trove_dipole.argtypes = [vp, P_i, PP_d]
@@ -636,7 +638,10 @@ class Trove:
         _trove_lib().gi_trove_vectors(self.h, jind, g, C.byref(ns), C.byref(nr), C.byref(p))
         return _arr(p, ns.value * nr.value, np.float64).reshape(nr.value, ns.value)
 
-    def load(self, engine):
+    def load(self, engine, apply_thresh_line=False):
+        """gi_trove_load; THRESH_LINE is applied only when asked (the
+        reference parses it and never applies it)."""
+        _trove_lib().gi_trove_apply_thresh_line(self.h, int(apply_thresh_line))
         rc = _trove_lib().gi_trove_load(self.h, engine.h)
         engine._chk(rc, "gi_trove_load")
 

@@ -203,7 +203,7 @@ int prepare_split(gi_ctx *c)
     for (const Block &b : c->blocks)
         if (b.nrec > 0 && b.nsize > 0) {
             const int Kp = (b.nsize + 63) / 64 * 64;
-            const int parts = corr_parts(c, Kp);
+            const int parts = corr_parts(c, Kp, true);
             Kc = std::max<long long>(Kc, (long long)(Kp + 64 * parts - 1) / (64 * parts) * 64);
         }
     const auto fits = [](double R, int S, long long K) {
@@ -322,6 +322,10 @@ int gi_create(int device, gi_ctx **out)
         return GI_EHIP;
     }
     c->main_slot.stream = c->stream;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        ncu > 0)
+        c->ncu = ncu;
     noalloc_init(device);  // once per device; a failure is reported by gi_half_ls_noalloc
     if (hipStreamCreate(&c->main_slot.cstream) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
@@ -869,7 +873,7 @@ int gi_prepare_split(gi_ctx *c)
 
 int gi_set_tiles(gi_ctx *c, int hl, int cr)
 {
-    if (!c || hl < -1 || hl > 3 || cr < -1 || cr > 3) return GI_EINVAL;
+    if (!c || hl < -1 || hl > kTileHalflsMax || cr < -1 || cr > 3) return GI_EINVAL;
     if (serving(c)) return GI_ESTATE;
     if (hl >= 0) c->tile_halfls = hl;
     if (cr >= 0) c->tile_corr = cr;

@@ -55,6 +55,38 @@ struct Tile {
     static constexpr int LA = BM_ * BK / 256, LB = BN_ * BK / 256;
 };
 
+// Half-ls tiles with NW_ wavefronts in a 2 x (NW_/2) layout (4: 2 x 2 as
+// Tile; 8: each wave owns (BM/2) x (BN/4), half the accumulators, so twice
+// the waves fit in the register file).
+template <int BM_, int BN_, int NW_>
+struct TileW {
+    static constexpr int NT = 64 * NW_, WR = 2, WC = NW_ / 2;
+    static constexpr int PA = BM_ + 4, PB = BN_ + 4;
+    static constexpr int MI = BM_ / (16 * WR), NI = BN_ / (16 * WC);
+    static constexpr int LA = BM_ * BK / NT, LB = BN_ * BK / NT;
+};
+
+template <int MI, int NI, int PA, int PB>
+__device__ inline void mfma_step_w(const double *__restrict__ As, const double *__restrict__ Bs,
+                                   int wm, int wn, int lane, d4 (&acc)[MI][NI])
+{
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+        const int k = kk + (lane >> 4);
+        const int i = lane & 15;
+        double a[MI], b[NI];
+#pragma unroll
+        for (int x = 0; x < MI; x++) a[x] = As[k * PA + wm + 16 * x + i];
+#pragma unroll
+        for (int y = 0; y < NI; y++) b[y] = Bs[k * PB + wn + 16 * y + i];
+#pragma unroll
+        for (int x = 0; x < MI; x++)
+#pragma unroll
+            for (int y = 0; y < NI; y++)
+                acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
+    }
+}
+
 template <int BM_, int BN_>
 __device__ inline void mfma_step(const double *__restrict__ As, const double *__restrict__ Bs,
                                  int wm, int wn, int lane,
@@ -76,6 +108,55 @@ __device__ inline void mfma_step(const double *__restrict__ As, const double *__
             for (int y = 0; y < T::NI; y++)
                 acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
     }
+}
+
+// mfma_step for the banded correlation: only the wave's 16 x 16 blocks whose
+// bit is set in `need` (wave-uniform) are multiplied; the others stay zero.
+template <int BM_, int BN_>
+__device__ inline void mfma_step_mask(const double *__restrict__ As,
+                                      const double *__restrict__ Bs, int wm, int wn, int lane,
+                                      unsigned need,
+                                      d4 (&acc)[Tile<BM_, BN_>::MI][Tile<BM_, BN_>::NI])
+{
+    using T = Tile<BM_, BN_>;
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+        const int k = kk + (lane >> 4);
+        const int i = lane & 15;
+        double a[T::MI], b[T::NI];
+#pragma unroll
+        for (int x = 0; x < T::MI; x++) a[x] = As[k * T::PA + wm + 16 * x + i];
+#pragma unroll
+        for (int y = 0; y < T::NI; y++) b[y] = Bs[k * T::PB + wn + 16 * y + i];
+#pragma unroll
+        for (int x = 0; x < T::MI; x++)
+#pragma unroll
+            for (int y = 0; y < T::NI; y++)
+                if ((need >> (x * T::NI + y)) & 1u)
+                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
+    }
+}
+
+// Bit x*NI + y set when the wave's 16 x 16 block (rows f0 + wm + 16x, columns
+// n0 + wn + 16y) holds an entry inside some column's window [item_lo, item_hi)
+// (rows and windows are indices into the upper block). Wave-uniform.
+__device__ inline unsigned band_blocks(int MI, int NI, int f0, int n0, int wm, int wn, int lane,
+                                       int Hi, int ncols, int ncb, const int *__restrict__ lo,
+                                       const int *__restrict__ hi)
+{
+    unsigned need = 0;
+    for (int x = 0; x < MI; x++)
+        for (int y = 0; y < NI; y++) {
+            const int r0 = f0 + wm + 16 * x;
+            const int col = n0 + wn + 16 * y + (lane & 15);
+            bool hit = false;
+            if (col < ncols && r0 < Hi) {
+                const int it = col / ncb;
+                hit = lo[it] < r0 + 16 && hi[it] > r0;
+            }
+            if (__ballot(hit) != 0ull) need |= 1u << (x * NI + y);
+        }
+    return __builtin_amdgcn_readfirstlane(need);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup w runs on
@@ -168,12 +249,14 @@ __global__ __launch_bounds__(256) void k_build_g_map(const GMapJob *__restrict__
 // K5 over jobs: H_co[m, (u, j)] = sum_k mu-block[m, k] * G[k, (u, j)] on the
 // matrix cores. K runs segment by segment (a tile never straddles two mu
 // blocks).
-template <int BM_, int BN_>
-__global__ __launch_bounds__(256) void k_halfls_jobs(int N, const HalfJob *__restrict__ jobs,
-                                                     const int *__restrict__ tbegin, int njobs,
-                                                     int ntiles, const double *__restrict__ mu)
+template <int BM_, int BN_, int NW_ = 4>
+__global__ __launch_bounds__(64 * NW_) __attribute__((amdgpu_waves_per_eu(NW_ == 8 ? 4 : 1)))
+void k_halfls_jobs(int N, const HalfJob *__restrict__ jobs,
+                                                          const int *__restrict__ tbegin,
+                                                          int njobs, int ntiles,
+                                                          const double *__restrict__ mu)
 {
-    using T = Tile<BM_, BN_>;
+    using T = TileW<BM_, BN_, NW_>;
     __shared__ double As[BK * T::PA];
     __shared__ double Bs[BK * T::PB];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -183,9 +266,9 @@ __global__ __launch_bounds__(256) void k_halfls_jobs(int N, const HalfJob *__res
     const int local = tile - J.tile_begin;
     const int m0 = (local % J.mt) * BM_, n0 = (local / J.mt) * BN_;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave & 1) * (BM_ / 2), wn = (wave >> 1) * (BN_ / 2);
+    const int wm = (wave & 1) * (BM_ / 2), wn = (wave >> 1) * (BN_ / T::WC);
     const int ncolsTot = J.nunits * cd.ncols;
-    // loads: A row am, K rows ak0 + q*(256/BM); B column bn, K rows bk0 + q*(256/BN)
+    // loads: A row am, K rows ak0 + q*(NT/BM); B column bn, K rows bk0 + q*(NT/BN)
     const int am = tid % BM_, ak0 = tid / BM_;
     const int bn = tid % BN_, bk0 = tid / BN_;
     const bool mok = m0 + am < cd.n;
@@ -210,21 +293,21 @@ __global__ __launch_bounds__(256) void k_halfls_jobs(int N, const HalfJob *__res
         const double *gp = gcol + (size_t)(cd.seg_row0[se] + sk0) * cd.ncols;
 #pragma unroll
         for (int q = 0; q < T::LA; q++) {
-            const int kl = ak0 + q * (256 / BM_);
+            const int kl = ak0 + q * (T::NT / BM_);
             ra[q] = (mok && kl < len) ? mp[(size_t)kl * N] : 0.0;
         }
 #pragma unroll
         for (int q = 0; q < T::LB; q++) {
-            const int kl = bk0 + q * (256 / BN_);
+            const int kl = bk0 + q * (T::NT / BN_);
             rb[q] = (nok && kl < len) ? gp[(size_t)kl * cd.ncols] : 0.0;
         }
     };
     if (cd.nseg > 0) fetch(0, 0);
     while (e < cd.nseg) {
 #pragma unroll
-        for (int q = 0; q < T::LA; q++) As[(ak0 + q * (256 / BM_)) * T::PA + am] = ra[q];
+        for (int q = 0; q < T::LA; q++) As[(ak0 + q * (T::NT / BM_)) * T::PA + am] = ra[q];
 #pragma unroll
-        for (int q = 0; q < T::LB; q++) Bs[(bk0 + q * (256 / BN_)) * T::PB + bn] = rb[q];
+        for (int q = 0; q < T::LB; q++) Bs[(bk0 + q * (T::NT / BN_)) * T::PB + bn] = rb[q];
         __syncthreads();
         kk0 += BK;
         if (kk0 >= cd.seg_n[e]) {
@@ -232,7 +315,7 @@ __global__ __launch_bounds__(256) void k_halfls_jobs(int N, const HalfJob *__res
             kk0 = 0;
         }
         if (e < cd.nseg) fetch(e, kk0);
-        mfma_step<BM_, BN_>(As, Bs, wm, wn, lane, acc);
+        mfma_step_w<T::MI, T::NI, T::PA, T::PB>(As, Bs, wm, wn, lane, acc);
         __syncthreads();
     }
 #pragma unroll
@@ -285,6 +368,9 @@ __global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict_
     __syncthreads();
     if (!s_need) return;
     const int wm = (wave & 1) * (BM_ / 2), wn = (wave >> 1) * (BN_ / 2);
+    // the wave's blocks outside every window are not multiplied (band edges)
+    const unsigned bneed = band_blocks(T::MI, T::NI, f0, n0, wm, wn, lane, a.Hi, ncols, a.ncb,
+                                       a.item_lo, a.item_hi);
     d4 acc[T::MI][T::NI];
 #pragma unroll
     for (int x = 0; x < T::MI; x++)
@@ -332,7 +418,7 @@ __global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict_
         // workgroup's loads and LDS stores fill the gaps (k_corr_mfma 1.47 -> 1.38 ms
         // at C2; the half-ls kernels gain nothing or lose)
         __builtin_amdgcn_s_setprio(1);
-        mfma_step<BM_, BN_>(As, Bs, wm, wn, lane, acc);
+        mfma_step_mask<BM_, BN_>(As, Bs, wm, wn, lane, bneed, acc);
         __builtin_amdgcn_s_setprio(0);
         __syncthreads();
     }
@@ -433,8 +519,8 @@ __global__ __launch_bounds__(256) void k_intensity_jobs(const IntensArgs *__rest
 
 // ---------------------------------------------------------------------------
 
-int tile_rows(int cfg) { return (cfg & 2) ? 128 : 64; }
-int tile_cols(int cfg) { return (cfg & 1) ? 128 : 64; }
+int tile_rows(int cfg) { return cfg == kTile128x128w8 ? 128 : (cfg & 2) ? 128 : 64; }
+int tile_cols(int cfg) { return cfg == kTile128x128w8 ? 128 : (cfg & 1) ? 128 : 64; }
 
 void launch_build_g_units(hipStream_t st, int N, const GBuildJob *d_jobs, const int *d_begin,
                           int njobs, int ntiles)
@@ -511,6 +597,11 @@ void launch_halfls_jobs(hipStream_t st, int cfg, int N, const HalfJob *d_jobs,
 {
     if (njobs <= 0 || ttiles <= 0) return;
     const dim3 g(xcd_grid(ttiles)), b(256);
+    if (cfg == kTile128x128w8) {
+        hipLaunchKernelGGL((k_halfls_jobs<128, 128, 8>), g, dim3(512), 0, st, N, d_jobs, d_tbegin,
+                           njobs, ttiles, mu);
+        return;
+    }
     switch (cfg & 3) {
     case 0: hipLaunchKernelGGL((k_halfls_jobs<64, 64>), g, b, 0, st, N, d_jobs, d_tbegin, njobs, ttiles, mu); break;
     case 1: hipLaunchKernelGGL((k_halfls_jobs<64, 128>), g, b, 0, st, N, d_jobs, d_tbegin, njobs, ttiles, mu); break;

@@ -438,6 +438,26 @@ static void fill_corr_args(gi_ctx *c, const Plan &P, const Chunk &ch, unsigned c
 
 static LineSoA shift(LineSoA s, long long o);
 
+// Multiply-add flops the banded correlation kernels execute for a chunk: the
+// 16 x 16 output blocks (rows from Lo, columns item*ncb + pair) that hold an
+// entry inside some column's window, each over K = nsize_F (the kernels skip
+// the others; work[8] against work[16] is the band waste).
+static double band_exec_flops(const Plan &P, const Chunk &ch, int ncb, int nsize)
+{
+    const int *lo = (const int *)(P.meta.data() + ch.off_lo);
+    const int *hi = (const int *)(P.meta.data() + ch.off_hi);
+    const int ncols = ch.nitems * ncb;
+    long long blocks = 0;
+    for (int r0 = ch.Lo; r0 < ch.Hi; r0 += 16)
+        for (int c0 = 0; c0 < ncols; c0 += 16) {
+            bool hit = false;
+            for (int c = c0; c < std::min(c0 + 16, ncols) && !hit; c++)
+                hit = lo[c / ncb] < r0 + 16 && hi[c / ncb] > r0;
+            blocks += hit;
+        }
+    return 2.0 * 256.0 * (double)blocks * nsize;
+}
+
 // Consecutive chunks grouped into rounds of at most kRoundBytes of device
 // scratch (GI_BATCH_BYTES overrides). With per-round output the first
 // c->pipe_rounds - 1 rounds each take c->pipe_share percent of the work still
@@ -535,7 +555,7 @@ static int corr_kparts(const gi_ctx *c, const Plan &P, const Chunk &ch)
 {
     if (P.mode == GI_MODE_FP64) return 1;
     const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
-    return corr_parts(c, P.mode == GI_MODE_SPLIT ? BF.Kp : BF.nsize);
+    return P.mode == GI_MODE_SPLIT ? corr_parts(c, BF.Kp, true) : corr_parts(c, BF.nsize, false);
 }
 
 // Arguments of one chunk's intensity step from its stored amplitudes.
@@ -707,7 +727,22 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
     unsigned char *d_mm = (unsigned char *)slot_scratch(
         c, S, "mmargs", (size_t)mchunks * (sizeof(CorrMMArgs) + 4), &rc);
     if (rc) return rc;
-    const int TM = tile_rows(c->tile_halfls), TN = tile_cols(c->tile_halfls);
+    // tile shape of the half-ls products of round [c0, c1): the setting, or
+    // with kTileAuto 128 x 128 on eight wavefronts while the round has
+    // kAutoTileWaves waves of them (two workgroups per CU), else 64 x 64
+    // (measured at C2: 128 x 128 / 8 waves 3.09 ms against 3.35 ms for four
+    // waves on a whole pass; 64 x 64 wins rounds under about two waves)
+    auto round_halfls_cfg = [&](size_t c0, size_t c1) {
+        if (c->tile_halfls != kTileAuto) return c->tile_halfls;
+        long long t = 0;
+        for (size_t i = c0; i < c1; i++) {
+            const Chunk &ch = chunks[i];
+            if (ch.nunits <= 0) continue;
+            for (const OutClass &oc : group_of(c, ch.jindI, ch.gI, ch.jindF).oc)
+                t += (long long)((oc.n + 127) / 128) * ((ch.nunits * (int)oc.cols.size() + 127) / 128);
+        }
+        return t >= (long long)kAutoTileWaves * 2 * c->ncu ? kTile128x128w8 : 0;
+    };
 
     GBuildJob *d_gjobs = (GBuildJob *)slot_scratch(
         c, S, "gjobs", (size_t)mchunks * (sizeof(GBuildJob) + 4), &rc);
@@ -735,6 +770,8 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
     for (auto &rg : rounds) {
         RoundCapture cap;
         if ((rc = cap.begin(c, P, S.stream)) != GI_OK) return rc;
+        const int hcfg = round_halfls_cfg(rg.first, rg.second);
+        const int TM = tile_rows(hcfg), TN = tile_cols(hcfg);
         std::vector<HalfJob> jobs;
         std::vector<GBuildJob> gjobs;
         std::vector<GMapJob> mjobs_v;
@@ -850,7 +887,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             const int *d_tb = (const int *)(d_jobs + jobs_bytes);
             TL_BEGIN("half_ls_mfma");
             P.work[14] += tt;
-            launch_halfls_jobs(S.stream, c->tile_halfls, N, (const HalfJob *)d_jobs, d_tb, njobs,
+            launch_halfls_jobs(S.stream, hcfg, N, (const HalfJob *)d_jobs, d_tb, njobs,
                                tt, c->d_mu_perm);
             TL_END();
         }
@@ -900,7 +937,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             P.work[16] += (double)ch.ncand * 2.0 * BF.nsize * __builtin_popcount(ch.cb_mask);
             P.work[6] += blk_bytes;
             P.work[7] += ch.nunits * 24.0 * (double)N * N;
-            P.work[8] += 2.0 * (double)(ch.Hi - ch.Lo) * ch.nitems * ndegI * ndegF * BF.nsize;
+            P.work[8] += band_exec_flops(P, ch, ndegI * ndegF, BF.nsize);
         }
         if (!fjobs.empty()) {
             if ((rc = send_jobs(c, S, d_fold, fjobs, fbegin, fcap)) != GI_OK) return rc;
@@ -1209,7 +1246,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             P.work[16] += (double)ch.ncand * 2.0 * BF.nsize * __builtin_popcount(ch.cb_mask);
             P.work[6] += blk_bytes;
             P.work[7] += ch.nunits * 24.0 * (double)N * N;
-            P.work[8] += 2.0 * (double)rows * ch.nitems * ndegI * ndegF * BF.nsize;
+            P.work[8] += band_exec_flops(P, ch, ndegI * ndegF, BF.nsize);
         }
         if (!fjobs.empty() && fused) {
             if ((rc = send_jobs(c, S, d_fsj, fsjobs, sbegin, fscap)) != GI_OK) return rc;
@@ -1675,8 +1712,9 @@ int autotune(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out, int
         *knob = pick;
         return GI_OK;
     };
-    static const int tiles[4] = {0, 1, 2, 3};
-    static const char *const tname[4] = {"64x64", "64x128", "128x64", "128x128"};
+    static const int tiles[6] = {0, 1, 2, 3, kTileAuto, kTile128x128w8};
+    static const char *const tname[6] = {"64x64", "64x128", "128x64", "128x128", "auto",
+                                         "128x128/8-waves"};
     static const int rounds[6] = {1, 2, 3, 4, 6, 8};
     static const int shares[4] = {50, 60, 70, 80};
     static const int stiles[3] = {0, 1, 2};
@@ -1687,7 +1725,7 @@ int autotune(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out, int
         return rc;
     }
     if (mode == GI_MODE_FP64_BATCHED) {
-        if ((rc = scan("halfls_tile", "half_ls_mfma", &c->tile_halfls, tiles, 4, tname)) ||
+        if ((rc = scan("halfls_tile", "half_ls_mfma", &c->tile_halfls, tiles, 6, tname)) ||
             (rc = scan("corr_tile", "corr_mfma", &c->tile_corr, tiles, 4, tname))) {
             c->tile_halfls = saved.halfls_cfg; c->tile_corr = saved.corr_cfg;
             return rc;
@@ -2127,7 +2165,7 @@ int gi_set_tuning(gi_ctx *c, const gi_tuning *t)
 {
     if (!c || !t) return GI_EINVAL;
     if (c->server) return fail(c, GI_ESTATE, "gi_set_tuning while the request server runs");
-    if (t->halfls_cfg < 0 || t->halfls_cfg > 3 || t->corr_cfg < 0 || t->corr_cfg > 3 ||
+    if (t->halfls_cfg < 0 || t->halfls_cfg > kTileHalflsMax || t->corr_cfg < 0 || t->corr_cfg > 3 ||
         t->pipe_rounds < 1 || t->pipe_rounds > kMaxPipeRounds || t->split_cfg < 0 ||
         t->split_cfg > 2 || t->pipe_share < 10 || t->pipe_share > 90)
         return fail(c, GI_EINVAL, "gi_set_tuning: value out of range");
@@ -2156,7 +2194,9 @@ int gi_set_path_limits(gi_ctx *c, int kpart_len, int rec_lds, int split2_steps,
         !pick(split2_steps, kSplitMaxSteps, 1, &d) || !pick(split2_segs, kSplitMaxSegs, 1, &e) ||
         !pick(fold_split_rows, kFoldSplitMax, 1, &f))
         return fail(c, GI_EINVAL, "gi_set_path_limits: a limit outside [minimum, default]");
-    c->lim_kpart = a;
+    // kpart_len 0: the defaults of each mode; else both modes use it
+    c->lim_kpart = kpart_len ? a : 1024;
+    c->lim_kpart_split = kpart_len ? a : 2048;
     c->lim_rec_lds = b;
     c->lim_split2_steps = d;
     c->lim_split2_segs = e;

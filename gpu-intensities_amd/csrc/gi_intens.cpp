@@ -3,7 +3,10 @@
 // stdin, the TROVE files from the working directory (or --dir), the line list
 // goes to stdout in the reference's format (cuda_host.cu:1165, 1444-1473).
 //
-//   gi_intens [--mode fp64|batched|split] [--device N] [--dir D] < deck
+//   gi_intens [--mode fp64|batched|split] [--device N] [--dir D] [--thresh-line] < deck
+//
+// --thresh-line applies the deck's THRESH_LINE as a linestrength cut; by
+// default, as in the reference (cuda_host.cu:1408), it is parsed and ignored.
 //
 // Exit status 0 on success; errors go to stderr with the library's message.
 #include <cstdio>
@@ -25,7 +28,7 @@ static int die(const char *what, const char *msg)
 
 int main(int argc, char **argv)
 {
-    int mode = GI_MODE_FP64, device = 0;
+    int mode = GI_MODE_FP64, device = 0, thresh_line = 0;
     std::string dir;
     for (int i = 1; i < argc; i++) {
         const std::string a = argv[i];
@@ -39,9 +42,11 @@ int main(int argc, char **argv)
             device = std::atoi(argv[++i]);
         } else if (a == "--dir" && i + 1 < argc) {
             dir = argv[++i];
+        } else if (a == "--thresh-line") {
+            thresh_line = 1;
         } else {
             std::fprintf(stderr, "usage: gi_intens [--mode fp64|batched|split] [--device N] "
-                                 "[--dir D] < deck\n");
+                                 "[--dir D] [--thresh-line] < deck\n");
             return 2;
         }
     }
@@ -58,6 +63,7 @@ int main(int argc, char **argv)
         gi_trove_close(t);
         return die("device", "no HIP device");
     }
+    gi_trove_apply_thresh_line(t, thresh_line);
     int rc = gi_trove_load(t, ctx);
     long long cap = 0;
     if (rc == GI_OK) rc = gi_count_candidates(ctx, 0, nullptr, &cap);

@@ -19,7 +19,14 @@ constexpr int kMaxJ = 128;
 // Largest pipeline round count (gi_set_tuning).
 constexpr int kMaxPipeRounds = 64;
 // Most split-K parts of a correlation product.
-constexpr int kMaxKParts = 4;
+constexpr int kMaxKParts = 16;
+// tile_halfls setting: per round, 128 x 128 tiles (eight wavefronts) when
+// they fill the GPU for at least kAutoTileWaves waves, else 64 x 64 (the same
+// bits either way)
+constexpr int kTileAuto = 4;
+// largest half-ls tile setting (5: 128 x 128 on 8 wavefronts, gi_kernels.h)
+constexpr int kTileHalflsMax = 5;
+constexpr int kAutoTileWaves = 3;
 // Counters of gi_work_stats.
 constexpr int kWorkStats = 17;
 }  // namespace gi
@@ -236,7 +243,8 @@ struct gi_ctx {
 
     // tile configurations of k_halfls_jobs and k_corr_mfma (see tile_rows), and
     // the rounds of a pipelined gi_compute (gi_set_tuning, gi_autotune)
-    int tile_halfls = 3, tile_corr = 0;
+    int tile_halfls = gi::kTileAuto, tile_corr = 0;
+    int ncu = 256;                   // compute units of the device (workgroup slots)
     int pipe_rounds = 4;
     int pipe_share = 50;             // percent of the remaining work per round but the last
     int tile_split = 2;              // split half-ls: 0 = 64 rows, 1 = 128, 2 = 2 steps in flight
@@ -244,8 +252,9 @@ struct gi_ctx {
     // defaults are the kernels' own limits): split-K part length of the
     // correlation, longest record staged in LDS by the G builds, step and
     // segment limits of k_halfls_split2, longest column of the fused fold-split
-    int lim_kpart = 2048, lim_rec_lds = gi::kRecLds, lim_split2_steps = gi::kSplitMaxSteps,
-        lim_split2_segs = gi::kSplitMaxSegs, lim_fold_split = gi::kFoldSplitMax;
+    int lim_kpart = 1024, lim_kpart_split = 2048, lim_rec_lds = gi::kRecLds,
+        lim_split2_steps = gi::kSplitMaxSteps, lim_split2_segs = gi::kSplitMaxSegs,
+        lim_fold_split = gi::kFoldSplitMax;
 
     // split mode. Requested format (0 = automatic): radix 128 or 254, digits of
     // the correlation (eigenvector blocks, folded vectors) and of the half-ls
@@ -290,10 +299,11 @@ inline const GroupPlan &group_of(const gi_ctx *c, int jindI, int gI, int jindF)
 }
 
 // Split-K parts of a correlation product of length K (a function of the
-// upper block only, so a candidate's sums never depend on its batch).
-inline int corr_parts(const gi_ctx *c, int K)
+// upper block and the mode only, so a candidate's sums never depend on its
+// batch): parts of about 1024 (FP64) or 2048 (split digits) K, at most 16.
+inline int corr_parts(const gi_ctx *c, int K, bool split)
 {
-    return std::min(kMaxKParts, std::max(1, K / c->lim_kpart));
+    return std::min(kMaxKParts, std::max(1, K / (split ? c->lim_kpart_split : c->lim_kpart)));
 }
 
 // Register-tile width (columns) of the half-ls kernels for `cols` columns.

@@ -218,6 +218,8 @@ void launch_corr_split(hipStream_t st, int S, double R, const CorrSplitArgs *d_a
 
 // Tile configurations of the two matrix-core kernels: cfg bit 1 selects 128
 // rows (else 64), bit 0 selects 128 columns (else 64).
+// half-ls tile setting 5: 128 x 128 tiles on 8 wavefronts (each 64 x 32)
+constexpr int kTile128x128w8 = 5;
 int tile_rows(int cfg);
 int tile_cols(int cfg);
 void launch_halfls_jobs(hipStream_t st, int cfg, int N, const HalfJob *d_jobs,

@@ -502,6 +502,52 @@ __device__ inline void oz_step(const int8_t *__restrict__ As, const int8_t *__re
     }
 }
 
+// oz_step for the banded correlation: only the wave's 16 x 16 blocks whose bit
+// (x*2 + y) is set in `need` (wave-uniform) are multiplied.
+template <int S>
+__device__ inline void oz_step_mask(const int8_t *__restrict__ As,
+                                    const int8_t *__restrict__ Bs, int wm, int wn, int lane,
+                                    unsigned need, v4i (&acc)[2][2][S])
+{
+    // nothing of the caller's next-step fetch moves past this point: its scalar
+    // loads are then complete, and the LDS waits below can count exactly
+    __builtin_amdgcn_sched_barrier(0);
+    const int i16 = lane & 15, k16 = (lane >> 4) * 16;
+    auto ldA = [&](int i, int x) {
+        return *(const v4i *)(As + (i * 64 + wm + 16 * x + i16) * kRowB + k16);
+    };
+    v4i a0 = ldA(0, 0), a1 = ldA(0, 1);
+    v4i b[S][2];
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+        b[j][0] = *(const v4i *)(Bs + (j * 64 + wn + i16) * kRowB + k16);
+        b[j][1] = *(const v4i *)(Bs + (j * 64 + wn + 16 + i16) * kRowB + k16);
+    }
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+        v4i n0 = a0, n1 = a1;
+        if (i + 1 < S) {
+            n0 = ldA(i + 1, 0);
+            n1 = ldA(i + 1, 1);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < S - i; j++) {
+            if (need & 1u)
+                acc[0][0][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b[j][0], acc[0][0][i + j], 0, 0, 0);
+            if (need & 2u)
+                acc[0][1][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b[j][1], acc[0][1][i + j], 0, 0, 0);
+            if (need & 4u)
+                acc[1][0][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b[j][0], acc[1][0][i + j], 0, 0, 0);
+            if (need & 8u)
+                acc[1][1][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b[j][1], acc[1][1][i + j], 0, 0, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        a0 = n0;
+        a1 = n1;
+    }
+}
+
 // sum over levels l of L_l * R^-(l+2), smallest first (rinv = 1/R; exact
 // powers of two for R = 128)
 template <int S>
@@ -881,6 +927,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     __syncthreads();
     if (!s_need) return;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    // the wave's 16 x 16 blocks outside every window are not multiplied
+    unsigned bneed = 0;
+    for (int x = 0; x < 2; x++)
+        for (int y = 0; y < 2; y++) {
+            const int r0 = f0 + wm + 16 * x;
+            const int col = n0 + wn + 16 * y + (lane & 15);
+            bool hit = false;
+            if (col < ncols && r0 < a.Hi) {
+                const int it = col / a.ncb;
+                hit = a.item_lo[it] < r0 + 16 && a.item_hi[it] > r0;
+            }
+            if (__ballot(hit) != 0ull) bneed |= 1u << (x * 2 + y);
+        }
+    bneed = __builtin_amdgcn_readfirstlane(bneed);
     const int lr = tid >> 2, lp = (tid & 3) * 16;
     const int fr = f0 + lr;
     const bool fok = fr < a.Hi;
@@ -924,7 +984,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         // raised issue priority while this wavefront multiplies (as in k_corr_mfma;
         // 0.83 -> 0.80 ms at C2)
         __builtin_amdgcn_s_setprio(1);
-        oz_step<S>(As, Bs, wm, wn, lane, acc);
+        oz_step_mask<S>(As, Bs, wm, wn, lane, bneed, acc);
         __builtin_amdgcn_s_setprio(0);
         __syncthreads();
         if (k0 + 64 >= ke) break;
@@ -932,7 +992,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         __syncthreads();
         if (k0 + 192 < ke) GI_FETCH(ra1, rb1, k0 + 192);
         __builtin_amdgcn_s_setprio(1);
-        oz_step<S>(As, Bs, wm, wn, lane, acc);
+        oz_step_mask<S>(As, Bs, wm, wn, lane, bneed, acc);
         __builtin_amdgcn_s_setprio(0);
         __syncthreads();
     }
@@ -957,7 +1017,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         __syncthreads();
         if (k0 + 64 < ke) fetch(k0 + 64);
         __builtin_amdgcn_s_setprio(1);
-        oz_step<S>(As, Bs, wm, wn, lane, acc);
+        oz_step_mask<S>(As, Bs, wm, wn, lane, bneed, acc);
         __builtin_amdgcn_s_setprio(0);
         __syncthreads();
     }

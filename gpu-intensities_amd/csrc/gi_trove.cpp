@@ -511,6 +511,10 @@ struct gi_trove {
     int nrec[2][8] = {};
     // views
     std::vector<const double *> rep_ptr[2];
+    // gi_trove_load applies the deck's THRESH_LINE as a linestrength cut only
+    // when asked (gi_trove_apply_thresh_line): the reference parses it
+    // (fields.cpp:204-206) but its cut is commented out (cuda_host.cu:1408)
+    bool apply_thresh_line = false;
 };
 
 namespace {
@@ -766,7 +770,7 @@ int gi_trove_params(const gi_trove *t, gi_params *p)
     p->reduced = d.reduced ? 1 : 0;
     p->quanta_lower = d.quanta_lower.data();
     p->quanta_upper = d.quanta_upper.data();
-    p->thresh_linestrength = d.thresh_line;  // applied (the reference parses but ignores it)
+    p->thresh_linestrength = d.thresh_line;  // as read; gi_trove_load applies it only on request
     p->thresh_intensity = 0.0;
     return GI_OK;
 }
@@ -834,6 +838,7 @@ int gi_trove_load(gi_trove *t, gi_ctx *ctx)
     if (!t->err.empty()) return GI_ESTATE;
     gi_params p;
     gi_trove_params(t, &p);
+    if (!t->apply_thresh_line) p.thresh_linestrength = 0.0;  // the reference prints every line
     int rc;
     if ((rc = gi_set_params(ctx, &p)) != GI_OK) return rc;
     for (int j = 0; j < 2; j++) {
@@ -854,6 +859,13 @@ int gi_trove_load(gi_trove *t, gi_ctx *ctx)
                 return rc;
         }
     return gi_prepare(ctx);
+}
+
+int gi_trove_apply_thresh_line(gi_trove *t, int on)
+{
+    if (!t) return GI_EINVAL;
+    t->apply_thresh_line = on != 0;
+    return GI_OK;
 }
 
 int gi_trove_header(const gi_trove *t, char *buf, long long cap, long long *len)

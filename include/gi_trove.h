@@ -32,8 +32,14 @@ const char *gi_trove_error(const gi_trove *t);
 void gi_trove_close(gi_trove *t);
 
 /* gi_set_params, gi_upload_bset x2, gi_upload_dipole, gi_set_levels,
- * gi_upload_vectors for every read block, gi_prepare. */
+ * gi_upload_vectors for every read block, gi_prepare. The deck's THRESH_LINE
+ * is not applied, as in the reference (parsed at fields.cpp:204-206, its cut
+ * commented out at cuda_host.cu:1408), unless gi_trove_apply_thresh_line
+ * asked for it: then lines with S below it are not emitted. */
 int gi_trove_load(gi_trove *t, gi_ctx *ctx);
+/* on != 0: gi_trove_load applies THRESH_LINE as a linestrength cut (no
+ * reference counterpart; the reference never applies it). */
+int gi_trove_apply_thresh_line(gi_trove *t, int on);
 
 /* Views of what was read (valid until gi_trove_close). */
 int gi_trove_params(const gi_trove *t, gi_params *p);

@@ -271,9 +271,13 @@ int gi_prepare_split(gi_ctx *ctx);
 
 /* Tile shapes of the two matrix-core kernels of GI_MODE_FP64_BATCHED (no
  * reference counterpart: the reference has no batched form). cfg 0..3 =
- * rows x columns 64x64, 64x128, 128x64, 128x128; -1 leaves a kernel as it
- * is. Results are bitwise the same for every shape (the K order of each
- * product does not depend on it). */
+ * rows x columns 64x64, 64x128, 128x64, 128x128; for the half linestrength
+ * also 4 (the default): per pipeline round 128x128 on eight wavefronts
+ * while the round has at least three waves of such tiles for the device,
+ * else 64x64 (the last, small rounds), and 5: 128x128 on eight wavefronts instead of four (half
+ * the accumulators per wavefront, twice the wavefronts per SIMD); -1 leaves
+ * a kernel as it is. Results are bitwise the same
+ * for every shape (the K order of each product does not depend on it). */
 int gi_set_tiles(gi_ctx *ctx, int halfls_cfg, int corr_cfg);
 int gi_get_tiles(gi_ctx *ctx, int *halfls_cfg, int *corr_cfg);
 
@@ -299,8 +303,9 @@ int gi_get_tuning(gi_ctx *ctx, gi_tuning *t);
 
 /* Routing thresholds of the batched modes (no reference counterpart), for
  * tests that reach at small sizes the kernels real sizes select: kpart_len,
- * the correlation K per split-K part (parts = min(4, K / kpart_len); default
- * 2048, at least 64); rec_lds, the longest eigenvector record the G builds
+ * the correlation K per split-K part in both matrix-core modes
+ * (parts = min(16, K / kpart_len); default 1024 for FP64 and 2048 for the
+ * split digits, at most 2048, at least 64); rec_lds, the longest eigenvector record the G builds
  * stage in LDS (longer ones use the global-gather builds; default 16384);
  * split2_steps / split2_segs, the largest class (64-byte K steps, segments)
  * the two-step split half-ls takes (larger ones run the one-step kernel;

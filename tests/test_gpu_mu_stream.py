@@ -31,7 +31,7 @@ def _same(a, b, what):
 @pytest.fixture
 def engine_resident(engine):
     yield engine
-    engine.set_tuning(3, 0, 4, 2, 50)
+    engine.set_tuning(4, 0, 4, 2, 50)
 
 
 @pytest.mark.parametrize("sym,kw", [

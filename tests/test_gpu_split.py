@@ -254,5 +254,5 @@ def test_split_tile_settings_same_bits(engine, sym, kw):
             for k in ref:
                 assert np.array_equal(ref[k], got[k]), (cfg, k)
     finally:
-        engine.set_tuning(3, 0, 4, 2, 50)
+        engine.set_tuning(4, 0, 4, 2, 50)
     assert split_error(ref, oracle_py.OracleJob(job).run())["max"] <= TOL_S

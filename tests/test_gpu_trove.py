@@ -39,8 +39,8 @@ def parse(text, labels):
     return out
 
 
-def run_driver(d, deck, mode):
-    p = subprocess.run([EXE, "--dir", d, "--mode", mode], input=deck.encode(),
+def run_driver(d, deck, mode, *extra):
+    p = subprocess.run([EXE, "--dir", d, "--mode", mode, *extra], input=deck.encode(),
                        capture_output=True, timeout=300)
     assert p.returncode == 0, p.stderr.decode()
     return p.stdout.decode()
@@ -98,3 +98,28 @@ def test_trove_load_equals_direct_load(tmp_path, engine):
         assert np.array_equal(a[k], b[k]), k
     text = t.format(b)
     assert text.count("\n") == len(b["S"])
+
+
+@pytest.mark.parametrize("mode", ["fp64", "batched"])
+def test_thresh_line_parsed_not_applied(tmp_path, mode):
+    """THRESH_LINE in the deck: the reference parses it (fields.cpp:204-206)
+    and never applies it (cuda_host.cu:1408 is commented out), so by default
+    the driver prints every line the oracle emits without a cut; with
+    --thresh-line exactly those with S at or above it."""
+    import dataclasses
+    job = synth.make_job("C2V", nvib=40, jvals=(0, 1), seed=13)
+    want_all = oracle_lines(job)
+    Ss = np.sort([v[0] for v in want_all.values()])
+    h = len(Ss) // 2
+    th = float(0.5 * (Ss[h - 1] + Ss[h]))  # half of the lines below it, none near it
+    assert Ss[h] - Ss[h - 1] > 1e-8 * Ss[h]
+    jt = dataclasses.replace(job, thresh_linestrength=th)
+    d = str(tmp_path)
+    deck = trove_io.write_trove(jt, d)
+    assert "THRESH_LINE" in deck
+    got = parse(run_driver(d, deck, mode), job.sym["labels"])
+    assert len(got) == len(want_all)
+    cut = parse(run_driver(d, deck, mode, "--thresh-line"), job.sym["labels"])
+    keep = {k for k, v in want_all.items() if v[0] >= th}
+    assert {(F, I) for F, I, *_ in cut} == keep
+    assert 0 < len(cut) < len(got)

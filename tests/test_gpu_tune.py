@@ -13,7 +13,7 @@ from test_gpu_parity import compare
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = dict(halfls_cfg=3, corr_cfg=0, pipe_rounds=4, split_cfg=2, pipe_share=50)
+DEFAULT = dict(halfls_cfg=4, corr_cfg=0, pipe_rounds=4, split_cfg=2, pipe_share=50)
 
 
 @pytest.fixture
@@ -38,7 +38,7 @@ def test_autotune_picks_a_setting_and_keeps_the_bits(tuned_engine, mode):
     lines = table.strip().splitlines()
     assert lines[0].startswith("# gi_autotune") and lines[-1].startswith("best ")
     kinds = [ln.split()[0] for ln in lines[1:-1]]
-    want = (["halfls_tile"] * 4 + ["corr_tile"] * 4 if mode == binding.GI_MODE_FP64_BATCHED
+    want = (["halfls_tile"] * 6 + ["corr_tile"] * 4 if mode == binding.GI_MODE_FP64_BATCHED
             else ["split_halfls_tile"] * 3) + ["pipe_rounds"] * 6 + ["pipe_share"] * 4
     assert kinds == want
     assert lines[-1] == ("best halfls_tile=%d corr_tile=%d pipe_rounds=%d split_tile=%d "
@@ -97,7 +97,7 @@ def test_set_tuning_rules(tuned_engine):
     eng = tuned_engine
     job = synth.make_job("C2V", nvib=16, jvals=(0, 1), seed=54)
     eng.load(job)
-    for bad in ((4, 0, 2, 0), (0, -1, 2, 0), (0, 0, 0, 0), (0, 0, 65, 0), (0, 0, 2, 3),
+    for bad in ((6, 0, 2, 0), (0, -1, 2, 0), (0, 0, 0, 0), (0, 0, 65, 0), (0, 0, 2, 3),
                 (0, 0, 2, 0, 9), (0, 0, 2, 0, 91)):
         with pytest.raises(binding.GIError):
             eng.set_tuning(*bad)
