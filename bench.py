@@ -446,6 +446,38 @@ def mu_stream_section(eng, binding, job, dev, bufs, parts, steps):
         eng.load(job, device_vectors=dev)
 
 
+def vec_stream_section(binding, job, bufs, ref_lines, steps):
+    """GI_MODE_FP64_BATCHED with every eigenvector block kept in pinned host
+    memory (gi_set_vector_budget): each chunk's records are copied to the
+    device while the previous chunk computes. The lines must be the bits of
+    resident blocks; reports the pass time and the copied bytes."""
+    import torch
+    mode = binding.GI_MODE_FP64_BATCHED
+    eng = binding.Engine(0)
+    try:
+        t = time.perf_counter()
+        eng.load(job, vector_budget=1)
+        t_load = time.perf_counter() - t
+        got = eng.compute(bufs=bufs, mode=mode)
+        same = all(np.array_equal(ref_lines[k], got[k]) for k in ref_lines)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(steps):
+            t = time.perf_counter()
+            eng.compute(bufs=bufs, mode=mode)
+            ts.append(time.perf_counter() - t)
+        ms = 1e3 * sum(ts) / len(ts)
+        out = dict(streamed_blocks=eng.streamed_blocks(), ms_per_step=ms,
+                   value=len(got["S"]) / (ms / 1e3), bitwise_equal_resident=bool(same),
+                   host_load_s=round(t_load, 2),
+                   kernels={k: {"total_ms": round(v["total_ms"], 4), "launches": v["launches"]}
+                            for k, v in eng.kernel_stats().items()})
+        log(f"blocks in host memory: {ms:.2f} ms/step, bitwise equal {same}")
+        return out
+    finally:
+        eng.close()
+
+
 def mode_rooflines(modes, nvib, pmc=False):
     """Roofline of the two product kernels of each mode: HBM GB/s for the plain
     FP64 (GEMV-form) kernels, matrix-core utilisation for the batched ones.
@@ -622,6 +654,10 @@ def compact_line(out):
     if par:
         line["parity_max_rel_S"] = {m: p.get("max_rel_S") for m, p in par.items()
                                     if isinstance(p, dict) and "max_rel_S" in p}
+    vs = out.get("vec_streamed")
+    if vs:
+        line["blocks_in_host_memory"] = {"ms_per_step": round(vs["ms_per_step"], 3),
+                                         "bitwise_equal_resident": vs["bitwise_equal_resident"]}
     mj = out.get("multi_j")
     if mj:
         line["multi_j"] = {m: round(v["value"], 1) for m, v in mj["modes"].items()}
@@ -668,6 +704,8 @@ def main():
                     help="offered duration of each serving rate")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device copy (PMC calibration)")
+    ap.add_argument("--no-vec-stream", dest="vec_stream", action="store_false",
+                    help="skip timing FP64 batched with every eigenvector block in host memory")
     ap.add_argument("--mu-parts", type=int, default=4,
                     help="also time GI_MODE_FP64 with mu streamed from host memory in this "
                          "many parts (0 to skip)")
@@ -763,6 +801,7 @@ def main():
 
     cpu = None
     parity = None
+    vec_streamed = None
     if oracle_py is not None:
         sample = lower_ok[::args.cpu_stride].astype(np.int32)
         t = time.time()
@@ -800,6 +839,10 @@ def main():
             parity["split_1500K"] = split_at_temperature(eng, job, binding, oracle_py, 1500.0,
                                                          nthr)
         log(f"parity {parity}")
+        if args.vec_stream and "fp64_batched" in mode_names:
+            resident = {k: v.copy() for k, v in
+                        eng.compute(bufs=bufs, mode=binding.GI_MODE_FP64_BATCHED).items()}
+            vec_streamed = vec_stream_section(binding, job, bufs, resident, 3)
         job.vectors.clear()
     mu_streamed = None
     if "fp64" in mode_names and args.mu_parts > 1:
@@ -822,7 +865,8 @@ def main():
 
     ok = (parity_ok(parity) and parity_ok((multi_j or {}).get("parity"))
           and parity_ok((c3v or {}).get("parity"))
-          and (mu_streamed or {}).get("bitwise_equal_resident", True))
+          and (mu_streamed or {}).get("bitwise_equal_resident", True)
+          and (vec_streamed or {}).get("bitwise_equal_resident", True))
     out = {
         "metric": METRIC, "value": value, "unit": "transitions/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
@@ -844,6 +888,7 @@ def main():
         "parity": parity,
         "parity_ok": bool(ok),
         "mu_streamed": mu_streamed,
+        "vec_streamed": vec_streamed,
         "multi_j": multi_j,
         "serving": serving,
         "c3v_multi_j": c3v,

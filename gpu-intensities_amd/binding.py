@@ -36,7 +36,7 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_params", "gi_uplo
            "gi_autotune", "gi_set_split_digits_each", "gi_sweep_create", "gi_sweep_run", "gi_sweep_candidates",
            "gi_sweep_destroy", "gi_amp_width", "gi_set_split_format", "gi_get_split_format",
            "gi_set_mu_budget", "gi_mu_parts", "gi_half_ls_noalloc",
-           "gi_set_path_limits"]
+           "gi_set_path_limits", "gi_set_vector_budget", "gi_streamed_blocks"]
 # (TROVE_EXPORTS below: the symbols of include/gi_trove.h)
 
 
@@ -130,6 +130,8 @@ def lib():
         L.gi_set_tuning.argtypes = [vp, C.POINTER(GiTuning)]
         L.gi_get_tuning.argtypes = [vp, C.POINTER(GiTuning)]
         L.gi_set_path_limits.argtypes = [vp, ci, ci, ci, ci, ci]
+        L.gi_set_vector_budget.argtypes = [vp, cll]
+        L.gi_streamed_blocks.argtypes = [vp]
         L.gi_autotune.argtypes = [vp, ci, ci, P_i, C.POINTER(GiLines), ci, C.POINTER(GiTuning),
                                   C.c_char_p, cll, C.POINTER(cll)]
         _LIB = L
@@ -186,10 +188,12 @@ class Engine:
         self._keep.append(a)
         return _p(a, P_d)
 
-    def load(self, job, device_vectors=None, device_dipole=None, mu_budget=0):
+    def load(self, job, device_vectors=None, device_dipole=None, mu_budget=0, vector_budget=0):
         """Set parameters, bases, dipole, levels and eigenvector blocks.
         device_vectors: optional {(jind, g): torch tensor on the device};
-        mu_budget: device bytes for mu (gi_set_mu_budget; 0: no limit)."""
+        mu_budget: device bytes for mu (gi_set_mu_budget; 0: no limit);
+        vector_budget: device bytes for the blocks uploaded from host memory
+        (gi_set_vector_budget; 0: no limit)."""
         L = lib()
         self.job = job
         nrep = job.sym["nrep"]
@@ -219,6 +223,7 @@ class Engine:
                        nsize=self._i32(b.nsize))
             self._chk(L.gi_upload_bset(self.h, jind, C.byref(s)), "gi_upload_bset")
         self._chk(L.gi_set_mu_budget(self.h, int(mu_budget)), "gi_set_mu_budget")
+        self._chk(L.gi_set_vector_budget(self.h, int(vector_budget)), "gi_set_vector_budget")
         if device_dipole is not None:
             self._chk(L.gi_upload_dipole(self.h, job.nvib, C.c_void_p(device_dipole.data_ptr()), 1),
                       "gi_upload_dipole")
@@ -262,6 +267,10 @@ class Engine:
             self._chk(lib().gi_count_candidates(self.h, len(ids), _p(ids, P_i), C.byref(n)),
                       "count")
         return n.value
+
+    def streamed_blocks(self):
+        """Eigenvector blocks kept in host memory (gi_streamed_blocks)."""
+        return lib().gi_streamed_blocks(self.h)
 
     def mu_parts(self):
         """Parts mu is streamed in (gi_mu_parts; 1: resident)."""

@@ -103,6 +103,10 @@ void free_slot(Slot &s)
         if (e) (void)hipEventDestroy(e);
         e = nullptr;
     }
+    for (auto &e : s.vec_ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
     for (auto &e : s.buf) (void)hipFree(e.second.first);
     s.buf.clear();
     if (s.pinned) (void)hipHostFree(s.pinned);
@@ -189,6 +193,9 @@ int prepare_split(gi_ctx *c)
 {
     if (!c->prepared) return fail(c, GI_ESTATE, "split mode before gi_prepare");
     if (c->mu_streamed) return fail(c, GI_ESTATE, "split mode needs mu resident");
+    if (host_blocks(c))
+        return fail(c, GI_ESTATE, "split mode needs every eigenvector block resident "
+                                  "(raise gi_set_vector_budget and upload the blocks again)");
     if (c->split_ready == c->split_key()) return GI_OK;
     HIPCHK(hipSetDevice(c->device));
     free_split(c);
@@ -296,6 +303,9 @@ static void free_plan(gi_ctx *c)
     c->d_mu_perm = nullptr;
     (void)hipHostFree(c->h_mu_perm);
     c->h_mu_perm = nullptr;
+    (void)hipFree(c->d_iota);
+    c->d_iota = nullptr;
+    c->iota_n = 0;
     (void)hipFree(c->d_lv_energy);
     (void)hipFree(c->d_lv_weight);
     c->d_lv_energy = c->d_lv_weight = nullptr;
@@ -344,8 +354,10 @@ void gi_destroy(gi_ctx *c)
     sweeps_shutdown(c);
     (void)hipStreamSynchronize(c->stream);
     free_plan(c);
-    for (auto &b : c->blocks)
+    for (auto &b : c->blocks) {
         if (b.owned) (void)hipFree((void *)b.d_vec);
+        if (b.h_vec) (void)hipHostFree(b.h_vec);
+    }
     if (c->mu_owned) (void)hipFree(c->d_mu);
     free_slot(c->main_slot);
     (void)hipStreamDestroy(c->main_slot.cstream);
@@ -358,8 +370,11 @@ const char *gi_last_error(const gi_ctx *c) { return c ? c->err.c_str() : "null c
 // Drops the bases and eigenvector blocks: they belong to the J list.
 static void reset_inputs(gi_ctx *c)
 {
-    for (auto &b : c->blocks)
+    for (auto &b : c->blocks) {
         if (b.owned) (void)hipFree((void *)b.d_vec);
+        if (b.h_vec) (void)hipHostFree(b.h_vec);
+    }
+    c->vec_dev_bytes = 0;
     free_split(c);
     c->blocks.clear();
     c->blocks.resize((size_t)c->nJ * c->nrep);
@@ -566,7 +581,12 @@ int gi_upload_vectors(gi_ctx *c, int jind, int igamma, int nsize, int nrec, cons
     HIPCHK(hipSetDevice(c->device));
     free_plan(c);
     Block &B = c->blocks[jind * c->nrep + igamma];
-    if (B.owned) (void)hipFree((void *)B.d_vec);
+    if (B.owned) {
+        (void)hipFree((void *)B.d_vec);
+        c->vec_dev_bytes -= (long long)sizeof(double) * B.nsize * B.nrec;
+    }
+    if (B.h_vec) (void)hipHostFree(B.h_vec);
+    B.h_vec = nullptr;
     B.d_vec = nullptr;
     B.owned = false;
     B.nsize = nsize;
@@ -577,14 +597,31 @@ int gi_upload_vectors(gi_ctx *c, int jind, int igamma, int nsize, int nrec, cons
         B.d_vec = v;
         return GI_OK;
     }
-    double *d = nullptr;
     const size_t bytes = sizeof(double) * (size_t)nsize * nrec;
+    if (c->vec_budget > 0 && c->vec_dev_bytes + (long long)bytes > c->vec_budget) {
+        // beyond the budget: pinned host memory, records staged per chunk
+        HIPCHK(hipHostMalloc((void **)&B.h_vec, bytes, hipHostMallocDefault));
+        std::memcpy(B.h_vec, v, bytes);
+        return GI_OK;
+    }
+    double *d = nullptr;
     HIPCHK(hipMalloc((void **)&d, bytes));
     B.owned = true;
     B.d_vec = d;
+    c->vec_dev_bytes += (long long)bytes;
     HIPCHK(hipMemcpy(d, v, bytes, hipMemcpyHostToDevice));
     return GI_OK;
 }
+
+int gi_set_vector_budget(gi_ctx *c, long long bytes)
+{
+    if (!c || bytes < 0) return GI_EINVAL;
+    if (serving(c)) return GI_ESTATE;
+    c->vec_budget = bytes;
+    return GI_OK;
+}
+
+int gi_streamed_blocks(const gi_ctx *c) { return c ? host_blocks(c) : 0; }
 
 int gi_prepare(gi_ctx *c)
 {
@@ -736,6 +773,16 @@ int gi_prepare(gi_ctx *c)
         if ((rc = upload(c, &B.d_col, col)) != GI_OK) return rc;
         if ((rc = upload(c, &B.d_en, B.en)) != GI_OK) return rc;
         if ((rc = upload(c, &B.d_upok, B.upok)) != GI_OK) return rc;
+    }
+    // identity map over block positions: staged upper records of a block kept
+    // in host memory are addressed through it
+    if (host_blocks(c)) {
+        int n = 1;
+        for (const Block &B : c->blocks) n = std::max(n, (int)B.lev.size());
+        std::vector<int> iota(n);
+        for (int i = 0; i < n; i++) iota[i] = i;
+        if ((rc = upload(c, &c->d_iota, iota)) != GI_OK) return rc;
+        c->iota_n = n;
     }
     // partition-function inputs: level energies and weights gns * ndeg * (2J + 1)
     {

@@ -24,6 +24,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
@@ -158,6 +159,11 @@ struct Item {
 
 struct Chunk {
     int query, jindI, gI, jindF, gF, nitems, nunits, ncand, cb_mask, Lo, Hi;
+    // blocks kept in host memory: the lower records (distinct, srec[srec0..
+    // srec0+nsrec) of the plan) and the upper records of [Lo, Hi) are staged
+    bool lhost = false, uhost = false;
+    long long srec0 = 0;
+    int nsrec = 0;
     size_t off_urec;       // per unit: record of the lower level in its block
     long long cand_begin;  // first candidate slot (slots are assigned in chunk order)
     size_t off_uptr, off_uideg, off_iunit, off_cstart, off_lo, off_hi, off_level, off_eI, off_out;
@@ -170,6 +176,12 @@ struct RoundOut {
     int nb = 0;
     size_t cbase = 0, obase = 0;
     hipEvent_t ev = nullptr;
+};
+
+// Staging buffers of the blocks kept in host memory (see vec_stage_setup).
+struct VecStage {
+    double *lbuf[2] = {nullptr, nullptr}, *ubuf[2] = {nullptr, nullptr};
+    hipStream_t cs = nullptr;
 };
 
 // Everything one launch sequence needs; lives until its work has finished.
@@ -189,6 +201,9 @@ struct Plan {
     long long ncand = 0;
     std::vector<Chunk> chunks;
     std::vector<unsigned char> meta;
+    bool vstream = false;                          // some chunk reads a block in host memory
+    std::vector<int> srec;                         // staged lower records of such chunks
+    VecStage vs;
     LineSoA comp{};                                // compacted lines (slot buffers)
     LineSoA cand{};                                // candidate lines (their own compaction
                                                    // when every candidate passes)
@@ -352,14 +367,19 @@ static int build_chunks(gi_ctx *c, Plan &P)
                 hi(n_it), lev(n_it);
             std::vector<double> eI(n_it);
             int cnt = 0, Lo = 1 << 30, Hi = 0;
+            const long long srec0 = (long long)P.srec.size();
             for (int k = 0; k < n_it; k++) {
                 const Item &it = P.items[its[s0 + k]];
+                const int rec = c->lv_irec0[it.level];
+                if (BI.h_vec) P.srec.push_back(rec);  // one staged copy per lower level
                 for (int ideg = 0; ideg < ndegI; ideg++) {
                     if (!degeneracy_filter(c, gI, gF, ideg, 0)) continue;
                     iunit[k * GI_MAXDEG + ideg] = (int)uideg.size();
                     uideg.push_back(ideg);
-                    urec.push_back(c->lv_irec0[it.level]);
-                    uptr.push_back((long long)(BI.d_vec + (size_t)c->lv_irec0[it.level] * BI.nsize));
+                    urec.push_back(rec);
+                    // host block: the staged slot k, made an address by launch_plan
+                    uptr.push_back(BI.h_vec ? (long long)k
+                                            : (long long)(BI.d_vec + (size_t)rec * BI.nsize));
                 }
                 cstart[k] = cnt;
                 lo[k] = it.lo;
@@ -382,6 +402,11 @@ static int build_chunks(gi_ctx *c, Plan &P)
             Chunk ch;
             ch.query = query;
             ch.jindI = jindI; ch.gI = gI; ch.jindF = jindF; ch.gF = gF;
+            ch.lhost = BI.h_vec != nullptr;
+            ch.uhost = c->blocks[jindF * c->nrep + gF].h_vec != nullptr;
+            ch.srec0 = srec0;
+            ch.nsrec = (int)((long long)P.srec.size() - srec0);
+            P.vstream = P.vstream || ch.lhost || ch.uhost;
             ch.nitems = n_it; ch.nunits = (int)uideg.size(); ch.ncand = cnt; ch.cb_mask = mask;
             ch.Lo = Lo; ch.Hi = Hi;
             ch.cand_begin = run;
@@ -472,6 +497,11 @@ static std::vector<std::pair<size_t, size_t>> plan_rounds(const gi_ctx *c, const
     if (const char *env = std::getenv("GI_BATCH_BYTES")) {
         const long long v = std::atoll(env);
         if (v > 0) kRoundBytes = (size_t)v;
+    }
+    if (P.vstream) {  // staged blocks: a round per chunk, its records staged before it
+        std::vector<std::pair<size_t, size_t>> one;
+        for (size_t i = 0; i < bytes.size(); i++) one.push_back({i, i + 1});
+        return one;
     }
     size_t left = 0;
     for (size_t b : bytes) left += b;
@@ -640,6 +670,100 @@ static unsigned char *stage(gi_ctx *c, Slot &S, const void *src, size_t n, int *
     return p;
 }
 
+// Staging of the blocks kept in host memory (gi_set_vector_budget): two
+// device buffers each for lower and upper records, used by alternate chunks;
+// chunk i's copies on the copy stream wait until chunk i-2's kernels are done
+// with the buffer, and chunk i's kernels wait until its copies are complete.
+// Device address of staged upper record f of a chunk whose window starts at
+// Lo, for kernels that index records as vec + blk_col[f] * nsize with the
+// identity map: the staging buffer shifted back by Lo records.
+static const double *staged_rows(const double *buf, int Lo, int nsize)
+{
+    return (const double *)((uintptr_t)buf - (uintptr_t)Lo * nsize * sizeof(double));
+}
+
+static int vec_stage_setup(gi_ctx *c, Slot &S, Plan &P, VecStage &V)
+{
+    if (!P.vstream) return GI_OK;
+    int rc = GI_OK;
+    size_t lmax = 1, umax = 1;
+    for (const Chunk &ch : P.chunks) {
+        if (ch.lhost)
+            lmax = std::max(lmax, (size_t)ch.nsrec * c->blocks[ch.jindI * c->nrep + ch.gI].nsize);
+        if (ch.uhost)
+            umax = std::max(umax, (size_t)std::max(ch.Hi - ch.Lo, 0) *
+                                      c->blocks[ch.jindF * c->nrep + ch.gF].nsize);
+    }
+    for (int b = 0; b < 2; b++) {
+        V.lbuf[b] = (double *)slot_scratch(c, S, b ? "vs_l1" : "vs_l0", 8 * lmax, &rc);
+        if (rc) return rc;
+        V.ubuf[b] = (double *)slot_scratch(c, S, b ? "vs_u1" : "vs_u0", 8 * umax, &rc);
+        if (rc) return rc;
+    }
+    for (auto &e : S.vec_ev)
+        if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            return fail(c, GI_EHIP, "hipEventCreate");
+    V.cs = S.cstream ? S.cstream : S.stream;
+    // the staged lower records' addresses into every such chunk's unit table
+    for (size_t i = 0; i < P.chunks.size(); i++) {
+        const Chunk &ch = P.chunks[i];
+        if (!ch.lhost) continue;
+        const int nsz = c->blocks[ch.jindI * c->nrep + ch.gI].nsize;
+        long long *up = (long long *)(P.meta.data() + ch.off_uptr);
+        for (int u = 0; u < ch.nunits; u++)
+            up[u] = (long long)(V.lbuf[i & 1] + (size_t)up[u] * nsz);
+    }
+    return GI_OK;
+}
+
+// Copies of chunk i's records in host memory to staging buffer i & 1 (runs
+// of consecutive records in one copy each), then the compute stream waits.
+static int vec_stage_chunk(gi_ctx *c, Slot &S, const Plan &P, size_t i, const VecStage &V)
+{
+    const Chunk &ch = P.chunks[i];
+    if (!(ch.lhost || ch.uhost)) return GI_OK;
+    const int b = (int)(i & 1);
+    if (i >= 2) HIPCHK(hipStreamWaitEvent(V.cs, S.vec_ev[2 + b], 0));
+    auto runs = [&](const Block &B, double *dst, int n, auto rec_of) -> int {
+        int s0 = 0;
+        while (s0 < n) {
+            int s1 = s0 + 1;
+            while (s1 < n && rec_of(s1) == rec_of(s1 - 1) + 1) s1++;
+            HIPCHK(hipMemcpyAsync(dst + (size_t)s0 * B.nsize,
+                                  B.h_vec + (size_t)rec_of(s0) * B.nsize,
+                                  (size_t)(s1 - s0) * B.nsize * sizeof(double),
+                                  hipMemcpyHostToDevice, V.cs));
+            s0 = s1;
+        }
+        return GI_OK;
+    };
+    int rc;
+    if (ch.lhost) {
+        const Block &BI = c->blocks[ch.jindI * c->nrep + ch.gI];
+        if ((rc = runs(BI, V.lbuf[b], ch.nsrec,
+                       [&](int s) { return P.srec[(size_t)ch.srec0 + s]; })) != GI_OK)
+            return rc;
+    }
+    if (ch.uhost && ch.Hi > ch.Lo) {
+        const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
+        if ((rc = runs(BF, V.ubuf[b], ch.Hi - ch.Lo,
+                       [&](int s) { return c->lv_irec0[BF.lev[ch.Lo + s]]; })) != GI_OK)
+            return rc;
+    }
+    HIPCHK(hipEventRecord(S.vec_ev[b], V.cs));
+    HIPCHK(hipStreamWaitEvent(S.stream, S.vec_ev[b], 0));
+    return GI_OK;
+}
+
+// After chunk i's last kernel that reads its staged records.
+static int vec_stage_done(gi_ctx *c, Slot &S, const Plan &P, size_t i)
+{
+    const Chunk &ch = P.chunks[i];
+    if (!(ch.lhost || ch.uhost)) return GI_OK;
+    HIPCHK(hipEventRecord(S.vec_ev[2 + (i & 1)], S.stream));
+    return GI_OK;
+}
+
 // Stream capture of one round into a graph (sweeps). On an error path the
 // destructor ends the capture so the stream is usable again.
 struct RoundCapture {
@@ -772,6 +896,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         if ((rc = cap.begin(c, P, S.stream)) != GI_OK) return rc;
         const int hcfg = round_halfls_cfg(rg.first, rg.second);
         const int TM = tile_rows(hcfg), TN = tile_cols(hcfg);
+        if (P.vstream && (rc = vec_stage_chunk(c, S, P, rg.first, P.vs)) != GI_OK) return rc;
         std::vector<HalfJob> jobs;
         std::vector<GBuildJob> gjobs;
         std::vector<GMapJob> mjobs_v;
@@ -923,6 +1048,10 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             m.item_hi = (const int *)(d_meta + ch.off_hi);
             m.item_out_off = (const long long *)(d_meta + ch.off_out);
             m.blk_col = BF.d_col; m.vec = BF.d_vec; m.Ht = Htc[i - rg.first]; m.amp = cand.amp;
+            if (ch.uhost) {  // staged upper records of this chunk's window
+                m.vec = staged_rows(P.vs.ubuf[i & 1], ch.Lo, BF.nsize);
+                m.blk_col = c->d_iota;
+            }
             m.amp_m = c->amp_m;
             m.kparts = corr_kparts(c, P, ch);
             m.kchunk = (BF.nsize + 16 * m.kparts - 1) / (16 * m.kparts) * 16;
@@ -967,6 +1096,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         launch_corr_mfma(S.stream, c->tile_corr, (const CorrMMArgs *)d_mm,
                          (const int *)((unsigned char *)d_mm + mm_bytes), (int)mm.size(), ctiles);
         TL_END();
+        if (P.vstream && (rc = vec_stage_done(c, S, P, rg.first)) != GI_OK) return rc;
         if ((rc = round_intensity(c, S, P, rg.first, rg.second, d_meta, K, flag, cand, d_int,
                                   icap, tl)) != GI_OK)
             return rc;
@@ -1346,6 +1476,8 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
         P.counts = (long long *)slot_pinned(c, S, P.q.size() * 8 + 64, &rc);
         if (rc) return rc;
     }
+    // staging of blocks in host memory (patches the unit tables before the upload)
+    if ((rc = vec_stage_setup(c, S, P, P.vs)) != GI_OK) return rc;
     unsigned char *d_meta = nullptr;
     if (!P.meta.empty()) {
         d_meta = (unsigned char *)slot_scratch(c, S, "meta", P.meta.size(), &rc); if (rc) return rc;
@@ -1392,7 +1524,9 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
                     return fail(c, GI_EHIP, "hipEventCreate");
         }
         int nparts = 0;  // parts copied so far; part q goes to buffer q & 1
-        for (auto &ch : P.chunks) {
+        for (size_t ci = 0; ci < P.chunks.size(); ci++) {
+            const Chunk &ch = P.chunks[ci];
+            if (P.vstream && (rc = vec_stage_chunk(c, S, P, ci, P.vs)) != GI_OK) return rc;
             const JPlan &pI = c->jp[ch.jindI], &pF = c->jp[ch.jindF];
             const GroupPlan &GP = group_of(c, ch.jindI, ch.gI, ch.jindF);
             const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
@@ -1461,6 +1595,10 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
             TL_END();
             CorrArgs a;
             fill_corr_args(c, P, ch, d_meta, K, a);
+            if (ch.uhost) {  // staged upper records of this chunk's window
+                a.vec = staged_rows(P.vs.ubuf[ci & 1], ch.Lo, BF.nsize);
+                a.blk_col = c->d_iota;
+            }
             a.Ht = Ht;
             a.flag = flag; a.out_iI = cand.iI; a.out_iF = cand.iF; a.out_nu = cand.nu;
             a.out_S = cand.S; a.out_A = cand.A; a.out_I = cand.I; a.out_amp = cand.amp;
@@ -1468,6 +1606,7 @@ static int launch_plan(gi_ctx *c, Slot &S, Plan &P, Timeline *tl)
             TL_BEGIN("corr_intensity");
             launch_corr_intensity(S.stream, a);
             TL_END();
+            if (P.vstream && (rc = vec_stage_done(c, S, P, ci)) != GI_OK) return rc;
             double blk_bytes = 0;
             for (const OutClass &oc : GP.oc) blk_bytes += 8.0 * oc.n * oc.K;
             P.work[3] += ch.nunits * 2.0 * coef_of(c, ch.jindI, ch.jindF).C * (double)N * N;
@@ -2071,6 +2210,8 @@ static int sweep_create(gi_ctx *c, int mode, int nlower, const int *ids, gi_swee
         return fail(c, GI_EINVAL, "gi_sweep_create: sweeps run the matrix-core modes");
     if (c->mu_streamed)
         return fail(c, GI_ESTATE, "gi_sweep_create: the matrix-core modes need mu resident");
+    if (host_blocks(c))
+        return fail(c, GI_ESTATE, "gi_sweep_create: sweeps need every eigenvector block resident");
     int rc;
     if (mode == GI_MODE_SPLIT && (rc = prepare_split(c)) != GI_OK) return rc;
     HIPCHK(hipSetDevice(c->device));

@@ -126,6 +126,9 @@ struct Block {
     int nsize = 0, nrec = 0;
     const double *d_vec = nullptr;
     bool owned = false;
+    // beyond the vector budget (gi_set_vector_budget): the block stays in
+    // pinned host memory and the FP64 modes copy the records a chunk needs
+    double *h_vec = nullptr;
     std::vector<int> lev;            // level ids, energy order
     std::vector<double> en;
     std::vector<unsigned char> upok; // energy_filter_upper per level
@@ -165,6 +168,7 @@ struct Slot {
     hipStream_t stream = nullptr;
     hipStream_t cstream = nullptr;   // device-to-host copies that overlap `stream`
     hipEvent_t mu_ev[4] = {};        // streamed mu: part ready / part free, two buffers
+    hipEvent_t vec_ev[4] = {};       // streamed blocks: records staged / staging free, two buffers
     std::map<std::string, std::pair<void *, size_t>> buf;
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
@@ -214,6 +218,12 @@ struct gi_ctx {
     std::vector<double> h_mu;        // mu as uploaded (streamed mode only)
     double *h_mu_perm = nullptr;     // pinned, permuted (streamed mode only)
     int mu_np = 0;                   // rows (vF) per part
+    // eigenvector blocks beyond a device budget (gi_set_vector_budget): device
+    // bytes of the blocks uploaded from host memory, the budget (0: none), and
+    // an identity map 0..iota_n-1 that addresses staged upper records
+    long long vec_budget = 0, vec_dev_bytes = 0;
+    int *d_iota = nullptr;
+    int iota_n = 0;
     int nlevels = 0;
     std::vector<double> lv_energy;
     std::vector<int> lv_jind, lv_igamma, lv_ndeg, lv_irec0, lv_quanta;
@@ -348,6 +358,14 @@ inline ClassDev classdev(const OutClass &oc)
     d.rsI = d.rptr + (ns * d.ncols + 1);
     d.rcoef = oc.d_rcoef;
     return d;
+}
+
+// Blocks kept in host memory beyond the vector budget.
+inline int host_blocks(const gi_ctx *c)
+{
+    int n = 0;
+    for (const Block &b : c->blocks) n += b.h_vec != nullptr;
+    return n;
 }
 
 int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out);
