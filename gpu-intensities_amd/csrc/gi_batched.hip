@@ -246,6 +246,65 @@ __global__ __launch_bounds__(256) void k_build_g_map(const GMapJob *__restrict__
     }
 }
 
+// k_build_g_map for maps of width <= 4: four consecutive G elements per thread
+// (vector loads of the map), the maps of both lower components kept in
+// registers (units of a degenerate block alternate them), and the gathers of
+// four units issued before their stores. The same sums in the same order.
+__global__ __launch_bounds__(256) void k_build_g_map4(const GMapJob *__restrict__ jobs,
+                                                      const int *__restrict__ begin, int njobs,
+                                                      int ntiles)
+{
+    constexpr int NB = 4;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
+    const GMapJob &J = jobs[find_job(begin, njobs, tile)];
+    const int local = tile - J.tile_begin;
+    const int ug = local / J.gtiles, gt = local - ug * J.gtiles;
+    const int g = (gt * 256 + (int)threadIdx.x) * 4;
+    if (g >= J.Gtot) return;
+    const int ng = min(4, J.Gtot - g);
+    int ix[2][4][4];
+    double wv[2][4][4];
+#pragma unroll
+    for (int d = 0; d < 2; d++)
+#pragma unroll
+        for (int w = 0; w < 4; w++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const bool ok = w < J.W && r < ng;
+                ix[d][w][r] = ok ? J.idx[d][(size_t)w * J.Gtot + g + r] : 0;
+                wv[d][w][r] = ok ? J.wt[d][(size_t)w * J.Gtot + g + r] : 0.0;
+            }
+    const int u0 = ug * kGUnits, u1 = min(u0 + kGUnits, J.nunits);
+    for (int ub = u0; ub < u1; ub += NB) {
+        double v[NB][4];
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            v[q][0] = v[q][1] = v[q][2] = v[q][3] = 0.0;
+            const int u = ub + q;
+            if (u >= u1) continue;
+            const bool d1 = J.uideg[u] != 0;
+            const double *rec = J.uptr[u];
+#pragma unroll
+            for (int w = 0; w < 4; w++)
+                if (w < J.W)
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        v[q][r] += (d1 ? wv[1][w][r] : wv[0][w][r]) *
+                                   rec[d1 ? ix[1][w][r] : ix[0][w][r]];
+        }
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int u = ub + q;
+            if (u >= u1) continue;
+            double *dst = J.G + (size_t)u * J.Gtot + g;
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                if (r < ng) dst[r] = v[q][r];
+        }
+    }
+}
+
 // K5 over jobs: H_co[m, (u, j)] = sum_k mu-block[m, k] * G[k, (u, j)] on the
 // matrix cores. K runs segment by segment (a tile never straddles two mu
 // blocks).
@@ -574,6 +633,14 @@ __global__ __launch_bounds__(1024) void k_build_g_map_lds(const GMapJob *__restr
             G[g] = v;
         }
     }
+}
+
+void launch_build_g_map4(hipStream_t st, const GMapJob *d_jobs, const int *d_begin, int njobs,
+                         int ntiles)
+{
+    if (njobs <= 0 || ntiles <= 0) return;
+    hipLaunchKernelGGL(k_build_g_map4, dim3(xcd_grid(ntiles)), dim3(256), 0, st, d_jobs, d_begin,
+                       njobs, ntiles);
 }
 
 void launch_build_g_map_lds(hipStream_t st, const GMapJob *d_jobs, const int *d_begin,

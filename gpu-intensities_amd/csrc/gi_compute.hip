@@ -874,6 +874,9 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
     GMapJob *d_mjobs = (GMapJob *)slot_scratch(
         c, S, "mjobs", (size_t)mchunks * (sizeof(GMapJob) + 4), &rc);
     if (rc) return rc;
+    GMapJob *d_mjobs2 = (GMapJob *)slot_scratch(
+        c, S, "mjobs2", (size_t)mchunks * (sizeof(GMapJob) + 4), &rc);
+    if (rc) return rc;
     const size_t fcap = (size_t)mchunks * sizeof(FoldJob), icap = (size_t)mchunks * sizeof(IntensArgs);
     unsigned char *d_fold = (unsigned char *)slot_scratch(c, S, "foldjobs", fcap + 4 * mchunks, &rc);
     if (rc) return rc;
@@ -899,11 +902,23 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         if (P.vstream && (rc = vec_stage_chunk(c, S, P, rg.first, P.vs)) != GI_OK) return rc;
         std::vector<HalfJob> jobs;
         std::vector<GBuildJob> gjobs;
-        std::vector<GMapJob> mjobs_v;
+        // G from the record map: one unit per workgroup with its record in LDS
+        // when the record fits and the map is at most 4 wide, else the
+        // global-gather build (the same bits)
+        std::vector<GMapJob> mjobs_v, mjobs_g;
         std::vector<int> gbegin, tbegin, mbegin, mubeg;
-        bool mlds = true;  // every map job fits the one-unit-per-workgroup build
         int munits = 0;
         int mt_tiles = 0;
+        // global-gather builds four elements per thread when every such map of
+        // the round is at most 4 wide
+        bool m4 = true;
+        for (size_t i = rg.first; i < rg.second; i++) {
+            const GroupPlan &GP = group_of(c, chunks[i].jindI, chunks[i].gI, chunks[i].jindF);
+            if (!(GP.gW <= 4 && c->blocks[chunks[i].jindI * c->nrep + chunks[i].gI].nsize <=
+                                    c->lim_rec_lds))
+                m4 = m4 && GP.gW <= 4;
+        }
+        const int mrows = m4 ? 1024 : 256;
         std::vector<CorrMMArgs> mm;
         std::vector<double *> Hc(rg.second - rg.first), Htc(rg.second - rg.first);
         int gt = 0, tt = 0;
@@ -927,18 +942,22 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                 mj.W = GP.gW;
                 mj.Gtot = GP.Gtot_p;
                 mj.nunits = ch.nunits;
-                mj.gtiles = (GP.Gtot_p + 255) / 256;
-                mj.tile_begin = mt_tiles;
-                mt_tiles += mj.gtiles * ((ch.nunits + kGUnits - 1) / kGUnits);
+                mj.gtiles = (GP.Gtot_p + mrows - 1) / mrows;
                 mj.uptr = (const double *const *)(d_meta + ch.off_uptr);
                 mj.uideg = (const int *)(d_meta + ch.off_uideg);
                 mj.G = G + og;
                 mj.nsize = c->blocks[ch.jindI * c->nrep + ch.gI].nsize;
-                mlds = mlds && GP.gW <= 4 && mj.nsize <= c->lim_rec_lds;
-                mjobs_v.push_back(mj);
-                mbegin.push_back(mj.tile_begin);
-                mubeg.push_back(munits);
-                munits += ch.nunits;
+                if (GP.gW <= 4 && mj.nsize <= c->lim_rec_lds) {
+                    mj.tile_begin = 0;
+                    mjobs_v.push_back(mj);
+                    mubeg.push_back(munits);
+                    munits += ch.nunits;
+                } else {
+                    mj.tile_begin = mt_tiles;
+                    mt_tiles += mj.gtiles * ((ch.nunits + kGUnits - 1) / kGUnits);
+                    mjobs_g.push_back(mj);
+                    mbegin.push_back(mj.tile_begin);
+                }
             } else if (ch.nunits > 0 && GP.Gtot_p > 0) {
                 GBuildJob gj;
                 gj.classes = GP.d_cd;
@@ -972,21 +991,34 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                 }
             og += need[i].g; oh += need[i].h; oht += need[i].ht;
         }
-        const int njobs = (int)jobs.size(), ngj = (int)gjobs.size(), nmj = (int)mjobs_v.size();
+        const int njobs = (int)jobs.size(), ngj = (int)gjobs.size(), nmj = (int)mjobs_v.size(),
+                  nmg = (int)mjobs_g.size();
         if (nmj > 0) {
             const size_t mb = nmj * sizeof(GMapJob);
             unsigned char *hb = stage(c, S, mjobs_v.data(), mb + 4 * (size_t)nmj, &rc);
             if (rc) return rc;
-            std::memcpy(hb + mb, (mlds ? mubeg : mbegin).data(), 4 * (size_t)nmj);
+            std::memcpy(hb + mb, mubeg.data(), 4 * (size_t)nmj);
             HIPCHK(hipMemcpyAsync(d_mjobs, hb, mb + 4 * (size_t)nmj, hipMemcpyHostToDevice,
                                   S.stream));
             TL_BEGIN("build_g");
-            if (mlds)
-                launch_build_g_map_lds(S.stream, d_mjobs,
-                                       (const int *)((unsigned char *)d_mjobs + mb), nmj, munits);
+            launch_build_g_map_lds(S.stream, d_mjobs, (const int *)((unsigned char *)d_mjobs + mb),
+                                   nmj, munits);
+            TL_END();
+        }
+        if (nmg > 0) {
+            const size_t mb = nmg * sizeof(GMapJob);
+            unsigned char *hb = stage(c, S, mjobs_g.data(), mb + 4 * (size_t)nmg, &rc);
+            if (rc) return rc;
+            std::memcpy(hb + mb, mbegin.data(), 4 * (size_t)nmg);
+            HIPCHK(hipMemcpyAsync(d_mjobs2, hb, mb + 4 * (size_t)nmg, hipMemcpyHostToDevice,
+                                  S.stream));
+            TL_BEGIN("build_g");
+            if (m4)
+                launch_build_g_map4(S.stream, d_mjobs2,
+                                    (const int *)((unsigned char *)d_mjobs2 + mb), nmg, mt_tiles);
             else
-                launch_build_g_map(S.stream, d_mjobs, (const int *)((unsigned char *)d_mjobs + mb),
-                                   nmj, mt_tiles);
+                launch_build_g_map(S.stream, d_mjobs2,
+                                   (const int *)((unsigned char *)d_mjobs2 + mb), nmg, mt_tiles);
             TL_END();
         }
         if (ngj > 0) {
@@ -1164,6 +1196,9 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
     const size_t gj_bytes = (size_t)mchunks * sizeof(GSplitJob);
     unsigned char *d_gj = (unsigned char *)slot_scratch(c, S, "sgjobs", gj_bytes + 4 * mchunks, &rc);
     if (rc) return rc;
+    unsigned char *d_gj2 =
+        (unsigned char *)slot_scratch(c, S, "sgjobs2", gj_bytes + 4 * mchunks, &rc);
+    if (rc) return rc;
     const size_t cj_bytes = (size_t)mchunks * sizeof(CorrSplitArgs);
     unsigned char *d_cj = (unsigned char *)slot_scratch(c, S, "scorr", cj_bytes + 4 * mchunks, &rc);
     if (rc) return rc;
@@ -1198,26 +1233,30 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
     long long *pc_offs = nullptr;
     if ((rc = setup_pipe(c, S, P, rounds, &pc_counts, &pc_offs)) != GI_OK) return rc;
 
-    // G digits four rows per thread when every record map is at most 4 wide, and
-    // one unit per workgroup with its record in LDS when every record also fits
-    bool g4 = true, glds = true;
-    for (const Chunk &ch : chunks) {
-        g4 = g4 && group_of(c, ch.jindI, ch.gI, ch.jindF).gW <= 4;
-        glds = glds && c->blocks[ch.jindI * c->nrep + ch.gI].nsize <= c->lim_rec_lds &&
-               group_of(c, ch.jindI, ch.gI, ch.jindF).oc.size() <= (size_t)kClsLds;
-    }
-    glds = glds && g4;
-    const int grows = g4 ? 1024 : 256;
+    // G digits of a chunk: one unit per workgroup with its record in LDS when the
+    // record fits, its classes fit the kernel's table and its map is at most 4
+    // wide; else the global-gather build (four rows per thread when every such
+    // map of the round is at most 4 wide). The two builds give the same bits.
+    const auto g_lds = [&](const Chunk &ch) {
+        const GroupPlan &GP = group_of(c, ch.jindI, ch.gI, ch.jindF);
+        return c->blocks[ch.jindI * c->nrep + ch.gI].nsize <= c->lim_rec_lds &&
+               GP.oc.size() <= (size_t)kClsLds && GP.gW <= 4;
+    };
     const int srows_tile = c->tile_split == 1 ? 128 : 64;  // k_halfls_split(_wide, 2) row tile
     size_t iround = 0;
     for (auto &rg : rounds) {
         RoundCapture cap;
         if ((rc = cap.begin(c, P, S.stream)) != GI_OK) return rc;
-        std::vector<GSplitJob> gj;
+        std::vector<GSplitJob> gj, gjg;  // G builds in LDS, global-gather G builds
         std::vector<SplitHalfJob> hj;
         std::vector<CorrSplitArgs> cj;
-        std::vector<int> gbegin, tbegin, cbegin;
+        std::vector<int> gbegin, gbegin_g, tbegin, cbegin;
         int gt = 0, tt = 0, ct = 0, ubeg = 0;
+        bool g4 = true;  // every global-gather map of the round at most 4 wide
+        for (size_t i = rg.first; i < rg.second; i++)
+            if (!g_lds(chunks[i])) g4 = g4 && group_of(c, chunks[i].jindI, chunks[i].gI,
+                                                       chunks[i].jindF).gW <= 4;
+        const int grows = g4 ? 1024 : 256;
         bool steps_fit = true;  // every class within k_halfls_split2's step table
         size_t ogd = 0, oge = 0, oh = 0, oht = 0, ohd = 0, ohc = 0;
         std::vector<size_t> Hoff, Htoff, Hdoff, Hcoff;
@@ -1243,7 +1282,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                 g.nunits = ch.nunits;
                 g.gtiles = (GP.GtotB + grows - 1) / grows;
                 g.tile_begin = gt;
-                gt += g.gtiles * ((ch.nunits + kGUnits - 1) / kGUnits);
+                if (!g_lds(ch)) gt += g.gtiles * ((ch.nunits + kGUnits - 1) / kGUnits);
                 g.uptr = (const double *const *)(d_meta + ch.off_uptr);
                 g.uideg = (const int *)(d_meta + ch.off_uideg);
                 g.urec = (const int *)(d_meta + ch.off_urec);
@@ -1252,9 +1291,14 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                 g.plane = (long long)mgd;
                 g.E = Ge + oge;
                 g.nsize = BI.nsize;
-                gj.push_back(g);
-                gbegin.push_back(glds ? ubeg : g.tile_begin);
-                ubeg += ch.nunits;
+                if (g_lds(ch)) {
+                    gj.push_back(g);
+                    gbegin.push_back(ubeg);
+                    ubeg += ch.nunits;
+                } else {
+                    gjg.push_back(g);
+                    gbegin_g.push_back(g.tile_begin);
+                }
                 for (const OutClass &oc : GP.oc) {
                     steps_fit = steps_fit &&
                                 split2_class_fits(oc.seg_n.data(), (int)oc.seg_n.size()) &&
@@ -1295,12 +1339,17 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             if ((rc = send(d_gj, gj.data(), gj.size() * sizeof(GSplitJob), gbegin, gj_bytes)) != GI_OK)
                 return rc;
             TL_BEGIN("build_g");
-            if (glds)
-                launch_build_g_split_lds(S.stream, SH, c->split_radix, (const GSplitJob *)d_gj,
-                                         (const int *)(d_gj + gj_bytes), (int)gj.size(), ubeg);
-            else
-                launch_build_g_split(S.stream, SH, c->split_radix, (const GSplitJob *)d_gj,
-                                     (const int *)(d_gj + gj_bytes), (int)gj.size(), gt, g4);
+            launch_build_g_split_lds(S.stream, SH, c->split_radix, (const GSplitJob *)d_gj,
+                                     (const int *)(d_gj + gj_bytes), (int)gj.size(), ubeg);
+            TL_END();
+        }
+        if (!gjg.empty()) {
+            if ((rc = send(d_gj2, gjg.data(), gjg.size() * sizeof(GSplitJob), gbegin_g,
+                           gj_bytes)) != GI_OK)
+                return rc;
+            TL_BEGIN("build_g");
+            launch_build_g_split(S.stream, SH, c->split_radix, (const GSplitJob *)d_gj2,
+                                 (const int *)(d_gj2 + gj_bytes), (int)gjg.size(), gt, g4);
             TL_END();
         }
         if (!hj.empty()) {

@@ -126,12 +126,18 @@ constexpr int kGMapMaxW = 16;
 constexpr int kClsLds = 32;  // output classes of a group that k_build_g_split_lds keeps in LDS
 constexpr int kGUnits = 16;
 // Records up to this many doubles are staged in LDS by the one-unit-per-workgroup
-// G builds (launch_build_g_map_lds, launch_build_g_split_lds).
-constexpr int kRecLds = 16384;
+// G builds (launch_build_g_map_lds, launch_build_g_split_lds): 158 KB of the
+// 160 KB a gfx950 workgroup can hold, leaving room for the split build's class
+// table.
+constexpr int kRecLds = 20224;
 void launch_build_g_map(hipStream_t st, const GMapJob *d_jobs, const int *d_begin, int njobs,
                         int ntiles);
 // One workgroup per unit, its record in LDS (maps of width <= 4, records of <=
 // kRecLds doubles); d_begin = first unit of each job.
+// k_build_g_map for maps of width <= 4 with 1024 G elements per tile (gtiles
+// counts 1024-element tiles).
+void launch_build_g_map4(hipStream_t st, const GMapJob *d_jobs, const int *d_begin, int njobs,
+                         int ntiles);
 void launch_build_g_map_lds(hipStream_t st, const GMapJob *d_jobs, const int *d_begin,
                             int njobs, int nunits);
 void launch_build_g_units(hipStream_t st, int N, const GBuildJob *d_jobs, const int *d_begin,

@@ -232,12 +232,17 @@ __global__ __launch_bounds__(256) void k_build_g_split(const GSplitJob *__restri
 
 // The same for record maps of width <= 4, four consecutive rows of one column
 // per thread (columns are padded to 64 rows): vector loads of the map and one
-// 4-byte store per digit plane instead of four byte stores.
+// 4-byte store per digit plane instead of four byte stores. The maps of both
+// lower components stay in registers (units of a degenerate block alternate
+// them), and the gathers of four units are issued before the first digit
+// store (byte stores may alias any load, so later loads cannot move above
+// them). Every element sums the same products in the same order.
 template <int S>
 __global__ __launch_bounds__(256) void k_build_g_split4(const GSplitJob *__restrict__ jobs,
                                                         const int *__restrict__ begin, int njobs,
                                                         int ntiles, double R)
 {
+    constexpr int NB = 4;  // units gathered together
     const int tile = xcd_tile_s(blockIdx.x, gridDim.x);
     if (tile >= ntiles) return;
     const GSplitJob &J = jobs[find_tile_job(begin, njobs, tile)];
@@ -250,46 +255,61 @@ __global__ __launch_bounds__(256) void k_build_g_split4(const GSplitJob *__restr
     const ClassDev &cd = J.classes[co];
     const int j = (g - cd.cbaseB) / cd.kpad;
     const bool first = (g - cd.cbaseB) - j * cd.kpad == 0;
-    int4 ix[4];
-    double wv[4][4];
-    double bnd = 0.0;
-    int cur = -1;
+    int4 ix[2][4];
+    double wv[2][4][4], bnd[2];
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            ix[d][w] = int4{0, 0, 0, 0};
+            wv[d][w][0] = wv[d][w][1] = wv[d][w][2] = wv[d][w][3] = 0.0;
+            if (w < J.W) {
+                const size_t o = (size_t)w * J.GtotB + g;
+                ix[d][w] = *(const int4 *)(J.idx[d] + o);
+                const double2 a = *(const double2 *)(J.wt[d] + o);
+                const double2 b = *(const double2 *)(J.wt[d] + o + 2);
+                wv[d][w][0] = a.x; wv[d][w][1] = a.y; wv[d][w][2] = b.x; wv[d][w][3] = b.y;
+            }
+        }
+        bnd[d] = J.bnd[d][cd.eoff + j];
+    }
     const int u0 = ug * kGUnits, u1 = min(u0 + kGUnits, J.nunits);
-    for (int u = u0; u < u1; u++) {
-        const int d = J.uideg[u];
-        if (d != cur) {
+    for (int ub = u0; ub < u1; ub += NB) {
+        double v[NB][4];
+        int ex[NB];
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            v[q][0] = v[q][1] = v[q][2] = v[q][3] = 0.0;
+            ex[q] = 0;
+            const int u = ub + q;
+            if (u >= u1) continue;
+            const bool d1 = J.uideg[u] != 0;
+            const double *rec = J.uptr[u];
 #pragma unroll
             for (int w = 0; w < 4; w++)
                 if (w < J.W) {
-                    const size_t o = (size_t)w * J.GtotB + g;
-                    ix[w] = *(const int4 *)(J.idx[d] + o);
-                    const double2 a = *(const double2 *)(J.wt[d] + o);
-                    const double2 b = *(const double2 *)(J.wt[d] + o + 2);
-                    wv[w][0] = a.x; wv[w][1] = a.y; wv[w][2] = b.x; wv[w][3] = b.y;
+                    const int4 x = d1 ? ix[1][w] : ix[0][w];
+                    v[q][0] += (d1 ? wv[1][w][0] : wv[0][w][0]) * rec[x.x];
+                    v[q][1] += (d1 ? wv[1][w][1] : wv[0][w][1]) * rec[x.y];
+                    v[q][2] += (d1 ? wv[1][w][2] : wv[0][w][2]) * rec[x.z];
+                    v[q][3] += (d1 ? wv[1][w][3] : wv[0][w][3]) * rec[x.w];
                 }
-            bnd = J.bnd[d][cd.eoff + j];
-            cur = d;
+            ex[q] = digit_exp((d1 ? bnd[1] : bnd[0]) * J.umax[J.urec[u]]);
         }
-        const double *rec = J.uptr[u];
-        const int e = digit_exp(bnd * J.umax[J.urec[u]]);
-        double v[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int w = 0; w < 4; w++)
-            if (w < J.W) {
-                v[0] += wv[w][0] * rec[ix[w].x];
-                v[1] += wv[w][1] * rec[ix[w].y];
-                v[2] += wv[w][2] * rec[ix[w].z];
-                v[3] += wv[w][3] * rec[ix[w].w];
-            }
-        int dg[4][S];
+        for (int q = 0; q < NB; q++) {
+            const int u = ub + q;
+            if (u >= u1) continue;
+            int dg[4][S];
 #pragma unroll
-        for (int r = 0; r < 4; r++) to_digits<S>(v[r], e, dg[r], R);
-        int8_t *dst = J.D + (size_t)u * J.GtotB + g;
+            for (int r = 0; r < 4; r++) to_digits<S>(v[q][r], ex[q], dg[r], R);
+            int8_t *dst = J.D + (size_t)u * J.GtotB + g;
 #pragma unroll
-        for (int q = 0; q < S; q++)
-            *(int *)(dst + q * J.plane) = (dg[0][q] & 255) | ((dg[1][q] & 255) << 8) |
-                                          ((dg[2][q] & 255) << 16) | ((dg[3][q] & 255) << 24);
-        if (first) J.E[(size_t)u * J.EGtot + cd.eoff + j] = e;
+            for (int p = 0; p < S; p++)
+                *(int *)(dst + p * J.plane) = (dg[0][p] & 255) | ((dg[1][p] & 255) << 8) |
+                                              ((dg[2][p] & 255) << 16) | ((dg[3][p] & 255) << 24);
+            if (first) J.E[(size_t)u * J.EGtot + cd.eoff + j] = ex[q];
+        }
     }
 }
 

@@ -321,7 +321,7 @@ int gi_get_tuning(gi_ctx *ctx, gi_tuning *t);
  * the correlation K per split-K part in both matrix-core modes
  * (parts = min(16, K / kpart_len); default 1024 for FP64 and 2048 for the
  * split digits, at most 2048, at least 64); rec_lds, the longest eigenvector record the G builds
- * stage in LDS (longer ones use the global-gather builds; default 16384);
+ * stage in LDS (longer ones use the global-gather builds; default 20224);
  * split2_steps / split2_segs, the largest class (64-byte K steps, segments)
  * the two-step split half-ls takes (larger ones run the one-step kernel;
  * default 512 / 64); fold_split_rows, the longest folded column the fused

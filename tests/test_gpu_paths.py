@@ -2,7 +2,7 @@
 
 At real sizes the batched modes switch kernels on the size of a product:
 split-K correlation (upper blocks of 2048 functions and more), global-gather
-G builds (lower records over 16,384 doubles), the one-step split half-ls
+G builds (lower records over 20,224 doubles), the one-step split half-ls
 (classes over 512 K steps or 64 segments) and the two-kernel fold and digit
 split (folded columns over 32,768). Lowering those thresholds sends small jobs
 down the same code. Each route is checked against the FP64 oracle, and the
@@ -102,7 +102,7 @@ def test_split_k_correlation_vs_oracle(routed, mode):
 
 def test_path_limits_rules(routed):
     job, eng, ref = routed
-    for bad in (dict(kpart_len=4096), dict(kpart_len=32), dict(rec_lds=20000),
+    for bad in (dict(kpart_len=4096), dict(kpart_len=32), dict(rec_lds=30000),
                 dict(split2_steps=513), dict(split2_segs=65), dict(fold_split_rows=40000),
                 dict(rec_lds=-1)):
         with pytest.raises(binding.GIError):
