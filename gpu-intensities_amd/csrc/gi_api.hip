@@ -174,7 +174,9 @@ void free_split(gi_ctx *c)
 {
     (void)hipFree(c->d_mud);
     (void)hipFree(c->d_mue);
+    if (c->h_mud) (void)hipHostFree(c->h_mud);
     c->d_mud = nullptr;
+    c->h_mud = nullptr;
     c->d_mue = nullptr;
     for (auto &b : c->blocks) {
         (void)hipFree(b.d_ud); (void)hipFree(b.d_ue); (void)hipFree(b.d_umax);
@@ -187,12 +189,45 @@ void free_split(gi_ctx *c)
     c->split_ready = 0;
 }
 
+// Digit planes of a streamed mu, built part by part on the device from the
+// pinned FP64 parts (the same rows and exponents as from resident mu: each
+// vF row's digits depend on that row only) and kept in pinned host memory;
+// the split mode's half-ls copies them back in parts.
+static int split_mu_streamed(gi_ctx *c, const int *d_vinv)
+{
+    const int N = c->nvib, np = c->mu_np, SH = c->split_Sh;
+    const size_t kp = (size_t)c->kpad_all;
+    HIPCHK(hipHostMalloc((void **)&c->h_mud, (size_t)SH * c->muplane, hipHostMallocDefault));
+    double *dpart = nullptr;
+    int8_t *gpart = nullptr;
+    hipError_t e = hipMalloc((void **)&dpart, (size_t)3 * N * np * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&gpart, (size_t)SH * 3 * np * kp);
+    for (int v0 = 0; e == hipSuccess && v0 < N; v0 += np) {
+        const int n = std::min(np, N - v0);
+        // rows (a, vI) of the part's n vF columns, then their digit rows (q, a, vF)
+        e = hipMemcpy2DAsync(dpart, (size_t)n * 8, c->h_mu_perm + v0, (size_t)N * 8,
+                             (size_t)n * 8, (size_t)3 * N, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) break;
+        launch_split_mu(c->stream, SH, c->split_radix, N, n, n, c->kpad_all, dpart, d_vinv, gpart,
+                        3LL * n * kp, c->d_mue + v0);
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpy2DAsync(c->h_mud + v0 * kp, (size_t)N * kp, gpart, (size_t)n * kp,
+                                 (size_t)n * kp, (size_t)3 * SH, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // the part buffers are reused
+    }
+    (void)hipFree(dpart);
+    (void)hipFree(gpart);
+    if (e != hipSuccess)
+        return fail(c, GI_EHIP, std::string("streamed mu digits: ") + hipGetErrorString(e));
+    return GI_OK;
+}
+
 // Digit planes of mu and of every resident eigenvector block for the split
 // mode (built once per digit count; freed with the plan).
 int prepare_split(gi_ctx *c)
 {
     if (!c->prepared) return fail(c, GI_ESTATE, "split mode before gi_prepare");
-    if (c->mu_streamed) return fail(c, GI_ESTATE, "split mode needs mu resident");
     if (host_blocks(c))
         return fail(c, GI_ESTATE, "split mode needs every eigenvector block resident "
                                   "(raise gi_set_vector_budget and upload the blocks again)");
@@ -241,11 +276,16 @@ int prepare_split(gi_ctx *c)
     int *d_vinv = nullptr;
     if ((rc = upload(c, &d_vinv, vinv)) != GI_OK) return rc;
     c->muplane = 3LL * N * c->kpad_all;
-    HIPCHK(hipMalloc((void **)&c->d_mud, (size_t)SH * c->muplane));
     HIPCHK(hipMalloc((void **)&c->d_mue, sizeof(int) * N));
-    launch_split_mu(c->stream, SH, c->split_radix, N, c->kpad_all, c->d_mu_perm, d_vinv, c->d_mud, c->muplane,
-                    c->d_mue);
-    HIPCHK(hipGetLastError());
+    if (!c->mu_streamed) {
+        HIPCHK(hipMalloc((void **)&c->d_mud, (size_t)SH * c->muplane));
+        launch_split_mu(c->stream, SH, c->split_radix, N, N, N, c->kpad_all, c->d_mu_perm, d_vinv,
+                        c->d_mud, c->muplane, c->d_mue);
+        HIPCHK(hipGetLastError());
+    } else if ((rc = split_mu_streamed(c, d_vinv)) != GI_OK) {
+        (void)hipFree(d_vinv);
+        return rc;
+    }
     for (auto &b : c->blocks) {
         if (b.nrec <= 0 || b.nsize <= 0 || !b.d_vec) continue;
         b.Kp = (b.nsize + 63) / 64 * 64;

@@ -3,7 +3,8 @@
 //
 // K5 (k_halfls_jobs): for one output class co and a chunk of units,
 //   H_co[m, (u, j)] = sum_k A[m, k] * B[k, (u, j)],
-//   A[m, k] = mu_{a(e)}[off_co + m, off_e + k - row0_e]   (column-major, ld N)
+//   A[m, k] = mu_{a(e)}[off_co + m, off_e + k - row0_e]   (column-major, ld N,
+//             or one streamed part of vF rows: MuRows)
 //   B[k, (u, j)] = G[u*Gtot + gbase_p + k*ncols + j]  (packed G)
 // so mu blocks are read once per chunk instead of once per lower level.
 //
@@ -310,10 +311,8 @@ __global__ __launch_bounds__(256) void k_build_g_map4(const GMapJob *__restrict_
 // blocks).
 template <int BM_, int BN_, int NW_ = 4>
 __global__ __launch_bounds__(64 * NW_) __attribute__((amdgpu_waves_per_eu(NW_ == 8 ? 4 : 1)))
-void k_halfls_jobs(int N, const HalfJob *__restrict__ jobs,
-                                                          const int *__restrict__ tbegin,
-                                                          int njobs, int ntiles,
-                                                          const double *__restrict__ mu)
+void k_halfls_jobs(const HalfJob *__restrict__ jobs, const int *__restrict__ tbegin, int njobs,
+                   int ntiles)
 {
     using T = TileW<BM_, BN_, NW_>;
     __shared__ double As[BK * T::PA];
@@ -323,14 +322,16 @@ void k_halfls_jobs(int N, const HalfJob *__restrict__ jobs,
     const HalfJob &J = jobs[find_job(tbegin, njobs, tile)];
     const ClassDev &cd = J.cd;
     const int local = tile - J.tile_begin;
-    const int m0 = (local % J.mt) * BM_, n0 = (local / J.mt) * BN_;
+    const int m0 = J.mr.m_lo + (local % J.mt) * BM_, n0 = (local / J.mt) * BN_;
+    const int mhi = J.mr.m_hi, ld = J.mr.ld;
+    const double *__restrict__ mu = (const double *)J.mr.mu;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * (BM_ / 2), wn = (wave >> 1) * (BN_ / T::WC);
     const int ncolsTot = J.nunits * cd.ncols;
     // loads: A row am, K rows ak0 + q*(NT/BM); B column bn, K rows bk0 + q*(NT/BN)
     const int am = tid % BM_, ak0 = tid / BM_;
     const int bn = tid % BN_, bk0 = tid / BN_;
-    const bool mok = m0 + am < cd.n;
+    const bool mok = m0 + am < mhi;
     const int n = n0 + bn;
     const bool nok = n < ncolsTot;
     const double *gcol = J.G;
@@ -347,13 +348,13 @@ void k_halfls_jobs(int N, const HalfJob *__restrict__ jobs,
     int e = 0, kk0 = 0;
     auto fetch = [&](int se, int sk0) {
         const int len = min(BK, cd.seg_n[se] - sk0);
-        const double *mp = mu + (size_t)cd.seg_a[se] * N * N +
-                           (size_t)(cd.seg_off[se] + sk0) * N + (cd.off + m0 + am);
+        const double *mp = mu + (size_t)cd.seg_a[se] * J.mr.plane +
+                           (size_t)(cd.seg_off[se] + sk0) * ld + (cd.off + m0 + am - J.mr.v0);
         const double *gp = gcol + (size_t)(cd.seg_row0[se] + sk0) * cd.ncols;
 #pragma unroll
         for (int q = 0; q < T::LA; q++) {
             const int kl = ak0 + q * (T::NT / BM_);
-            ra[q] = (mok && kl < len) ? mp[(size_t)kl * N] : 0.0;
+            ra[q] = (mok && kl < len) ? mp[(size_t)kl * ld] : 0.0;
         }
 #pragma unroll
         for (int q = 0; q < T::LB; q++) {
@@ -385,7 +386,7 @@ void k_halfls_jobs(int N, const HalfJob *__restrict__ jobs,
             for (int r = 0; r < 4; r++) {
                 const int m = m0 + wm + 16 * x + 4 * r + (lane >> 4);
                 const int nn = n0 + wn + 16 * y + (lane & 15);
-                if (m >= cd.n || nn >= ncolsTot) continue;
+                if (m >= mhi || nn >= ncolsTot) continue;
                 const int u = nn / cd.ncols, jj = nn - u * cd.ncols;
                 J.H[(size_t)u * J.Htot + cd.hbase + (size_t)jj * cd.n + m] = acc[x][y][r];
             }
@@ -659,21 +660,21 @@ void launch_build_g_map(hipStream_t st, const GMapJob *d_jobs, const int *d_begi
                        njobs, ntiles);
 }
 
-void launch_halfls_jobs(hipStream_t st, int cfg, int N, const HalfJob *d_jobs,
-                        const int *d_tbegin, int njobs, int ttiles, const double *mu)
+void launch_halfls_jobs(hipStream_t st, int cfg, const HalfJob *d_jobs, const int *d_tbegin,
+                        int njobs, int ttiles)
 {
     if (njobs <= 0 || ttiles <= 0) return;
     const dim3 g(xcd_grid(ttiles)), b(256);
     if (cfg == kTile128x128w8) {
-        hipLaunchKernelGGL((k_halfls_jobs<128, 128, 8>), g, dim3(512), 0, st, N, d_jobs, d_tbegin,
-                           njobs, ttiles, mu);
+        hipLaunchKernelGGL((k_halfls_jobs<128, 128, 8>), g, dim3(512), 0, st, d_jobs, d_tbegin,
+                           njobs, ttiles);
         return;
     }
     switch (cfg & 3) {
-    case 0: hipLaunchKernelGGL((k_halfls_jobs<64, 64>), g, b, 0, st, N, d_jobs, d_tbegin, njobs, ttiles, mu); break;
-    case 1: hipLaunchKernelGGL((k_halfls_jobs<64, 128>), g, b, 0, st, N, d_jobs, d_tbegin, njobs, ttiles, mu); break;
-    case 2: hipLaunchKernelGGL((k_halfls_jobs<128, 64>), g, b, 0, st, N, d_jobs, d_tbegin, njobs, ttiles, mu); break;
-    default: hipLaunchKernelGGL((k_halfls_jobs<128, 128>), g, b, 0, st, N, d_jobs, d_tbegin, njobs, ttiles, mu); break;
+    case 0: hipLaunchKernelGGL((k_halfls_jobs<64, 64>), g, b, 0, st, d_jobs, d_tbegin, njobs, ttiles); break;
+    case 1: hipLaunchKernelGGL((k_halfls_jobs<64, 128>), g, b, 0, st, d_jobs, d_tbegin, njobs, ttiles); break;
+    case 2: hipLaunchKernelGGL((k_halfls_jobs<128, 64>), g, b, 0, st, d_jobs, d_tbegin, njobs, ttiles); break;
+    default: hipLaunchKernelGGL((k_halfls_jobs<128, 128>), g, b, 0, st, d_jobs, d_tbegin, njobs, ttiles); break;
     }
 }
 

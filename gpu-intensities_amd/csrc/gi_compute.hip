@@ -764,6 +764,74 @@ static int vec_stage_done(gi_ctx *c, Slot &S, const Plan &P, size_t i)
     return GI_OK;
 }
 
+// Streamed mu in the matrix-core modes (gi_set_mu_budget): parts of mu_np vF
+// rows of the permuted FP64 mu (batched mode) or of its digit planes (split
+// mode) go from pinned host memory into two device buffers on the copy
+// stream, and each part's half-ls products compute the class rows inside it,
+// every row a whole sum in the order of resident mu (the reference splits mu
+// into parts of vF rows the same way, do_1st_half_ls_blocks,
+// cuda_host.cu:783-817). Part q fills buffer q & 1, which is refilled only
+// after the products of part q - 2 are done.
+struct MuParts {
+    void *buf[2] = {nullptr, nullptr};
+    int nparts = 0;  // parts copied so far in this call
+};
+
+static int mu_parts_setup(gi_ctx *c, Slot &S, size_t bytes, MuParts &M)
+{
+    int rc = GI_OK;
+    for (int b = 0; b < 2; b++) {
+        M.buf[b] = slot_scratch(c, S, b ? "mupart1" : "mupart0", bytes, &rc);
+        if (rc) return rc;
+    }
+    for (auto &e : S.mu_ev)
+        if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            return fail(c, GI_EHIP, "hipEventCreate");
+    return GI_OK;
+}
+
+// Copy of the next part: `height` host rows of `width` bytes, `spitch` bytes
+// apart from src, packed into its buffer; the compute stream waits for it.
+// *b is the buffer; the caller records mu_ev[2 + *b] after the part's products.
+static int mu_part_copy(gi_ctx *c, Slot &S, MuParts &M, const void *src, size_t spitch,
+                        size_t width, size_t height, int *b, Timeline *tl)
+{
+    hipStream_t cs = S.cstream ? S.cstream : S.stream;
+    *b = M.nparts++ & 1;
+    if (M.nparts > 2) HIPCHK(hipStreamWaitEvent(cs, S.mu_ev[2 + *b], 0));
+    TL_BEGIN_ON("mu_copy", cs);
+    HIPCHK(hipMemcpy2DAsync(M.buf[*b], width, src, spitch, width, height, hipMemcpyHostToDevice,
+                            cs));
+    TL_END_ON(cs);
+    HIPCHK(hipEventRecord(S.mu_ev[*b], cs));
+    HIPCHK(hipStreamWaitEvent(S.stream, S.mu_ev[*b], 0));
+    return GI_OK;
+}
+
+// The half-ls jobs of one mu part: each job cut to its class rows inside
+// vF rows [v0, v0 + np) and reading mu from `mr` (its row range set here),
+// tiles renumbered from 0 (TM x TN tiles). Returns the tile count.
+template <class Job>
+static int part_jobs(const std::vector<Job> &all, int v0, int np, MuRows mr, int TM, int TN,
+                     std::vector<Job> &out, std::vector<int> &begin)
+{
+    int tiles = 0;
+    for (const Job &j0 : all) {
+        const int lo = std::max(0, v0 - j0.cd.off), hi = std::min(j0.cd.n, v0 + np - j0.cd.off);
+        if (lo >= hi) continue;
+        Job j = j0;
+        j.mr = mr;
+        j.mr.m_lo = lo;
+        j.mr.m_hi = hi;
+        j.mt = (hi - lo + TM - 1) / TM;
+        j.tile_begin = tiles;
+        tiles += j.mt * ((j.nunits * j.cd.ncols + TN - 1) / TN);
+        out.push_back(j);
+        begin.push_back(j.tile_begin);
+    }
+    return tiles;
+}
+
 // Stream capture of one round into a graph (sweeps). On an error path the
 // destructor ends the capture so the stream is usable again.
 struct RoundCapture {
@@ -892,6 +960,9 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
     int *pc_counts = nullptr;
     long long *pc_offs = nullptr;
     if ((rc = setup_pipe(c, S, P, rounds, &pc_counts, &pc_offs)) != GI_OK) return rc;
+    MuParts MP;
+    if (c->mu_streamed && (rc = mu_parts_setup(c, S, (size_t)3 * N * c->mu_np * 8, MP)) != GI_OK)
+        return rc;
     size_t iround = 0;
 
     for (auto &rg : rounds) {
@@ -986,6 +1057,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                     j.uideg = (const int *)(d_meta + ch.off_uideg);
                     j.r0 = r0; j.r1 = r1;
                     j.G = G + og; j.H = H + oh;
+                    j.mr = MuRows{c->d_mu_perm, (long long)N * N, N, 0, 0, oc.n};
                     jobs.push_back(j);
                     tbegin.push_back(j.tile_begin);
                 }
@@ -1034,19 +1106,37 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                                  (const int *)((unsigned char *)d_gjobs + gb), ngj, gt);
             TL_END();
         }
-        if (njobs > 0) {
+        // the half-ls products: one launch over resident mu, or one per mu part
+        auto halfls = [&](const std::vector<HalfJob> &hj, const std::vector<int> &hb0,
+                          int tiles) -> int {
             unsigned char *hb = stage(c, S, nullptr, jobs_bytes + 4 * (size_t)mjobs, &rc);
             if (rc) return rc;
-            std::memcpy(hb, jobs.data(), njobs * sizeof(HalfJob));
-            std::memcpy(hb + jobs_bytes, tbegin.data(), njobs * 4);
+            std::memcpy(hb, hj.data(), hj.size() * sizeof(HalfJob));
+            std::memcpy(hb + jobs_bytes, hb0.data(), hj.size() * 4);
             HIPCHK(hipMemcpyAsync(d_jobs, hb, jobs_bytes + 4 * (size_t)mjobs,
                                   hipMemcpyHostToDevice, S.stream));
-            const int *d_tb = (const int *)(d_jobs + jobs_bytes);
             TL_BEGIN("half_ls_mfma");
-            P.work[14] += tt;
-            launch_halfls_jobs(S.stream, hcfg, N, (const HalfJob *)d_jobs, d_tb, njobs,
-                               tt, c->d_mu_perm);
+            P.work[14] += tiles;
+            launch_halfls_jobs(S.stream, hcfg, (const HalfJob *)d_jobs,
+                               (const int *)(d_jobs + jobs_bytes), (int)hj.size(), tiles);
             TL_END();
+            return GI_OK;
+        };
+        if (njobs > 0 && !c->mu_streamed && (rc = halfls(jobs, tbegin, tt)) != GI_OK) return rc;
+        for (int v0 = 0; njobs > 0 && c->mu_streamed && v0 < N; v0 += c->mu_np) {
+            const int np = std::min(c->mu_np, N - v0);
+            std::vector<HalfJob> pj;
+            std::vector<int> pb;
+            const int pt = part_jobs(jobs, v0, np, MuRows{nullptr, (long long)N * np, np, v0, 0, 0},
+                                     TM, TN, pj, pb);
+            if (pt == 0) continue;
+            int b = 0;
+            if ((rc = mu_part_copy(c, S, MP, c->h_mu_perm + v0, (size_t)N * 8, (size_t)np * 8,
+                                   (size_t)3 * N, &b, tl)) != GI_OK)
+                return rc;
+            for (HalfJob &j : pj) j.mr.mu = MP.buf[b];
+            if ((rc = halfls(pj, pb, pt)) != GI_OK) return rc;
+            HIPCHK(hipEventRecord(S.mu_ev[2 + b], S.stream));
         }
         std::vector<FoldJob> fjobs;
         std::vector<int> fbegin;
@@ -1243,6 +1333,10 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                GP.oc.size() <= (size_t)kClsLds && GP.gW <= 4;
     };
     const int srows_tile = c->tile_split == 1 ? 128 : 64;  // k_halfls_split(_wide, 2) row tile
+    MuParts MP;
+    if (c->mu_streamed &&
+        (rc = mu_parts_setup(c, S, (size_t)SH * 3 * c->mu_np * c->kpad_all, MP)) != GI_OK)
+        return rc;
     size_t iround = 0;
     for (auto &rg : rounds) {
         RoundCapture cap;
@@ -1318,6 +1412,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                     h.gplane = (long long)mgd;
                     h.Ge = Ge + oge;
                     h.H = H + oh;
+                    h.mr = MuRows{c->d_mud, c->muplane, N, 0, 0, oc.n};
                     hj.push_back(h);
                     tbegin.push_back(h.tile_begin);
                 }
@@ -1352,16 +1447,36 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                                  (const int *)(d_gj2 + gj_bytes), (int)gjg.size(), gt, g4);
             TL_END();
         }
-        if (!hj.empty()) {
-            if ((rc = send(d_hj, hj.data(), hj.size() * sizeof(SplitHalfJob), tbegin, hj_bytes)) != GI_OK)
-                return rc;
+        // the half-ls products: one launch over resident mu digits, or one per part
+        const int scfg = c->tile_split == 2 && !steps_fit ? 0 : c->tile_split;
+        auto halfls = [&](const std::vector<SplitHalfJob> &jobs, const std::vector<int> &beg,
+                          int tiles) -> int {
+            int e = send(d_hj, jobs.data(), jobs.size() * sizeof(SplitHalfJob), beg, hj_bytes);
+            if (e != GI_OK) return e;
             TL_BEGIN("half_ls_split");
-            P.work[14] += tt;
-            launch_halfls_split(S.stream, SH, c->split_radix, N, c->kpad_all, c->d_mud, c->muplane, c->d_mue,
+            P.work[14] += tiles;
+            launch_halfls_split(S.stream, SH, c->split_radix, c->kpad_all, c->d_mue,
                                 (const SplitHalfJob *)d_hj, (const int *)(d_hj + hj_bytes),
-                                (int)hj.size(), tt,
-                                c->tile_split == 2 && !steps_fit ? 0 : c->tile_split);
+                                (int)jobs.size(), tiles, scfg);
             TL_END();
+            return GI_OK;
+        };
+        if (!hj.empty() && !c->mu_streamed && (rc = halfls(hj, tbegin, tt)) != GI_OK) return rc;
+        for (int v0 = 0; !hj.empty() && c->mu_streamed && v0 < N; v0 += c->mu_np) {
+            const int np = std::min(c->mu_np, N - v0);
+            const size_t kp = (size_t)c->kpad_all;
+            std::vector<SplitHalfJob> pj;
+            std::vector<int> pb;
+            const int pt = part_jobs(hj, v0, np, MuRows{nullptr, 3LL * np * (long long)kp, np, v0, 0, 0},
+                                     srows_tile, 64, pj, pb);
+            if (pt == 0) continue;
+            int b = 0;
+            if ((rc = mu_part_copy(c, S, MP, c->h_mud + v0 * kp, (size_t)N * kp, (size_t)np * kp,
+                                   (size_t)3 * SH, &b, tl)) != GI_OK)
+                return rc;
+            for (SplitHalfJob &j : pj) j.mr.mu = MP.buf[b];
+            if ((rc = halfls(pj, pb, pt)) != GI_OK) return rc;
+            HIPCHK(hipEventRecord(S.mu_ev[2 + b], S.stream));
         }
         std::vector<FoldJob> fjobs;
         std::vector<SplitRowsJob> sjobs;
@@ -1766,9 +1881,6 @@ int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
     if (c->server) return fail(c, GI_ESTATE, "gi_compute while the request server runs");
     if (mode != GI_MODE_FP64 && mode != GI_MODE_FP64_BATCHED && mode != GI_MODE_SPLIT)
         return fail(c, GI_EINVAL, "gi_compute: unknown mode");
-    if (mode != GI_MODE_FP64 && c->mu_streamed)
-        return fail(c, GI_ESTATE, "gi_compute: the matrix-core modes need mu resident "
-                                  "(raise gi_set_mu_budget and upload mu again)");
     int rc;
     if (mode == GI_MODE_SPLIT && (rc = prepare_split(c)) != GI_OK) return rc;
     using clk = std::chrono::steady_clock;
@@ -2258,7 +2370,8 @@ static int sweep_create(gi_ctx *c, int mode, int nlower, const int *ids, gi_swee
     if (mode != GI_MODE_FP64_BATCHED && mode != GI_MODE_SPLIT)
         return fail(c, GI_EINVAL, "gi_sweep_create: sweeps run the matrix-core modes");
     if (c->mu_streamed)
-        return fail(c, GI_ESTATE, "gi_sweep_create: the matrix-core modes need mu resident");
+        return fail(c, GI_ESTATE, "gi_sweep_create: sweeps need mu resident (a captured "
+                                  "sweep does not stream mu parts)");
     if (host_blocks(c))
         return fail(c, GI_ESTATE, "gi_sweep_create: sweeps need every eigenvector block resident");
     int rc;
@@ -2478,8 +2591,6 @@ int gi_serve_submit(gi_ctx *c, const gi_request *r, long long *ticket)
         return GI_EINVAL;
     if (r->mode == GI_MODE_SPLIT && c->split_ready != c->split_key())
         return fail(c, GI_ESTATE, "split-mode request before gi_prepare_split");
-    if (r->mode != GI_MODE_FP64 && c->mu_streamed)
-        return fail(c, GI_ESTATE, "gi_serve_submit: the matrix-core modes need mu resident");
     if (!(r->temperature > 0.0)) return GI_EINVAL;
     auto R = std::make_shared<Request>();
     R->r = *r;

@@ -276,6 +276,7 @@ struct gi_ctx {
     int split_S = 5, split_Sh = 5, split_ready = 0;
     int split_key() const { return 1 + split_req_radix * 64 + split_req_S * 8 + split_req_Sh; }
     int8_t *d_mud = nullptr;
+    int8_t *h_mud = nullptr;         // streamed mu: the digit planes in pinned host memory
     long long muplane = 0;
     int *d_mue = nullptr;
 

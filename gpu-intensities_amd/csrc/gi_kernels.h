@@ -86,9 +86,22 @@ struct IntensArgs {
     double *out_nu, *out_S, *out_A, *out_I;
 };
 
+// Where a half-ls job reads mu and which rows of its output class it
+// computes: rows [m_lo, m_hi); element (a, vI, vF) of the permuted FP64 mu at
+// mu + a*plane + vI*ld + (vF - v0), digit row (a, vF) of the split mode at
+// (a*ld + vF - v0)*kpad_all in planes `plane` bytes apart. Resident mu: the
+// whole class, ld = N, v0 = 0. Streamed mu: the class rows inside one part
+// of ld vF rows starting at v0 (each row's sum is the same either way).
+struct MuRows {
+    const void *mu;
+    long long plane;
+    int ld, v0, m_lo, m_hi;
+};
+
 // One (chunk, output class) of the batched half linestrength.
 struct HalfJob {
     ClassDev cd;
+    MuRows mr;
     int nunits, Gtot, Htot;
     int tile_begin;                  // first tile of the product
     int mt;                          // product tiles along rows
@@ -160,6 +173,7 @@ struct GSplitJob {
 };
 struct SplitHalfJob {
     ClassDev cd;
+    MuRows mr;                       // mu digit rows (int8, plane 0 at mr.mu)
     int nunits, tile_begin, mt, GtotB, EGtot, Htot;
     const int8_t *Gd;
     long long gplane;
@@ -187,8 +201,12 @@ struct CorrSplitArgs {
 };
 void launch_split_rows(hipStream_t st, int S, double R, const double *X, long long ld, int rows, int K,
                        int Kp, int8_t *D, long long plane, int *E, double *Mx);
-void launch_split_mu(hipStream_t st, int S, double R, int N, int kpad_all, const double *mu,
-                     const int *vinv, int8_t *D, long long plane, int *E);
+// Digit rows of nrows vF rows of mu: element (a, vI, vF) read at
+// mu[vF + vI*ld + a*N*ld] (ld = N: resident permuted mu; ld = nrows: a part
+// of vF rows), digit row (a, vF) written at D + (a*ld + vF)*kpad_all, the
+// exponent of row vF at E[vF].
+void launch_split_mu(hipStream_t st, int S, double R, int N, int ld, int nrows, int kpad_all,
+                     const double *mu, const int *vinv, int8_t *D, long long plane, int *E);
 // four: every job's record map has width <= 4 and gtiles counts 1024 rows
 // (k_build_g_split4); otherwise 256 rows per tile.
 void launch_build_g_split(hipStream_t st, int S, double R, const GSplitJob *d_jobs, const int *d_begin,
@@ -216,9 +234,9 @@ __host__ __device__ inline bool split2_class_fits(const int *seg_n, int nseg)
 {
     return nseg <= kSplitMaxSegs && split2_steps(seg_n, nseg) <= kSplitMaxSteps;
 }
-void launch_halfls_split(hipStream_t st, int S, double R, int N, int kpad_all, const int8_t *mud,
-                         long long muplane, const int *mue, const SplitHalfJob *d_jobs,
-                         const int *d_tbegin, int njobs, int ntiles, int cfg);
+void launch_halfls_split(hipStream_t st, int S, double R, int kpad_all, const int *mue,
+                         const SplitHalfJob *d_jobs, const int *d_tbegin, int njobs, int ntiles,
+                         int cfg);
 void launch_corr_split(hipStream_t st, int S, double R, const CorrSplitArgs *d_args, const int *d_cbegin,
                        int nchunks, int ntiles);
 
@@ -228,8 +246,8 @@ void launch_corr_split(hipStream_t st, int S, double R, const CorrSplitArgs *d_a
 constexpr int kTile128x128w8 = 5;
 int tile_rows(int cfg);
 int tile_cols(int cfg);
-void launch_halfls_jobs(hipStream_t st, int cfg, int N, const HalfJob *d_jobs,
-                        const int *d_tbegin, int njobs, int ttiles, const double *mu);
+void launch_halfls_jobs(hipStream_t st, int cfg, const HalfJob *d_jobs, const int *d_tbegin,
+                        int njobs, int ttiles);
 void launch_corr_mfma(hipStream_t st, int cfg, const CorrMMArgs *d_args, const int *d_cbegin,
                       int nchunks, int ntiles);
 void launch_intensity(hipStream_t st, const IntensArgs &a);

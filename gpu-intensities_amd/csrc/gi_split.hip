@@ -130,11 +130,12 @@ __global__ __launch_bounds__(256) void k_split_rows_jobs(const SplitRowsJob *__r
                  R);
 }
 
-// mu (permuted, mu_a(vF, vI) at vF + vI*N + a*N^2) to digit planes with row
-// (a, vF) at (a*N + vF)*kpad_all and column vpad(vI) (classes padded to 64;
-// vinv[p] = vI or -1 for padding); one exponent per vF over all a and vI.
+// mu (permuted, mu_a(vF, vI) at vF + vI*ld + a*N*ld: ld = N for the whole
+// mu, ld = rows of one part of vF rows) to digit planes with row (a, vF) at
+// (a*ld + vF)*kpad_all and column vpad(vI) (classes padded to 64; vinv[p] =
+// vI or -1 for padding); one exponent per vF over all a and vI.
 template <int S>
-__global__ __launch_bounds__(256) void k_split_mu(int N, int kpad_all,
+__global__ __launch_bounds__(256) void k_split_mu(int N, int ld, int kpad_all,
                                                   const double *__restrict__ mu,
                                                   const int *__restrict__ vinv,
                                                   int8_t *__restrict__ D, long long plane,
@@ -142,11 +143,11 @@ __global__ __launch_bounds__(256) void k_split_mu(int N, int kpad_all,
 {
     __shared__ double red[256];
     const int vF = blockIdx.x;
-    const long long NN = (long long)N * N;
+    const long long NL = (long long)N * ld;
     double m = 0.0;
     for (int i = threadIdx.x; i < 3 * N; i += 256) {
         const int a = i / N, vI = i - a * N;
-        m = fmax(m, fabs(mu[vF + (long long)vI * N + a * NN]));
+        m = fmax(m, fabs(mu[vF + (long long)vI * ld + a * NL]));
     }
     m = block_max(m, red);
     const int e = digit_exp(m);
@@ -155,8 +156,8 @@ __global__ __launch_bounds__(256) void k_split_mu(int N, int kpad_all,
         const int a = i / kpad_all, p = i - a * kpad_all;
         const int vI = vinv[p];
         int dg[S];
-        to_digits<S>(vI >= 0 ? mu[vF + (long long)vI * N + a * NN] : 0.0, e, dg, R);
-        int8_t *d = D + ((long long)a * N + vF) * kpad_all + p;
+        to_digits<S>(vI >= 0 ? mu[vF + (long long)vI * ld + a * NL] : 0.0, e, dg, R);
+        int8_t *d = D + ((long long)a * ld + vF) * kpad_all + p;
 #pragma unroll
         for (int q = 0; q < S; q++) d[q * plane] = (int8_t)dg[q];
     }
@@ -586,10 +587,7 @@ __device__ inline double oz_value(const v4i (&acc)[S], int r, double rinv)
 // Split half linestrength: H_co[m, (u, j)] = sum over segments of mu digits x
 // G digits, per (chunk, class) job; m-tiles fastest.
 template <int S>
-__global__ __launch_bounds__(256) void k_halfls_split(int N, int kpad_all,
-                                                      const int8_t *__restrict__ mud,
-                                                      long long muplane,
-                                                      const int *__restrict__ mue,
+__global__ __launch_bounds__(256) void k_halfls_split(int kpad_all, const int *__restrict__ mue,
                                                       const SplitHalfJob *__restrict__ jobs,
                                                       const int *__restrict__ tbegin, int njobs,
                                                       int ntiles, double R)
@@ -601,13 +599,16 @@ __global__ __launch_bounds__(256) void k_halfls_split(int N, int kpad_all,
     const SplitHalfJob &J = jobs[find_tile_job(tbegin, njobs, tile)];
     const ClassDev &cd = J.cd;
     const int local = tile - J.tile_begin;
-    const int m0 = (local % J.mt) * 64, n0 = (local / J.mt) * 64;
+    const int m0 = J.mr.m_lo + (local % J.mt) * 64, n0 = (local / J.mt) * 64;
+    const int mhi = J.mr.m_hi;
+    const int8_t *__restrict__ mud = (const int8_t *)J.mr.mu;
+    const long long muplane = J.mr.plane;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
     const int ncolsTot = J.nunits * cd.ncols;
     // loads: row/column tid/4, 16 bytes at K offset 16*(tid%4), all S planes
     const int lr = tid >> 2, lp = (tid & 3) * 16;
-    const bool mok = m0 + lr < cd.n;
+    const bool mok = m0 + lr < mhi;
     const int n = n0 + lr;
     const bool nok = n < ncolsTot;
     long long bcol = 0;
@@ -624,8 +625,9 @@ __global__ __launch_bounds__(256) void k_halfls_split(int N, int kpad_all,
             for (int l = 0; l < S; l++) acc[x][y][l] = v4i{0, 0, 0, 0};
     v4i ra[S], rb[S];
     auto fetch = [&](int se, int kk) {
-        const long long arow = ((long long)cd.seg_a[se] * N + cd.off + m0 + lr) * kpad_all +
-                               cd.seg_pad[2 * se + 1] + kk + lp;
+        const long long arow =
+            ((long long)cd.seg_a[se] * J.mr.ld + cd.off + m0 + lr - J.mr.v0) * kpad_all +
+            cd.seg_pad[2 * se + 1] + kk + lp;
         const long long bofs = bcol + cd.seg_pad[2 * se] + kk + lp;
 #pragma unroll
         for (int q = 0; q < S; q++) {
@@ -659,7 +661,7 @@ __global__ __launch_bounds__(256) void k_halfls_split(int N, int kpad_all,
             for (int r = 0; r < 4; r++) {
                 const int m = m0 + wm + 16 * x + 4 * (lane >> 4) + r;
                 const int nn = n0 + wn + 16 * y + (lane & 15);
-                if (m >= cd.n || nn >= ncolsTot) continue;
+                if (m >= mhi || nn >= ncolsTot) continue;
                 const int u = nn / cd.ncols, jj = nn - u * cd.ncols;
                 const int ex = mue[cd.off + m] + J.Ge[(size_t)u * J.EGtot + cd.eoff + jj];
                 J.H[(size_t)u * J.Htot + cd.hbase + (size_t)jj * cd.n + m] =
@@ -676,13 +678,8 @@ __global__ __launch_bounds__(256) void k_halfls_split(int N, int kpad_all,
 // k_halfls_split.
 template <int S>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_halfls_split2(
-    int N, int kpad_all,
-                                                       const int8_t *__restrict__ mud,
-                                                       long long muplane,
-                                                       const int *__restrict__ mue,
-                                                       const SplitHalfJob *__restrict__ jobs,
-                                                       const int *__restrict__ tbegin, int njobs,
-                                                       int ntiles, double R)
+    int kpad_all, const int *__restrict__ mue, const SplitHalfJob *__restrict__ jobs,
+    const int *__restrict__ tbegin, int njobs, int ntiles, double R)
 {
     __shared__ __attribute__((aligned(16))) int8_t As[S * 64 * kRowB];
     __shared__ __attribute__((aligned(16))) int8_t Bs[S * 64 * kRowB];
@@ -694,7 +691,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const SplitHalfJob &J = jobs[find_tile_job(tbegin, njobs, tile)];
     const ClassDev &cd = J.cd;
     const int local = tile - J.tile_begin;
-    const int m0 = (local % J.mt) * 64, n0 = (local / J.mt) * 64;
+    const int m0 = J.mr.m_lo + (local % J.mt) * 64, n0 = (local / J.mt) * 64;
+    const int mhi = J.mr.m_hi;
+    const int8_t *__restrict__ mud = (const int8_t *)J.mr.mu;
+    const long long muplane = J.mr.plane;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
     const int ncolsTot = J.nunits * cd.ncols;
@@ -715,7 +715,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if (!s_fits) {
         for (int i = tid; i < 64 * 64; i += 256) {
             const int m = m0 + (i & 63), nn = n0 + (i >> 6);
-            if (m >= cd.n || nn >= ncolsTot) continue;
+            if (m >= mhi || nn >= ncolsTot) continue;
             const int u = nn / cd.ncols, jj = nn - u * cd.ncols;
             J.H[(size_t)u * J.Htot + cd.hbase + (size_t)jj * cd.n + m] = __builtin_nan("");
         }
@@ -725,13 +725,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     for (int e = 0; e < cd.nseg; e++)
         for (int st = s_first[e] + tid; st < s_first[e + 1]; st += 256) {
             const int kk = (st - s_first[e]) * 64;
-            s_a[st] = ((long long)cd.seg_a[e] * N + cd.off + m0) * kpad_all +
+            s_a[st] = ((long long)cd.seg_a[e] * J.mr.ld + cd.off + m0 - J.mr.v0) * kpad_all +
                       cd.seg_pad[2 * e + 1] + kk;
             s_b[st] = cd.seg_pad[2 * e] + kk;
         }
     __syncthreads();
     const int lr = tid >> 2, lp = (tid & 3) * 16;
-    const bool mok = m0 + lr < cd.n;
+    const bool mok = m0 + lr < mhi;
     const int n = n0 + lr;
     const bool nok = n < ncolsTot;
     long long bcol = 0;
@@ -790,7 +790,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             for (int r = 0; r < 4; r++) {
                 const int m = m0 + wm + 16 * x + 4 * (lane >> 4) + r;
                 const int nn = n0 + wn + 16 * y + (lane & 15);
-                if (m >= cd.n || nn >= ncolsTot) continue;
+                if (m >= mhi || nn >= ncolsTot) continue;
                 const int u = nn / cd.ncols, jj = nn - u * cd.ncols;
                 const int ex = mue[cd.off + m] + J.Ge[(size_t)u * J.EGtot + cd.eoff + jj];
                 J.H[(size_t)u * J.Htot + cd.hbase + (size_t)jj * cd.n + m] =
@@ -834,8 +834,7 @@ __device__ inline void oz_step_wide(const int8_t *__restrict__ As,
 // so the result is bitwise that of the 64-row kernel.
 template <int S>
 __global__ __launch_bounds__(256, 1) void k_halfls_split_wide(
-    int N, int kpad_all, const int8_t *__restrict__ mud, long long muplane,
-    const int *__restrict__ mue, const SplitHalfJob *__restrict__ jobs,
+    int kpad_all, const int *__restrict__ mue, const SplitHalfJob *__restrict__ jobs,
     const int *__restrict__ tbegin, int njobs, int ntiles, double R)
 {
     __shared__ __attribute__((aligned(16))) int8_t As[S * 128 * kRowB];
@@ -845,12 +844,15 @@ __global__ __launch_bounds__(256, 1) void k_halfls_split_wide(
     const SplitHalfJob &J = jobs[find_tile_job(tbegin, njobs, tile)];
     const ClassDev &cd = J.cd;
     const int local = tile - J.tile_begin;
-    const int m0 = (local % J.mt) * 128, n0 = (local / J.mt) * 64;
+    const int m0 = J.mr.m_lo + (local % J.mt) * 128, n0 = (local / J.mt) * 64;
+    const int mhi = J.mr.m_hi;
+    const int8_t *__restrict__ mud = (const int8_t *)J.mr.mu;
+    const long long muplane = J.mr.plane;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 64, wn = (wave >> 1) * 32;
     const int ncolsTot = J.nunits * cd.ncols;
     const int lr = tid >> 2, lp = (tid & 3) * 16;
-    const bool mok0 = m0 + lr < cd.n, mok1 = m0 + 64 + lr < cd.n;
+    const bool mok0 = m0 + lr < mhi, mok1 = m0 + 64 + lr < mhi;
     const int n = n0 + lr;
     const bool nok = n < ncolsTot;
     long long bcol = 0;
@@ -867,8 +869,9 @@ __global__ __launch_bounds__(256, 1) void k_halfls_split_wide(
             for (int l = 0; l < S; l++) acc[x][y][l] = v4i{0, 0, 0, 0};
     v4i ra0[S], ra1[S], rb[S];
     auto fetch = [&](int se, int kk) {
-        const long long arow = ((long long)cd.seg_a[se] * N + cd.off + m0 + lr) * kpad_all +
-                               cd.seg_pad[2 * se + 1] + kk + lp;
+        const long long arow =
+            ((long long)cd.seg_a[se] * J.mr.ld + cd.off + m0 + lr - J.mr.v0) * kpad_all +
+            cd.seg_pad[2 * se + 1] + kk + lp;
         const long long bofs = bcol + cd.seg_pad[2 * se] + kk + lp;
 #pragma unroll
         for (int q = 0; q < S; q++) {
@@ -905,7 +908,7 @@ __global__ __launch_bounds__(256, 1) void k_halfls_split_wide(
             for (int r = 0; r < 4; r++) {
                 const int m = m0 + wm + 16 * x + 4 * (lane >> 4) + r;
                 const int nn = n0 + wn + 16 * y + (lane & 15);
-                if (m >= cd.n || nn >= ncolsTot) continue;
+                if (m >= mhi || nn >= ncolsTot) continue;
                 const int u = nn / cd.ncols, jj = nn - u * cd.ncols;
                 const int ex = mue[cd.off + m] + J.Ge[(size_t)u * J.EGtot + cd.eoff + jj];
                 J.H[(size_t)u * J.Htot + cd.hbase + (size_t)jj * cd.n + m] =
@@ -1104,10 +1107,11 @@ void launch_fold_split_jobs(hipStream_t st, int S, double R, const FoldSplitJob 
 #undef L_
 }
 
-void launch_split_mu(hipStream_t st, int S, double R, int N, int kpad_all, const double *mu,
-                     const int *vinv, int8_t *D, long long plane, int *E)
+void launch_split_mu(hipStream_t st, int S, double R, int N, int ld, int nrows, int kpad_all,
+                     const double *mu, const int *vinv, int8_t *D, long long plane, int *E)
 {
-#define L_(s) hipLaunchKernelGGL(k_split_mu<s>, dim3(N), dim3(256), 0, st, N, kpad_all, mu, vinv, D, plane, E, R)
+    if (nrows <= 0) return;
+#define L_(s) hipLaunchKernelGGL(k_split_mu<s>, dim3(nrows), dim3(256), 0, st, N, ld, kpad_all, mu, vinv, D, plane, E, R)
     GI_SPLIT_DISPATCH(S, L_)
 #undef L_
 }
@@ -1136,13 +1140,13 @@ void launch_build_g_split_lds(hipStream_t st, int S, double R, const GSplitJob *
 #undef L_
 }
 
-void launch_halfls_split(hipStream_t st, int S, double R, int N, int kpad_all, const int8_t *mud,
-                         long long muplane, const int *mue, const SplitHalfJob *d_jobs,
-                         const int *d_tbegin, int njobs, int ntiles, int cfg)
+void launch_halfls_split(hipStream_t st, int S, double R, int kpad_all, const int *mue,
+                         const SplitHalfJob *d_jobs, const int *d_tbegin, int njobs, int ntiles,
+                         int cfg)
 {
     if (njobs <= 0 || ntiles <= 0) return;
     if (cfg == 2 && S <= 5) {  // two register sets exceed the registers at 6 digits
-#define L_(s) hipLaunchKernelGGL(k_halfls_split2<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, N, kpad_all, mud, muplane, mue, d_jobs, d_tbegin, njobs, ntiles, R)
+#define L_(s) hipLaunchKernelGGL(k_halfls_split2<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, kpad_all, mue, d_jobs, d_tbegin, njobs, ntiles, R)
         switch (S) {
         case 3: L_(3); break;
         case 4: L_(4); break;
@@ -1152,12 +1156,12 @@ void launch_halfls_split(hipStream_t st, int S, double R, int N, int kpad_all, c
         return;
     }
     if (cfg == 1) {
-#define L_(s) hipLaunchKernelGGL(k_halfls_split_wide<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, N, kpad_all, mud, muplane, mue, d_jobs, d_tbegin, njobs, ntiles, R)
+#define L_(s) hipLaunchKernelGGL(k_halfls_split_wide<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, kpad_all, mue, d_jobs, d_tbegin, njobs, ntiles, R)
         GI_SPLIT_DISPATCH(S, L_)
 #undef L_
         return;
     }
-#define L_(s) hipLaunchKernelGGL(k_halfls_split<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, N, kpad_all, mud, muplane, mue, d_jobs, d_tbegin, njobs, ntiles, R)
+#define L_(s) hipLaunchKernelGGL(k_halfls_split<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, kpad_all, mue, d_jobs, d_tbegin, njobs, ntiles, R)
     GI_SPLIT_DISPATCH(S, L_)
 #undef L_
 }

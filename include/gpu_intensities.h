@@ -136,16 +136,20 @@ int gi_upload_dipole(gi_ctx *ctx, int nvib, const double *dipole, int on_device)
 
 /* Device bytes allowed for mu (0, the default: no limit), applied by the next
  * gi_upload_dipole. A larger mu stays in host memory: gi_prepare keeps its
- * permuted copy in pinned host memory, and GI_MODE_FP64 copies it to the device
- * in parts of floor(bytes / (24 * nvib)) vF rows, computing the half
- * linestrength rows of each part from it (the same sums in the same order as
- * from resident mu, so the same bits). Replaces the reference's split dipole
- * (max_dipole_size, cuda_host.cu:10; read_dipole_flipped_blocks,
- * trove_functions.cpp:207-318; do_1st_half_ls_blocks, cuda_host.cu:783-817),
- * which re-copies every part for every lower level; here each part is copied
- * once per batch of up to 256 lower levels. The matrix-core modes, sweeps,
- * their requests and gi_half_ls need mu resident (GI_ESTATE). GI_EINVAL from
- * gi_upload_dipole when the budget is below one row (24 * nvib bytes). */
+ * permuted copy in pinned host memory, and every mode copies it to the device
+ * in parts of floor(bytes / (24 * nvib)) vF rows (two parts on the device at
+ * a time, the next copied while the current one computes), computing the
+ * half linestrength rows of each part from it (the same sums in the same
+ * order as from resident mu, so the same bits): GI_MODE_FP64 once per batch
+ * of up to 256 lower levels, the matrix-core modes once per pipeline round.
+ * GI_MODE_SPLIT builds mu's digit planes part by part at gi_prepare_split and
+ * keeps them in pinned host memory, streaming their parts the same way.
+ * Replaces the reference's split dipole (max_dipole_size, cuda_host.cu:10;
+ * read_dipole_flipped_blocks, trove_functions.cpp:207-318;
+ * do_1st_half_ls_blocks, cuda_host.cu:783-817), which re-copies every part
+ * for every lower level. Sweeps and gi_half_ls need mu resident (GI_ESTATE).
+ * GI_EINVAL from gi_upload_dipole when the budget is below one row
+ * (24 * nvib bytes). */
 int gi_set_mu_budget(gi_ctx *ctx, long long bytes);
 /* Parts mu is streamed in (1: resident; 0: no mu uploaded). */
 int gi_mu_parts(const gi_ctx *ctx);

@@ -4,9 +4,11 @@ The reference splits mu into parts of vF rows when it exceeds max_dipole_size
 (cuda_host.cu:10; read_dipole_flipped_blocks, trove_functions.cpp:207-318) and
 computes each part's half-linestrength rows from it (do_1st_half_ls_blocks,
 cuda_host.cu:783-817). Here a mu larger than the device budget stays in pinned
-host memory and GI_MODE_FP64 copies it in parts of vF rows. Every output row
-is a whole sum over the same terms in the same order, so the lines are the
-bits of resident mu, and they match the FP64 oracle to its tolerance.
+host memory and every mode copies it in parts of vF rows: GI_MODE_FP64 and
+GI_MODE_FP64_BATCHED the permuted FP64 rows, GI_MODE_SPLIT their digit rows
+(built part by part, each row's digits depending on that row only). Every
+output row is a whole sum over the same terms in the same order, so the lines
+are the bits of resident mu, and they match the FP64 oracle to its tolerance.
 """
 import numpy as np
 import pytest
@@ -15,8 +17,11 @@ import torch
 from gpu_intensities_amd import binding, synth
 import oracle_py
 from test_gpu_parity import compare
+from test_gpu_split import split_error
 
 pytestmark = pytest.mark.gpu
+
+MODES = (binding.GI_MODE_FP64, binding.GI_MODE_FP64_BATCHED, binding.GI_MODE_SPLIT)
 
 
 def _copy(r):
@@ -43,24 +48,56 @@ def test_streamed_mu_same_bits(engine_resident, sym, kw):
     job = synth.make_job(sym, **kw)
     eng.load(job)
     assert eng.mu_parts() == 1
-    ref = _copy(eng.compute(mode=binding.GI_MODE_FP64))
-    compare(ref, oracle_py.OracleJob(job).run())
+    ref = {m: _copy(eng.compute(mode=m)) for m in MODES}
+    oref = oracle_py.OracleJob(job).run()
+    compare(ref[binding.GI_MODE_FP64], oref)
+    compare(ref[binding.GI_MODE_FP64_BATCHED], oref)
     row = 24 * job.nvib  # bytes of one vF row of mu (3 components)
     for rows in (job.nvib // 3 + 1, 7, 1):
+        print(f"[mu_stream] {sym} parts of {rows} rows", flush=True)
         eng.load(job, mu_budget=rows * row + row // 2)
         assert eng.mu_parts() == -(-job.nvib // rows), rows
-        got = eng.compute(mode=binding.GI_MODE_FP64)
-        _same(ref, got, rows)
-        ks = eng.kernel_stats()
-        assert ks["mu_copy"]["launches"] >= 1
+        for m in MODES:
+            _same(ref[m], eng.compute(mode=m), (rows, m))
+            ks = eng.kernel_stats()
+            assert ks["mu_copy"]["launches"] >= eng.mu_parts(), (rows, m)
     # mu handed over on the device is copied to the host when streamed
     dev = torch.from_numpy(np.ascontiguousarray(job.dipole_flipped)).cuda()
     eng.load(job, device_dipole=dev, mu_budget=5 * row)
     assert eng.mu_parts() == -(-job.nvib // 5)
-    _same(ref, eng.compute(mode=binding.GI_MODE_FP64), "device mu")
+    for m in MODES:
+        _same(ref[m], eng.compute(mode=m), ("device mu", m))
     eng.load(job)  # resident again
     assert eng.mu_parts() == 1
-    _same(ref, eng.compute(mode=binding.GI_MODE_FP64), "resident again")
+    for m in MODES:
+        _same(ref[m], eng.compute(mode=m), ("resident again", m))
+
+
+def test_streamed_mu_every_tile_and_route(engine_resident):
+    """The part loop under every half-ls tile shape of both matrix-core modes,
+    with several pipeline rounds, and with the eigenvector blocks streamed as
+    well (FP64 batched): the bits of resident mu."""
+    eng = engine_resident
+    job = synth.make_job("C2V", nvib=120, jvals=(2, 3), seed=45, e_span=12000.0)
+    eng.load(job)
+    B, SP = binding.GI_MODE_FP64_BATCHED, binding.GI_MODE_SPLIT
+    ref = {m: _copy(eng.compute(mode=m)) for m in (B, SP)}
+    assert split_error(ref[SP], oracle_py.OracleJob(job).run())["max"] <= 1e-6
+    eng.load(job, mu_budget=17 * 24 * job.nvib)
+    assert eng.mu_parts() == -(-job.nvib // 17)
+    for hl in range(6):
+        print(f"[mu_stream] half-ls tile {hl}", flush=True)
+        eng.set_tuning(hl, 0, 3, 2, 50)
+        _same(ref[B], eng.compute(mode=B), ("tile", hl))
+    for sc in range(3):
+        print(f"[mu_stream] split half-ls tile {sc}", flush=True)
+        eng.set_tuning(4, 0, 3, sc, 50)
+        _same(ref[SP], eng.compute(mode=SP), ("split tile", sc))
+    eng.set_tuning(4, 0, 4, 2, 50)
+    eng.load(job, mu_budget=17 * 24 * job.nvib, vector_budget=1)
+    assert eng.streamed_blocks() > 0
+    _same(ref[B], eng.compute(mode=B), "blocks and mu streamed")
+    eng.load(job)
 
 
 def test_streamed_mu_rules(engine_resident):
@@ -72,37 +109,34 @@ def test_streamed_mu_rules(engine_resident):
     assert e.value.status == 1
     eng.load(job, mu_budget=4 * row)
     assert eng.mu_parts() == 10
-    for mode in (binding.GI_MODE_FP64_BATCHED, binding.GI_MODE_SPLIT):
-        with pytest.raises(binding.GIError) as e:
-            eng.compute(mode=mode)
-        assert e.value.status == 4
-    with pytest.raises(binding.GIError):
+    with pytest.raises(binding.GIError) as e:  # a captured sweep does not stream parts
         eng.sweep_create(binding.GI_MODE_FP64_BATCHED)
+    assert e.value.status == 4
     ref = oracle_py.OracleJob(job).run()
     compare(eng.compute(mode=binding.GI_MODE_FP64), ref)
+    compare(eng.compute(mode=binding.GI_MODE_FP64_BATCHED), ref)
+    assert split_error(eng.compute(mode=binding.GI_MODE_SPLIT), ref)["max"] <= 1e-6
     eng.load(job)
     assert eng.mu_parts() == 1
 
 
 def test_streamed_mu_requests(engine_resident):
-    """FP64 requests on the request server stream mu the same way: the lines
-    of a request equal those with resident mu; matrix-core requests are
-    refused while mu is streamed."""
+    """Requests on the request server stream mu the same way in every mode:
+    the lines of a request equal those with resident mu."""
     eng = engine_resident
     job = synth.make_job("C2V", nvib=48, jvals=(1, 2), seed=44, e_span=9000.0)
     req = dict(jindI=1, igammaI=0, jindF=0, temperature=800.0, q_stat=300.0)
     out = {}
     for budget in (0, 6 * 24 * job.nvib):
         eng.load(job, mu_budget=budget)
+        eng.prepare_split()
         eng.serve_start(1 << 28)
         try:
-            out[budget] = _copy(eng.serve_fetch(eng.serve_submit(mode=binding.GI_MODE_FP64,
-                                                                 **req)))
-            if budget:
-                with pytest.raises(binding.GIError):
-                    eng.serve_submit(mode=binding.GI_MODE_FP64_BATCHED, **req)
+            for m in MODES:
+                out[budget, m] = _copy(eng.serve_fetch(eng.serve_submit(mode=m, **req)))
         finally:
             eng.serve_stop()
-    assert len(out[0]["S"]) > 0
-    _same(out[0], out[6 * 24 * job.nvib], "request")
+    for m in MODES:
+        assert len(out[0, m]["S"]) > 0
+        _same(out[0, m], out[6 * 24 * job.nvib, m], ("request", m))
     eng.load(job)
