@@ -15,7 +15,7 @@ the timed region starts.
 `value` is the fastest FP64 mode (GI_MODE_FP64_BATCHED: FP64 matrix-core
 products, the reference's arithmetic); the plain FP64 GEMV mode and the split
 mode (int8 digit products, error reported against the FP64 oracle) are
-reported beside it. After the headline workload, GI_MODE_FP64 on it again
+reported beside it. After the headline workload, every mode on it again
 with mu held in host memory and streamed in parts (`mu_streamed`), then the
 same measurement on
 one context holding C2v J = 0..20 (Nvib = 2000: 84 blocks of 500..20,500
@@ -409,41 +409,57 @@ def sample_parity(eng, binding, ref, sample, bufs, mode_names):
     return out
 
 
-def mu_stream_section(eng, binding, job, dev, bufs, parts, steps):
-    """GI_MODE_FP64 with mu kept in host memory and copied to the device in
-    `parts` parts of vF rows for every batch of lower levels (gi_set_mu_budget,
-    the reference's split dipole); the lines must be the bits of resident mu."""
-    mode = binding.GI_MODE_FP64
-    ref = {k: v.copy() for k, v in eng.compute(bufs=bufs, mode=mode).items()}
+def mu_stream_section(eng, binding, job, dev, bufs, parts, steps, mode_names, saved=None):
+    """Every mode with mu kept in host memory and copied to the device in
+    `parts` parts of vF rows (gi_set_mu_budget, the reference's split dipole):
+    GI_MODE_FP64 per batch of lower levels, the matrix-core modes per pipeline
+    round (split mode: mu's digit rows). The lines must be the bits of
+    resident mu."""
     row = 24 * job.nvib
-    budget = -(-job.nvib // parts) * row
+    np_rows = -(-job.nvib // parts)
+    budget = np_rows * row
+    refs = {}
+    for mname in mode_names:
+        mode = binding.MODES[mname]
+        if saved and mname in saved and mode != binding.GI_MODE_FP64:
+            eng.set_tuning(*saved[mname])
+        refs[mname] = {k: v.copy() for k, v in eng.compute(bufs=bufs, mode=mode).items()}
     eng.load(job, device_vectors=dev, mu_budget=budget)
+    out = {"parts": eng.mu_parts(), "budget_bytes": budget}
     try:
-        got = eng.compute(bufs=bufs, mode=mode)
-        same = all(np.array_equal(ref[k], got[k]) for k in ref)
-        ts = []
-        for _ in range(steps):
-            t = time.perf_counter()
-            eng.compute(bufs=bufs, mode=mode)
-            ts.append(time.perf_counter() - t)
-        ks = eng.kernel_stats()
-        ms = 1e3 * sum(ts) / len(ts)
-        cp = ks.get("mu_copy", {"total_ms": 0.0, "launches": 0})
-        np_rows = -(-job.nvib // parts)
-        part_bytes = 3 * job.nvib * np_rows * 8
-        out = dict(parts=eng.mu_parts(), budget_bytes=budget, part_bytes=part_bytes,
-                   ms_per_step=ms, value=len(ref["S"]) / (ms / 1e3), transitions=len(ref["S"]),
-                   bitwise_equal_resident=bool(same), mu_copies=int(cp["launches"]),
-                   mu_copy_ms=cp["total_ms"],
-                   mu_copy_GBps=(cp["launches"] * part_bytes / (cp["total_ms"] / 1e3) / 1e9
-                                 if cp["total_ms"] > 0 else None),
-                   kernels={k: {"total_ms": round(v["total_ms"], 4), "launches": v["launches"]}
-                            for k, v in ks.items()})
-        log(f"mu streamed in {parts} parts: {ms:.2f} ms/step, bitwise equal {same}, "
-            f"{out['mu_copies']} part copies, {out['mu_copy_GBps']} GB/s")
+        for mname in mode_names:
+            mode = binding.MODES[mname]
+            if saved and mname in saved and mode != binding.GI_MODE_FP64:
+                eng.set_tuning(*saved[mname])
+            ref = refs[mname]
+            got = eng.compute(bufs=bufs, mode=mode)
+            same = all(np.array_equal(ref[k], got[k]) for k in ref)
+            n = steps if mode == binding.GI_MODE_FP64 else 3 * steps
+            ts = []
+            for _ in range(n):
+                t = time.perf_counter()
+                eng.compute(bufs=bufs, mode=mode)
+                ts.append(time.perf_counter() - t)
+            ks = eng.kernel_stats()
+            ms = 1e3 * sum(ts) / len(ts)
+            cp = ks.get("mu_copy", {"total_ms": 0.0, "launches": 0})
+            # bytes of one FP64 part (the split mode copies padded digit rows)
+            part_bytes = 3 * job.nvib * np_rows * 8 if mode != binding.GI_MODE_SPLIT else None
+            out[mname] = dict(
+                ms_per_step=ms, value=len(ref["S"]) / (ms / 1e3), transitions=len(ref["S"]),
+                bitwise_equal_resident=bool(same), mu_copies=int(cp["launches"]),
+                mu_copy_ms=cp["total_ms"],
+                mu_copy_GBps=(cp["launches"] * part_bytes / (cp["total_ms"] / 1e3) / 1e9
+                              if part_bytes and cp["total_ms"] > 0 else None),
+                kernels={k: {"total_ms": round(v["total_ms"], 4), "launches": v["launches"]}
+                         for k, v in ks.items()})
+            log(f"mu streamed in {out['parts']} parts, {mname}: {ms:.2f} ms/step, bitwise "
+                f"equal {same}, {out[mname]['mu_copies']} part copies")
         return out
     finally:
         eng.load(job, device_vectors=dev)
+        if "split" in mode_names:
+            eng.prepare_split()
 
 
 def vec_stream_section(binding, job, bufs, ref_lines, steps):
@@ -654,6 +670,12 @@ def compact_line(out):
     if par:
         line["parity_max_rel_S"] = {m: p.get("max_rel_S") for m, p in par.items()
                                     if isinstance(p, dict) and "max_rel_S" in p}
+    mu = out.get("mu_streamed")
+    if mu:
+        line["mu_in_host_memory"] = {"parts": mu.get("parts")}
+        line["mu_in_host_memory"].update(
+            {m: [round(v["ms_per_step"], 3), v["bitwise_equal_resident"]]
+             for m, v in mu.items() if isinstance(v, dict)})
     vs = out.get("vec_streamed")
     if vs:
         line["blocks_in_host_memory"] = {"ms_per_step": round(vs["ms_per_step"], 3),
@@ -845,8 +867,9 @@ def main():
             vec_streamed = vec_stream_section(binding, job, bufs, resident, 3)
         job.vectors.clear()
     mu_streamed = None
-    if "fp64" in mode_names and args.mu_parts > 1:
-        mu_streamed = mu_stream_section(eng, binding, job, dev, bufs, args.mu_parts, 2)
+    if args.mu_parts > 1:
+        mu_streamed = mu_stream_section(eng, binding, job, dev, bufs, args.mu_parts, 2,
+                                        mode_names, saved)
     eng.close()
     del dev, bufs
     torch.cuda.empty_cache()
@@ -865,7 +888,8 @@ def main():
 
     ok = (parity_ok(parity) and parity_ok((multi_j or {}).get("parity"))
           and parity_ok((c3v or {}).get("parity"))
-          and (mu_streamed or {}).get("bitwise_equal_resident", True)
+          and all(v.get("bitwise_equal_resident", True) for v in (mu_streamed or {}).values()
+                  if isinstance(v, dict))
           and (vec_streamed or {}).get("bitwise_equal_resident", True))
     out = {
         "metric": METRIC, "value": value, "unit": "transitions/s", "n_gpus": 1,

@@ -311,8 +311,8 @@ __global__ __launch_bounds__(256) void k_build_g_map4(const GMapJob *__restrict_
 // blocks).
 template <int BM_, int BN_, int NW_ = 4>
 __global__ __launch_bounds__(64 * NW_) __attribute__((amdgpu_waves_per_eu(NW_ == 8 ? 4 : 1)))
-void k_halfls_jobs(const HalfJob *__restrict__ jobs, const int *__restrict__ tbegin, int njobs,
-                   int ntiles)
+void k_halfls_jobs(MuSrc src, const HalfJob *__restrict__ jobs, const int *__restrict__ tbegin,
+                   int njobs, int ntiles)
 {
     using T = TileW<BM_, BN_, NW_>;
     __shared__ double As[BK * T::PA];
@@ -322,9 +322,9 @@ void k_halfls_jobs(const HalfJob *__restrict__ jobs, const int *__restrict__ tbe
     const HalfJob &J = jobs[find_job(tbegin, njobs, tile)];
     const ClassDev &cd = J.cd;
     const int local = tile - J.tile_begin;
-    const int m0 = J.mr.m_lo + (local % J.mt) * BM_, n0 = (local / J.mt) * BN_;
-    const int mhi = J.mr.m_hi, ld = J.mr.ld;
-    const double *__restrict__ mu = (const double *)J.mr.mu;
+    const int m0 = J.m_lo + (local % J.mt) * BM_, n0 = (local / J.mt) * BN_;
+    const int mhi = J.m_hi, ld = src.ld;
+    const double *__restrict__ mu = (const double *)src.mu;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * (BM_ / 2), wn = (wave >> 1) * (BN_ / T::WC);
     const int ncolsTot = J.nunits * cd.ncols;
@@ -348,8 +348,8 @@ void k_halfls_jobs(const HalfJob *__restrict__ jobs, const int *__restrict__ tbe
     int e = 0, kk0 = 0;
     auto fetch = [&](int se, int sk0) {
         const int len = min(BK, cd.seg_n[se] - sk0);
-        const double *mp = mu + (size_t)cd.seg_a[se] * J.mr.plane +
-                           (size_t)(cd.seg_off[se] + sk0) * ld + (cd.off + m0 + am - J.mr.v0);
+        const double *mp = mu + (size_t)cd.seg_a[se] * src.plane +
+                           (size_t)(cd.seg_off[se] + sk0) * ld + (cd.off + m0 + am - src.v0);
         const double *gp = gcol + (size_t)(cd.seg_row0[se] + sk0) * cd.ncols;
 #pragma unroll
         for (int q = 0; q < T::LA; q++) {
@@ -660,21 +660,21 @@ void launch_build_g_map(hipStream_t st, const GMapJob *d_jobs, const int *d_begi
                        njobs, ntiles);
 }
 
-void launch_halfls_jobs(hipStream_t st, int cfg, const HalfJob *d_jobs, const int *d_tbegin,
-                        int njobs, int ttiles)
+void launch_halfls_jobs(hipStream_t st, int cfg, const MuSrc &src, const HalfJob *d_jobs,
+                        const int *d_tbegin, int njobs, int ttiles)
 {
     if (njobs <= 0 || ttiles <= 0) return;
     const dim3 g(xcd_grid(ttiles)), b(256);
     if (cfg == kTile128x128w8) {
-        hipLaunchKernelGGL((k_halfls_jobs<128, 128, 8>), g, dim3(512), 0, st, d_jobs, d_tbegin,
-                           njobs, ttiles);
+        hipLaunchKernelGGL((k_halfls_jobs<128, 128, 8>), g, dim3(512), 0, st, src, d_jobs,
+                           d_tbegin, njobs, ttiles);
         return;
     }
     switch (cfg & 3) {
-    case 0: hipLaunchKernelGGL((k_halfls_jobs<64, 64>), g, b, 0, st, d_jobs, d_tbegin, njobs, ttiles); break;
-    case 1: hipLaunchKernelGGL((k_halfls_jobs<64, 128>), g, b, 0, st, d_jobs, d_tbegin, njobs, ttiles); break;
-    case 2: hipLaunchKernelGGL((k_halfls_jobs<128, 64>), g, b, 0, st, d_jobs, d_tbegin, njobs, ttiles); break;
-    default: hipLaunchKernelGGL((k_halfls_jobs<128, 128>), g, b, 0, st, d_jobs, d_tbegin, njobs, ttiles); break;
+    case 0: hipLaunchKernelGGL((k_halfls_jobs<64, 64>), g, b, 0, st, src, d_jobs, d_tbegin, njobs, ttiles); break;
+    case 1: hipLaunchKernelGGL((k_halfls_jobs<64, 128>), g, b, 0, st, src, d_jobs, d_tbegin, njobs, ttiles); break;
+    case 2: hipLaunchKernelGGL((k_halfls_jobs<128, 64>), g, b, 0, st, src, d_jobs, d_tbegin, njobs, ttiles); break;
+    default: hipLaunchKernelGGL((k_halfls_jobs<128, 128>), g, b, 0, st, src, d_jobs, d_tbegin, njobs, ttiles); break;
     }
 }
 

@@ -505,8 +505,11 @@ static std::vector<std::pair<size_t, size_t>> plan_rounds(const gi_ctx *c, const
     }
     size_t left = 0;
     for (size_t b : bytes) left += b;
+    // streamed mu is copied once per round: its rounds are only those the
+    // device budget needs (at J 10<->11, four rounds re-copy 3 x 96 MB of mu
+    // to hide 73 MB of line copies)
     auto budget = [&](size_t r) {
-        if (!P.pipe || (int)r + 1 >= c->pipe_rounds) return kRoundBytes;
+        if (!P.pipe || c->mu_streamed || (int)r + 1 >= c->pipe_rounds) return kRoundBytes;
         return std::min(kRoundBytes, std::max(left / 100 * c->pipe_share, (size_t)1 << 20));
     };
     std::vector<std::pair<size_t, size_t>> rounds;
@@ -809,10 +812,10 @@ static int mu_part_copy(gi_ctx *c, Slot &S, MuParts &M, const void *src, size_t 
 }
 
 // The half-ls jobs of one mu part: each job cut to its class rows inside
-// vF rows [v0, v0 + np) and reading mu from `mr` (its row range set here),
-// tiles renumbered from 0 (TM x TN tiles). Returns the tile count.
+// vF rows [v0, v0 + np), tiles renumbered from 0 (TM x TN tiles). Returns
+// the tile count.
 template <class Job>
-static int part_jobs(const std::vector<Job> &all, int v0, int np, MuRows mr, int TM, int TN,
+static int part_jobs(const std::vector<Job> &all, int v0, int np, int TM, int TN,
                      std::vector<Job> &out, std::vector<int> &begin)
 {
     int tiles = 0;
@@ -820,9 +823,8 @@ static int part_jobs(const std::vector<Job> &all, int v0, int np, MuRows mr, int
         const int lo = std::max(0, v0 - j0.cd.off), hi = std::min(j0.cd.n, v0 + np - j0.cd.off);
         if (lo >= hi) continue;
         Job j = j0;
-        j.mr = mr;
-        j.mr.m_lo = lo;
-        j.mr.m_hi = hi;
+        j.m_lo = lo;
+        j.m_hi = hi;
         j.mt = (hi - lo + TM - 1) / TM;
         j.tile_begin = tiles;
         tiles += j.mt * ((j.nunits * j.cd.ncols + TN - 1) / TN);
@@ -1057,7 +1059,8 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                     j.uideg = (const int *)(d_meta + ch.off_uideg);
                     j.r0 = r0; j.r1 = r1;
                     j.G = G + og; j.H = H + oh;
-                    j.mr = MuRows{c->d_mu_perm, (long long)N * N, N, 0, 0, oc.n};
+                    j.m_lo = 0;
+                    j.m_hi = oc.n;
                     jobs.push_back(j);
                     tbegin.push_back(j.tile_begin);
                 }
@@ -1108,7 +1111,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         }
         // the half-ls products: one launch over resident mu, or one per mu part
         auto halfls = [&](const std::vector<HalfJob> &hj, const std::vector<int> &hb0,
-                          int tiles) -> int {
+                          int tiles, const MuSrc &src) -> int {
             unsigned char *hb = stage(c, S, nullptr, jobs_bytes + 4 * (size_t)mjobs, &rc);
             if (rc) return rc;
             std::memcpy(hb, hj.data(), hj.size() * sizeof(HalfJob));
@@ -1117,25 +1120,26 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                                   hipMemcpyHostToDevice, S.stream));
             TL_BEGIN("half_ls_mfma");
             P.work[14] += tiles;
-            launch_halfls_jobs(S.stream, hcfg, (const HalfJob *)d_jobs,
+            launch_halfls_jobs(S.stream, hcfg, src, (const HalfJob *)d_jobs,
                                (const int *)(d_jobs + jobs_bytes), (int)hj.size(), tiles);
             TL_END();
             return GI_OK;
         };
-        if (njobs > 0 && !c->mu_streamed && (rc = halfls(jobs, tbegin, tt)) != GI_OK) return rc;
+        if (njobs > 0 && !c->mu_streamed &&
+            (rc = halfls(jobs, tbegin, tt, MuSrc{c->d_mu_perm, (long long)N * N, N, 0})) != GI_OK)
+            return rc;
         for (int v0 = 0; njobs > 0 && c->mu_streamed && v0 < N; v0 += c->mu_np) {
             const int np = std::min(c->mu_np, N - v0);
             std::vector<HalfJob> pj;
             std::vector<int> pb;
-            const int pt = part_jobs(jobs, v0, np, MuRows{nullptr, (long long)N * np, np, v0, 0, 0},
-                                     TM, TN, pj, pb);
+            const int pt = part_jobs(jobs, v0, np, TM, TN, pj, pb);
             if (pt == 0) continue;
             int b = 0;
             if ((rc = mu_part_copy(c, S, MP, c->h_mu_perm + v0, (size_t)N * 8, (size_t)np * 8,
                                    (size_t)3 * N, &b, tl)) != GI_OK)
                 return rc;
-            for (HalfJob &j : pj) j.mr.mu = MP.buf[b];
-            if ((rc = halfls(pj, pb, pt)) != GI_OK) return rc;
+            if ((rc = halfls(pj, pb, pt, MuSrc{MP.buf[b], (long long)N * np, np, v0})) != GI_OK)
+                return rc;
             HIPCHK(hipEventRecord(S.mu_ev[2 + b], S.stream));
         }
         std::vector<FoldJob> fjobs;
@@ -1412,7 +1416,8 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                     h.gplane = (long long)mgd;
                     h.Ge = Ge + oge;
                     h.H = H + oh;
-                    h.mr = MuRows{c->d_mud, c->muplane, N, 0, 0, oc.n};
+                    h.m_lo = 0;
+                    h.m_hi = oc.n;
                     hj.push_back(h);
                     tbegin.push_back(h.tile_begin);
                 }
@@ -1450,32 +1455,34 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
         // the half-ls products: one launch over resident mu digits, or one per part
         const int scfg = c->tile_split == 2 && !steps_fit ? 0 : c->tile_split;
         auto halfls = [&](const std::vector<SplitHalfJob> &jobs, const std::vector<int> &beg,
-                          int tiles) -> int {
+                          int tiles, const MuSrc &src) -> int {
             int e = send(d_hj, jobs.data(), jobs.size() * sizeof(SplitHalfJob), beg, hj_bytes);
             if (e != GI_OK) return e;
             TL_BEGIN("half_ls_split");
             P.work[14] += tiles;
-            launch_halfls_split(S.stream, SH, c->split_radix, c->kpad_all, c->d_mue,
+            launch_halfls_split(S.stream, SH, c->split_radix, c->kpad_all, src, c->d_mue,
                                 (const SplitHalfJob *)d_hj, (const int *)(d_hj + hj_bytes),
                                 (int)jobs.size(), tiles, scfg);
             TL_END();
             return GI_OK;
         };
-        if (!hj.empty() && !c->mu_streamed && (rc = halfls(hj, tbegin, tt)) != GI_OK) return rc;
+        if (!hj.empty() && !c->mu_streamed &&
+            (rc = halfls(hj, tbegin, tt, MuSrc{c->d_mud, c->muplane, N, 0})) != GI_OK)
+            return rc;
         for (int v0 = 0; !hj.empty() && c->mu_streamed && v0 < N; v0 += c->mu_np) {
             const int np = std::min(c->mu_np, N - v0);
             const size_t kp = (size_t)c->kpad_all;
             std::vector<SplitHalfJob> pj;
             std::vector<int> pb;
-            const int pt = part_jobs(hj, v0, np, MuRows{nullptr, 3LL * np * (long long)kp, np, v0, 0, 0},
-                                     srows_tile, 64, pj, pb);
+            const int pt = part_jobs(hj, v0, np, srows_tile, 64, pj, pb);
             if (pt == 0) continue;
             int b = 0;
             if ((rc = mu_part_copy(c, S, MP, c->h_mud + v0 * kp, (size_t)N * kp, (size_t)np * kp,
                                    (size_t)3 * SH, &b, tl)) != GI_OK)
                 return rc;
-            for (SplitHalfJob &j : pj) j.mr.mu = MP.buf[b];
-            if ((rc = halfls(pj, pb, pt)) != GI_OK) return rc;
+            if ((rc = halfls(pj, pb, pt, MuSrc{MP.buf[b], 3LL * np * (long long)kp, np, v0})) !=
+                GI_OK)
+                return rc;
             HIPCHK(hipEventRecord(S.mu_ev[2 + b], S.stream));
         }
         std::vector<FoldJob> fjobs;

@@ -86,22 +86,22 @@ struct IntensArgs {
     double *out_nu, *out_S, *out_A, *out_I;
 };
 
-// Where a half-ls job reads mu and which rows of its output class it
-// computes: rows [m_lo, m_hi); element (a, vI, vF) of the permuted FP64 mu at
-// mu + a*plane + vI*ld + (vF - v0), digit row (a, vF) of the split mode at
-// (a*ld + vF - v0)*kpad_all in planes `plane` bytes apart. Resident mu: the
-// whole class, ld = N, v0 = 0. Streamed mu: the class rows inside one part
-// of ld vF rows starting at v0 (each row's sum is the same either way).
-struct MuRows {
+// Where one half-ls launch reads mu: element (a, vI, vF) of the permuted
+// FP64 mu at mu + a*plane + vI*ld + (vF - v0), digit row (a, vF) of the split
+// mode at (a*ld + vF - v0)*kpad_all in planes `plane` bytes apart. Resident
+// mu: ld = N, v0 = 0. Streamed mu: one part of ld vF rows starting at v0,
+// whose jobs compute the class rows [m_lo, m_hi) inside it (each row's sum is
+// the same either way). A kernel argument, so it stays in scalar registers.
+struct MuSrc {
     const void *mu;
     long long plane;
-    int ld, v0, m_lo, m_hi;
+    int ld, v0;
 };
 
 // One (chunk, output class) of the batched half linestrength.
 struct HalfJob {
     ClassDev cd;
-    MuRows mr;
+    int m_lo, m_hi;                  // rows of the class computed (see MuSrc)
     int nunits, Gtot, Htot;
     int tile_begin;                  // first tile of the product
     int mt;                          // product tiles along rows
@@ -173,7 +173,7 @@ struct GSplitJob {
 };
 struct SplitHalfJob {
     ClassDev cd;
-    MuRows mr;                       // mu digit rows (int8, plane 0 at mr.mu)
+    int m_lo, m_hi;                  // rows of the class computed (see MuSrc)
     int nunits, tile_begin, mt, GtotB, EGtot, Htot;
     const int8_t *Gd;
     long long gplane;
@@ -234,9 +234,9 @@ __host__ __device__ inline bool split2_class_fits(const int *seg_n, int nseg)
 {
     return nseg <= kSplitMaxSegs && split2_steps(seg_n, nseg) <= kSplitMaxSteps;
 }
-void launch_halfls_split(hipStream_t st, int S, double R, int kpad_all, const int *mue,
-                         const SplitHalfJob *d_jobs, const int *d_tbegin, int njobs, int ntiles,
-                         int cfg);
+void launch_halfls_split(hipStream_t st, int S, double R, int kpad_all, const MuSrc &src,
+                         const int *mue, const SplitHalfJob *d_jobs, const int *d_tbegin,
+                         int njobs, int ntiles, int cfg);
 void launch_corr_split(hipStream_t st, int S, double R, const CorrSplitArgs *d_args, const int *d_cbegin,
                        int nchunks, int ntiles);
 
@@ -246,8 +246,8 @@ void launch_corr_split(hipStream_t st, int S, double R, const CorrSplitArgs *d_a
 constexpr int kTile128x128w8 = 5;
 int tile_rows(int cfg);
 int tile_cols(int cfg);
-void launch_halfls_jobs(hipStream_t st, int cfg, const HalfJob *d_jobs, const int *d_tbegin,
-                        int njobs, int ttiles);
+void launch_halfls_jobs(hipStream_t st, int cfg, const MuSrc &src, const HalfJob *d_jobs,
+                        const int *d_tbegin, int njobs, int ttiles);
 void launch_corr_mfma(hipStream_t st, int cfg, const CorrMMArgs *d_args, const int *d_cbegin,
                       int nchunks, int ntiles);
 void launch_intensity(hipStream_t st, const IntensArgs &a);

@@ -587,7 +587,8 @@ __device__ inline double oz_value(const v4i (&acc)[S], int r, double rinv)
 // Split half linestrength: H_co[m, (u, j)] = sum over segments of mu digits x
 // G digits, per (chunk, class) job; m-tiles fastest.
 template <int S>
-__global__ __launch_bounds__(256) void k_halfls_split(int kpad_all, const int *__restrict__ mue,
+__global__ __launch_bounds__(256) void k_halfls_split(int kpad_all, MuSrc src,
+                                                      const int *__restrict__ mue,
                                                       const SplitHalfJob *__restrict__ jobs,
                                                       const int *__restrict__ tbegin, int njobs,
                                                       int ntiles, double R)
@@ -599,10 +600,10 @@ __global__ __launch_bounds__(256) void k_halfls_split(int kpad_all, const int *_
     const SplitHalfJob &J = jobs[find_tile_job(tbegin, njobs, tile)];
     const ClassDev &cd = J.cd;
     const int local = tile - J.tile_begin;
-    const int m0 = J.mr.m_lo + (local % J.mt) * 64, n0 = (local / J.mt) * 64;
-    const int mhi = J.mr.m_hi;
-    const int8_t *__restrict__ mud = (const int8_t *)J.mr.mu;
-    const long long muplane = J.mr.plane;
+    const int m0 = J.m_lo + (local % J.mt) * 64, n0 = (local / J.mt) * 64;
+    const int mhi = J.m_hi;
+    const int8_t *__restrict__ mud = (const int8_t *)src.mu;
+    const long long muplane = src.plane;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
     const int ncolsTot = J.nunits * cd.ncols;
@@ -626,7 +627,7 @@ __global__ __launch_bounds__(256) void k_halfls_split(int kpad_all, const int *_
     v4i ra[S], rb[S];
     auto fetch = [&](int se, int kk) {
         const long long arow =
-            ((long long)cd.seg_a[se] * J.mr.ld + cd.off + m0 + lr - J.mr.v0) * kpad_all +
+            ((long long)cd.seg_a[se] * src.ld + cd.off + m0 + lr - src.v0) * kpad_all +
             cd.seg_pad[2 * se + 1] + kk + lp;
         const long long bofs = bcol + cd.seg_pad[2 * se] + kk + lp;
 #pragma unroll
@@ -678,7 +679,7 @@ __global__ __launch_bounds__(256) void k_halfls_split(int kpad_all, const int *_
 // k_halfls_split.
 template <int S>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_halfls_split2(
-    int kpad_all, const int *__restrict__ mue, const SplitHalfJob *__restrict__ jobs,
+    int kpad_all, MuSrc src, const int *__restrict__ mue, const SplitHalfJob *__restrict__ jobs,
     const int *__restrict__ tbegin, int njobs, int ntiles, double R)
 {
     __shared__ __attribute__((aligned(16))) int8_t As[S * 64 * kRowB];
@@ -691,10 +692,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const SplitHalfJob &J = jobs[find_tile_job(tbegin, njobs, tile)];
     const ClassDev &cd = J.cd;
     const int local = tile - J.tile_begin;
-    const int m0 = J.mr.m_lo + (local % J.mt) * 64, n0 = (local / J.mt) * 64;
-    const int mhi = J.mr.m_hi;
-    const int8_t *__restrict__ mud = (const int8_t *)J.mr.mu;
-    const long long muplane = J.mr.plane;
+    const int m0 = J.m_lo + (local % J.mt) * 64, n0 = (local / J.mt) * 64;
+    const int8_t *__restrict__ mud = (const int8_t *)src.mu;
+    const long long muplane = src.plane;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
     const int ncolsTot = J.nunits * cd.ncols;
@@ -715,7 +715,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if (!s_fits) {
         for (int i = tid; i < 64 * 64; i += 256) {
             const int m = m0 + (i & 63), nn = n0 + (i >> 6);
-            if (m >= mhi || nn >= ncolsTot) continue;
+            if (m >= J.m_hi || nn >= ncolsTot) continue;
             const int u = nn / cd.ncols, jj = nn - u * cd.ncols;
             J.H[(size_t)u * J.Htot + cd.hbase + (size_t)jj * cd.n + m] = __builtin_nan("");
         }
@@ -725,13 +725,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     for (int e = 0; e < cd.nseg; e++)
         for (int st = s_first[e] + tid; st < s_first[e + 1]; st += 256) {
             const int kk = (st - s_first[e]) * 64;
-            s_a[st] = ((long long)cd.seg_a[e] * J.mr.ld + cd.off + m0 - J.mr.v0) * kpad_all +
+            s_a[st] = ((long long)cd.seg_a[e] * src.ld + cd.off + m0 - src.v0) * kpad_all +
                       cd.seg_pad[2 * e + 1] + kk;
             s_b[st] = cd.seg_pad[2 * e] + kk;
         }
     __syncthreads();
     const int lr = tid >> 2, lp = (tid & 3) * 16;
-    const bool mok = m0 + lr < mhi;
+    const bool mok = m0 + lr < J.m_hi;
     const int n = n0 + lr;
     const bool nok = n < ncolsTot;
     long long bcol = 0;
@@ -790,7 +790,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             for (int r = 0; r < 4; r++) {
                 const int m = m0 + wm + 16 * x + 4 * (lane >> 4) + r;
                 const int nn = n0 + wn + 16 * y + (lane & 15);
-                if (m >= mhi || nn >= ncolsTot) continue;
+                if (m >= J.m_hi || nn >= ncolsTot) continue;
                 const int u = nn / cd.ncols, jj = nn - u * cd.ncols;
                 const int ex = mue[cd.off + m] + J.Ge[(size_t)u * J.EGtot + cd.eoff + jj];
                 J.H[(size_t)u * J.Htot + cd.hbase + (size_t)jj * cd.n + m] =
@@ -834,7 +834,7 @@ __device__ inline void oz_step_wide(const int8_t *__restrict__ As,
 // so the result is bitwise that of the 64-row kernel.
 template <int S>
 __global__ __launch_bounds__(256, 1) void k_halfls_split_wide(
-    int kpad_all, const int *__restrict__ mue, const SplitHalfJob *__restrict__ jobs,
+    int kpad_all, MuSrc src, const int *__restrict__ mue, const SplitHalfJob *__restrict__ jobs,
     const int *__restrict__ tbegin, int njobs, int ntiles, double R)
 {
     __shared__ __attribute__((aligned(16))) int8_t As[S * 128 * kRowB];
@@ -844,10 +844,10 @@ __global__ __launch_bounds__(256, 1) void k_halfls_split_wide(
     const SplitHalfJob &J = jobs[find_tile_job(tbegin, njobs, tile)];
     const ClassDev &cd = J.cd;
     const int local = tile - J.tile_begin;
-    const int m0 = J.mr.m_lo + (local % J.mt) * 128, n0 = (local / J.mt) * 64;
-    const int mhi = J.mr.m_hi;
-    const int8_t *__restrict__ mud = (const int8_t *)J.mr.mu;
-    const long long muplane = J.mr.plane;
+    const int m0 = J.m_lo + (local % J.mt) * 128, n0 = (local / J.mt) * 64;
+    const int mhi = J.m_hi;
+    const int8_t *__restrict__ mud = (const int8_t *)src.mu;
+    const long long muplane = src.plane;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 64, wn = (wave >> 1) * 32;
     const int ncolsTot = J.nunits * cd.ncols;
@@ -870,7 +870,7 @@ __global__ __launch_bounds__(256, 1) void k_halfls_split_wide(
     v4i ra0[S], ra1[S], rb[S];
     auto fetch = [&](int se, int kk) {
         const long long arow =
-            ((long long)cd.seg_a[se] * J.mr.ld + cd.off + m0 + lr - J.mr.v0) * kpad_all +
+            ((long long)cd.seg_a[se] * src.ld + cd.off + m0 + lr - src.v0) * kpad_all +
             cd.seg_pad[2 * se + 1] + kk + lp;
         const long long bofs = bcol + cd.seg_pad[2 * se] + kk + lp;
 #pragma unroll
@@ -1140,13 +1140,13 @@ void launch_build_g_split_lds(hipStream_t st, int S, double R, const GSplitJob *
 #undef L_
 }
 
-void launch_halfls_split(hipStream_t st, int S, double R, int kpad_all, const int *mue,
-                         const SplitHalfJob *d_jobs, const int *d_tbegin, int njobs, int ntiles,
-                         int cfg)
+void launch_halfls_split(hipStream_t st, int S, double R, int kpad_all, const MuSrc &src,
+                         const int *mue, const SplitHalfJob *d_jobs, const int *d_tbegin,
+                         int njobs, int ntiles, int cfg)
 {
     if (njobs <= 0 || ntiles <= 0) return;
     if (cfg == 2 && S <= 5) {  // two register sets exceed the registers at 6 digits
-#define L_(s) hipLaunchKernelGGL(k_halfls_split2<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, kpad_all, mue, d_jobs, d_tbegin, njobs, ntiles, R)
+#define L_(s) hipLaunchKernelGGL(k_halfls_split2<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, kpad_all, src, mue, d_jobs, d_tbegin, njobs, ntiles, R)
         switch (S) {
         case 3: L_(3); break;
         case 4: L_(4); break;
@@ -1156,12 +1156,12 @@ void launch_halfls_split(hipStream_t st, int S, double R, int kpad_all, const in
         return;
     }
     if (cfg == 1) {
-#define L_(s) hipLaunchKernelGGL(k_halfls_split_wide<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, kpad_all, mue, d_jobs, d_tbegin, njobs, ntiles, R)
+#define L_(s) hipLaunchKernelGGL(k_halfls_split_wide<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, kpad_all, src, mue, d_jobs, d_tbegin, njobs, ntiles, R)
         GI_SPLIT_DISPATCH(S, L_)
 #undef L_
         return;
     }
-#define L_(s) hipLaunchKernelGGL(k_halfls_split<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, kpad_all, mue, d_jobs, d_tbegin, njobs, ntiles, R)
+#define L_(s) hipLaunchKernelGGL(k_halfls_split<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, kpad_all, src, mue, d_jobs, d_tbegin, njobs, ntiles, R)
     GI_SPLIT_DISPATCH(S, L_)
 #undef L_
 }

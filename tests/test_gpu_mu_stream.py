@@ -73,10 +73,11 @@ def test_streamed_mu_same_bits(engine_resident, sym, kw):
         _same(ref[m], eng.compute(mode=m), ("resident again", m))
 
 
-def test_streamed_mu_every_tile_and_route(engine_resident):
+def test_streamed_mu_every_tile_and_route(engine_resident, monkeypatch):
     """The part loop under every half-ls tile shape of both matrix-core modes,
-    with several pipeline rounds, and with the eigenvector blocks streamed as
-    well (FP64 batched): the bits of resident mu."""
+    in rounds cut by a small device budget (streamed mu is otherwise one round
+    per call), and with the eigenvector blocks streamed as well (FP64
+    batched): the bits of resident mu."""
     eng = engine_resident
     job = synth.make_job("C2V", nvib=120, jvals=(2, 3), seed=45, e_span=12000.0)
     eng.load(job)
@@ -94,6 +95,11 @@ def test_streamed_mu_every_tile_and_route(engine_resident):
         eng.set_tuning(4, 0, 3, sc, 50)
         _same(ref[SP], eng.compute(mode=SP), ("split tile", sc))
     eng.set_tuning(4, 0, 4, 2, 50)
+    monkeypatch.setenv("GI_BATCH_BYTES", str(1 << 20))
+    for m in (B, SP):
+        _same(ref[m], eng.compute(mode=m), ("small rounds", m))
+        assert eng.kernel_stats()["mu_copy"]["launches"] > eng.mu_parts()
+    monkeypatch.delenv("GI_BATCH_BYTES")
     eng.load(job, mu_budget=17 * 24 * job.nvib, vector_budget=1)
     assert eng.streamed_blocks() > 0
     _same(ref[B], eng.compute(mode=B), "blocks and mu streamed")
