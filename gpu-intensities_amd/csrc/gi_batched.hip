@@ -315,8 +315,11 @@ void k_halfls_jobs(MuSrc src, const HalfJob *__restrict__ jobs, const int *__res
                    int njobs, int ntiles)
 {
     using T = TileW<BM_, BN_, NW_>;
-    __shared__ double As[BK * T::PA];
-    __shared__ double Bs[BK * T::PB];
+    // two LDS buffers: step s multiplies from buffer s & 1 while the loads of
+    // step s + 1 are in flight, then stores them into the other buffer (one
+    // barrier per K step)
+    __shared__ double As[2][BK * T::PA];
+    __shared__ double Bs[2][BK * T::PB];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= ntiles) return;
     const HalfJob &J = jobs[find_job(tbegin, njobs, tile)];
@@ -362,20 +365,27 @@ void k_halfls_jobs(MuSrc src, const HalfJob *__restrict__ jobs, const int *__res
             rb[q] = (nok && kl < len) ? gp[(size_t)kl * cd.ncols] : 0.0;
         }
     };
-    if (cd.nseg > 0) fetch(0, 0);
-    while (e < cd.nseg) {
+    auto stage = [&](int b) {
 #pragma unroll
-        for (int q = 0; q < T::LA; q++) As[(ak0 + q * (T::NT / BM_)) * T::PA + am] = ra[q];
+        for (int q = 0; q < T::LA; q++) As[b][(ak0 + q * (T::NT / BM_)) * T::PA + am] = ra[q];
 #pragma unroll
-        for (int q = 0; q < T::LB; q++) Bs[(bk0 + q * (T::NT / BN_)) * T::PB + bn] = rb[q];
+        for (int q = 0; q < T::LB; q++) Bs[b][(bk0 + q * (T::NT / BN_)) * T::PB + bn] = rb[q];
+    };
+    if (cd.nseg > 0) {
+        fetch(0, 0);
+        stage(0);
         __syncthreads();
+    }
+    for (int b = 0; e < cd.nseg; b ^= 1) {
         kk0 += BK;
         if (kk0 >= cd.seg_n[e]) {
             e++;
             kk0 = 0;
         }
-        if (e < cd.nseg) fetch(e, kk0);
-        mfma_step_w<T::MI, T::NI, T::PA, T::PB>(As, Bs, wm, wn, lane, acc);
+        const bool more = e < cd.nseg;
+        if (more) fetch(e, kk0);
+        mfma_step_w<T::MI, T::NI, T::PA, T::PB>(As[b], Bs[b], wm, wn, lane, acc);
+        if (more) stage(b ^ 1);
         __syncthreads();
     }
 #pragma unroll
@@ -476,7 +486,8 @@ __global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict_
         if (k0 + BK < ke) fetch(k0 + BK);
         // raised issue priority while this wavefront multiplies, so the other
         // workgroup's loads and LDS stores fill the gaps (k_corr_mfma 1.47 -> 1.38 ms
-        // at C2; the half-ls kernels gain nothing or lose)
+        // at C2; the half-ls kernels gain nothing or lose; two LDS buffers with
+        // one barrier per step measured 1.50 ms)
         __builtin_amdgcn_s_setprio(1);
         mfma_step_mask<BM_, BN_>(As, Bs, wm, wn, lane, bneed, acc);
         __builtin_amdgcn_s_setprio(0);
