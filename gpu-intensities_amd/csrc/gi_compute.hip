@@ -278,6 +278,7 @@ static void items_job(const gi_ctx *c, int nlower, const int *ids, int query, Pl
     // every level: only those that pass the lower-state filters, in id order
     const int n = ids ? nlower : (int)c->lv_low.size();
     std::vector<WindowCursor> cur(c->blocks.size());
+    P.items.reserve(P.items.size() + (size_t)n * std::min(c->nJ, 3));
     for (int k = 0; k < n; k++) {
         const int L = ids ? ids[k] : c->lv_low[k];
         if (L < 0 || L >= c->nlevels) continue;
@@ -289,7 +290,10 @@ static void items_job(const gi_ctx *c, int nlower, const int *ids, int query, Pl
         const double eI = c->lv_energy[L];
         const int gF = c->igamma_pair[gI];
         if (c->isym_pairs[gI] != c->isym_pairs[gF]) continue;
-        for (int jindF = 0; jindF < c->nJ; jindF++) {
+        // |J' - J| <= 1 keeps J' within one list position of J (a J list is
+        // strictly ascending; the deck's two J values are both in range)
+        const int jlo = std::max(0, jindI - 1), jhi = std::min(c->nJ - 1, jindI + 1);
+        for (int jindF = jlo; jindF <= jhi; jindF++) {
             const int jF = c->jvals[jindF];
             if (!((jF != c->jvals[0]) || (jI != c->jvals[0]))) continue;
             if (std::abs(jF - jI) > 1 || jI + jF < 1) continue;
@@ -338,6 +342,8 @@ static int build_chunks(gi_ctx *c, Plan &P)
         groups[std::make_tuple(it.query, it.jindI, it.gI, it.jindF)].push_back(i);
     }
     long long run = 0;  // candidate slots in chunk order
+    // about 60 bytes of descriptors per item plus 10 aligned arrays per chunk
+    P.meta.reserve(P.meta.size() + P.items.size() * 64 + (P.items.size() / 64 + 16) * 2560);
     auto put = [&](const void *src, size_t bytes) {
         const size_t o = (P.meta.size() + 255) & ~size_t(255);
         P.meta.resize(o + std::max<size_t>(bytes, 8));
