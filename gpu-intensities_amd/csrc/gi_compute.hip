@@ -640,13 +640,62 @@ static int send_jobs(gi_ctx *c, Slot &S, unsigned char *dev, const std::vector<T
     return GI_OK;
 }
 
-// Intensity steps of chunks [c0, c1) in one launch (table in dev, room for cap bytes).
-static int round_intensity(gi_ctx *c, Slot &S, const Plan &P, size_t c0, size_t c1,
-                           unsigned char *d_meta, const Consts &K, int *flag, LineSoA cand,
-                           unsigned char *dev, size_t cap, Timeline *tl)
+// The job tables of one round packed into one pinned staging block, each
+// table 256-aligned with its int array of first tiles right after it, and
+// sent to the device in one copy before the round's first launch.
+struct TablePack {
+    struct Ent {
+        const void *src;
+        size_t bytes;
+        const int *ints;
+        size_t nints, off;
+    };
+    unsigned char *dev;
+    size_t n = 0;
+    std::vector<Ent> ents;
+    explicit TablePack(unsigned char *d) : dev(d) {}
+    // device address of table v; *dbegin gets that of its first tiles
+    template <class T>
+    const T *add(const std::vector<T> &v, const std::vector<int> &begin, const int **dbegin)
+    {
+        const size_t o = (n + 255) & ~size_t(255), b = v.size() * sizeof(T);
+        ents.push_back(Ent{v.data(), b, begin.data(), begin.size(), o});
+        n = o + b + 4 * begin.size();
+        *dbegin = (const int *)(dev + o + b);
+        return (const T *)(dev + o);
+    }
+};
+
+static int send_pack(gi_ctx *c, Slot &S, const TablePack &T, size_t cap)
 {
-    std::vector<IntensArgs> jobs;
-    std::vector<int> begin;
+    if (T.n == 0) return GI_OK;
+    if (T.n > cap) return fail(c, GI_EINVAL, "internal: a round's job tables exceed their buffer");
+    int rc = GI_OK;
+    unsigned char *hb = stage(c, S, nullptr, T.n, &rc);
+    if (rc) return rc;
+    for (const auto &e : T.ents) {
+        if (e.bytes) std::memcpy(hb + e.off, e.src, e.bytes);
+        if (e.nints) std::memcpy(hb + e.off + e.bytes, e.ints, 4 * e.nints);
+    }
+    HIPCHK(hipMemcpyAsync(T.dev, hb, T.n, hipMemcpyHostToDevice, S.stream));
+    return GI_OK;
+}
+
+// Bytes a round's tables can take: each table kind at its per-round maximum
+// plus the alignment of each.
+static size_t pack_cap(std::initializer_list<size_t> tables)
+{
+    size_t n = 0;
+    for (size_t t : tables) n += t + 256;
+    return n;
+}
+
+// Intensity jobs of chunks [c0, c1): one per chunk with candidates, blocks of
+// 256 candidates; returns the block count.
+static int round_intensity_jobs(gi_ctx *c, const Plan &P, size_t c0, size_t c1,
+                                unsigned char *d_meta, const Consts &K, int *flag, LineSoA cand,
+                                std::vector<IntensArgs> &jobs, std::vector<int> &begin)
+{
     int nb = 0;
     for (size_t i = c0; i < c1; i++) {
         const Chunk &ch = P.chunks[i];
@@ -655,14 +704,7 @@ static int round_intensity(gi_ctx *c, Slot &S, const Plan &P, size_t c0, size_t 
         begin.push_back(nb);
         nb += (int)((ch.ncand + 255) / 256);
     }
-    if (jobs.empty()) return GI_OK;
-    int rc;
-    if ((rc = send_jobs(c, S, dev, jobs, begin, cap)) != GI_OK) return rc;
-    TL_BEGIN("intensity");
-    launch_intensity_jobs(S.stream, (const IntensArgs *)dev, (const int *)(dev + cap),
-                          (int)jobs.size(), nb);
-    TL_END();
-    return GI_OK;
+    return nb;
 }
 
 // Pinned staging of an upload on slot S (valid until the slot's next launch
@@ -920,12 +962,21 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
     double *G = (double *)slot_scratch(c, S, "G", mg * 8, &rc); if (rc) return rc;
     double *H = (double *)slot_scratch(c, S, "H", mh * 8, &rc); if (rc) return rc;
     double *Ht = (double *)slot_scratch(c, S, "Ht", mht * 8, &rc); if (rc) return rc;
+    // per-mu-part half-ls tables (streamed mu); the round's other tables go
+    // to rtab in one copy
     const size_t jobs_bytes = (size_t)mjobs * sizeof(HalfJob);
-    unsigned char *d_jobs =
-        (unsigned char *)slot_scratch(c, S, "jobs", jobs_bytes + 2 * 4 * (size_t)mjobs, &rc);
-    if (rc) return rc;
-    unsigned char *d_mm = (unsigned char *)slot_scratch(
-        c, S, "mmargs", (size_t)mchunks * (sizeof(CorrMMArgs) + 4), &rc);
+    unsigned char *d_jobs = nullptr;
+    if (c->mu_streamed) {
+        d_jobs = (unsigned char *)slot_scratch(c, S, "jobs", jobs_bytes + 4 * (size_t)mjobs, &rc);
+        if (rc) return rc;
+    }
+    const size_t mc = (size_t)mchunks;
+    const size_t rtab_cap = pack_cap({mc * (sizeof(GMapJob) + 4), mc * (sizeof(GMapJob) + 4),
+                                      mc * (sizeof(GBuildJob) + 4),
+                                      (size_t)mjobs * (sizeof(HalfJob) + 4),
+                                      mc * (sizeof(FoldJob) + 4), mc * (sizeof(CorrMMArgs) + 4),
+                                      mc * (sizeof(IntensArgs) + 4)});
+    unsigned char *rtab = (unsigned char *)slot_scratch(c, S, "rtab", rtab_cap, &rc);
     if (rc) return rc;
     // tile shape of the half-ls products of round [c0, c1): the setting, or
     // with kTileAuto 128 x 128 on eight wavefronts while the round has
@@ -944,20 +995,6 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         return t >= (long long)kAutoTileWaves * 2 * c->ncu ? kTile128x128w8 : 0;
     };
 
-    GBuildJob *d_gjobs = (GBuildJob *)slot_scratch(
-        c, S, "gjobs", (size_t)mchunks * (sizeof(GBuildJob) + 4), &rc);
-    if (rc) return rc;
-    GMapJob *d_mjobs = (GMapJob *)slot_scratch(
-        c, S, "mjobs", (size_t)mchunks * (sizeof(GMapJob) + 4), &rc);
-    if (rc) return rc;
-    GMapJob *d_mjobs2 = (GMapJob *)slot_scratch(
-        c, S, "mjobs2", (size_t)mchunks * (sizeof(GMapJob) + 4), &rc);
-    if (rc) return rc;
-    const size_t fcap = (size_t)mchunks * sizeof(FoldJob), icap = (size_t)mchunks * sizeof(IntensArgs);
-    unsigned char *d_fold = (unsigned char *)slot_scratch(c, S, "foldjobs", fcap + 4 * mchunks, &rc);
-    if (rc) return rc;
-    unsigned char *d_int = (unsigned char *)slot_scratch(c, S, "intjobs", icap + 4 * mchunks, &rc);
-    if (rc) return rc;
     int maxparts = 1;
     for (const Chunk &ch : chunks) maxparts = std::max(maxparts, corr_kparts(c, P, ch));
     P.ampx_stride = (long long)c->amp_m * c->amp_m * std::max<long long>(P.ncand, 1);
@@ -1074,80 +1111,8 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         }
         const int njobs = (int)jobs.size(), ngj = (int)gjobs.size(), nmj = (int)mjobs_v.size(),
                   nmg = (int)mjobs_g.size();
-        if (nmj > 0) {
-            const size_t mb = nmj * sizeof(GMapJob);
-            unsigned char *hb = stage(c, S, mjobs_v.data(), mb + 4 * (size_t)nmj, &rc);
-            if (rc) return rc;
-            std::memcpy(hb + mb, mubeg.data(), 4 * (size_t)nmj);
-            HIPCHK(hipMemcpyAsync(d_mjobs, hb, mb + 4 * (size_t)nmj, hipMemcpyHostToDevice,
-                                  S.stream));
-            TL_BEGIN("build_g");
-            launch_build_g_map_lds(S.stream, d_mjobs, (const int *)((unsigned char *)d_mjobs + mb),
-                                   nmj, munits);
-            TL_END();
-        }
-        if (nmg > 0) {
-            const size_t mb = nmg * sizeof(GMapJob);
-            unsigned char *hb = stage(c, S, mjobs_g.data(), mb + 4 * (size_t)nmg, &rc);
-            if (rc) return rc;
-            std::memcpy(hb + mb, mbegin.data(), 4 * (size_t)nmg);
-            HIPCHK(hipMemcpyAsync(d_mjobs2, hb, mb + 4 * (size_t)nmg, hipMemcpyHostToDevice,
-                                  S.stream));
-            TL_BEGIN("build_g");
-            if (m4)
-                launch_build_g_map4(S.stream, d_mjobs2,
-                                    (const int *)((unsigned char *)d_mjobs2 + mb), nmg, mt_tiles);
-            else
-                launch_build_g_map(S.stream, d_mjobs2,
-                                   (const int *)((unsigned char *)d_mjobs2 + mb), nmg, mt_tiles);
-            TL_END();
-        }
-        if (ngj > 0) {
-            const size_t gb = ngj * sizeof(GBuildJob);
-            unsigned char *hb = stage(c, S, nullptr, gb + 4 * (size_t)ngj, &rc);
-            if (rc) return rc;
-            std::memcpy(hb, gjobs.data(), gb);
-            std::memcpy(hb + gb, gbegin.data(), 4 * (size_t)ngj);
-            HIPCHK(hipMemcpyAsync(d_gjobs, hb, gb + 4 * (size_t)ngj, hipMemcpyHostToDevice,
-                                  S.stream));
-            TL_BEGIN("build_g");
-            launch_build_g_units(S.stream, N, d_gjobs,
-                                 (const int *)((unsigned char *)d_gjobs + gb), ngj, gt);
-            TL_END();
-        }
-        // the half-ls products: one launch over resident mu, or one per mu part
-        auto halfls = [&](const std::vector<HalfJob> &hj, const std::vector<int> &hb0,
-                          int tiles, const MuSrc &src) -> int {
-            unsigned char *hb = stage(c, S, nullptr, jobs_bytes + 4 * (size_t)mjobs, &rc);
-            if (rc) return rc;
-            std::memcpy(hb, hj.data(), hj.size() * sizeof(HalfJob));
-            std::memcpy(hb + jobs_bytes, hb0.data(), hj.size() * 4);
-            HIPCHK(hipMemcpyAsync(d_jobs, hb, jobs_bytes + 4 * (size_t)mjobs,
-                                  hipMemcpyHostToDevice, S.stream));
-            TL_BEGIN("half_ls_mfma");
-            P.work[14] += tiles;
-            launch_halfls_jobs(S.stream, hcfg, src, (const HalfJob *)d_jobs,
-                               (const int *)(d_jobs + jobs_bytes), (int)hj.size(), tiles);
-            TL_END();
-            return GI_OK;
-        };
-        if (njobs > 0 && !c->mu_streamed &&
-            (rc = halfls(jobs, tbegin, tt, MuSrc{c->d_mu_perm, (long long)N * N, N, 0})) != GI_OK)
-            return rc;
-        for (int v0 = 0; njobs > 0 && c->mu_streamed && v0 < N; v0 += c->mu_np) {
-            const int np = std::min(c->mu_np, N - v0);
-            std::vector<HalfJob> pj;
-            std::vector<int> pb;
-            const int pt = part_jobs(jobs, v0, np, TM, TN, pj, pb);
-            if (pt == 0) continue;
-            int b = 0;
-            if ((rc = mu_part_copy(c, S, MP, c->h_mu_perm + v0, (size_t)N * 8, (size_t)np * 8,
-                                   (size_t)3 * N, &b, tl)) != GI_OK)
-                return rc;
-            if ((rc = halfls(pj, pb, pt, MuSrc{MP.buf[b], (long long)N * np, np, v0})) != GI_OK)
-                return rc;
-            HIPCHK(hipEventRecord(S.mu_ev[2 + b], S.stream));
-        }
+        // every table of the round is built on the host before its first launch
+        // and goes to the device in one copy
         std::vector<FoldJob> fjobs;
         std::vector<int> fbegin;
         int fblocks = 0;
@@ -1200,13 +1165,6 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             P.work[7] += ch.nunits * 24.0 * (double)N * N;
             P.work[8] += band_exec_flops(P, ch, ndegI * ndegF, BF.nsize);
         }
-        if (!fjobs.empty()) {
-            if ((rc = send_jobs(c, S, d_fold, fjobs, fbegin, fcap)) != GI_OK) return rc;
-            TL_BEGIN("fold");
-            launch_fold_jobs(S.stream, (const FoldJob *)d_fold, (const int *)(d_fold + fcap),
-                             (int)fjobs.size(), fblocks);
-            TL_END();
-        }
         // corr tiles: per chunk ceil(rows / CM) x ceil(cols / CN), flattened
         const int CM = tile_rows(c->tile_corr), CN = tile_cols(c->tile_corr);
         std::vector<int> cbegin(mm.size());
@@ -1217,21 +1175,81 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             if (rows > 0 && cols > 0)
                 ctiles += ((rows + CM - 1) / CM) * ((cols + CN - 1) / CN) * mm[i].kparts;
         }
-        const size_t mm_bytes = mm.size() * sizeof(CorrMMArgs);
-        unsigned char *hmm = stage(c, S, mm.data(), mm_bytes + 4 * mm.size(), &rc);
-        if (rc) return rc;
-        std::memcpy(hmm + mm_bytes, cbegin.data(), 4 * mm.size());
-        HIPCHK(hipMemcpyAsync(d_mm, hmm, mm_bytes + 4 * mm.size(), hipMemcpyHostToDevice,
-                              S.stream));
+        std::vector<IntensArgs> ijobs;
+        std::vector<int> ibegin;
+        const int iblocks = round_intensity_jobs(c, P, rg.first, rg.second, d_meta, K, flag, cand,
+                                                 ijobs, ibegin);
+        TablePack T(rtab);
+        const int *b_mv = nullptr, *b_mg = nullptr, *b_gj = nullptr, *b_hl = nullptr,
+                  *b_fj = nullptr, *b_mm = nullptr, *b_ij = nullptr;
+        const GMapJob *t_mv = T.add(mjobs_v, mubeg, &b_mv), *t_mg = T.add(mjobs_g, mbegin, &b_mg);
+        const GBuildJob *t_gj = T.add(gjobs, gbegin, &b_gj);
+        const HalfJob *t_hl = c->mu_streamed ? nullptr : T.add(jobs, tbegin, &b_hl);
+        const FoldJob *t_fj = T.add(fjobs, fbegin, &b_fj);
+        const CorrMMArgs *t_mm = T.add(mm, cbegin, &b_mm);
+        const IntensArgs *t_ij = T.add(ijobs, ibegin, &b_ij);
+        if ((rc = send_pack(c, S, T, rtab_cap)) != GI_OK) return rc;
+        if (nmj > 0) {
+            TL_BEGIN("build_g");
+            launch_build_g_map_lds(S.stream, t_mv, b_mv, nmj, munits);
+            TL_END();
+        }
+        if (nmg > 0) {
+            TL_BEGIN("build_g");
+            if (m4)
+                launch_build_g_map4(S.stream, t_mg, b_mg, nmg, mt_tiles);
+            else
+                launch_build_g_map(S.stream, t_mg, b_mg, nmg, mt_tiles);
+            TL_END();
+        }
+        if (ngj > 0) {
+            TL_BEGIN("build_g");
+            launch_build_g_units(S.stream, N, t_gj, b_gj, ngj, gt);
+            TL_END();
+        }
+        // the half-ls products: one launch over resident mu from the round's
+        // tables, or one launch per mu part with its own table
+        if (njobs > 0 && !c->mu_streamed) {
+            TL_BEGIN("half_ls_mfma");
+            P.work[14] += tt;
+            launch_halfls_jobs(S.stream, hcfg, MuSrc{c->d_mu_perm, (long long)N * N, N, 0}, t_hl,
+                               b_hl, njobs, tt);
+            TL_END();
+        }
+        for (int v0 = 0; njobs > 0 && c->mu_streamed && v0 < N; v0 += c->mu_np) {
+            const int np = std::min(c->mu_np, N - v0);
+            std::vector<HalfJob> pj;
+            std::vector<int> pb;
+            const int pt = part_jobs(jobs, v0, np, TM, TN, pj, pb);
+            if (pt == 0) continue;
+            int b = 0;
+            if ((rc = mu_part_copy(c, S, MP, c->h_mu_perm + v0, (size_t)N * 8, (size_t)np * 8,
+                                   (size_t)3 * N, &b, tl)) != GI_OK)
+                return rc;
+            if ((rc = send_jobs(c, S, d_jobs, pj, pb, jobs_bytes)) != GI_OK) return rc;
+            TL_BEGIN("half_ls_mfma");
+            P.work[14] += pt;
+            launch_halfls_jobs(S.stream, hcfg, MuSrc{MP.buf[b], (long long)N * np, np, v0},
+                               (const HalfJob *)d_jobs, (const int *)(d_jobs + jobs_bytes),
+                               (int)pj.size(), pt);
+            TL_END();
+            HIPCHK(hipEventRecord(S.mu_ev[2 + b], S.stream));
+        }
+        if (!fjobs.empty()) {
+            TL_BEGIN("fold");
+            launch_fold_jobs(S.stream, t_fj, b_fj, (int)fjobs.size(), fblocks);
+            TL_END();
+        }
         TL_BEGIN("corr_mfma");
         P.work[15] += ctiles;
-        launch_corr_mfma(S.stream, c->tile_corr, (const CorrMMArgs *)d_mm,
-                         (const int *)((unsigned char *)d_mm + mm_bytes), (int)mm.size(), ctiles);
+        launch_corr_mfma(S.stream, c->tile_corr, t_mm, b_mm, (int)mm.size(), ctiles);
         TL_END();
         if (P.vstream && (rc = vec_stage_done(c, S, P, rg.first)) != GI_OK) return rc;
-        if ((rc = round_intensity(c, S, P, rg.first, rg.second, d_meta, K, flag, cand, d_int,
-                                  icap, tl)) != GI_OK)
-            return rc;
+        if (!ijobs.empty()) {
+            TL_BEGIN("intensity");
+            launch_intensity_jobs(S.stream, t_ij, b_ij, (int)ijobs.size(), iblocks);
+            TL_END();
+        }
         if ((rc = round_compact(c, S, P, iround, flag, cand, pc_counts, pc_offs, tl)) != GI_OK)
             return rc;
         if ((rc = cap.end(c, P)) != GI_OK) return rc;
@@ -1290,28 +1308,21 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
     double *Ht = (double *)slot_scratch(c, S, "Ht", 8 * mht, &rc); if (rc) return rc;
     int8_t *Hd = (int8_t *)slot_scratch(c, S, "Hd", SD * mhd, &rc); if (rc) return rc;
     int *He = (int *)slot_scratch(c, S, "He", 4 * mhc, &rc); if (rc) return rc;
+    // per-mu-part half-ls tables (streamed mu); the round's other tables go
+    // to rtab in one copy
     const size_t hj_bytes = (size_t)mjobs * sizeof(SplitHalfJob);
-    unsigned char *d_hj = (unsigned char *)slot_scratch(c, S, "sjobs", hj_bytes + 4 * mjobs, &rc);
-    if (rc) return rc;
-    const size_t gj_bytes = (size_t)mchunks * sizeof(GSplitJob);
-    unsigned char *d_gj = (unsigned char *)slot_scratch(c, S, "sgjobs", gj_bytes + 4 * mchunks, &rc);
-    if (rc) return rc;
-    unsigned char *d_gj2 =
-        (unsigned char *)slot_scratch(c, S, "sgjobs2", gj_bytes + 4 * mchunks, &rc);
-    if (rc) return rc;
-    const size_t cj_bytes = (size_t)mchunks * sizeof(CorrSplitArgs);
-    unsigned char *d_cj = (unsigned char *)slot_scratch(c, S, "scorr", cj_bytes + 4 * mchunks, &rc);
-    if (rc) return rc;
-    const size_t fcap = (size_t)mchunks * sizeof(FoldJob), icap = (size_t)mchunks * sizeof(IntensArgs),
-                 scap = (size_t)mchunks * sizeof(SplitRowsJob);
-    unsigned char *d_fold = (unsigned char *)slot_scratch(c, S, "foldjobs", fcap + 4 * mchunks, &rc);
-    if (rc) return rc;
-    unsigned char *d_int = (unsigned char *)slot_scratch(c, S, "intjobs", icap + 4 * mchunks, &rc);
-    if (rc) return rc;
-    unsigned char *d_srj = (unsigned char *)slot_scratch(c, S, "srjobs", scap + 4 * mchunks, &rc);
-    if (rc) return rc;
-    const size_t fscap = (size_t)mchunks * sizeof(FoldSplitJob);
-    unsigned char *d_fsj = (unsigned char *)slot_scratch(c, S, "fsjobs", fscap + 4 * mchunks, &rc);
+    unsigned char *d_hj = nullptr;
+    if (c->mu_streamed) {
+        d_hj = (unsigned char *)slot_scratch(c, S, "sjobs", hj_bytes + 4 * mjobs, &rc);
+        if (rc) return rc;
+    }
+    const size_t mc = (size_t)mchunks;
+    const size_t rtab_cap = pack_cap(
+        {mc * (sizeof(GSplitJob) + 4), mc * (sizeof(GSplitJob) + 4),
+         (size_t)mjobs * (sizeof(SplitHalfJob) + 4), mc * (sizeof(FoldSplitJob) + 4),
+         mc * (sizeof(FoldJob) + 4), mc * (sizeof(SplitRowsJob) + 4),
+         mc * (sizeof(CorrSplitArgs) + 4), mc * (sizeof(IntensArgs) + 4)});
+    unsigned char *rtab = (unsigned char *)slot_scratch(c, S, "rtab", rtab_cap, &rc);
     if (rc) return rc;
     // fold and digit split fused when every upper block's column fits one
     // workgroup (GI_FOLD_SPLIT=0 forces the two-kernel path; same bits)
@@ -1431,66 +1442,8 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             ogd += need[i].gd; oge += need[i].ge; oh += need[i].h;
             oht += need[i].ht; ohd += need[i].hd; ohc += need[i].hc;
         }
-        auto send = [&](unsigned char *dst, const void *src, size_t nb, const std::vector<int> &beg,
-                        size_t cap) -> int {
-            int e = GI_OK;
-            unsigned char *hb = stage(c, S, nullptr, cap + 4 * beg.size(), &e);
-            if (e) return e;
-            std::memcpy(hb, src, nb);
-            std::memcpy(hb + cap, beg.data(), 4 * beg.size());
-            HIPCHK(hipMemcpyAsync(dst, hb, cap + 4 * beg.size(), hipMemcpyHostToDevice, S.stream));
-            return GI_OK;
-        };
-        if (!gj.empty()) {
-            if ((rc = send(d_gj, gj.data(), gj.size() * sizeof(GSplitJob), gbegin, gj_bytes)) != GI_OK)
-                return rc;
-            TL_BEGIN("build_g");
-            launch_build_g_split_lds(S.stream, SH, c->split_radix, (const GSplitJob *)d_gj,
-                                     (const int *)(d_gj + gj_bytes), (int)gj.size(), ubeg);
-            TL_END();
-        }
-        if (!gjg.empty()) {
-            if ((rc = send(d_gj2, gjg.data(), gjg.size() * sizeof(GSplitJob), gbegin_g,
-                           gj_bytes)) != GI_OK)
-                return rc;
-            TL_BEGIN("build_g");
-            launch_build_g_split(S.stream, SH, c->split_radix, (const GSplitJob *)d_gj2,
-                                 (const int *)(d_gj2 + gj_bytes), (int)gjg.size(), gt, g4);
-            TL_END();
-        }
-        // the half-ls products: one launch over resident mu digits, or one per part
-        const int scfg = c->tile_split == 2 && !steps_fit ? 0 : c->tile_split;
-        auto halfls = [&](const std::vector<SplitHalfJob> &jobs, const std::vector<int> &beg,
-                          int tiles, const MuSrc &src) -> int {
-            int e = send(d_hj, jobs.data(), jobs.size() * sizeof(SplitHalfJob), beg, hj_bytes);
-            if (e != GI_OK) return e;
-            TL_BEGIN("half_ls_split");
-            P.work[14] += tiles;
-            launch_halfls_split(S.stream, SH, c->split_radix, c->kpad_all, src, c->d_mue,
-                                (const SplitHalfJob *)d_hj, (const int *)(d_hj + hj_bytes),
-                                (int)jobs.size(), tiles, scfg);
-            TL_END();
-            return GI_OK;
-        };
-        if (!hj.empty() && !c->mu_streamed &&
-            (rc = halfls(hj, tbegin, tt, MuSrc{c->d_mud, c->muplane, N, 0})) != GI_OK)
-            return rc;
-        for (int v0 = 0; !hj.empty() && c->mu_streamed && v0 < N; v0 += c->mu_np) {
-            const int np = std::min(c->mu_np, N - v0);
-            const size_t kp = (size_t)c->kpad_all;
-            std::vector<SplitHalfJob> pj;
-            std::vector<int> pb;
-            const int pt = part_jobs(hj, v0, np, srows_tile, 64, pj, pb);
-            if (pt == 0) continue;
-            int b = 0;
-            if ((rc = mu_part_copy(c, S, MP, c->h_mud + v0 * kp, (size_t)N * kp, (size_t)np * kp,
-                                   (size_t)3 * SH, &b, tl)) != GI_OK)
-                return rc;
-            if ((rc = halfls(pj, pb, pt, MuSrc{MP.buf[b], 3LL * np * (long long)kp, np, v0})) !=
-                GI_OK)
-                return rc;
-            HIPCHK(hipEventRecord(S.mu_ev[2 + b], S.stream));
-        }
+        // every table of the round is built on the host before its first launch
+        // and goes to the device in one copy
         std::vector<FoldJob> fjobs;
         std::vector<SplitRowsJob> sjobs;
         std::vector<FoldSplitJob> fsjobs;
@@ -1555,37 +1508,94 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             P.work[7] += ch.nunits * 24.0 * (double)N * N;
             P.work[8] += band_exec_flops(P, ch, ndegI * ndegF, BF.nsize);
         }
+        std::vector<IntensArgs> ijobs;
+        std::vector<int> ibegin;
+        const int iblocks = round_intensity_jobs(c, P, rg.first, rg.second, d_meta, K, flag, cand,
+                                                 ijobs, ibegin);
+        TablePack T(rtab);
+        const int *b_gj = nullptr, *b_gg = nullptr, *b_hj = nullptr, *b_fs = nullptr,
+                  *b_fj = nullptr, *b_sj = nullptr, *b_cj = nullptr, *b_ij = nullptr;
+        const GSplitJob *t_gj = T.add(gj, gbegin, &b_gj), *t_gg = T.add(gjg, gbegin_g, &b_gg);
+        const SplitHalfJob *t_hj = c->mu_streamed ? nullptr : T.add(hj, tbegin, &b_hj);
+        const FoldSplitJob *t_fs = nullptr;
+        const FoldJob *t_fj = nullptr;
+        const SplitRowsJob *t_sj = nullptr;
+        if (fused) {
+            t_fs = T.add(fsjobs, sbegin, &b_fs);
+        } else {
+            t_fj = T.add(fjobs, fbegin, &b_fj);
+            t_sj = T.add(sjobs, sbegin, &b_sj);
+        }
+        const CorrSplitArgs *t_cj = T.add(cj, cbegin, &b_cj);
+        const IntensArgs *t_ij = T.add(ijobs, ibegin, &b_ij);
+        if ((rc = send_pack(c, S, T, rtab_cap)) != GI_OK) return rc;
+        if (!gj.empty()) {
+            TL_BEGIN("build_g");
+            launch_build_g_split_lds(S.stream, SH, c->split_radix, t_gj, b_gj, (int)gj.size(), ubeg);
+            TL_END();
+        }
+        if (!gjg.empty()) {
+            TL_BEGIN("build_g");
+            launch_build_g_split(S.stream, SH, c->split_radix, t_gg, b_gg, (int)gjg.size(), gt, g4);
+            TL_END();
+        }
+        // the half-ls products: one launch over resident mu digits from the
+        // round's tables, or one launch per mu part with its own table
+        const int scfg = c->tile_split == 2 && !steps_fit ? 0 : c->tile_split;
+        if (!hj.empty() && !c->mu_streamed) {
+            TL_BEGIN("half_ls_split");
+            P.work[14] += tt;
+            launch_halfls_split(S.stream, SH, c->split_radix, c->kpad_all,
+                                MuSrc{c->d_mud, c->muplane, N, 0}, c->d_mue, t_hj, b_hj,
+                                (int)hj.size(), tt, scfg);
+            TL_END();
+        }
+        for (int v0 = 0; !hj.empty() && c->mu_streamed && v0 < N; v0 += c->mu_np) {
+            const int np = std::min(c->mu_np, N - v0);
+            const size_t kp = (size_t)c->kpad_all;
+            std::vector<SplitHalfJob> pj;
+            std::vector<int> pb;
+            const int pt = part_jobs(hj, v0, np, srows_tile, 64, pj, pb);
+            if (pt == 0) continue;
+            int b = 0;
+            if ((rc = mu_part_copy(c, S, MP, c->h_mud + v0 * kp, (size_t)N * kp, (size_t)np * kp,
+                                   (size_t)3 * SH, &b, tl)) != GI_OK)
+                return rc;
+            if ((rc = send_jobs(c, S, d_hj, pj, pb, hj_bytes)) != GI_OK) return rc;
+            TL_BEGIN("half_ls_split");
+            P.work[14] += pt;
+            launch_halfls_split(S.stream, SH, c->split_radix, c->kpad_all,
+                                MuSrc{MP.buf[b], 3LL * np * (long long)kp, np, v0}, c->d_mue,
+                                (const SplitHalfJob *)d_hj, (const int *)(d_hj + hj_bytes),
+                                (int)pj.size(), pt, scfg);
+            TL_END();
+            HIPCHK(hipEventRecord(S.mu_ev[2 + b], S.stream));
+        }
         if (!fjobs.empty() && fused) {
-            if ((rc = send_jobs(c, S, d_fsj, fsjobs, sbegin, fscap)) != GI_OK) return rc;
             TL_BEGIN("fold_split");
-            launch_fold_split_jobs(S.stream, SD, c->split_radix, (const FoldSplitJob *)d_fsj,
-                                   (const int *)(d_fsj + fscap), (int)fsjobs.size(), srows,
-                                   maxkp);
+            launch_fold_split_jobs(S.stream, SD, c->split_radix, t_fs, b_fs, (int)fsjobs.size(),
+                                   srows, maxkp);
             TL_END();
         } else if (!fjobs.empty()) {
-            if ((rc = send_jobs(c, S, d_fold, fjobs, fbegin, fcap)) != GI_OK) return rc;
             TL_BEGIN("fold");
-            launch_fold_jobs(S.stream, (const FoldJob *)d_fold, (const int *)(d_fold + fcap),
-                             (int)fjobs.size(), fblocks);
+            launch_fold_jobs(S.stream, t_fj, b_fj, (int)fjobs.size(), fblocks);
             TL_END();
-            if ((rc = send_jobs(c, S, d_srj, sjobs, sbegin, scap)) != GI_OK) return rc;
             TL_BEGIN("split_ht");
-            launch_split_rows_jobs(S.stream, SD, c->split_radix, (const SplitRowsJob *)d_srj,
-                                   (const int *)(d_srj + scap), (int)sjobs.size(), srows);
+            launch_split_rows_jobs(S.stream, SD, c->split_radix, t_sj, b_sj, (int)sjobs.size(),
+                                   srows);
             TL_END();
         }
         if (!cj.empty()) {
-            if ((rc = send(d_cj, cj.data(), cj.size() * sizeof(CorrSplitArgs), cbegin, cj_bytes)) != GI_OK)
-                return rc;
             TL_BEGIN("corr_split");
             P.work[15] += ct;
-            launch_corr_split(S.stream, SD, c->split_radix, (const CorrSplitArgs *)d_cj, (const int *)(d_cj + cj_bytes),
-                              (int)cj.size(), ct);
+            launch_corr_split(S.stream, SD, c->split_radix, t_cj, b_cj, (int)cj.size(), ct);
             TL_END();
         }
-        if ((rc = round_intensity(c, S, P, rg.first, rg.second, d_meta, K, flag, cand, d_int,
-                                  icap, tl)) != GI_OK)
-            return rc;
+        if (!ijobs.empty()) {
+            TL_BEGIN("intensity");
+            launch_intensity_jobs(S.stream, t_ij, b_ij, (int)ijobs.size(), iblocks);
+            TL_END();
+        }
         if ((rc = round_compact(c, S, P, iround, flag, cand, pc_counts, pc_offs, tl)) != GI_OK)
             return rc;
         if ((rc = cap.end(c, P)) != GI_OK) return rc;
