@@ -272,11 +272,15 @@ struct WindowCursor {
     }
 };
 
-// Host threads for planning a large job (its items and chunks are built in
-// independent pieces and joined in order, so the plan is the same for any
-// count); small jobs plan on the calling thread.
+// Host threads for building a large job's chunks (built per group and
+// joined in order, so the plan is the same for any count); small jobs plan
+// on the calling thread. GI_PLAN_THREADS=n forces n (1..64) at any size.
 static int plan_threads(size_t work)
 {
+    if (const char *env = std::getenv("GI_PLAN_THREADS")) {
+        const int n = std::atoi(env);
+        if (n >= 1 && n <= 64) return n;
+    }
     if (work < 16384) return 1;
     const unsigned hw = std::thread::hardware_concurrency();
     return (int)std::max(1u, std::min(8u, hw ? hw : 1u));

@@ -249,6 +249,35 @@ def test_batched_many_rounds(engine, monkeypatch):
     compare(got, ref)
 
 
+def test_plan_threads_same_lines(engine, monkeypatch):
+    """Chunks of a large job are built per item group on several host threads
+    and joined in group order; forced to 1, 3 and 8 threads on a small job,
+    every mode gives the same bits, also for a J list and for requests."""
+    job = synth.make_job("C2V", nvib=40, jvals=(1, 2, 3, 4), seed=61, e_span=12000.0)
+    engine.load(job)
+    modes = (binding.GI_MODE_FP64, binding.GI_MODE_FP64_BATCHED, binding.GI_MODE_SPLIT)
+    req = dict(jindI=2, igammaI=0, jindF=3, temperature=700.0, q_stat=0.0)
+    out = {}
+    for n in ("1", "3", "8"):
+        monkeypatch.setenv("GI_PLAN_THREADS", n)
+        for m in modes:
+            out[n, m] = {k: v.copy() for k, v in engine.compute(mode=m).items()}
+        engine.prepare_split()
+        engine.serve_start(1 << 28)
+        try:
+            t = [engine.serve_submit(mode=m, **req) for m in modes]
+            for m, x in zip(modes, t):
+                out[n, "req", m] = {k: v.copy() for k, v in engine.serve_fetch(x).items()}
+        finally:
+            engine.serve_stop()
+    monkeypatch.delenv("GI_PLAN_THREADS")
+    for key in [k for k in out if k[0] == "1"]:
+        assert len(out[key]["S"]) > 0, key
+        for n in ("3", "8"):
+            for f, v in out[key].items():
+                assert np.array_equal(v, out[(n,) + key[1:]][f]), (key, n, f)
+
+
 def test_tile_shapes_bitwise_equal(engine):
     """Every tile shape of the batched kernels gives the same bits (each
     product sums K in the same order whatever the tile), and matches the
