@@ -1093,8 +1093,10 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
     // tile shape of the half-ls products of round [c0, c1): the setting, or
     // with kTileAuto 128 x 128 on eight wavefronts while the round has
     // kAutoTileWaves waves of them (two workgroups per CU), else 64 x 64
-    // (measured at C2: 128 x 128 / 8 waves 3.09 ms against 3.35 ms for four
-    // waves on a whole pass; 64 x 64 wins rounds under about two waves)
+    // (measured, half-ls per pass: C3v J 0..15 1116 ms against 1189-1258 ms
+    // for the 64-row and 64-column tiles; C2v J 10<->11 3.46 ms against 3.39 ms
+    // for 64 x 128, which its tuning table picks; 64 x 64 wins rounds under
+    // about two waves)
     auto round_halfls_cfg = [&](size_t c0, size_t c1) {
         if (c->tile_halfls != kTileAuto) return c->tile_halfls;
         long long t = 0;
