@@ -337,10 +337,10 @@ void k_halfls_jobs(MuSrc src, const HalfJob *__restrict__ jobs, const int *__res
     const bool mok = m0 + am < mhi;
     const int n = n0 + bn;
     const bool nok = n < ncolsTot;
-    const double *gcol = J.G;
+    gptr<double> gcol = gmem(J.G);
     if (nok) {
         const int u = n / cd.ncols, jj = n - u * cd.ncols;
-        gcol = J.G + (size_t)u * J.Gtot + cd.gbase_p + jj;
+        gcol = gmem(J.G) + (size_t)u * J.Gtot + cd.gbase_p + jj;
     }
     d4 acc[T::MI][T::NI];
 #pragma unroll
@@ -349,11 +349,14 @@ void k_halfls_jobs(MuSrc src, const HalfJob *__restrict__ jobs, const int *__res
         for (int y = 0; y < T::NI; y++) acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
     double ra[T::LA], rb[T::LB];
     int e = 0, kk0 = 0;
+    // the class's segment table, read every K step
+    const gptr<int> seg_a = gmem(cd.seg_a), seg_off = gmem(cd.seg_off), seg_n = gmem(cd.seg_n),
+                    seg_row0 = gmem(cd.seg_row0);
     auto fetch = [&](int se, int sk0) {
-        const int len = min(BK, cd.seg_n[se] - sk0);
-        const double *mp = mu + (size_t)cd.seg_a[se] * src.plane +
-                           (size_t)(cd.seg_off[se] + sk0) * ld + (cd.off + m0 + am - src.v0);
-        const double *gp = gcol + (size_t)(cd.seg_row0[se] + sk0) * cd.ncols;
+        const int len = min(BK, seg_n[se] - sk0);
+        const double *mp = mu + (size_t)seg_a[se] * src.plane +
+                           (size_t)(seg_off[se] + sk0) * ld + (cd.off + m0 + am - src.v0);
+        gptr<double> gp = gcol + (size_t)(seg_row0[se] + sk0) * cd.ncols;
 #pragma unroll
         for (int q = 0; q < T::LA; q++) {
             const int kl = ak0 + q * (T::NT / BM_);
@@ -378,7 +381,7 @@ void k_halfls_jobs(MuSrc src, const HalfJob *__restrict__ jobs, const int *__res
     }
     for (int b = 0; e < cd.nseg; b ^= 1) {
         kk0 += BK;
-        if (kk0 >= cd.seg_n[e]) {
+        if (kk0 >= seg_n[e]) {
             e++;
             kk0 = 0;
         }
@@ -450,17 +453,17 @@ __global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict_
     // the 4 lanes of a row read 32 contiguous bytes per load and store into 4
     // consecutive LDS rows, BM + 4 doubles apart, so on distinct banks
     const int lr = tid >> 2, lt = tid & 3;
-    const double *Urow[BM_ / 64];
-    const double *Hrow[BN_ / 64];
+    gptr<double> Urow[BM_ / 64];
+    gptr<double> Hrow[BN_ / 64];
 #pragma unroll
     for (int p = 0; p < BM_ / 64; p++) {
         const int fr = f0 + lr + 64 * p;
-        Urow[p] = fr < a.Hi ? a.vec + (size_t)a.blk_col[fr] * a.nsize : nullptr;
+        Urow[p] = fr < a.Hi ? gmem(a.vec) + (size_t)gmem(a.blk_col)[fr] * a.nsize : nullptr;
     }
 #pragma unroll
     for (int p = 0; p < BN_ / 64; p++) {
         const int nr = n0 + lr + 64 * p;
-        Hrow[p] = nr < ncols ? a.Ht + (size_t)nr * a.nsize : nullptr;
+        Hrow[p] = nr < ncols ? gmem(a.Ht) + (size_t)nr * a.nsize : nullptr;
     }
     double ra[BM_ / 64][4], rb[BN_ / 64][4];
     auto fetch = [&](int k0) {
@@ -611,14 +614,14 @@ __global__ __launch_bounds__(1024) void k_build_g_map_lds(const GMapJob *__restr
     const int ji = find_job(ubegin, njobs, blockIdx.x);
     const GMapJob &J = jobs[ji];
     const int u = blockIdx.x - ubegin[ji];
-    const double *src = J.uptr[u];
+    const gptr<double> src = gmem(J.uptr[u]);
     for (int i = threadIdx.x; i < J.nsize; i += 1024) rec[i] = src[i];
     __syncthreads();
     const int d = J.uideg[u];
-    const int *ip = J.idx[d];
-    const double *wp = J.wt[d];
+    const gptr<int> ip = gmem(J.idx[d]);
+    const gptr<double> wp = gmem(J.wt[d]);
     const int Gt = J.Gtot, W = J.W;
-    double *G = J.G + (size_t)u * Gt;
+    const gout<double> G = gmem_w(J.G) + (size_t)u * Gt;
     // four elements per thread per pass, 1024 apart: every map load of the pass
     // is in flight before the first gather (one element at a time left the
     // waves waiting on L2 latency most of the time)

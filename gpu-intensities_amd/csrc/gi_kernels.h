@@ -11,6 +11,26 @@
 
 namespace gi {
 
+// Device memory seen through a pointer read from a job table. The compiler
+// treats such pointers as generic and loads through them as flat loads, which
+// also count against the LDS wait counter: an LDS wait in a kernel's multiply
+// step then also waits for the next step's operand loads. gmem() marks the
+// pointer as device (global) memory, so only the loads' own wait covers them.
+template <class T>
+using gptr = const __attribute__((address_space(1))) T *;
+template <class T>
+__device__ inline gptr<T> gmem(const T *p)
+{
+    return (gptr<T>)p;
+}
+template <class T>
+using gout = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ inline gout<T> gmem_w(T *p)
+{
+    return (gout<T>)p;
+}
+
 // Device view of one symmetrisation map (R and R^T in CSR form).
 struct SymDev {
     const int *rptr, *rcol;

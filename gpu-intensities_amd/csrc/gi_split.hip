@@ -40,6 +40,7 @@ namespace gi {
 constexpr int kRowB = 64;  // LDS row pitch of the digit tiles, bytes
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
 
 __device__ inline int digit_exp(double m)
 {
@@ -322,9 +323,12 @@ template <int S, int NC, int WMAX>
 __device__ inline void g_split_passes(const GSplitJob &J, int d, int u, double um,
                                       const double *rec, const int (*cls)[kClsLds], double R)
 {
-    int8_t *D = J.D + (size_t)u * J.GtotB;
+    const gout<int8_t> D = gmem_w(J.D) + (size_t)u * J.GtotB;
     const int W = J.W, Gt = J.GtotB;
-    const double *bnd = J.bnd[d];
+    const gptr<double> bnd = gmem(J.bnd[d]);
+    const gptr<int> ixp = gmem(J.idx[d]);
+    const gptr<double> wtp = gmem(J.wt[d]);
+    const gout<int> E = gmem_w(J.E) + (size_t)u * J.EGtot;
     int co = 0;
     for (int g0 = threadIdx.x * 4; g0 < Gt; g0 += 4096 * NC) {
         int4 ix[NC][WMAX];
@@ -344,9 +348,11 @@ __device__ inline void g_split_passes(const GSplitJob &J, int d, int u, double u
             for (int w = 0; w < WMAX; w++)
                 if (w < W) {
                     const size_t o = (size_t)w * Gt + g;
-                    ix[c][w] = *(const int4 *)(J.idx[d] + o);
-                    wa[c][w] = *(const double2 *)(J.wt[d] + o);
-                    wb[c][w] = *(const double2 *)(J.wt[d] + o + 2);
+                    const v4i t = *(gptr<v4i>)(ixp + o);
+                    const d2v a = *(gptr<d2v>)(wtp + o), b = *(gptr<d2v>)(wtp + o + 2);
+                    ix[c][w] = make_int4(t.x, t.y, t.z, t.w);
+                    wa[c][w] = make_double2(a.x, a.y);
+                    wb[c][w] = make_double2(b.x, b.y);
                 }
         }
 #pragma unroll
@@ -368,9 +374,10 @@ __device__ inline void g_split_passes(const GSplitJob &J, int d, int u, double u
             for (int r = 0; r < 4; r++) to_digits<S>(v[r], e, dg[r], R);
 #pragma unroll
             for (int q = 0; q < S; q++)
-                *(int *)(D + q * J.plane + g) = (dg[0][q] & 255) | ((dg[1][q] & 255) << 8) |
-                                                ((dg[2][q] & 255) << 16) | ((dg[3][q] & 255) << 24);
-            if (col[c] >= 0) J.E[(size_t)u * J.EGtot + col[c]] = e;
+                *(gout<int>)(D + q * J.plane + g) = (dg[0][q] & 255) | ((dg[1][q] & 255) << 8) |
+                                                    ((dg[2][q] & 255) << 16) |
+                                                    ((dg[3][q] & 255) << 24);
+            if (col[c] >= 0) E[col[c]] = e;
         }
     }
 }
@@ -389,7 +396,7 @@ __global__ __launch_bounds__(1024) void k_build_g_split_lds(const GSplitJob *__r
     const int ji = find_tile_job(ubegin, njobs, blockIdx.x);
     const GSplitJob &J = jobs[ji];
     const int u = blockIdx.x - ubegin[ji];
-    const double *src = J.uptr[u];
+    const gptr<double> src = gmem(J.uptr[u]);
     for (int i = threadIdx.x; i < J.nsize; i += 1024) rec[i] = src[i];
     if (threadIdx.x < min(J.nclasses, kClsLds)) {
         const ClassDev &cd = J.classes[threadIdx.x];
@@ -625,15 +632,16 @@ __global__ __launch_bounds__(256) void k_halfls_split(int kpad_all, MuSrc src,
 #pragma unroll
             for (int l = 0; l < S; l++) acc[x][y][l] = v4i{0, 0, 0, 0};
     v4i ra[S], rb[S];
+    const gptr<int> seg_a = gmem(cd.seg_a), seg_pad = gmem(cd.seg_pad), seg_n = gmem(cd.seg_n);
     auto fetch = [&](int se, int kk) {
         const long long arow =
-            ((long long)cd.seg_a[se] * src.ld + cd.off + m0 + lr - src.v0) * kpad_all +
-            cd.seg_pad[2 * se + 1] + kk + lp;
-        const long long bofs = bcol + cd.seg_pad[2 * se] + kk + lp;
+            ((long long)seg_a[se] * src.ld + cd.off + m0 + lr - src.v0) * kpad_all +
+            seg_pad[2 * se + 1] + kk + lp;
+        const long long bofs = bcol + seg_pad[2 * se] + kk + lp;
 #pragma unroll
         for (int q = 0; q < S; q++) {
             ra[q] = mok ? *(const v4i *)(mud + q * muplane + arow) : v4i{0, 0, 0, 0};
-            rb[q] = nok ? *(const v4i *)(J.Gd + q * J.gplane + bofs) : v4i{0, 0, 0, 0};
+            rb[q] = nok ? *(gptr<v4i>)(gmem(J.Gd) + q * J.gplane + bofs) : v4i{0, 0, 0, 0};
         }
     };
     int e = 0, kk = 0;
@@ -646,7 +654,7 @@ __global__ __launch_bounds__(256) void k_halfls_split(int kpad_all, MuSrc src,
         }
         __syncthreads();
         kk += 64;
-        if (kk >= cd.seg_n[e]) {
+        if (kk >= seg_n[e]) {
             e++;
             kk = 0;
         }
@@ -755,7 +763,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         const long long bo = bofs + s_b[ST];                                                   \
         _Pragma("unroll") for (int q = 0; q < S; q++) {                                        \
             RA[q] = mok ? *(const v4i *)(mud + q * muplane + ao) : v4i{0, 0, 0, 0};           \
-            RB[q] = nok ? *(const v4i *)(J.Gd + q * J.gplane + bo) : v4i{0, 0, 0, 0};          \
+            RB[q] = nok ? *(gptr<v4i>)(gmem(J.Gd) + q * J.gplane + bo) : v4i{0, 0, 0, 0};          \
         }                                                                                      \
     } while (0)
 #define GI_STAGE(RA, RB)                                                                       \
@@ -868,17 +876,18 @@ __global__ __launch_bounds__(256, 1) void k_halfls_split_wide(
 #pragma unroll
             for (int l = 0; l < S; l++) acc[x][y][l] = v4i{0, 0, 0, 0};
     v4i ra0[S], ra1[S], rb[S];
+    const gptr<int> seg_a = gmem(cd.seg_a), seg_pad = gmem(cd.seg_pad), seg_n = gmem(cd.seg_n);
     auto fetch = [&](int se, int kk) {
         const long long arow =
-            ((long long)cd.seg_a[se] * src.ld + cd.off + m0 + lr - src.v0) * kpad_all +
-            cd.seg_pad[2 * se + 1] + kk + lp;
-        const long long bofs = bcol + cd.seg_pad[2 * se] + kk + lp;
+            ((long long)seg_a[se] * src.ld + cd.off + m0 + lr - src.v0) * kpad_all +
+            seg_pad[2 * se + 1] + kk + lp;
+        const long long bofs = bcol + seg_pad[2 * se] + kk + lp;
 #pragma unroll
         for (int q = 0; q < S; q++) {
             ra0[q] = mok0 ? *(const v4i *)(mud + q * muplane + arow) : v4i{0, 0, 0, 0};
             ra1[q] = mok1 ? *(const v4i *)(mud + q * muplane + arow + 64LL * kpad_all)
                           : v4i{0, 0, 0, 0};
-            rb[q] = nok ? *(const v4i *)(J.Gd + q * J.gplane + bofs) : v4i{0, 0, 0, 0};
+            rb[q] = nok ? *(gptr<v4i>)(gmem(J.Gd) + q * J.gplane + bofs) : v4i{0, 0, 0, 0};
         }
     };
     int e = 0, kk = 0;
@@ -892,7 +901,7 @@ __global__ __launch_bounds__(256, 1) void k_halfls_split_wide(
         }
         __syncthreads();
         kk += 64;
-        if (kk >= cd.seg_n[e]) {
+        if (kk >= seg_n[e]) {
             e++;
             kk = 0;
         }
@@ -985,9 +994,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #define GI_FETCH(RA, RB, K0)                                                                   \
     do {                                                                                       \
         _Pragma("unroll") for (int q = 0; q < S; q++) {                                        \
-            RA[q] = fok ? *(const v4i *)(a.Ud + q * a.uplane + arow + (K0) + lp)                 \
+            RA[q] = fok ? *(gptr<v4i>)(gmem(a.Ud) + q * a.uplane + arow + (K0) + lp)                 \
                         : v4i{0, 0, 0, 0};                                                     \
-            RB[q] = nok ? *(const v4i *)(a.Hd + q * a.hplane + brow + (K0) + lp)                 \
+            RB[q] = nok ? *(gptr<v4i>)(gmem(a.Hd) + q * a.hplane + brow + (K0) + lp)                 \
                         : v4i{0, 0, 0, 0};                                                     \
         }                                                                                      \
     } while (0)
@@ -1026,8 +1035,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < S; q++) {
-            ra[q] = fok ? *(const v4i *)(a.Ud + q * a.uplane + arow + k0 + lp) : v4i{0, 0, 0, 0};
-            rb[q] = nok ? *(const v4i *)(a.Hd + q * a.hplane + brow + k0 + lp) : v4i{0, 0, 0, 0};
+            ra[q] = fok ? *(gptr<v4i>)(gmem(a.Ud) + q * a.uplane + arow + k0 + lp) : v4i{0, 0, 0, 0};
+            rb[q] = nok ? *(gptr<v4i>)(gmem(a.Hd) + q * a.hplane + brow + k0 + lp) : v4i{0, 0, 0, 0};
         }
     };
     if (kb < ke) fetch(kb);  // a part past the end of K (kb >= ke) stays zero
