@@ -36,7 +36,8 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_params", "gi_uplo
            "gi_autotune", "gi_set_split_digits_each", "gi_sweep_create", "gi_sweep_run", "gi_sweep_candidates",
            "gi_sweep_destroy", "gi_amp_width", "gi_set_split_format", "gi_get_split_format",
            "gi_set_mu_budget", "gi_mu_parts", "gi_half_ls_noalloc",
-           "gi_set_path_limits", "gi_set_vector_budget", "gi_streamed_blocks"]
+           "gi_set_path_limits", "gi_set_vector_budget", "gi_streamed_blocks",
+           "gi_set_persist", "gi_get_persist"]
 # (TROVE_EXPORTS below: the symbols of include/gi_trove.h)
 
 
@@ -130,6 +131,8 @@ def lib():
         L.gi_set_tuning.argtypes = [vp, C.POINTER(GiTuning)]
         L.gi_get_tuning.argtypes = [vp, C.POINTER(GiTuning)]
         L.gi_set_path_limits.argtypes = [vp, ci, ci, ci, ci, ci]
+        L.gi_set_persist.argtypes = [vp, ci]
+        L.gi_get_persist.argtypes = [vp, P_i]
         L.gi_set_vector_budget.argtypes = [vp, cll]
         L.gi_streamed_blocks.argtypes = [vp]
         L.gi_autotune.argtypes = [vp, ci, ci, P_i, C.POINTER(GiLines), ci, C.POINTER(GiTuning),
@@ -371,6 +374,16 @@ class Engine:
         kernels (0 restores a default)."""
         self._chk(lib().gi_set_path_limits(self.h, kpart_len, rec_lds, split2_steps,
                                            split2_segs, fold_split_rows), "gi_set_path_limits")
+
+    def set_persist(self, on):
+        """gi_set_persist: FP64 batched rounds as one persistent launch (1) or
+        one launch per stage (0)."""
+        self._chk(lib().gi_set_persist(self.h, int(on)), "gi_set_persist")
+
+    def get_persist(self):
+        v = np.zeros(1, np.int32)
+        self._chk(lib().gi_get_persist(self.h, _p(v, P_i)), "gi_get_persist")
+        return int(v[0])
 
     def get_tuning(self):
         t = GiTuning()

@@ -21,6 +21,8 @@
 // holds row 4*(l/16) + r).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "gi_kernels.h"
 
 namespace gi {
@@ -309,22 +311,17 @@ __global__ __launch_bounds__(256) void k_build_g_map4(const GMapJob *__restrict_
 // K5 over jobs: H_co[m, (u, j)] = sum_k mu-block[m, k] * G[k, (u, j)] on the
 // matrix cores. K runs segment by segment (a tile never straddles two mu
 // blocks).
-template <int BM_, int BN_, int NW_ = 4>
-__global__ __launch_bounds__(64 * NW_) __attribute__((amdgpu_waves_per_eu(NW_ == 8 ? 4 : 1)))
-void k_halfls_jobs(MuSrc src, const HalfJob *__restrict__ jobs, const int *__restrict__ tbegin,
-                   int njobs, int ntiles)
+// One output tile (tile `local` of job J). Two LDS buffers: step s multiplies
+// from buffer s & 1 while the loads of step s + 1 are in flight, then stores
+// them into the other buffer (one barrier per K step). Ends with a barrier
+// after the last read of LDS.
+template <int BM_, int BN_, int NW_>
+__device__ __attribute__((always_inline)) inline void halfls_tile(const MuSrc &src, const HalfJob &J, int local,
+                                   double (*__restrict__ As)[BK * TileW<BM_, BN_, NW_>::PA],
+                                   double (*__restrict__ Bs)[BK * TileW<BM_, BN_, NW_>::PB])
 {
     using T = TileW<BM_, BN_, NW_>;
-    // two LDS buffers: step s multiplies from buffer s & 1 while the loads of
-    // step s + 1 are in flight, then stores them into the other buffer (one
-    // barrier per K step)
-    __shared__ double As[2][BK * T::PA];
-    __shared__ double Bs[2][BK * T::PB];
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    if (tile >= ntiles) return;
-    const HalfJob &J = jobs[find_job(tbegin, njobs, tile)];
     const ClassDev &cd = J.cd;
-    const int local = tile - J.tile_begin;
     const int m0 = J.m_lo + (local % J.mt) * BM_, n0 = (local / J.mt) * BN_;
     const int mhi = J.m_hi, ld = src.ld;
     const double *__restrict__ mu = (const double *)src.mu;
@@ -405,26 +402,34 @@ void k_halfls_jobs(MuSrc src, const HalfJob *__restrict__ jobs, const int *__res
             }
 }
 
+template <int BM_, int BN_, int NW_ = 4>
+__global__ __launch_bounds__(64 * NW_) __attribute__((amdgpu_waves_per_eu(NW_ == 8 ? 4 : 1)))
+void k_halfls_jobs(MuSrc src, const HalfJob *__restrict__ jobs, const int *__restrict__ tbegin,
+                   int njobs, int ntiles)
+{
+    using T = TileW<BM_, BN_, NW_>;
+    __shared__ double As[2][BK * T::PA];
+    __shared__ double Bs[2][BK * T::PB];
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
+    const HalfJob &J = jobs[find_job(tbegin, njobs, tile)];
+    halfls_tile<BM_, BN_, NW_>(src, J, tile - J.tile_begin, As, Bs);
+}
+
 // ---------------------------------------------------------------------------
 // K6 over chunks: tile t of chunk c covers rows f0 = Lo + (local % rt) * BM
 // and columns n0 = (local / rt) * BN.
+// One correlation tile (tile `local` of chunk a). Returns at once (after one
+// barrier) for a tile outside every window; otherwise ends with a barrier
+// after the last read of LDS.
 template <int BM_, int BN_>
-__global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict__ args,
-                                                   const int *__restrict__ cbegin, int nchunks,
-                                                   int ntiles)
+__device__ __attribute__((always_inline)) inline void corr_tile(const CorrMMArgs &a, int local, double *__restrict__ As,
+                                 double *__restrict__ Bs, int &s_need)
 {
     using T = Tile<BM_, BN_>;
-    __shared__ double As[BK * T::PA];
-    __shared__ double Bs[BK * T::PB];
-    __shared__ int s_need;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    if (tile >= ntiles) return;
-    const int ci = find_job(cbegin, nchunks, tile);
-    const CorrMMArgs &a = args[ci];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int ncols = a.nitems * a.ncb;
     const int rt = (a.Hi - a.Lo + BM_ - 1) / BM_, per = rt * ((ncols + BN_ - 1) / BN_);
-    const int local = tile - cbegin[ci];
     const int kp = local / per, l2 = local - kp * per;
     const int f0 = a.Lo + (l2 % rt) * BM_;  // block-level index of the first row
     const int n0 = (l2 / rt) * BN_;
@@ -513,10 +518,25 @@ __global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict_
             }
 }
 
+template <int BM_, int BN_>
+__global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict__ args,
+                                                   const int *__restrict__ cbegin, int nchunks,
+                                                   int ntiles)
+{
+    using T = Tile<BM_, BN_>;
+    __shared__ double As[BK * T::PA];
+    __shared__ double Bs[BK * T::PB];
+    __shared__ int s_need;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
+    const int ci = find_job(cbegin, nchunks, tile);
+    corr_tile<BM_, BN_>(args[ci], tile - cbegin[ci], As, Bs, s_need);
+}
+
 // ---------------------------------------------------------------------------
 // Degeneracy sum, Einstein A, intensity and emit test per candidate, from the
 // amplitudes K6 stored (cuda_host.cu:1388-1407).
-__device__ inline void intensity_one(const IntensArgs &a, int c)
+__device__ __attribute__((always_inline)) inline void intensity_one(const IntensArgs &a, int c)
 {
     if (c >= a.ncand) return;
     int lo = 0, hi = a.nitems - 1;
@@ -589,6 +609,137 @@ __global__ __launch_bounds__(256) void k_intensity_jobs(const IntensArgs *__rest
         if (begin[mid] <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
     }
     intensity_one(jobs[lo], (blockIdx.x - begin[lo]) * 256 + threadIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent round (see PersistRound).
+
+// Every thread's stores of the task are made visible device-wide, then one
+// thread counts the task done.
+__device__ inline void persist_signal(int *cnt)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One thread polls until *cnt reaches target, then every thread acquires what
+// the signalling tasks stored. A wait that exceeds kPersistSpinMax polls sets
+// *err; once *err is set every wait of the launch returns at once, so the
+// launch ends after one such wait.
+__device__ inline void persist_wait(const int *cnt, int target, int *err)
+{
+    if (threadIdx.x == 0 && target > 0) {
+        long long n = 0;
+        while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++n > kPersistSpinMax ||
+                __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// Three workgroups per CU (two for 128 x 128 half-ls tiles), as the
+// half-ls kernel alone keeps.
+template <int HM, int HN, int CM, int CN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HM * HN > 8192 ? 2 : 3)))
+void k_round_persist(PersistRound p)
+{
+    using TH = TileW<HM, HN, 4>;
+    using TC = Tile<CM, CN>;
+    constexpr int kH = 2 * BK * (TH::PA + TH::PB), kC = BK * (TC::PA + TC::PB);
+    __shared__ double sm[kH > kC ? kH : kC];
+    __shared__ int s_task, s_need;
+    int *done = p.sync, *ticket = p.sync + 3 * p.nch;
+    const int total = p.nh + p.nf + p.nc + p.ni;
+    for (;;) {
+        if (threadIdx.x == 0) s_task = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        int t = s_task;
+        __syncthreads();
+        if (t >= total) return;
+        if (t < p.nh) {
+            const int j = find_job(p.hbeg, p.nhj, t);
+            halfls_tile<HM, HN, 4>(p.src, p.hj[j], t - p.hbeg[j],
+                                   (double(*)[BK * TH::PA])sm,
+                                   (double(*)[BK * TH::PB])(sm + 2 * BK * TH::PA));
+            persist_signal(done + p.hch[j]);
+            continue;
+        }
+        t -= p.nh;
+        if (t < p.nf) {
+            const int j = find_job(p.fbeg, p.nfj, t);
+            const int ch = p.fch[j];
+            persist_wait(done + ch, p.need[ch], p.err);
+            fold_block(p.fj[j], t - p.fbeg[j]);
+            persist_signal(done + p.nch + ch);
+            continue;
+        }
+        t -= p.nf;
+        if (t < p.nc) {
+            const int ch = find_job(p.cbeg, p.ncj, t);
+            persist_wait(done + p.nch + ch, p.need[p.nch + ch], p.err);
+            corr_tile<CM, CN>(p.cj[ch], t - p.cbeg[ch], sm, sm + BK * TC::PA, s_need);
+            persist_signal(done + 2 * p.nch + ch);
+            continue;
+        }
+        t -= p.nc;
+        const int j = find_job(p.ibeg, p.nij, t);
+        const int ch = p.ich[j];
+        persist_wait(done + 2 * p.nch + ch, p.need[2 * p.nch + ch], p.err);
+        intensity_one(p.ij[j], (t - p.ibeg[j]) * 256 + threadIdx.x);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_zero_ints(int *p, int n)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+void launch_zero_ints(hipStream_t st, int *p, int n)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_zero_ints, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+}
+
+template <int HM, int HN, int CM, int CN>
+static int persist_launch(hipStream_t st, const PersistRound &p, int ncu)
+{
+    static int per_cu = 0;  // resident workgroups per CU (the same on every gfx950)
+    if (per_cu == 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_round_persist<HM, HN, CM, CN>,
+                                                          256, 0) != hipSuccess || n < 1)
+            n = 1;
+        per_cu = n;
+    }
+    const int total = p.nh + p.nf + p.nc + p.ni;
+    const int grid = std::min(total, per_cu * std::max(ncu, 1));
+    if (grid <= 0) return 0;
+    hipLaunchKernelGGL((k_round_persist<HM, HN, CM, CN>), dim3(grid), dim3(256), 0, st, p);
+    return grid;
+}
+
+int launch_round_persist(hipStream_t st, int hcfg, int ccfg, const PersistRound &p, int ncu)
+{
+    const int hm = tile_rows(hcfg), hn = tile_cols(hcfg);
+    if (ccfg == 2) {
+        if (hm == 64 && hn == 64) return persist_launch<64, 64, 128, 64>(st, p, ncu);
+        if (hm == 64) return persist_launch<64, 128, 128, 64>(st, p, ncu);
+        if (hn == 64) return persist_launch<128, 64, 128, 64>(st, p, ncu);
+        return persist_launch<128, 128, 128, 64>(st, p, ncu);
+    }
+    if (hm == 64 && hn == 64) return persist_launch<64, 64, 64, 64>(st, p, ncu);
+    if (hm == 64) return persist_launch<64, 128, 64, 64>(st, p, ncu);
+    if (hn == 64) return persist_launch<128, 64, 64, 64>(st, p, ncu);
+    return persist_launch<128, 128, 64, 64>(st, p, ncu);
 }
 
 // ---------------------------------------------------------------------------

@@ -220,6 +220,7 @@ struct Plan {
     bool device_q = false;
     double *qdev = nullptr;
     long long *counts = nullptr;                   // pinned, lines per query
+    int *perr = nullptr;                           // error word of the persistent rounds
     double work[kWorkStats] = {};
 };
 
@@ -1087,9 +1088,21 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                                       mc * (sizeof(GBuildJob) + 4),
                                       (size_t)mjobs * (sizeof(HalfJob) + 4),
                                       mc * (sizeof(FoldJob) + 4), mc * (sizeof(CorrMMArgs) + 4),
-                                      mc * (sizeof(IntensArgs) + 4)});
+                                      mc * (sizeof(IntensArgs) + 4), (size_t)mjobs * 4, mc * 4,
+                                      mc * 4, 3 * mc * 4});
     unsigned char *rtab = (unsigned char *)slot_scratch(c, S, "rtab", rtab_cap, &rc);
     if (rc) return rc;
+    // persistent rounds (gi_set_persist): the half-ls, fold, correlation and
+    // intensity of a round in one launch after its G build; not with mu or
+    // eigenvector blocks in host memory, whose copies sit between the stages
+    const bool persist = c->persist && !c->mu_streamed && !P.vstream;
+    const int pccfg = persist_corr_cfg_ok(c->tile_corr) ? c->tile_corr : 0;
+    int *psync = nullptr;
+    if (persist) {
+        psync = (int *)slot_scratch(c, S, "psync", (3 * mc + 2) * 4, &rc);
+        if (rc) return rc;
+        P.perr = psync + 3 * mc + 1;
+    }
     // tile shape of the half-ls products of round [c0, c1): the setting, or
     // with kTileAuto 128 x 128 on eight wavefronts while the round has
     // kAutoTileWaves waves of them (two workgroups per CU), else 64 x 64
@@ -1137,6 +1150,10 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         // global-gather build (the same bits)
         std::vector<GMapJob> mjobs_v, mjobs_g;
         std::vector<int> gbegin, tbegin, mbegin, mubeg;
+        // persistent round: round-local chunk of each half-ls / fold /
+        // intensity job, and per chunk the tasks of each stage
+        const int nch = (int)(rg.second - rg.first);
+        std::vector<int> hch, fch, ich, ptasks(3 * (size_t)nch, 0), none;
         int munits = 0;
         int mt_tiles = 0;
         // global-gather builds four elements per thread when every such map of
@@ -1211,7 +1228,10 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                     j.nunits = ch.nunits; j.Gtot = GP.Gtot_p; j.Htot = GP.Htot;
                     j.tile_begin = tt;
                     j.mt = (oc.n + TM - 1) / TM;
-                    tt += j.mt * ((ch.nunits * (int)oc.cols.size() + TN - 1) / TN);
+                    const int jt = j.mt * ((ch.nunits * (int)oc.cols.size() + TN - 1) / TN);
+                    tt += jt;
+                    hch.push_back((int)(i - rg.first));
+                    ptasks[i - rg.first] += jt;
                     j.uptr = (const double *const *)(d_meta + ch.off_uptr);
                     j.uideg = (const int *)(d_meta + ch.off_uideg);
                     j.r0 = r0; j.r1 = r1;
@@ -1251,6 +1271,8 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                 fjobs.push_back(fj);
                 fbegin.push_back(fblocks);
                 fblocks += fj.xt * ch.nitems * ndegI * ndegF;
+                fch.push_back((int)(i - rg.first));
+                ptasks[nch + (i - rg.first)] += fj.xt * ch.nitems * ndegI * ndegF;
             }
             CorrMMArgs m;
             m.Lo = ch.Lo; m.Hi = ch.Hi; m.nitems = ch.nitems; m.ncb = ndegI * ndegF;
@@ -1280,19 +1302,25 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             P.work[8] += band_exec_flops(P, ch, ndegI * ndegF, BF.nsize);
         }
         // corr tiles: per chunk ceil(rows / CM) x ceil(cols / CN), flattened
-        const int CM = tile_rows(c->tile_corr), CN = tile_cols(c->tile_corr);
+        const int ccfg = persist ? pccfg : c->tile_corr;
+        const int CM = tile_rows(ccfg), CN = tile_cols(ccfg);
         std::vector<int> cbegin(mm.size());
         int ctiles = 0;
         for (size_t i = 0; i < mm.size(); i++) {
             cbegin[i] = ctiles;
             const int rows = mm[i].Hi - mm[i].Lo, cols = mm[i].nitems * mm[i].ncb;
-            if (rows > 0 && cols > 0)
-                ctiles += ((rows + CM - 1) / CM) * ((cols + CN - 1) / CN) * mm[i].kparts;
+            if (rows > 0 && cols > 0) {
+                const int ct = ((rows + CM - 1) / CM) * ((cols + CN - 1) / CN) * mm[i].kparts;
+                ctiles += ct;
+                ptasks[2 * nch + i] = ct;
+            }
         }
         std::vector<IntensArgs> ijobs;
         std::vector<int> ibegin;
         const int iblocks = round_intensity_jobs(c, P, rg.first, rg.second, d_meta, K, flag, cand,
                                                  ijobs, ibegin);
+        for (size_t i = rg.first; i < rg.second; i++)
+            if (chunks[i].ncand > 0) ich.push_back((int)(i - rg.first));
         TablePack T(rtab);
         const int *b_mv = nullptr, *b_mg = nullptr, *b_gj = nullptr, *b_hl = nullptr,
                   *b_fj = nullptr, *b_mm = nullptr, *b_ij = nullptr;
@@ -1302,6 +1330,13 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         const FoldJob *t_fj = T.add(fjobs, fbegin, &b_fj);
         const CorrMMArgs *t_mm = T.add(mm, cbegin, &b_mm);
         const IntensArgs *t_ij = T.add(ijobs, ibegin, &b_ij);
+        const int *t_hch = nullptr, *t_fch = nullptr, *t_ich = nullptr, *t_need = nullptr, *nb;
+        if (persist) {
+            t_hch = T.add(hch, none, &nb);
+            t_fch = T.add(fch, none, &nb);
+            t_ich = T.add(ich, none, &nb);
+            t_need = T.add(ptasks, none, &nb);
+        }
         if ((rc = send_pack(c, S, T, rtab_cap)) != GI_OK) return rc;
         if (nmj > 0) {
             TL_BEGIN("build_g");
@@ -1321,9 +1356,32 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             launch_build_g_units(S.stream, N, t_gj, b_gj, ngj, gt);
             TL_END();
         }
+        if (persist) {
+            PersistRound pr;
+            pr.src = MuSrc{c->d_mu_perm, (long long)N * N, N, 0};
+            pr.hj = t_hl; pr.hbeg = b_hl; pr.hch = t_hch; pr.nhj = njobs; pr.nh = tt;
+            pr.fj = t_fj; pr.fbeg = b_fj; pr.fch = t_fch; pr.nfj = (int)fjobs.size();
+            pr.nf = fblocks;
+            pr.cj = t_mm; pr.cbeg = b_mm; pr.ncj = (int)mm.size(); pr.nc = ctiles;
+            pr.ij = t_ij; pr.ibeg = b_ij; pr.ich = t_ich; pr.nij = (int)ijobs.size();
+            pr.ni = iblocks;
+            pr.need = t_need;
+            pr.nch = nch;
+            pr.sync = psync;
+            pr.err = P.perr;
+            // counters and ticket cleared by a kernel (graph-capturable); the
+            // error word once per call, in the first round
+            launch_zero_ints(S.stream, psync, 3 * nch + 1);
+            if (iround == 0) launch_zero_ints(S.stream, P.perr, 1);
+            TL_BEGIN("round_persist");
+            P.work[14] += tt;
+            P.work[15] += ctiles;
+            launch_round_persist(S.stream, hcfg, pccfg, pr, c->ncu);
+            TL_END();
+        }
         // the half-ls products: one launch over resident mu from the round's
         // tables, or one launch per mu part with its own table
-        if (njobs > 0 && !c->mu_streamed) {
+        if (njobs > 0 && !c->mu_streamed && !persist) {
             TL_BEGIN("half_ls_mfma");
             P.work[14] += tt;
             launch_halfls_jobs(S.stream, hcfg, MuSrc{c->d_mu_perm, (long long)N * N, N, 0}, t_hl,
@@ -1349,17 +1407,19 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             TL_END();
             HIPCHK(hipEventRecord(S.mu_ev[2 + b], S.stream));
         }
-        if (!fjobs.empty()) {
+        if (!fjobs.empty() && !persist) {
             TL_BEGIN("fold");
             launch_fold_jobs(S.stream, t_fj, b_fj, (int)fjobs.size(), fblocks);
             TL_END();
         }
-        TL_BEGIN("corr_mfma");
-        P.work[15] += ctiles;
-        launch_corr_mfma(S.stream, c->tile_corr, t_mm, b_mm, (int)mm.size(), ctiles);
-        TL_END();
+        if (!persist) {
+            TL_BEGIN("corr_mfma");
+            P.work[15] += ctiles;
+            launch_corr_mfma(S.stream, c->tile_corr, t_mm, b_mm, (int)mm.size(), ctiles);
+            TL_END();
+        }
         if (P.vstream && (rc = vec_stage_done(c, S, P, rg.first)) != GI_OK) return rc;
-        if (!ijobs.empty()) {
+        if (!ijobs.empty() && !persist) {
             TL_BEGIN("intensity");
             launch_intensity_jobs(S.stream, t_ij, b_ij, (int)ijobs.size(), iblocks);
             TL_END();
@@ -2000,6 +2060,17 @@ static int drain_rounds(gi_ctx *c, Slot &S, const Plan &P, gi_lines *out, long l
     return GI_OK;
 }
 
+// After the plan's work has finished: whether a persistent round gave up a
+// dependency wait (its results are then void).
+static int persist_check(gi_ctx *c, const Plan &P)
+{
+    if (!P.perr) return GI_OK;
+    int e = 0;
+    HIPCHK(hipMemcpy(&e, P.perr, 4, hipMemcpyDeviceToHost));
+    if (e) return fail(c, GI_EHIP, "persistent round: a dependency wait gave up");
+    return GI_OK;
+}
+
 // The deck job's query (temperature, Q, thresholds of gi_set_params).
 static Query job_query(const gi_ctx *c)
 {
@@ -2057,6 +2128,7 @@ int compute(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out)
         HIPCHK(hipStreamSynchronize(S.stream));
         tl.collect(c);
     }
+    if ((rc = persist_check(c, P)) != GI_OK) return rc;
     for (int i = 0; i < kWorkStats; i++) c->work[i] = P.work[i];
     if (!P.pipe) nout = P.counts[0];
     c->work[2] = (double)nout;
@@ -2338,6 +2410,13 @@ static void finish_batch(Server &sv, Inflight &b)
         rc = GI_EHIP;
         err = "request batch failed on the device";
     }
+    if (rc == GI_OK && b.plan->perr) {
+        int e = 0;
+        if (hipMemcpy(&e, b.plan->perr, 4, hipMemcpyDeviceToHost) != hipSuccess || e) {
+            rc = GI_EHIP;
+            err = "persistent round: a dependency wait gave up";
+        }
+    }
     if (rc == GI_OK) {
         for (size_t k = 0; k < b.reqs.size(); k++) {
             Request &R = *b.reqs[k];
@@ -2576,6 +2655,7 @@ static int sweep_run(gi_sweep *w, double T, double Q, double thS, double thI, gi
     long long nout = 0;
     if ((rc = drain_rounds(c, S, P, out, &nout)) != GI_OK) return rc;
     HIPCHK(hipStreamSynchronize(S.stream));
+    if ((rc = persist_check(c, P)) != GI_OK) return rc;
     out->n = nout;
     return GI_OK;
 }
@@ -2641,6 +2721,23 @@ int gi_set_path_limits(gi_ctx *c, int kpart_len, int rec_lds, int split2_steps,
     c->lim_split2_steps = d;
     c->lim_split2_segs = e;
     c->lim_fold_split = f;
+    return GI_OK;
+}
+
+int gi_set_persist(gi_ctx *c, int on)
+{
+    if (!c) return GI_EINVAL;
+    if (c->server || !c->sweeps.empty())
+        return fail(c, GI_ESTATE, "gi_set_persist while the server runs or sweeps exist");
+    if (on != 0 && on != 1) return fail(c, GI_EINVAL, "gi_set_persist: 0 or 1");
+    c->persist = on;
+    return GI_OK;
+}
+
+int gi_get_persist(gi_ctx *c, int *on)
+{
+    if (!c || !on) return GI_EINVAL;
+    *on = c->persist;
     return GI_OK;
 }
 

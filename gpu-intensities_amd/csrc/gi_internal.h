@@ -258,6 +258,7 @@ struct gi_ctx {
     int pipe_rounds = 4;
     int pipe_share = 50;             // percent of the remaining work per round but the last
     int tile_split = 2;              // split half-ls: 0 = 64 rows, 1 = 128, 2 = 2 steps in flight
+    int persist = 0;                 // FP64 batched rounds in one persistent launch (gi_set_persist)
     // routing thresholds of the batched kernels (gi_set_path_limits; the
     // defaults are the kernels' own limits): split-K part length of the
     // correlation, longest record staged in LDS by the G builds, step and

@@ -338,6 +338,30 @@ __device__ inline void fold_rows(const SymDev &T, const double *__restrict__ Hu,
         acc[r] = v;
     }
 }
+// Block `local` of fold job J (256 threads, kFoldRows rows of one folded
+// column).
+__device__ __attribute__((always_inline)) inline void fold_block(const FoldJob &J, int local)
+{
+    const int xb = local % J.xt, rest = local / J.xt;
+    const int item = rest % J.nitems, cb = rest / J.nitems;
+    const int idegF = cb / J.ndegI, idegI = cb % J.ndegI;
+    const int unit = J.item_unit[item * GI_MAXDEG + idegI];
+    const SymDev &T = idegF ? J.t1 : J.t0;
+    const double *Hu = J.H + (size_t)(unit >= 0 ? unit : 0) * J.SFN;
+    double *out = J.Ht + ((size_t)item * J.ndegI * J.ndegF + cb) * J.nsizeF;
+    int t[4];
+    bool ok[4];
+    double acc[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        t[r] = xb * kFoldRows + r * 256 + (int)threadIdx.x;
+        ok[r] = t[r] < J.nsizeF && unit >= 0;
+    }
+    fold_rows<4>(T, Hu, t, ok, acc);
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+        if (t[r] < J.nsizeF) out[t[r]] = acc[r];
+}
 void launch_fold_jobs(hipStream_t st, const FoldJob *d_jobs, const int *d_begin, int njobs,
                       int nblocks);
 // The digit splits of every chunk of a round: one workgroup per row.
@@ -369,6 +393,43 @@ void launch_fold_split_jobs(hipStream_t st, int S, double R, const FoldSplitJob 
 void launch_intensity_jobs(hipStream_t st, const IntensArgs *d_jobs, const int *d_begin,
                            int njobs, int nblocks);
 void launch_corr_intensity(hipStream_t st, const CorrArgs &a);
+
+// One round of the FP64 batched mode after its G build in one persistent
+// launch: workgroups take tasks in ticket order from one counter, stage by
+// stage (every half-ls tile of the round, then every fold block, correlation
+// tile and intensity block, each stage in chunk order). A task of stage s > 0
+// waits until every stage s - 1 task of its chunk has signalled, so a chunk's
+// fold starts while later chunks' products still run and no stage waits for
+// the whole round's previous stage. Every task depends only on tasks with
+// lower tickets, which were taken by running workgroups, so the waits end
+// whatever the number of resident workgroups; a wait that exceeds
+// kPersistSpinMax polls sets the error word and the round's results are void.
+struct PersistRound {
+    MuSrc src;
+    const HalfJob *hj;
+    const int *hbeg, *hch;           // first tile and round-local chunk of each half-ls job
+    const FoldJob *fj;
+    const int *fbeg, *fch;
+    const CorrMMArgs *cj;            // one per chunk of the round
+    const int *cbeg;
+    const IntensArgs *ij;
+    const int *ibeg, *ich;
+    const int *need;                 // [3][nch]: half-ls tiles, fold blocks, correlation tiles
+    int nhj, nfj, ncj, nij;          // jobs per stage
+    int nh, nf, nc, ni;              // tasks per stage
+    int nch;
+    int *sync;                       // [3][nch] done counts, then the ticket; zeroed per launch
+    int *err;                        // set when a wait gave up (never cleared by the kernel)
+};
+constexpr long long kPersistSpinMax = 1ll << 22;  // polls (a few seconds)
+// hcfg: half-ls tile as tile_rows/tile_cols (4 wavefronts; the 8-wavefront
+// setting runs 128 x 128 on 4); ccfg: correlation tile (0: 64 x 64, 2: 128 x 64).
+// Returns the grid used (workgroups resident on the device, at most the tasks).
+int launch_round_persist(hipStream_t st, int hcfg, int ccfg, const PersistRound &p, int ncu);
+// n ints set to 0 by a kernel (no memset node in a captured graph)
+void launch_zero_ints(hipStream_t st, int *p, int n);
+// whether launch_round_persist has a kernel for the correlation tile setting
+inline bool persist_corr_cfg_ok(int ccfg) { return ccfg == 0 || ccfg == 2; }
 int compact_blocks(long long n);
 void launch_compact(hipStream_t st, long long n, const int *flag, int *counts, long long *offs,
                     LineSoA in, LineSoA out);

@@ -272,27 +272,7 @@ __global__ __launch_bounds__(256) void k_fold_jobs(const FoldJob *__restrict__ j
                                                    const int *__restrict__ begin, int njobs)
 {
     const int j = job_of(begin, njobs, blockIdx.x);
-    const FoldJob &J = jobs[j];
-    const int local = blockIdx.x - begin[j];
-    const int xb = local % J.xt, rest = local / J.xt;
-    const int item = rest % J.nitems, cb = rest / J.nitems;
-    const int idegF = cb / J.ndegI, idegI = cb % J.ndegI;
-    const int unit = J.item_unit[item * GI_MAXDEG + idegI];
-    const SymDev &T = idegF ? J.t1 : J.t0;
-    const double *Hu = J.H + (size_t)(unit >= 0 ? unit : 0) * J.SFN;
-    double *out = J.Ht + ((size_t)item * J.ndegI * J.ndegF + cb) * J.nsizeF;
-    int t[4];
-    bool ok[4];
-    double acc[4];
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-        t[r] = xb * kFoldRows + r * 256 + (int)threadIdx.x;
-        ok[r] = t[r] < J.nsizeF && unit >= 0;
-    }
-    fold_rows<4>(T, Hu, t, ok, acc);
-#pragma unroll
-    for (int r = 0; r < 4; r++)
-        if (t[r] < J.nsizeF) out[t[r]] = acc[r];
+    fold_block(jobs[j], blockIdx.x - begin[j]);
 }
 
 __device__ inline double wave_sum(double v)

@@ -337,6 +337,18 @@ int gi_get_tuning(gi_ctx *ctx, gi_tuning *t);
 int gi_set_path_limits(gi_ctx *ctx, int kpart_len, int rec_lds, int split2_steps,
                        int split2_segs, int fold_split_rows);
 
+/* Persistent rounds (no reference counterpart; the reference launches one
+ * kernel per lower level and pair). on = 1: in GI_MODE_FP64_BATCHED every
+ * pipeline round runs its half linestrength, fold, correlation and intensity
+ * steps as one persistent launch after its G build: resident workgroups take
+ * tasks in ticket order, stage by stage, and a task waits only for the
+ * previous stage's tasks of its own chunk, so the stages of different chunks
+ * overlap instead of each stage draining the device. on = 0 (the default):
+ * one launch per stage. Lines are bitwise the same either way. Calls with mu
+ * or eigenvector blocks in host memory use the per-stage launches. */
+int gi_set_persist(gi_ctx *ctx, int on);
+int gi_get_persist(gi_ctx *ctx, int *on);
+
 /* Autotuner (no reference counterpart): runs gi_compute in `mode` over the
  * given lower levels (NULL = all) `reps` times per setting into `out`,
  * and keeps the fastest setting of each knob: the two tile shapes (FP64
