@@ -757,6 +757,9 @@ void launch_build_g_units(hipStream_t st, int N, const GBuildJob *d_jobs, const 
 
 // G of one unit per workgroup: the unit's record read once into LDS, the map's
 // gathers served from there (in global memory they land on scattered lines).
+// PART: a record longer than nlds keeps its tail in global memory, and a
+// gather beyond nlds reads it there (the same sums in the same order).
+template <bool PART>
 __global__ __launch_bounds__(1024) void k_build_g_map_lds(const GMapJob *__restrict__ jobs,
                                                           const int *__restrict__ ubegin,
                                                           int njobs)
@@ -766,8 +769,10 @@ __global__ __launch_bounds__(1024) void k_build_g_map_lds(const GMapJob *__restr
     const GMapJob &J = jobs[ji];
     const int u = blockIdx.x - ubegin[ji];
     const gptr<double> src = gmem(J.uptr[u]);
-    for (int i = threadIdx.x; i < J.nsize; i += 1024) rec[i] = src[i];
+    const int nl = PART ? min(J.nsize, J.nlds) : J.nsize;
+    for (int i = threadIdx.x; i < nl; i += 1024) rec[i] = src[i];
     __syncthreads();
+    auto at = [&](int i) { return PART ? (i < nl ? rec[i] : src[i]) : rec[i]; };
     const int d = J.uideg[u];
     const gptr<int> ip = gmem(J.idx[d]);
     const gptr<double> wp = gmem(J.wt[d]);
@@ -795,7 +800,7 @@ __global__ __launch_bounds__(1024) void k_build_g_map_lds(const GMapJob *__restr
             double v = 0.0;
 #pragma unroll
             for (int w = 0; w < 4; w++)
-                if (w < W) v += wv[r][w] * rec[ix[r][w]];
+                if (w < W) v += wv[r][w] * at(ix[r][w]);
             G[g] = v;
         }
     }
@@ -810,11 +815,15 @@ void launch_build_g_map4(hipStream_t st, const GMapJob *d_jobs, const int *d_beg
 }
 
 void launch_build_g_map_lds(hipStream_t st, const GMapJob *d_jobs, const int *d_begin,
-                            int njobs, int nunits)
+                            int njobs, int nunits, bool part)
 {
     if (njobs <= 0 || nunits <= 0) return;
-    hipLaunchKernelGGL(k_build_g_map_lds, dim3(nunits), dim3(1024), 0, st, d_jobs, d_begin,
-                       njobs);
+    if (part)
+        hipLaunchKernelGGL(k_build_g_map_lds<true>, dim3(nunits), dim3(1024), 0, st, d_jobs,
+                           d_begin, njobs);
+    else
+        hipLaunchKernelGGL(k_build_g_map_lds<false>, dim3(nunits), dim3(1024), 0, st, d_jobs,
+                           d_begin, njobs);
 }
 
 void launch_build_g_map(hipStream_t st, const GMapJob *d_jobs, const int *d_begin, int njobs,

@@ -1161,10 +1161,9 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         bool m4 = true;
         for (size_t i = rg.first; i < rg.second; i++) {
             const GroupPlan &GP = group_of(c, chunks[i].jindI, chunks[i].gI, chunks[i].jindF);
-            if (!(GP.gW <= 4 && c->blocks[chunks[i].jindI * c->nrep + chunks[i].gI].nsize <=
-                                    c->lim_rec_lds))
-                m4 = m4 && GP.gW <= 4;
+            if (!(GP.gW <= 4 && c->lim_rec_lds > 0)) m4 = m4 && GP.gW <= 4;
         }
+        bool mpart = false;  // some LDS-staged record longer than its LDS part
         const int mrows = m4 ? 1024 : 256;
         std::vector<CorrMMArgs> mm;
         std::vector<double *> Hc(rg.second - rg.first), Htc(rg.second - rg.first);
@@ -1194,7 +1193,9 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                 mj.uideg = (const int *)(d_meta + ch.off_uideg);
                 mj.G = G + og;
                 mj.nsize = c->blocks[ch.jindI * c->nrep + ch.gI].nsize;
-                if (GP.gW <= 4 && mj.nsize <= c->lim_rec_lds) {
+                mj.nlds = std::min(mj.nsize, std::max(c->lim_rec_lds, 0));
+                if (GP.gW <= 4 && c->lim_rec_lds > 0) {
+                    mpart = mpart || mj.nsize > mj.nlds;
                     mj.tile_begin = 0;
                     mjobs_v.push_back(mj);
                     mubeg.push_back(munits);
@@ -1340,7 +1341,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         if ((rc = send_pack(c, S, T, rtab_cap)) != GI_OK) return rc;
         if (nmj > 0) {
             TL_BEGIN("build_g");
-            launch_build_g_map_lds(S.stream, t_mv, b_mv, nmj, munits);
+            launch_build_g_map_lds(S.stream, t_mv, b_mv, nmj, munits, mpart);
             TL_END();
         }
         if (nmg > 0) {
@@ -1518,14 +1519,14 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
     long long *pc_offs = nullptr;
     if ((rc = setup_pipe(c, S, P, rounds, &pc_counts, &pc_offs)) != GI_OK) return rc;
 
-    // G digits of a chunk: one unit per workgroup with its record in LDS when the
-    // record fits, its classes fit the kernel's table and its map is at most 4
-    // wide; else the global-gather build (four rows per thread when every such
-    // map of the round is at most 4 wide). The two builds give the same bits.
+    // G digits of a chunk: one unit per workgroup with its record in LDS (the
+    // first lim_rec_lds doubles of a longer one) when its classes fit the
+    // kernel's table and its map is at most 4 wide; else the global-gather
+    // build (four rows per thread when every such map of the round is at most 4
+    // wide). The builds give the same bits.
     const auto g_lds = [&](const Chunk &ch) {
         const GroupPlan &GP = group_of(c, ch.jindI, ch.gI, ch.jindF);
-        return c->blocks[ch.jindI * c->nrep + ch.gI].nsize <= c->lim_rec_lds &&
-               GP.oc.size() <= (size_t)kClsLds && GP.gW <= 4;
+        return c->lim_rec_lds > 0 && GP.oc.size() <= (size_t)kClsLds && GP.gW <= 4;
     };
     const int srows_tile = c->tile_split == 1 ? 128 : 64;  // k_halfls_split(_wide, 2) row tile
     MuParts MP;
@@ -1542,6 +1543,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
         std::vector<int> gbegin, gbegin_g, tbegin, cbegin;
         int gt = 0, tt = 0, ct = 0, ubeg = 0;
         bool g4 = true;  // every global-gather map of the round at most 4 wide
+        bool gpart = false;  // some LDS-staged record longer than its LDS part
         for (size_t i = rg.first; i < rg.second; i++)
             if (!g_lds(chunks[i])) g4 = g4 && group_of(c, chunks[i].jindI, chunks[i].gI,
                                                        chunks[i].jindF).gW <= 4;
@@ -1580,7 +1582,9 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                 g.plane = (long long)mgd;
                 g.E = Ge + oge;
                 g.nsize = BI.nsize;
+                g.nlds = std::min(BI.nsize, std::max(c->lim_rec_lds, 0));
                 if (g_lds(ch)) {
+                    gpart = gpart || g.nsize > g.nlds;
                     gj.push_back(g);
                     gbegin.push_back(ubeg);
                     ubeg += ch.nunits;
@@ -1705,7 +1709,8 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
         if ((rc = send_pack(c, S, T, rtab_cap)) != GI_OK) return rc;
         if (!gj.empty()) {
             TL_BEGIN("build_g");
-            launch_build_g_split_lds(S.stream, SH, c->split_radix, t_gj, b_gj, (int)gj.size(), ubeg);
+            launch_build_g_split_lds(S.stream, SH, c->split_radix, t_gj, b_gj, (int)gj.size(), ubeg,
+                                     gpart);
             TL_END();
         }
         if (!gjg.empty()) {
@@ -2709,8 +2714,9 @@ int gi_set_path_limits(gi_ctx *c, int kpart_len, int rec_lds, int split2_steps,
         *out = v;
         return true;
     };
-    int a, b, d, e, f;
-    if (!pick(kpart_len, 2048, 64, &a) || !pick(rec_lds, kRecLds, 1, &b) ||
+    int a, b = -1, d, e, f;
+    // rec_lds -1: no record staged in LDS (the global-gather G builds)
+    if (!pick(kpart_len, 2048, 64, &a) || (rec_lds != -1 && !pick(rec_lds, kRecLds, 1, &b)) ||
         !pick(split2_steps, kSplitMaxSteps, 1, &d) || !pick(split2_segs, kSplitMaxSegs, 1, &e) ||
         !pick(fold_split_rows, kFoldSplitMax, 1, &f))
         return fail(c, GI_EINVAL, "gi_set_path_limits: a limit outside [minimum, default]");

@@ -154,25 +154,27 @@ struct GMapJob {
     const int *uideg;
     double *G;
     int nsize;                       // record length of the lower block
+    int nlds;                        // leading doubles of a record staged in LDS (<= kRecLds)
 };
 constexpr int kGMapMaxW = 16;
 constexpr int kClsLds = 32;  // output classes of a group that k_build_g_split_lds keeps in LDS
 constexpr int kGUnits = 16;
-// Records up to this many doubles are staged in LDS by the one-unit-per-workgroup
-// G builds (launch_build_g_map_lds, launch_build_g_split_lds): 158 KB of the
-// 160 KB a gfx950 workgroup can hold, leaving room for the split build's class
-// table.
+// Doubles of a record the one-unit-per-workgroup G builds stage in LDS
+// (launch_build_g_map_lds, launch_build_g_split_lds): 158 KB of the 160 KB a
+// gfx950 workgroup can hold, leaving room for the split build's class table.
+// Longer records keep their tail in global memory.
 constexpr int kRecLds = 20224;
 void launch_build_g_map(hipStream_t st, const GMapJob *d_jobs, const int *d_begin, int njobs,
                         int ntiles);
-// One workgroup per unit, its record in LDS (maps of width <= 4, records of <=
-// kRecLds doubles); d_begin = first unit of each job.
+// One workgroup per unit, the first nlds doubles of its record in LDS (maps of
+// width <= 4); d_begin = first unit of each job. part: some job's records are
+// longer than its nlds, and their gathers beyond it read global memory.
 // k_build_g_map for maps of width <= 4 with 1024 G elements per tile (gtiles
 // counts 1024-element tiles).
 void launch_build_g_map4(hipStream_t st, const GMapJob *d_jobs, const int *d_begin, int njobs,
                          int ntiles);
 void launch_build_g_map_lds(hipStream_t st, const GMapJob *d_jobs, const int *d_begin,
-                            int njobs, int nunits);
+                            int njobs, int nunits, bool part);
 void launch_build_g_units(hipStream_t st, int N, const GBuildJob *d_jobs, const int *d_begin,
                           int njobs, int ntiles);
 
@@ -190,6 +192,7 @@ struct GSplitJob {
     long long plane;
     int *E;                          // this chunk's column exponents [unit][EGtot]
     int nsize;                       // record length of the lower block
+    int nlds;                        // leading doubles of a record staged in LDS (<= kRecLds)
 };
 struct SplitHalfJob {
     ClassDev cd;
@@ -231,10 +234,11 @@ void launch_split_mu(hipStream_t st, int S, double R, int N, int ld, int nrows, 
 // (k_build_g_split4); otherwise 256 rows per tile.
 void launch_build_g_split(hipStream_t st, int S, double R, const GSplitJob *d_jobs, const int *d_begin,
                           int njobs, int ntiles, bool four);
-// One workgroup per unit: its record in LDS, then every G row of the unit
-// (maps of width <= 4, records of <= kRecLds doubles); d_begin = first unit of each job.
+// One workgroup per unit: the first nlds doubles of its record in LDS, then
+// every G row of the unit (maps of width <= 4); d_begin = first unit of each
+// job; part as for launch_build_g_map_lds.
 void launch_build_g_split_lds(hipStream_t st, int S, double R, const GSplitJob *d_jobs,
-                              const int *d_begin, int njobs, int nunits);
+                              const int *d_begin, int njobs, int nunits, bool part);
 // cfg 0: 64 x 64 tiles; 1: 128-row tiles (SplitHalfJob.mt counts 128-row
 // tiles); 2: 64 x 64 with two K steps of loads in flight, for classes of at
 // most kSplitMaxSteps K steps and kSplitMaxSegs segments and at most 5 digits

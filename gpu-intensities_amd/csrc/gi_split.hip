@@ -319,10 +319,13 @@ __global__ __launch_bounds__(256) void k_build_g_split4(const GSplitJob *__restr
 // rows of one column per thread and pass (chunks 4096 rows apart), every map
 // load of the pass in flight before the first gather from the record in LDS
 // (maps of width <= WMAX).
-template <int S, int NC, int WMAX>
+template <int S, int NC, int WMAX, bool PART>
 __device__ inline void g_split_passes(const GSplitJob &J, int d, int u, double um,
-                                      const double *rec, const int (*cls)[kClsLds], double R)
+                                      const double *rec, gptr<double> src, int nl,
+                                      const int (*cls)[kClsLds], double R)
 {
+    // a record longer than nl keeps its tail in global memory (PART)
+    auto at = [&](int i) { return PART ? (i < nl ? rec[i] : src[i]) : rec[i]; };
     const gout<int8_t> D = gmem_w(J.D) + (size_t)u * J.GtotB;
     const int W = J.W, Gt = J.GtotB;
     const gptr<double> bnd = gmem(J.bnd[d]);
@@ -364,10 +367,10 @@ __device__ inline void g_split_passes(const GSplitJob &J, int d, int u, double u
 #pragma unroll
             for (int w = 0; w < WMAX; w++)
                 if (w < W) {
-                    v[0] += wa[c][w].x * rec[ix[c][w].x];
-                    v[1] += wa[c][w].y * rec[ix[c][w].y];
-                    v[2] += wb[c][w].x * rec[ix[c][w].z];
-                    v[3] += wb[c][w].y * rec[ix[c][w].w];
+                    v[0] += wa[c][w].x * at(ix[c][w].x);
+                    v[1] += wa[c][w].y * at(ix[c][w].y);
+                    v[2] += wb[c][w].x * at(ix[c][w].z);
+                    v[3] += wb[c][w].y * at(ix[c][w].w);
                 }
             int dg[4][S];
 #pragma unroll
@@ -385,8 +388,9 @@ __device__ inline void g_split_passes(const GSplitJob &J, int d, int u, double u
 // G digits of one unit per workgroup: the unit's record is read once, with
 // coalesced loads, into LDS, and the gathers of the record map read it there
 // (they land on scattered cache lines when they read global memory); four
-// consecutive rows of one column per thread.
-template <int S>
+// consecutive rows of one column per thread. PART: records longer than nlds
+// keep their tail in global memory, where the gathers beyond nlds read it.
+template <int S, bool PART>
 __global__ __launch_bounds__(1024) void k_build_g_split_lds(const GSplitJob *__restrict__ jobs,
                                                             const int *__restrict__ ubegin,
                                                             int njobs, double R)
@@ -397,7 +401,8 @@ __global__ __launch_bounds__(1024) void k_build_g_split_lds(const GSplitJob *__r
     const GSplitJob &J = jobs[ji];
     const int u = blockIdx.x - ubegin[ji];
     const gptr<double> src = gmem(J.uptr[u]);
-    for (int i = threadIdx.x; i < J.nsize; i += 1024) rec[i] = src[i];
+    const int nl = PART ? min(J.nsize, J.nlds) : J.nsize;
+    for (int i = threadIdx.x; i < nl; i += 1024) rec[i] = src[i];
     if (threadIdx.x < min(J.nclasses, kClsLds)) {
         const ClassDev &cd = J.classes[threadIdx.x];
         cls[0][threadIdx.x] = cd.cbaseB;
@@ -407,8 +412,8 @@ __global__ __launch_bounds__(1024) void k_build_g_split_lds(const GSplitJob *__r
     __syncthreads();
     const int d = J.uideg[u];
     const double um = J.umax[J.urec[u]];
-    if (J.W <= 2) g_split_passes<S, 2, 2>(J, d, u, um, rec, cls, R);
-    else g_split_passes<S, 1, 4>(J, d, u, um, rec, cls, R);
+    if (J.W <= 2) g_split_passes<S, 2, 2, PART>(J, d, u, um, rec, src, nl, cls, R);
+    else g_split_passes<S, 1, 4, PART>(J, d, u, um, rec, src, nl, cls, R);
 }
 
 template <int S, int PER>
@@ -1141,12 +1146,18 @@ void launch_build_g_split(hipStream_t st, int S, double R, const GSplitJob *d_jo
 }
 
 void launch_build_g_split_lds(hipStream_t st, int S, double R, const GSplitJob *d_jobs,
-                              const int *d_begin, int njobs, int nunits)
+                              const int *d_begin, int njobs, int nunits, bool part)
 {
     if (njobs <= 0 || nunits <= 0) return;
-#define L_(s) hipLaunchKernelGGL(k_build_g_split_lds<s>, dim3(nunits), dim3(1024), 0, st, d_jobs, d_begin, njobs, R)
-    GI_SPLIT_DISPATCH(S, L_)
+    if (part) {
+#define L_(s) hipLaunchKernelGGL((k_build_g_split_lds<s, true>), dim3(nunits), dim3(1024), 0, st, d_jobs, d_begin, njobs, R)
+        GI_SPLIT_DISPATCH(S, L_)
 #undef L_
+    } else {
+#define L_(s) hipLaunchKernelGGL((k_build_g_split_lds<s, false>), dim3(nunits), dim3(1024), 0, st, d_jobs, d_begin, njobs, R)
+        GI_SPLIT_DISPATCH(S, L_)
+#undef L_
+    }
 }
 
 void launch_halfls_split(hipStream_t st, int S, double R, int kpad_all, const MuSrc &src,

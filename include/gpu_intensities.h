@@ -324,14 +324,15 @@ int gi_get_tuning(gi_ctx *ctx, gi_tuning *t);
  * tests that reach at small sizes the kernels real sizes select: kpart_len,
  * the correlation K per split-K part in both matrix-core modes
  * (parts = min(16, K / kpart_len); default 1024 for FP64 and 2048 for the
- * split digits, at most 2048, at least 64); rec_lds, the longest eigenvector record the G builds
- * stage in LDS (longer ones use the global-gather builds; default 20224);
+ * split digits, at most 2048, at least 64); rec_lds, the doubles of an eigenvector record the
+ * G builds stage in LDS (gathers beyond them read a longer record in global memory; default
+ * 20224), or -1: no record staged, every chunk takes the global-gather builds;
  * split2_steps / split2_segs, the largest class (64-byte K steps, segments)
  * the two-step split half-ls takes (larger ones run the one-step kernel;
  * default 512 / 64); fold_split_rows, the longest folded column the fused
  * fold-and-digit-split kernel takes (default 32768). 0 restores a default;
  * values above the default are GI_EINVAL (the kernels' own limits). The
- * global-gather builds, the one-step split kernel and the two-kernel fold
+ * partly staged and global-gather builds, the one-step split kernel and the two-kernel fold
  * give bitwise the same lines as the paths they replace; another split-K
  * part count changes the correlation's summation order. */
 int gi_set_path_limits(gi_ctx *ctx, int kpart_len, int rec_lds, int split2_steps,

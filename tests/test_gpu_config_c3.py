@@ -4,7 +4,7 @@ One context holds C2v(M) J = 0..20 at Nvib = 2000: 84 symmetry blocks of 500
 to 20,500 functions (99 GB of FP64 eigenvectors and 62 GB of split digit
 planes resident), every |dJ| <= 1 pair in the same launches. Lower levels of
 J = 20 have records longer than the 20,224 doubles the G builds stage in LDS,
-so their G comes from the global-gather builds; the upper blocks run split-K
+so their G builds gather the rest of the record from global memory; the upper blocks run split-K
 correlations. The sample (J = 1 every 4th lower level, four lower levels each
 of J = 16 and J = 20, symmetry A1) runs in every mode against the oracle's
 chain of pair runs (the reference runs J lists one (J, J+1) pair at a time);

@@ -1,8 +1,9 @@
 """Size-dependent kernel routes reached at small sizes (gi_set_path_limits).
 
 At real sizes the batched modes switch kernels on the size of a product:
-split-K correlation (upper blocks of 2048 functions and more), global-gather
-G builds (lower records over 20,224 doubles), the one-step split half-ls
+split-K correlation (upper blocks of 2048 functions and more), G builds with
+only part of a record in LDS (lower records over 20,224 doubles; the
+global-gather builds serve maps wider than 4), the one-step split half-ls
 (classes over 512 K steps or 64 segments) and the two-kernel fold and digit
 split (folded columns over 32,768). Lowering those thresholds sends small jobs
 down the same code. Each route is checked against the FP64 oracle, and the
@@ -53,11 +54,14 @@ def routed(request):
 
 
 @pytest.mark.parametrize("mode", [B, SP])
-def test_global_gather_g_build_same_bits(routed, mode):
+@pytest.mark.parametrize("rec_lds", [-1, 1, 37])
+def test_global_gather_g_build_same_bits(routed, mode, rec_lds):
+    # -1: the global-gather builds; 1 and 37: records longer than their part
+    # staged in LDS, the rest gathered from global memory
     job, eng, ref = routed
-    print(f"[paths] {job.sym['name']} mode {mode}: global-gather G", flush=True)
+    print(f"[paths] {job.sym['name']} mode {mode}: G builds, rec_lds {rec_lds}", flush=True)
     base = _run(eng, mode)
-    glob = _run(eng, mode, rec_lds=1)
+    glob = _run(eng, mode, rec_lds=rec_lds)
     _bitwise(glob, base)
     if mode == B:
         compare(glob, ref)
@@ -93,7 +97,7 @@ def test_split_k_correlation_vs_oracle(routed, mode):
     else:
         assert split_error(got, ref)["max"] <= 1e-6
     # every route lowered at once
-    got = _run(eng, mode, kpart_len=64, rec_lds=1, split2_steps=1, fold_split_rows=1)
+    got = _run(eng, mode, kpart_len=64, rec_lds=-1, split2_steps=1, fold_split_rows=1)
     if mode == B:
         compare(got, ref)
     else:
@@ -104,7 +108,7 @@ def test_path_limits_rules(routed):
     job, eng, ref = routed
     for bad in (dict(kpart_len=4096), dict(kpart_len=32), dict(rec_lds=30000),
                 dict(split2_steps=513), dict(split2_segs=65), dict(fold_split_rows=40000),
-                dict(rec_lds=-1)):
+                dict(rec_lds=-2)):
         with pytest.raises(binding.GIError):
             eng.set_path_limits(**bad)
     eng.set_path_limits()
