@@ -354,9 +354,9 @@ int gi_get_persist(gi_ctx *ctx, int *on);
  * given lower levels (NULL = all) `reps` times per setting into `out`,
  * and keeps the fastest setting of each knob: the two tile shapes (FP64
  * batched mode) or the split half-ls tile shape (split mode), each judged by
- * its kernel's time, and the pipeline round count
- * (both matrix-core modes when `out` has room for every candidate, judged by
- * the call's wall time). Settings are scanned in a fixed order starting from
+ * its kernel's time, and the pipeline round count together with the share of
+ * the work per round, as pairs (both matrix-core modes when `out` has room for
+ * every candidate, judged by the call's wall time). Settings are scanned in a fixed order starting from
  * the current one, and a setting replaces the incumbent only when it is more
  * than 3% faster, so repeated runs pick the same table unless the timings
  * really differ. The chosen setting is applied and returned in *best;

@@ -39,7 +39,7 @@ def test_autotune_picks_a_setting_and_keeps_the_bits(tuned_engine, mode):
     assert lines[0].startswith("# gi_autotune") and lines[-1].startswith("best ")
     kinds = [ln.split()[0] for ln in lines[1:-1]]
     want = (["halfls_tile"] * 6 + ["corr_tile"] * 4 if mode == binding.GI_MODE_FP64_BATCHED
-            else ["split_halfls_tile"] * 3) + ["pipe_rounds"] * 6 + ["pipe_share"] * 4
+            else ["split_halfls_tile"] * 3) + ["pipe_rounds"] * 21  # 1 round, 5 x 4 pairs
     assert kinds == want
     assert lines[-1] == ("best halfls_tile=%d corr_tile=%d pipe_rounds=%d split_tile=%d "
                          "pipe_share=%d" % (best["halfls_cfg"], best["corr_cfg"],
