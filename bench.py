@@ -463,32 +463,37 @@ def mu_stream_section(eng, binding, job, dev, bufs, parts, steps, mode_names, sa
 
 
 def vec_stream_section(binding, job, bufs, ref_lines, steps):
-    """GI_MODE_FP64_BATCHED with every eigenvector block kept in pinned host
-    memory (gi_set_vector_budget): each chunk's records are copied to the
-    device while the previous chunk computes. The lines must be the bits of
-    resident blocks; reports the pass time and the copied bytes."""
+    """The matrix-core modes with every eigenvector block kept in pinned host
+    memory (gi_set_vector_budget): each chunk's records (in split mode the
+    upper records' digit rows) are copied to the device while the previous
+    chunk computes. The lines must be the bits of resident blocks (ref_lines:
+    mode name -> lines); reports each mode's pass time."""
     import torch
-    mode = binding.GI_MODE_FP64_BATCHED
     eng = binding.Engine(0)
     try:
         t = time.perf_counter()
         eng.load(job, vector_budget=1)
         t_load = time.perf_counter() - t
-        got = eng.compute(bufs=bufs, mode=mode)
-        same = all(np.array_equal(ref_lines[k], got[k]) for k in ref_lines)
-        torch.cuda.synchronize()
-        ts = []
-        for _ in range(steps):
+        out = dict(streamed_blocks=eng.streamed_blocks(), host_load_s=round(t_load, 2))
+        for mname, ref in ref_lines.items():
+            mode = binding.MODES[mname]
             t = time.perf_counter()
-            eng.compute(bufs=bufs, mode=mode)
-            ts.append(time.perf_counter() - t)
-        ms = 1e3 * sum(ts) / len(ts)
-        out = dict(streamed_blocks=eng.streamed_blocks(), ms_per_step=ms,
-                   value=len(got["S"]) / (ms / 1e3), bitwise_equal_resident=bool(same),
-                   host_load_s=round(t_load, 2),
-                   kernels={k: {"total_ms": round(v["total_ms"], 4), "launches": v["launches"]}
-                            for k, v in eng.kernel_stats().items()})
-        log(f"blocks in host memory: {ms:.2f} ms/step, bitwise equal {same}")
+            got = eng.compute(bufs=bufs, mode=mode)  # split: builds the host digit planes first
+            t_first = time.perf_counter() - t
+            same = all(np.array_equal(ref[k], got[k]) for k in ref)
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(steps):
+                t = time.perf_counter()
+                eng.compute(bufs=bufs, mode=mode)
+                ts.append(time.perf_counter() - t)
+            ms = 1e3 * sum(ts) / len(ts)
+            out[mname] = dict(ms_per_step=ms, value=len(got["S"]) / (ms / 1e3),
+                              bitwise_equal_resident=bool(same), first_call_s=round(t_first, 2),
+                              kernels={k: {"total_ms": round(v["total_ms"], 4),
+                                           "launches": v["launches"]}
+                                       for k, v in eng.kernel_stats().items()})
+            log(f"blocks in host memory, {mname}: {ms:.2f} ms/step, bitwise equal {same}")
         return out
     finally:
         eng.close()
@@ -678,8 +683,9 @@ def compact_line(out):
              for m, v in mu.items() if isinstance(v, dict)})
     vs = out.get("vec_streamed")
     if vs:
-        line["blocks_in_host_memory"] = {"ms_per_step": round(vs["ms_per_step"], 3),
-                                         "bitwise_equal_resident": vs["bitwise_equal_resident"]}
+        line["blocks_in_host_memory"] = {m: [round(v["ms_per_step"], 3),
+                                             v["bitwise_equal_resident"]]
+                                         for m, v in vs.items() if isinstance(v, dict)}
     mj = out.get("multi_j")
     if mj:
         line["multi_j"] = {m: round(v["value"], 1) for m, v in mj["modes"].items()}
@@ -861,9 +867,11 @@ def main():
             parity["split_1500K"] = split_at_temperature(eng, job, binding, oracle_py, 1500.0,
                                                          nthr)
         log(f"parity {parity}")
-        if args.vec_stream and "fp64_batched" in mode_names:
-            resident = {k: v.copy() for k, v in
-                        eng.compute(bufs=bufs, mode=binding.GI_MODE_FP64_BATCHED).items()}
+        vs_modes = [m for m in ("fp64_batched", "split") if m in mode_names]
+        if args.vec_stream and vs_modes:
+            resident = {m: {k: v.copy() for k, v in
+                            eng.compute(bufs=bufs, mode=binding.MODES[m]).items()}
+                        for m in vs_modes}
             vec_streamed = vec_stream_section(binding, job, bufs, resident, 3)
         job.vectors.clear()
     mu_streamed = None
@@ -890,7 +898,8 @@ def main():
           and parity_ok((c3v or {}).get("parity"))
           and all(v.get("bitwise_equal_resident", True) for v in (mu_streamed or {}).values()
                   if isinstance(v, dict))
-          and (vec_streamed or {}).get("bitwise_equal_resident", True))
+          and all(v.get("bitwise_equal_resident", True) for v in (vec_streamed or {}).values()
+                  if isinstance(v, dict)))
     out = {
         "metric": METRIC, "value": value, "unit": "transitions/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,

@@ -180,9 +180,13 @@ void free_split(gi_ctx *c)
     c->d_mue = nullptr;
     for (auto &b : c->blocks) {
         (void)hipFree(b.d_ud); (void)hipFree(b.d_ue); (void)hipFree(b.d_umax);
+        if (b.h_ud) (void)hipHostFree(b.h_ud);
+        if (b.h_ue) (void)hipHostFree(b.h_ue);
         b.d_ud = nullptr;
         b.d_ue = nullptr;
         b.d_umax = nullptr;
+        b.h_ud = nullptr;
+        b.h_ue = nullptr;
         b.Kp = 0;
         b.uplane = 0;
     }
@@ -225,12 +229,55 @@ static int split_mu_streamed(gi_ctx *c, const int *d_vinv)
 
 // Digit planes of mu and of every resident eigenvector block for the split
 // mode (built once per digit count; freed with the plan).
+// Digit planes of a block kept in host memory (gi_set_vector_budget): its
+// records go to the device in batches, are cut into digits there (each
+// record's digits and exponent depend on that record only, so they are the
+// bytes a resident block would hold) and come back to pinned host memory;
+// the exponents and max |u| per record stay on the device as well.
+static int split_host_block(gi_ctx *c, Block &b, int SD)
+{
+    const size_t batch_bytes = (size_t)256 << 20;
+    const int rb = (int)std::max<size_t>(1, std::min<size_t>(b.nrec,
+                                                             batch_bytes / (8 * (size_t)b.nsize)));
+    HIPCHK(hipHostMalloc((void **)&b.h_ud, (size_t)SD * b.uplane, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&b.h_ue, sizeof(int) * b.nrec, hipHostMallocDefault));
+    double *dx = nullptr;
+    int8_t *dd = nullptr;
+    if (hipMalloc((void **)&dx, (size_t)rb * b.nsize * 8) != hipSuccess ||
+        hipMalloc((void **)&dd, (size_t)SD * rb * b.Kp) != hipSuccess) {
+        (void)hipFree(dx);
+        return fail(c, GI_ENOMEM, "split mode: staging of a host block's digits");
+    }
+    int rc = GI_OK;
+    for (int r0 = 0; r0 < b.nrec && rc == GI_OK; r0 += rb) {
+        const int n = std::min(rb, b.nrec - r0);
+        const long long plane = (long long)n * b.Kp;
+        if (hipMemcpyAsync(dx, b.h_vec + (size_t)r0 * b.nsize, (size_t)n * b.nsize * 8,
+                           hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+            rc = fail(c, GI_EHIP, "split mode: copy of a host block");
+            break;
+        }
+        launch_split_rows(c->stream, SD, c->split_radix, dx, b.nsize, n, b.nsize, b.Kp, dd, plane,
+                          b.d_ue + r0, b.d_umax + r0);
+        for (int q = 0; q < SD && rc == GI_OK; q++)
+            if (hipMemcpyAsync(b.h_ud + q * b.uplane + (size_t)r0 * b.Kp, dd + q * plane,
+                               (size_t)plane, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+                rc = fail(c, GI_EHIP, "split mode: copy of a host block's digits");
+        // the buffers are reused by the next batch
+        if (rc == GI_OK && hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = fail(c, GI_EHIP, "split mode: digits of a host block");
+    }
+    if (rc == GI_OK &&
+        (hipMemcpy(b.h_ue, b.d_ue, sizeof(int) * b.nrec, hipMemcpyDeviceToHost) != hipSuccess))
+        rc = fail(c, GI_EHIP, "split mode: exponents of a host block");
+    (void)hipFree(dx);
+    (void)hipFree(dd);
+    return rc;
+}
+
 int prepare_split(gi_ctx *c)
 {
     if (!c->prepared) return fail(c, GI_ESTATE, "split mode before gi_prepare");
-    if (host_blocks(c))
-        return fail(c, GI_ESTATE, "split mode needs every eigenvector block resident "
-                                  "(raise gi_set_vector_budget and upload the blocks again)");
     if (c->split_ready == c->split_key()) return GI_OK;
     HIPCHK(hipSetDevice(c->device));
     free_split(c);
@@ -287,15 +334,20 @@ int prepare_split(gi_ctx *c)
         return rc;
     }
     for (auto &b : c->blocks) {
-        if (b.nrec <= 0 || b.nsize <= 0 || !b.d_vec) continue;
+        if (b.nrec <= 0 || b.nsize <= 0 || !(b.d_vec || b.h_vec)) continue;
         b.Kp = (b.nsize + 63) / 64 * 64;
         b.uplane = (long long)b.nrec * b.Kp;
-        HIPCHK(hipMalloc((void **)&b.d_ud, (size_t)SD * b.uplane));
         HIPCHK(hipMalloc((void **)&b.d_ue, sizeof(int) * b.nrec));
         HIPCHK(hipMalloc((void **)&b.d_umax, sizeof(double) * b.nrec));
-        launch_split_rows(c->stream, SD, c->split_radix, b.d_vec, b.nsize, b.nrec, b.nsize, b.Kp, b.d_ud, b.uplane,
-                          b.d_ue, b.d_umax);
-        HIPCHK(hipGetLastError());
+        if (b.d_vec) {
+            HIPCHK(hipMalloc((void **)&b.d_ud, (size_t)SD * b.uplane));
+            launch_split_rows(c->stream, SD, c->split_radix, b.d_vec, b.nsize, b.nrec, b.nsize,
+                              b.Kp, b.d_ud, b.uplane, b.d_ue, b.d_umax);
+            HIPCHK(hipGetLastError());
+        } else if ((rc = split_host_block(c, b, SD)) != GI_OK) {
+            (void)hipFree(d_vinv);
+            return rc;
+        }
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     (void)hipFree(d_vinv);

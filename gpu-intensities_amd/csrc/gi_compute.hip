@@ -182,6 +182,10 @@ struct RoundOut {
 // Staging buffers of the blocks kept in host memory (see vec_stage_setup).
 struct VecStage {
     double *lbuf[2] = {nullptr, nullptr}, *ubuf[2] = {nullptr, nullptr};
+    // split mode: the upper digit rows (planes uplane bytes apart) and exponents
+    int8_t *ubd[2] = {nullptr, nullptr};
+    int *ube[2] = {nullptr, nullptr};
+    long long uplane = 0;
     hipStream_t cs = nullptr;
 };
 
@@ -850,18 +854,31 @@ static int vec_stage_setup(gi_ctx *c, Slot &S, Plan &P, VecStage &V)
 {
     if (!P.vstream) return GI_OK;
     int rc = GI_OK;
-    size_t lmax = 1, umax = 1;
+    const bool split = P.mode == GI_MODE_SPLIT;
+    size_t lmax = 1, umax = 1, urows = 1, ukp = 64;
     for (const Chunk &ch : P.chunks) {
         if (ch.lhost)
             lmax = std::max(lmax, (size_t)ch.nsrec * c->blocks[ch.jindI * c->nrep + ch.gI].nsize);
-        if (ch.uhost)
-            umax = std::max(umax, (size_t)std::max(ch.Hi - ch.Lo, 0) *
-                                      c->blocks[ch.jindF * c->nrep + ch.gF].nsize);
+        if (ch.uhost) {
+            const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
+            umax = std::max(umax, (size_t)std::max(ch.Hi - ch.Lo, 0) * BF.nsize);
+            urows = std::max(urows, (size_t)std::max(ch.Hi - ch.Lo, 0));
+            ukp = std::max(ukp, (size_t)BF.Kp);
+        }
     }
+    // the split mode stages the upper records' digits instead of their FP64 values
+    V.uplane = (long long)(urows * ukp);
     for (int b = 0; b < 2; b++) {
         V.lbuf[b] = (double *)slot_scratch(c, S, b ? "vs_l1" : "vs_l0", 8 * lmax, &rc);
         if (rc) return rc;
-        V.ubuf[b] = (double *)slot_scratch(c, S, b ? "vs_u1" : "vs_u0", 8 * umax, &rc);
+        if (split) {
+            V.ubd[b] = (int8_t *)slot_scratch(c, S, b ? "vs_ud1" : "vs_ud0",
+                                              (size_t)c->split_S * V.uplane, &rc);
+            if (rc) return rc;
+            V.ube[b] = (int *)slot_scratch(c, S, b ? "vs_ue1" : "vs_ue0", 4 * urows, &rc);
+        } else {
+            V.ubuf[b] = (double *)slot_scratch(c, S, b ? "vs_u1" : "vs_u0", 8 * umax, &rc);
+        }
         if (rc) return rc;
     }
     for (auto &e : S.vec_ev)
@@ -908,7 +925,25 @@ static int vec_stage_chunk(gi_ctx *c, Slot &S, const Plan &P, size_t i, const Ve
                        [&](int s) { return P.srec[(size_t)ch.srec0 + s]; })) != GI_OK)
             return rc;
     }
-    if (ch.uhost && ch.Hi > ch.Lo) {
+    if (ch.uhost && ch.Hi > ch.Lo && P.mode == GI_MODE_SPLIT) {
+        // digit rows of the window's records (each plane), then their exponents
+        const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
+        const int n = ch.Hi - ch.Lo;
+        auto rec_of = [&](int s) { return c->lv_irec0[BF.lev[ch.Lo + s]]; };
+        int s0 = 0;
+        while (s0 < n) {
+            int s1 = s0 + 1;
+            while (s1 < n && rec_of(s1) == rec_of(s1 - 1) + 1) s1++;
+            const size_t r0 = (size_t)rec_of(s0);
+            for (int q = 0; q < c->split_S; q++)
+                HIPCHK(hipMemcpyAsync(V.ubd[b] + q * V.uplane + (size_t)s0 * BF.Kp,
+                                      BF.h_ud + q * BF.uplane + r0 * BF.Kp,
+                                      (size_t)(s1 - s0) * BF.Kp, hipMemcpyHostToDevice, V.cs));
+            HIPCHK(hipMemcpyAsync(V.ube[b] + s0, BF.h_ue + r0, (size_t)(s1 - s0) * 4,
+                                  hipMemcpyHostToDevice, V.cs));
+            s0 = s1;
+        }
+    } else if (ch.uhost && ch.Hi > ch.Lo) {
         const Block &BF = c->blocks[ch.jindF * c->nrep + ch.gF];
         if ((rc = runs(BF, V.ubuf[b], ch.Hi - ch.Lo,
                        [&](int s) { return c->lv_irec0[BF.lev[ch.Lo + s]]; })) != GI_OK)
@@ -1537,6 +1572,8 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
     for (auto &rg : rounds) {
         RoundCapture cap;
         if ((rc = cap.begin(c, P, S.stream)) != GI_OK) return rc;
+        // blocks in host memory: one chunk per round, its records staged first
+        if (P.vstream && (rc = vec_stage_chunk(c, S, P, rg.first, P.vs)) != GI_OK) return rc;
         std::vector<GSplitJob> gj, gjg;  // G builds in LDS, global-gather G builds
         std::vector<SplitHalfJob> hj;
         std::vector<CorrSplitArgs> cj;
@@ -1665,6 +1702,13 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             a.item_out_off = (const long long *)(d_meta + ch.off_out);
             a.blk_col = BF.d_col;
             a.Ud = BF.d_ud; a.uplane = BF.uplane; a.Ue = BF.d_ue;
+            if (ch.uhost) {  // staged digit rows of this chunk's window, identity map
+                const int bsel = (int)(i & 1);
+                a.Ud = P.vs.ubd[bsel] - (long long)ch.Lo * BF.Kp;
+                a.uplane = P.vs.uplane;
+                a.Ue = P.vs.ube[bsel] - ch.Lo;
+                a.blk_col = c->d_iota;
+            }
             a.Hd = Hd + Hdoff[k]; a.hplane = (long long)mhd; a.He = He + Hcoff[k];
             a.amp = cand.amp;
             a.amp_m = c->amp_m;
@@ -1770,6 +1814,7 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             launch_corr_split(S.stream, SD, c->split_radix, t_cj, b_cj, (int)cj.size(), ct);
             TL_END();
         }
+        if (P.vstream && (rc = vec_stage_done(c, S, P, rg.first)) != GI_OK) return rc;
         if (!ijobs.empty()) {
             TL_BEGIN("intensity");
             launch_intensity_jobs(S.stream, t_ij, b_ij, (int)ijobs.size(), iblocks);

@@ -141,6 +141,10 @@ struct Block {
     int Kp = 0;
     int *d_ue = nullptr;
     double *d_umax = nullptr;
+    // a block in host memory keeps its digit planes and exponents in pinned
+    // host memory too (the device keeps d_ue and d_umax)
+    int8_t *h_ud = nullptr;
+    int *h_ue = nullptr;
 };
 
 struct KStat {

@@ -157,14 +157,16 @@ int gi_mu_parts(const gi_ctx *ctx);
 /* Device bytes allowed for the eigenvector blocks uploaded from host memory
  * (0, the default: no limit), applied by the next gi_upload_vectors calls: a
  * block that would take the total over it stays in pinned host memory, and
- * the FP64 modes (GI_MODE_FP64, GI_MODE_FP64_BATCHED, also for requests)
- * copy the records each chunk of lower levels needs (its lower records and
- * the upper records of its frequency window) to the device while the
- * previous chunk computes. The lines are the same bits as with resident
- * blocks. Replaces the reference's upper-vector batches sized to free device
- * memory and read from disk for every lower level (cuda_host.cu:1011-1027,
- * 1235-1372). The split mode, its planes and sweeps need every block resident
- * (GI_ESTATE). Blocks passed as device pointers are always resident. */
+ * every mode (also for requests) copies the records each chunk of lower
+ * levels needs (its lower records and the upper records of its frequency
+ * window; in GI_MODE_SPLIT the upper records' digit rows, which
+ * gi_prepare_split builds on the device and keeps in pinned host memory for
+ * such a block) to the device while the previous chunk computes. The lines
+ * are the same bits as with resident blocks. Replaces the reference's
+ * upper-vector batches sized to free device memory and read from disk for
+ * every lower level (cuda_host.cu:1011-1027, 1235-1372). Captured sweeps need
+ * every block resident (GI_ESTATE). Blocks passed as device pointers are
+ * always resident. */
 int gi_set_vector_budget(gi_ctx *ctx, long long bytes);
 /* Number of eigenvector blocks kept in host memory. */
 int gi_streamed_blocks(const gi_ctx *ctx);

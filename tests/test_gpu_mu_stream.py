@@ -103,6 +103,7 @@ def test_streamed_mu_every_tile_and_route(engine_resident, monkeypatch):
     eng.load(job, mu_budget=17 * 24 * job.nvib, vector_budget=1)
     assert eng.streamed_blocks() > 0
     _same(ref[B], eng.compute(mode=B), "blocks and mu streamed")
+    _same(ref[SP], eng.compute(mode=SP), "blocks and mu digits streamed")
     eng.load(job)
 
 

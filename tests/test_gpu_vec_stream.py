@@ -1,9 +1,11 @@
 """Eigenvector blocks beyond a device budget (gi_set_vector_budget).
 
-Blocks that do not fit the budget stay in pinned host memory; the FP64 modes
-copy each chunk's lower records and the upper records of its window to the
-device while the previous chunk computes (the reference reads upper vectors
-in batches sized to free device memory, cuda_host.cu:1011-1027, 1235-1372).
+Blocks that do not fit the budget stay in pinned host memory; every mode
+copies each chunk's lower records and the upper records of its window (in
+split mode their digit rows, built at gi_prepare_split and kept in pinned
+host memory) to the device while the previous chunk computes (the reference
+reads upper vectors in batches sized to free device memory,
+cuda_host.cu:1011-1027, 1235-1372).
 The kernels sum the same terms in the same order, so every line must be the
 same bits as with resident blocks, also when records are stored out of energy
 order (staged in several runs), for a J list, and for requests.
@@ -19,7 +21,7 @@ from test_gpu_parity import compare
 
 pytestmark = pytest.mark.gpu
 
-MODES = (binding.GI_MODE_FP64, binding.GI_MODE_FP64_BATCHED)
+MODES = (binding.GI_MODE_FP64, binding.GI_MODE_FP64_BATCHED, binding.GI_MODE_SPLIT)
 KEYS = ("ilevelI", "ilevelF", "nu", "S", "A_einst", "I", "amp")
 
 
@@ -109,9 +111,6 @@ def test_streamed_blocks_rules(engine):
     job = synth.make_job("C2V", nvib=40, jvals=(0, 1), seed=74)
     engine.load(job, vector_budget=1)
     try:
-        with pytest.raises(binding.GIError) as e:
-            engine.compute(mode=binding.GI_MODE_SPLIT)
-        assert e.value.status == 4
         with pytest.raises(binding.GIError) as e:
             engine.sweep_create(binding.GI_MODE_FP64_BATCHED)
         assert e.value.status == 4
