@@ -111,6 +111,9 @@ def roofline_entry(name, kstats, work, nvib, split_cfg=0):
     """Roofline of one kernel: algorithmic work of all its launches in the last
     step divided by their HIP-event time (average per launch)."""
     r = _roofline(name, kstats, work, nvib)
+    # per step: launches (one per pipeline round) and their summed time
+    r["launches_per_step"] = kstats[name]["launches"]
+    r["kernel_ms_per_step"] = kstats[name]["total_ms"]
     sym = "k_halfls_split2" if name == "half_ls_split" and split_cfg == 2 else None
     r["traffic"] = measured_traffic(name, sym)
     if r["traffic"] is not None:
@@ -649,8 +652,9 @@ def compact_line(out):
     line["config"] = {k: out["config"][k] for k in ("workload", "mode", "transitions",
                                                     "lower_levels", "parallelism")}
     line["roofline"] = {k: r.get(k) for k in ("bound", "achieved", "peak", "unit", "frac",
-                                              "traffic", "kernel", "avg_launch_ms", "count",
-                                              "measured_copy_GBps")}
+                                              "traffic", "kernel", "avg_launch_ms",
+                                              "launches_per_step", "kernel_ms_per_step",
+                                              "count", "measured_copy_GBps")}
     cpu = out.get("cpu_baseline")
     if cpu:
         line["cpu_baseline"] = {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")}
@@ -741,6 +745,11 @@ def main():
                     help="saved gi_autotune table for the headline workload (default: the "
                          "committed profiles/autotune_c2.txt, whose settings the committed "
                          "kernel trace and PMC traffic were taken with)")
+    ap.add_argument("--pipe-rounds", type=int, default=0,
+                    help="override the saved table's pipeline round count for the headline "
+                         "workload (1: each kernel launches once per step and no line copy "
+                         "runs beside the kernels, the setting the committed kernel trace is "
+                         "taken with, since rocprofv3 slows kernels that run beside the copies)")
     ap.add_argument("--retune", action="store_true",
                     help="run gi_autotune on the headline workload instead of applying the "
                          "saved table (roofline traffic is then left null)")
@@ -790,6 +799,8 @@ def main():
         log(f"split digit planes {time.time() - t:.1f}s")
     tuning_file = args.tuning or (TUNING_FILE if os.path.exists(TUNING_FILE) else None)
     saved = read_tuning(tuning_file) if tuning_file and not args.retune else None
+    if saved and args.pipe_rounds > 0:  # one launch per kernel and step when 1 (profiling)
+        saved = {m: (t[0], t[1], args.pipe_rounds) + tuple(t[3:]) for m, t in saved.items()}
     modes = time_modes(eng, binding, bufs, mode_names, args.steps, args.warmup, saved=saved)
     # the headline is the fastest FP64 mode (the reference's arithmetic); the
     # split mode (int8 digit products) is reported beside it
