@@ -454,9 +454,12 @@ __device__ __attribute__((always_inline)) inline void corr_tile(const CorrMMArgs
     for (int x = 0; x < T::MI; x++)
 #pragma unroll
         for (int y = 0; y < T::NI; y++) acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
-    // loads: K = lt + 4q (q < 4) of row lr + 64p (p < BM/64 for A, BN/64 for B);
-    // the 4 lanes of a row read 32 contiguous bytes per load and store into 4
-    // consecutive LDS rows, BM + 4 doubles apart, so on distinct banks
+    // loads: K = 2 lt + 8 q + {0, 1} (q < 2) of row lr + 64p (p < BM/64 for A,
+    // BN/64 for B): each lane reads 16 contiguous bytes per load, so one load
+    // instruction reads 64 contiguous bytes of each of its 16 rows (rows of
+    // Ht have an even stride; eigenvector records when vec_al); a pair that
+    // would cross the end of the K part, or a record not 16-byte aligned, is
+    // read as two doubles
     const int lr = tid >> 2, lt = tid & 3;
     gptr<double> Urow[BM_ / 64];
     gptr<double> Hrow[BN_ / 64];
@@ -468,27 +471,40 @@ __device__ __attribute__((always_inline)) inline void corr_tile(const CorrMMArgs
 #pragma unroll
     for (int p = 0; p < BN_ / 64; p++) {
         const int nr = n0 + lr + 64 * p;
-        Hrow[p] = nr < ncols ? gmem(a.Ht) + (size_t)nr * a.nsize : nullptr;
+        Hrow[p] = nr < ncols ? gmem(a.Ht) + (size_t)nr * a.ldh : nullptr;
     }
-    double ra[BM_ / 64][4], rb[BN_ / 64][4];
+    typedef double d2a __attribute__((ext_vector_type(2)));
+    auto ld2 = [&](gptr<double> row, bool al, int t) -> d2a {
+        if (!row) return d2a{0.0, 0.0};
+        if (al && t + 1 < ke) return *(gptr<d2a>)(row + t);
+        return d2a{t < ke ? row[t] : 0.0, t + 1 < ke ? row[t + 1] : 0.0};
+    };
+    const bool ual = a.vec_al != 0;
+    d2a ra[BM_ / 64][2], rb[BN_ / 64][2];
     auto fetch = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int t = k0 + lt + 4 * q;
+        for (int q = 0; q < 2; q++) {
+            const int t = k0 + 2 * lt + 8 * q;
 #pragma unroll
-            for (int p = 0; p < BM_ / 64; p++) ra[p][q] = (Urow[p] && t < ke) ? Urow[p][t] : 0.0;
+            for (int p = 0; p < BM_ / 64; p++) ra[p][q] = ld2(Urow[p], ual, t);
 #pragma unroll
-            for (int p = 0; p < BN_ / 64; p++) rb[p][q] = (Hrow[p] && t < ke) ? Hrow[p][t] : 0.0;
+            for (int p = 0; p < BN_ / 64; p++) rb[p][q] = ld2(Hrow[p], true, t);
         }
     };
     fetch(kb);
     for (int k0 = kb; k0 < ke; k0 += BK) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < 2; q++) {
 #pragma unroll
-            for (int p = 0; p < BM_ / 64; p++) As[(lt + 4 * q) * T::PA + lr + 64 * p] = ra[p][q];
+            for (int p = 0; p < BM_ / 64; p++) {
+                As[(2 * lt + 8 * q) * T::PA + lr + 64 * p] = ra[p][q].x;
+                As[(2 * lt + 8 * q + 1) * T::PA + lr + 64 * p] = ra[p][q].y;
+            }
 #pragma unroll
-            for (int p = 0; p < BN_ / 64; p++) Bs[(lt + 4 * q) * T::PB + lr + 64 * p] = rb[p][q];
+            for (int p = 0; p < BN_ / 64; p++) {
+                Bs[(2 * lt + 8 * q) * T::PB + lr + 64 * p] = rb[p][q].x;
+                Bs[(2 * lt + 8 * q + 1) * T::PB + lr + 64 * p] = rb[p][q].y;
+            }
         }
         __syncthreads();
         if (k0 + BK < ke) fetch(k0 + BK);

@@ -1070,6 +1070,10 @@ struct RoundCapture {
 };
 
 // Batched FP64 mode over rounds of chunks (see the file comment).
+// Row stride of the folded vectors Ht in the FP64 batched mode: even, so
+// every row starts 16-byte aligned for the correlation's 16-byte loads.
+static int ht_stride(int nsize) { return (nsize + 1) & ~1; }
+
 static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timeline *tl,
                        int *flag, LineSoA cand)
 {
@@ -1089,7 +1093,8 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
         const int ncb = c->degen[ch.gI] * c->degen[ch.gF];
         const int nsizeF = c->blocks[ch.jindF * c->nrep + ch.gF].nsize;
         need[i] = Need{(size_t)ch.nunits * GP.Gtot_p, (size_t)ch.nunits * GP.Htot,
-                       (size_t)ch.nitems * ncb * nsizeF, ch.nunits > 0 ? (int)GP.oc.size() : 0};
+                       (size_t)ch.nitems * ncb * ht_stride(nsizeF),
+                       ch.nunits > 0 ? (int)GP.oc.size() : 0};
         bytes[i] = 8 * (need[i].g + need[i].h + need[i].ht);
     }
     const auto rounds = plan_rounds(c, P, bytes);
@@ -1303,7 +1308,7 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                 FoldJob fj{ch.nitems, BF.nsize, ndegI, ndegF, GP.Htot,
                            (BF.nsize + kFoldRows - 1) / kFoldRows,
                            (const int *)(d_meta + ch.off_iunit), t0, t1, Hc[i - rg.first],
-                           Htc[i - rg.first]};
+                           Htc[i - rg.first], ht_stride(BF.nsize)};
                 fjobs.push_back(fj);
                 fbegin.push_back(fblocks);
                 fblocks += fj.xt * ch.nitems * ndegI * ndegF;
@@ -1321,6 +1326,9 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
                 m.vec = staged_rows(P.vs.ubuf[i & 1], ch.Lo, BF.nsize);
                 m.blk_col = c->d_iota;
             }
+            m.ldh = ht_stride(BF.nsize);
+            // 16-byte loads of the records: an even record length and an aligned base
+            m.vec_al = (BF.nsize % 2 == 0 && ((uintptr_t)m.vec & 15) == 0) ? 1 : 0;
             m.amp_m = c->amp_m;
             m.kparts = corr_kparts(c, P, ch);
             m.kchunk = (BF.nsize + 16 * m.kparts - 1) / (16 * m.kparts) * 16;

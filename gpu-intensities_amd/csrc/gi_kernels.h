@@ -84,6 +84,8 @@ struct CorrMMArgs {
     int kparts, kchunk;              // split K as in CorrSplitArgs
     double *ampx;
     long long ampx_stride;
+    int ldh;                         // row stride of Ht (even: 16-byte aligned rows)
+    int vec_al;                      // every record of vec starts 16-byte aligned
 };
 
 // Intensity step from stored amplitudes (batched mode).
@@ -305,6 +307,7 @@ struct FoldJob {
     SymDev t0, t1;
     const double *H;
     double *Ht;
+    int ldh;                         // row stride of Ht (0: nsizeF)
 };
 constexpr int kFoldRows = 1024;  // rows of one k_fold_jobs block (4 per thread)
 
@@ -352,7 +355,7 @@ __device__ __attribute__((always_inline)) inline void fold_block(const FoldJob &
     const int unit = J.item_unit[item * GI_MAXDEG + idegI];
     const SymDev &T = idegF ? J.t1 : J.t0;
     const double *Hu = J.H + (size_t)(unit >= 0 ? unit : 0) * J.SFN;
-    double *out = J.Ht + ((size_t)item * J.ndegI * J.ndegF + cb) * J.nsizeF;
+    double *out = J.Ht + ((size_t)item * J.ndegI * J.ndegF + cb) * (J.ldh ? J.ldh : J.nsizeF);
     int t[4];
     bool ok[4];
     double acc[4];
