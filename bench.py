@@ -95,14 +95,15 @@ PMC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
 
 
 def measured_traffic(name, sym=None):
-    """HBM bytes per dispatch of the kernel behind timer `name`, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this workload,
-    each scaled by the factor the same counter gives for a plain 1 GiB copy.
+    """HBM bytes per step of the kernel behind timer `name`, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this workload
+    (taken with one pipeline round per step, so one dispatch per step), each
+    scaled by the factor the same counter gives for a plain 1 GiB copy.
     None when no such profile is committed."""
     try:
         with open(PMC_FILE) as f:
             d = json.load(f)
-        return d["kernels"][sym or KERNEL_SYMBOL[name]]["hbm_bytes"]
+        return d["kernels"][sym or KERNEL_SYMBOL[name]]["hbm_bytes_per_step"]
     except (OSError, KeyError, ValueError):
         return None
 
@@ -115,9 +116,11 @@ def roofline_entry(name, kstats, work, nvib, split_cfg=0):
     r["launches_per_step"] = kstats[name]["launches"]
     r["kernel_ms_per_step"] = kstats[name]["total_ms"]
     sym = "k_halfls_split2" if name == "half_ls_split" and split_cfg == 2 else None
-    r["traffic"] = measured_traffic(name, sym)
+    step_bytes = measured_traffic(name, sym)
+    r["traffic"] = None if step_bytes is None else step_bytes / max(r["launches_per_step"], 1)
     if r["traffic"] is not None:
-        r["traffic_unit"] = "bytes per launch (rocprofv3 PMC, copy-calibrated)"
+        r["traffic_unit"] = ("bytes per launch (rocprofv3 PMC bytes per step, copy-calibrated, "
+                             "over the launches per step)")
     return r
 
 
@@ -507,7 +510,7 @@ def mode_rooflines(modes, nvib, pmc=False):
     FP64 (GEMV-form) kernels, matrix-core utilisation for the batched ones.
     With `pmc` (the headline workload, whose rocprofv3 counter passes are
     committed) each entry also carries the HBM GB/s the counters measured
-    for that kernel (bytes per dispatch / HIP-event time per launch), and the
+    for that kernel (bytes per step / HIP-event time per step), and the
     GEMV half-ls its FP64 issue rate: caches serve most of its mu re-reads, so
     the algorithmic GB/s overstates what HBM delivers."""
     out = {}
@@ -521,9 +524,9 @@ def mode_rooflines(modes, nvib, pmc=False):
                 if pmc:
                     sym = ("k_halfls_split2" if k == "half_ls_split"
                            and (m.get("tuning") or {}).get("split_cfg", 2) == 2 else None)
-                    tb = measured_traffic(k, sym)
+                    tb = measured_traffic(k, sym)  # bytes per step
                     if tb is not None:
-                        e["pmc_GBps"] = tb / (r["avg_launch_ms"] / 1e3) / 1e9
+                        e["pmc_GBps"] = tb / (ks[k]["total_ms"] / 1e3) / 1e9
                         e["pmc_frac_hbm"] = e["pmc_GBps"] / HBM_PEAK_GBS
                 if k == "half_ls_gemv":
                     t = ks[k]["total_ms"] / 1e3
