@@ -40,6 +40,8 @@ def test_autotune_picks_a_setting_and_keeps_the_bits(tuned_engine, mode):
     kinds = [ln.split()[0] for ln in lines[1:-1]]
     want = (["halfls_tile"] * 6 + ["corr_tile"] * 4 if mode == binding.GI_MODE_FP64_BATCHED
             else ["split_halfls_tile"] * 3) + ["pipe_rounds"] * 21  # 1 round, 5 x 4 pairs
+    if mode == binding.GI_MODE_FP64_BATCHED:
+        want += ["halfls_tile"] * 6  # the half-ls tile again at the chosen rounds
     assert kinds == want
     assert lines[-1] == ("best halfls_tile=%d corr_tile=%d pipe_rounds=%d split_tile=%d "
                          "pipe_share=%d" % (best["halfls_cfg"], best["corr_cfg"],
