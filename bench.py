@@ -115,7 +115,8 @@ def roofline_entry(name, kstats, work, nvib, split_cfg=0):
     # per step: launches (one per pipeline round) and their summed time
     r["launches_per_step"] = kstats[name]["launches"]
     r["kernel_ms_per_step"] = kstats[name]["total_ms"]
-    sym = "k_halfls_split2" if name == "half_ls_split" and split_cfg == 2 else None
+    # split half-ls settings 2..4 run k_halfls_split2 (4 or 8 wavefronts)
+    sym = "k_halfls_split2" if name == "half_ls_split" and split_cfg in (2, 3, 4) else None
     step_bytes = measured_traffic(name, sym)
     r["traffic"] = None if step_bytes is None else step_bytes / max(r["launches_per_step"], 1)
     if r["traffic"] is not None:
@@ -523,7 +524,8 @@ def mode_rooflines(modes, nvib, pmc=False):
                 e = {x: r[x] for x in ("achieved", "peak", "unit", "frac")}
                 if pmc:
                     sym = ("k_halfls_split2" if k == "half_ls_split"
-                           and (m.get("tuning") or {}).get("split_cfg", 2) == 2 else None)
+                           and (m.get("tuning") or {}).get("split_cfg", 4) in (2, 3, 4)
+                           else None)
                     tb = measured_traffic(k, sym)  # bytes per step
                     if tb is not None:
                         e["pmc_GBps"] = tb / (ks[k]["total_ms"] / 1e3) / 1e9

@@ -1571,7 +1571,27 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
         const GroupPlan &GP = group_of(c, ch.jindI, ch.gI, ch.jindF);
         return c->lim_rec_lds > 0 && GP.oc.size() <= (size_t)kClsLds && GP.gW <= 4;
     };
-    const int srows_tile = c->tile_split == 1 ? 128 : 64;  // k_halfls_split(_wide, 2) row tile
+    // split half-ls tile of round [c0, c1): the setting, or with kSplitTileAuto
+    // 128 x 64 on eight wavefronts when the round's tiles average at least
+    // kAutoSplitSteps K steps, else 64 x 64 (measured, half-ls per pass: C3v J
+    // 0..15, 129 steps a tile, 524 ms against 597 ms; C2v J 10<->11 and J
+    // 0..20, 23 steps, 1.52 and 34.6 ms against 1.41 and 32.5 ms: one
+    // workgroup per CU leaves the short tiles' set-up and stores exposed)
+    auto round_split_cfg = [&](size_t c0, size_t c1) {
+        if (c->tile_split != kSplitTileAuto) return c->tile_split;
+        double tiles = 0, steps = 0;
+        for (size_t i = c0; i < c1; i++) {
+            const Chunk &ch = chunks[i];
+            if (ch.nunits <= 0) continue;
+            for (const OutClass &oc : group_of(c, ch.jindI, ch.gI, ch.jindF).oc) {
+                const double t = (double)((oc.n + 63) / 64) *
+                                 ((ch.nunits * (int)oc.cols.size() + 63) / 64);
+                tiles += t;
+                steps += t * split2_steps(oc.seg_n.data(), (int)oc.seg_n.size());
+            }
+        }
+        return tiles > 0 && steps >= kAutoSplitSteps * tiles ? 3 : 2;
+    };
     MuParts MP;
     if (c->mu_streamed &&
         (rc = mu_parts_setup(c, S, (size_t)SH * 3 * c->mu_np * c->kpad_all, MP)) != GI_OK)
@@ -1594,6 +1614,10 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                                                        chunks[i].jindF).gW <= 4;
         const int grows = g4 ? 1024 : 256;
         bool steps_fit = true;  // every class within k_halfls_split2's step table
+        // row tile of the split half-ls kernels: k_halfls_split, k_halfls_split2
+        // on 4 wavefronts 64; k_halfls_split_wide, k_halfls_split2 on 8 wavefronts 128
+        const int rcfg = round_split_cfg(rg.first, rg.second);
+        const int srows_tile = rcfg == 1 || rcfg == 3 ? 128 : 64;
         size_t ogd = 0, oge = 0, oh = 0, oht = 0, ohd = 0, ohc = 0;
         std::vector<size_t> Hoff, Htoff, Hdoff, Hcoff;
         for (size_t i = rg.first; i < rg.second; i++) {
@@ -1772,7 +1796,8 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
         }
         // the half-ls products: one launch over resident mu digits from the
         // round's tables, or one launch per mu part with its own table
-        const int scfg = c->tile_split == 2 && !steps_fit ? 0 : c->tile_split;
+        // classes beyond k_halfls_split2's step table: the one-set kernel of the same rows
+        const int scfg = rcfg >= 2 && !steps_fit ? rcfg - 2 : rcfg;
         if (!hj.empty() && !c->mu_streamed) {
             TL_BEGIN("half_ls_split");
             P.work[14] += tt;
@@ -2284,10 +2309,11 @@ int autotune(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out, int
                                          "128x128/8-waves"};
     static const int rounds[6] = {1, 2, 3, 4, 6, 8};
     static const int shares[4] = {50, 60, 70, 80};
-    static const int stiles[3] = {0, 1, 2};
-    static const char *const sname[3] = {"64x64", "128x64", "64x64/2-steps"};
+    static const int stiles[5] = {0, 1, 2, 3, kSplitTileAuto};
+    static const char *const sname[5] = {"64x64", "128x64", "64x64/2-steps",
+                                         "128x64/2-steps", "auto"};
     if (mode == GI_MODE_SPLIT &&
-        (rc = scan("split_halfls_tile", "half_ls_split", &c->tile_split, stiles, 3, sname))) {
+        (rc = scan("split_halfls_tile", "half_ls_split", &c->tile_split, stiles, 5, sname))) {
         c->tile_split = saved.split_cfg;
         return rc;
     }
@@ -2779,7 +2805,7 @@ int gi_set_tuning(gi_ctx *c, const gi_tuning *t)
     if (c->server) return fail(c, GI_ESTATE, "gi_set_tuning while the request server runs");
     if (t->halfls_cfg < 0 || t->halfls_cfg > kTileHalflsMax || t->corr_cfg < 0 || t->corr_cfg > 3 ||
         t->pipe_rounds < 1 || t->pipe_rounds > kMaxPipeRounds || t->split_cfg < 0 ||
-        t->split_cfg > 2 || t->pipe_share < 10 || t->pipe_share > 90)
+        t->split_cfg > kSplitTileAuto || t->pipe_share < 10 || t->pipe_share > 90)
         return fail(c, GI_EINVAL, "gi_set_tuning: value out of range");
     c->tile_halfls = t->halfls_cfg;
     c->tile_corr = t->corr_cfg;

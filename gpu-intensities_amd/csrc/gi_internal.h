@@ -27,6 +27,12 @@ constexpr int kTileAuto = 4;
 // largest half-ls tile setting (5: 128 x 128 on 8 wavefronts, gi_kernels.h)
 constexpr int kTileHalflsMax = 5;
 constexpr int kAutoTileWaves = 3;
+// tile_split setting: per round, 128 x 64 on eight wavefronts with two K steps
+// in flight when the round's 64 x 64 tiles average at least
+// kAutoSplitSteps K steps of 64, else 64 x 64 with two steps in flight (the
+// same bits either way)
+constexpr int kSplitTileAuto = 4;
+constexpr int kAutoSplitSteps = 40;
 // Counters of gi_work_stats.
 constexpr int kWorkStats = 17;
 }  // namespace gi
@@ -261,7 +267,9 @@ struct gi_ctx {
     int ncu = 256;                   // compute units of the device (workgroup slots)
     int pipe_rounds = 4;
     int pipe_share = 50;             // percent of the remaining work per round but the last
-    int tile_split = 2;              // split half-ls: 0 = 64 rows, 1 = 128, 2 = 2 steps in flight
+    // split half-ls: 0 = 64 rows, 1 = 128, 2 = 64 with 2 steps in flight, 3 = 128
+    // with 2, 4 = per round 3 or 2 (kSplitTileAuto)
+    int tile_split = gi::kSplitTileAuto;
     int persist = 0;                 // FP64 batched rounds in one persistent launch (gi_set_persist)
     // routing thresholds of the batched kernels (gi_set_path_limits; the
     // defaults are the kernels' own limits): split-K part length of the

@@ -242,9 +242,10 @@ void launch_build_g_split(hipStream_t st, int S, double R, const GSplitJob *d_jo
 void launch_build_g_split_lds(hipStream_t st, int S, double R, const GSplitJob *d_jobs,
                               const int *d_begin, int njobs, int nunits, bool part);
 // cfg 0: 64 x 64 tiles; 1: 128-row tiles (SplitHalfJob.mt counts 128-row
-// tiles); 2: 64 x 64 with two K steps of loads in flight, for classes of at
-// most kSplitMaxSteps K steps and kSplitMaxSegs segments and at most 5 digits
-// (else the 64 x 64 kernel runs).
+// tiles); 2: 64 x 64 with two K steps of loads in flight, 3: 128 x 64 on eight
+// wavefronts with two K steps in flight, both for classes of at most
+// kSplitMaxSteps K steps and kSplitMaxSegs segments and at most 5 digits (else
+// the one-set kernel of the same rows runs).
 constexpr int kSplitMaxSteps = 512, kSplitMaxSegs = 64;
 // K steps (64 bytes of one segment each) of a class with segments of seg_n rows.
 __host__ __device__ inline int split2_steps(const int *seg_n, int nseg)

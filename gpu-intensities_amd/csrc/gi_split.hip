@@ -496,7 +496,7 @@ __global__ __launch_bounds__(1024) void k_fold_split_jobs(const FoldSplitJob *__
 // products of digit i (sched_barrier keeps them there), so their LDS latency
 // hides behind those products instead of stalling the next group. Each
 // accumulator takes its products in the same order as one digit at a time.
-template <int S>
+template <int S, int AROWS = 64>
 __device__ inline void oz_step(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs,
                                int wm, int wn, int lane, v4i (&acc)[2][2][S])
 {
@@ -505,7 +505,7 @@ __device__ inline void oz_step(const int8_t *__restrict__ As, const int8_t *__re
     __builtin_amdgcn_sched_barrier(0);
     const int i16 = lane & 15, k16 = (lane >> 4) * 16;
     auto ldA = [&](int i, int x) {
-        return *(const v4i *)(As + (i * 64 + wm + 16 * x + i16) * kRowB + k16);
+        return *(const v4i *)(As + (i * AROWS + wm + 16 * x + i16) * kRowB + k16);
     };
     v4i a0 = ldA(0, 0), a1 = ldA(0, 1);
     v4i b[S][2];
@@ -690,13 +690,25 @@ __global__ __launch_bounds__(256) void k_halfls_split(int kpad_all, MuSrc src,
 // step s + 2 while step s multiplies. Same products in the same order as
 // k_halfls_split: the same bits. Classes with more than kSplitMaxSteps steps use
 // k_halfls_split.
-template <int S>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_halfls_split2(
+// MW_ wavefronts along the rows: a (32*MW_) x 64 tile on 2*MW_ wavefronts of
+// 32 x 32 (MW_ 2: 64 x 64 on 4; MW_ 4: 128 x 64 on 8, which stage each K step's
+// 64 G columns once for twice the rows); every thread loads one A row's 16 bytes
+// per digit, the first 256 one B row's. The 8-wavefront tile runs one
+// workgroup per CU, so it double-buffers its LDS operands (up to 120 KB) and
+// waits at one barrier per K step: the step that writes a buffer follows the
+// barrier of the step after the one that read it.
+template <int S, int MW_ = 2>
+__global__ __launch_bounds__(128 * MW_) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_halfls_split2(
     int kpad_all, MuSrc src, const int *__restrict__ mue, const SplitHalfJob *__restrict__ jobs,
     const int *__restrict__ tbegin, int njobs, int ntiles, double R)
 {
-    __shared__ __attribute__((aligned(16))) int8_t As[S * 64 * kRowB];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[S * 64 * kRowB];
+    constexpr int TMR = 32 * MW_, NT = 128 * MW_;
+    constexpr bool DB = MW_ == 4;
+    constexpr int ASZ = S * TMR * kRowB, BSZ = S * 64 * kRowB;
+    __shared__ __attribute__((aligned(16))) int8_t As[(DB ? 2 : 1) * ASZ];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[(DB ? 2 : 1) * BSZ];
+    int8_t *const As1 = DB ? As + ASZ : As;
+    int8_t *const Bs1 = DB ? Bs + BSZ : Bs;
     __shared__ long long s_a[kSplitMaxSteps];
     __shared__ int s_b[kSplitMaxSteps];
     __shared__ int s_first[kSplitMaxSegs + 1];
@@ -705,11 +717,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const SplitHalfJob &J = jobs[find_tile_job(tbegin, njobs, tile)];
     const ClassDev &cd = J.cd;
     const int local = tile - J.tile_begin;
-    const int m0 = J.m_lo + (local % J.mt) * 64, n0 = (local / J.mt) * 64;
+    const int m0 = J.m_lo + (local % J.mt) * TMR, n0 = (local / J.mt) * 64;
     const int8_t *__restrict__ mud = (const int8_t *)src.mu;
     const long long muplane = src.plane;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int wm = (wave % MW_) * 32, wn = (wave / MW_) * 32;
     const int ncolsTot = J.nunits * cd.ncols;
     // step table: segment e's steps start at s_first[e]
     __shared__ int s_fits;
@@ -726,8 +738,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     // the host sends only classes within both limits to this kernel; one that
     // is not gets NaN in its tile rather than a sum with K steps left out
     if (!s_fits) {
-        for (int i = tid; i < 64 * 64; i += 256) {
-            const int m = m0 + (i & 63), nn = n0 + (i >> 6);
+        for (int i = tid; i < TMR * 64; i += NT) {
+            const int m = m0 + (i % TMR), nn = n0 + (i / TMR);
             if (m >= J.m_hi || nn >= ncolsTot) continue;
             const int u = nn / cd.ncols, jj = nn - u * cd.ncols;
             J.H[(size_t)u * J.Htot + cd.hbase + (size_t)jj * cd.n + m] = __builtin_nan("");
@@ -736,7 +748,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
     const int nsteps = s_first[cd.nseg];
     for (int e = 0; e < cd.nseg; e++)
-        for (int st = s_first[e] + tid; st < s_first[e + 1]; st += 256) {
+        for (int st = s_first[e] + tid; st < s_first[e + 1]; st += NT) {
             const int kk = (st - s_first[e]) * 64;
             s_a[st] = ((long long)cd.seg_a[e] * src.ld + cd.off + m0 - src.v0) * kpad_all +
                       cd.seg_pad[2 * e + 1] + kk;
@@ -745,8 +757,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     __syncthreads();
     const int lr = tid >> 2, lp = (tid & 3) * 16;
     const bool mok = m0 + lr < J.m_hi;
+    const bool bload = tid < 256;  // the threads that stage a B row (64 rows)
     const int n = n0 + lr;
-    const bool nok = n < ncolsTot;
+    const bool nok = bload && n < ncolsTot;
     long long bcol = 0;
     if (nok) {
         const int u = n / cd.ncols, jj = n - u * cd.ncols;
@@ -771,27 +784,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             RB[q] = nok ? *(gptr<v4i>)(gmem(J.Gd) + q * J.gplane + bo) : v4i{0, 0, 0, 0};          \
         }                                                                                      \
     } while (0)
-#define GI_STAGE(RA, RB)                                                                       \
+#define GI_STAGE(AS, BS, RA, RB)                                                               \
     do {                                                                                       \
         _Pragma("unroll") for (int q = 0; q < S; q++) {                                        \
-            *(v4i *)(As + (q * 64 + lr) * kRowB + lp) = RA[q];                                 \
-            *(v4i *)(Bs + (q * 64 + lr) * kRowB + lp) = RB[q];                                 \
+            *(v4i *)(AS + (q * TMR + lr) * kRowB + lp) = RA[q];                                \
+            if (bload) *(v4i *)(BS + (q * 64 + lr) * kRowB + lp) = RB[q];                      \
         }                                                                                      \
     } while (0)
     if (nsteps > 0) GI_FETCH(ra0, rb0, 0);
     if (nsteps > 1) GI_FETCH(ra1, rb1, 1);
     for (int st = 0; st < nsteps; st += 2) {
-        GI_STAGE(ra0, rb0);
+        GI_STAGE(As, Bs, ra0, rb0);
         __syncthreads();
         if (st + 2 < nsteps) GI_FETCH(ra0, rb0, st + 2);
-        oz_step<S>(As, Bs, wm, wn, lane, acc);
-        __syncthreads();
+        oz_step<S, TMR>(As, Bs, wm, wn, lane, acc);
+        if (!DB) __syncthreads();
         if (st + 1 >= nsteps) break;
-        GI_STAGE(ra1, rb1);
+        GI_STAGE(As1, Bs1, ra1, rb1);
         __syncthreads();
         if (st + 3 < nsteps) GI_FETCH(ra1, rb1, st + 3);
-        oz_step<S>(As, Bs, wm, wn, lane, acc);
-        __syncthreads();
+        oz_step<S, TMR>(As1, Bs1, wm, wn, lane, acc);
+        if (!DB) __syncthreads();
     }
 #undef GI_FETCH
 #undef GI_STAGE
@@ -1165,17 +1178,21 @@ void launch_halfls_split(hipStream_t st, int S, double R, int kpad_all, const Mu
                          int njobs, int ntiles, int cfg)
 {
     if (njobs <= 0 || ntiles <= 0) return;
-    if (cfg == 2 && S <= 5) {  // two register sets exceed the registers at 6 digits
-#define L_(s) hipLaunchKernelGGL(k_halfls_split2<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, kpad_all, src, mue, d_jobs, d_tbegin, njobs, ntiles, R)
-        switch (S) {
-        case 3: L_(3); break;
-        case 4: L_(4); break;
-        default: L_(5); break;
+    if ((cfg == 2 || cfg == 3) && S <= 5) {  // two register sets exceed the registers at 6 digits
+        const int mw = cfg == 3 ? 4 : 2;
+#define L_(s, w) hipLaunchKernelGGL((k_halfls_split2<s, w>), dim3(xcd_grid_s(ntiles)), dim3(128 * w), 0, st, kpad_all, src, mue, d_jobs, d_tbegin, njobs, ntiles, R)
+        switch (S * 8 + mw) {
+        case 3 * 8 + 2: L_(3, 2); break;
+        case 4 * 8 + 2: L_(4, 2); break;
+        case 5 * 8 + 2: L_(5, 2); break;
+        case 3 * 8 + 4: L_(3, 4); break;
+        case 4 * 8 + 4: L_(4, 4); break;
+        default: L_(5, 4); break;
         }
 #undef L_
         return;
     }
-    if (cfg == 1) {
+    if (cfg == 1 || cfg == 3) {  // 128 rows; setting 3 at 6 digits
 #define L_(s) hipLaunchKernelGGL(k_halfls_split_wide<s>, dim3(xcd_grid_s(ntiles)), dim3(256), 0, st, kpad_all, src, mue, d_jobs, d_tbegin, njobs, ntiles, R)
         GI_SPLIT_DISPATCH(S, L_)
 #undef L_

@@ -316,7 +316,10 @@ typedef struct gi_tuning {
     int corr_cfg;
     int pipe_rounds;
     int split_cfg;    /* GI_MODE_SPLIT half-ls tiles: 0 = 64 x 64, 1 = 128 x 64,
-                         2 = 64 x 64 with two K steps of loads in flight */
+                         2 = 64 x 64 with two K steps of loads in flight,
+                         3 = 128 x 64 with two K steps in flight (8 wavefronts),
+                         4 = per round 3 when its tiles average at least 40 K
+                         steps of 64, else 2 (the default) */
     int pipe_share;   /* percent of the remaining work per round but the last */
 } gi_tuning;
 int gi_set_tuning(gi_ctx *ctx, const gi_tuning *t);

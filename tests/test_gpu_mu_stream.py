@@ -36,7 +36,7 @@ def _same(a, b, what):
 @pytest.fixture
 def engine_resident(engine):
     yield engine
-    engine.set_tuning(4, 0, 4, 2, 50)
+    engine.set_tuning(4, 0, 4, 4, 50)
 
 
 @pytest.mark.parametrize("sym,kw", [
@@ -90,11 +90,11 @@ def test_streamed_mu_every_tile_and_route(engine_resident, monkeypatch):
         print(f"[mu_stream] half-ls tile {hl}", flush=True)
         eng.set_tuning(hl, 0, 3, 2, 50)
         _same(ref[B], eng.compute(mode=B), ("tile", hl))
-    for sc in range(3):
+    for sc in range(5):
         print(f"[mu_stream] split half-ls tile {sc}", flush=True)
         eng.set_tuning(4, 0, 3, sc, 50)
         _same(ref[SP], eng.compute(mode=SP), ("split tile", sc))
-    eng.set_tuning(4, 0, 4, 2, 50)
+    eng.set_tuning(4, 0, 4, 4, 50)
     monkeypatch.setenv("GI_BATCH_BYTES", str(1 << 20))
     for m in (B, SP):
         _same(ref[m], eng.compute(mode=m), ("small rounds", m))

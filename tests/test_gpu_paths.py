@@ -69,12 +69,21 @@ def test_global_gather_g_build_same_bits(routed, mode, rec_lds):
         assert split_error(glob, ref)["max"] <= 1e-6
 
 
-def test_one_step_split_halfls_same_bits(routed):
+@pytest.mark.parametrize("split_cfg", [2, 3])
+def test_one_step_split_halfls_same_bits(routed, split_cfg):
+    # over the step table, 64 x 64 with two steps in flight falls back to
+    # 64 x 64 and 128 x 64 with two steps to 128 x 64
     job, eng, ref = routed
-    print(f"[paths] {job.sym['name']}: one-step split half-ls", flush=True)
-    base = _run(eng, SP)
-    for lim in (dict(split2_steps=1), dict(split2_segs=1)):
-        _bitwise(_run(eng, SP, **lim), base)
+    print(f"[paths] {job.sym['name']}: one-step split half-ls from tile {split_cfg}",
+          flush=True)
+    t = eng.get_tuning()
+    eng.set_tuning(**dict(t, split_cfg=split_cfg))
+    try:
+        base = _run(eng, SP)
+        for lim in (dict(split2_steps=1), dict(split2_segs=1)):
+            _bitwise(_run(eng, SP, **lim), base)
+    finally:
+        eng.set_tuning(**t)
 
 
 def test_two_kernel_fold_split_same_bits(routed):

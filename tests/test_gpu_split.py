@@ -241,18 +241,19 @@ def test_split_requests_alone_and_mixed(engine):
 @pytest.mark.parametrize("sym,kw", [("C3V", dict(nvib=40, jvals=(3, 4), seed=19)),
                                     ("C2V", dict(nvib=64, jvals=(4, 5), seed=31, e_span=30000.0))])
 def test_split_tile_settings_same_bits(engine, sym, kw):
-    """Every split half-ls setting (64 x 64, 128 x 64, two K steps in flight)
-    gives the same bits: the int32 level sums are exact in any order."""
+    """Every split half-ls setting (64 x 64, 128 x 64, each also with two K
+    steps in flight, and the per-round choice) gives the same bits: the int32
+    level sums are exact in any order."""
     job = synth.make_job(sym, **kw)
     engine.load(job)
     try:
         engine.set_tuning(3, 0, 3, 0)
         ref = {k: v.copy() for k, v in engine.compute(mode=binding.GI_MODE_SPLIT).items()}
-        for cfg in (1, 2):
+        for cfg in (1, 2, 3, 4):
             engine.set_tuning(3, 0, 3, cfg)
             got = engine.compute(mode=binding.GI_MODE_SPLIT)
             for k in ref:
                 assert np.array_equal(ref[k], got[k]), (cfg, k)
     finally:
-        engine.set_tuning(4, 0, 4, 2, 50)
+        engine.set_tuning(4, 0, 4, 4, 50)
     assert split_error(ref, oracle_py.OracleJob(job).run())["max"] <= TOL_S

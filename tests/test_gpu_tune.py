@@ -13,7 +13,7 @@ from test_gpu_parity import compare
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = dict(halfls_cfg=4, corr_cfg=0, pipe_rounds=4, split_cfg=2, pipe_share=50)
+DEFAULT = dict(halfls_cfg=4, corr_cfg=0, pipe_rounds=4, split_cfg=4, pipe_share=50)
 
 
 @pytest.fixture
@@ -39,7 +39,7 @@ def test_autotune_picks_a_setting_and_keeps_the_bits(tuned_engine, mode):
     assert lines[0].startswith("# gi_autotune") and lines[-1].startswith("best ")
     kinds = [ln.split()[0] for ln in lines[1:-1]]
     want = (["halfls_tile"] * 6 + ["corr_tile"] * 4 if mode == binding.GI_MODE_FP64_BATCHED
-            else ["split_halfls_tile"] * 3) + ["pipe_rounds"] * 21  # 1 round, 5 x 4 pairs
+            else ["split_halfls_tile"] * 5) + ["pipe_rounds"] * 21  # 1 round, 5 x 4 pairs
     if mode == binding.GI_MODE_FP64_BATCHED:
         want += ["halfls_tile"] * 6  # the half-ls tile again at the chosen rounds
     assert kinds == want
@@ -79,7 +79,7 @@ def test_every_round_count_gives_the_same_bits(tuned_engine, monkeypatch):
     # the split half-ls tile shapes: exact integer level sums, so the same bits
     eng.set_tuning(3, 0, 2, 0)
     ref = _copy(eng.compute(mode=binding.GI_MODE_SPLIT))
-    for cfg in (1, 2):
+    for cfg in (1, 2, 3, 4):
         eng.set_tuning(3, 0, 2, cfg)
         got = eng.compute(mode=binding.GI_MODE_SPLIT)
         for k in ref:
@@ -99,7 +99,7 @@ def test_set_tuning_rules(tuned_engine):
     eng = tuned_engine
     job = synth.make_job("C2V", nvib=16, jvals=(0, 1), seed=54)
     eng.load(job)
-    for bad in ((6, 0, 2, 0), (0, -1, 2, 0), (0, 0, 0, 0), (0, 0, 65, 0), (0, 0, 2, 3),
+    for bad in ((6, 0, 2, 0), (0, -1, 2, 0), (0, 0, 0, 0), (0, 0, 65, 0), (0, 0, 2, 5),
                 (0, 0, 2, 0, 9), (0, 0, 2, 0, 91)):
         with pytest.raises(binding.GIError):
             eng.set_tuning(*bad)
