@@ -80,7 +80,8 @@ def test_split_d3h_reduced_digits(engine):
     synthetic job (cancellation in its sums) ends 1.2e-6 from FP64, above the
     1e-6 bar; 6 digits of radix 254 bring every line within it. The error is
     the truncation of the digits, not the structure: it falls by about the
-    radix per digit."""
+    radix per digit. It is the half-ls product's: 6 digits there alone bring
+    the job within the bar, 6 in the correlation alone do not."""
     job = synth.make_job("D3H", nvib=24, jvals=(2, 3), seed=61, reduced=1)
     engine.load(job)
     ref = oracle_py.OracleJob(job).run()
@@ -89,11 +90,17 @@ def test_split_d3h_reduced_digits(engine):
         engine.set_split_digits(6)
         e6 = split_error(engine.compute(mode=binding.GI_MODE_SPLIT), ref)["max"]
         assert engine.split_format() == (254, 6, 6)
+        engine.set_split_format(254, 6, 5)
+        e65 = split_error(engine.compute(mode=binding.GI_MODE_SPLIT), ref)["max"]
+        engine.set_split_format(254, 5, 6)
+        e56 = split_error(engine.compute(mode=binding.GI_MODE_SPLIT), ref)["max"]
     finally:
         engine.set_split_format(0, 0, 0)
-    print(f"split D3h reduced: 5 digits {e5:.2e}, 6 digits {e6:.2e}")
+    print(f"split D3h reduced: 5 digits {e5:.2e}, 6 digits {e6:.2e}, "
+          f"6 half-ls / 5 correlation {e65:.2e}, 5 / 6 {e56:.2e}")
     assert e6 <= TOL_S
     assert e5 > 30 * e6 and e5 < 1e-5
+    assert e65 <= TOL_S and e56 > TOL_S
 
 
 def test_split_larger_many_rounds(engine, monkeypatch):
