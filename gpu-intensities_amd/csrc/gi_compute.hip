@@ -1535,9 +1535,11 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
         if (rc) return rc;
     }
     const size_t mc = (size_t)mchunks;
+    // (the fused fold-split jobs come in three tables, by column length, whose
+    // jobs together are at most mc: the zeros reserve the others' alignment)
     const size_t rtab_cap = pack_cap(
         {mc * (sizeof(GSplitJob) + 4), mc * (sizeof(GSplitJob) + 4),
-         (size_t)mjobs * (sizeof(SplitHalfJob) + 4), mc * (sizeof(FoldSplitJob) + 4),
+         (size_t)mjobs * (sizeof(SplitHalfJob) + 4), mc * (sizeof(FoldSplitJob) + 4), 0, 0,
          mc * (sizeof(FoldJob) + 4), mc * (sizeof(SplitRowsJob) + 4),
          mc * (sizeof(CorrSplitArgs) + 4), mc * (sizeof(IntensArgs) + 4)});
     unsigned char *rtab = (unsigned char *)slot_scratch(c, S, "rtab", rtab_cap, &rc);
@@ -1693,9 +1695,11 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
         // and goes to the device in one copy
         std::vector<FoldJob> fjobs;
         std::vector<SplitRowsJob> sjobs;
-        std::vector<FoldSplitJob> fsjobs;
-        std::vector<int> fbegin, sbegin;
-        int fblocks = 0, srows = 0;
+        // fused fold-split jobs by column length (one launch per class): at most
+        // kFoldSplitSmall rows, at most 16,384, longer
+        std::vector<FoldSplitJob> fsjobs[3];
+        std::vector<int> fbegin, sbegin, fsbegin[3];
+        int fblocks = 0, srows = 0, fscols[3] = {0, 0, 0};
         for (size_t i = rg.first; i < rg.second; i++) {
             const Chunk &ch = chunks[i];
             const size_t k = i - rg.first;
@@ -1721,8 +1725,11 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
                 fblocks += fj.xt * cols;
                 sjobs.push_back(SplitRowsJob{Ht + Htoff[k], (long long)BF.nsize, BF.nsize, BF.Kp,
                                              Hd + Hdoff[k], (long long)mhd, He + Hcoff[k]});
-                fsjobs.push_back(FoldSplitJob{fj, BF.Kp, Hd + Hdoff[k], (long long)mhd,
-                                              He + Hcoff[k]});
+                const int fc = BF.Kp <= kFoldSplitSmall ? 0 : BF.Kp <= 16384 ? 1 : 2;
+                fsjobs[fc].push_back(
+                    FoldSplitJob{fj, BF.Kp, Hd + Hdoff[k], (long long)mhd, He + Hcoff[k]});
+                fsbegin[fc].push_back(fscols[fc]);
+                fscols[fc] += cols;
                 sbegin.push_back(srows);
                 srows += cols;
             }
@@ -1767,15 +1774,16 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
         const int iblocks = round_intensity_jobs(c, P, rg.first, rg.second, d_meta, K, flag, cand,
                                                  ijobs, ibegin);
         TablePack T(rtab);
-        const int *b_gj = nullptr, *b_gg = nullptr, *b_hj = nullptr, *b_fs = nullptr,
+        const int *b_gj = nullptr, *b_gg = nullptr, *b_hj = nullptr,
                   *b_fj = nullptr, *b_sj = nullptr, *b_cj = nullptr, *b_ij = nullptr;
         const GSplitJob *t_gj = T.add(gj, gbegin, &b_gj), *t_gg = T.add(gjg, gbegin_g, &b_gg);
         const SplitHalfJob *t_hj = c->mu_streamed ? nullptr : T.add(hj, tbegin, &b_hj);
-        const FoldSplitJob *t_fs = nullptr;
+        const FoldSplitJob *t_fs[3] = {nullptr, nullptr, nullptr};
+        const int *b_fsc[3] = {nullptr, nullptr, nullptr};
         const FoldJob *t_fj = nullptr;
         const SplitRowsJob *t_sj = nullptr;
         if (fused) {
-            t_fs = T.add(fsjobs, sbegin, &b_fs);
+            for (int fc = 0; fc < 3; fc++) t_fs[fc] = T.add(fsjobs[fc], fsbegin[fc], &b_fsc[fc]);
         } else {
             t_fj = T.add(fjobs, fbegin, &b_fj);
             t_sj = T.add(sjobs, sbegin, &b_sj);
@@ -1829,8 +1837,10 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
         }
         if (!fjobs.empty() && fused) {
             TL_BEGIN("fold_split");
-            launch_fold_split_jobs(S.stream, SD, c->split_radix, t_fs, b_fs, (int)fsjobs.size(),
-                                   srows, maxkp);
+            const int fmax[3] = {kFoldSplitSmall, 16384, maxkp};
+            for (int fc = 0; fc < 3; fc++)
+                launch_fold_split_jobs(S.stream, SD, c->split_radix, t_fs[fc], b_fsc[fc],
+                                       (int)fsjobs[fc].size(), fscols[fc], fmax[fc]);
             TL_END();
         } else if (!fjobs.empty()) {
             TL_BEGIN("fold");

@@ -394,7 +394,9 @@ struct FoldSplitJob {
     long long plane;
     int *E;                          // exponent per column
 };
-// maxrows: the largest Kp of the jobs (<= kFoldSplitMax).
+// maxrows: the largest Kp of the jobs (<= kFoldSplitMax); at most
+// kFoldSplitSmall rows run at two workgroups per CU.
+constexpr int kFoldSplitSmall = 12288;
 void launch_fold_split_jobs(hipStream_t st, int S, double R, const FoldSplitJob *d_jobs,
                             const int *d_begin, int njobs, int ncols, int maxrows);
 // The intensity steps of every chunk of a round: blocks of a job = ceil(ncand / 256).

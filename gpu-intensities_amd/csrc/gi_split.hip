@@ -416,8 +416,12 @@ __global__ __launch_bounds__(1024) void k_build_g_split_lds(const GSplitJob *__r
     else g_split_passes<S, 1, 4, PART>(J, d, u, um, rec, src, nl, cls, R);
 }
 
+// Fold and digit split of one folded column per workgroup: the column's rows
+// (PER per thread), its maximum, its digits. Columns of at most 12,288 rows
+// (PER 12) fit 64 registers, so two workgroups share a CU and hide each
+// other's gather latency.
 template <int S, int PER>
-__global__ __launch_bounds__(1024) void k_fold_split_jobs(const FoldSplitJob *__restrict__ jobs,
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(PER * 1024 <= kFoldSplitSmall ? 8 : 1))) void k_fold_split_jobs(const FoldSplitJob *__restrict__ jobs,
                                                           const int *__restrict__ begin,
                                                           int njobs, double R)
 {
@@ -435,20 +439,21 @@ __global__ __launch_bounds__(1024) void k_fold_split_jobs(const FoldSplitJob *__
     double v[PER];
     double m = 0.0;
     if constexpr (PER <= 16) {
-        // four rows at a time with their loads in flight together (fold_rows)
+        // two rows at a time with their loads in flight together (fold_rows;
+        // four at a time cost the register room of a second workgroup per CU)
 #pragma unroll
-        for (int k0 = 0; k0 < PER; k0 += 4) {
-            int t[4];
-            bool ok[4];
-            double acc[4];
+        for (int k0 = 0; k0 < PER; k0 += 2) {
+            int t[2];
+            bool ok[2];
+            double acc[2];
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
+            for (int r = 0; r < 2; r++) {
                 t[r] = (int)threadIdx.x + (k0 + r) * 1024;
                 ok[r] = t[r] < F.nsizeF && unit >= 0;
             }
-            fold_rows<4>(T, Hu, t, ok, acc);
+            fold_rows<2>(T, Hu, t, ok, acc);
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
+            for (int r = 0; r < 2; r++) {
                 v[k0 + r] = acc[r];
                 m = fmax(m, fabs(acc[r]));
             }
@@ -1123,6 +1128,12 @@ void launch_fold_split_jobs(hipStream_t st, int S, double R, const FoldSplitJob 
                             const int *d_begin, int njobs, int ncols, int maxrows)
 {
     if (njobs <= 0 || ncols <= 0) return;
+    if (maxrows <= kFoldSplitSmall) {
+#define L_(s) hipLaunchKernelGGL((k_fold_split_jobs<s, kFoldSplitSmall / 1024>), dim3(ncols), dim3(1024), 0, st, d_jobs, d_begin, njobs, R)
+        GI_SPLIT_DISPATCH(S, L_)
+#undef L_
+        return;
+    }
     if (maxrows <= 16384) {
 #define L_(s) hipLaunchKernelGGL((k_fold_split_jobs<s, 16>), dim3(ncols), dim3(1024), 0, st, d_jobs, d_begin, njobs, R)
         GI_SPLIT_DISPATCH(S, L_)
