@@ -388,6 +388,9 @@ static void free_plan(gi_ctx *c)
             (void)hipFree(G.d_gidxB[d]);
             (void)hipFree(G.d_gwtB[d]);
             (void)hipFree(G.d_gbnd[d]);
+            (void)hipFree(G.d_cidx[d]);
+            (void)hipFree(G.d_ccode[d]);
+            (void)hipFree(G.d_cwt[d]);
         }
     }
     c->groups.clear();
@@ -822,12 +825,20 @@ int gi_prepare(gi_ctx *c)
                         if ((rc = upload(c, &G.d_gidxB[d], G.gidxB[d])) != GI_OK) return rc;
                         if ((rc = upload(c, &G.d_gwtB[d], G.gwtB[d])) != GI_OK) return rc;
                         if ((rc = upload(c, &G.d_gbnd[d], G.gbnd[d])) != GI_OK) return rc;
+                        if (G.cmap &&
+                            ((rc = upload(c, &G.d_cidx[d], G.cidx[d])) != GI_OK ||
+                             (rc = upload(c, &G.d_ccode[d], G.ccode[d])) != GI_OK ||
+                             (rc = upload(c, &G.d_cwt[d], G.cwt[d])) != GI_OK))
+                            return rc;
                     }
                 for (int d = 0; d < GI_MAXDEG; d++) {  // host copies are not needed again
                     std::vector<int>().swap(G.gidx[d]);
                     std::vector<double>().swap(G.gwt[d]);
                     std::vector<int>().swap(G.gidxB[d]);
                     std::vector<double>().swap(G.gwtB[d]);
+                    std::vector<uint16_t>().swap(G.cidx[d]);
+                    std::vector<uint8_t>().swap(G.ccode[d]);
+                    std::vector<double>().swap(G.cwt[d]);
                 }
             }
     // level filters and per-block energy order

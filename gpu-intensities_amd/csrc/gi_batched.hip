@@ -781,19 +781,50 @@ __global__ __launch_bounds__(1024) void k_build_g_map_lds(const GMapJob *__restr
                                                           int njobs)
 {
     __shared__ double rec[kRecLds];
+    __shared__ double cw[kCmapCodes];  // the compact map's weights
     const int ji = find_job(ubegin, njobs, blockIdx.x);
     const GMapJob &J = jobs[ji];
     const int u = blockIdx.x - ubegin[ji];
     const gptr<double> src = gmem(J.uptr[u]);
     const int nl = PART ? min(J.nsize, J.nlds) : J.nsize;
+    const int d = J.uideg[u];
+    const bool cm = J.cidx[d] != nullptr;  // uniform over the workgroup
     for (int i = threadIdx.x; i < nl; i += 1024) rec[i] = src[i];
+    if (cm && threadIdx.x < kCmapCodes) cw[threadIdx.x] = J.cwt[d][threadIdx.x];
     __syncthreads();
     auto at = [&](int i) { return PART ? (i < nl ? rec[i] : src[i]) : rec[i]; };
-    const int d = J.uideg[u];
-    const gptr<int> ip = gmem(J.idx[d]);
-    const gptr<double> wp = gmem(J.wt[d]);
     const int Gt = J.Gtot, W = J.W;
     const gout<double> G = gmem_w(J.G) + (size_t)u * Gt;
+    if (cm) {
+        // the compact map: 3 bytes per entry, the weight from LDS (same doubles)
+        const gptr<unsigned short> ip = gmem(J.cidx[d]);
+        const gptr<unsigned char> cp = gmem(J.ccode[d]);
+        for (int g0 = threadIdx.x; g0 < Gt; g0 += 4096) {
+            int ix[4][4], cd[4][4];
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int w = 0; w < 4; w++) {
+                    const int g = g0 + 1024 * r;
+                    const bool ok = w < W && g < Gt;
+                    ix[r][w] = ok ? (int)ip[(size_t)w * Gt + g] : 0;
+                    cd[r][w] = ok ? (int)cp[(size_t)w * Gt + g] : 0;
+                }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int g = g0 + 1024 * r;
+                if (g >= Gt) break;
+                double v = 0.0;
+#pragma unroll
+                for (int w = 0; w < 4; w++)
+                    if (w < W) v += cw[cd[r][w]] * at(ix[r][w]);
+                G[g] = v;
+            }
+        }
+        return;
+    }
+    const gptr<int> ip = gmem(J.idx[d]);
+    const gptr<double> wp = gmem(J.wt[d]);
     // four elements per thread per pass, 1024 apart: every map load of the pass
     // is in flight before the first gather (one element at a time left the
     // waves waiting on L2 latency most of the time)

@@ -1220,10 +1220,16 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             Htc[i - rg.first] = Ht + oht;
             if (ch.nunits > 0 && GP.Gtot_p > 0 && GP.gW <= kGMapMaxW && GP.d_gidx[0]) {
                 GMapJob mj;
+                // the compact map at the default routing (another rec_lds limit
+                // selects the full map, for comparison)
+                const bool cm = GP.cmap && c->lim_rec_lds == kRecLds;
                 for (int d = 0; d < 2; d++) {
                     const int dd = d < ndegI ? d : 0;
                     mj.idx[d] = GP.d_gidx[dd];
                     mj.wt[d] = GP.d_gwt[dd];
+                    mj.cidx[d] = cm ? GP.d_cidx[dd] : nullptr;
+                    mj.ccode[d] = cm ? GP.d_ccode[dd] : nullptr;
+                    mj.cwt[d] = cm ? GP.d_cwt[dd] : nullptr;
                 }
                 mj.W = GP.gW;
                 mj.Gtot = GP.Gtot_p;

@@ -121,6 +121,16 @@ struct GroupPlan {
     int *d_gidxB[GI_MAXDEG] = {nullptr, nullptr};
     double *d_gwtB[GI_MAXDEG] = {nullptr, nullptr}, *d_gbnd[GI_MAXDEG] = {nullptr, nullptr};
     int gW = 0;
+    // compact form of the packed (FP64) layout of the map (build_compact_map):
+    // 16-bit record index and 8-bit weight code per entry, kCmapCodes weights
+    // per component
+    bool cmap = false;
+    std::vector<uint16_t> cidx[GI_MAXDEG];
+    std::vector<uint8_t> ccode[GI_MAXDEG];
+    std::vector<double> cwt[GI_MAXDEG];
+    uint16_t *d_cidx[GI_MAXDEG] = {nullptr, nullptr};
+    uint8_t *d_ccode[GI_MAXDEG] = {nullptr, nullptr};
+    double *d_cwt[GI_MAXDEG] = {nullptr, nullptr};
     std::vector<int> gidx[GI_MAXDEG];
     std::vector<double> gwt[GI_MAXDEG];
     int *d_gidx[GI_MAXDEG] = {nullptr, nullptr};
@@ -396,6 +406,7 @@ int build_symmaps(gi_ctx *c, int jind);
 int build_coef(gi_ctx *c, int jindI, int jindF);
 int build_group(gi_ctx *c, int jindI, int gI, int jindF);
 int build_gmap(gi_ctx *c, GroupPlan &G);
+void build_compact_map(gi_ctx *c, GroupPlan &G);
 double three_j(int j1, int j2, int j3, int k1, int k2, int k3);
 void threej_table(int jmax, double *out);
 bool energy_filter_lower(const gi_ctx *c, int J, double e, const int *q);

@@ -157,15 +157,23 @@ struct GMapJob {
     double *G;
     int nsize;                       // record length of the lower block
     int nlds;                        // leading doubles of a record staged in LDS (<= kRecLds)
+    // the compact map (GroupPlan::cmap; null when the group has none): 16-bit
+    // record index and 8-bit code into kCmapCodes weights per entry
+    const unsigned short *cidx[2];
+    const unsigned char *ccode[2];
+    const double *cwt[2];
 };
+// Weights of a compact record map per lower component (code 0 = weight 0).
+constexpr int kCmapCodes = 256;
 constexpr int kGMapMaxW = 16;
 constexpr int kClsLds = 32;  // output classes of a group that k_build_g_split_lds keeps in LDS
 constexpr int kGUnits = 16;
 // Doubles of a record the one-unit-per-workgroup G builds stage in LDS
-// (launch_build_g_map_lds, launch_build_g_split_lds): 158 KB of the 160 KB a
-// gfx950 workgroup can hold, leaving room for the split build's class table.
-// Longer records keep their tail in global memory.
-constexpr int kRecLds = 20224;
+// (launch_build_g_map_lds, launch_build_g_split_lds): 156 KB of the 160 KB a
+// gfx950 workgroup can hold, leaving room for the compact map's weights and
+// the split build's class table. Longer records keep their tail in global
+// memory.
+constexpr int kRecLds = 19968;
 void launch_build_g_map(hipStream_t st, const GMapJob *d_jobs, const int *d_begin, int njobs,
                         int ntiles);
 // One workgroup per unit, the first nlds doubles of its record in LDS (maps of

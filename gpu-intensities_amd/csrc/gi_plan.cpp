@@ -471,7 +471,71 @@ int build_gmap(gi_ctx *c, GroupPlan &G)
             G.gwtB[d].resize((size_t)G.gW * G.GtotB, 0.0);
         }
     }
+    build_compact_map(c, G);
     return GI_OK;
+}
+
+// Compact record map for the LDS-staged FP64 G build: the record index of
+// each entry in 16 bits and its weight as an 8-bit code into a table of the
+// component's distinct weights (code 0 = weight 0, the padding), packed
+// layout. A G element then reads 3 bytes per entry instead of 12; its products
+// are the same doubles. Only for maps of width <= 4 whose indices fit 16 bits
+// and whose components have at most kCmapCodes distinct weights (bit
+// patterns). (The split layout keeps the full map: its build loads four rows'
+// entries as 16-byte vectors, and the table lookups cost it more than the
+// bytes they save.)
+void build_compact_map(gi_ctx *c, GroupPlan &G)
+{
+    G.cmap = false;
+    for (int d = 0; d < GI_MAXDEG; d++) {
+        G.cidx[d].clear();
+        G.ccode[d].clear();
+        G.cwt[d].clear();
+    }
+    if (G.gW > 4 || G.Gtot_p <= 0) return;
+    const int nd = c->degen[G.gI];
+    for (int d = 0; d < nd; d++) {
+        std::map<uint64_t, int> code;
+        std::vector<double> tab(1, 0.0);
+        code[0] = 0;
+        bool ok = true;
+        auto enc = [&](const std::vector<int> &ix, const std::vector<double> &wt,
+                       std::vector<uint16_t> &cix, std::vector<uint8_t> &ccd) {
+            cix.resize(ix.size());
+            ccd.resize(wt.size());
+            for (size_t i = 0; i < ix.size() && ok; i++) {
+                if (ix[i] < 0 || ix[i] > 65535) {
+                    ok = false;
+                    break;
+                }
+                cix[i] = (uint16_t)ix[i];
+                uint64_t b;
+                std::memcpy(&b, &wt[i], 8);
+                auto it = code.find(b);
+                if (it == code.end()) {
+                    if ((int)tab.size() == kCmapCodes) {
+                        ok = false;
+                        break;
+                    }
+                    it = code.emplace(b, (int)tab.size()).first;
+                    tab.push_back(wt[i]);
+                }
+                ccd[i] = (uint8_t)it->second;
+            }
+        };
+        enc(G.gidx[d], G.gwt[d], G.cidx[d], G.ccode[d]);
+        if (!ok) {
+            for (int e = 0; e < GI_MAXDEG; e++) {
+                std::vector<uint16_t>().swap(G.cidx[e]);
+                std::vector<uint8_t>().swap(G.ccode[e]);
+                std::vector<double>().swap(G.cwt[e]);
+            }
+            return;
+        }
+        tab.resize(kCmapCodes, 0.0);
+        G.cwt[d] = tab;
+    }
+    G.cmap = true;
 }
 
 // Coefficients of the half linestrength jvals[jindI] -> jvals[jindF]: the

@@ -41,7 +41,7 @@ def test_c2_sizes(c2):
     job, eng, sample, ref = c2
     nsz = [int(n) for b in job.bsets for n in b.nsize]
     assert min(nsz) >= 4 * 2048            # split K: 4 parts in both matrix-core modes
-    assert max(nsz) <= 20224               # records staged in LDS by the G builds
+    assert max(nsz) <= 19968               # records staged in LDS by the G builds
     assert len(cu.lower_ok(job)) > 2500 and len(ref["S"]) > 10000
 
 

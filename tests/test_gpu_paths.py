@@ -2,7 +2,7 @@
 
 At real sizes the batched modes switch kernels on the size of a product:
 split-K correlation (upper blocks of 2048 functions and more), G builds with
-only part of a record in LDS (lower records over 20,224 doubles; the
+only part of a record in LDS (lower records over 19,968 doubles; the
 global-gather builds serve maps wider than 4), the one-step split half-ls
 (classes over 512 K steps or 64 segments) and the two-kernel fold and digit
 split (folded columns over 32,768). Lowering those thresholds sends small jobs
