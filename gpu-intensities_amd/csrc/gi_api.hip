@@ -380,7 +380,11 @@ static void free_plan(gi_ctx *c)
             (void)hipFree(oc.d_rcoef);
             (void)hipFree(oc.d_segpad);
         }
-        for (int d = 0; d < GI_MAXDEG; d++) (void)hipFree(G.d_tcol[d]);
+        for (int d = 0; d < GI_MAXDEG; d++) {
+            (void)hipFree(G.d_tcol[d]);
+            (void)hipFree(G.d_tfc[d]);
+            (void)hipFree(G.d_tfw[d]);
+        }
         (void)hipFree(G.d_cd);
         for (int d = 0; d < GI_MAXDEG; d++) {
             (void)hipFree(G.d_gidx[d]);
@@ -810,8 +814,14 @@ int gi_prepare(gi_ctx *c)
                     if ((rc = upload(c, &oc.d_ints, ints)) != GI_OK) return rc;
                     if ((rc = upload(c, &oc.d_rcoef, oc.rcoef)) != GI_OK) return rc;
                 }
-                for (int d = 0; d < c->degen[G.gF]; d++)
-                    if ((rc = upload(c, &G.d_tcol[d], G.tcol[d])) != GI_OK) return rc;
+                for (int d = 0; d < c->degen[G.gF]; d++) {
+                    if ((rc = upload(c, &G.d_tcol[d], G.tcol[d])) != GI_OK ||
+                        (rc = upload(c, &G.d_tfc[d], G.tfc[d])) != GI_OK ||
+                        (rc = upload(c, &G.d_tfw[d], G.tfw[d])) != GI_OK)
+                        return rc;
+                    std::vector<int>().swap(G.tfc[d]);
+                    std::vector<double>().swap(G.tfw[d]);
+                }
                 if ((rc = build_gmap(c, G)) != GI_OK) return rc;
                 for (auto &oc : G.oc)
                     if ((rc = upload(c, &oc.d_segpad, oc.seg_pad)) != GI_OK) return rc;

@@ -1305,10 +1305,14 @@ static int run_batched(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timel
             const int ndegI = c->degen[ch.gI], ndegF = c->degen[ch.gF];
             SymDev t0 = symdev(pF.sym[ch.gF * GI_MAXDEG + 0]);
             t0.tcol = GP.d_tcol[0];
+            t0.tfc = GP.d_tfc[0];
+            t0.tfw = GP.d_tfw[0];
             SymDev t1 = t0;
             if (ndegF > 1) {
                 t1 = symdev(pF.sym[ch.gF * GI_MAXDEG + 1]);
                 t1.tcol = GP.d_tcol[1];
+                t1.tfc = GP.d_tfc[1];
+                t1.tfw = GP.d_tfw[1];
             }
             if (ch.nitems > 0 && BF.nsize > 0) {
                 FoldJob fj{ch.nitems, BF.nsize, ndegI, ndegF, GP.Htot,
@@ -1715,10 +1719,14 @@ static int run_split(gi_ctx *c, Slot &S, Plan &P, unsigned char *d_meta, Timelin
             const int ndegI = c->degen[ch.gI], ndegF = c->degen[ch.gF];
             SymDev t0 = symdev(pF.sym[ch.gF * GI_MAXDEG + 0]);
             t0.tcol = GP.d_tcol[0];
+            t0.tfc = GP.d_tfc[0];
+            t0.tfw = GP.d_tfw[0];
             SymDev t1 = t0;
             if (ndegF > 1) {
                 t1 = symdev(pF.sym[ch.gF * GI_MAXDEG + 1]);
                 t1.tcol = GP.d_tcol[1];
+                t1.tfc = GP.d_tfc[1];
+                t1.tfw = GP.d_tfw[1];
             }
             const int cols = ch.nitems * ndegI * ndegF;
             if (cols > 0 && BF.nsize > 0) {

@@ -109,6 +109,11 @@ struct GroupPlan {
     double macs = 0;                 // multiply-adds per unit
     std::vector<int> tcol[GI_MAXDEG];    // fold map: symmetrised index -> H offset
     int *d_tcol[GI_MAXDEG] = {nullptr, nullptr};
+    // the fold map's rows as fixed pairs (SymDev::tfc, tfw)
+    std::vector<int> tfc[GI_MAXDEG];
+    std::vector<double> tfw[GI_MAXDEG];
+    int *d_tfc[GI_MAXDEG] = {nullptr, nullptr};
+    double *d_tfw[GI_MAXDEG] = {nullptr, nullptr};
     ClassDev *d_cd = nullptr;        // device copy of classdev() of every output class
     // packed G as a sparse map of the lower eigenvector record, per lower
     // component d: G[g] = sum_{w < gW} gwt[d][w*Gtot_p + g] * u[gidx[d][w*Gtot_p + g]]

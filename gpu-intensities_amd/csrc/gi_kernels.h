@@ -37,6 +37,11 @@ struct SymDev {
     const double *rval;
     const int *tptr, *tcol;
     const double *tval;
+    // the fold's rows of at most two entries as fixed pairs (GroupPlan::tfc):
+    // per row two H offsets (-1: no entry; -2: a longer row, read from the CSR)
+    // and their weights; null: the CSR only
+    const int *tfc;
+    const double *tfw;
 };
 
 // Structure-of-arrays line records; amp_w amplitudes per line.
@@ -329,6 +334,32 @@ template <int R>
 __device__ inline void fold_rows(const SymDev &T, const double *__restrict__ Hu,
                                  const int (&t)[R], const bool (&ok)[R], double (&acc)[R])
 {
+    if (T.tfc) {
+        // fixed pairs: one load of a row's offsets and weights, then its H values
+        int2 cc[R];
+        double2 ww[R];
+        double h0[R], h1[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            cc[r] = ok[r] ? ((const int2 *)T.tfc)[t[r]] : make_int2(-1, -1);
+            ww[r] = ok[r] ? ((const double2 *)T.tfw)[t[r]] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            h0[r] = cc[r].x >= 0 ? Hu[cc[r].x] : 0.0;
+            h1[r] = cc[r].y >= 0 ? Hu[cc[r].y] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            double v = 0.0;
+            if (cc[r].x >= 0) v += ww[r].x * h0[r];
+            if (cc[r].y >= 0) v += ww[r].y * h1[r];
+            if (cc[r].x == -2)  // a row of more than two entries, in entry order
+                for (int e = T.tptr[t[r]]; e < T.tptr[t[r] + 1]; e++) v += T.tval[e] * Hu[T.tcol[e]];
+            acc[r] = v;
+        }
+        return;
+    }
     int p0[R], p1[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {

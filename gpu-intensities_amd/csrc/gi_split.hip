@@ -465,8 +465,17 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(PER * 1024
         for (int k = 0; k < PER; k++) {
             const int t = (int)threadIdx.x + k * 1024;
             double acc = 0.0;
-            if (t < F.nsizeF && unit >= 0)
-                for (int e = T.tptr[t]; e < T.tptr[t + 1]; e++) acc += T.tval[e] * Hu[T.tcol[e]];
+            if (t < F.nsizeF && unit >= 0) {
+                int2 cc = make_int2(-2, -2);
+                if (T.tfc) {  // fixed pairs, as in fold_rows
+                    cc = ((const int2 *)T.tfc)[t];
+                    const double2 ww = ((const double2 *)T.tfw)[t];
+                    if (cc.x >= 0) acc += ww.x * Hu[cc.x];
+                    if (cc.y >= 0) acc += ww.y * Hu[cc.y];
+                }
+                if (cc.x == -2)
+                    for (int e = T.tptr[t]; e < T.tptr[t + 1]; e++) acc += T.tval[e] * Hu[T.tcol[e]];
+            }
             v[k] = acc;
             m = fmax(m, fabs(acc));
         }
