@@ -815,10 +815,16 @@ int gi_prepare(gi_ctx *c)
                     if ((rc = upload(c, &oc.d_rcoef, oc.rcoef)) != GI_OK) return rc;
                 }
                 for (int d = 0; d < c->degen[G.gF]; d++) {
-                    if ((rc = upload(c, &G.d_tcol[d], G.tcol[d])) != GI_OK ||
-                        (rc = upload(c, &G.d_tfc[d], G.tfc[d])) != GI_OK ||
-                        (rc = upload(c, &G.d_tfw[d], G.tfw[d])) != GI_OK)
+                    if ((rc = upload(c, &G.d_tcol[d], G.tcol[d])) != GI_OK) return rc;
+                    if (G.tfc[d].empty()) {  // the CSR only
+                        (void)hipFree(G.d_tfc[d]);
+                        (void)hipFree(G.d_tfw[d]);
+                        G.d_tfc[d] = nullptr;
+                        G.d_tfw[d] = nullptr;
+                    } else if ((rc = upload(c, &G.d_tfc[d], G.tfc[d])) != GI_OK ||
+                               (rc = upload(c, &G.d_tfw[d], G.tfw[d])) != GI_OK) {
                         return rc;
+                    }
                     std::vector<int>().swap(G.tfc[d]);
                     std::vector<double>().swap(G.tfw[d]);
                 }

@@ -109,7 +109,8 @@ struct GroupPlan {
     double macs = 0;                 // multiply-adds per unit
     std::vector<int> tcol[GI_MAXDEG];    // fold map: symmetrised index -> H offset
     int *d_tcol[GI_MAXDEG] = {nullptr, nullptr};
-    // the fold map's rows as fixed pairs (SymDev::tfc, tfw)
+    // the fold map's rows as fixed pairs (SymDev::tfc, tfw; empty when a row
+    // has more than two entries)
     std::vector<int> tfc[GI_MAXDEG];
     std::vector<double> tfw[GI_MAXDEG];
     int *d_tfc[GI_MAXDEG] = {nullptr, nullptr};

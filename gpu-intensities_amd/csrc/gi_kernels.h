@@ -37,9 +37,10 @@ struct SymDev {
     const double *rval;
     const int *tptr, *tcol;
     const double *tval;
-    // the fold's rows of at most two entries as fixed pairs (GroupPlan::tfc):
-    // per row two H offsets (-1: no entry; -2: a longer row, read from the CSR)
-    // and their weights; null: the CSR only
+    // the fold's rows as fixed pairs (GroupPlan::tfc) when every row has at
+    // most two entries: per row two H offsets (-1: no entry; -2, a longer row
+    // read from the CSR, is not produced by the planner but kept by the
+    // kernels) and their weights; null: the CSR only
     const int *tfc;
     const double *tfw;
 };
