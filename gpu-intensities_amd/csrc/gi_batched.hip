@@ -534,8 +534,11 @@ __device__ __attribute__((always_inline)) inline void corr_tile(const CorrMMArgs
             }
 }
 
+// The 64 x 64 tile at six wavefronts per SIMD (79 registers, accumulators in
+// VGPRs) instead of five: the kernel waits on its operand loads, and the sixth
+// wavefront hides more of that (seven spill).
 template <int BM_, int BN_>
-__global__ __launch_bounds__(256) void k_corr_mfma(const CorrMMArgs *__restrict__ args,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BM_ * BN_ <= 4096 ? 6 : 1))) void k_corr_mfma(const CorrMMArgs *__restrict__ args,
                                                    const int *__restrict__ cbegin, int nchunks,
                                                    int ntiles)
 {
