@@ -331,7 +331,9 @@ int gi_get_tuning(gi_ctx *ctx, gi_tuning *t);
  * (parts = min(16, K / kpart_len); default 1024 for FP64 and 2048 for the
  * split digits, at most 2048, at least 64); rec_lds, the doubles of an eigenvector record the
  * G builds stage in LDS (gathers beyond them read a longer record in global memory; default
- * 19968), or -1: no record staged, every chunk takes the global-gather builds;
+ * 19968; below it the FP64 build also reads the full record map instead of
+ * the compact one), or -1: no record staged, every chunk takes the
+ * global-gather builds;
  * split2_steps / split2_segs, the largest class (64-byte K steps, segments)
  * the two-step split half-ls takes (larger ones run the one-step kernel;
  * default 512 / 64); fold_split_rows, the longest folded column the fused
