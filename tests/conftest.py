@@ -16,6 +16,17 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
 
 
+def pytest_collection_finish(session):
+    """torch ships a HIP runtime of its own beside the one the library links;
+    in a process where the library reaches the GPU first, torch's first GPU
+    call reports no device. GPU tests build eigenvector blocks with torch, so
+    torch's runtime is started before any test runs."""
+    if any(item.get_closest_marker("gpu") for item in session.items):
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+
+
 @pytest.fixture(scope="session")
 def engine():
     from gpu_intensities_amd import binding
