@@ -2383,7 +2383,10 @@ int autotune(gi_ctx *c, int mode, int nlower, const int *ids, gi_lines *out, int
         c->pipe_share = pick.second;
         // the half-ls tile again at the chosen rounds (the best shape depends on the
         // size of the rounds: at C2 one round of 128 x 128 tiles on eight wavefronts
-        // measured 2.92 ms against 3.17 ms for 64 x 64, which wins small rounds)
+        // measured 2.92 ms against 3.17 ms for 64 x 64, which wins small rounds),
+        // from the setting in force at the call again, so that the first scan's
+        // pick at other rounds takes no incumbency into this one
+        if (mode == GI_MODE_FP64_BATCHED) c->tile_halfls = saved.halfls_cfg;
         if (mode == GI_MODE_FP64_BATCHED &&
             (rc = scan("halfls_tile", "half_ls_mfma", &c->tile_halfls, tiles, 6, tname))) {
             c->tile_halfls = saved.halfls_cfg; c->tile_corr = saved.corr_cfg;

@@ -363,7 +363,8 @@ int gi_get_persist(gi_ctx *ctx, int *on);
  * batched mode) or the split half-ls tile shape (split mode), each judged by
  * its kernel's time, and the pipeline round count together with the share of
  * the work per round, as pairs (both matrix-core modes when `out` has room for
- * every candidate, judged by the call's wall time), then the FP64 half-ls tile once more at the
+ * every candidate, judged by the call's wall time), then the FP64 half-ls tile once more (from
+ * the setting in force at the call, so the first scan's pick holds no incumbency) at the
  * chosen rounds. Settings are scanned in a fixed order starting from
  * the current one, and a setting replaces the incumbent only when it is more
  * than 3% faster, so repeated runs pick the same table unless the timings
